@@ -1,0 +1,159 @@
+// gc_internal.hpp -- definitions shared by the HIP kernels and the host
+// runtime of libgcvol.so.  Everything here is internal; the public surface is
+// include/gcvol.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "gcvol.h"
+
+namespace gc {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define GC_HIP(expr)                                                                \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return ::gc::fail(GC_ERR_HIP, std::string(#expr) + ": " +              \
+                                              hipGetErrorString(e_));               \
+    } while (0)
+
+// ------------------------------------------------- FP32 bookkeeping formulas
+// Evaluated identically on host and device (the library is compiled with
+// -ffp-contract=off and IEEE-rounded divide/sqrt).
+
+// Page-table position of brick `b` at `level`: b / (virt * (1/(bs-2c))) + FLT_EPSILON
+// (CacheManagerKernels.cu:75-86, epsilon from CacheManager.hpp:560).
+__host__ __device__ inline void volume_position(const gc_geometry &g, uint32_t level,
+                                                const uint32_t b[3], float out[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        const float den = (float)g.virt_vox[level][a] * g.one_over[a];
+        const float q = (float)b[a] / den;
+        out[a] = q + 0x1p-23f; // FLT_EPSILON
+    }
+}
+
+// MRPD cell of a normalised position (K_MultiResolutionPageDirectory.hpp:168-180).
+__host__ __device__ inline uint64_t mrpd_index(const gc_geometry &g, uint32_t level,
+                                               const float p[3])
+{
+    const uint32_t x = g.level_coord[level] + (uint32_t)(p[0] * (float)g.level_dim[level][0]);
+    const uint32_t y = (uint32_t)(p[1] * (float)g.level_dim[level][1]);
+    const uint32_t z = (uint32_t)(p[2] * (float)g.level_dim[level][2]);
+    return (uint64_t)x + (uint64_t)g.mrpd_dim[0] * ((uint64_t)y + (uint64_t)g.mrpd_dim[1] * z);
+}
+
+// Per-slot LRU metadata.  The LRU order itself is a permutation of slot ids;
+// the slot's voxel origin in the reference's 3-D pool (its cachePosition) is
+// a function of the slot id, so only level and volumePosition are stored.
+struct SlotMeta {
+    float vol[3];
+    uint32_t level;
+};
+
+__host__ __device__ inline void slot_origin(const gc_geometry &g, uint32_t s, uint32_t o[3])
+{
+    const uint32_t cx = g.cache_size[0], cy = g.cache_size[1];
+    o[0] = (s % cx) * g.brick_size;
+    o[1] = ((s / cx) % cy) * g.brick_size;
+    o[2] = (s / (cx * cy)) * g.brick_size;
+}
+
+// ------------------------------------------------------ kernel entry points
+// (implemented in kernels.hip; all enqueue on `stream` and return hipError_t)
+
+struct LruScratch {
+    uint64_t *masks;        // one bit per LRU entry
+    uint32_t *counts;       // per-tile used counts, then exclusive offsets
+    uint32_t *total;        // [0] used total
+    uint32_t nblocks;
+};
+
+hipError_t launch_lru_update(const uint32_t *order_in, uint32_t *order_out, const uint32_t *usage,
+                             uint32_t n, uint32_t ts, const LruScratch &s, hipStream_t stream);
+
+struct MissScratch {
+    uint64_t *masks;
+    uint32_t *counts;
+    uint32_t *total;        // [0] number of requested cells (uncapped)
+    uint32_t nblocks;
+};
+
+// Requested cells (req == ts) in linear order over [0, cells-1), first `cap`
+// of them decoded to (x_in_level, y, z, level) into out4.
+hipError_t launch_miss_compact(const uint32_t *req, uint64_t cells, uint32_t ts,
+                               const gc_geometry *d_geom, uint32_t cap, uint32_t *out4,
+                               const MissScratch &s, hipStream_t stream);
+
+hipError_t launch_deref(uint4 *mrpd, const uint32_t *order, uint32_t nslots, uint32_t nne,
+                        const SlotMeta *meta, const gc_geometry *d_geom, hipStream_t stream);
+
+// Victims take the new level / position and receive the brick voxels from
+// the device staging buffer (big bricks, Bricker_v2 layout).
+hipError_t launch_assign_copy(const uint32_t *order, uint32_t nslots, uint32_t nne,
+                              const uint32_t *asked4, const uint32_t *bb_index,
+                              const uint32_t *bb_coord4, SlotMeta *meta, void *pool,
+                              const void *staging, const gc_geometry *d_geom,
+                              const uint32_t big_brick[3], uint32_t bytes, hipStream_t stream);
+
+// Victims take the new level / position and receive bricks cut from device
+// levels (fully-resident start-up).
+hipError_t launch_assign_cut(const uint32_t *order, uint32_t nslots, uint32_t n,
+                             const uint32_t *asked4, SlotMeta *meta, void *pool,
+                             const void *const *d_levels, const uint32_t *d_level_dims,
+                             const gc_geometry *d_geom, uint32_t bytes, hipStream_t stream);
+
+// MRPD insert of the asked bricks (reference_new_bricks, MRPD only): the first
+// nne map to their victim slot (w = 1), the rest are Empty (w = 2).  serial=1
+// replays the reference's in-order single-thread insert.
+hipError_t launch_insert(uint4 *mrpd, const uint32_t *order, uint32_t nslots, uint32_t n,
+                         uint32_t nne, const uint32_t *asked4, const gc_geometry *d_geom,
+                         int serial, hipStream_t stream);
+
+hipError_t launch_occupancy(const SlotMeta *meta, uint32_t nslots, uint32_t *count,
+                            hipStream_t stream);
+
+hipError_t launch_mip(const void *in, const uint32_t dims[3], uint32_t bytes, void *out,
+                      hipStream_t stream);
+
+hipError_t launch_brick_cut(const void *level, const uint32_t dims[3], uint32_t bytes,
+                            uint32_t bs, uint32_t cov, const uint32_t *bricks_xyz, uint32_t n,
+                            void *out, hipStream_t stream);
+
+// ------------------------------------------------------------------ DVR
+struct DvrParams {
+    const uint4 *mrpd;
+    uint32_t *request;
+    uint32_t *usage;
+    const void *pool;
+    const gc_geometry *geom;
+    const float4 *tf;
+    uint32_t tf_size;
+    uint32_t ts;
+    uint32_t *pixels;
+    float *tbuf;
+    void *cstate;
+    gc_render_stats *stats;
+    uint32_t width, height;
+    float m[12];
+    float bmin[3], bmax[3];
+    float tstep, alpha_stop;
+    uint32_t nb_lods;
+    uint32_t tiles_x, tiles_y;
+    float inv_levels[GC_MAX_LEVELS][3];
+    float lod_step[GC_MAX_LEVELS];
+    float kexp[GC_MAX_LEVELS];
+};
+
+hipError_t launch_dvr(const DvrParams &p, uint32_t bytes, uint32_t precision, uint32_t tile,
+                      hipStream_t stream);
+
+} // namespace gc

@@ -1,0 +1,1165 @@
+// runtime.cpp -- host runtime of libgcvol.so: brick sources, the request
+// handler, the cache manager state machine, frames, and the C ABI of
+// include/gcvol.h.
+//
+// The cache manager follows CacheManager<T> (CacheManager.hpp:136-672):
+//   update() = LRU update, then either load the bricks of the finished
+//   request or compact the misses of the last frame into a new request.
+// Device state (HBM):
+//   mrpd     uint4 per page-directory cell (levels side by side along x)
+//   request  uint32 per page-directory cell (timestamp of the last miss)
+//   pool     slots x bs^3 voxels, brick-major (slot = x + cx*(y + cy*z))
+//   usage    uint32 per slot (timestamp of the last frame that sampled it)
+//   order    LRU order as a permutation of slot ids (most recent first)
+//   meta     per slot {volumePosition, level}
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_set>
+#include <vector>
+
+#include "gc_internal.hpp"
+#include "synth.hpp"
+
+namespace gc {
+
+// =================================================================== errors
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+int fail(int code, const std::string &msg)
+{
+    set_error(msg);
+    return code;
+}
+
+#define GC_TRY(expr)                          \
+    do {                                      \
+        int rc_ = (expr);                     \
+        if (rc_ != GC_OK) return rc_;         \
+    } while (0)
+
+static void parallel_for(uint32_t n, uint32_t threads, const std::function<void(uint32_t)> &fn)
+{
+    if (threads <= 1 || n <= 1) {
+        for (uint32_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    threads = std::min(threads, n);
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> pool;
+    auto work = [&]() {
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= n) break;
+            fn(i);
+        }
+    };
+    for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+}
+
+static uint32_t hw_threads()
+{
+    const uint32_t h = std::thread::hardware_concurrency();
+    return h ? h : 1;
+}
+
+// ================================================================== sources
+struct Source {
+    gc_volume_cfg vc{};
+    uint32_t bs = 0, cov = 0, bytes = 0, BB[3] = {1, 1, 1};
+    uint64_t bb_vox = 0, bb_bytes = 0;
+    virtual ~Source() = default;
+    virtual int get(uint32_t level, const uint32_t bbc[3], void *dst) = 0;   // GC_BRICK_*
+    virtual bool thread_safe() const { return true; }
+    virtual uint32_t threads() const { return 1; }
+    void init(const gc_volume_cfg &v)
+    {
+        vc = v;
+        bs = v.brick_size[0];
+        cov = v.covering[0];
+        bytes = v.encoded_bytes;
+        for (int a = 0; a < 3; ++a) BB[a] = v.big_brick_size[a];
+        bb_vox = (uint64_t)bs * bs * bs * BB[0] * BB[1] * BB[2];
+        bb_bytes = bb_vox * bytes;
+    }
+    // Offset (in voxels) of small brick (sx,sy,sz) inside a big brick
+    // (get_iterator_from_indexes, Bricker_v2.cpp:295-299).
+    uint64_t small_offset(uint32_t sx, uint32_t sy, uint32_t sz) const
+    {
+        const uint64_t BX = (uint64_t)bs * BB[0], BY = (uint64_t)bs * BB[1];
+        return (uint64_t)sx * bs + BX * ((uint64_t)sy * bs + BY * (uint64_t)sz * bs);
+    }
+};
+
+// Host pyramid, bricks cut with the covering rule of Bricker_v2.cpp:138-222.
+struct PyramidSource : Source {
+    std::vector<const uint8_t *> levels;
+    int get(uint32_t level, const uint32_t bbc[3], void *dst) override
+    {
+        if (level >= vc.nb_levels) return GC_BRICK_ERROR;
+        const uint32_t *d = vc.initial_sizes[level];
+        const uint8_t *src = levels[level];
+        const uint32_t inner = bs - 2 * cov;
+        const uint64_t BX = (uint64_t)bs * BB[0], BY = (uint64_t)bs * BB[1];
+        std::memset(dst, 0, bb_bytes);
+        uint8_t *out = static_cast<uint8_t *>(dst);
+        for (uint32_t sz = 0; sz < BB[2]; ++sz)
+            for (uint32_t sy = 0; sy < BB[1]; ++sy)
+                for (uint32_t sx = 0; sx < BB[0]; ++sx) {
+                    const int64_t ox = (int64_t)(bbc[0] * BB[0] + sx) * inner - cov;
+                    const int64_t oy = (int64_t)(bbc[1] * BB[1] + sy) * inner - cov;
+                    const int64_t oz = (int64_t)(bbc[2] * BB[2] + sz) * inner - cov;
+                    const int64_t x0 = std::max<int64_t>(ox, 0), x1 = std::min<int64_t>(ox + bs, d[0]);
+                    if (x1 <= x0) continue;
+                    for (uint32_t k = 0; k < bs; ++k) {
+                        const int64_t gz = oz + k;
+                        if (gz < 0 || gz >= d[2]) continue;
+                        for (uint32_t j = 0; j < bs; ++j) {
+                            const int64_t gy = oy + j;
+                            if (gy < 0 || gy >= d[1]) continue;
+                            const uint64_t so = ((uint64_t)x0 + (uint64_t)d[0] * ((uint64_t)gy + (uint64_t)d[1] * gz)) * bytes;
+                            const uint64_t dof = (sx * (uint64_t)bs + (uint64_t)(x0 - ox)) +
+                                                 BX * ((sy * (uint64_t)bs + j) + BY * (sz * (uint64_t)bs + k));
+                            std::memcpy(out + dof * bytes, src + so, (size_t)(x1 - x0) * bytes);
+                        }
+                    }
+                }
+        return GC_BRICK_SUCCESS;
+    }
+};
+
+struct SynthSource : Source {
+    std::unique_ptr<Synth> synth;
+    uint32_t nthreads = 1;
+    uint32_t threads() const override { return nthreads; }
+    int get(uint32_t level, const uint32_t bbc[3], void *dst) override
+    {
+        if (level >= vc.nb_levels) return GC_BRICK_ERROR;
+        const uint32_t inner = bs - 2 * cov;
+        const uint32_t n[3] = {bs, bs, bs};
+        if (BB[0] == 1 && BB[1] == 1 && BB[2] == 1) {
+            std::memset(dst, 0, bb_bytes);
+            const int64_t o[3] = {(int64_t)bbc[0] * inner - cov, (int64_t)bbc[1] * inner - cov,
+                                  (int64_t)bbc[2] * inner - cov};
+            synth->fill_box(level, o, n, vc.initial_sizes[level], dst);
+            return GC_BRICK_SUCCESS;
+        }
+        std::vector<uint8_t> tmp((size_t)bs * bs * bs * bytes);
+        const uint64_t BX = (uint64_t)bs * BB[0], BY = (uint64_t)bs * BB[1];
+        for (uint32_t sz = 0; sz < BB[2]; ++sz)
+            for (uint32_t sy = 0; sy < BB[1]; ++sy)
+                for (uint32_t sx = 0; sx < BB[0]; ++sx) {
+                    std::fill(tmp.begin(), tmp.end(), 0);
+                    const int64_t o[3] = {(int64_t)(bbc[0] * BB[0] + sx) * inner - cov,
+                                          (int64_t)(bbc[1] * BB[1] + sy) * inner - cov,
+                                          (int64_t)(bbc[2] * BB[2] + sz) * inner - cov};
+                    synth->fill_box(level, o, n, vc.initial_sizes[level], tmp.data());
+                    for (uint32_t k = 0; k < bs; ++k)
+                        for (uint32_t j = 0; j < bs; ++j) {
+                            const uint64_t dof = (uint64_t)sx * bs + BX * ((uint64_t)sy * bs + j + BY * ((uint64_t)sz * bs + k));
+                            std::memcpy(static_cast<uint8_t *>(dst) + dof * bytes,
+                                        tmp.data() + ((uint64_t)bs * (j + (uint64_t)bs * k)) * bytes, (size_t)bs * bytes);
+                        }
+                }
+        return GC_BRICK_SUCCESS;
+    }
+};
+
+struct CallbackSource : Source {
+    gc_brick_fn fn = nullptr;
+    void *user = nullptr;
+    bool thread_safe() const override { return false; }
+    int get(uint32_t level, const uint32_t bbc[3], void *dst) override
+    {
+        return fn(user, level, bbc[0], bbc[1], bbc[2], dst);
+    }
+};
+
+} // namespace gc
+
+struct gc_source {
+    std::unique_ptr<gc::Source> impl;
+};
+
+namespace gc {
+
+// ==================================================================== cache
+static bool validate(const gc_volume_cfg &vc, const gc_cache_cfg &cc, std::string &why)
+{
+    if (vc.nb_levels == 0 || vc.nb_levels > GC_MAX_LEVELS) { why = "nb_levels out of range"; return false; }
+    if (vc.brick_size[0] != vc.brick_size[1] || vc.brick_size[0] != vc.brick_size[2]) { why = "BrickSize must be cubic"; return false; }
+    if (vc.covering[0] != vc.covering[1] || vc.covering[0] != vc.covering[2]) { why = "VoxelCovering must be uniform"; return false; }
+    if (vc.brick_size[0] <= 2 * vc.covering[0]) { why = "BrickSize must exceed 2*VoxelCovering"; return false; }
+    if (vc.encoded_bytes != 1 && vc.encoded_bytes != 2 && vc.encoded_bytes != 4) { why = "NumberEncodedBytes must be 1, 2 or 4"; return false; }
+    if (vc.channels != 1) { why = "NumberChannels must be 1"; return false; }
+    for (int a = 0; a < 3; ++a) {
+        if (vc.big_brick_size[a] == 0) { why = "BigBrickSize must be >= 1"; return false; }
+        if (cc.cache_size[a] == 0) { why = "CacheSize must be >= 1"; return false; }
+        if (cc.block_size[a] != vc.brick_size[a]) { why = "BlockSize[0] must equal BrickSize"; return false; }
+    }
+    for (uint32_t l = 0; l < vc.nb_levels; ++l)
+        for (int a = 0; a < 3; ++a)
+            if (vc.initial_sizes[l][a] == 0 || vc.real_sizes[l][a] == 0) { why = "zero level size"; return false; }
+    return true;
+}
+
+static void compute_geometry(const gc_volume_cfg &vc, const gc_cache_cfg &cc, gc_geometry &g)
+{
+    // CacheManager.hpp:157-285 for a single pagination level
+    std::memset(&g, 0, sizeof(g));
+    g.nb_levels = vc.nb_levels;
+    g.brick_size = vc.brick_size[0];
+    g.covering = vc.covering[0];
+    g.inner = g.brick_size - 2 * g.covering;
+    g.encoded_bytes = vc.encoded_bytes;
+    const double inner = (double)g.brick_size - 2.0 * (double)g.covering;
+    for (uint32_t l = 0; l < vc.nb_levels; ++l)
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t r1 = (uint32_t)std::ceil((double)(float)vc.real_sizes[l][a] / inner);
+            g.level_dim[l][a] = r1;
+            g.virt_vox[l][a] = (uint32_t)std::ceil((double)(float)r1 * inner);
+            g.rdv[l][a] = (float)vc.real_sizes[l][a] / (float)g.virt_vox[l][a];
+            g.ior[l][a] = (float)vc.initial_sizes[l][a] / (float)vc.real_sizes[l][a];
+        }
+    g.level_coord[0] = 0;
+    for (uint32_t l = 1; l < vc.nb_levels; ++l) g.level_coord[l] = g.level_coord[l - 1] + g.level_dim[l - 1][0];
+    for (uint32_t l = 0; l < vc.nb_levels; ++l) {
+        g.mrpd_dim[0] += g.level_dim[l][0];
+        g.mrpd_dim[1] = std::max(g.mrpd_dim[1], g.level_dim[l][1]);
+        g.mrpd_dim[2] = std::max(g.mrpd_dim[2], g.level_dim[l][2]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        g.one_over[a] = 1.f / (float)(vc.brick_size[a] - 2 * vc.covering[a]);
+        g.cache_size[a] = cc.cache_size[a];
+    }
+    g.nslots = cc.cache_size[0] * cc.cache_size[1] * cc.cache_size[2];
+    g.mrpd_cells = (uint64_t)g.mrpd_dim[0] * g.mrpd_dim[1] * g.mrpd_dim[2];
+}
+
+struct Cache;
+
+// Request handler (RequestHandler.hpp:30-374): a worker thread that turns a
+// miss list into bricks in pinned memory and uploads them on a side stream.
+struct Handler {
+    Cache *c = nullptr;
+    std::thread worker;
+    std::mutex m;
+    std::condition_variable cv;
+    bool run = false;
+    std::atomic<bool> new_request{false};
+    std::vector<uint32_t> incoming;   // uint4 list as compacted
+    int status = GC_OK;
+    std::string error;
+    // results of the last handled request
+    uint32_t n = 0, nne = 0, nuniq = 0;
+    bool upload_pending = false;
+
+    void start();
+    void stop();
+    void notify(std::vector<uint32_t> &&list);
+    int handle(const std::vector<uint32_t> &list);
+    void loop();
+};
+
+struct Cache {
+    gc_volume_cfg vc{};
+    gc_cache_cfg cc{};
+    gc_geometry g{};
+    Source *src = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr, upload = nullptr;
+    hipEvent_t uploaded = nullptr, consumed = nullptr;
+    uint32_t ts = 1;
+    uint32_t cap = 50;
+    // device state
+    uint4 *mrpd = nullptr;
+    uint32_t *request = nullptr, *usage = nullptr, *order = nullptr, *order_tmp = nullptr;
+    SlotMeta *meta = nullptr;
+    uint8_t *pool = nullptr;
+    gc_geometry *d_geom = nullptr;
+    LruScratch lru_s{};
+    MissScratch miss_s{};
+    uint32_t *d_misses = nullptr;       // cap x uint4
+    uint32_t *d_count = nullptr;        // scratch counter
+    // host staging
+    uint32_t *h_misses = nullptr;       // pinned: [total, pad x3, cap x uint4]
+    uint8_t *h_staging = nullptr;       // pinned: cap big bricks
+    uint8_t *d_staging = nullptr;
+    uint32_t *h_meta = nullptr;         // pinned: asked (cap x4) | bb_index (cap) | bb_coord (cap x4)
+    uint32_t *d_meta = nullptr;
+    Handler handler;
+    std::mutex done_m;
+    bool request_done = false;
+
+    ~Cache() { release(); }
+    void release();
+    int create();
+    int lru_update();
+    int prepare_bricks(bool &done);
+    int load_new_bricks();
+    int update(int *done);
+    int insert_serial_needed(const uint32_t *asked, uint32_t n, int &serial) const;
+};
+
+void Cache::release()
+{
+    handler.stop();
+    if (stream) hipStreamSynchronize(stream);
+    if (upload) hipStreamSynchronize(upload);
+    hipFree(mrpd); hipFree(request); hipFree(usage); hipFree(order); hipFree(order_tmp);
+    hipFree(meta); hipFree(pool); hipFree(d_geom);
+    hipFree(lru_s.masks); hipFree(lru_s.counts); hipFree(lru_s.total);
+    hipFree(miss_s.masks); hipFree(miss_s.counts); hipFree(miss_s.total);
+    hipFree(d_misses); hipFree(d_count); hipFree(d_staging); hipFree(d_meta);
+    if (h_misses) hipHostFree(h_misses);
+    if (h_staging) hipHostFree(h_staging);
+    if (h_meta) hipHostFree(h_meta);
+    if (uploaded) hipEventDestroy(uploaded);
+    if (consumed) hipEventDestroy(consumed);
+    if (stream) hipStreamDestroy(stream);
+    if (upload) hipStreamDestroy(upload);
+    mrpd = nullptr; request = usage = order = order_tmp = nullptr; meta = nullptr; pool = nullptr;
+    d_geom = nullptr; h_misses = nullptr; h_staging = nullptr; h_meta = nullptr;
+    uploaded = consumed = nullptr; stream = upload = nullptr;
+}
+
+template <typename T>
+static int dalloc(T **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) return fail(GC_ERR_OOM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    return GC_OK;
+}
+
+template <typename T>
+static int halloc(T **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipHostMalloc((void **)p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(GC_ERR_OOM, std::string("hipHostMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    return GC_OK;
+}
+
+int Cache::create()
+{
+    GC_HIP(hipSetDevice(device));
+    compute_geometry(vc, cc, g);
+    if (g.mrpd_cells >= (1ull << 32)) return fail(GC_ERR_CONFIG, "page directory exceeds 2^32 cells");
+    cap = cc.max_requests ? cc.max_requests : 50;
+    cap = std::min(cap, g.nslots);
+    GC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    GC_HIP(hipStreamCreateWithFlags(&upload, hipStreamNonBlocking));
+    GC_HIP(hipEventCreateWithFlags(&uploaded, hipEventDisableTiming));
+    GC_HIP(hipEventCreateWithFlags(&consumed, hipEventDisableTiming));
+    const uint64_t bvox = (uint64_t)g.brick_size * g.brick_size * g.brick_size;
+    GC_TRY(dalloc(&mrpd, g.mrpd_cells * sizeof(uint4)));
+    GC_TRY(dalloc(&request, g.mrpd_cells * sizeof(uint32_t)));
+    GC_TRY(dalloc(&usage, (size_t)g.nslots * sizeof(uint32_t)));
+    GC_TRY(dalloc(&order, (size_t)g.nslots * sizeof(uint32_t)));
+    GC_TRY(dalloc(&order_tmp, (size_t)g.nslots * sizeof(uint32_t)));
+    GC_TRY(dalloc(&meta, (size_t)g.nslots * sizeof(SlotMeta)));
+    GC_TRY(dalloc(&pool, (size_t)g.nslots * bvox * vc.encoded_bytes));
+    GC_TRY(dalloc(&d_geom, sizeof(gc_geometry)));
+    // Surfaces and thrust vectors of the reference start zeroed
+    // (Surface3dDevice.hpp:97-109, DynamicArray3dDevice).
+    GC_HIP(hipMemsetAsync(mrpd, 0, g.mrpd_cells * sizeof(uint4), stream));
+    GC_HIP(hipMemsetAsync(request, 0, g.mrpd_cells * sizeof(uint32_t), stream));
+    GC_HIP(hipMemsetAsync(usage, 0, (size_t)g.nslots * sizeof(uint32_t), stream));
+    GC_HIP(hipMemsetAsync(pool, 0, (size_t)g.nslots * bvox * vc.encoded_bytes, stream));
+    {
+        // LRU initialisation, x fastest (CacheDevice.hpp:151-165)
+        std::vector<uint32_t> ord(g.nslots);
+        std::vector<SlotMeta> mt(g.nslots);
+        for (uint32_t s = 0; s < g.nslots; ++s) {
+            ord[s] = s;
+            mt[s].vol[0] = mt[s].vol[1] = mt[s].vol[2] = -1.f;
+            mt[s].level = 0;
+        }
+        GC_HIP(hipMemcpyAsync(order, ord.data(), ord.size() * 4, hipMemcpyHostToDevice, stream));
+        GC_HIP(hipMemcpyAsync(meta, mt.data(), mt.size() * sizeof(SlotMeta), hipMemcpyHostToDevice, stream));
+        GC_HIP(hipMemcpyAsync(d_geom, &g, sizeof(g), hipMemcpyHostToDevice, stream));
+        GC_HIP(hipStreamSynchronize(stream));
+    }
+    lru_s.nblocks = (g.nslots + 1023) / 1024;
+    GC_TRY(dalloc(&lru_s.masks, ((size_t)g.nslots + 63) / 64 * 8 + 8));
+    GC_TRY(dalloc(&lru_s.counts, (size_t)lru_s.nblocks * 4 + 4));
+    GC_TRY(dalloc(&lru_s.total, 16));
+    miss_s.nblocks = (uint32_t)((g.mrpd_cells + 1023) / 1024);
+    GC_TRY(dalloc(&miss_s.masks, ((size_t)g.mrpd_cells + 63) / 64 * 8 + 8));
+    GC_TRY(dalloc(&miss_s.counts, (size_t)miss_s.nblocks * 4 + 4));
+    GC_TRY(dalloc(&miss_s.total, 16));
+    GC_TRY(dalloc(&d_misses, (size_t)cap * 16));
+    GC_TRY(dalloc(&d_count, 16));
+    GC_TRY(halloc(&h_misses, 16 + (size_t)cap * 16));
+    GC_TRY(halloc(&h_staging, (size_t)cap * src->bb_bytes));
+    GC_TRY(dalloc(&d_staging, (size_t)cap * src->bb_bytes));
+    GC_TRY(halloc(&h_meta, (size_t)cap * 9 * 4));
+    GC_TRY(dalloc(&d_meta, (size_t)cap * 9 * 4));
+    handler.c = this;
+    if (!cc.sync_requests) handler.start();
+    return GC_OK;
+}
+
+int Cache::lru_update()
+{
+    // CacheManager.hpp:387-406
+    GC_HIP(launch_lru_update(order, order_tmp, usage, g.nslots, ts, lru_s, stream));
+    std::swap(order, order_tmp);
+    return GC_OK;
+}
+
+int Cache::prepare_bricks(bool &done)
+{
+    // prepare_bricks + create_request_list (CacheManager.hpp:410-491)
+    GC_HIP(launch_miss_compact(request, g.mrpd_cells, ts, d_geom, cap, d_misses, miss_s, stream));
+    GC_HIP(hipMemcpyAsync(h_misses, miss_s.total, 4, hipMemcpyDeviceToHost, stream));
+    GC_HIP(hipMemcpyAsync(h_misses + 4, d_misses, (size_t)cap * 16, hipMemcpyDeviceToHost, stream));
+    GC_HIP(hipStreamSynchronize(stream));
+    const uint32_t n = std::min(h_misses[0], cap);
+    if (n == 0) {
+        done = true;
+        return GC_OK;
+    }
+    done = false;
+    std::vector<uint32_t> list(h_misses + 4, h_misses + 4 + 4 * (size_t)n);
+    if (cc.sync_requests) {
+        GC_TRY(handler.handle(list));
+        request_done = true;
+    } else {
+        handler.notify(std::move(list));
+    }
+    return GC_OK;
+}
+
+int Cache::insert_serial_needed(const uint32_t *asked, uint32_t n, int &serial) const
+{
+    // The parallel insert is exact when no two asked bricks share a page-table
+    // cell under the FP32 position formula; otherwise replay the reference's
+    // serial order.
+    std::unordered_set<uint64_t> seen;
+    seen.reserve(n * 2);
+    serial = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        float vol[3];
+        volume_position(g, asked[4 * k + 3], asked + 4 * k, vol);
+        if (!seen.insert(mrpd_index(g, asked[4 * k + 3], vol)).second) { serial = 1; break; }
+    }
+    return GC_OK;
+}
+
+int Cache::load_new_bricks()
+{
+    // CacheManager.hpp:503-632
+    const uint32_t n = handler.n, nne = handler.nne;
+    if (n == 0) return GC_OK;
+    GC_HIP(hipStreamWaitEvent(stream, uploaded, 0));
+    const uint32_t *d_asked = d_meta;
+    const uint32_t *d_bbi = d_meta + 4 * (size_t)cap;
+    const uint32_t *d_bbc = d_meta + 5 * (size_t)cap;
+    GC_HIP(launch_deref(mrpd, order, g.nslots, nne, meta, d_geom, stream));
+    GC_HIP(launch_assign_copy(order, g.nslots, nne, d_asked, d_bbi, d_bbc, meta, pool, d_staging, d_geom,
+                              vc.big_brick_size, vc.encoded_bytes, stream));
+    int serial = 0;
+    GC_TRY(insert_serial_needed(h_meta, n, serial));
+    GC_HIP(launch_insert(mrpd, order, g.nslots, n, nne, d_asked, d_geom, serial, stream));
+    GC_HIP(hipEventRecord(consumed, stream));
+    return GC_OK;
+}
+
+int Cache::update(int *done)
+{
+    // CacheManager.hpp:319-354
+    GC_TRY(lru_update());
+    if (handler.new_request.load()) {
+        ++ts;
+        if (done) *done = 0;
+        return GC_OK;
+    }
+    if (handler.status != GC_OK) {
+        const int rc = handler.status;
+        handler.status = GC_OK;
+        return fail(rc, handler.error);
+    }
+    bool check = false;
+    bool rd;
+    {
+        std::lock_guard<std::mutex> lk(done_m);
+        rd = request_done;
+    }
+    if (rd) {
+        GC_TRY(load_new_bricks());
+        std::lock_guard<std::mutex> lk(done_m);
+        request_done = false;
+        check = false;
+    } else {
+        GC_TRY(prepare_bricks(check));
+    }
+    ++ts;
+    if (done) *done = check ? 1 : 0;
+    return GC_OK;
+}
+
+// ------------------------------------------------------------------ handler
+void Handler::start()
+{
+    run = true;
+    new_request = false;
+    worker = std::thread(&Handler::loop, this);
+}
+
+void Handler::stop()
+{
+    if (worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            incoming.clear();
+            run = false;
+            cv.notify_one();
+        }
+        worker.join();
+    }
+}
+
+void Handler::notify(std::vector<uint32_t> &&list)
+{
+    std::lock_guard<std::mutex> lk(m);
+    incoming = std::move(list);
+    new_request = true;
+    cv.notify_one();
+}
+
+void Handler::loop()
+{
+    hipSetDevice(c->device);
+    for (;;) {
+        std::vector<uint32_t> list;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return !run || new_request.load(); });
+            if (!run) return;
+            list = incoming;
+        }
+        const int rc = handle(list);
+        if (rc != GC_OK) {
+            status = rc;
+            error = gc_last_error();
+        }
+        {
+            std::lock_guard<std::mutex> lk(c->done_m);
+            c->request_done = rc == GC_OK;
+        }
+        new_request = false;
+    }
+}
+
+int Handler::handle(const std::vector<uint32_t> &list)
+{
+    // RequestHandler::handle_request (RequestHandler.hpp:279-372)
+    Source &S = *c->src;
+    const uint32_t nreq = (uint32_t)(list.size() / 4);
+    const uint32_t cap = c->cap;
+    // the previous upload must be finished before pinned memory is rewritten
+    if (upload_pending) {
+        GC_HIP(hipEventSynchronize(c->uploaded));
+        upload_pending = false;
+    }
+    // unique big bricks, first-occurrence order (std::map of :289)
+    std::map<std::array<uint32_t, 4>, uint32_t> uid_of;
+    std::vector<std::array<uint32_t, 4>> uniq;
+    std::vector<uint32_t> uid(nreq);
+    for (uint32_t i = 0; i < nreq; ++i) {
+        const uint32_t *a = &list[4 * i];
+        const std::array<uint32_t, 4> key{a[0] / S.BB[0], a[1] / S.BB[1], a[2] / S.BB[2], a[3]};
+        auto it = uid_of.find(key);
+        if (it == uid_of.end()) {
+            it = uid_of.emplace(key, (uint32_t)uniq.size()).first;
+            uniq.push_back(key);
+        }
+        uid[i] = it->second;
+    }
+    const uint32_t nu = (uint32_t)uniq.size();
+    std::vector<int> st(nu, GC_BRICK_SUCCESS);
+    const float thr = c->cc.empty_threshold;
+    auto fetch = [&](uint32_t u) {
+        uint8_t *dst = c->h_staging + (uint64_t)u * S.bb_bytes;
+        const uint32_t bbc[3] = {uniq[u][0], uniq[u][1], uniq[u][2]};
+        int s = S.get(uniq[u][3], bbc, dst);
+        if (s == GC_BRICK_SUCCESS) {
+            // empty classification (BrickProcessorPredicate.hpp:11-45)
+            bool all_below = true;
+            for (uint64_t v = 0; v < S.bb_vox && all_below; ++v) {
+                float x;
+                if (S.bytes == 1) x = (float)dst[v];
+                else if (S.bytes == 2) x = (float)reinterpret_cast<const uint16_t *>(dst)[v];
+                else x = reinterpret_cast<const float *>(dst)[v];
+                if (!(x < thr)) all_below = false;
+            }
+            if (all_below) s = GC_BRICK_EMPTY;
+        }
+        st[u] = s;
+    };
+    const uint32_t nth = S.thread_safe() ? std::max(1u, S.threads()) : 1u;
+    parallel_for(nu, nth, fetch);
+    for (uint32_t u = 0; u < nu; ++u)
+        if (st[u] == GC_BRICK_ERROR)
+            return fail(GC_ERR_SOURCE, "brick source failed for level " + std::to_string(uniq[u][3]));
+    // non-empty first, stable (RequestHandler.hpp:354-369)
+    uint32_t *asked = c->h_meta;
+    uint32_t *bbi = c->h_meta + 4 * (size_t)cap;
+    uint32_t *bbc = c->h_meta + 5 * (size_t)cap;
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < nreq; ++i)
+        if (st[uid[i]] != GC_BRICK_EMPTY) {
+            std::memcpy(asked + 4 * k, &list[4 * i], 16);
+            bbi[k] = uid[i];
+            ++k;
+        }
+    const uint32_t nne_ = k;
+    for (uint32_t i = 0; i < nreq; ++i)
+        if (st[uid[i]] == GC_BRICK_EMPTY) std::memcpy(asked + 4 * k++, &list[4 * i], 16);
+    for (uint32_t u = 0; u < nu; ++u) std::memcpy(bbc + 4 * u, uniq[u].data(), 16);
+    // upload on the side stream once the previous load has consumed the
+    // device staging buffer
+    GC_HIP(hipStreamWaitEvent(c->upload, c->consumed, 0));
+    GC_HIP(hipMemcpyAsync(c->d_meta, c->h_meta, (size_t)cap * 9 * 4, hipMemcpyHostToDevice, c->upload));
+    GC_HIP(hipMemcpyAsync(c->d_staging, c->h_staging, (size_t)nu * S.bb_bytes, hipMemcpyHostToDevice, c->upload));
+    GC_HIP(hipEventRecord(c->uploaded, c->upload));
+    upload_pending = true;
+    n = nreq;
+    nne = nne_;
+    nuniq = nu;
+    return GC_OK;
+}
+
+// ==================================================================== frame
+struct Frame {
+    uint32_t w = 0, h = 0, precision = 32;
+    uint32_t *pixels = nullptr;
+    float *tbuf = nullptr;
+    void *cstate = nullptr;
+    size_t cbytes() const { return (size_t)w * h * (precision == 64 ? 32 : 16); }
+    ~Frame()
+    {
+        hipFree(pixels);
+        hipFree(tbuf);
+        hipFree(cstate);
+    }
+};
+
+} // namespace gc
+
+struct gc_cache {
+    gc::Cache impl;
+};
+struct gc_frame {
+    gc::Frame impl;
+};
+
+using namespace gc;
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char *gc_last_error(void) { return g_error.c_str(); }
+const char *gc_version(void) { return "gcvol 0.1 (gfx950)"; }
+
+int gc_volume_cfg_init(gc_volume_cfg *out, const uint32_t dims[3], uint32_t brick_size, uint32_t covering,
+                       const uint32_t big_brick[3], uint32_t encoded_bytes)
+{
+    if (!out || !dims || !big_brick) return fail(GC_ERR_ARG, "null argument");
+    if (brick_size <= 2 * covering) return fail(GC_ERR_CONFIG, "BrickSize must exceed 2*VoxelCovering");
+    const uint32_t m = std::max(dims[0], std::max(dims[1], dims[2]));
+    if (m / brick_size == 0) return fail(GC_ERR_CONFIG, "volume smaller than a brick");
+    std::memset(out, 0, sizeof(*out));
+    const uint32_t nl = level_count(dims, brick_size);
+    if (nl > GC_MAX_LEVELS) return fail(GC_ERR_CONFIG, "too many levels");
+    for (int a = 0; a < 3; ++a) {
+        out->brick_size[a] = brick_size;
+        out->big_brick_size[a] = big_brick[a];
+        out->covering[a] = covering;
+    }
+    out->encoded_bytes = encoded_bytes;
+    out->channels = 1;
+    out->nb_levels = nl;
+    level_sizes(dims, nl, out->initial_sizes);
+    real_sizes(nl, out->initial_sizes, brick_size, covering, big_brick, out->real_sizes);
+    return GC_OK;
+}
+
+int gc_pool_shape(uint64_t budget_bytes, uint64_t brick_bytes, uint32_t out[3])
+{
+    if (!out || brick_bytes == 0) return fail(GC_ERR_ARG, "bad argument");
+    pool_shape(budget_bytes, brick_bytes, out);
+    return GC_OK;
+}
+
+int gc_cfg_get(const char *cfg_path, const char *key, char *value, size_t value_size)
+{
+    if (!cfg_path || !key || !value || value_size == 0) return fail(GC_ERR_ARG, "bad argument");
+    std::string v;
+    if (!cfg_lookup(cfg_path, key, v)) return fail(GC_ERR_ARG, std::string("key not found: ") + key);
+    std::snprintf(value, value_size, "%s", v.c_str());
+    return GC_OK;
+}
+
+int gc_device_count(int *count)
+{
+    if (!count) return fail(GC_ERR_ARG, "null argument");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return GC_OK;
+}
+int gc_set_device(int device) { GC_HIP(hipSetDevice(device)); return GC_OK; }
+int gc_malloc(void **dptr, size_t bytes) { return dalloc(dptr, bytes); }
+int gc_free(void *dptr) { GC_HIP(hipFree(dptr)); return GC_OK; }
+int gc_memcpy_h2d(void *dst, const void *src, size_t bytes) { GC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); return GC_OK; }
+int gc_memcpy_d2h(void *dst, const void *src, size_t bytes) { GC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); return GC_OK; }
+int gc_memset(void *dptr, int value, size_t bytes) { GC_HIP(hipMemset(dptr, value, bytes)); return GC_OK; }
+int gc_synchronize(void) { GC_HIP(hipDeviceSynchronize()); return GC_OK; }
+int gc_stream_create(void **stream)
+{
+    if (!stream) return fail(GC_ERR_ARG, "null argument");
+    hipStream_t s;
+    GC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return GC_OK;
+}
+int gc_stream_destroy(void *stream) { GC_HIP(hipStreamDestroy((hipStream_t)stream)); return GC_OK; }
+int gc_stream_synchronize(void *stream) { GC_HIP(hipStreamSynchronize((hipStream_t)stream)); return GC_OK; }
+
+int gc_mip_reduce(const void *d_in, const uint32_t dims[3], uint32_t encoded_bytes, void *d_out, void *stream)
+{
+    if (!d_in || !d_out || !dims) return fail(GC_ERR_ARG, "null argument");
+    GC_HIP(launch_mip(d_in, dims, encoded_bytes, d_out, (hipStream_t)stream));
+    return GC_OK;
+}
+
+int gc_brick_cut(const void *d_level, const uint32_t dims[3], uint32_t encoded_bytes, uint32_t brick_size,
+                 uint32_t covering, const uint32_t *d_bricks_xyz, uint32_t nb_bricks, void *d_out, void *stream)
+{
+    if (!d_level || !dims || !d_bricks_xyz || !d_out) return fail(GC_ERR_ARG, "null argument");
+    if (brick_size <= 2 * covering) return fail(GC_ERR_CONFIG, "BrickSize must exceed 2*VoxelCovering");
+    GC_HIP(launch_brick_cut(d_level, dims, encoded_bytes, brick_size, covering, d_bricks_xyz, nb_bricks, d_out,
+                            (hipStream_t)stream));
+    return GC_OK;
+}
+
+int gc_source_create_pyramid(const gc_volume_cfg *vc, const void *const *levels, gc_source **out)
+{
+    if (!vc || !levels || !out) return fail(GC_ERR_ARG, "null argument");
+    auto s = std::make_unique<PyramidSource>();
+    s->init(*vc);
+    for (uint32_t l = 0; l < vc->nb_levels; ++l) {
+        if (!levels[l]) return fail(GC_ERR_ARG, "null level pointer");
+        s->levels.push_back(static_cast<const uint8_t *>(levels[l]));
+    }
+    *out = new gc_source{std::move(s)};
+    return GC_OK;
+}
+
+int gc_source_create_synthetic(const gc_volume_cfg *vc, const gc_synth_params *p, gc_source **out)
+{
+    if (!vc || !p || !out) return fail(GC_ERR_ARG, "null argument");
+    auto s = std::make_unique<SynthSource>();
+    s->init(*vc);
+    s->synth = std::make_unique<Synth>(*p, vc->initial_sizes[0], vc->encoded_bytes);
+    s->nthreads = p->threads ? p->threads : hw_threads();
+    *out = new gc_source{std::move(s)};
+    return GC_OK;
+}
+
+int gc_source_create_callback(const gc_volume_cfg *vc, gc_brick_fn fn, void *user, gc_source **out)
+{
+    if (!vc || !fn || !out) return fail(GC_ERR_ARG, "null argument");
+    auto s = std::make_unique<CallbackSource>();
+    s->init(*vc);
+    s->fn = fn;
+    s->user = user;
+    *out = new gc_source{std::move(s)};
+    return GC_OK;
+}
+
+int gc_source_destroy(gc_source *src)
+{
+    delete src;
+    return GC_OK;
+}
+
+int gc_source_get(gc_source *src, uint32_t level, const uint32_t big_brick[3], void *dst, int *status)
+{
+    if (!src || !big_brick || !dst) return fail(GC_ERR_ARG, "null argument");
+    const int s = src->impl->get(level, big_brick, dst);
+    if (status) *status = s;
+    return GC_OK;
+}
+
+int gc_synth_volume(const gc_synth_params *p, const uint32_t dims[3], uint32_t encoded_bytes, void *host_out)
+{
+    if (!p || !dims || !host_out) return fail(GC_ERR_ARG, "null argument");
+    Synth S(*p, dims, encoded_bytes);
+    const uint32_t slab = 8;
+    const uint32_t nslab = (dims[2] + slab - 1) / slab;
+    const uint32_t nth = p->threads ? p->threads : hw_threads();
+    std::memset(host_out, 0, (size_t)dims[0] * dims[1] * dims[2] * encoded_bytes);
+    parallel_for(nslab, nth, [&](uint32_t i) {
+        const int64_t o[3] = {0, 0, (int64_t)i * slab};
+        const uint32_t n[3] = {dims[0], dims[1], std::min(slab, dims[2] - i * slab)};
+        uint8_t *dst = static_cast<uint8_t *>(host_out) + (uint64_t)o[2] * dims[0] * dims[1] * encoded_bytes;
+        S.fill_box(0, o, n, dims, dst);
+    });
+    return GC_OK;
+}
+
+int gc_cache_create(const gc_volume_cfg *vc, const gc_cache_cfg *cc, gc_source *src, int device, gc_cache **out)
+{
+    if (!vc || !cc || !src || !out) return fail(GC_ERR_ARG, "null argument");
+    std::string why;
+    if (!validate(*vc, *cc, why)) return fail(GC_ERR_CONFIG, why);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(GC_ERR_NODEV, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(GC_ERR_ARG, "bad device id");
+    auto c = std::make_unique<gc_cache>();
+    c->impl.vc = *vc;
+    c->impl.cc = *cc;
+    c->impl.src = src->impl.get();
+    c->impl.device = device;
+    const int rc = c->impl.create();
+    if (rc != GC_OK) return rc;
+    *out = c.release();
+    return GC_OK;
+}
+
+int gc_cache_destroy(gc_cache *c)
+{
+    delete c;
+    return GC_OK;
+}
+
+int gc_cache_update(gc_cache *c, int *done)
+{
+    if (!c) return fail(GC_ERR_ARG, "null cache");
+    GC_HIP(hipSetDevice(c->impl.device));
+    return c->impl.update(done);
+}
+
+int gc_cache_completude(gc_cache *c, float *out)
+{
+    if (!c || !out) return fail(GC_ERR_ARG, "null argument");
+    Cache &C = c->impl;
+    GC_HIP(launch_occupancy(C.meta, C.g.nslots, C.d_count, C.stream));
+    uint32_t cnt = 0;
+    GC_HIP(hipMemcpyAsync(&cnt, C.d_count, 4, hipMemcpyDeviceToHost, C.stream));
+    GC_HIP(hipStreamSynchronize(C.stream));
+    *out = (float)cnt / (float)C.g.nslots;
+    return GC_OK;
+}
+
+int gc_cache_load_all(gc_cache *c, const void *const *d_levels)
+{
+    if (!c || !d_levels) return fail(GC_ERR_ARG, "null argument");
+    Cache &C = c->impl;
+    if (C.cc.empty_threshold > 0.f)
+        return fail(GC_ERR_CONFIG, "load_all maps every brick and does not classify empty bricks");
+    if (C.vc.big_brick_size[0] != 1 || C.vc.big_brick_size[1] != 1 || C.vc.big_brick_size[2] != 1)
+        return fail(GC_ERR_CONFIG, "load_all needs BigBrickSize 1");
+    std::vector<uint32_t> asked;
+    const gc_geometry &g = C.g;
+    for (uint32_t z = 0; z < g.mrpd_dim[2]; ++z)
+        for (uint32_t y = 0; y < g.mrpd_dim[1]; ++y)
+            for (uint32_t x = 0; x < g.mrpd_dim[0]; ++x) {
+                uint32_t level = 0;
+                while (x >= g.level_coord[level] + g.level_dim[level][0]) ++level;
+                if (y >= g.level_dim[level][1] || z >= g.level_dim[level][2]) continue;
+                asked.insert(asked.end(), {x - g.level_coord[level], y, z, level});
+            }
+    const uint32_t n = (uint32_t)(asked.size() / 4);
+    if (n > g.nslots) return fail(GC_ERR_CONFIG, "pool too small to hold every brick");
+    uint32_t *d_asked = nullptr, *d_dims = nullptr;
+    const void **d_lv = nullptr;
+    GC_TRY(dalloc(&d_asked, asked.size() * 4));
+    GC_TRY(dalloc(&d_dims, (size_t)g.nb_levels * 12));
+    GC_TRY(dalloc(&d_lv, (size_t)g.nb_levels * sizeof(void *)));
+    GC_HIP(hipMemcpyAsync(d_asked, asked.data(), asked.size() * 4, hipMemcpyHostToDevice, C.stream));
+    GC_HIP(hipMemcpyAsync(d_dims, C.vc.initial_sizes, (size_t)g.nb_levels * 12, hipMemcpyHostToDevice, C.stream));
+    GC_HIP(hipMemcpyAsync(d_lv, d_levels, (size_t)g.nb_levels * sizeof(void *), hipMemcpyHostToDevice, C.stream));
+    GC_HIP(launch_deref(C.mrpd, C.order, g.nslots, n, C.meta, C.d_geom, C.stream));
+    GC_HIP(launch_assign_cut(C.order, g.nslots, n, d_asked, C.meta, C.pool, d_lv, d_dims, C.d_geom,
+                             C.vc.encoded_bytes, C.stream));
+    int serial = 0;
+    GC_TRY(C.insert_serial_needed(asked.data(), n, serial));
+    GC_HIP(launch_insert(C.mrpd, C.order, g.nslots, n, n, d_asked, C.d_geom, serial, C.stream));
+    GC_HIP(hipStreamSynchronize(C.stream));
+    hipFree(d_asked);
+    hipFree(d_dims);
+    hipFree(d_lv);
+    return GC_OK;
+}
+
+int gc_cache_timestamp(gc_cache *c, uint32_t *ts)
+{
+    if (!c || !ts) return fail(GC_ERR_ARG, "null argument");
+    *ts = c->impl.ts;
+    return GC_OK;
+}
+
+int gc_cache_stream(gc_cache *c, void **stream)
+{
+    if (!c || !stream) return fail(GC_ERR_ARG, "null argument");
+    *stream = c->impl.stream;
+    return GC_OK;
+}
+
+int gc_cache_view_get(gc_cache *c, gc_cache_view *out)
+{
+    if (!c || !out) return fail(GC_ERR_ARG, "null argument");
+    Cache &C = c->impl;
+    out->mrpd = C.mrpd;
+    out->request = C.request;
+    out->usage = C.usage;
+    out->pool = C.pool;
+    out->geometry = C.d_geom;
+    out->timestamp = C.ts;
+    out->brick_size = C.g.brick_size;
+    out->covering = C.g.covering;
+    out->encoded_bytes = C.g.encoded_bytes;
+    return GC_OK;
+}
+
+int gc_cache_geometry(gc_cache *c, gc_geometry *out)
+{
+    if (!c || !out) return fail(GC_ERR_ARG, "null argument");
+    *out = c->impl.g;
+    return GC_OK;
+}
+
+static int d2h_sync(Cache &C, void *dst, const void *src, size_t bytes)
+{
+    GC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, C.stream));
+    GC_HIP(hipStreamSynchronize(C.stream));
+    return GC_OK;
+}
+static int h2d_sync(Cache &C, void *dst, const void *src, size_t bytes)
+{
+    GC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, C.stream));
+    GC_HIP(hipStreamSynchronize(C.stream));
+    return GC_OK;
+}
+
+int gc_cache_read_mrpd(gc_cache *c, uint32_t *host)
+{
+    if (!c || !host) return fail(GC_ERR_ARG, "null argument");
+    return d2h_sync(c->impl, host, c->impl.mrpd, c->impl.g.mrpd_cells * 16);
+}
+int gc_cache_read_request(gc_cache *c, uint32_t *host)
+{
+    if (!c || !host) return fail(GC_ERR_ARG, "null argument");
+    return d2h_sync(c->impl, host, c->impl.request, c->impl.g.mrpd_cells * 4);
+}
+int gc_cache_write_request(gc_cache *c, const uint32_t *host)
+{
+    if (!c || !host) return fail(GC_ERR_ARG, "null argument");
+    return h2d_sync(c->impl, c->impl.request, host, c->impl.g.mrpd_cells * 4);
+}
+int gc_cache_read_usage(gc_cache *c, uint32_t *host)
+{
+    if (!c || !host) return fail(GC_ERR_ARG, "null argument");
+    return d2h_sync(c->impl, host, c->impl.usage, (size_t)c->impl.g.nslots * 4);
+}
+int gc_cache_write_usage(gc_cache *c, const uint32_t *host)
+{
+    if (!c || !host) return fail(GC_ERR_ARG, "null argument");
+    return h2d_sync(c->impl, c->impl.usage, host, (size_t)c->impl.g.nslots * 4);
+}
+
+int gc_cache_read_lru(gc_cache *c, gc_lru_entry *host)
+{
+    if (!c || !host) return fail(GC_ERR_ARG, "null argument");
+    Cache &C = c->impl;
+    std::vector<uint32_t> ord(C.g.nslots);
+    std::vector<SlotMeta> mt(C.g.nslots);
+    GC_TRY(d2h_sync(C, ord.data(), C.order, ord.size() * 4));
+    GC_TRY(d2h_sync(C, mt.data(), C.meta, mt.size() * sizeof(SlotMeta)));
+    for (uint32_t i = 0; i < C.g.nslots; ++i) {
+        slot_origin(C.g, ord[i], host[i].cache_pos);
+        host[i].level = mt[ord[i]].level;
+        std::memcpy(host[i].vol_pos, mt[ord[i]].vol, sizeof(host[i].vol_pos));
+    }
+    return GC_OK;
+}
+
+int gc_cache_read_pool(gc_cache *c, void *host)
+{
+    if (!c || !host) return fail(GC_ERR_ARG, "null argument");
+    Cache &C = c->impl;
+    const uint32_t bs = C.g.brick_size, B = C.vc.encoded_bytes;
+    const uint64_t bvox = (uint64_t)bs * bs * bs;
+    std::vector<uint8_t> tmp((size_t)C.g.nslots * bvox * B);
+    GC_TRY(d2h_sync(C, tmp.data(), C.pool, tmp.size()));
+    const uint64_t PX = (uint64_t)C.g.cache_size[0] * bs, PY = (uint64_t)C.g.cache_size[1] * bs;
+    for (uint32_t s = 0; s < C.g.nslots; ++s) {
+        uint32_t o[3];
+        slot_origin(C.g, s, o);
+        for (uint32_t z = 0; z < bs; ++z)
+            for (uint32_t y = 0; y < bs; ++y) {
+                const uint64_t d = (o[0] + PX * ((uint64_t)o[1] + y + PY * ((uint64_t)o[2] + z))) * B;
+                const uint64_t so = ((uint64_t)s * bvox + (uint64_t)bs * (y + (uint64_t)bs * z)) * B;
+                std::memcpy(static_cast<uint8_t *>(host) + d, tmp.data() + so, (size_t)bs * B);
+            }
+    }
+    return GC_OK;
+}
+
+int gc_cache_read_asked(gc_cache *c, uint32_t *host4, uint32_t *nb_asked, uint32_t *nb_non_empty)
+{
+    if (!c) return fail(GC_ERR_ARG, "null cache");
+    Cache &C = c->impl;
+    if (C.handler.upload_pending) GC_HIP(hipEventSynchronize(C.uploaded));
+    if (host4) std::memcpy(host4, C.h_meta, (size_t)C.handler.n * 16);
+    if (nb_asked) *nb_asked = C.handler.n;
+    if (nb_non_empty) *nb_non_empty = C.handler.nne;
+    return GC_OK;
+}
+
+int gc_frame_create(uint32_t width, uint32_t height, uint32_t precision, gc_frame **out)
+{
+    if (!out || width == 0 || height == 0) return fail(GC_ERR_ARG, "bad frame size");
+    if (precision != 32 && precision != 64) return fail(GC_ERR_ARG, "precision must be 32 or 64");
+    auto f = std::make_unique<gc_frame>();
+    Frame &F = f->impl;
+    F.w = width; F.h = height; F.precision = precision;
+    GC_TRY(dalloc(&F.pixels, (size_t)width * height * 4));
+    GC_TRY(dalloc(&F.tbuf, (size_t)width * height * 4));
+    GC_TRY(dalloc(&F.cstate, F.cbytes()));
+    GC_HIP(hipMemset(F.pixels, 0, (size_t)width * height * 4));
+    GC_HIP(hipMemset(F.tbuf, 0, (size_t)width * height * 4));
+    GC_HIP(hipMemset(F.cstate, 0, F.cbytes()));
+    *out = f.release();
+    return GC_OK;
+}
+
+int gc_frame_destroy(gc_frame *f)
+{
+    delete f;
+    return GC_OK;
+}
+
+int gc_frame_reset(gc_frame *f, void *stream)
+{
+    if (!f) return fail(GC_ERR_ARG, "null frame");
+    Frame &F = f->impl;
+    if (!stream) {
+        // no stream given: reset synchronously so that any stream sees it
+        GC_HIP(hipMemset(F.tbuf, 0, (size_t)F.w * F.h * 4));
+        GC_HIP(hipMemset(F.cstate, 0, F.cbytes()));
+        GC_HIP(hipDeviceSynchronize());
+        return GC_OK;
+    }
+    GC_HIP(hipMemsetAsync(F.tbuf, 0, (size_t)F.w * F.h * 4, (hipStream_t)stream));
+    GC_HIP(hipMemsetAsync(F.cstate, 0, F.cbytes(), (hipStream_t)stream));
+    return GC_OK;
+}
+
+int gc_frame_read_pixels(gc_frame *f, uint32_t *host)
+{
+    if (!f || !host) return fail(GC_ERR_ARG, "null argument");
+    GC_HIP(hipDeviceSynchronize());
+    GC_HIP(hipMemcpy(host, f->impl.pixels, (size_t)f->impl.w * f->impl.h * 4, hipMemcpyDeviceToHost));
+    return GC_OK;
+}
+
+int gc_frame_read_state(gc_frame *f, float *tbuf, void *cstate)
+{
+    if (!f) return fail(GC_ERR_ARG, "null frame");
+    GC_HIP(hipDeviceSynchronize());
+    if (tbuf) GC_HIP(hipMemcpy(tbuf, f->impl.tbuf, (size_t)f->impl.w * f->impl.h * 4, hipMemcpyDeviceToHost));
+    if (cstate) GC_HIP(hipMemcpy(cstate, f->impl.cstate, f->impl.cbytes(), hipMemcpyDeviceToHost));
+    return GC_OK;
+}
+
+int gc_frame_buffers(gc_frame *f, uint32_t **pixels, float **tbuf, void **cstate)
+{
+    if (!f) return fail(GC_ERR_ARG, "null frame");
+    if (pixels) *pixels = f->impl.pixels;
+    if (tbuf) *tbuf = f->impl.tbuf;
+    if (cstate) *cstate = f->impl.cstate;
+    return GC_OK;
+}
+
+int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t tf_size, gc_frame *f,
+                  gc_render_stats *d_stats, void *stream)
+{
+    if (!c || !view || !d_tf || !f || tf_size == 0) return fail(GC_ERR_ARG, "bad argument");
+    Cache &C = c->impl;
+    Frame &F = f->impl;
+    if (view->width != F.w || view->height != F.h) return fail(GC_ERR_ARG, "view and frame sizes differ");
+    if (view->precision != F.precision) return fail(GC_ERR_ARG, "view and frame precision differ");
+    DvrParams p{};
+    p.mrpd = C.mrpd;
+    p.request = C.request;
+    p.usage = C.usage;
+    p.pool = C.pool;
+    p.geom = C.d_geom;
+    p.tf = reinterpret_cast<const float4 *>(d_tf);
+    p.tf_size = tf_size;
+    p.ts = C.ts;
+    p.pixels = F.pixels;
+    p.tbuf = F.tbuf;
+    p.cstate = F.cstate;
+    p.stats = d_stats;
+    p.width = view->width;
+    p.height = view->height;
+    std::memcpy(p.m, view->inv_mv, sizeof(p.m));
+    std::memcpy(p.bmin, view->bbox_min, sizeof(p.bmin));
+    std::memcpy(p.bmax, view->bbox_max, sizeof(p.bmax));
+    p.alpha_stop = view->alpha_stop;
+    // VolumeRayCaster.cu:136-161 with the start-up box [-1,1]^3
+    const float smin = 2.f;
+    p.tstep = smin / 256.f;
+    p.nb_lods = C.vc.nb_levels;
+    for (uint32_t i = 0; i < C.vc.nb_levels; ++i) {
+        const uint32_t *ls = C.vc.initial_sizes[i];
+        for (int a = 0; a < 3; ++a) p.inv_levels[i][a] = smin / (float)ls[a];
+        const uint32_t m = std::min(ls[0], std::min(ls[1], ls[2]));
+        p.lod_step[i] = (smin / (float)m) / 2.f;
+        p.kexp[i] = p.lod_step[i] / p.tstep;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : C.stream;
+    GC_HIP(launch_dvr(p, C.vc.encoded_bytes, view->precision, view->tile, s));
+    return GC_OK;
+}
+
+int gc_tf_samples(const float *knots, uint32_t nb_knots, uint32_t n, float *out)
+{
+    if ((!knots && nb_knots) || !out) return fail(GC_ERR_ARG, "null argument");
+    tf_samples(knots, nb_knots, n, out);
+    return GC_OK;
+}
+
+int gc_tf_seeded(uint64_t seed, uint32_t nb_knots, float *knots_out)
+{
+    if (!knots_out) return fail(GC_ERR_ARG, "null argument");
+    tf_seeded(seed, nb_knots, knots_out);
+    return GC_OK;
+}
+
+int gc_camera_matrix(float rx_deg, float ry_deg, const float t[3], float out12[12])
+{
+    if (!t || !out12) return fail(GC_ERR_ARG, "null argument");
+    camera_matrix(rx_deg, ry_deg, t, out12);
+    return GC_OK;
+}
+
+} // extern "C"

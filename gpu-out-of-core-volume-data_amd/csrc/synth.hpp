@@ -1,0 +1,46 @@
+// synth.hpp -- host utilities of libgcvol.so (see synth.cpp).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gcvol.h"
+
+namespace gc {
+
+uint64_t splitmix64(uint64_t &state);
+
+// Seeded synthetic volume: Gaussian blobs plus uniform integer noise,
+// rounded and clamped to the voxel type.  Level l is evaluated at the level-l
+// voxel centres, so every level is synthesised directly (no mip needed).
+struct Synth {
+    struct Blob {
+        double c[3];
+        double k;        // 1 / (2 sigma^2)
+        double amp;
+        double radius;   // 4 sigma
+    };
+    gc_synth_params prm;
+    uint32_t dims[3];
+    uint32_t bytes;
+    double maxval;
+    std::vector<Blob> blobs;
+
+    Synth(const gc_synth_params &p, const uint32_t dims0[3], uint32_t bytes);
+    int noise_at(uint32_t level, int64_t x, int64_t y, int64_t z) const;
+    void fill_box(uint32_t level, const int64_t o[3], const uint32_t n[3], const uint32_t ext[3],
+                  void *dst) const;
+};
+
+void tf_samples(const float *knots, uint32_t nk, uint32_t n, float *out);
+void tf_seeded(uint64_t seed, uint32_t nk, float *knots);
+void camera_matrix(float rx, float ry, const float t[3], float out[12]);
+uint32_t level_count(const uint32_t dims[3], uint32_t bs);
+void level_sizes(const uint32_t dims[3], uint32_t nl, uint32_t (*out)[3]);
+void real_sizes(uint32_t nl, const uint32_t (*levels)[3], uint32_t bs, uint32_t cov, const uint32_t bb[3],
+                uint32_t (*out)[3]);
+void pool_shape(uint64_t budget, uint64_t brick_bytes, uint32_t out[3]);
+bool cfg_lookup(const std::string &path, const std::string &key, std::string &value);
+
+} // namespace gc

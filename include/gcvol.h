@@ -1,0 +1,279 @@
+/*
+ * gcvol.h -- C ABI of the MI355X-native out-of-core volume path
+ * (libgcvol.so, built from gpu-out-of-core-volume-data_amd/csrc).
+ *
+ * Plain C: opaque handles, POD structs, plain pointers and sizes, int status
+ * codes.  Device pointers are HIP device pointers of the calling process;
+ * `stream` arguments are hipStream_t values passed as void* (NULL = the
+ * library's own stream of the handle, or the null stream for free functions).
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the GcCore reference tree).  Where the reference throws
+ * std::runtime_error (CudaError.hpp:24-59) these functions return a non-zero
+ * status and gc_last_error() holds the message; the C++ mirror in
+ * include/tdns/gpucache/CacheManager.hpp turns that back into a throw.
+ */
+#ifndef GCVOL_H
+#define GCVOL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define GC_OK 0
+#define GC_ERR_HIP 1       /* a HIP runtime call failed                        */
+#define GC_ERR_OOM 2       /* device or pinned allocation failed               */
+#define GC_ERR_CONFIG 3    /* invalid volume / cache configuration             */
+#define GC_ERR_SOURCE 4    /* the brick source failed                          */
+#define GC_ERR_ARG 5       /* invalid argument (NULL handle, bad size, ...)    */
+#define GC_ERR_STATE 6     /* call not valid in the current state              */
+#define GC_ERR_NODEV 7     /* no usable GPU                                    */
+
+#define GC_MAX_LEVELS 16
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *gc_last_error(void);
+/* Library version string. */
+const char *gc_version(void);
+
+/* ----------------------------------------------------------- configuration */
+
+/* tdns::data::VolumeConfiguration (VolumeConfiguration.hpp:14-31); field
+ * names follow the .cfg keys BrickSize, BigBrickSizeX/Y/Z, VoxelCovering,
+ * NumberEncodedBytes, NumberChannels (VolumeConfiguration.cpp:24-49). */
+typedef struct gc_volume_cfg {
+    uint32_t brick_size[3];      /* BrickSize (cubic in every .cfg)            */
+    uint32_t big_brick_size[3];  /* BigBrickSize                               */
+    uint32_t covering[3];        /* Covering (VoxelCovering)                   */
+    uint32_t encoded_bytes;      /* EncodedBytes: 1 (u8), 2 (u16), 4 (f32)     */
+    uint32_t channels;           /* Channels: 1                                */
+    uint32_t nb_levels;          /* NbLevels                                   */
+    uint32_t initial_sizes[GC_MAX_LEVELS][3]; /* InitialVolumeSizes            */
+    uint32_t real_sizes[GC_MAX_LEVELS][3];    /* RealVolumesSizes              */
+} gc_volume_cfg;
+
+/* tdns::data::CacheConfiguration (CacheConfiguration.hpp:13-19) plus the
+ * runtime knobs the reference hard-codes. */
+typedef struct gc_cache_cfg {
+    uint32_t cache_size[3];      /* CacheSize[0]: brick slots per axis         */
+    uint32_t block_size[3];      /* BlockSize[0]: the brick size               */
+    uint32_t data_cache_flags;   /* DataCacheFlags (1 = NormalizedAccess)      */
+    uint32_t max_requests;       /* bricks per update; 50 in the reference     */
+    uint32_t sync_requests;      /* 1: requests are served inside update()    */
+    float    empty_threshold;    /* a big brick is Empty if every voxel < this */
+} gc_cache_cfg;
+
+/* Fill a gc_volume_cfg from the preprocessing rules: level count and sizes
+ * (Mipmapper.cu:441-443,583-597) and real sizes (Bricker_v2.cpp:45-76). */
+int gc_volume_cfg_init(gc_volume_cfg *out, const uint32_t dims[3], uint32_t brick_size,
+                       uint32_t covering, const uint32_t big_brick[3], uint32_t encoded_bytes);
+
+/* Pool shape from a byte budget (Application.cu:79-93). */
+int gc_pool_shape(uint64_t budget_bytes, uint64_t brick_bytes, uint32_t out[3]);
+
+/* Parse a GcCore .cfg (<Fields><Field key=.. value=../></Fields>,
+ * ConfigurationPolicies.cpp:16-44) and look one key up.  Returns GC_OK and
+ * copies the value, or GC_ERR_ARG if the key is absent. */
+int gc_cfg_get(const char *cfg_path, const char *key, char *value, size_t value_size);
+
+/* ------------------------------------------------------------ device memory */
+
+int gc_device_count(int *count);
+int gc_set_device(int device);
+int gc_malloc(void **dptr, size_t bytes);
+int gc_free(void *dptr);
+int gc_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int gc_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int gc_memset(void *dptr, int value, size_t bytes);
+int gc_synchronize(void);
+int gc_stream_create(void **stream);
+int gc_stream_destroy(void *stream);
+int gc_stream_synchronize(void *stream);
+
+/* ------------------------------------------------------------ preprocessor */
+
+/* One mip level on the device: out = floor(mean of each 2x2x2 group) of
+ * the level in `in` (x-fastest, dims voxels), output dims floor(dims/2).
+ * Replaces down_sampling / Mipmapper::down_scale (Mipmapper.cu:30-180). */
+int gc_mip_reduce(const void *d_in, const uint32_t dims[3], uint32_t encoded_bytes,
+                  void *d_out, void *stream);
+
+/* Cut bricks (with covering, zero outside the level) out of a device level
+ * into a brick-major device buffer: brick i of the list (uint32 xyz triples
+ * in brick units) goes to d_out + i*bs^3 voxels.  Replaces the per-brick cut
+ * of process_bricking (Bricker_v2.cpp:79-252). */
+int gc_brick_cut(const void *d_level, const uint32_t dims[3], uint32_t encoded_bytes,
+                 uint32_t brick_size, uint32_t covering, const uint32_t *d_bricks_xyz,
+                 uint32_t nb_bricks, void *d_out, void *stream);
+
+/* ------------------------------------------------------------ brick sources */
+
+typedef struct gc_source gc_source;
+
+/* Brick statuses of BricksManager::get_brick (BricksManager.cpp:48-77). */
+#define GC_BRICK_SUCCESS 0
+#define GC_BRICK_EMPTY 1
+#define GC_BRICK_ERROR 2
+
+/* User brick source: write the big brick (level, bx, by, bz) -- BigBrickSize
+ * small bricks laid out as Bricker_v2.cpp:295-299 -- into dst. */
+typedef int (*gc_brick_fn)(void *user, uint32_t level, uint32_t bx, uint32_t by,
+                           uint32_t bz, void *dst);
+
+/* Host-resident pyramid: levels[l] is the l-th level at its initial size
+ * (borrowed, must outlive the source).  Bricks are cut on the host with the
+ * Bricker_v2 covering rule when requested. */
+int gc_source_create_pyramid(const gc_volume_cfg *vc, const void *const *levels,
+                             gc_source **out);
+
+/* Seeded synthetic volume, synthesised per brick on the host on demand
+ * (stands in for the LZ4 brick store, BricksManager.cpp:29-227). */
+typedef struct gc_synth_params {
+    uint64_t seed;
+    uint32_t nb_blobs;           /* Gaussian blobs                             */
+    float    sigma_min, sigma_max;     /* in level-0 voxels                    */
+    float    amp_min, amp_max;         /* in voxel units                       */
+    uint32_t noise;              /* uniform integer noise in [-noise, noise]   */
+    uint32_t threads;            /* host threads per request (0 = hardware)    */
+} gc_synth_params;
+int gc_source_create_synthetic(const gc_volume_cfg *vc, const gc_synth_params *p,
+                               gc_source **out);
+int gc_source_create_callback(const gc_volume_cfg *vc, gc_brick_fn fn, void *user,
+                              gc_source **out);
+int gc_source_destroy(gc_source *src);
+/* Fetch one big brick from a source (for tests / tools). */
+int gc_source_get(gc_source *src, uint32_t level, const uint32_t big_brick[3], void *dst,
+                  int *status);
+
+/* Synthesise a whole level-0 volume (x-fastest) with the synthetic rule. */
+int gc_synth_volume(const gc_synth_params *p, const uint32_t dims[3], uint32_t encoded_bytes,
+                    void *host_out);
+
+/* ------------------------------------------------------------------- cache */
+
+typedef struct gc_cache gc_cache;
+
+/* CacheManager<T>::CacheManager (CacheManager.hpp:136-286). */
+int gc_cache_create(const gc_volume_cfg *vc, const gc_cache_cfg *cc, gc_source *src,
+                    int device, gc_cache **out);
+int gc_cache_destroy(gc_cache *c);                        /* ~CacheManager (:290-295) */
+/* CacheManager<T>::update (:319-354): done = 1 when the last frame raised no
+ * brick request. */
+int gc_cache_update(gc_cache *c, int *done);
+/* CacheManager<T>::completude (:358-383). */
+int gc_cache_completude(gc_cache *c, float *out);
+/* Map every brick of every level in one load, asked list = MRPD linear order,
+ * bricks cut on the device from a device-resident pyramid (d_levels[l] at
+ * the initial sizes).  Fully-resident start-up; no reference counterpart. */
+int gc_cache_load_all(gc_cache *c, const void *const *d_levels);
+int gc_cache_timestamp(gc_cache *c, uint32_t *ts);
+int gc_cache_stream(gc_cache *c, void **stream);
+
+/* Device view handed to kernels: K_CacheManager<T> (K_CacheManager.hpp:34-133)
+ * as plain pointers.  Valid until the next update() or destroy(). */
+typedef struct gc_cache_view {
+    const void *mrpd;            /* uint32 x4 per entry, mrpd_dim, x-fastest   */
+    uint32_t   *request;         /* uint32 per MRPD cell                       */
+    uint32_t   *usage;           /* uint32 per slot                            */
+    const void *pool;            /* slots x bs^3 voxels, brick-major           */
+    const void *geometry;        /* device copy of gc_geometry                 */
+    uint32_t    timestamp;
+    uint32_t    brick_size, covering, encoded_bytes;
+} gc_cache_view;
+int gc_cache_view_get(gc_cache *c, gc_cache_view *out);
+
+/* Derived geometry (CacheManager.hpp:157-285). */
+typedef struct gc_geometry {
+    uint32_t nb_levels, brick_size, covering, inner, encoded_bytes;
+    uint32_t level_dim[GC_MAX_LEVELS][3];   /* bricks per axis (MRPD)          */
+    uint32_t level_coord[GC_MAX_LEVELS];    /* MRPD x offset of each level     */
+    uint32_t virt_vox[GC_MAX_LEVELS][3];    /* virtualised voxels per level    */
+    float    ior[GC_MAX_LEVELS][3];         /* initialOverReal                 */
+    float    rdv[GC_MAX_LEVELS][3];         /* real / virtualised              */
+    float    one_over[3];                   /* 1/(bs-2c)                       */
+    uint32_t mrpd_dim[3];                   /* MRPD and request buffer shape   */
+    uint32_t cache_size[3];                 /* slots per axis                  */
+    uint32_t nslots;
+    uint32_t pad;
+    uint64_t mrpd_cells;
+} gc_geometry;
+int gc_cache_geometry(gc_cache *c, gc_geometry *out);
+
+/* State export/import for parity checks and trace replay.  LRU records are
+ * the reference's LruContent (LruContent.hpp:10-15), in LRU order. */
+typedef struct gc_lru_entry {
+    uint32_t cache_pos[3];
+    uint32_t level;
+    float    vol_pos[3];
+} gc_lru_entry;
+int gc_cache_read_mrpd(gc_cache *c, uint32_t *host);     /* mrpd_cells x 4  */
+int gc_cache_read_request(gc_cache *c, uint32_t *host);  /* mrpd_cells      */
+int gc_cache_write_request(gc_cache *c, const uint32_t *host);
+int gc_cache_read_usage(gc_cache *c, uint32_t *host);    /* nslots          */
+int gc_cache_write_usage(gc_cache *c, const uint32_t *host);
+int gc_cache_read_lru(gc_cache *c, gc_lru_entry *host);  /* nslots          */
+/* Pool in the reference's 3-D layout (cache_size*bs per axis, x-fastest). */
+int gc_cache_read_pool(gc_cache *c, void *host);
+/* Last asked list (uint32 x4, non-empty first) and its non-empty count. */
+int gc_cache_read_asked(gc_cache *c, uint32_t *host4, uint32_t *nb_asked,
+                        uint32_t *nb_non_empty);
+
+/* --------------------------------------------------------------------- DVR */
+
+/* Host-side per-view parameters of RayCastDVR (RayCasters.cu:352-370,
+ * VolumeRayCaster.cu:136-174,374-400). */
+typedef struct gc_view {
+    uint32_t width, height;
+    float    inv_mv[12];         /* d_invModelViewMatrix rows (float4x3)       */
+    float    bbox_min[3], bbox_max[3];
+    float    alpha_stop;         /* early termination; 0.95 in the reference   */
+    uint32_t precision;          /* 32: FP32 compositing, 64: FP64 compositing */
+    uint32_t tile;               /* ray tile: 0 8x8, 1 16x8, 2 32x4, 3 64x1    */
+    uint32_t reserved;
+} gc_view;
+
+/* Per-view progressive state (tBuffer, colorStateBuffer, pixel buffer) on the
+ * device (VolumeRayCaster.cu:225-238). */
+typedef struct gc_frame gc_frame;
+int gc_frame_create(uint32_t width, uint32_t height, uint32_t precision, gc_frame **out);
+int gc_frame_destroy(gc_frame *f);
+/* Camera change: zero tBuffer and colour state (VolumeRayCaster.cu:320-329). */
+int gc_frame_reset(gc_frame *f, void *stream);
+int gc_frame_read_pixels(gc_frame *f, uint32_t *host);          /* W*H ABGR  */
+int gc_frame_read_state(gc_frame *f, float *tbuf, void *cstate); /* either may be NULL */
+/* Device pointers of the frame buffers. */
+int gc_frame_buffers(gc_frame *f, uint32_t **pixels, float **tbuf, void **cstate);
+
+/* Render statistics accumulated by the ray kernel on the device. */
+typedef struct gc_render_stats {
+    uint64_t mapped_samples;     /* S: ray steps whose translation was Mapped  */
+    uint64_t brick_visits;       /* V: page-entry loads                        */
+    uint64_t pixels;             /* P                                           */
+    uint64_t unmapped_rays;      /* rays that stopped at an Unmapped brick     */
+} gc_render_stats;
+
+/* One launch of the DVR ray caster over the cache (RayCastDVR,
+ * RayCasters.cu:36-198,352-408).  tf: device pointer to tf_size float4.
+ * stats: device pointer to a gc_render_stats to accumulate into, or NULL. */
+int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t tf_size,
+                  gc_frame *f, gc_render_stats *d_stats, void *stream);
+
+/* ------------------------------------------------------------------ helpers */
+
+/* Transfer-function samples at x = i/N from control points
+ * {value, alpha, r, g, b} (TransferFunction.cpp:121-157). */
+int gc_tf_samples(const float *knots, uint32_t nb_knots, uint32_t n, float *out);
+/* Seeded piecewise-linear ramp with nb_knots control points. */
+int gc_tf_seeded(uint64_t seed, uint32_t nb_knots, float *knots_out);
+/* Camera of the reference viewer: rotate by -rx about X, by -ry about Y,
+ * translate by -t (Display_helper.cpp:93-118); degrees. */
+int gc_camera_matrix(float rx_deg, float ry_deg, const float t[3], float out12[12]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,179 @@
+/*
+ * gcvol_oracle.h -- CPU restatement of GcCore's out-of-core volume path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the CPU
+ * baseline; it is never linked into, loaded by, or called from the product
+ * library (libgcvol.so).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it.
+ *
+ * Every function cites the reference file:line whose rule it restates
+ * (paths relative to the GcCore reference tree).  Pinning of each function
+ * (closed forms, brute force, invariants) lives in tests/test_oracle_*.py.
+ *
+ * Conventions
+ *   - 3-D arrays are x-fastest: idx = x + X*(y + Y*z)
+ *     (K_DynamicArray3dDevice.hpp:119-123).
+ *   - Voxel types: bytes = 1 (uint8), 2 (uint16), 4 (float32)
+ *     (Application.cu:104-128).
+ *   - FP32 bookkeeping formulas are evaluated in IEEE single precision with
+ *     no contraction (the library is built with -ffp-contract=off).
+ */
+#ifndef GCVOL_ORACLE_H
+#define GCVOL_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAX_LEVELS 16
+
+/* ---------------------------------------------------------------- geometry */
+
+/* levelMax + 1 with levelMax = ceil(log(maxDim / bs) / log 2), integer
+ * division maxDim/bs (Mipmapper.cu:441-443). */
+uint32_t or_level_count(const uint32_t dims[3], uint32_t brick_size);
+
+/* Level sizes: size_{l+1} = size_l >= r ? size_l / r : size_l
+ * (Mipmapper.cu:583-597, fill_metaData). out: nb_levels x 3. */
+void or_level_sizes(const uint32_t dims[3], const uint32_t ratio[3],
+                    uint32_t nb_levels, uint32_t *out);
+
+/* Bricker metadata (Bricker_v2.cpp:45-76): per level bricks per axis rounded
+ * up to a multiple of the big-brick size, big bricks per axis and the real
+ * (padded) level size.  Any output pointer may be NULL. */
+void or_init_meta(uint32_t nb_levels, const uint32_t *levels, uint32_t brick_size,
+                  uint32_t covering, const uint32_t big_brick[3],
+                  uint32_t *nb_bricks, uint32_t *nb_big_bricks, uint32_t *real_sizes);
+
+/* Pool shape from a byte budget (Application.cu:79-93). */
+void or_pool_shape(uint64_t budget_bytes, uint64_t brick_bytes, uint32_t out[3]);
+
+/* ------------------------------------------------------------ preprocessor */
+
+/* Whole-level down-sampling, out[x,y,z] = mean of the r.x*r.y*r.z group,
+ * floored for integer types (Mipmapper.cu:30-105): uint8 sums in uint64 and
+ * divides; uint16 sums in double (start 0) and truncates; float sums in double
+ * and rounds to float.  Output dims are floor(dims / r) per axis (the writer
+ * drops the trailing odd voxel, Mipmapper.cu:44-45,248). */
+void or_mip_reduce(const void *in, const uint32_t dims[3], uint32_t bytes,
+                   const uint32_t ratio[3], void *out);
+
+/* Cut one big brick (BB*bs per axis, small bricks at brickPos*bs) out of a
+ * level with the covering rule of Bricker_v2.cpp:138-222: local voxel k of
+ * small brick b maps to global b*(bs-2c) + k - c, zero outside the level. */
+void or_brick_extract(const void *level, const uint32_t dims[3], uint32_t bytes,
+                      uint32_t brick_size, uint32_t covering, const uint32_t big_brick[3],
+                      const uint32_t big_brick_coord[3], void *out);
+
+/* ------------------------------------------------------------------- cache */
+
+typedef struct {
+    uint32_t brick_size;              /* BrickSize                          */
+    uint32_t covering;                /* VoxelCovering                      */
+    uint32_t bytes;                   /* NumberEncodedBytes                 */
+    uint32_t big_brick[3];            /* BigBrickSizeX/Y/Z                  */
+    uint32_t nb_levels;
+    uint32_t initial[OR_MAX_LEVELS][3];   /* InitialVolumeSizes             */
+    uint32_t real[OR_MAX_LEVELS][3];      /* RealVolumesSizes               */
+    uint32_t cache_size[3];           /* CacheSize[0], in bricks            */
+    uint32_t max_requests;            /* _nbMaxRequests (50)                */
+    float    empty_threshold;         /* big brick Empty iff all voxels < t */
+} or_cfg;
+
+typedef struct {
+    uint32_t level_dim[OR_MAX_LEVELS][3];   /* realNumberOfEntries[1]        */
+    uint32_t level_coord[OR_MAX_LEVELS];    /* MRPD x offset of each level   */
+    uint32_t virt_vox[OR_MAX_LEVELS][3];    /* virtualizedNumberOfEntries[0] */
+    float    ior[OR_MAX_LEVELS][3];         /* initialOverReal               */
+    float    rdv[OR_MAX_LEVELS][3];         /* data cache real/virtual       */
+    float    one_over[3];                   /* 1/(bs-2c)                     */
+    uint32_t mrpd_dim[3];                   /* also the request buffer shape */
+    uint32_t pool_dim[3];                   /* voxels                        */
+    uint32_t nslots;
+} or_geom;
+
+/* LRU record (LruContent.hpp:10-15), 28 bytes. */
+typedef struct {
+    uint32_t cache_pos[3];
+    uint32_t level;
+    float    vol_pos[3];
+} or_lru;
+
+typedef struct or_cache or_cache;
+
+/* CacheManager ctor geometry (CacheManager.hpp:157-285). */
+void or_geometry(const or_cfg *cfg, or_geom *g);
+
+/* Create a cache over a fully host-resident pyramid: levels[l] points to the
+ * initial-size level-l voxels.  The pointers are borrowed. */
+or_cache *or_cache_create(const or_cfg *cfg, const void *const *levels);
+void      or_cache_destroy(or_cache *c);
+const or_geom *or_cache_geom(const or_cache *c);
+
+/* Raw state, exposed for comparison and for trace replay. */
+uint32_t *or_cache_mrpd(or_cache *c);      /* 4 x uint32 per entry          */
+uint32_t *or_cache_request(or_cache *c);   /* uint32 per MRPD cell          */
+uint32_t *or_cache_usage(or_cache *c);     /* uint32 per slot, x-fastest    */
+or_lru   *or_cache_lru(or_cache *c);       /* nslots records in LRU order   */
+void     *or_cache_pool(or_cache *c);      /* pool_dim voxels, x-fastest    */
+uint32_t  or_cache_timestamp(const or_cache *c);
+
+/* CacheManager::update with a synchronous request handler
+ * (CacheManager.hpp:319-354; the handler finishes inside notify). */
+int  or_cache_update(or_cache *c);
+/* The individual steps, for step-wise comparison. */
+void or_lru_update(or_cache *c);                                        /* :387-406 */
+uint32_t or_request_list(or_cache *c, uint32_t *out4, uint32_t cap);    /* :410-474 */
+float or_completude(const or_cache *c);                                 /* :358-383 */
+/* Last handled request: asked list (uint4, non-empty first) and counts. */
+uint32_t or_asked(const or_cache *c, uint32_t *out4, uint32_t *nb_non_empty);
+
+/* Map every brick as one load with the asked list = all bricks of every
+ * level in MRPD linear order (no cap). */
+void or_cache_load_all(or_cache *c);
+
+/* FP32 page-table position of a brick (CacheManagerKernels.cu:75-86). */
+void or_volume_position(const or_geom *g, uint32_t level, const uint32_t brick[3], float out[3]);
+/* MRPD cell of a float position (K_MultiResolutionPageDirectory.hpp:168-180). */
+uint64_t or_mrpd_index(const or_geom *g, uint32_t level, const float pos[3]);
+
+/* ------------------------------------------------------------------- DVR */
+
+typedef struct {
+    uint32_t width, height;
+    float    inv_mv[12];               /* float4x3, rows (RayCasters_helpers.hpp:22-27) */
+    float    bbox_min[3], bbox_max[3];
+    float    tstep;                    /* marchingStep                   */
+    float    alpha_stop;               /* 0.95f in the reference         */
+    uint32_t nb_lods;
+    float    inv_levels_size[OR_MAX_LEVELS][3];
+    float    lod_step[OR_MAX_LEVELS];
+    const float *tf;                   /* tf_size float4 (RGBA)          */
+    uint32_t tf_size;
+} or_view;
+
+/* Host-side LOD tables (VolumeRayCaster.cu:136-161) for a [-1,1]^3 box. */
+void or_view_tables(or_view *v, const or_cfg *cfg);
+
+/* Transfer-function samples at x = i/N, piecewise linear between control
+ * points with implicit end points (TransferFunction.cpp:121-157).
+ * knots: nk x {value, alpha, r, g, b}, sorted by value. out: N float4. */
+void or_tf_samples(const float *knots, uint32_t nk, uint32_t n, float *out);
+
+/* One launch of RayCastDVRImpl (RayCasters.cu:36-198) over the cache.
+ * Geometry (ray set-up, positions, LOD, page-table and voxel indices) is
+ * evaluated in FP32 exactly as the reference orders it; value filtering,
+ * transfer function, opacity correction and compositing are evaluated in
+ * double when precision == 64, in float when precision == 32.
+ * tbuf: W*H floats; cstate: W*H*4 doubles (colour state, RGBA).
+ * pixels: W*H uint32 ABGR.  stats (may be NULL): {mapped samples, brick
+ * visits, pixels, rays that stopped at an Unmapped brick}. */
+void or_render_dvr(or_cache *c, const or_view *v, float *tbuf, double *cstate,
+                   uint32_t *pixels, int precision, int nthreads, uint64_t stats[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
