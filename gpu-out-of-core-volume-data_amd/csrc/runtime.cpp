@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <functional>
 #include <map>
@@ -300,6 +301,9 @@ struct Cache {
     Handler handler;
     std::mutex done_m;
     bool request_done = false;
+    // frame driver
+    gc_render_stats *d_frame_stats = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~Cache() { release(); }
     void release();
@@ -326,6 +330,11 @@ void Cache::release()
     if (h_meta) hipHostFree(h_meta);
     if (uploaded) hipEventDestroy(uploaded);
     if (consumed) hipEventDestroy(consumed);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    hipFree(d_frame_stats);
+    d_frame_stats = nullptr;
+    ev0 = ev1 = nullptr;
     if (stream) hipStreamDestroy(stream);
     if (upload) hipStreamDestroy(upload);
     mrpd = nullptr; request = usage = order = order_tmp = nullptr; meta = nullptr; pool = nullptr;
@@ -1138,6 +1147,58 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
     }
     hipStream_t s = stream ? (hipStream_t)stream : C.stream;
     GC_HIP(launch_dvr(p, C.vc.encoded_bytes, view->precision, view->tile, s));
+    return GC_OK;
+}
+
+int gc_cache_wait_idle(gc_cache *c)
+{
+    if (!c) return fail(GC_ERR_ARG, "null cache");
+    Cache &C = c->impl;
+    while (C.handler.new_request.load()) std::this_thread::sleep_for(std::chrono::microseconds(50));
+    return GC_OK;
+}
+
+int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t tf_size, gc_frame *f,
+                    uint32_t max_launches, gc_frame_report *report)
+{
+    if (!c || !view || !d_tf || !f || !report) return fail(GC_ERR_ARG, "null argument");
+    Cache &C = c->impl;
+    GC_HIP(hipSetDevice(C.device));
+    std::memset(report, 0, sizeof(*report));
+    if (!C.d_frame_stats) {
+        GC_TRY(dalloc(&C.d_frame_stats, sizeof(gc_render_stats)));
+        GC_HIP(hipEventCreate(&C.ev0));
+        GC_HIP(hipEventCreate(&C.ev1));
+    }
+    GC_HIP(hipMemsetAsync(C.d_frame_stats, 0, sizeof(gc_render_stats), C.stream));
+    for (uint32_t it = 0; it < max_launches; ++it) {
+        GC_HIP(hipEventRecord(C.ev0, C.stream));
+        GC_TRY(gc_render_dvr(c, view, d_tf, tf_size, f, C.d_frame_stats, C.stream));
+        GC_HIP(hipEventRecord(C.ev1, C.stream));
+        ++report->launches;
+        int done = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        GC_TRY(C.update(&done));
+        ++report->updates;
+        GC_HIP(hipEventSynchronize(C.ev1));
+        float ms = 0.f;
+        GC_HIP(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+        report->kernel_ms += ms;
+        if (done) {
+            report->update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            report->complete = 1;
+            break;
+        }
+        // a request was issued: wait for it and load it before the next launch
+        GC_TRY(gc_cache_wait_idle(c));
+        const uint32_t before = C.handler.nne;
+        GC_TRY(C.update(&done));
+        ++report->updates;
+        report->bricks_loaded += before;
+        report->update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    GC_HIP(hipMemcpyAsync(&report->stats, C.d_frame_stats, sizeof(gc_render_stats), hipMemcpyDeviceToHost, C.stream));
+    GC_HIP(hipStreamSynchronize(C.stream));
     return GC_OK;
 }
 

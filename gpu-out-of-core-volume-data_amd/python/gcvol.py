@@ -74,6 +74,22 @@ class RenderStats(C.Structure):
     _fields_ = [("mapped_samples", u64), ("brick_visits", u64), ("pixels", u64), ("unmapped_rays", u64)]
 
 
+class FrameReport(C.Structure):
+    _fields_ = [
+        ("launches", u32), ("updates", u32), ("complete", u32), ("reserved", u32), ("bricks_loaded", u64),
+        ("stats", RenderStats), ("kernel_ms", C.c_double), ("update_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {
+            "launches": self.launches, "updates": self.updates, "complete": bool(self.complete),
+            "bricks_loaded": int(self.bricks_loaded), "mapped_samples": int(self.stats.mapped_samples),
+            "brick_visits": int(self.stats.brick_visits), "pixels": int(self.stats.pixels),
+            "unmapped_rays": int(self.stats.unmapped_rays), "kernel_ms": self.kernel_ms,
+            "update_ms": self.update_ms,
+        }
+
+
 class CacheView(C.Structure):
     _fields_ = [
         ("mrpd", vp), ("request", vp), ("usage", vp), ("pool", vp), ("geometry", vp),
@@ -92,7 +108,7 @@ EXPORTS = [
     "gc_cache_read_request", "gc_cache_write_request", "gc_cache_read_usage", "gc_cache_write_usage",
     "gc_cache_read_lru", "gc_cache_read_pool", "gc_cache_read_asked", "gc_frame_create", "gc_frame_destroy",
     "gc_frame_reset", "gc_frame_read_pixels", "gc_frame_read_state", "gc_frame_buffers", "gc_render_dvr",
-    "gc_tf_samples", "gc_tf_seeded", "gc_camera_matrix",
+    "gc_tf_samples", "gc_tf_seeded", "gc_camera_matrix", "gc_cache_wait_idle", "gc_render_frame",
 ]
 
 _lib = None
@@ -108,20 +124,44 @@ def lib():
             getattr(L, name).restype = C.c_int
         L.gc_last_error.restype = C.c_char_p
         L.gc_version.restype = C.c_char_p
-        L.gc_pool_shape.argtypes = [u64, u64, vp]
-        L.gc_tf_seeded.argtypes = [u64, u32, vp]
-        L.gc_camera_matrix.argtypes = [f32, f32, vp, vp]
-        L.gc_render_dvr.argtypes = [vp, C.POINTER(View), vp, u32, vp, vp, vp]
-        L.gc_frame_create.argtypes = [u32, u32, u32, C.POINTER(vp)]
-        L.gc_mip_reduce.argtypes = [vp, vp, u32, vp, vp]
-        L.gc_brick_cut.argtypes = [vp, vp, u32, u32, u32, vp, u32, vp, vp]
-        L.gc_malloc.argtypes = [C.POINTER(vp), C.c_size_t]
-        L.gc_free.argtypes = [vp]
-        L.gc_memcpy_h2d.argtypes = [vp, vp, C.c_size_t]
-        L.gc_memcpy_d2h.argtypes = [vp, vp, C.c_size_t]
-        L.gc_memset.argtypes = [vp, C.c_int, C.c_size_t]
-        L.gc_source_get.argtypes = [vp, u32, vp, vp, C.POINTER(C.c_int)]
-        L.gc_cache_update.argtypes = [vp, C.POINTER(C.c_int)]
+        # Every export gets explicit argument types: without them ctypes passes
+        # Python ints as 32-bit C ints and silently truncates pointers.
+        P, I, SZ = vp, C.c_int, C.c_size_t
+        sig = {
+            "gc_last_error": [], "gc_version": [],
+            "gc_volume_cfg_init": [C.POINTER(VolumeCfg), P, u32, u32, P, u32],
+            "gc_pool_shape": [u64, u64, P], "gc_cfg_get": [C.c_char_p, C.c_char_p, C.c_char_p, SZ],
+            "gc_device_count": [C.POINTER(I)], "gc_set_device": [I],
+            "gc_malloc": [C.POINTER(vp), SZ], "gc_free": [P], "gc_memcpy_h2d": [P, P, SZ],
+            "gc_memcpy_d2h": [P, P, SZ], "gc_memset": [P, I, SZ], "gc_synchronize": [],
+            "gc_stream_create": [C.POINTER(vp)], "gc_stream_destroy": [P], "gc_stream_synchronize": [P],
+            "gc_mip_reduce": [P, P, u32, P, P], "gc_brick_cut": [P, P, u32, u32, u32, P, u32, P, P],
+            "gc_source_create_pyramid": [C.POINTER(VolumeCfg), P, C.POINTER(vp)],
+            "gc_source_create_synthetic": [C.POINTER(VolumeCfg), C.POINTER(SynthParams), C.POINTER(vp)],
+            "gc_source_create_callback": [C.POINTER(VolumeCfg), P, P, C.POINTER(vp)],
+            "gc_source_destroy": [P], "gc_source_get": [P, u32, P, P, C.POINTER(I)],
+            "gc_synth_volume": [C.POINTER(SynthParams), P, u32, P],
+            "gc_cache_create": [C.POINTER(VolumeCfg), C.POINTER(CacheCfg), P, I, C.POINTER(vp)],
+            "gc_cache_destroy": [P], "gc_cache_update": [P, C.POINTER(I)],
+            "gc_cache_completude": [P, C.POINTER(f32)], "gc_cache_load_all": [P, P],
+            "gc_cache_timestamp": [P, C.POINTER(u32)], "gc_cache_stream": [P, C.POINTER(vp)],
+            "gc_cache_view_get": [P, C.POINTER(CacheView)], "gc_cache_geometry": [P, C.POINTER(Geometry)],
+            "gc_cache_read_mrpd": [P, P], "gc_cache_read_request": [P, P], "gc_cache_write_request": [P, P],
+            "gc_cache_read_usage": [P, P], "gc_cache_write_usage": [P, P], "gc_cache_read_lru": [P, P],
+            "gc_cache_read_pool": [P, P], "gc_cache_read_asked": [P, P, C.POINTER(u32), C.POINTER(u32)],
+            "gc_frame_create": [u32, u32, u32, C.POINTER(vp)], "gc_frame_destroy": [P],
+            "gc_frame_reset": [P, P], "gc_frame_read_pixels": [P, P], "gc_frame_read_state": [P, P, P],
+            "gc_frame_buffers": [P, P, P, P],
+            "gc_render_dvr": [P, C.POINTER(View), P, u32, P, P, P],
+            "gc_tf_samples": [P, u32, u32, P], "gc_tf_seeded": [u64, u32, P],
+            "gc_camera_matrix": [f32, f32, P, P], "gc_cache_wait_idle": [P],
+            "gc_render_frame": [P, C.POINTER(View), P, u32, P, u32, C.POINTER(FrameReport)],
+        }
+        missing = [n for n in EXPORTS if n not in sig]
+        if missing:
+            raise GcError("binding lacks argument types for %s" % missing)
+        for name, args in sig.items():
+            getattr(L, name).argtypes = args
         _lib = L
     return _lib
 
@@ -420,6 +460,12 @@ def make_view(width, height, inv_mv, alpha_stop=0.95, precision=32, tile=0,
 def render(cache, view, d_tf, tf_size, frame, d_stats=None, stream=None):
     check(lib().gc_render_dvr(cache.h, C.byref(view), d_tf.p, tf_size, frame.h,
                               d_stats.p if d_stats is not None else None, stream))
+
+
+def render_frame(cache, view, d_tf, tf_size, frame, max_launches=100000):
+    rep = FrameReport()
+    check(lib().gc_render_frame(cache.h, C.byref(view), d_tf.p, tf_size, frame.h, max_launches, C.byref(rep)))
+    return rep
 
 
 def unpack_rgba(px):

@@ -262,6 +262,29 @@ typedef struct gc_render_stats {
 int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t tf_size,
                   gc_frame *f, gc_render_stats *d_stats, void *stream);
 
+/* Block until the request handler has no request in flight. */
+int gc_cache_wait_idle(gc_cache *c);
+
+/* Per-frame report of gc_render_frame. */
+typedef struct gc_frame_report {
+    uint32_t launches;           /* ray-caster launches                        */
+    uint32_t updates;            /* CacheManager::update calls                 */
+    uint32_t complete;           /* 1: the last launch raised no request       */
+    uint32_t reserved;
+    uint64_t bricks_loaded;      /* non-empty bricks written into the pool     */
+    gc_render_stats stats;       /* summed over the launches                   */
+    double   kernel_ms;          /* summed ray-caster time (HIP events)        */
+    double   update_ms;          /* summed host time inside update()           */
+} gc_frame_report;
+
+/* The plain single-view renderer's progressive loop (VolumeRayCaster.cu:
+ * 261-330 without the display): launch the ray caster, update the cache,
+ * repeat until an update reports no request or max_launches is reached.
+ * While a request is in flight the loop waits for it and loads it before the
+ * next launch instead of re-launching on unchanged bricks. */
+int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t tf_size,
+                    gc_frame *f, uint32_t max_launches, gc_frame_report *report);
+
 /* ------------------------------------------------------------------ helpers */
 
 /* Transfer-function samples at x = i/N from control points

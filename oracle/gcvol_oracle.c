@@ -615,7 +615,7 @@ typedef struct {
     double *cstate;
     uint32_t *pixels;
     int precision;
-    uint32_t next_row;
+    uint32_t next_row, end_row;
     pthread_mutex_t lock;
     uint64_t stats[4];
 } dvr_job;
@@ -863,7 +863,7 @@ static void *dvr_worker(void *arg)
         pthread_mutex_lock(&job->lock);
         const uint32_t y = job->next_row++;
         pthread_mutex_unlock(&job->lock);
-        if (y >= job->v->height) break;
+        if (y >= job->end_row) break;
         for (uint32_t x = 0; x < job->v->width; ++x) dvr_pixel(job, x, y, st);
     }
     pthread_mutex_lock(&job->lock);
@@ -875,10 +875,19 @@ static void *dvr_worker(void *arg)
 void or_render_dvr(or_cache *c, const or_view *v, float *tbuf, double *cstate,
                    uint32_t *pixels, int precision, int nthreads, uint64_t stats[4])
 {
+    or_render_dvr_rows(c, v, tbuf, cstate, pixels, precision, nthreads, 0, v->height, stats);
+}
+
+void or_render_dvr_rows(or_cache *c, const or_view *v, float *tbuf, double *cstate,
+                        uint32_t *pixels, int precision, int nthreads, uint32_t y0, uint32_t rows,
+                        uint64_t stats[4])
+{
     dvr_job job;
     memset(&job, 0, sizeof(job));
     job.c = c; job.v = v; job.tbuf = tbuf; job.cstate = cstate; job.pixels = pixels;
     job.precision = precision;
+    job.next_row = y0;
+    job.end_row = y0 + rows > v->height ? v->height : y0 + rows;
     pthread_mutex_init(&job.lock, NULL);
     if (nthreads < 1) nthreads = 1;
     pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);

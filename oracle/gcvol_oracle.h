@@ -172,6 +172,10 @@ void or_tf_samples(const float *knots, uint32_t nk, uint32_t n, float *out);
  * visits, pixels, rays that stopped at an Unmapped brick}. */
 void or_render_dvr(or_cache *c, const or_view *v, float *tbuf, double *cstate,
                    uint32_t *pixels, int precision, int nthreads, uint64_t stats[4]);
+/* The same launch restricted to pixel rows [y0, y0+rows). */
+void or_render_dvr_rows(or_cache *c, const or_view *v, float *tbuf, double *cstate,
+                        uint32_t *pixels, int precision, int nthreads, uint32_t y0, uint32_t rows,
+                        uint64_t stats[4]);
 
 #ifdef __cplusplus
 }
