@@ -130,6 +130,9 @@ def main():
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of known traffic "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -140,6 +143,15 @@ def main():
     if G.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+
+    if args.calib_copy_mib > 0:
+        nbytes = args.calib_copy_mib << 20
+        a, b = G.DeviceBuffer(nbytes), G.DeviceBuffer(nbytes)
+        G.check(G.lib().gc_memset(a.p, 1, nbytes))
+        G.check(G.lib().gc_memcpy_d2d(b.p, a.p, nbytes))
+        G.synchronize()
+        a.free()
+        b.free()
 
     vc = G.volume_cfg(DIMS, BRICK, COVER, BYTES)
     params = G.synth_params(SEED, 64, sigma=(40.0, 200.0), amp=(15000.0, 65535.0), noise=2048, threads=0)

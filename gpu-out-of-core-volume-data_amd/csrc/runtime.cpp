@@ -736,6 +736,7 @@ int gc_malloc(void **dptr, size_t bytes) { return dalloc(dptr, bytes); }
 int gc_free(void *dptr) { GC_HIP(hipFree(dptr)); return GC_OK; }
 int gc_memcpy_h2d(void *dst, const void *src, size_t bytes) { GC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); return GC_OK; }
 int gc_memcpy_d2h(void *dst, const void *src, size_t bytes) { GC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); return GC_OK; }
+int gc_memcpy_d2d(void *dst, const void *src, size_t bytes) { GC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); return GC_OK; }
 int gc_memset(void *dptr, int value, size_t bytes) { GC_HIP(hipMemset(dptr, value, bytes)); return GC_OK; }
 int gc_synchronize(void) { GC_HIP(hipDeviceSynchronize()); return GC_OK; }
 int gc_stream_create(void **stream)

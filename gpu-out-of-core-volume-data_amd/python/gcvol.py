@@ -99,7 +99,7 @@ class CacheView(C.Structure):
 
 EXPORTS = [
     "gc_last_error", "gc_version", "gc_volume_cfg_init", "gc_pool_shape", "gc_cfg_get",
-    "gc_device_count", "gc_set_device", "gc_malloc", "gc_free", "gc_memcpy_h2d", "gc_memcpy_d2h",
+    "gc_device_count", "gc_set_device", "gc_malloc", "gc_free", "gc_memcpy_h2d", "gc_memcpy_d2h", "gc_memcpy_d2d",
     "gc_memset", "gc_synchronize", "gc_stream_create", "gc_stream_destroy", "gc_stream_synchronize",
     "gc_mip_reduce", "gc_brick_cut", "gc_source_create_pyramid", "gc_source_create_synthetic",
     "gc_source_create_callback", "gc_source_destroy", "gc_source_get", "gc_synth_volume",
@@ -133,7 +133,8 @@ def lib():
             "gc_pool_shape": [u64, u64, P], "gc_cfg_get": [C.c_char_p, C.c_char_p, C.c_char_p, SZ],
             "gc_device_count": [C.POINTER(I)], "gc_set_device": [I],
             "gc_malloc": [C.POINTER(vp), SZ], "gc_free": [P], "gc_memcpy_h2d": [P, P, SZ],
-            "gc_memcpy_d2h": [P, P, SZ], "gc_memset": [P, I, SZ], "gc_synchronize": [],
+            "gc_memcpy_d2h": [P, P, SZ], "gc_memcpy_d2d": [P, P, SZ], "gc_memset": [P, I, SZ],
+            "gc_synchronize": [],
             "gc_stream_create": [C.POINTER(vp)], "gc_stream_destroy": [P], "gc_stream_synchronize": [P],
             "gc_mip_reduce": [P, P, u32, P, P], "gc_brick_cut": [P, P, u32, u32, u32, P, u32, P, P],
             "gc_source_create_pyramid": [C.POINTER(VolumeCfg), P, C.POINTER(vp)],

@@ -88,6 +88,7 @@ int gc_malloc(void **dptr, size_t bytes);
 int gc_free(void *dptr);
 int gc_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int gc_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int gc_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int gc_memset(void *dptr, int value, size_t bytes);
 int gc_synchronize(void);
 int gc_stream_create(void **stream);
