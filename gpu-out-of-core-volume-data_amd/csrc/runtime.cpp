@@ -955,6 +955,15 @@ int gc_cache_geometry(gc_cache *c, gc_geometry *out)
     return GC_OK;
 }
 
+int gc_geometry_compute(const gc_volume_cfg *vc, const gc_cache_cfg *cc, gc_geometry *out)
+{
+    if (!vc || !cc || !out) return fail(GC_ERR_ARG, "null argument");
+    std::string why;
+    if (!validate(*vc, *cc, why)) return fail(GC_ERR_CONFIG, why);
+    compute_geometry(*vc, *cc, *out);
+    return GC_OK;
+}
+
 static int d2h_sync(Cache &C, void *dst, const void *src, size_t bytes)
 {
     GC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, C.stream));

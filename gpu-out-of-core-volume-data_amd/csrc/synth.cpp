@@ -245,7 +245,12 @@ bool cfg_lookup(const std::string &path, const std::string &key, std::string &va
     if (!f) return false;
     std::stringstream ss;
     ss << f.rdbuf();
-    const std::string s = ss.str();
+    std::string s = ss.str();
+    // drop <!-- ... --> comments
+    for (size_t c0; (c0 = s.find("<!--")) != std::string::npos;) {
+        const size_t c1 = s.find("-->", c0 + 4);
+        s.erase(c0, c1 == std::string::npos ? std::string::npos : c1 + 3 - c0);
+    }
     size_t pos = 0;
     auto attr = [&](size_t from, size_t to, const char *name, std::string &out) {
         const std::string pat = std::string(name) + "=\"";
@@ -259,7 +264,6 @@ bool cfg_lookup(const std::string &path, const std::string &key, std::string &va
     };
     bool found = false;
     while ((pos = s.find("<Field", pos)) != std::string::npos) {
-        if (pos >= 4 && s.compare(pos - 4, 4, "<!--") == 0) { pos += 6; continue; }
         const size_t end = s.find('>', pos);
         if (end == std::string::npos) break;
         std::string k, v;

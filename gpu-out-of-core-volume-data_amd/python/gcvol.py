@@ -104,7 +104,8 @@ EXPORTS = [
     "gc_mip_reduce", "gc_brick_cut", "gc_source_create_pyramid", "gc_source_create_synthetic",
     "gc_source_create_callback", "gc_source_destroy", "gc_source_get", "gc_synth_volume",
     "gc_cache_create", "gc_cache_destroy", "gc_cache_update", "gc_cache_completude", "gc_cache_load_all",
-    "gc_cache_timestamp", "gc_cache_stream", "gc_cache_view_get", "gc_cache_geometry", "gc_cache_read_mrpd",
+    "gc_cache_timestamp", "gc_cache_stream", "gc_cache_view_get", "gc_cache_geometry", "gc_geometry_compute",
+    "gc_cache_read_mrpd",
     "gc_cache_read_request", "gc_cache_write_request", "gc_cache_read_usage", "gc_cache_write_usage",
     "gc_cache_read_lru", "gc_cache_read_pool", "gc_cache_read_asked", "gc_frame_create", "gc_frame_destroy",
     "gc_frame_reset", "gc_frame_read_pixels", "gc_frame_read_state", "gc_frame_buffers", "gc_render_dvr",
@@ -147,6 +148,7 @@ def lib():
             "gc_cache_completude": [P, C.POINTER(f32)], "gc_cache_load_all": [P, P],
             "gc_cache_timestamp": [P, C.POINTER(u32)], "gc_cache_stream": [P, C.POINTER(vp)],
             "gc_cache_view_get": [P, C.POINTER(CacheView)], "gc_cache_geometry": [P, C.POINTER(Geometry)],
+            "gc_geometry_compute": [C.POINTER(VolumeCfg), C.POINTER(CacheCfg), C.POINTER(Geometry)],
             "gc_cache_read_mrpd": [P, P], "gc_cache_read_request": [P, P], "gc_cache_write_request": [P, P],
             "gc_cache_read_usage": [P, P], "gc_cache_write_usage": [P, P], "gc_cache_read_lru": [P, P],
             "gc_cache_read_pool": [P, P], "gc_cache_read_asked": [P, P, C.POINTER(u32), C.POINTER(u32)],
@@ -199,6 +201,12 @@ def cache_cfg(cache_size, brick_size, max_requests=50, sync=True, empty_threshol
     cc.sync_requests = 1 if sync else 0
     cc.empty_threshold = empty_threshold
     return cc
+
+
+def geometry_compute(vc, cc):
+    g = Geometry()
+    check(lib().gc_geometry_compute(C.byref(vc), C.byref(cc), C.byref(g)))
+    return g
 
 
 def pool_shape(budget, brick_bytes):

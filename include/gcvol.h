@@ -203,6 +203,8 @@ typedef struct gc_geometry {
     uint64_t mrpd_cells;
 } gc_geometry;
 int gc_cache_geometry(gc_cache *c, gc_geometry *out);
+/* The same geometry from the configurations alone (host only, no device). */
+int gc_geometry_compute(const gc_volume_cfg *vc, const gc_cache_cfg *cc, gc_geometry *out);
 
 /* State export/import for parity checks and trace replay.  LRU records are
  * the reference's LruContent (LruContent.hpp:10-15), in LRU order. */

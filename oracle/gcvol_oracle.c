@@ -644,6 +644,32 @@ static inline double sample_box(const or_cache *c, uint32_t ex, uint32_t ey, uin
     return s;
 }
 
+int or_sample(const or_cache *c, uint32_t lod, const float np[3], double *sum8,
+              uint32_t brick[3], uint32_t voxel[3])
+{
+    const or_geom *g = &c->g;
+    const uint32_t inner = c->cfg.brick_size - 2 * c->cfg.covering;
+    float pv[3];
+    uint32_t b[3], v[3];
+    for (int a = 0; a < 3; ++a) {
+        const float pr = np[a] * g->ior[lod][a];
+        pv[a] = pr * g->rdv[lod][a];
+        b[a] = (uint32_t)(pv[a] * (float)g->level_dim[lod][a]);
+        v[a] = (uint32_t)(pv[a] * (float)g->virt_vox[lod][a]);
+    }
+    if (brick) memcpy(brick, b, sizeof(b));
+    if (voxel) memcpy(voxel, v, sizeof(v));
+    const uint64_t mi = (uint64_t)(g->level_coord[lod] + b[0]) +
+                        (uint64_t)g->mrpd_dim[0] * (b[1] + (uint64_t)g->mrpd_dim[1] * b[2]);
+    const uint32_t *e = c->mrpd + 4 * mi;
+    if (e[3] == 0) return 1;
+    if (e[3] == 2) return 2;
+    const uint32_t cov = c->cfg.covering;
+    if (sum8) *sum8 = sample_box(c, cov + e[0] + v[0] % inner, cov + e[1] + v[1] % inner,
+                                 cov + e[2] + v[2] % inner, NULL);
+    return 0;
+}
+
 static inline uint32_t pack_abgr(double r, double g, double b, double a)
 {
     /* rgbaFloatToInt (RayCasters.cu:528-536) */

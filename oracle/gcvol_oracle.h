@@ -154,6 +154,14 @@ typedef struct {
     uint32_t tf_size;
 } or_view;
 
+/* One page-table translation and box-filtered fetch at a normalised position
+ * (K_CacheManager::get_normalized, K_CacheManager.hpp:267-361, without the
+ * usage / request side effects).  Returns 0 Mapped, 1 Unmapped, 2 Empty;
+ * sum8 receives the sum of the 8 texels (Mapped only); brick/voxel (may be
+ * NULL) receive the FP32-derived brick index and in-level voxel index. */
+int or_sample(const or_cache *c, uint32_t level, const float pos[3], double *sum8,
+              uint32_t brick[3], uint32_t voxel[3]);
+
 /* Host-side LOD tables (VolumeRayCaster.cu:136-161) for a [-1,1]^3 box. */
 void or_view_tables(or_view *v, const or_cfg *cfg);
 

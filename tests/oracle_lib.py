@@ -101,6 +101,9 @@ def lib():
         L.or_view_tables.argtypes = [C.POINTER(OrView), C.POINTER(OrCfg)]
         L.or_tf_samples.argtypes = [P, u32, u32, P]
         L.or_render_dvr.argtypes = [P, C.POINTER(OrView), P, P, P, C.c_int, C.c_int, P]
+        L.or_render_dvr_rows.argtypes = [P, C.POINTER(OrView), P, P, P, C.c_int, C.c_int, u32, u32, P]
+        L.or_sample.restype = C.c_int
+        L.or_sample.argtypes = [P, u32, P, C.POINTER(C.c_double), P, P]
         _lib = L
     return _lib
 
@@ -252,6 +255,13 @@ class Cache:
 
     def load_all(self):
         lib().or_cache_load_all(self.h)
+
+    def sample(self, level, pos):
+        s = C.c_double(0)
+        b = np.zeros(3, np.uint32)
+        v = np.zeros(3, np.uint32)
+        st = lib().or_sample(self.h, level, ptr(np.asarray(pos, np.float32)), C.byref(s), ptr(b), ptr(v))
+        return st, s.value, b, v
 
     def render(self, view, tbuf, cstate, pixels, precision=64, nthreads=1):
         st = np.zeros(4, np.uint64)
