@@ -4,25 +4,31 @@
 // K_CacheDevice.hpp:191-201,316-334, inlined).
 //
 // Layout and mapping
-//  * One lane per pixel; a workgroup covers a TWxTH ray tile (8x8, 16x8, 32x4
-//    or 64x1), so neighbouring rays of a wave walk the same bricks.
+//  * One lane per pixel.  A wave owns a 64-ray tile (8x8, 16x4, 32x2 or 64x1
+//    pixels) so its rays walk the same bricks; a workgroup holds 4 waves on 4
+//    consecutive tiles and shares the per-level tables and the transfer
+//    function in LDS.
 //  * Workgroup ids are remapped so that consecutive tiles land on the same
 //    XCD (workgroups are dealt round-robin over the 8 XCDs): each XCD walks a
 //    contiguous band of the image and keeps its bricks in its own L2.
-//  * Per-level tables and the transfer function live in LDS.
-//  * The page entry of the current brick is cached per lane (the reference's
-//    _previousCoords cache); the usage stamp is written once per brick visit.
 //  * The pool is brick-major (slot * bs^3), so the 2x2x2 footprint of a
-//    sample is at most four rows inside one brick.
+//    sample is four row pairs inside one brick, each fetched with one 8-byte
+//    load.
 //
-// Arithmetic
-//  * Ray set-up, positions, LOD choice and every page-table / voxel index are
-//    FP32 in the reference's operation order (no contraction, IEEE divide and
-//    square root), so the sample trace is reproducible on the host.
-//  * The sample value is the exact mean of the 8 texels around the voxel
-//    (the linear-filter texture fetch at an integer texel coordinate).
-//  * Compositing is FP32 (precision 32, opacity correction with the hardware
-//    exp2/log2) or FP64 (precision 64).
+// Exactness of the sample trace.  Ray set-up, positions, LOD choice and every
+// page-table / voxel index are FP32 in the reference's operation order (no
+// contraction, IEEE divide and square root).  Three rewrites keep that trace
+// bit-for-bit:
+//  * LOD: the reference evaluates pixelDist = endPixelDist * (t / tfar) at
+//    every sample.  That predicate is monotone in t, so the level can only
+//    grow along a ray; the first float t at which the current level stops
+//    qualifying is found once per level by bisection over float bit patterns
+//    and each sample compares t against it.
+//  * Voxel reuse: consecutive samples in the same voxel of the same brick and
+//    level read the same eight texels; their classified colour is reused.
+//  * The page entry and the usage stamp are refreshed once per brick visit.
+// Compositing is FP32 (precision 32: opacity correction with the hardware
+// exp2/log2, FMA blending) or FP64 (precision 64).
 #include "gc_internal.hpp"
 
 namespace gc {
@@ -30,11 +36,14 @@ namespace gc {
 namespace {
 
 constexpr int TF_LDS = 256;
+constexpr int WAVES = 4;
+constexpr float FINF = __builtin_huge_valf();
 
 struct LevelLds {
     float ior[3], rdv[3], ldf[3], vvf[3], inv[3];
     uint32_t lcoord;
     float step, kexp;
+    uint32_t unit_rdv;   // rdv == (1,1,1): the virtual position is the real one
 };
 
 struct F3 {
@@ -65,25 +74,77 @@ __device__ inline float spacing(F3 d, const float *inv)
     const F3 a = d * f3(inv[0], inv[1], inv[2]);
     return __fsqrt_rn(dot(a, a));
 }
+__device__ inline float clamp099(float v) { return fmaxf(0.f, fminf(v, 0.99f)); }
 
-// Sum of the 8 texels {e-1, e}^3 for a voxel strictly inside its brick
-// (covering >= 1): q points at texel e.
-template <int BYTES>
-__device__ inline uint32_t box_sum_inner(const uint8_t *q, uint32_t bs)
+// pixelDist at distance t: endPixelDist * (t / tfar) (RayCasters.cu:124-126)
+__device__ inline float pixel_dist(float ep, float t, float tfar) { return ep * (t / tfar); }
+
+// Smallest float t in (lo, hi] with pixel_dist(ep, t) > s, given
+// pixel_dist(ep, lo) <= s < pixel_dist(ep, hi).  Bisection over the bit
+// patterns of non-negative floats (monotone in value), after a bracket guess.
+__device__ float first_above(float ep, float s, float tfar, float lo, float hi)
 {
-    const uint32_t r = bs, s = bs * bs;
-    if constexpr (BYTES == 1) {
-        return (uint32_t)q[0] + q[-1] + q[-(int)r] + q[-(int)r - 1] + q[-(int)s] + q[-(int)s - 1] +
-               q[-(int)s - (int)r] + q[-(int)s - (int)r - 1];
-    } else {
-        const uint16_t *p = reinterpret_cast<const uint16_t *>(q);
-        return (uint32_t)p[0] + p[-1] + p[-(int)r] + p[-(int)r - 1] + p[-(int)s] + p[-(int)s - 1] +
-               p[-(int)s - (int)r] + p[-(int)s - (int)r - 1];
+    // lo >= 0 (a -0.0 lower bound is read as +0.0)
+    uint32_t lb = __float_as_uint(lo) & 0x7fffffffu, hb = __float_as_uint(hi);
+    const float est = (s / ep) * tfar;
+    if (est > lo && est < hi) {
+        const uint32_t eb = __float_as_uint(est);
+        const uint32_t w = 64;
+        const uint32_t a = eb - lb > w ? eb - w : lb, b = hb - eb > w ? eb + w : hb;
+        const bool a_ok = a == lb || pixel_dist(ep, __uint_as_float(a), tfar) <= s;
+        const bool b_ok = b == hb || pixel_dist(ep, __uint_as_float(b), tfar) > s;
+        if (a_ok && b_ok) { lb = a; hb = b; }
     }
+    while (hb - lb > 1) {
+        const uint32_t mb = lb + ((hb - lb) >> 1);
+        if (pixel_dist(ep, __uint_as_float(mb), tfar) > s) hb = mb;
+        else lb = mb;
+    }
+    return __uint_as_float(hb);
 }
 
-// General texel read in the reference's 3-D pool coordinates (used when the
-// covering is 0 and the footprint may leave the brick): x, y clamp, z wraps.
+// First t >= t0 at which level `lod` stops being chosen: both pixel
+// distances exceed the next level's spacings (compute_LOD,
+// RayCasters.cu:551-567).  +inf if that never happens before tfar.
+__device__ float level_switch(float t0, float tfar, float epx, float epy, float sl, float su)
+{
+    const bool x_far = pixel_dist(epx, tfar, tfar) > sl;
+    const bool y_far = pixel_dist(epy, tfar, tfar) > su;
+    if (!x_far || !y_far) return FINF;
+    const float tx = pixel_dist(epx, t0, tfar) > sl ? t0 : first_above(epx, sl, tfar, t0, tfar);
+    const float ty = pixel_dist(epy, t0, tfar) > su ? t0 : first_above(epy, su, tfar, t0, tfar);
+    return fmaxf(tx, ty);
+}
+
+// Sum of the 8 texels {e-1, e}^3 of a voxel strictly inside its brick
+// (covering >= 1).  Each (x-1, x) pair is taken from one 8-byte load at the
+// 4-byte-aligned address below it (the pool is padded for the overrun).
+template <int BYTES>
+__device__ inline uint32_t box_sum_pairs(const uint8_t *brick, uint32_t bs, uint32_t ex, uint32_t ey, uint32_t ez)
+{
+    const uint64_t row = (uint64_t)bs * BYTES;
+    const uint8_t *r0 = brick + ((uint64_t)(ez - 1) * bs + (ey - 1)) * row + (uint64_t)(ex - 1) * BYTES;
+    const uint8_t *rows[4] = {r0, r0 + row, r0 + row * bs, r0 + row * bs + row};
+    uint2 v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uintptr_t a = reinterpret_cast<uintptr_t>(rows[i]);
+        __builtin_memcpy(&v[i], __builtin_assume_aligned(reinterpret_cast<const void *>(a & ~(uintptr_t)3), 4), 8);
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t sh = (uint32_t)(reinterpret_cast<uintptr_t>(rows[i]) & 3) * 8;
+        const uint64_t q = ((uint64_t)v[i].y << 32) | v[i].x;
+        const uint32_t w = (uint32_t)(q >> sh);
+        if constexpr (BYTES == 1) sum += (w & 0xffu) + ((w >> 8) & 0xffu);
+        else sum += (w & 0xffffu) + (w >> 16);
+    }
+    return sum;
+}
+
+// General texel read in the reference's 3-D pool coordinates (covering 0, or
+// float voxels): x, y clamp, z wraps (Texture3dDevice.hpp:104-122).
 template <int BYTES>
 __device__ inline float texel(const uint8_t *pool, const gc_geometry *g, int px, int py, int pz)
 {
@@ -119,7 +180,7 @@ __device__ inline uint32_t pack(double r, double g, double b, double a)
     return (A << 24) | (B << 16) | (G << 8) | R;
 }
 
-__device__ inline uint32_t remap_tile(uint32_t bid, uint32_t nblocks)
+__device__ inline uint32_t remap_block(uint32_t bid, uint32_t nblocks)
 {
     // Workgroup bid runs on XCD bid % 8; give each XCD a contiguous range.
     const uint32_t xcd = bid & 7u, k = bid >> 3;
@@ -134,9 +195,40 @@ __device__ inline uint64_t wave_sum(uint64_t v)
     return v;
 }
 
-template <int BYTES, typename ACC, int TW, int TH>
-__global__ __launch_bounds__(TW *TH) void k_dvr(const DvrParams p)
+// Classified, opacity-corrected, premultiplied colour of a sample value
+// (tex1D linear + clamp, RayCasters.cu:163-175).  skip: alpha <= 0.001.
+template <typename ACC>
+__device__ inline Acc4<ACC> classify(ACC s, const float4 *TF, uint32_t N, float kexp, bool &skip)
 {
+    const ACC xf = s * (ACC)N - (ACC)0.5;
+    const ACC fl = floor(xf);
+    const ACC a = xf - fl;
+    int i0 = (int)fl, i1 = i0 + 1;
+    i0 = i0 < 0 ? 0 : (i0 > (int)N - 1 ? (int)N - 1 : i0);
+    i1 = i1 < 0 ? 0 : (i1 > (int)N - 1 ? (int)N - 1 : i1);
+    const float4 T0 = TF[i0], T1 = TF[i1];
+    Acc4<ACC> col;
+    if constexpr (sizeof(ACC) == 8) {
+        const double b = 1.0 - a;
+        col = Acc4<ACC>{b * (double)T0.x + a * (double)T1.x, b * (double)T0.y + a * (double)T1.y,
+                        b * (double)T0.z + a * (double)T1.z, b * (double)T0.w + a * (double)T1.w};
+        col.w = 1.0 - pow(1.0 - col.w, (double)kexp);
+    } else {
+        col = Acc4<ACC>{fmaf(a, T1.x - T0.x, T0.x), fmaf(a, T1.y - T0.y, T0.y), fmaf(a, T1.z - T0.z, T0.z),
+                        fmaf(a, T1.w - T0.w, T0.w)};
+        col.w = 1.f - __builtin_amdgcn_exp2f(kexp * __builtin_amdgcn_logf(1.f - col.w));
+    }
+    skip = !(col.w > (ACC)0.001f);
+    col.x *= col.w;
+    col.y *= col.w;
+    col.z *= col.w;
+    return col;
+}
+
+template <int BYTES, typename ACC, int TW, int TH>
+__global__ __launch_bounds__(64 * WAVES) void k_dvr(const DvrParams p)
+{
+    static_assert(TW * TH == 64, "one wave per ray tile");
     __shared__ LevelLds lv[GC_MAX_LEVELS];
     __shared__ float4 tfl[TF_LDS];
     const int tid = threadIdx.x;
@@ -153,26 +245,31 @@ __global__ __launch_bounds__(TW *TH) void k_dvr(const DvrParams p)
         L.lcoord = g->level_coord[tid];
         L.step = p.lod_step[tid];
         L.kexp = p.kexp[tid];
+        L.unit_rdv = L.rdv[0] == 1.f && L.rdv[1] == 1.f && L.rdv[2] == 1.f;
     }
-    for (uint32_t i = tid; i < p.tf_size && i < (uint32_t)TF_LDS; i += TW * TH) tfl[i] = p.tf[i];
+    for (uint32_t i = tid; i < p.tf_size && i < (uint32_t)TF_LDS; i += 64 * WAVES) tfl[i] = p.tf[i];
     __syncthreads();
     const float4 *TF = p.tf_size <= (uint32_t)TF_LDS ? tfl : p.tf;
 
     const uint32_t ntiles = p.tiles_x * p.tiles_y;
-    const uint32_t tile = remap_tile(blockIdx.x, ntiles);
-    const uint32_t x = (tile % p.tiles_x) * TW + (tid % TW);
-    const uint32_t y = (tile / p.tiles_x) * TH + (tid / TW);
-    const bool active = x < p.width && y < p.height;
+    const uint32_t nblocks = (ntiles + WAVES - 1) / WAVES;
+    const uint32_t tile = remap_block(blockIdx.x, nblocks) * WAVES + (tid >> 6);
+    const uint32_t lane = tid & 63;
+    const uint32_t x = (tile % p.tiles_x) * TW + (lane % TW);
+    const uint32_t y = (tile / p.tiles_x) * TH + (lane / TW);
+    const bool active = tile < ntiles && x < p.width && y < p.height;
 
     uint64_t n_samples = 0, n_visits = 0, n_unmapped = 0;
     const uint32_t W = p.width, H = p.height;
     const uint32_t bs = g->brick_size, cov = g->covering, inner = g->inner;
     const uint32_t cx_ = g->cache_size[0], cy_ = g->cache_size[1];
     const uint32_t MX = g->mrpd_dim[0], MY = g->mrpd_dim[1];
-    const uint64_t bvox = (uint64_t)bs * bs * bs;
+    const uint64_t bbytes = (uint64_t)bs * bs * bs * BYTES;
     const uint8_t *pool = static_cast<const uint8_t *>(p.pool);
     const float *M = p.m;
     const uint32_t nl = p.nb_lods;
+    const bool fast_fetch = BYTES != 4 && cov >= 1;
+    const ACC inv_norm = BYTES == 1 ? (ACC)(1.0 / 2040.0) : (ACC)(1.0 / 524280.0);
 
     if (active) {
         // RayCasters.cu:57-84
@@ -220,32 +317,45 @@ __global__ __launch_bounds__(TW *TH) void k_dvr(const DvrParams p)
             F3 pos = origin + dir * tnear;
             const float epx = fabsf(dot(dir - dxn, dleft) * tfar);
             const float epy = fabsf(dot(dir - dyn, dup) * tfar);
-            // LOD state (compute_LOD, RayCasters.cu:551-567).  pixelDist grows
-            // monotonically with t, so the chosen level never decreases along
-            // the ray and the scan can resume from the current level.
+            const ACC astop = (ACC)p.alpha_stop;
+            const uint32_t N = p.tf_size;
+
+            // level state: lod, the distance where it stops qualifying, and
+            // the level's constants
             uint32_t lod = 0;
-            float nsl = 0.f, nsu = 0.f;
-            if (nl > 1) { nsl = spacing(dleft, lv[1].inv); nsu = spacing(dup, lv[1].inv); }
+            float tswitch = -1.f;   // forces a settle before the first sample
+            float step = 0.f, kexp = 0.f;
+            F3 dstep = f3(0.f, 0.f, 0.f);
+            // brick / voxel caches
             uint32_t pcx = 0xffffffffu, pcy = 0xffffffffu, pcz = 0xffffffffu;
             uint4 e = make_uint4(0, 0, 0, 0);
             const uint8_t *bbase = pool;
-            const ACC astop = (ACC)p.alpha_stop;
-            const uint32_t N = p.tf_size;
+            uint32_t pvx = 0xffffffffu, pvy = 0, pvz = 0;
+            Acc4<ACC> cc{0, 0, 0, 0};
+            bool cskip = true;
+
             for (;;) {
                 if (!(fc.w < astop && t < tfar)) break;
-                const float al = t / tfar;
-                const float pdx = epx * al, pdy = epy * al;
-                F3 tp = pos * 0.495f + f3(0.495f, 0.495f, 0.495f);
-                tp.z = 1.0f - tp.z;
-                const F3 np = f3(fmaxf(0.f, fminf(tp.x, 0.99f)), fmaxf(0.f, fminf(tp.y, 0.99f)),
-                                 fmaxf(0.f, fminf(tp.z, 0.99f)));
-                while (lod + 1 < nl && !(nsl >= pdx * 1.0f || nsu >= pdy * 1.0f)) {
-                    ++lod;
-                    if (lod + 1 < nl) { nsl = spacing(dleft, lv[lod + 1].inv); nsu = spacing(dup, lv[lod + 1].inv); }
+                if (t >= tswitch) {
+                    // settle the level at t exactly as compute_LOD does, then
+                    // find where it next changes
+                    const float pdx = pixel_dist(epx, t, tfar), pdy = pixel_dist(epy, t, tfar);
+                    while (lod + 1 < nl &&
+                           !(spacing(dleft, lv[lod + 1].inv) >= pdx * 1.0f || spacing(dup, lv[lod + 1].inv) >= pdy * 1.0f))
+                        ++lod;
+                    tswitch = lod + 1 < nl ? level_switch(t, tfar, epx, epy, spacing(dleft, lv[lod + 1].inv),
+                                                          spacing(dup, lv[lod + 1].inv))
+                                           : FINF;
+                    step = lv[lod].step;
+                    kexp = lv[lod].kexp;
+                    dstep = dir * step;
                 }
                 const LevelLds &L = lv[lod];
+                F3 tp = pos * 0.495f + f3(0.495f, 0.495f, 0.495f);
+                tp.z = 1.0f - tp.z;
+                const F3 np = f3(clamp099(tp.x), clamp099(tp.y), clamp099(tp.z));
                 const F3 pr = np * f3(L.ior[0], L.ior[1], L.ior[2]);
-                const F3 pv = pr * f3(L.rdv[0], L.rdv[1], L.rdv[2]);
+                const F3 pv = L.unit_rdv ? pr : pr * f3(L.rdv[0], L.rdv[1], L.rdv[2]);
                 const uint32_t bx = (uint32_t)(pv.x * L.ldf[0]);
                 const uint32_t by = (uint32_t)(pv.y * L.ldf[1]);
                 const uint32_t bz = (uint32_t)(pv.z * L.ldf[2]);
@@ -254,10 +364,11 @@ __global__ __launch_bounds__(TW *TH) void k_dvr(const DvrParams p)
                     pcx = mcx; pcy = by; pcz = bz;
                     e = p.mrpd[mcx + (uint64_t)MX * (by + (uint64_t)MY * bz)];
                     ++n_visits;
+                    pvx = 0xffffffffu;
                     if (e.w == 1) {
                         const uint32_t slot = e.x / bs + cx_ * (e.y / bs + cy_ * (e.z / bs));
                         p.usage[slot] = p.ts;
-                        bbase = pool + (uint64_t)slot * bvox * BYTES;
+                        bbase = pool + (uint64_t)slot * bbytes;
                     }
                 }
                 if (e.w == 0) {
@@ -282,69 +393,60 @@ __global__ __launch_bounds__(TW *TH) void k_dvr(const DvrParams p)
                     pos = pos + dir * p.tstep;
                     continue;
                 }
-                // voxel within the brick: (v mod (bs-2c)) using the brick index
-                // already computed (exact; falls back when FP32 rounding puts
-                // the voxel index in a neighbouring brick).
                 const uint32_t vx = (uint32_t)(pv.x * L.vvf[0]);
                 const uint32_t vy = (uint32_t)(pv.y * L.vvf[1]);
                 const uint32_t vz = (uint32_t)(pv.z * L.vvf[2]);
-                uint32_t lx = vx - bx * inner, ly = vy - by * inner, lz = vz - bz * inner;
-                if (lx >= inner) lx = vx % inner;
-                if (ly >= inner) ly = vy % inner;
-                if (lz >= inner) lz = vz % inner;
-                ++n_samples;
-                ACC s;
-                if (BYTES != 4 && cov >= 1) {
-                    const uint8_t *q = bbase + ((uint64_t)(cov + lx) + (uint64_t)bs * ((cov + ly) + (uint64_t)bs * (cov + lz))) * BYTES;
-                    const uint32_t sum = box_sum_inner<BYTES == 4 ? 1 : BYTES>(q, bs);
-                    if constexpr (sizeof(ACC) == 8) s = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
-                    else s = (float)sum / (BYTES == 1 ? 2040.0f : 524280.0f);
-                } else {
-                    // general footprint in pool coordinates
-                    const int ex = (int)(cov + e.x + lx), ey = (int)(cov + e.y + ly), ez = (int)(cov + e.z + lz);
-                    float fsum = 0.f;
-                    double dsum = 0.0;
-                    for (int k = 0; k < 2; ++k)
-                        for (int j = 0; j < 2; ++j)
-                            for (int i = 0; i < 2; ++i) {
-                                const float tv = texel<BYTES>(pool, g, ex - 1 + i, ey - 1 + j, ez - 1 + k);
-                                fsum += tv;
-                                dsum += tv;
-                            }
-                    if constexpr (BYTES == 4) {
-                        if constexpr (sizeof(ACC) == 8) s = dsum / 8.0;
-                        else s = fsum / 8.0f;
+                if (vx != pvx || vy != pvy || vz != pvz) {
+                    pvx = vx; pvy = vy; pvz = vz;
+                    // voxel within the brick: v mod (bs-2c), via the brick
+                    // index already known (falls back when FP32 rounding put
+                    // the voxel index in a neighbouring brick)
+                    uint32_t lx = vx - bx * inner, ly = vy - by * inner, lz = vz - bz * inner;
+                    if (lx >= inner) lx = vx % inner;
+                    if (ly >= inner) ly = vy % inner;
+                    if (lz >= inner) lz = vz % inner;
+                    ACC s;
+                    if (fast_fetch) {
+                        const uint32_t sum = box_sum_pairs<BYTES == 4 ? 1 : BYTES>(bbase, bs, cov + lx, cov + ly, cov + lz);
+                        if constexpr (sizeof(ACC) == 8) s = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
+                        else s = (float)sum * inv_norm;
                     } else {
-                        if constexpr (sizeof(ACC) == 8) s = dsum / (BYTES == 1 ? 2040.0 : 524280.0);
-                        else s = (float)dsum / (BYTES == 1 ? 2040.0f : 524280.0f);
+                        const int ex = (int)(cov + e.x + lx), ey = (int)(cov + e.y + ly), ez = (int)(cov + e.z + lz);
+                        float fsum = 0.f;
+                        double dsum = 0.0;
+                        for (int k = 0; k < 2; ++k)
+                            for (int j = 0; j < 2; ++j)
+                                for (int i = 0; i < 2; ++i) {
+                                    const float tv = texel<BYTES>(pool, g, ex - 1 + i, ey - 1 + j, ez - 1 + k);
+                                    fsum += tv;
+                                    dsum += tv;
+                                }
+                        if constexpr (BYTES == 4) {
+                            if constexpr (sizeof(ACC) == 8) s = dsum / 8.0;
+                            else s = fsum / 8.0f;
+                        } else {
+                            if constexpr (sizeof(ACC) == 8) s = dsum / (BYTES == 1 ? 2040.0 : 524280.0);
+                            else s = (float)dsum * inv_norm;
+                        }
+                    }
+                    cc = classify<ACC>(s, TF, N, kexp, cskip);
+                }
+                ++n_samples;
+                if (!cskip) {
+                    // front-to-back blending (RayCasters.cu:176-178)
+                    const ACC wgt = (ACC)1 - fc.w;
+                    if constexpr (sizeof(ACC) == 8) {
+                        const ACC tx = cc.x * wgt, ty = cc.y * wgt, tz = cc.z * wgt, tw = cc.w * wgt;
+                        fc.x += tx; fc.y += ty; fc.z += tz; fc.w += tw;
+                    } else {
+                        fc.x = fmaf(cc.x, wgt, fc.x);
+                        fc.y = fmaf(cc.y, wgt, fc.y);
+                        fc.z = fmaf(cc.z, wgt, fc.z);
+                        fc.w = fmaf(cc.w, wgt, fc.w);
                     }
                 }
-                // tex1D<float4>, linear filter, normalised, clamp
-                const ACC xf = s * (ACC)N - (ACC)0.5;
-                const ACC fl = floor(xf);
-                const ACC a = xf - fl;
-                int i0 = (int)fl, i1 = i0 + 1;
-                i0 = i0 < 0 ? 0 : (i0 > (int)N - 1 ? (int)N - 1 : i0);
-                i1 = i1 < 0 ? 0 : (i1 > (int)N - 1 ? (int)N - 1 : i1);
-                const float4 T0 = TF[i0], T1 = TF[i1];
-                const ACC b = (ACC)1 - a;
-                Acc4<ACC> col{b * (ACC)T0.x + a * (ACC)T1.x, b * (ACC)T0.y + a * (ACC)T1.y,
-                              b * (ACC)T0.z + a * (ACC)T1.z, b * (ACC)T0.w + a * (ACC)T1.w};
-                // opacity correction 1 - (1-a)^(LODStep/tstep) (RayCasters.cu:165)
-                if constexpr (sizeof(ACC) == 8) {
-                    col.w = 1.0 - pow(1.0 - col.w, (double)L.kexp);
-                } else {
-                    const float base = 1.f - col.w;
-                    col.w = 1.f - __builtin_amdgcn_exp2f(L.kexp * __builtin_amdgcn_logf(base));
-                }
-                if (col.w > (ACC)0.001f) {
-                    col.x *= col.w; col.y *= col.w; col.z *= col.w;
-                    const ACC wgt = (ACC)1 - fc.w;
-                    const ACC tx = col.x * wgt, ty = col.y * wgt, tz = col.z * wgt, tw = col.w * wgt;
-                    fc.x += tx; fc.y += ty; fc.z += tz; fc.w += tw;
-                }
-                t += L.step;
-                pos = pos + dir * L.step;
+                t += step;
+                pos = pos + dstep;
             }
             if (!unmapped) {
                 p.tbuf[pi] = t;
@@ -372,23 +474,19 @@ __global__ __launch_bounds__(TW *TH) void k_dvr(const DvrParams p)
 template <int BYTES, typename ACC>
 hipError_t launch_tile(DvrParams p, uint32_t tile, hipStream_t stream)
 {
-    int tw, th;
-    switch (tile) {
-    case 0: tw = 8; th = 8; break;
-    case 1: tw = 16; th = 8; break;
-    case 2: tw = 32; th = 4; break;
-    case 3: tw = 64; th = 1; break;
-    default: return hipErrorInvalidValue;
-    }
+    static const int TW[4] = {8, 16, 32, 64};
+    if (tile > 3) return hipErrorInvalidValue;
+    const int tw = TW[tile], th = 64 / tw;
     p.tiles_x = (p.width + tw - 1) / tw;
     p.tiles_y = (p.height + th - 1) / th;
-    const uint32_t nb = p.tiles_x * p.tiles_y;
+    const uint32_t ntiles = p.tiles_x * p.tiles_y;
+    const uint32_t nb = (ntiles + WAVES - 1) / WAVES;
     if (nb == 0) return hipSuccess;
     switch (tile) {
-    case 0: k_dvr<BYTES, ACC, 8, 8><<<nb, 64, 0, stream>>>(p); break;
-    case 1: k_dvr<BYTES, ACC, 16, 8><<<nb, 128, 0, stream>>>(p); break;
-    case 2: k_dvr<BYTES, ACC, 32, 4><<<nb, 128, 0, stream>>>(p); break;
-    case 3: k_dvr<BYTES, ACC, 64, 1><<<nb, 64, 0, stream>>>(p); break;
+    case 0: k_dvr<BYTES, ACC, 8, 8><<<nb, 64 * WAVES, 0, stream>>>(p); break;
+    case 1: k_dvr<BYTES, ACC, 16, 4><<<nb, 64 * WAVES, 0, stream>>>(p); break;
+    case 2: k_dvr<BYTES, ACC, 32, 2><<<nb, 64 * WAVES, 0, stream>>>(p); break;
+    case 3: k_dvr<BYTES, ACC, 64, 1><<<nb, 64 * WAVES, 0, stream>>>(p); break;
     }
     return hipGetLastError();
 }

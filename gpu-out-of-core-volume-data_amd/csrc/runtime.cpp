@@ -378,14 +378,16 @@ int Cache::create()
     GC_TRY(dalloc(&order, (size_t)g.nslots * sizeof(uint32_t)));
     GC_TRY(dalloc(&order_tmp, (size_t)g.nslots * sizeof(uint32_t)));
     GC_TRY(dalloc(&meta, (size_t)g.nslots * sizeof(SlotMeta)));
-    GC_TRY(dalloc(&pool, (size_t)g.nslots * bvox * vc.encoded_bytes));
+    // +64 bytes: the ray caster reads texel pairs with 8-byte loads that may
+    // run up to 5 bytes past the last row of the last slot
+    GC_TRY(dalloc(&pool, (size_t)g.nslots * bvox * vc.encoded_bytes + 64));
     GC_TRY(dalloc(&d_geom, sizeof(gc_geometry)));
     // Surfaces and thrust vectors of the reference start zeroed
     // (Surface3dDevice.hpp:97-109, DynamicArray3dDevice).
     GC_HIP(hipMemsetAsync(mrpd, 0, g.mrpd_cells * sizeof(uint4), stream));
     GC_HIP(hipMemsetAsync(request, 0, g.mrpd_cells * sizeof(uint32_t), stream));
     GC_HIP(hipMemsetAsync(usage, 0, (size_t)g.nslots * sizeof(uint32_t), stream));
-    GC_HIP(hipMemsetAsync(pool, 0, (size_t)g.nslots * bvox * vc.encoded_bytes, stream));
+    GC_HIP(hipMemsetAsync(pool, 0, (size_t)g.nslots * bvox * vc.encoded_bytes + 64, stream));
     {
         // LRU initialisation, x fastest (CacheDevice.hpp:151-165)
         std::vector<uint32_t> ord(g.nslots);
