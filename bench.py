@@ -224,7 +224,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": None,
-            "kernel": "k_dvr<2,%s,%s>" % ("double" if args.precision == 64 else "float", ["8,8", "16,8", "32,4", "64,1"][args.tile]),
+            "kernel": "k_dvr<2,%s,%s>" % ("double" if args.precision == 64 else "float", ["8,8", "16,4", "32,2", "64,1"][args.tile]),
             "avg_launch_ms": kms / launches if launches else 0.0,
             "algorithmic_bytes_per_launch": B / launches if launches else 0.0,
         },

@@ -129,31 +129,63 @@ hipError_t launch_brick_cut(const void *level, const uint32_t dims[3], uint32_t 
                             void *out, hipStream_t stream);
 
 // ------------------------------------------------------------------ DVR
-struct DvrParams {
+// A launch renders one or more views over one cache.  Work is split into
+// packets of 64 rays: either the TWxTH pixel tiles of every view, or 64
+// consecutive entries of an explicit ray list (view << 26 | pixel).
+constexpr uint32_t DVR_MAX_VIEWS = 64;
+constexpr uint32_t DVR_PIXEL_BITS = 26;
+
+struct DvrView {
+    uint32_t width, height, tiles_x, packet_off;   // packet_off: first tile packet
+    float m[12];                                   // d_invModelViewMatrix rows
+    float bmin[3], bmax[3];
+    float alpha_stop;
+    uint32_t tf_size;
+    const float4 *tf;
+    uint32_t *pixels;
+    float *tbuf;
+    void *cstate;                                  // float4 (fp32) or double4 (fp64)
+};
+
+struct DvrLaunch {
     const uint4 *mrpd;
     uint32_t *request;
     uint32_t *usage;
     const void *pool;
     const gc_geometry *geom;
-    const float4 *tf;
-    uint32_t tf_size;
     uint32_t ts;
-    uint32_t *pixels;
-    float *tbuf;
-    void *cstate;
-    gc_render_stats *stats;
-    uint32_t width, height;
-    float m[12];
-    float bmin[3], bmax[3];
-    float tstep, alpha_stop;
+    float tstep;
     uint32_t nb_lods;
-    uint32_t tiles_x, tiles_y;
     float inv_levels[GC_MAX_LEVELS][3];
     float lod_step[GC_MAX_LEVELS];
     float kexp[GC_MAX_LEVELS];
+    DvrView view0;              // the view when `views` is null
+    const DvrView *views;       // device table of n_views views, or null
+    uint32_t n_views;
+    uint32_t n_packets;         // tile packets (ignored with a ray list)
+    const uint32_t *list;       // explicit ray list, or null for all tiles
+    const uint32_t *list_count; // device count of `list`
+    uint32_t *queue;            // 8 per-XCD packet cursors, zero at launch
+    uint32_t *out_list;         // rays stopped at an Unmapped brick (or null)
+    uint32_t *out_count;
+    gc_render_stats *stats;
+    uint32_t refill_min;        // refill idle lanes once this many are idle
 };
 
-hipError_t launch_dvr(const DvrParams &p, uint32_t bytes, uint32_t precision, uint32_t tile,
-                      hipStream_t stream);
+// Tile shape of a ray packet: 0 8x8, 1 16x4, 2 32x2, 3 64x1.
+inline void tile_shape(uint32_t tile, uint32_t &tw, uint32_t &th)
+{
+    static const uint32_t TW[4] = {8, 16, 32, 64};
+    tw = TW[tile & 3];
+    th = 64 / tw;
+}
+
+// Dynamic LDS of a DVR launch (per-lane level-switch table and view table)
+// and the persistent grid for it (workgroups of 4 waves).
+uint32_t dvr_lds_bytes(uint32_t nb_lods, uint32_t n_views);
+uint32_t dvr_grid(uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t n_packets, uint32_t lds_bytes);
+hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t tile,
+                      uint32_t grid, hipStream_t stream);
+hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream);
 
 } // namespace gc

@@ -3,33 +3,49 @@
 // K_CacheManager.hpp:267-361, and K_CacheDevice::get_normalized,
 // K_CacheDevice.hpp:191-201,316-334, inlined).
 //
-// Layout and mapping
-//  * One lane per pixel.  A wave owns a 64-ray tile (8x8, 16x4, 32x2 or 64x1
-//    pixels) so its rays walk the same bricks; a workgroup holds 4 waves on 4
-//    consecutive tiles and shares the per-level tables and the transfer
-//    function in LDS.
-//  * Workgroup ids are remapped so that consecutive tiles land on the same
-//    XCD (workgroups are dealt round-robin over the 8 XCDs): each XCD walks a
-//    contiguous band of the image and keeps its bricks in its own L2.
-//  * The pool is brick-major (slot * bs^3), so the 2x2x2 footprint of a
-//    sample is four row pairs inside one brick, each fetched with one 8-byte
-//    load.
+// Work distribution
+//  * Rays come in packets of 64: a TWxTH pixel tile (8x8, 16x4, 32x2 or 64x1)
+//    of one view, or 64 entries of an explicit ray list (the rays a previous
+//    launch of the same frame stopped at an Unmapped brick).  One launch can
+//    cover many views (batched rendering).
+//  * The kernel is persistent: waves pull packets from 8 per-XCD queues
+//    (workgroup b runs on XCD b % 8 and starts on that XCD's contiguous share
+//    of the packets, so neighbouring tiles share an L2), stealing from the
+//    other queues when their own is empty.
+//  * A lane whose ray ends is refilled with the next ray of the wave's packet
+//    stream once enough lanes are idle, so long and short rays do not hold a
+//    wave hostage.
+//
+// Per-sample cost.  The march loop is issue-bound, so everything that does
+// not change from one sample to the next is hoisted out of it:
+//  * the parameters of the ray's current level live in registers and are
+//    reloaded from LDS only when the level changes;
+//  * the distances at which a ray changes level are computed once, at ray
+//    set-up, into a per-lane LDS table (no search inside the march);
+//  * the page entry is read with one 16-byte load per brick visit, which also
+//    yields the brick's base address in the brick-major pool;
+//  * the 2x2x2 footprint of a sample is four texel pairs (x-1, x) in one
+//    brick: four 4-byte (u16) or 2-byte (u8) loads at 2- or 1-byte alignment,
+//    summed with v_sad_u16 / v_sad_u8;
+//  * consecutive samples of a ray in the same voxel (steps are half a voxel of
+//    the current level) reuse the box sum: those lanes issue no loads, which
+//    is what bounds close-up views, where the memory pipeline saturates.
 //
 // Exactness of the sample trace.  Ray set-up, positions, LOD choice and every
 // page-table / voxel index are FP32 in the reference's operation order (no
-// contraction, IEEE divide and square root).  Three rewrites keep that trace
-// bit-for-bit:
-//  * LOD: the reference evaluates pixelDist = endPixelDist * (t / tfar) at
-//    every sample.  That predicate is monotone in t, so the level can only
-//    grow along a ray; the first float t at which the current level stops
-//    qualifying is found once per level by bisection over float bit patterns
-//    and each sample compares t against it.
-//  * Voxel reuse: consecutive samples in the same voxel of the same brick and
-//    level read the same eight texels; their classified colour is reused.
-//  * The page entry and the usage stamp are refreshed once per brick visit.
+// contraction, IEEE divide and square root).  The LOD rewrite keeps that trace
+// bit-for-bit: the reference evaluates pixelDist = endPixelDist * (t / tfar)
+// at every sample and picks the coarsest level whose voxel spacing both pixel
+// distances exceed.  pixelDist is monotone in t and the spacings are monotone
+// in the level, so level(t) = max{l : T_l <= t} where T_l is the first float
+// t at which level l qualifies; T_l is found once per level by bisection over
+// float bit patterns.  Rays are independent, so processing order (packets,
+// refills, ray lists) does not change any pixel, usage stamp or request.
 // Compositing is FP32 (precision 32: opacity correction with the hardware
 // exp2/log2, FMA blending) or FP64 (precision 64).
 #include "gc_internal.hpp"
+
+#include <algorithm>
 
 namespace gc {
 
@@ -37,13 +53,15 @@ namespace {
 
 constexpr int TF_LDS = 256;
 constexpr int WAVES = 4;
+constexpr int LANES = 64 * WAVES;
+constexpr int MARCH = 4;   // samples per lane between refill checks
 constexpr float FINF = __builtin_huge_valf();
+constexpr uint32_t NONE = 0xffffffffu;
 
 struct LevelLds {
     float ior[3], rdv[3], ldf[3], vvf[3], inv[3];
     uint32_t lcoord;
     float step, kexp;
-    uint32_t unit_rdv;   // rdv == (1,1,1): the virtual position is the real one
 };
 
 struct F3 {
@@ -103,9 +121,9 @@ __device__ float first_above(float ep, float s, float tfar, float lo, float hi)
     return __uint_as_float(hb);
 }
 
-// First t >= t0 at which level `lod` stops being chosen: both pixel
-// distances exceed the next level's spacings (compute_LOD,
-// RayCasters.cu:551-567).  +inf if that never happens before tfar.
+// First t >= t0 at which a level with voxel spacings (sl, su) qualifies:
+// both pixel distances exceed them (compute_LOD, RayCasters.cu:551-567).
+// +inf if that never happens before tfar.
 __device__ float level_switch(float t0, float tfar, float epx, float epy, float sl, float su)
 {
     const bool x_far = pixel_dist(epx, tfar, tfar) > sl;
@@ -117,30 +135,32 @@ __device__ float level_switch(float t0, float tfar, float epx, float epy, float 
 }
 
 // Sum of the 8 texels {e-1, e}^3 of a voxel strictly inside its brick
-// (covering >= 1).  Each (x-1, x) pair is taken from one 8-byte load at the
-// 4-byte-aligned address below it (the pool is padded for the overrun).
+// (covering >= 1).  `a` addresses texel (ex-1, ey-1, ez-1); each (x-1, x)
+// pair is one unaligned load.
 template <int BYTES>
-__device__ inline uint32_t box_sum_pairs(const uint8_t *brick, uint32_t bs, uint32_t ex, uint32_t ey, uint32_t ez)
+__device__ inline uint32_t box_sum(const uint8_t *a, uint32_t row, uint32_t plane)
 {
-    const uint64_t row = (uint64_t)bs * BYTES;
-    const uint8_t *r0 = brick + ((uint64_t)(ez - 1) * bs + (ey - 1)) * row + (uint64_t)(ex - 1) * BYTES;
-    const uint8_t *rows[4] = {r0, r0 + row, r0 + row * bs, r0 + row * bs + row};
-    uint2 v[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const uintptr_t a = reinterpret_cast<uintptr_t>(rows[i]);
-        __builtin_memcpy(&v[i], __builtin_assume_aligned(reinterpret_cast<const void *>(a & ~(uintptr_t)3), 4), 8);
+    if constexpr (BYTES == 1) {
+        uint16_t h0, h1, h2, h3;
+        __builtin_memcpy(&h0, a, 2);
+        __builtin_memcpy(&h1, a + row, 2);
+        __builtin_memcpy(&h2, a + plane, 2);
+        __builtin_memcpy(&h3, a + plane + row, 2);
+        uint32_t s = __builtin_amdgcn_sad_u8(h0, 0u, 0u);
+        s = __builtin_amdgcn_sad_u8(h1, 0u, s);
+        s = __builtin_amdgcn_sad_u8(h2, 0u, s);
+        return __builtin_amdgcn_sad_u8(h3, 0u, s);
+    } else {
+        uint32_t w0, w1, w2, w3;
+        __builtin_memcpy(&w0, a, 4);
+        __builtin_memcpy(&w1, a + row, 4);
+        __builtin_memcpy(&w2, a + plane, 4);
+        __builtin_memcpy(&w3, a + plane + row, 4);
+        uint32_t s = __builtin_amdgcn_sad_u16(w0, 0u, 0u);
+        s = __builtin_amdgcn_sad_u16(w1, 0u, s);
+        s = __builtin_amdgcn_sad_u16(w2, 0u, s);
+        return __builtin_amdgcn_sad_u16(w3, 0u, s);
     }
-    uint32_t sum = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const uint32_t sh = (uint32_t)(reinterpret_cast<uintptr_t>(rows[i]) & 3) * 8;
-        const uint64_t q = ((uint64_t)v[i].y << 32) | v[i].x;
-        const uint32_t w = (uint32_t)(q >> sh);
-        if constexpr (BYTES == 1) sum += (w & 0xffu) + ((w >> 8) & 0xffu);
-        else sum += (w & 0xffffu) + (w >> 16);
-    }
-    return sum;
 }
 
 // General texel read in the reference's 3-D pool coordinates (covering 0, or
@@ -180,33 +200,17 @@ __device__ inline uint32_t pack(double r, double g, double b, double a)
     return (A << 24) | (B << 16) | (G << 8) | R;
 }
 
-__device__ inline uint32_t remap_block(uint32_t bid, uint32_t nblocks)
-{
-    // Workgroup bid runs on XCD bid % 8; give each XCD a contiguous range.
-    const uint32_t xcd = bid & 7u, k = bid >> 3;
-    uint32_t off = 0;
-    for (uint32_t y = 0; y < xcd; ++y) off += (nblocks + 7u - y) >> 3;
-    return off + k;
-}
-
-__device__ inline uint64_t wave_sum(uint64_t v)
+__device__ inline uint32_t wave_sum(uint32_t v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
-// Classified, opacity-corrected, premultiplied colour of a sample value
-// (tex1D linear + clamp, RayCasters.cu:163-175).  skip: alpha <= 0.001.
+// Opacity correction and premultiplication of an interpolated TF sample
+// (RayCasters.cu:163-175).  skip: alpha <= 0.001.
 template <typename ACC>
-__device__ inline Acc4<ACC> classify(ACC s, const float4 *TF, uint32_t N, float kexp, bool &skip)
+__device__ inline Acc4<ACC> shade(ACC a, float4 T0, float4 T1, float kexp, bool &skip)
 {
-    const ACC xf = s * (ACC)N - (ACC)0.5;
-    const ACC fl = floor(xf);
-    const ACC a = xf - fl;
-    int i0 = (int)fl, i1 = i0 + 1;
-    i0 = i0 < 0 ? 0 : (i0 > (int)N - 1 ? (int)N - 1 : i0);
-    i1 = i1 < 0 ? 0 : (i1 > (int)N - 1 ? (int)N - 1 : i1);
-    const float4 T0 = TF[i0], T1 = TF[i1];
     Acc4<ACC> col;
     if constexpr (sizeof(ACC) == 8) {
         const double b = 1.0 - a;
@@ -225,15 +229,61 @@ __device__ inline Acc4<ACC> classify(ACC s, const float4 *TF, uint32_t N, float 
     return col;
 }
 
-template <int BYTES, typename ACC, int TW, int TH>
-__global__ __launch_bounds__(64 * WAVES) void k_dvr(const DvrParams p)
+// tex1D with linear filtering and clamp addressing over N entries.
+template <typename ACC>
+__device__ inline Acc4<ACC> classify(ACC s, const float4 *TF, uint32_t N, float kexp, bool &skip)
 {
-    static_assert(TW * TH == 64, "one wave per ray tile");
+    const ACC xf = s * (ACC)N - (ACC)0.5;
+    const ACC fl = floor(xf);
+    const ACC a = xf - fl;
+    int i0 = (int)fl, i1 = i0 + 1;
+    i0 = i0 < 0 ? 0 : (i0 > (int)N - 1 ? (int)N - 1 : i0);
+    i1 = i1 < 0 ? 0 : (i1 > (int)N - 1 ? (int)N - 1 : i1);
+    return shade<ACC>(a, TF[i0], TF[i1], kexp, skip);
+}
+
+// The same lookup in the LDS copy of the table, which carries one clamped
+// entry on each side (TFp[j] = TF[clamp(j - 1)]).  Valid for s in [0, 1]
+// (integer voxels), where floor(s * N - 0.5) lies in [-1, N - 1].
+template <typename ACC>
+__device__ inline Acc4<ACC> classify_padded(ACC s, const float4 *TFp, uint32_t N, float kexp, bool &skip)
+{
+    const ACC xf = s * (ACC)N - (ACC)0.5;
+    const ACC fl = floor(xf);
+    const ACC a = xf - fl;
+    const int j = (int)fl + 1;
+    return shade<ACC>(a, TFp[j], TFp[j + 1], kexp, skip);
+}
+
+// Next packet for a wave on XCD `xcd`: its own contiguous share first, then
+// the other XCDs' shares.  NONE when every queue is drained.
+__device__ inline uint32_t fetch_packet(const DvrLaunch &p, uint32_t n_packets, uint32_t xcd)
+{
+    for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t x = (xcd + k) & 7u;
+        const uint32_t lo = (uint32_t)((uint64_t)n_packets * x / 8);
+        const uint32_t hi = (uint32_t)((uint64_t)n_packets * (x + 1) / 8);
+        if (lo >= hi) continue;
+        const uint32_t c = atomicAdd(&p.queue[x], 1u);
+        if (lo + c < hi) return lo + c;
+    }
+    return NONE;
+}
+
+template <int BYTES, typename ACC, int TW, int TH>
+__global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
+{
+    static_assert(TW * TH == 64, "one wave per ray packet");
     __shared__ LevelLds lv[GC_MAX_LEVELS];
-    __shared__ float4 tfl[TF_LDS];
+    __shared__ float4 tfl[TF_LDS + 2];
+    // dynamic LDS: level-switch distances [nb_lods][LANES], then the views
+    extern __shared__ __align__(16) uint8_t dyn_lds[];
+    const uint32_t nl = p.nb_lods;
+    float *tsw = reinterpret_cast<float *>(dyn_lds);
+    DvrView *vw = reinterpret_cast<DvrView *>(dyn_lds + (size_t)nl * LANES * sizeof(float));
     const int tid = threadIdx.x;
     const gc_geometry *g = p.geom;
-    if (tid < (int)p.nb_lods) {
+    if (tid < (int)nl) {
         LevelLds &L = lv[tid];
         for (int a = 0; a < 3; ++a) {
             L.ior[a] = g->ior[tid][a];
@@ -245,192 +295,287 @@ __global__ __launch_bounds__(64 * WAVES) void k_dvr(const DvrParams p)
         L.lcoord = g->level_coord[tid];
         L.step = p.lod_step[tid];
         L.kexp = p.kexp[tid];
-        L.unit_rdv = L.rdv[0] == 1.f && L.rdv[1] == 1.f && L.rdv[2] == 1.f;
     }
-    for (uint32_t i = tid; i < p.tf_size && i < (uint32_t)TF_LDS; i += 64 * WAVES) tfl[i] = p.tf[i];
+    const uint32_t n_views = p.views ? p.n_views : 1u;
+    if (p.views) {
+        for (uint32_t i = tid; i < n_views; i += LANES) vw[i] = p.views[i];
+    } else if (tid == 0) {
+        vw[0] = p.view0;
+    }
     __syncthreads();
-    const float4 *TF = p.tf_size <= (uint32_t)TF_LDS ? tfl : p.tf;
+    const uint32_t tf_n0 = __builtin_amdgcn_readfirstlane(vw[0].tf_size);
+    const bool tf_lds = n_views == 1 && tf_n0 <= (uint32_t)TF_LDS && BYTES != 4;
+    if (tf_lds) {
+        const int N = (int)vw[0].tf_size;
+        for (int i = tid; i < N + 2; i += LANES) tfl[i] = vw[0].tf[i < 1 ? 0 : (i > N ? N - 1 : i - 1)];
+    }
+    __syncthreads();
+    const uint32_t list_count = p.list ? *p.list_count : 0u;
+    const uint32_t n_packets = p.list ? (list_count + 63u) / 64u : p.n_packets;
 
-    const uint32_t ntiles = p.tiles_x * p.tiles_y;
-    const uint32_t nblocks = (ntiles + WAVES - 1) / WAVES;
-    const uint32_t tile = remap_block(blockIdx.x, nblocks) * WAVES + (tid >> 6);
     const uint32_t lane = tid & 63;
-    const uint32_t x = (tile % p.tiles_x) * TW + (lane % TW);
-    const uint32_t y = (tile / p.tiles_x) * TH + (lane / TW);
-    const bool active = tile < ntiles && x < p.width && y < p.height;
-
-    uint64_t n_samples = 0, n_visits = 0, n_unmapped = 0;
-    const uint32_t W = p.width, H = p.height;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const uint32_t xcd = blockIdx.x & 7u;
     const uint32_t bs = g->brick_size, cov = g->covering, inner = g->inner;
     const uint32_t cx_ = g->cache_size[0], cy_ = g->cache_size[1];
     const uint32_t MX = g->mrpd_dim[0], MY = g->mrpd_dim[1];
+    const uint32_t bs_shift = (bs & (bs - 1)) == 0 ? (uint32_t)__builtin_ctz(bs) : NONE;
     const uint64_t bbytes = (uint64_t)bs * bs * bs * BYTES;
+    const uint32_t row = bs * BYTES, plane = bs * bs * BYTES;
     const uint8_t *pool = static_cast<const uint8_t *>(p.pool);
-    const float *M = p.m;
-    const uint32_t nl = p.nb_lods;
+    // byte offset of texel (cov-1, cov-1, cov-1) inside a brick
+    const uint32_t corner = cov >= 1 ? (cov - 1) * (plane + row + BYTES) : 0u;
     const bool fast_fetch = BYTES != 4 && cov >= 1;
     const ACC inv_norm = BYTES == 1 ? (ACC)(1.0 / 2040.0) : (ACC)(1.0 / 524280.0);
+    const float tstep = p.tstep;
 
-    if (active) {
-        // RayCasters.cu:57-84
-        uint32_t xn = x + 1;
-        if (xn > W) xn = x - 1;
-        uint32_t yn = y + 1;
-        if (yn > H) yn = y - 1;
-        const float u = (((float)x + 0.5f) / (float)W) * 2.0f - 1.0f;
-        const float un = (((float)xn + 0.5f) / (float)W) * 2.0f - 1.0f;
-        const float v = (((float)y + 0.5f) / (float)H) * 2.0f - 1.0f;
-        const float vn = (((float)yn + 0.5f) / (float)H) * 2.0f - 1.0f;
-        const F3 origin = f3(M[3], M[7], M[11]);
-        const F3 dir = mulm(M, normalize(f3(u, v, -2.f)));
-        const F3 dxn = mulm(M, normalize(f3(un, v, -2.f)));
-        const F3 dyn = mulm(M, normalize(f3(u, vn, -2.f)));
-        const F3 dleft = normalize(cross(dir, dyn));
-        const F3 dup = normalize(cross(dir, dxn));
+    // wave-uniform packet stream: the current packet and its next free slot
+    uint32_t cur_pk = 0, cur_j = 64;
+    bool exhausted = false;
 
-        // intersectBox (RayCasters.cu:506-525)
-        const F3 invr = f3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
-        const F3 tbot = invr * (f3(p.bmin[0], p.bmin[1], p.bmin[2]) - origin);
-        const F3 ttop = invr * (f3(p.bmax[0], p.bmax[1], p.bmax[2]) - origin);
-        const F3 tmin = f3(fminf(ttop.x, tbot.x), fminf(ttop.y, tbot.y), fminf(ttop.z, tbot.z));
-        const F3 tmax = f3(fmaxf(ttop.x, tbot.x), fmaxf(ttop.y, tbot.y), fmaxf(ttop.z, tbot.z));
-        float tnear = fmaxf(fmaxf(tmin.x, tmin.y), fmaxf(tmin.x, tmin.z));
-        const float tfar = fminf(fminf(tmax.x, tmax.y), fminf(tmax.x, tmax.z));
-        const bool hit = tfar > tnear;
+    // per-lane ray state
+    bool have = false;
+    uint32_t v = 0, pix = 0;
+    float t = 0.f, tfar = 0.f, astop = 0.f;
+    F3 pos = f3(0, 0, 0), dir = f3(0, 0, 0), dstep = f3(0, 0, 0);
+    // current level: index, next switch distance and its parameters
+    uint32_t lod = 0, lcoord = 0;
+    float tnext = FINF, step = 0.f, kexp = 0.f;
+    F3 ior = f3(0, 0, 0), rdv = f3(0, 0, 0), ldf = f3(0, 0, 0), vvf = f3(0, 0, 0);
+    // current brick: page-table cell, entry state, slot and base address
+    uint32_t pcx = NONE, pcy = NONE, pcz = NONE, ew = 0, slot = 0, vkey = NONE;
+    ACC vs = 0;
+    const uint8_t *bbase = pool;
+    Acc4<ACC> fc{0, 0, 0, 0};
+    uint32_t n_samples = 0, n_visits = 0;      // per lane
+    uint32_t n_rays = 0, n_unmapped = 0;       // per wave
 
-        const uint64_t pi = (uint64_t)y * W + x;
-        tnear = fmaxf(tnear, p.tbuf[pi]);
-        Acc4<ACC> fc;
-        if constexpr (sizeof(ACC) == 8) {
-            const double4 c = reinterpret_cast<const double4 *>(p.cstate)[pi];
-            fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
-        } else {
-            const float4 c = reinterpret_cast<const float4 *>(p.cstate)[pi];
-            fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
-        }
-        bool unmapped = false;
-        if (!hit) {
-            fc.x = fc.y = fc.z = (ACC)0;
-        } else {
-            if (tnear < 0.f) tnear = 0.f;
-            float t = tnear;
-            F3 pos = origin + dir * tnear;
-            const float epx = fabsf(dot(dir - dxn, dleft) * tfar);
-            const float epy = fabsf(dot(dir - dyn, dup) * tfar);
-            const ACC astop = (ACC)p.alpha_stop;
-            const uint32_t N = p.tf_size;
+    auto set_level = [&](uint32_t l) {
+        const LevelLds &L = lv[l];
+        ior = f3(L.ior[0], L.ior[1], L.ior[2]);
+        rdv = f3(L.rdv[0], L.rdv[1], L.rdv[2]);
+        ldf = f3(L.ldf[0], L.ldf[1], L.ldf[2]);
+        vvf = f3(L.vvf[0], L.vvf[1], L.vvf[2]);
+        lcoord = L.lcoord;
+        step = L.step;
+        kexp = L.kexp;
+        dstep = dir * step;
+        tnext = l + 1 < nl ? tsw[(l + 1) * LANES + tid] : FINF;
+    };
 
-            // level state: lod, the distance where it stops qualifying, and
-            // the level's constants
-            uint32_t lod = 0;
-            float tswitch = -1.f;   // forces a settle before the first sample
-            float step = 0.f, kexp = 0.f;
-            F3 dstep = f3(0.f, 0.f, 0.f);
-            // brick / voxel caches
-            uint32_t pcx = 0xffffffffu, pcy = 0xffffffffu, pcz = 0xffffffffu;
-            uint4 e = make_uint4(0, 0, 0, 0);
-            const uint8_t *bbase = pool;
-            uint32_t pvx = 0xffffffffu, pvy = 0, pvz = 0;
-            Acc4<ACC> cc{0, 0, 0, 0};
-            bool cskip = true;
-
-            for (;;) {
-                if (!(fc.w < astop && t < tfar)) break;
-                if (t >= tswitch) {
-                    // settle the level at t exactly as compute_LOD does, then
-                    // find where it next changes
-                    const float pdx = pixel_dist(epx, t, tfar), pdy = pixel_dist(epy, t, tfar);
-                    while (lod + 1 < nl &&
-                           !(spacing(dleft, lv[lod + 1].inv) >= pdx * 1.0f || spacing(dup, lv[lod + 1].inv) >= pdy * 1.0f))
-                        ++lod;
-                    tswitch = lod + 1 < nl ? level_switch(t, tfar, epx, epy, spacing(dleft, lv[lod + 1].inv),
-                                                          spacing(dup, lv[lod + 1].inv))
-                                           : FINF;
-                    step = lv[lod].step;
-                    kexp = lv[lod].kexp;
-                    dstep = dir * step;
+    for (;;) {
+        // ------------------------------------------------------------ refill
+        const uint64_t idle = __ballot(!have);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (n_idle && !exhausted && (n_idle >= p.refill_min || n_idle == 64)) {
+            const uint32_t avail = 64 - cur_j;
+            uint32_t next_pk = NONE;
+            if (n_idle > avail) {
+                uint32_t got = NONE;
+                if (lane == 0) got = fetch_packet(p, n_packets, xcd);
+                next_pk = __shfl(got, 0, 64);
+                if (next_pk == NONE) exhausted = true;
+            }
+            uint32_t my_pk = NONE, my_j = 0;
+            if (!have) {
+                const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
+                if (rank < avail) { my_pk = cur_pk; my_j = cur_j + rank; }
+                else if (next_pk != NONE && rank - avail < 64) { my_pk = next_pk; my_j = rank - avail; }
+            }
+            if (n_idle >= avail) {
+                if (next_pk != NONE) { cur_pk = next_pk; cur_j = n_idle - avail; }
+                else cur_j = 64;
+            } else {
+                cur_j += n_idle;
+            }
+            bool counted = false;
+            if (my_pk != NONE) {
+                // resolve the slot to (view, pixel)
+                bool valid = false;
+                uint32_t x = 0, y = 0;
+                if (p.list) {
+                    const uint64_t idx = (uint64_t)my_pk * 64 + my_j;
+                    if (idx < list_count) {
+                        const uint32_t e = p.list[idx];
+                        v = e >> DVR_PIXEL_BITS;
+                        const uint32_t px = e & ((1u << DVR_PIXEL_BITS) - 1u);
+                        x = px % vw[v].width;
+                        y = px / vw[v].width;
+                        valid = true;
+                    }
+                } else {
+                    uint32_t vv = 0;
+                    while (vv + 1 < n_views && vw[vv + 1].packet_off <= my_pk) ++vv;
+                    v = vv;
+                    const uint32_t tt = my_pk - vw[v].packet_off;
+                    x = (tt % vw[v].tiles_x) * TW + my_j % TW;
+                    y = (tt / vw[v].tiles_x) * TH + my_j / TW;
+                    valid = x < vw[v].width && y < vw[v].height;
                 }
-                const LevelLds &L = lv[lod];
+                if (valid) {
+                    // ray set-up (RayCasters.cu:57-120)
+                    const DvrView &V = vw[v];
+                    const uint32_t W = V.width, H = V.height;
+                    uint32_t xn = x + 1;
+                    if (xn > W) xn = x - 1;
+                    uint32_t yn = y + 1;
+                    if (yn > H) yn = y - 1;
+                    const float u = (((float)x + 0.5f) / (float)W) * 2.0f - 1.0f;
+                    const float un = (((float)xn + 0.5f) / (float)W) * 2.0f - 1.0f;
+                    const float vy = (((float)y + 0.5f) / (float)H) * 2.0f - 1.0f;
+                    const float vn = (((float)yn + 0.5f) / (float)H) * 2.0f - 1.0f;
+                    const float *M = V.m;
+                    const F3 origin = f3(M[3], M[7], M[11]);
+                    dir = mulm(M, normalize(f3(u, vy, -2.f)));
+                    const F3 dxn = mulm(M, normalize(f3(un, vy, -2.f)));
+                    const F3 dyn = mulm(M, normalize(f3(u, vn, -2.f)));
+                    const F3 dleft = normalize(cross(dir, dyn));
+                    const F3 dup = normalize(cross(dir, dxn));
+                    // intersectBox (RayCasters.cu:506-525)
+                    const F3 invr = f3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+                    const F3 tbot = invr * (f3(V.bmin[0], V.bmin[1], V.bmin[2]) - origin);
+                    const F3 ttop = invr * (f3(V.bmax[0], V.bmax[1], V.bmax[2]) - origin);
+                    const F3 tmin = f3(fminf(ttop.x, tbot.x), fminf(ttop.y, tbot.y), fminf(ttop.z, tbot.z));
+                    const F3 tmax = f3(fmaxf(ttop.x, tbot.x), fmaxf(ttop.y, tbot.y), fmaxf(ttop.z, tbot.z));
+                    float tnear = fmaxf(fmaxf(tmin.x, tmin.y), fmaxf(tmin.x, tmin.z));
+                    tfar = fminf(fminf(tmax.x, tmax.y), fminf(tmax.x, tmax.z));
+                    const bool hit = tfar > tnear;
+                    pix = y * W + x;
+                    tnear = fmaxf(tnear, V.tbuf[pix]);
+                    if constexpr (sizeof(ACC) == 8) {
+                        const double4 c = reinterpret_cast<const double4 *>(V.cstate)[pix];
+                        fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
+                    } else {
+                        const float4 c = reinterpret_cast<const float4 *>(V.cstate)[pix];
+                        fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
+                    }
+                    counted = true;
+                    if (!hit) {
+                        fc.x = fc.y = fc.z = (ACC)0;
+                        V.pixels[pix] = pack(fc.x, fc.y, fc.z, fc.w);
+                    } else {
+                        if (tnear < 0.f) tnear = 0.f;
+                        t = tnear;
+                        pos = origin + dir * tnear;
+                        const float epx = fabsf(dot(dir - dxn, dleft) * tfar);
+                        const float epy = fabsf(dot(dir - dyn, dup) * tfar);
+                        astop = V.alpha_stop;
+                        // level-switch distances from t on; the level at t
+                        // is the last one already reached (compute_LOD)
+                        uint32_t l0 = 0;
+                        for (uint32_t l = 1; l < nl; ++l) {
+                            const float T = level_switch(t, tfar, epx, epy, spacing(dleft, lv[l].inv),
+                                                         spacing(dup, lv[l].inv));
+                            tsw[l * LANES + tid] = T;
+                            if (T <= t && l0 == l - 1) l0 = l;
+                            if (T == FINF) break;
+                        }
+                        lod = l0;
+                        set_level(l0);
+                        pcx = pcy = pcz = NONE;
+                        vkey = NONE;
+                        have = true;
+                    }
+                }
+            }
+            n_rays += (uint32_t)__popcll(__ballot(counted));
+        }
+        if (__ballot(have) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+
+        // ------------------------------------------------------------- march
+        bool stopped = false;   // stopped at an Unmapped brick in this round
+        if (have) {
+            bool done = false;
+            for (int k = 0; k < MARCH; ++k) {
+                if (!((ACC)fc.w < (ACC)astop && t < tfar)) { done = true; break; }
+                while (t >= tnext) set_level(++lod);
                 F3 tp = pos * 0.495f + f3(0.495f, 0.495f, 0.495f);
                 tp.z = 1.0f - tp.z;
                 const F3 np = f3(clamp099(tp.x), clamp099(tp.y), clamp099(tp.z));
-                const F3 pr = np * f3(L.ior[0], L.ior[1], L.ior[2]);
-                const F3 pv = L.unit_rdv ? pr : pr * f3(L.rdv[0], L.rdv[1], L.rdv[2]);
-                const uint32_t bx = (uint32_t)(pv.x * L.ldf[0]);
-                const uint32_t by = (uint32_t)(pv.y * L.ldf[1]);
-                const uint32_t bz = (uint32_t)(pv.z * L.ldf[2]);
-                const uint32_t mcx = L.lcoord + bx;
+                const F3 pr = np * ior;
+                const F3 pv = pr * rdv;   // exact when rdv == 1
+                const uint32_t bx = (uint32_t)(pv.x * ldf.x);
+                const uint32_t by = (uint32_t)(pv.y * ldf.y);
+                const uint32_t bz = (uint32_t)(pv.z * ldf.z);
+                const uint32_t mcx = lcoord + bx;
                 if (mcx != pcx || by != pcy || bz != pcz) {
                     pcx = mcx; pcy = by; pcz = bz;
-                    e = p.mrpd[mcx + (uint64_t)MX * (by + (uint64_t)MY * bz)];
+                    const uint4 e = p.mrpd[mcx + (uint64_t)MX * (by + (uint64_t)MY * bz)];
+                    ew = e.w;
                     ++n_visits;
-                    pvx = 0xffffffffu;
-                    if (e.w == 1) {
-                        const uint32_t slot = e.x / bs + cx_ * (e.y / bs + cy_ * (e.z / bs));
-                        p.usage[slot] = p.ts;
-                        bbase = pool + (uint64_t)slot * bbytes;
-                    }
+                    slot = bs_shift != NONE
+                               ? (e.x >> bs_shift) + cx_ * ((e.y >> bs_shift) + cy_ * (e.z >> bs_shift))
+                               : e.x / bs + cx_ * (e.y / bs + cy_ * (e.z / bs));
+                    bbase = pool + (uint64_t)slot * bbytes + corner;
+                    vkey = NONE;
+                    if (ew == 1) p.usage[slot] = p.ts;
                 }
-                if (e.w == 0) {
+                if (ew == 0) {
                     // raise_brick_request with the real-volume position
-                    const uint32_t rx = L.lcoord + (uint32_t)(pr.x * L.ldf[0]);
-                    const uint32_t ry = (uint32_t)(pr.y * L.ldf[1]);
-                    const uint32_t rz = (uint32_t)(pr.z * L.ldf[2]);
+                    const uint32_t rx = lcoord + (uint32_t)(pr.x * ldf.x);
+                    const uint32_t ry = (uint32_t)(pr.y * ldf.y);
+                    const uint32_t rz = (uint32_t)(pr.z * ldf.z);
                     p.request[rx + (uint64_t)MX * (ry + (uint64_t)MY * rz)] = p.ts;
-                    p.tbuf[pi] = t;
-                    if constexpr (sizeof(ACC) == 8)
-                        reinterpret_cast<double4 *>(p.cstate)[pi] = make_double4(fc.x, fc.y, fc.z, fc.w);
-                    else
-                        reinterpret_cast<float4 *>(p.cstate)[pi] = make_float4(fc.x, fc.y, fc.z, fc.w);
-                    unmapped = true;
-                    ++n_unmapped;
+                    stopped = true;
                     break;
                 }
-                if (e.w != 1) {
-                    // Empty brick: advance by tstep (RayCasters.cu:154-159)
-                    t += p.tstep;
-                    if (t > tfar) break;
-                    pos = pos + dir * p.tstep;
+                if (ew != 1) {
+                    // Empty brick: advance by tstep (RayCasters.cu:154-159);
+                    // a ray that passes tfar ends at the next check
+                    t += tstep;
+                    pos = pos + dir * tstep;
                     continue;
                 }
-                const uint32_t vx = (uint32_t)(pv.x * L.vvf[0]);
-                const uint32_t vy = (uint32_t)(pv.y * L.vvf[1]);
-                const uint32_t vz = (uint32_t)(pv.z * L.vvf[2]);
-                if (vx != pvx || vy != pvy || vz != pvz) {
-                    pvx = vx; pvy = vy; pvz = vz;
-                    // voxel within the brick: v mod (bs-2c), via the brick
-                    // index already known (falls back when FP32 rounding put
-                    // the voxel index in a neighbouring brick)
-                    uint32_t lx = vx - bx * inner, ly = vy - by * inner, lz = vz - bz * inner;
-                    if (lx >= inner) lx = vx % inner;
-                    if (ly >= inner) ly = vy % inner;
-                    if (lz >= inner) lz = vz % inner;
-                    ACC s;
-                    if (fast_fetch) {
-                        const uint32_t sum = box_sum_pairs<BYTES == 4 ? 1 : BYTES>(bbase, bs, cov + lx, cov + ly, cov + lz);
-                        if constexpr (sizeof(ACC) == 8) s = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
-                        else s = (float)sum * inv_norm;
-                    } else {
-                        const int ex = (int)(cov + e.x + lx), ey = (int)(cov + e.y + ly), ez = (int)(cov + e.z + lz);
-                        float fsum = 0.f;
-                        double dsum = 0.0;
-                        for (int k = 0; k < 2; ++k)
-                            for (int j = 0; j < 2; ++j)
-                                for (int i = 0; i < 2; ++i) {
-                                    const float tv = texel<BYTES>(pool, g, ex - 1 + i, ey - 1 + j, ez - 1 + k);
-                                    fsum += tv;
-                                    dsum += tv;
-                                }
-                        if constexpr (BYTES == 4) {
-                            if constexpr (sizeof(ACC) == 8) s = dsum / 8.0;
-                            else s = fsum / 8.0f;
-                        } else {
-                            if constexpr (sizeof(ACC) == 8) s = dsum / (BYTES == 1 ? 2040.0 : 524280.0);
-                            else s = (float)dsum * inv_norm;
-                        }
-                    }
-                    cc = classify<ACC>(s, TF, N, kexp, cskip);
+                // voxel within the brick: v mod (bs-2c), via the brick index
+                // already known (falls back when FP32 rounding put the voxel
+                // index in a neighbouring brick)
+                const uint32_t vx = (uint32_t)(pv.x * vvf.x);
+                const uint32_t vy = (uint32_t)(pv.y * vvf.y);
+                const uint32_t vz = (uint32_t)(pv.z * vvf.z);
+                uint32_t lx = vx - bx * inner, ly = vy - by * inner, lz = vz - bz * inner;
+                if (__builtin_expect(lx >= inner || ly >= inner || lz >= inner, 0)) {
+                    lx = vx % inner;
+                    ly = vy % inner;
+                    lz = vz % inner;
                 }
+                ACC s;
+                if (fast_fetch) {
+                    const uint32_t key = (lz * bs + ly) * bs + lx + (lod << 24);
+                    if (key != vkey) {
+                        vkey = key;
+                        const uint32_t sum = box_sum<BYTES == 4 ? 1 : BYTES>(
+                            bbase + ((lz * bs + ly) * bs + lx) * BYTES, row, plane);
+                        if constexpr (sizeof(ACC) == 8) vs = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
+                        else vs = (float)sum * inv_norm;
+                    }
+                    s = vs;
+                } else {
+                    uint32_t o[3];
+                    slot_origin(*g, slot, o);
+                    const int ex = (int)(cov + o[0] + lx), ey = (int)(cov + o[1] + ly), ez = (int)(cov + o[2] + lz);
+                    float fsum = 0.f;
+                    double dsum = 0.0;
+                    for (int kk = 0; kk < 2; ++kk)
+                        for (int j = 0; j < 2; ++j)
+                            for (int i = 0; i < 2; ++i) {
+                                const float tv = texel<BYTES>(pool, g, ex - 1 + i, ey - 1 + j, ez - 1 + kk);
+                                fsum += tv;
+                                dsum += tv;
+                            }
+                    if constexpr (BYTES == 4) {
+                        if constexpr (sizeof(ACC) == 8) s = dsum / 8.0;
+                        else s = fsum / 8.0f;
+                    } else {
+                        if constexpr (sizeof(ACC) == 8) s = dsum / (BYTES == 1 ? 2040.0 : 524280.0);
+                        else s = (float)dsum * inv_norm;
+                    }
+                }
+                bool cskip;
+                const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(s, tfl, tf_n0, kexp, cskip)
+                                            : classify<ACC>(s, vw[v].tf, vw[v].tf_size, kexp, cskip);
                 ++n_samples;
                 if (!cskip) {
                     // front-to-back blending (RayCasters.cu:176-178)
@@ -448,21 +593,34 @@ __global__ __launch_bounds__(64 * WAVES) void k_dvr(const DvrParams p)
                 t += step;
                 pos = pos + dstep;
             }
-            if (!unmapped) {
-                p.tbuf[pi] = t;
+            if (done || stopped) {
+                // store the progressive state and the pixel (RayCasters.cu:138-147,189-197)
+                const DvrView &V = vw[v];
+                V.tbuf[pix] = t;
                 if constexpr (sizeof(ACC) == 8)
-                    reinterpret_cast<double4 *>(p.cstate)[pi] = make_double4(fc.x, fc.y, fc.z, fc.w);
+                    reinterpret_cast<double4 *>(V.cstate)[pix] = make_double4(fc.x, fc.y, fc.z, fc.w);
                 else
-                    reinterpret_cast<float4 *>(p.cstate)[pi] = make_float4(fc.x, fc.y, fc.z, fc.w);
+                    reinterpret_cast<float4 *>(V.cstate)[pix] = make_float4(fc.x, fc.y, fc.z, fc.w);
+                V.pixels[pix] = pack(fc.x, fc.y, fc.z, fc.w);
+                have = false;
             }
         }
-        p.pixels[pi] = pack(fc.x, fc.y, fc.z, fc.w);
+        // rays stopped at a miss go to the next launch's ray list
+        const uint64_t st = __ballot(stopped);
+        n_unmapped += (uint32_t)__popcll(st);
+        if (p.out_list) {
+            if (st) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(p.out_count, (uint32_t)__popcll(st));
+                base = __shfl(base, 0, 64);
+                if (stopped) p.out_list[base + (uint32_t)__popcll(st & lt_mask)] = (v << DVR_PIXEL_BITS) | pix;
+            }
+        }
     }
 
     if (p.stats) {
-        const uint64_t s0 = wave_sum(n_samples), s1 = wave_sum(n_visits),
-                       s2 = wave_sum(active ? 1ull : 0ull), s3 = wave_sum(n_unmapped);
-        if ((tid & 63) == 0) {
+        const uint32_t s0 = wave_sum(n_samples), s1 = wave_sum(n_visits), s2 = n_rays, s3 = n_unmapped;
+        if (lane == 0) {
             if (s0) atomicAdd((unsigned long long *)&p.stats->mapped_samples, (unsigned long long)s0);
             if (s1) atomicAdd((unsigned long long *)&p.stats->brick_visits, (unsigned long long)s1);
             if (s2) atomicAdd((unsigned long long *)&p.stats->pixels, (unsigned long long)s2);
@@ -472,37 +630,77 @@ __global__ __launch_bounds__(64 * WAVES) void k_dvr(const DvrParams p)
 }
 
 template <int BYTES, typename ACC>
-hipError_t launch_tile(DvrParams p, uint32_t tile, hipStream_t stream)
+const void *kernel_of(uint32_t tile)
 {
-    static const int TW[4] = {8, 16, 32, 64};
-    if (tile > 3) return hipErrorInvalidValue;
-    const int tw = TW[tile], th = 64 / tw;
-    p.tiles_x = (p.width + tw - 1) / tw;
-    p.tiles_y = (p.height + th - 1) / th;
-    const uint32_t ntiles = p.tiles_x * p.tiles_y;
-    const uint32_t nb = (ntiles + WAVES - 1) / WAVES;
-    if (nb == 0) return hipSuccess;
-    switch (tile) {
-    case 0: k_dvr<BYTES, ACC, 8, 8><<<nb, 64 * WAVES, 0, stream>>>(p); break;
-    case 1: k_dvr<BYTES, ACC, 16, 4><<<nb, 64 * WAVES, 0, stream>>>(p); break;
-    case 2: k_dvr<BYTES, ACC, 32, 2><<<nb, 64 * WAVES, 0, stream>>>(p); break;
-    case 3: k_dvr<BYTES, ACC, 64, 1><<<nb, 64 * WAVES, 0, stream>>>(p); break;
+    switch (tile & 3) {
+    case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8>);
+    case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4>);
+    case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2>);
+    default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1>);
     }
-    return hipGetLastError();
+}
+
+const void *dvr_kernel(uint32_t bytes, uint32_t precision, uint32_t tile)
+{
+    const bool d = precision == 64;
+    switch (bytes) {
+    case 1: return d ? kernel_of<1, double>(tile) : kernel_of<1, float>(tile);
+    case 2: return d ? kernel_of<2, double>(tile) : kernel_of<2, float>(tile);
+    default: return d ? kernel_of<4, double>(tile) : kernel_of<4, float>(tile);
+    }
+}
+
+__global__ void k_fill32(uint32_t *p, uint32_t n, uint32_t value)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
 }
 
 } // namespace
 
-hipError_t launch_dvr(const DvrParams &p, uint32_t bytes, uint32_t precision, uint32_t tile,
+uint32_t dvr_lds_bytes(uint32_t nb_lods, uint32_t n_views)
+{
+    return nb_lods * LANES * (uint32_t)sizeof(float) + std::max(1u, n_views) * (uint32_t)sizeof(DvrView);
+}
+
+uint32_t dvr_grid(uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t n_packets, uint32_t lds_bytes)
+{
+    // enough resident workgroups to fill every CU, never more than the packets
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dvr_kernel(bytes, precision, tile), LANES,
+                                                     lds_bytes) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 2;
+    const uint32_t want = (uint32_t)cus * (uint32_t)per_cu;
+    const uint32_t need = (n_packets + WAVES - 1) / WAVES;
+    return std::max(1u, std::min(want, need));
+}
+
+hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t grid,
                       hipStream_t stream)
 {
-    const bool d = precision == 64;
-    switch (bytes) {
-    case 1: return d ? launch_tile<1, double>(p, tile, stream) : launch_tile<1, float>(p, tile, stream);
-    case 2: return d ? launch_tile<2, double>(p, tile, stream) : launch_tile<2, float>(p, tile, stream);
-    case 4: return d ? launch_tile<4, double>(p, tile, stream) : launch_tile<4, float>(p, tile, stream);
-    default: return hipErrorInvalidValue;
-    }
+    if ((!p.list && p.n_packets == 0) || grid == 0) return hipSuccess;
+    if (bytes != 1 && bytes != 2 && bytes != 4) return hipErrorInvalidValue;
+    if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
+    if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
+    if (p.list && !p.list_count) return hipErrorInvalidValue;
+    void *args[] = {const_cast<DvrLaunch *>(&p)};
+    return hipLaunchKernel(dvr_kernel(bytes, precision, tile), dim3(grid), dim3(LANES), args,
+                           dvr_lds_bytes(p.nb_lods, p.views ? p.n_views : 1), stream);
+}
+
+hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    k_fill32<<<(n + 255) / 256, 256, 0, stream>>>(p, n, value);
+    return hipGetLastError();
 }
 
 } // namespace gc

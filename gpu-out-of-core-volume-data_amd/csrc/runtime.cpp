@@ -659,14 +659,68 @@ struct Frame {
     uint32_t *pixels = nullptr;
     float *tbuf = nullptr;
     void *cstate = nullptr;
+    // launch scratch: [0..7] per-XCD packet cursors, [8], [9] ray-list counts
+    uint32_t *ctl = nullptr;
+    uint32_t *lists[2] = {nullptr, nullptr};   // rays stopped at a miss
     size_t cbytes() const { return (size_t)w * h * (precision == 64 ? 32 : 16); }
     ~Frame()
     {
         hipFree(pixels);
         hipFree(tbuf);
         hipFree(cstate);
+        hipFree(ctl);
+        hipFree(lists[0]);
+        hipFree(lists[1]);
     }
 };
+
+// Shared part of a ray-caster launch over cache C: device state, timestamp
+// and the LOD tables of VolumeRayCaster.cu:136-161 (start-up box [-1,1]^3).
+static void launch_base(const Cache &C, DvrLaunch &L)
+{
+    L.mrpd = C.mrpd;
+    L.request = C.request;
+    L.usage = C.usage;
+    L.pool = C.pool;
+    L.geom = C.d_geom;
+    L.ts = C.ts;
+    const float smin = 2.f;
+    L.tstep = smin / 256.f;
+    L.nb_lods = C.vc.nb_levels;
+    for (uint32_t i = 0; i < C.vc.nb_levels; ++i) {
+        const uint32_t *ls = C.vc.initial_sizes[i];
+        for (int a = 0; a < 3; ++a) L.inv_levels[i][a] = smin / (float)ls[a];
+        const uint32_t m = std::min(ls[0], std::min(ls[1], ls[2]));
+        L.lod_step[i] = (smin / (float)m) / 2.f;
+        L.kexp[i] = L.lod_step[i] / L.tstep;
+    }
+}
+
+static void view_of(const gc_view &v, const float *d_tf, uint32_t tf_size, const Frame &F, DvrView &out)
+{
+    uint32_t tw, th;
+    tile_shape(v.tile, tw, th);
+    out.width = v.width;
+    out.height = v.height;
+    out.tiles_x = (v.width + tw - 1) / tw;
+    out.packet_off = 0;
+    std::memcpy(out.m, v.inv_mv, sizeof(out.m));
+    std::memcpy(out.bmin, v.bbox_min, sizeof(out.bmin));
+    std::memcpy(out.bmax, v.bbox_max, sizeof(out.bmax));
+    out.alpha_stop = v.alpha_stop;
+    out.tf_size = tf_size;
+    out.tf = reinterpret_cast<const float4 *>(d_tf);
+    out.pixels = F.pixels;
+    out.tbuf = F.tbuf;
+    out.cstate = F.cstate;
+}
+
+static uint32_t packets_of(const gc_view &v)
+{
+    uint32_t tw, th;
+    tile_shape(v.tile, tw, th);
+    return ((v.width + tw - 1) / tw) * ((v.height + th - 1) / th);
+}
 
 } // namespace gc
 
@@ -1061,9 +1115,14 @@ int gc_frame_create(uint32_t width, uint32_t height, uint32_t precision, gc_fram
     auto f = std::make_unique<gc_frame>();
     Frame &F = f->impl;
     F.w = width; F.h = height; F.precision = precision;
+    if ((uint64_t)width * height >= (1ull << DVR_PIXEL_BITS)) return fail(GC_ERR_ARG, "frame larger than 2^26 pixels");
     GC_TRY(dalloc(&F.pixels, (size_t)width * height * 4));
     GC_TRY(dalloc(&F.tbuf, (size_t)width * height * 4));
     GC_TRY(dalloc(&F.cstate, F.cbytes()));
+    GC_TRY(dalloc(&F.ctl, 64));
+    GC_TRY(dalloc(&F.lists[0], (size_t)width * height * 4));
+    GC_TRY(dalloc(&F.lists[1], (size_t)width * height * 4));
+    GC_HIP(hipMemset(F.ctl, 0, 64));
     GC_HIP(hipMemset(F.pixels, 0, (size_t)width * height * 4));
     GC_HIP(hipMemset(F.tbuf, 0, (size_t)width * height * 4));
     GC_HIP(hipMemset(F.cstate, 0, F.cbytes()));
@@ -1127,38 +1186,51 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
     Frame &F = f->impl;
     if (view->width != F.w || view->height != F.h) return fail(GC_ERR_ARG, "view and frame sizes differ");
     if (view->precision != F.precision) return fail(GC_ERR_ARG, "view and frame precision differ");
-    DvrParams p{};
-    p.mrpd = C.mrpd;
-    p.request = C.request;
-    p.usage = C.usage;
-    p.pool = C.pool;
-    p.geom = C.d_geom;
-    p.tf = reinterpret_cast<const float4 *>(d_tf);
-    p.tf_size = tf_size;
-    p.ts = C.ts;
-    p.pixels = F.pixels;
-    p.tbuf = F.tbuf;
-    p.cstate = F.cstate;
-    p.stats = d_stats;
-    p.width = view->width;
-    p.height = view->height;
-    std::memcpy(p.m, view->inv_mv, sizeof(p.m));
-    std::memcpy(p.bmin, view->bbox_min, sizeof(p.bmin));
-    std::memcpy(p.bmax, view->bbox_max, sizeof(p.bmax));
-    p.alpha_stop = view->alpha_stop;
-    // VolumeRayCaster.cu:136-161 with the start-up box [-1,1]^3
-    const float smin = 2.f;
-    p.tstep = smin / 256.f;
-    p.nb_lods = C.vc.nb_levels;
-    for (uint32_t i = 0; i < C.vc.nb_levels; ++i) {
-        const uint32_t *ls = C.vc.initial_sizes[i];
-        for (int a = 0; a < 3; ++a) p.inv_levels[i][a] = smin / (float)ls[a];
-        const uint32_t m = std::min(ls[0], std::min(ls[1], ls[2]));
-        p.lod_step[i] = (smin / (float)m) / 2.f;
-        p.kexp[i] = p.lod_step[i] / p.tstep;
-    }
+    if (view->tile > 3) return fail(GC_ERR_ARG, "tile must be 0..3");
+    DvrLaunch L{};
+    launch_base(C, L);
+    view_of(*view, d_tf, tf_size, F, L.view0);
+    L.n_views = 1;
+    L.n_packets = packets_of(*view);
+    L.stats = d_stats;
+    L.queue = F.ctl;
+    L.refill_min = view->refill_min ? view->refill_min : 16;
     hipStream_t s = stream ? (hipStream_t)stream : C.stream;
-    GC_HIP(launch_dvr(p, C.vc.encoded_bytes, view->precision, view->tile, s));
+    GC_HIP(launch_fill32(F.ctl, 8, 0, s));
+    const uint32_t grid = dvr_grid(C.vc.encoded_bytes, view->precision, view->tile, L.n_packets,
+                                   dvr_lds_bytes(L.nb_lods, 1));
+    GC_HIP(launch_dvr(L, C.vc.encoded_bytes, view->precision, view->tile, grid, s));
+    return GC_OK;
+}
+
+// One launch of the progressive frame loop: the first covers every tile of
+// the view, the next ones only the rays the previous launch stopped at a miss
+// (the other rays are finished and would neither sample nor write anything
+// new).  Returns the index of the list the launch fills.
+static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_tf, uint32_t tf_size,
+                        gc_render_stats *d_stats, int first, int &cur, hipStream_t s)
+{
+    DvrLaunch L{};
+    launch_base(C, L);
+    view_of(view, d_tf, tf_size, F, L.view0);
+    L.n_views = 1;
+    L.n_packets = packets_of(view);
+    L.stats = d_stats;
+    L.queue = F.ctl;
+    L.refill_min = view.refill_min ? view.refill_min : 16;
+    const int out = first ? 0 : 1 - cur;
+    if (!first) {
+        L.list = F.lists[cur];
+        L.list_count = F.ctl + 8 + cur;
+    }
+    L.out_list = F.lists[out];
+    L.out_count = F.ctl + 8 + out;
+    GC_HIP(launch_fill32(F.ctl, 8, 0, s));
+    GC_HIP(launch_fill32(F.ctl + 8 + out, 1, 0, s));
+    const uint32_t grid = dvr_grid(C.vc.encoded_bytes, view.precision, view.tile,
+                                   first ? L.n_packets : (F.w * F.h + 63) / 64, dvr_lds_bytes(L.nb_lods, 1));
+    GC_HIP(launch_dvr(L, C.vc.encoded_bytes, view.precision, view.tile, grid, s));
+    cur = out;
     return GC_OK;
 }
 
@@ -1182,10 +1254,13 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
         GC_HIP(hipEventCreate(&C.ev0));
         GC_HIP(hipEventCreate(&C.ev1));
     }
+    if (view->width != f->impl.w || view->height != f->impl.h || view->precision != f->impl.precision || view->tile > 3)
+        return fail(GC_ERR_ARG, "view does not match the frame");
     GC_HIP(hipMemsetAsync(C.d_frame_stats, 0, sizeof(gc_render_stats), C.stream));
+    int cur = 0;
     for (uint32_t it = 0; it < max_launches; ++it) {
         GC_HIP(hipEventRecord(C.ev0, C.stream));
-        GC_TRY(gc_render_dvr(c, view, d_tf, tf_size, f, C.d_frame_stats, C.stream));
+        GC_TRY(frame_launch(C, f->impl, *view, d_tf, tf_size, C.d_frame_stats, it == 0, cur, C.stream));
         GC_HIP(hipEventRecord(C.ev1, C.stream));
         ++report->launches;
         int done = 0;

@@ -26,13 +26,6 @@ uint64_t splitmix64(uint64_t &state)
     return z ^ (z >> 31);
 }
 
-static inline uint64_t mix64(uint64_t z)
-{
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
-}
-
 static inline double uniform01(uint64_t &s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
 
 Synth::Synth(const gc_synth_params &p, const uint32_t dims0[3], uint32_t bytes) : prm(p), bytes(bytes)
@@ -53,32 +46,44 @@ Synth::Synth(const gc_synth_params &p, const uint32_t dims0[3], uint32_t bytes) 
 }
 
 // Separable Gaussian factor along one axis for a level-l voxel coordinate:
-// the voxel centre in level-0 units is (i + 0.5) * 2^l.
-static inline double axis_factor(const Synth::Blob &b, int a, uint32_t level, int64_t i)
+// the voxel centre in level-0 units is (i + 0.5) * 2^l.  It depends on the
+// global coordinate only, so a voxel gets the same value in every box.
+static inline float axis_factor(const Synth::Blob &b, int a, uint32_t level, int64_t i)
 {
     const double c = ((double)i + 0.5) * (double)(1u << level);
     const double d = c - b.c[a];
-    if (std::fabs(d) > b.radius) return 0.0;
-    return std::exp(-d * d * b.k);
+    if (std::fabs(d) > b.radius) return 0.f;
+    return (float)std::exp(-d * d * b.k);
+}
+
+// 32-bit avalanche hash (murmur3 finaliser) of a voxel key.
+static inline uint32_t hash32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
 }
 
 int Synth::noise_at(uint32_t level, int64_t x, int64_t y, int64_t z) const
 {
     if (prm.noise == 0) return 0;
-    uint64_t h = prm.seed ^ 0x5bd1e9955bd1e995ull;
-    h = mix64(h + (uint64_t)level * 0x9e3779b97f4a7c15ull);
-    h = mix64(h + (uint64_t)x);
-    h = mix64(h + (uint64_t)y * 0xc2b2ae3d27d4eb4full);
-    h = mix64(h + (uint64_t)z * 0x165667b19e3779f9ull);
-    return (int)(h % (2ull * prm.noise + 1ull)) - (int)prm.noise;
+    const uint32_t span = 2u * prm.noise + 1u;
+    const uint32_t key = (uint32_t)prm.seed ^ ((uint32_t)level * 0x27d4eb2fu) ^ ((uint32_t)z * 0x165667b1u) ^
+                         ((uint32_t)y * 0xc2b2ae3du);
+    const uint32_t h = hash32(key ^ ((uint32_t)x * 0x9e3779b1u));
+    return (int)(((uint64_t)h * span) >> 32) - (int)prm.noise;
 }
 
 // Fill a box [o, o+n) of level-`level` voxels (x-fastest into dst, row pitch
-// n[0]); voxels outside the level extent `ext` are 0.
+// n[0]); voxels outside the level extent `ext` are 0.  value = clamp(
+// round(sum of blobs) + noise), with the blob sum in float in blob order.
 void Synth::fill_box(uint32_t level, const int64_t o[3], const uint32_t n[3], const uint32_t ext[3],
                      void *dst) const
 {
-    // blobs that reach the box
+    // blobs that reach the box (the others contribute exact zeros)
     std::vector<const Blob *> near;
     const double sc = (double)(1u << level);
     for (const auto &b : blobs) {
@@ -90,41 +95,51 @@ void Synth::fill_box(uint32_t level, const int64_t o[3], const uint32_t n[3], co
         if (ok) near.push_back(&b);
     }
     const size_t nb = near.size();
-    std::vector<double> fx(nb * n[0]), fy(nb * n[1]), fz(nb * n[2]);
+    std::vector<float> fx(nb * n[0]), fy(nb * n[1]), fz(nb * n[2]);
     for (size_t k = 0; k < nb; ++k) {
         for (uint32_t i = 0; i < n[0]; ++i) fx[k * n[0] + i] = axis_factor(*near[k], 0, level, o[0] + i);
         for (uint32_t i = 0; i < n[1]; ++i) fy[k * n[1] + i] = axis_factor(*near[k], 1, level, o[1] + i);
-        for (uint32_t i = 0; i < n[2]; ++i) fz[k * n[2] + i] = near[k]->amp * axis_factor(*near[k], 2, level, o[2] + i);
+        for (uint32_t i = 0; i < n[2]; ++i) fz[k * n[2] + i] = (float)near[k]->amp * axis_factor(*near[k], 2, level, o[2] + i);
     }
-    std::vector<double> row(n[0]);
+    // per-row x keys of the noise hash
+    std::vector<uint32_t> xkey(n[0]);
+    for (uint32_t i = 0; i < n[0]; ++i) xkey[i] = (uint32_t)(o[0] + i) * 0x9e3779b1u;
+    const uint32_t span = 2u * prm.noise + 1u;
+    const float fmax = (float)maxval;
+    std::vector<float> row(n[0]);
+    std::vector<int32_t> nz(n[0]);
+    // x range inside the level
+    const int64_t x0 = std::max<int64_t>(0, -o[0]), x1 = std::min<int64_t>((int64_t)n[0], (int64_t)ext[0] - o[0]);
     for (uint32_t kz = 0; kz < n[2]; ++kz) {
         const int64_t gz = o[2] + kz;
         for (uint32_t ky = 0; ky < n[1]; ++ky) {
             const int64_t gy = o[1] + ky;
-            const uint64_t rbase = (uint64_t)n[0] * (ky + (uint64_t)n[1] * kz);
             const bool row_in = gz >= 0 && gy >= 0 && gz < (int64_t)ext[2] && gy < (int64_t)ext[1];
-            std::fill(row.begin(), row.end(), 0.0);
-            if (row_in) {
-                for (size_t k = 0; k < nb; ++k) {
-                    const double w = fz[k * n[2] + kz] * fy[k * n[1] + ky];
-                    if (w == 0.0) continue;
-                    const double *f = &fx[k * n[0]];
-                    for (uint32_t i = 0; i < n[0]; ++i) row[i] += w * f[i];
-                }
+            if (!row_in || x1 <= x0) continue;   // stays 0
+            const uint64_t rbase = (uint64_t)n[0] * (ky + (uint64_t)n[1] * kz);
+            std::fill(row.begin(), row.end(), 0.f);
+            for (size_t k = 0; k < nb; ++k) {
+                const float w = fz[k * n[2] + kz] * fy[k * n[1] + ky];
+                if (w == 0.f) continue;
+                const float *f = &fx[k * n[0]];
+                float *r = row.data();
+                for (uint32_t i = 0; i < n[0]; ++i) r[i] += w * f[i];
             }
-            for (uint32_t kx = 0; kx < n[0]; ++kx) {
-                const int64_t gx = o[0] + kx;
-                double v = 0.0;
-                if (row_in && gx >= 0 && gx < (int64_t)ext[0]) {
-                    v = std::nearbyint(row[kx]) + (double)noise_at(level, gx, gy, gz);
-                    v = v < 0.0 ? 0.0 : (v > maxval ? maxval : v);
-                } else {
-                    continue;   // stays 0
-                }
+            if (prm.noise) {
+                const uint32_t rk = (uint32_t)prm.seed ^ (level * 0x27d4eb2fu) ^ ((uint32_t)gz * 0x165667b1u) ^
+                                    ((uint32_t)gy * 0xc2b2ae3du);
+                for (uint32_t i = 0; i < n[0]; ++i)
+                    nz[i] = (int32_t)(((uint64_t)hash32(rk ^ xkey[i]) * span) >> 32) - (int32_t)prm.noise;
+            } else {
+                std::fill(nz.begin(), nz.end(), 0);
+            }
+            for (int64_t kx = x0; kx < x1; ++kx) {
+                float v = (float)(int32_t)(row[kx] + 0.5f) + (float)nz[kx];   // row >= 0: round half up
+                v = v < 0.f ? 0.f : (v > fmax ? fmax : v);
                 const uint64_t di = rbase + kx;
                 if (bytes == 1) static_cast<uint8_t *>(dst)[di] = (uint8_t)v;
                 else if (bytes == 2) static_cast<uint16_t *>(dst)[di] = (uint16_t)v;
-                else static_cast<float *>(dst)[di] = (float)v;
+                else static_cast<float *>(dst)[di] = v;
             }
         }
     }

@@ -235,8 +235,9 @@ typedef struct gc_view {
     float    bbox_min[3], bbox_max[3];
     float    alpha_stop;         /* early termination; 0.95 in the reference   */
     uint32_t precision;          /* 32: FP32 compositing, 64: FP64 compositing */
-    uint32_t tile;               /* ray tile: 0 8x8, 1 16x8, 2 32x4, 3 64x1    */
-    uint32_t reserved;
+    uint32_t tile;               /* 64-ray packet: 0 8x8, 1 16x4, 2 32x2, 3 64x1 */
+    uint32_t refill_min;         /* refill a wave's idle lanes once this many are
+                                    idle (0: default 16)                      */
 } gc_view;
 
 /* Per-view progressive state (tBuffer, colorStateBuffer, pixel buffer) on the
