@@ -778,6 +778,94 @@ int gc_cfg_get(const char *cfg_path, const char *key, char *value, size_t value_
     return GC_OK;
 }
 
+// Level records of a bricks.xml (<Levels><Level initial_size_X=".." ...
+// real_size_Z=".."/>..</Levels>, VolumeConfiguration.cpp:64-98).
+static bool read_bricks_xml(const std::string &path, gc_volume_cfg &vc)
+{
+    std::FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::string text;
+    char buf[4096];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+    std::fclose(f);
+    auto attr = [](const std::string &tag, const char *name, uint32_t &out) {
+        const std::string key = std::string(name) + "=\"";
+        const size_t p = tag.find(key);
+        if (p == std::string::npos) return false;
+        out = (uint32_t)std::strtoul(tag.c_str() + p + key.size(), nullptr, 10);
+        return true;
+    };
+    const size_t lv = text.find("<Levels");
+    if (lv == std::string::npos) return false;
+    const size_t lv_end = text.find("</Levels>", lv);
+    uint32_t nl = 0;
+    for (size_t p = text.find("<Level ", lv); p != std::string::npos && (lv_end == std::string::npos || p < lv_end);
+         p = text.find("<Level ", p + 1)) {
+        const std::string tag = text.substr(p, text.find('>', p) - p);
+        if (nl >= GC_MAX_LEVELS) return false;
+        const char *in[3] = {"initial_size_X", "initial_size_Y", "initial_size_Z"};
+        const char *re[3] = {"real_size_X", "real_size_Y", "real_size_Z"};
+        for (int a = 0; a < 3; ++a)
+            if (!attr(tag, in[a], vc.initial_sizes[nl][a]) || !attr(tag, re[a], vc.real_sizes[nl][a])) return false;
+        ++nl;
+    }
+    if (nl == 0) return false;
+    vc.nb_levels = nl;
+    return true;
+}
+
+int gc_volume_cfg_load(const char *cfg_path, gc_volume_cfg *out, char *volume_path, size_t volume_path_size)
+{
+    if (!cfg_path || !out) return fail(GC_ERR_ARG, "null argument");
+    if (std::FILE *f = std::fopen(cfg_path, "rb")) std::fclose(f);
+    else return fail(GC_ERR_CONFIG, std::string("VolumeConfiguration: unable to open ") + cfg_path);
+    auto need = [&](const char *key, std::string &v) { return cfg_lookup(cfg_path, key, v); };
+    auto need_u = [&](const char *key, uint32_t &v) {
+        std::string s;
+        if (!need(key, s)) return false;
+        v = (uint32_t)std::strtoul(s.c_str(), nullptr, 10);
+        return true;
+    };
+    uint32_t bs = 0, bb[3] = {0, 0, 0}, cov = 0, bytes = 0, channels = 0;
+    std::string file, wd;
+    const struct { const char *key; uint32_t *v; } req[] = {
+        {"BrickSize", &bs}, {"BigBrickSizeX", &bb[0]}, {"BigBrickSizeY", &bb[1]}, {"BigBrickSizeZ", &bb[2]},
+        {"VoxelCovering", &cov}, {"NumberEncodedBytes", &bytes}, {"NumberChannels", &channels}};
+    for (const auto &r : req)
+        if (!need_u(r.key, *r.v))
+            return fail(GC_ERR_CONFIG, std::string("VolumeConfiguration: Missing ") + r.key + " in configuration.");
+    if (!need("VolumeFile", file)) return fail(GC_ERR_CONFIG, "VolumeConfiguration: Missing VolumeFile in configuration.");
+    need("WorkingDirectory", wd);
+    if (bytes != 1 && bytes != 2 && bytes != 4) return fail(GC_ERR_CONFIG, "NumberEncodedBytes must be 1, 2 or 4");
+    if (channels != 1) return fail(GC_ERR_CONFIG, "NumberChannels must be 1");
+    if (bb[0] == 0 || bb[1] == 0 || bb[2] == 0) return fail(GC_ERR_CONFIG, "BigBrickSize must be positive");
+    const size_t slash = file.find_last_of("/\\"), dot = file.find_last_of('.');
+    const std::string base = file.substr(slash + 1, dot - (slash + 1));
+    const std::string bricks_xml = wd + base + "/bricks_" + std::to_string(bs) + "_" + std::to_string(bs) + "_" +
+                                   std::to_string(bs) + "/bricks.xml";
+    gc_volume_cfg vc;
+    std::memset(&vc, 0, sizeof(vc));
+    if (read_bricks_xml(bricks_xml, vc)) {
+        for (int a = 0; a < 3; ++a) {
+            vc.brick_size[a] = bs;
+            vc.big_brick_size[a] = bb[a];
+            vc.covering[a] = cov;
+        }
+        vc.encoded_bytes = bytes;
+        vc.channels = channels;
+    } else {
+        uint32_t dims[3];
+        if (!need_u("size_X", dims[0]) || !need_u("size_Y", dims[1]) || !need_u("size_Z", dims[2]))
+            return fail(GC_ERR_CONFIG, "VolumeConfiguration: no " + bricks_xml + " and no size_X/Y/Z in configuration.");
+        const int rc = gc_volume_cfg_init(&vc, dims, bs, cov, bb, bytes);
+        if (rc != GC_OK) return rc;
+    }
+    *out = vc;
+    if (volume_path && volume_path_size) std::snprintf(volume_path, volume_path_size, "%s%s", wd.c_str(), file.c_str());
+    return GC_OK;
+}
+
 int gc_device_count(int *count)
 {
     if (!count) return fail(GC_ERR_ARG, "null argument");

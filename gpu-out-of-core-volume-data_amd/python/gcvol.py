@@ -98,7 +98,7 @@ class CacheView(C.Structure):
 
 
 EXPORTS = [
-    "gc_last_error", "gc_version", "gc_volume_cfg_init", "gc_pool_shape", "gc_cfg_get",
+    "gc_last_error", "gc_version", "gc_volume_cfg_init", "gc_pool_shape", "gc_cfg_get", "gc_volume_cfg_load",
     "gc_device_count", "gc_set_device", "gc_malloc", "gc_free", "gc_memcpy_h2d", "gc_memcpy_d2h", "gc_memcpy_d2d",
     "gc_memset", "gc_synchronize", "gc_stream_create", "gc_stream_destroy", "gc_stream_synchronize",
     "gc_mip_reduce", "gc_brick_cut", "gc_source_create_pyramid", "gc_source_create_synthetic",
@@ -132,6 +132,7 @@ def lib():
             "gc_last_error": [], "gc_version": [],
             "gc_volume_cfg_init": [C.POINTER(VolumeCfg), P, u32, u32, P, u32],
             "gc_pool_shape": [u64, u64, P], "gc_cfg_get": [C.c_char_p, C.c_char_p, C.c_char_p, SZ],
+            "gc_volume_cfg_load": [C.c_char_p, P, C.c_char_p, SZ],
             "gc_device_count": [C.POINTER(I)], "gc_set_device": [I],
             "gc_malloc": [C.POINTER(vp), SZ], "gc_free": [P], "gc_memcpy_h2d": [P, P, SZ],
             "gc_memcpy_d2h": [P, P, SZ], "gc_memcpy_d2d": [P, P, SZ], "gc_memset": [P, I, SZ],
@@ -190,6 +191,14 @@ def volume_cfg(dims, brick_size, covering, encoded_bytes, big_brick=(1, 1, 1)):
     check(lib().gc_volume_cfg_init(C.byref(vc), ptr(np.asarray(dims, np.uint32)), u32(brick_size),
                                    u32(covering), ptr(np.asarray(big_brick, np.uint32)), u32(encoded_bytes)))
     return vc
+
+
+def volume_cfg_load(cfg_path):
+    """load_volume_configuration of a GcCore .cfg: (gc_volume_cfg, volume path)."""
+    vc = VolumeCfg()
+    buf = C.create_string_buffer(4096)
+    check(lib().gc_volume_cfg_load(str(cfg_path).encode(), C.byref(vc), buf, 4096))
+    return vc, buf.value.decode()
 
 
 def cache_cfg(cache_size, brick_size, max_requests=50, sync=True, empty_threshold=0.0):

@@ -80,6 +80,16 @@ int gc_pool_shape(uint64_t budget_bytes, uint64_t brick_bytes, uint32_t out[3]);
  * copies the value, or GC_ERR_ARG if the key is absent. */
 int gc_cfg_get(const char *cfg_path, const char *key, char *value, size_t value_size);
 
+/* tdns::data::load_volume_configuration (VolumeConfiguration.cpp:14-110):
+ * BrickSize, BigBrickSizeX/Y/Z, VoxelCovering, NumberEncodedBytes,
+ * NumberChannels and VolumeFile are required (GC_ERR_CONFIG names the missing
+ * field).  The level sizes come from
+ * <WorkingDirectory><VolumeFile base name>/bricks_B_B_B/bricks.xml when that
+ * file exists, else from size_X/Y/Z by the preprocessing rules.
+ * volume_path (optional) receives WorkingDirectory + VolumeFile. */
+int gc_volume_cfg_load(const char *cfg_path, gc_volume_cfg *out, char *volume_path,
+                       size_t volume_path_size);
+
 /* ------------------------------------------------------------ device memory */
 
 int gc_device_count(int *count);

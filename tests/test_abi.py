@@ -73,3 +73,106 @@ def test_cfg_reader(gcv, tmp_path):
     p.write_text(ref)
     assert gcv.lib().gc_cfg_get(str(p).encode(), b"ScreenWidth", buf, 64) == 0 and buf.value == b"1920"
     assert gcv.lib().gc_cfg_get(str(p).encode(), b"NumberEncodedBytes", buf, 64) == 0 and buf.value == b"1"
+
+
+REF_CFG = ('<Fields>\n<Field key="BrickSize" value="16"/>\n<Field key="BigBrickSizeX" value="1"/>\n'
+           '<Field key="BigBrickSizeY" value="1"/>\n<Field key="BigBrickSizeZ" value="1"/>\n'
+           '<Field key="VoxelCovering" value="1"/>\n<Field key="NumberEncodedBytes" value="1" />\n'
+           '<Field key="NumberChannels" value="1" />\n<Field key="MultipleFiles" value="0" />\n'
+           '<Field key="size_X" value="256" />\n<Field key="size_Y" value="256" />\n<Field key="size_Z" value="256" />\n'
+           '<Field key="WorkingDirectory" value="{wd}" />\n'
+           '<Field key="VolumeFile" value="skull_256x256x256_uint8.raw" />\n</Fields>\n')
+
+
+def test_volume_cfg_load_sizes(gcv, tmp_path):
+    """load_volume_configuration on the reference's sample .cfg: without a
+    bricks.xml the levels follow the preprocessing rules."""
+    p = tmp_path / "skull.cfg"
+    p.write_text(REF_CFG.format(wd=str(tmp_path) + "/"))
+    vc, path = gcv.volume_cfg_load(p)
+    ref = gcv.volume_cfg((256, 256, 256), 16, 1, 1)
+    assert vc.nb_levels == ref.nb_levels == 5
+    assert np.array_equal(vc.initial(), ref.initial()) and np.array_equal(vc.real(), ref.real())
+    assert path == str(tmp_path) + "/skull_256x256x256_uint8.raw"
+
+
+def test_volume_cfg_load_bricks_xml(gcv, tmp_path):
+    """With <WorkingDirectory><base>/bricks_16_16_16/bricks.xml present, the
+    level records come from it (VolumeConfiguration.cpp:57-98)."""
+    p = tmp_path / "skull.cfg"
+    p.write_text(REF_CFG.format(wd=str(tmp_path) + "/"))
+    d = tmp_path / "skull_256x256x256_uint8" / "bricks_16_16_16"
+    d.mkdir(parents=True)
+    (d / "bricks.xml").write_text(
+        '<?xml version="1.0"?>\n<Bricks>\n<Levels>\n'
+        '<Level initial_size_X="250" initial_size_Y="240" initial_size_Z="230" '
+        'real_size_X="252" real_size_Y="252" real_size_Z="238"/>\n'
+        '<Level initial_size_X="125" initial_size_Y="120" initial_size_Z="115" '
+        'real_size_X="126" real_size_Y="126" real_size_Z="126"/>\n'
+        '</Levels>\n<Empty/>\n</Bricks>\n')
+    vc, _ = gcv.volume_cfg_load(p)
+    assert vc.nb_levels == 2
+    assert vc.initial().tolist() == [[250, 240, 230], [125, 120, 115]]
+    assert vc.real().tolist() == [[252, 252, 238], [126, 126, 126]]
+
+
+def test_volume_cfg_load_errors(gcv, tmp_path):
+    L = gcv.lib()
+    vc = gcv.VolumeCfg()
+    assert L.gc_volume_cfg_load(str(tmp_path / "none.cfg").encode(), C.byref(vc), None, 0) == 3
+    p = tmp_path / "c.cfg"
+    p.write_text(REF_CFG.format(wd="").replace('<Field key="BigBrickSizeY" value="1"/>\n', ""))
+    assert L.gc_volume_cfg_load(str(p).encode(), C.byref(vc), None, 0) == 3
+    assert b"Missing BigBrickSizeY" in L.gc_last_error()
+
+
+SHIM_PROG = r'''
+#include <tdns/gpucache/CacheManager.hpp>
+int main(int, char **argv)
+{
+    const tdns::data::VolumeConfiguration vc = tdns::data::load_volume_configuration(argv[1]);
+    std::printf("%u %u %u %u %u %s %s\n", vc.BrickSize[0], vc.Covering[0], vc.EncodedBytes, vc.NbLevels,
+                vc.NbBricks[0][0], vc.VolumeDirectory.c_str(), vc.VolumeFileName.c_str());
+    tdns::data::MetaData md;
+    tdns::data::CacheConfiguration cc;
+    try {
+        tdns::gpucache::CacheManager<uint8_t> m(vc, md, cc, 0);
+    } catch (const std::runtime_error &e) {
+        std::printf("%s\n", e.what());
+    }
+    try {
+        tdns::data::load_volume_configuration("/nonexistent.cfg");
+    } catch (const std::runtime_error &e) {
+        std::printf("%s\n", e.what());
+    }
+    return 0;
+}
+'''
+
+
+def test_cpp_shim_host_compile_and_config(gcv, tmp_path):
+    """The C++ drop-in header compiles with the host compiler alone, links
+    against libgcvol.so, and reports errors as std::runtime_error."""
+    src = tmp_path / "prog.cpp"
+    src.write_text(SHIM_PROG)
+    exe = tmp_path / "prog"
+    pkg = os.path.dirname(gcv.LIB_PATH)
+    subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                           str(src), "-L" + pkg, "-lgcvol", "-Wl,-rpath," + pkg, "-o", str(exe)])
+    cfg = tmp_path / "skull.cfg"
+    cfg.write_text(REF_CFG.format(wd="/data/"))
+    out = subprocess.check_output([str(exe), str(cfg)]).decode().splitlines()
+    # NbBricks = RealVolumesSizes / BrickSize as the reference computes it
+    # (VolumeConfiguration.cpp:101-108): 266 / 16 = 16
+    assert out[0] == "16 1 1 5 16 /data/ skull_256x256x256_uint8.raw"
+    assert out[1] == "CacheManager: CacheConfiguration needs CacheSize[0] and BlockSize[0]"
+    assert out[2].startswith("VolumeConfiguration: unable to open")
+
+
+def test_shim_demo_built(gcv):
+    """The example application over the drop-in headers is built beside the
+    library (its run is a GPU test)."""
+    exe = os.path.join(os.path.dirname(gcv.LIB_PATH), "shim_demo")
+    assert os.access(exe, os.X_OK)
+    out = subprocess.check_output(["ldd", exe]).decode()
+    assert "libgcvol.so" in out and "oracle" not in out
