@@ -130,8 +130,8 @@ hipError_t launch_brick_cut(const void *level, const uint32_t dims[3], uint32_t 
 
 // ------------------------------------------------------------------ DVR
 // A launch renders one or more views over one cache.  Work is split into
-// packets of 64 rays: either the TWxTH pixel tiles of every view, or 64
-// consecutive entries of an explicit ray list (view << 26 | pixel).
+// packets of 64 rays: 64 consecutive entries of an explicit ray list
+// (view << 26 | pixel), then the TWxTH pixel tiles of the starting views.
 constexpr uint32_t DVR_MAX_VIEWS = 64;
 constexpr uint32_t DVR_PIXEL_BITS = 26;
 
@@ -154,6 +154,7 @@ struct DvrLaunch {
     const void *pool;
     const gc_geometry *geom;
     uint32_t ts;
+    const uint32_t *ts_dev;     // timestamp read on the device instead (graph replay), or null
     float tstep;
     uint32_t nb_lods;
     float inv_levels[GC_MAX_LEVELS][3];
@@ -162,14 +163,16 @@ struct DvrLaunch {
     DvrView view0;              // the view when `views` is null
     const DvrView *views;       // device table of n_views views, or null
     uint32_t n_views;
-    uint32_t n_packets;         // tile packets (ignored with a ray list)
+    uint32_t n_packets;         // tile packets, after the ray list's packets
     const uint32_t *list;       // explicit ray list, or null for all tiles
     const uint32_t *list_count; // device count of `list`
     uint32_t *queue;            // 8 per-XCD packet cursors, zero at launch
     uint32_t *out_list;         // rays stopped at an Unmapped brick (or null)
     uint32_t *out_count;
+    uint32_t *view_stopped;     // per view: rays stopped at an Unmapped brick (or null)
     gc_render_stats *stats;
     uint32_t refill_min;        // refill idle lanes once this many are idle
+    uint32_t march;             // samples per lane between refill checks (1..8)
 };
 
 // Tile shape of a ray packet: 0 8x8, 1 16x4, 2 32x2, 3 64x1.

@@ -4,10 +4,11 @@
 // K_CacheDevice.hpp:191-201,316-334, inlined).
 //
 // Work distribution
-//  * Rays come in packets of 64: a TWxTH pixel tile (8x8, 16x4, 32x2 or 64x1)
-//    of one view, or 64 entries of an explicit ray list (the rays a previous
-//    launch of the same frame stopped at an Unmapped brick).  One launch can
-//    cover many views (batched rendering).
+//  * Rays come in packets of 64: 64 entries of an explicit ray list (the
+//    rays a previous launch stopped at an Unmapped brick, view << 26 | pixel)
+//    come first, then the TWxTH pixel tiles (8x8, 16x4, 32x2 or 64x1) of the
+//    views that start in this launch.  One launch can cover many views
+//    (batched rendering) and mix continuing and starting views (serving).
 //  * The kernel is persistent: waves pull packets from 8 per-XCD queues
 //    (workgroup b runs on XCD b % 8 and starts on that XCD's contiguous share
 //    of the packets, so neighbouring tiles share an L2), stealing from the
@@ -54,7 +55,6 @@ namespace {
 constexpr int TF_LDS = 256;
 constexpr int WAVES = 4;
 constexpr int LANES = 64 * WAVES;
-constexpr int MARCH = 4;   // samples per lane between refill checks
 constexpr float FINF = __builtin_huge_valf();
 constexpr uint32_t NONE = 0xffffffffu;
 
@@ -79,7 +79,7 @@ __device__ inline F3 cross(F3 a, F3 b)
 }
 __device__ inline F3 normalize(F3 v)
 {
-    const float inv = 1.0f / __fsqrt_rn(dot(v, v));
+    const float inv = 1.0f / sqrtf(dot(v, v));   // correctly rounded (build flag)
     return v * inv;
 }
 __device__ inline F3 mulm(const float *m, F3 v)
@@ -90,7 +90,7 @@ __device__ inline F3 mulm(const float *m, F3 v)
 __device__ inline float spacing(F3 d, const float *inv)
 {
     const F3 a = d * f3(inv[0], inv[1], inv[2]);
-    return __fsqrt_rn(dot(a, a));
+    return sqrtf(dot(a, a));
 }
 __device__ inline float clamp099(float v) { return fmaxf(0.f, fminf(v, 0.99f)); }
 
@@ -310,8 +310,10 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
         for (int i = tid; i < N + 2; i += LANES) tfl[i] = vw[0].tf[i < 1 ? 0 : (i > N ? N - 1 : i - 1)];
     }
     __syncthreads();
+    // packets: first the ray list (64 entries each), then the views' tiles
     const uint32_t list_count = p.list ? *p.list_count : 0u;
-    const uint32_t n_packets = p.list ? (list_count + 63u) / 64u : p.n_packets;
+    const uint32_t list_packets = (list_count + 63u) / 64u;
+    const uint32_t n_packets = list_packets + p.n_packets;
 
     const uint32_t lane = tid & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -328,6 +330,8 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
     const bool fast_fetch = BYTES != 4 && cov >= 1;
     const ACC inv_norm = BYTES == 1 ? (ACC)(1.0 / 2040.0) : (ACC)(1.0 / 524280.0);
     const float tstep = p.tstep;
+    const uint32_t ts = p.ts_dev ? *p.ts_dev : p.ts;
+    const int march = (int)p.march;
 
     // wave-uniform packet stream: the current packet and its next free slot
     uint32_t cur_pk = 0, cur_j = 64;
@@ -393,7 +397,7 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
                 // resolve the slot to (view, pixel)
                 bool valid = false;
                 uint32_t x = 0, y = 0;
-                if (p.list) {
+                if (my_pk < list_packets) {
                     const uint64_t idx = (uint64_t)my_pk * 64 + my_j;
                     if (idx < list_count) {
                         const uint32_t e = p.list[idx];
@@ -404,10 +408,11 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
                         valid = true;
                     }
                 } else {
+                    const uint32_t tp = my_pk - list_packets;
                     uint32_t vv = 0;
-                    while (vv + 1 < n_views && vw[vv + 1].packet_off <= my_pk) ++vv;
+                    while (vv + 1 < n_views && vw[vv + 1].packet_off <= tp) ++vv;
                     v = vv;
-                    const uint32_t tt = my_pk - vw[v].packet_off;
+                    const uint32_t tt = tp - vw[v].packet_off;
                     x = (tt % vw[v].tiles_x) * TW + my_j % TW;
                     y = (tt / vw[v].tiles_x) * TH + my_j / TW;
                     valid = x < vw[v].width && y < vw[v].height;
@@ -489,7 +494,7 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
         bool stopped = false;   // stopped at an Unmapped brick in this round
         if (have) {
             bool done = false;
-            for (int k = 0; k < MARCH; ++k) {
+            for (int k = 0; k < march; ++k) {
                 if (!((ACC)fc.w < (ACC)astop && t < tfar)) { done = true; break; }
                 while (t >= tnext) set_level(++lod);
                 F3 tp = pos * 0.495f + f3(0.495f, 0.495f, 0.495f);
@@ -511,14 +516,14 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
                                : e.x / bs + cx_ * (e.y / bs + cy_ * (e.z / bs));
                     bbase = pool + (uint64_t)slot * bbytes + corner;
                     vkey = NONE;
-                    if (ew == 1) p.usage[slot] = p.ts;
+                    if (ew == 1) p.usage[slot] = ts;
                 }
                 if (ew == 0) {
                     // raise_brick_request with the real-volume position
                     const uint32_t rx = lcoord + (uint32_t)(pr.x * ldf.x);
                     const uint32_t ry = (uint32_t)(pr.y * ldf.y);
                     const uint32_t rz = (uint32_t)(pr.z * ldf.z);
-                    p.request[rx + (uint64_t)MX * (ry + (uint64_t)MY * rz)] = p.ts;
+                    p.request[rx + (uint64_t)MX * (ry + (uint64_t)MY * rz)] = ts;
                     stopped = true;
                     break;
                 }
@@ -608,6 +613,7 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
         // rays stopped at a miss go to the next launch's ray list
         const uint64_t st = __ballot(stopped);
         n_unmapped += (uint32_t)__popcll(st);
+        if (stopped && p.view_stopped) atomicAdd(&p.view_stopped[v], 1u);
         if (p.out_list) {
             if (st) {
                 uint32_t base = 0;
@@ -687,10 +693,12 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
                       hipStream_t stream)
 {
     if ((!p.list && p.n_packets == 0) || grid == 0) return hipSuccess;
+    if (p.n_packets && !p.views && p.n_views != 1) return hipErrorInvalidValue;
     if (bytes != 1 && bytes != 2 && bytes != 4) return hipErrorInvalidValue;
     if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
     if (p.list && !p.list_count) return hipErrorInvalidValue;
+    if (p.march < 1 || p.march > 8) return hipErrorInvalidValue;
     void *args[] = {const_cast<DvrLaunch *>(&p)};
     return hipLaunchKernel(dvr_kernel(bytes, precision, tile), dim3(grid), dim3(LANES), args,
                            dvr_lds_bytes(p.nb_lods, p.views ? p.n_views : 1), stream);

@@ -281,7 +281,8 @@ struct Cache {
     hipStream_t stream = nullptr, upload = nullptr;
     hipEvent_t uploaded = nullptr, consumed = nullptr;
     uint32_t ts = 1;
-    uint32_t cap = 50;
+    uint32_t cap = 50;                  // request buffers' capacity (max_requests at create)
+    uint32_t limit = 50;                // bricks per update, <= cap
     // device state
     uint4 *mrpd = nullptr;
     uint32_t *request = nullptr, *usage = nullptr, *order = nullptr, *order_tmp = nullptr;
@@ -367,6 +368,7 @@ int Cache::create()
     if (g.mrpd_cells >= (1ull << 32)) return fail(GC_ERR_CONFIG, "page directory exceeds 2^32 cells");
     cap = cc.max_requests ? cc.max_requests : 50;
     cap = std::min(cap, g.nslots);
+    limit = cap;
     GC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     GC_HIP(hipStreamCreateWithFlags(&upload, hipStreamNonBlocking));
     GC_HIP(hipEventCreateWithFlags(&uploaded, hipEventDisableTiming));
@@ -433,11 +435,11 @@ int Cache::lru_update()
 int Cache::prepare_bricks(bool &done)
 {
     // prepare_bricks + create_request_list (CacheManager.hpp:410-491)
-    GC_HIP(launch_miss_compact(request, g.mrpd_cells, ts, d_geom, cap, d_misses, miss_s, stream));
+    GC_HIP(launch_miss_compact(request, g.mrpd_cells, ts, d_geom, limit, d_misses, miss_s, stream));
     GC_HIP(hipMemcpyAsync(h_misses, miss_s.total, 4, hipMemcpyDeviceToHost, stream));
     GC_HIP(hipMemcpyAsync(h_misses + 4, d_misses, (size_t)cap * 16, hipMemcpyDeviceToHost, stream));
     GC_HIP(hipStreamSynchronize(stream));
-    const uint32_t n = std::min(h_misses[0], cap);
+    const uint32_t n = std::min(h_misses[0], limit);
     if (n == 0) {
         done = true;
         return GC_OK;
@@ -722,6 +724,209 @@ static uint32_t packets_of(const gc_view &v)
     return ((v.width + tw - 1) / tw) * ((v.height + th - 1) / th);
 }
 
+
+
+// ------------------------------------------------------------------ batches
+// A set of view slots over one cache.  Each launch covers the rays the
+// previous launch stopped at a miss (any slot) and the tiles of the slots
+// that start in it.
+struct Batch {
+    Cache *C = nullptr;                 // borrowed: the handle's cache
+    gc_cache *h = nullptr;
+    uint32_t precision = 32, max_views = 0;
+    uint64_t max_pixels = 0;
+    uint32_t *pixels = nullptr;
+    float *tbuf = nullptr;
+    void *cstate = nullptr;
+    uint32_t *lists[2] = {nullptr, nullptr};
+    // [0..7] per-XCD packet cursors, [8], [9] ray-list counts, [10] timestamp,
+    // [16..16+max_views) rays stopped per slot in the last launch
+    uint32_t *ctl = nullptr;
+    uint32_t *h_stopped = nullptr;      // pinned copy of the per-slot counters
+    DvrView *d_views = nullptr, *h_views = nullptr;   // h_views pinned
+    gc_render_stats *d_stats = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, views_free = nullptr, launched = nullptr;
+    struct Slot {
+        bool used = false, starting = false;
+        DvrView v{};
+        uint64_t off = 0, npix = 0;
+    };
+    std::vector<Slot> slots;
+    uint32_t table = 0;                 // slots in the device table (highest used + 1)
+    uint32_t tile = 0, march = 4, refill_min = 16;
+    int cur = -1;                       // list holding the last launch's stopped rays (-1: none)
+    bool use_graph = false;
+    std::map<std::array<uint32_t, 8>, hipGraphExec_t> graphs;
+    size_t cbytes_per_pixel() const { return precision == 64 ? 32 : 16; }
+    uint64_t slot_pixels() const { return max_pixels / max_views; }
+    void drop_graphs()
+    {
+        for (auto &g : graphs) hipGraphExecDestroy(g.second);
+        graphs.clear();
+    }
+    ~Batch()
+    {
+        drop_graphs();
+        hipFree(pixels);
+        hipFree(tbuf);
+        hipFree(cstate);
+        hipFree(lists[0]);
+        hipFree(lists[1]);
+        hipFree(ctl);
+        hipFree(d_views);
+        hipFree(d_stats);
+        if (h_views) hipHostFree(h_views);
+        if (h_stopped) hipHostFree(h_stopped);
+        if (ev0) hipEventDestroy(ev0);
+        if (ev1) hipEventDestroy(ev1);
+        if (views_free) hipEventDestroy(views_free);
+        if (launched) hipEventDestroy(launched);
+    }
+};
+
+static int batch_place(Batch &B, uint32_t slot, const gc_view &v, const float *d_tf, uint32_t tf_size, uint64_t off)
+{
+    if (v.width == 0 || v.height == 0 || (uint64_t)v.width * v.height >= (1ull << DVR_PIXEL_BITS))
+        return fail(GC_ERR_ARG, "slot " + std::to_string(slot) + ": bad view size");
+    if (v.precision != B.precision) return fail(GC_ERR_ARG, "slot " + std::to_string(slot) + ": precision differs from the batch");
+    if (!d_tf || tf_size == 0) return fail(GC_ERR_ARG, "slot " + std::to_string(slot) + ": no transfer function");
+    Batch::Slot &S = B.slots[slot];
+    S.used = true;
+    S.starting = true;
+    S.off = off;
+    S.npix = (uint64_t)v.width * v.height;
+    DvrView &D = S.v;
+    D.width = v.width;
+    D.height = v.height;
+    std::memcpy(D.m, v.inv_mv, sizeof(D.m));
+    std::memcpy(D.bmin, v.bbox_min, sizeof(D.bmin));
+    std::memcpy(D.bmax, v.bbox_max, sizeof(D.bmax));
+    D.alpha_stop = v.alpha_stop;
+    D.tf_size = tf_size;
+    D.tf = reinterpret_cast<const float4 *>(d_tf);
+    D.pixels = B.pixels + off;
+    D.tbuf = B.tbuf + off;
+    D.cstate = static_cast<uint8_t *>(B.cstate) + off * B.cbytes_per_pixel();
+    B.table = std::max(B.table, slot + 1);
+    // a starting view begins from an empty progressive state
+    hipStream_t s = B.C->stream;
+    GC_HIP(launch_fill32(reinterpret_cast<uint32_t *>(D.tbuf), (uint32_t)S.npix, 0, s));
+    GC_HIP(launch_fill32(static_cast<uint32_t *>(D.cstate), (uint32_t)(S.npix * B.cbytes_per_pixel() / 4), 0, s));
+    return GC_OK;
+}
+
+// One launch: the previous launch's stopped rays, then the starting slots'
+// tiles.  No cache update.
+static int batch_launch(Batch &B, gc_frame_report *rep)
+{
+    Cache &C = *B.C;
+    hipStream_t s = C.stream;
+    uint32_t tw, th;
+    tile_shape(B.tile, tw, th);
+    // view table with this launch's tile ranges
+    GC_HIP(hipEventSynchronize(B.views_free));
+    uint32_t packets = 0;
+    for (uint32_t i = 0; i < B.table; ++i) {
+        Batch::Slot &S = B.slots[i];
+        DvrView &D = B.h_views[i];
+        D = S.v;
+        D.packet_off = packets;
+        if (!S.used) {
+            D.width = D.height = D.tiles_x = 0;
+            continue;
+        }
+        D.tiles_x = (D.width + tw - 1) / tw;
+        if (S.starting) packets += D.tiles_x * ((D.height + th - 1) / th);
+    }
+    const bool has_list = B.cur >= 0;
+    if (packets == 0 && !has_list) return GC_OK;
+    GC_HIP(hipMemcpyAsync(B.d_views, B.h_views, (size_t)B.table * sizeof(DvrView), hipMemcpyHostToDevice, s));
+    GC_HIP(hipEventRecord(B.views_free, s));
+    DvrLaunch L{};
+    launch_base(C, L);
+    L.views = B.d_views;
+    L.n_views = B.table;
+    L.n_packets = packets;
+    L.stats = B.d_stats;
+    L.queue = B.ctl;
+    L.refill_min = B.refill_min;
+    L.march = B.march;
+    L.view_stopped = B.ctl + 16;
+    const int out = has_list ? 1 - B.cur : 0;
+    if (has_list) {
+        L.list = B.lists[B.cur];
+        L.list_count = B.ctl + 8 + B.cur;
+    }
+    L.out_list = B.lists[out];
+    L.out_count = B.ctl + 8 + out;
+    uint64_t listed = 0;
+    for (uint32_t i = 0; i < B.table; ++i)
+        if (B.slots[i].used && !B.slots[i].starting) listed += B.slots[i].npix;
+    const uint32_t lds = dvr_lds_bytes(L.nb_lods, B.table);
+    const uint32_t grid = dvr_grid(C.vc.encoded_bytes, B.precision, B.tile, packets + (uint32_t)((listed + 63) / 64), lds);
+    GC_HIP(hipEventRecord(B.ev0, s));
+    if (B.use_graph) {
+        // the timestamp is read on the device so that a captured launch
+        // stays valid across updates
+        GC_HIP(launch_fill32(B.ctl + 10, 1, C.ts, s));
+        L.ts_dev = B.ctl + 10;
+        const std::array<uint32_t, 8> key{B.table, B.tile, B.march, B.refill_min, packets, grid,
+                                          has_list ? (uint32_t)B.cur : 2u, (uint32_t)out};
+        auto it = B.graphs.find(key);
+        if (it == B.graphs.end()) {
+            hipGraph_t g = nullptr;
+            GC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t e = launch_fill32(B.ctl, 8, 0, s);
+            if (e == hipSuccess) e = launch_fill32(B.ctl + 8 + out, 1, 0, s);
+            if (e == hipSuccess) e = launch_fill32(B.ctl + 16, B.max_views, 0, s);
+            if (e == hipSuccess) e = launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
+            const hipError_t e_end = hipStreamEndCapture(s, &g);
+            GC_HIP(e);
+            GC_HIP(e_end);
+            hipGraphExec_t x = nullptr;
+            const hipError_t e_inst = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            GC_HIP(e_inst);
+            it = B.graphs.emplace(key, x).first;
+        }
+        GC_HIP(hipGraphLaunch(it->second, s));
+    } else {
+        GC_HIP(launch_fill32(B.ctl, 8, 0, s));
+        GC_HIP(launch_fill32(B.ctl + 8 + out, 1, 0, s));
+        GC_HIP(launch_fill32(B.ctl + 16, B.max_views, 0, s));
+        GC_HIP(launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s));
+    }
+    GC_HIP(hipEventRecord(B.ev1, s));
+    GC_HIP(hipMemcpyAsync(B.h_stopped, B.ctl + 16, (size_t)B.max_views * 4, hipMemcpyDeviceToHost, s));
+    GC_HIP(hipEventRecord(B.launched, s));
+    B.cur = out;
+    for (uint32_t i = 0; i < B.table; ++i) B.slots[i].starting = false;
+    if (rep) ++rep->launches;
+    return GC_OK;
+}
+
+// A slot whose rays stopped at a miss in the last launch is still in that
+// launch's ray list and cannot be replaced.
+static int slot_busy(Batch &B, uint32_t slot, bool &busy)
+{
+    busy = false;
+    const Batch::Slot &S = B.slots[slot];
+    if (!S.used || S.starting || B.cur < 0) return GC_OK;
+    GC_HIP(hipEventSynchronize(B.launched));
+    busy = B.h_stopped[slot] != 0;
+    return GC_OK;
+}
+
+// Kernel time of the last launch; waits for it.
+static int batch_kernel_ms(Batch &B, double &ms)
+{
+    GC_HIP(hipEventSynchronize(B.ev1));
+    float f = 0.f;
+    GC_HIP(hipEventElapsedTime(&f, B.ev0, B.ev1));
+    ms = f;
+    return GC_OK;
+}
+
 } // namespace gc
 
 struct gc_cache {
@@ -729,6 +934,9 @@ struct gc_cache {
 };
 struct gc_frame {
     gc::Frame impl;
+};
+struct gc_batch {
+    gc::Batch impl;
 };
 
 using namespace gc;
@@ -1283,6 +1491,8 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
     L.stats = d_stats;
     L.queue = F.ctl;
     L.refill_min = view->refill_min ? view->refill_min : 16;
+    L.march = view->march ? view->march : 4;
+    if (L.march > 8) return fail(GC_ERR_ARG, "march must be 1..8");
     hipStream_t s = stream ? (hipStream_t)stream : C.stream;
     GC_HIP(launch_fill32(F.ctl, 8, 0, s));
     const uint32_t grid = dvr_grid(C.vc.encoded_bytes, view->precision, view->tile, L.n_packets,
@@ -1302,10 +1512,12 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
     launch_base(C, L);
     view_of(view, d_tf, tf_size, F, L.view0);
     L.n_views = 1;
-    L.n_packets = packets_of(view);
+    L.n_packets = first ? packets_of(view) : 0;
     L.stats = d_stats;
     L.queue = F.ctl;
     L.refill_min = view.refill_min ? view.refill_min : 16;
+    L.march = view.march ? view.march : 4;
+    if (L.march > 8) return fail(GC_ERR_ARG, "march must be 1..8");
     const int out = first ? 0 : 1 - cur;
     if (!first) {
         L.list = F.lists[cur];
@@ -1374,6 +1586,229 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
     }
     GC_HIP(hipMemcpyAsync(&report->stats, C.d_frame_stats, sizeof(gc_render_stats), hipMemcpyDeviceToHost, C.stream));
     GC_HIP(hipStreamSynchronize(C.stream));
+    return GC_OK;
+}
+
+int gc_cache_set_max_requests(gc_cache *c, uint32_t max_requests)
+{
+    if (!c || max_requests == 0) return fail(GC_ERR_ARG, "bad argument");
+    Cache &C = c->impl;
+    if (max_requests > C.cap)
+        return fail(GC_ERR_ARG, "max_requests above the capacity the cache was created with (" + std::to_string(C.cap) + ")");
+    C.limit = max_requests;
+    return GC_OK;
+}
+
+int gc_batch_create(gc_cache *c, uint32_t max_views, uint64_t max_pixels, uint32_t precision, gc_batch **out)
+{
+    if (!c || !out) return fail(GC_ERR_ARG, "null argument");
+    if (max_views == 0 || max_views > DVR_MAX_VIEWS) return fail(GC_ERR_ARG, "max_views must be 1..64");
+    if (precision != 32 && precision != 64) return fail(GC_ERR_ARG, "precision must be 32 or 64");
+    if (max_pixels == 0 || max_pixels * (precision == 64 ? 8 : 4) >= (1ull << 32))
+        return fail(GC_ERR_ARG, "max_pixels must be positive and below 2^29");
+    auto b = std::make_unique<gc_batch>();
+    Batch &B = b->impl;
+    B.C = &c->impl;
+    B.h = c;
+    B.precision = precision;
+    B.max_views = max_views;
+    B.max_pixels = max_pixels;
+    B.slots.resize(max_views);
+    GC_HIP(hipSetDevice(B.C->device));
+    GC_TRY(dalloc(&B.pixels, max_pixels * 4));
+    GC_TRY(dalloc(&B.tbuf, max_pixels * 4));
+    GC_TRY(dalloc(&B.cstate, max_pixels * B.cbytes_per_pixel()));
+    GC_TRY(dalloc(&B.lists[0], max_pixels * 4));
+    GC_TRY(dalloc(&B.lists[1], max_pixels * 4));
+    GC_TRY(dalloc(&B.ctl, (16 + DVR_MAX_VIEWS) * 4));
+    GC_TRY(dalloc(&B.d_views, (size_t)max_views * sizeof(DvrView)));
+    GC_TRY(dalloc(&B.d_stats, sizeof(gc_render_stats)));
+    GC_TRY(halloc(&B.h_views, (size_t)max_views * sizeof(DvrView)));
+    GC_TRY(halloc(&B.h_stopped, (size_t)DVR_MAX_VIEWS * 4));
+    std::memset(B.h_stopped, 0, (size_t)DVR_MAX_VIEWS * 4);
+    GC_HIP(hipEventCreate(&B.ev0));
+    GC_HIP(hipEventCreate(&B.ev1));
+    GC_HIP(hipEventCreateWithFlags(&B.views_free, hipEventDisableTiming));
+    GC_HIP(hipEventCreateWithFlags(&B.launched, hipEventDisableTiming));
+    GC_HIP(hipEventRecord(B.views_free, B.C->stream));
+    GC_HIP(hipEventRecord(B.launched, B.C->stream));
+    GC_HIP(hipMemset(B.ctl, 0, (16 + DVR_MAX_VIEWS) * 4));
+    GC_HIP(hipMemset(B.pixels, 0, max_pixels * 4));
+    GC_HIP(hipMemset(B.d_stats, 0, sizeof(gc_render_stats)));
+    *out = b.release();
+    return GC_OK;
+}
+
+int gc_batch_destroy(gc_batch *b)
+{
+    if (!b) return GC_OK;
+    hipSetDevice(b->impl.C->device);
+    hipStreamSynchronize(b->impl.C->stream);
+    delete b;
+    return GC_OK;
+}
+
+int gc_batch_set_graph(gc_batch *b, int enable)
+{
+    if (!b) return fail(GC_ERR_ARG, "null batch");
+    Batch &B = b->impl;
+    B.use_graph = enable != 0;
+    if (!B.use_graph) {
+        GC_HIP(hipStreamSynchronize(B.C->stream));
+        B.drop_graphs();
+    }
+    return GC_OK;
+}
+
+int gc_batch_set_launch(gc_batch *b, uint32_t tile, uint32_t march, uint32_t refill_min)
+{
+    if (!b) return fail(GC_ERR_ARG, "null batch");
+    if (tile > 3) return fail(GC_ERR_ARG, "tile must be 0..3");
+    if (march > 8) return fail(GC_ERR_ARG, "march must be 1..8");
+    if (refill_min > 64) return fail(GC_ERR_ARG, "refill_min must be 0..64");
+    Batch &B = b->impl;
+    B.tile = tile;
+    B.march = march ? march : 4;
+    B.refill_min = refill_min ? refill_min : 16;
+    return GC_OK;
+}
+
+int gc_batch_begin(gc_batch *b, const gc_view *views, const float *const *d_tfs, const uint32_t *tf_sizes,
+                   uint32_t n_views)
+{
+    if (!b || !views || !d_tfs || !tf_sizes) return fail(GC_ERR_ARG, "null argument");
+    Batch &B = b->impl;
+    if (n_views == 0 || n_views > B.max_views) return fail(GC_ERR_ARG, "view count outside 1..max_views");
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n_views; ++i) total += (uint64_t)views[i].width * views[i].height;
+    if (total > B.max_pixels) return fail(GC_ERR_ARG, "views exceed the batch's max_pixels");
+    GC_TRY(gc_batch_set_launch(b, views[0].tile, views[0].march, views[0].refill_min));
+    GC_HIP(hipSetDevice(B.C->device));
+    for (auto &S : B.slots) S = Batch::Slot{};
+    B.table = 0;
+    B.cur = -1;
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < n_views; ++i) {
+        GC_TRY(batch_place(B, i, views[i], d_tfs[i], tf_sizes[i], off));
+        off += B.slots[i].npix;
+    }
+    GC_HIP(launch_fill32(reinterpret_cast<uint32_t *>(B.d_stats), sizeof(gc_render_stats) / 4, 0, B.C->stream));
+    return GC_OK;
+}
+
+int gc_batch_set_view(gc_batch *b, uint32_t slot, const gc_view *view, const float *d_tf, uint32_t tf_size)
+{
+    if (!b || !view) return fail(GC_ERR_ARG, "null argument");
+    Batch &B = b->impl;
+    if (slot >= B.max_views) return fail(GC_ERR_ARG, "no such slot");
+    if ((uint64_t)view->width * view->height > B.slot_pixels())
+        return fail(GC_ERR_ARG, "view larger than a slot (max_pixels / max_views)");
+    GC_HIP(hipSetDevice(B.C->device));
+    bool busy = false;
+    GC_TRY(slot_busy(B, slot, busy));
+    if (busy) return fail(GC_ERR_STATE, "slot still has rays waiting for bricks");
+    return batch_place(B, slot, *view, d_tf, tf_size, (uint64_t)slot * B.slot_pixels());
+}
+
+int gc_batch_clear_view(gc_batch *b, uint32_t slot)
+{
+    if (!b) return fail(GC_ERR_ARG, "null batch");
+    Batch &B = b->impl;
+    if (slot >= B.max_views) return fail(GC_ERR_ARG, "no such slot");
+    bool busy = false;
+    GC_TRY(slot_busy(B, slot, busy));
+    if (busy) return fail(GC_ERR_STATE, "slot still has rays waiting for bricks");
+    B.slots[slot].used = false;
+    B.slots[slot].starting = false;
+    while (B.table > 0 && !B.slots[B.table - 1].used) --B.table;
+    return GC_OK;
+}
+
+int gc_batch_launch(gc_batch *b, double *kernel_ms)
+{
+    if (!b) return fail(GC_ERR_ARG, "null batch");
+    Batch &B = b->impl;
+    GC_HIP(hipSetDevice(B.C->device));
+    GC_TRY(batch_launch(B, nullptr));
+    if (kernel_ms) GC_TRY(batch_kernel_ms(B, *kernel_ms));
+    return GC_OK;
+}
+
+int gc_batch_pending(gc_batch *b, uint32_t *stopped)
+{
+    if (!b || !stopped) return fail(GC_ERR_ARG, "null argument");
+    Batch &B = b->impl;
+    GC_HIP(hipEventSynchronize(B.launched));
+    std::memcpy(stopped, B.h_stopped, (size_t)B.max_views * 4);
+    for (uint32_t i = 0; i < B.max_views; ++i)
+        if (i >= B.table || !B.slots[i].used) stopped[i] = 0;
+        else if (B.slots[i].starting) stopped[i] = 0xffffffffu;   // not launched yet
+    return GC_OK;
+}
+
+int gc_batch_stats(gc_batch *b, gc_render_stats *out)
+{
+    if (!b || !out) return fail(GC_ERR_ARG, "null argument");
+    Batch &B = b->impl;
+    GC_HIP(hipMemcpyAsync(out, B.d_stats, sizeof(gc_render_stats), hipMemcpyDeviceToHost, B.C->stream));
+    GC_HIP(hipStreamSynchronize(B.C->stream));
+    return GC_OK;
+}
+
+int gc_batch_render(gc_batch *b, const gc_view *views, const float *const *d_tfs, const uint32_t *tf_sizes,
+                    uint32_t n_views, uint32_t max_launches, gc_frame_report *report)
+{
+    if (!b || !report) return fail(GC_ERR_ARG, "null argument");
+    GC_TRY(gc_batch_begin(b, views, d_tfs, tf_sizes, n_views));
+    Batch &B = b->impl;
+    Cache &C = *B.C;
+    std::memset(report, 0, sizeof(*report));
+    for (uint32_t it = 0; it < max_launches; ++it) {
+        GC_TRY(batch_launch(B, report));
+        int done = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        GC_TRY(C.update(&done));
+        ++report->updates;
+        double ms = 0.0;
+        GC_TRY(batch_kernel_ms(B, ms));
+        report->kernel_ms += ms;
+        if (done) {
+            report->update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            report->complete = 1;
+            break;
+        }
+        GC_TRY(gc_cache_wait_idle(B.h));
+        const uint32_t before = C.handler.nne;
+        GC_TRY(C.update(&done));
+        ++report->updates;
+        report->bricks_loaded += before;
+        report->update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    GC_TRY(gc_batch_stats(b, &report->stats));
+    return GC_OK;
+}
+
+int gc_batch_read_pixels(gc_batch *b, uint32_t slot, uint32_t *host)
+{
+    if (!b || !host) return fail(GC_ERR_ARG, "null argument");
+    Batch &B = b->impl;
+    if (slot >= B.max_views || !B.slots[slot].used) return fail(GC_ERR_ARG, "no view in this slot");
+    GC_HIP(hipSetDevice(B.C->device));
+    GC_HIP(hipStreamSynchronize(B.C->stream));
+    GC_HIP(hipMemcpy(host, B.slots[slot].v.pixels, B.slots[slot].npix * 4, hipMemcpyDeviceToHost));
+    return GC_OK;
+}
+
+int gc_batch_read_state(gc_batch *b, uint32_t slot, float *tbuf, void *cstate)
+{
+    if (!b) return fail(GC_ERR_ARG, "null argument");
+    Batch &B = b->impl;
+    if (slot >= B.max_views || !B.slots[slot].used) return fail(GC_ERR_ARG, "no view in this slot");
+    GC_HIP(hipSetDevice(B.C->device));
+    GC_HIP(hipStreamSynchronize(B.C->stream));
+    const Batch::Slot &S = B.slots[slot];
+    if (tbuf) GC_HIP(hipMemcpy(tbuf, S.v.tbuf, S.npix * 4, hipMemcpyDeviceToHost));
+    if (cstate) GC_HIP(hipMemcpy(cstate, S.v.cstate, S.npix * B.cbytes_per_pixel(), hipMemcpyDeviceToHost));
     return GC_OK;
 }
 

@@ -66,7 +66,7 @@ class Geometry(C.Structure):
 class View(C.Structure):
     _fields_ = [
         ("width", u32), ("height", u32), ("inv_mv", f32 * 12), ("bbox_min", f32 * 3), ("bbox_max", f32 * 3),
-        ("alpha_stop", f32), ("precision", u32), ("tile", u32), ("refill_min", u32),
+        ("alpha_stop", f32), ("precision", u32), ("tile", u32), ("refill_min", u32), ("march", u32),
     ]
 
 
@@ -110,6 +110,9 @@ EXPORTS = [
     "gc_cache_read_lru", "gc_cache_read_pool", "gc_cache_read_asked", "gc_frame_create", "gc_frame_destroy",
     "gc_frame_reset", "gc_frame_read_pixels", "gc_frame_read_state", "gc_frame_buffers", "gc_render_dvr",
     "gc_tf_samples", "gc_tf_seeded", "gc_camera_matrix", "gc_cache_wait_idle", "gc_render_frame",
+    "gc_batch_create", "gc_batch_destroy", "gc_batch_set_graph", "gc_batch_render", "gc_batch_read_pixels",
+    "gc_batch_set_launch", "gc_batch_begin", "gc_batch_set_view", "gc_batch_clear_view", "gc_batch_launch",
+    "gc_batch_pending", "gc_batch_stats", "gc_batch_read_state", "gc_cache_set_max_requests",
 ]
 
 _lib = None
@@ -160,6 +163,15 @@ def lib():
             "gc_tf_samples": [P, u32, u32, P], "gc_tf_seeded": [u64, u32, P],
             "gc_camera_matrix": [f32, f32, P, P], "gc_cache_wait_idle": [P],
             "gc_render_frame": [P, C.POINTER(View), P, u32, P, u32, C.POINTER(FrameReport)],
+            "gc_batch_create": [P, u32, u64, u32, C.POINTER(vp)], "gc_batch_destroy": [P],
+            "gc_batch_set_graph": [P, C.c_int],
+            "gc_batch_render": [P, C.POINTER(View), C.POINTER(vp), P, u32, u32, C.POINTER(FrameReport)],
+            "gc_batch_read_pixels": [P, u32, P], "gc_cache_set_max_requests": [P, u32],
+            "gc_batch_set_launch": [P, u32, u32, u32],
+            "gc_batch_begin": [P, C.POINTER(View), C.POINTER(vp), P, u32],
+            "gc_batch_set_view": [P, u32, C.POINTER(View), P, u32], "gc_batch_clear_view": [P, u32],
+            "gc_batch_launch": [P, C.POINTER(C.c_double)], "gc_batch_pending": [P, P],
+            "gc_batch_stats": [P, C.POINTER(RenderStats)], "gc_batch_read_state": [P, u32, P, P],
         }
         missing = [n for n in EXPORTS if n not in sig]
         if missing:
@@ -463,7 +475,7 @@ class Frame:
 
 
 def make_view(width, height, inv_mv, alpha_stop=0.95, precision=32, tile=0,
-              bbox=((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)), refill_min=0):
+              bbox=((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)), refill_min=0, march=0):
     v = View()
     v.width, v.height = width, height
     v.inv_mv[:] = [float(x) for x in np.asarray(inv_mv, np.float32).reshape(12)]
@@ -473,6 +485,7 @@ def make_view(width, height, inv_mv, alpha_stop=0.95, precision=32, tile=0,
     v.precision = precision
     v.tile = tile
     v.refill_min = refill_min
+    v.march = march
     return v
 
 
@@ -485,6 +498,92 @@ def render_frame(cache, view, d_tf, tf_size, frame, max_launches=100000):
     rep = FrameReport()
     check(lib().gc_render_frame(cache.h, C.byref(view), d_tf.p, tf_size, frame.h, max_launches, C.byref(rep)))
     return rep
+
+
+class Batch:
+    """View slots of one cache rendered together (gc_batch_*)."""
+
+    def __init__(self, cache, max_views, max_pixels, precision=32):
+        self.h = vp()
+        self.cache = cache
+        self.max_views = max_views
+        self.precision = precision
+        check(lib().gc_batch_create(cache.h, max_views, max_pixels, precision, C.byref(self.h)))
+        self.sizes = {}
+
+    def set_graph(self, enable):
+        check(lib().gc_batch_set_graph(self.h, 1 if enable else 0))
+
+    def set_launch(self, tile=0, march=0, refill_min=0):
+        check(lib().gc_batch_set_launch(self.h, tile, march, refill_min))
+
+    @staticmethod
+    def _arrays(views, d_tfs, tf_sizes):
+        n = len(views)
+        return n, (View * n)(*views), (vp * n)(*[t.p for t in d_tfs]), np.asarray(tf_sizes, np.uint32)
+
+    def begin(self, views, d_tfs, tf_sizes):
+        n, va, ta, sa = self._arrays(views, d_tfs, tf_sizes)
+        check(lib().gc_batch_begin(self.h, va, ta, ptr(sa), n))
+        self.sizes = {i: (v.width, v.height) for i, v in enumerate(views)}
+
+    def render(self, views, d_tfs, tf_sizes, max_launches=100000):
+        n, va, ta, sa = self._arrays(views, d_tfs, tf_sizes)
+        rep = FrameReport()
+        check(lib().gc_batch_render(self.h, va, ta, ptr(sa), n, max_launches, C.byref(rep)))
+        self.sizes = {i: (v.width, v.height) for i, v in enumerate(views)}
+        return rep
+
+    def set_view(self, slot, view, d_tf, tf_size):
+        check(lib().gc_batch_set_view(self.h, slot, C.byref(view), d_tf.p, tf_size))
+        self.sizes[slot] = (view.width, view.height)
+
+    def clear_view(self, slot):
+        check(lib().gc_batch_clear_view(self.h, slot))
+        self.sizes.pop(slot, None)
+
+    def launch(self):
+        ms = C.c_double(0)
+        check(lib().gc_batch_launch(self.h, C.byref(ms)))
+        return ms.value
+
+    def pending(self):
+        out = np.zeros(self.max_views, np.uint32)
+        check(lib().gc_batch_pending(self.h, ptr(out)))
+        return out
+
+    def stats(self):
+        st = RenderStats()
+        check(lib().gc_batch_stats(self.h, C.byref(st)))
+        return st
+
+    def pixels(self, i):
+        w, h = self.sizes[i]
+        out = np.zeros(w * h, np.uint32)
+        check(lib().gc_batch_read_pixels(self.h, i, ptr(out)))
+        return out
+
+    def state(self, i):
+        w, h = self.sizes[i]
+        tb = np.zeros(w * h, np.float32)
+        cs = np.zeros(w * h * 4, np.float64 if self.precision == 64 else np.float32)
+        check(lib().gc_batch_read_state(self.h, i, ptr(tb), ptr(cs)))
+        return tb, cs
+
+    def close(self):
+        if self.h:
+            lib().gc_batch_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_max_requests(cache, n):
+    check(lib().gc_cache_set_max_requests(cache.h, n))
 
 
 def unpack_rgba(px):

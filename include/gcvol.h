@@ -248,6 +248,8 @@ typedef struct gc_view {
     uint32_t tile;               /* 64-ray packet: 0 8x8, 1 16x4, 2 32x2, 3 64x1 */
     uint32_t refill_min;         /* refill a wave's idle lanes once this many are
                                     idle (0: default 16)                      */
+    uint32_t march;              /* samples per lane between refill checks,
+                                    1..8 (0: default 4)                       */
 } gc_view;
 
 /* Per-view progressive state (tBuffer, colorStateBuffer, pixel buffer) on the
@@ -298,6 +300,58 @@ typedef struct gc_frame_report {
  * next launch instead of re-launching on unchanged bricks. */
 int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t tf_size,
                     gc_frame *f, uint32_t max_launches, gc_frame_report *report);
+
+/* ----------------------------------------------------------------- batches */
+
+/* A set of up to 64 view slots over one cache, rendered together: each launch
+ * covers the rays the previous launch stopped at a miss (any slot) and the
+ * tiles of the slots that start in it, so small views share the GPU instead of
+ * each leaving it mostly idle, and views can join while others are still
+ * waiting for bricks.  No reference counterpart (the reference renders one
+ * view per launch, VolumeRayCaster.cu:261-330).  Every view has its own size,
+ * camera, bounding box, alpha stop and transfer function (device pointer to
+ * tf_size float4); precision is the batch's.  max_pixels is the pixel
+ * capacity: gc_batch_begin packs its views in it, gc_batch_set_view gives
+ * slot i the region [i, i+1) * max_pixels / max_views. */
+typedef struct gc_batch gc_batch;
+int gc_batch_create(gc_cache *c, uint32_t max_views, uint64_t max_pixels, uint32_t precision,
+                    gc_batch **out);
+int gc_batch_destroy(gc_batch *b);
+/* Launch shape: tile 0..3 (8x8, 16x4, 32x2, 64x1), march 1..8 samples per
+ * lane between refill checks (0: 4), refill_min (0: 16). */
+int gc_batch_set_launch(gc_batch *b, uint32_t tile, uint32_t march, uint32_t refill_min);
+/* 1: launches are replays of captured hipGraphs (counter fills + the ray
+ * caster), one per launch shape (slot count, tile, march, packet count, ...). */
+int gc_batch_set_graph(gc_batch *b, int enable);
+/* Replace all slots with views 0..n-1 (packed), all starting; the launch
+ * shape is taken from views[0]. */
+int gc_batch_begin(gc_batch *b, const gc_view *views, const float *const *d_tfs,
+                   const uint32_t *tf_sizes, uint32_t n_views);
+/* Put a view in one slot (starting in the next launch).  GC_ERR_STATE if the
+ * slot's previous view still has rays waiting for bricks. */
+int gc_batch_set_view(gc_batch *b, uint32_t slot, const gc_view *view, const float *d_tf,
+                      uint32_t tf_size);
+int gc_batch_clear_view(gc_batch *b, uint32_t slot);
+/* One launch over the batch, without a cache update (the caller updates, as
+ * in the reference's loop).  kernel_ms (optional) waits for the launch and
+ * returns its time. */
+int gc_batch_launch(gc_batch *b, double *kernel_ms);
+/* Per slot (max_views entries): rays of the last launch that stopped at a
+ * miss -- 0 for a complete view, 0xffffffff for a view not launched yet. */
+int gc_batch_pending(gc_batch *b, uint32_t *stopped);
+/* Render counters summed since gc_batch_begin / gc_batch_create. */
+int gc_batch_stats(gc_batch *b, gc_render_stats *out);
+/* gc_batch_begin, then launch + update until no view has a ray waiting (the
+ * progressive loop of gc_render_frame over the whole batch). */
+int gc_batch_render(gc_batch *b, const gc_view *views, const float *const *d_tfs,
+                    const uint32_t *tf_sizes, uint32_t n_views, uint32_t max_launches,
+                    gc_frame_report *report);
+int gc_batch_read_pixels(gc_batch *b, uint32_t slot, uint32_t *host);   /* W*H ABGR */
+int gc_batch_read_state(gc_batch *b, uint32_t slot, float *tbuf, void *cstate);
+
+/* Bricks per update (the upload batch), at most the max_requests the cache
+ * was created with. */
+int gc_cache_set_max_requests(gc_cache *c, uint32_t max_requests);
 
 /* ------------------------------------------------------------------ helpers */
 

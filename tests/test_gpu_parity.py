@@ -106,6 +106,7 @@ def test_resident_load_and_render(gpu, bytes_):
             if prec == 64:
                 assert gst.tolist() == ost.tolist()
                 assert err.max() <= 1
+                assert np.array_equal(frame.state()[0].reshape(-1), otb)
                 assert np.array_equal(gc.usage(), oc.usage())
                 assert np.array_equal(gc.request(), oc.request())
             else:
@@ -135,6 +136,7 @@ def test_out_of_core_sequence(gpu, bytes_, tile):
             gst, g, ost, o = _render_pair(sc, gc, oc, cam, 64, W, H, tf, d_tf, frame, otb, ocs, tile=tile)
             assert gst.tolist() == ost.tolist(), "render counters differ at view %d step %d" % (k, it)
             assert np.abs(g - o).max() <= 1
+            assert np.array_equal(frame.state()[0].reshape(-1), otb), "t buffer differs"   # FP32 ray geometry
             gd, od = gc.update(), oc.update()
             assert gd == od
             _compare_state(gc, oc, pool=(it % 5 == 0))
