@@ -14,13 +14,22 @@ A step is one frame: the plain single-view renderer's progressive loop
 view is complete.  `value` is ray samples per second over the timed frames,
 wall clock, including the on-demand brick synthesis and upload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-traffic]
+
+roofline.traffic is the ray caster's HBM bytes per launch, measured before
+the timed run by two short child runs of this benchmark under
+`rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`, each scaled by what the
+counter reads for a device-to-device copy of known size in the same run.
 """
 import argparse
 import json
 import math
 import os
+import shutil
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -120,6 +129,66 @@ def render_rows(O, oc, view, tb, cs, px, y0, h, threads):
     return st
 
 
+CALIB_MIB = 256
+
+
+def _pmc_pass(counter, outdir, args):
+    """One child run of this benchmark under rocprofv3 collecting `counter`;
+    returns (per-k_dvr-dispatch values in KiB after the first frame, copy
+    kernel value in KiB)."""
+    cmd = ["rocprofv3", "--pmc", counter, "-d", outdir, "-o", counter, "--", sys.executable,
+           os.path.abspath(__file__), "--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-traffic",
+           "--calib-copy-mib", str(CALIB_MIB), "--max-requests", str(args.max_requests), "--tile", str(args.tile),
+           "--precision", str(args.precision)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError("rocprofv3 %s pass failed (%d): %s" % (counter, r.returncode, r.stderr[-400:]))
+    first_launches = None
+    for line in r.stderr.splitlines():
+        if line.startswith("frame   0 "):
+            first_launches = int(line.split("launches")[1].split()[0])
+    dbs = [os.path.join(dp, f) for dp, _, fs in os.walk(outdir) for f in fs if f.endswith(".db") and counter in f]
+    if not dbs or first_launches is None:
+        raise RuntimeError("rocprofv3 %s pass left no results" % counter)
+    con = sqlite3.connect(dbs[0])
+    rows = con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection where counter_name = ? "
+                       "group by dispatch_id order by dispatch_id", (counter,)).fetchall()
+    con.close()
+    dvr = [v for _, k, v in rows if "k_dvr" in k]
+    copy = [v for _, k, v in rows if "copy" in k.lower()]
+    if len(dvr) <= first_launches or not copy:
+        raise RuntimeError("rocprofv3 %s pass: no ray-caster or copy dispatches" % counter)
+    return dvr[first_launches:], max(copy)
+
+
+def measure_traffic(args):
+    """HBM bytes per ray-caster launch from FETCH_SIZE + WRITE_SIZE (KiB per
+    dispatch), each scaled by (bytes the calibration copy moves) / (what the
+    counter reads for it)."""
+    if not shutil.which("rocprofv3"):
+        return None, "rocprofv3 not found"
+    tmp = tempfile.mkdtemp(prefix="gcvol_pmc_", dir="/tmp")
+    try:
+        out = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            t0 = time.perf_counter()
+            vals, copy_kib = _pmc_pass(counter, os.path.join(tmp, counter), args)
+            scale = (CALIB_MIB * 1024.0) / copy_kib
+            print("traffic pass %s: %d launches, copy %.0f KiB, %.1f s" % (counter, len(vals), copy_kib,
+                                                                           time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
+            out[counter] = (sum(vals) / len(vals) * scale * 1024.0, scale, len(vals))
+        total = out["FETCH_SIZE"][0] + out["WRITE_SIZE"][0]
+        note = ("FETCH_SIZE x%.3f + WRITE_SIZE x%.3f (copy calibration, %d MiB), mean over %d launches of frames 1-3"
+                % (out["FETCH_SIZE"][1], out["WRITE_SIZE"][1], CALIB_MIB, out["FETCH_SIZE"][2]))
+        return total, note
+    except (RuntimeError, subprocess.TimeoutExpired, sqlite3.Error) as e:
+        return None, str(e)[:300]
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +198,7 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known traffic "
@@ -137,6 +207,8 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    traffic, traffic_note = (None, "not measured") if args.no_traffic else measure_traffic(args)
 
     import gcvol as G
 
@@ -209,7 +281,7 @@ def main():
                         "32^3 bricks, covering 1, 7 levels, 4 GiB pool (%dx%dx%d slots), 1024x1024, 120-frame "
                         "fly-through, step 0.5 voxel, alpha stop 0.99, %d requests/update" % (cs + (args.max_requests,)),
             "precision": "fp%d compositing" % args.precision,
-            "tile": ["8x8", "16x8", "32x4", "64x1"][args.tile],
+            "tile": ["8x8", "16x4", "32x2", "64x1"][args.tile],
         },
         "frames_per_s": len(lat) / total_s if total_s > 0 else 0.0,
         "p50_ms": percentile(lat, 50),
@@ -223,7 +295,8 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_note": traffic_note,
             "kernel": "k_dvr<2,%s,%s>" % ("double" if args.precision == 64 else "float", ["8,8", "16,4", "32,2", "64,1"][args.tile]),
             "avg_launch_ms": kms / launches if launches else 0.0,
             "algorithmic_bytes_per_launch": B / launches if launches else 0.0,
