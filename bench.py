@@ -131,13 +131,99 @@ def render_rows(O, oc, view, tb, cs, px, y0, h, threads):
 
 CALIB_MIB = 256
 
+# batching workload: 64 uneven views of one 1024^3 u8 volume
+BATCH_DIMS = (1024, 1024, 1024)
+BATCH_BRICK = 16
+BATCH_VIEWS = 64
+BATCH_SIZES = [(256, 256), (640, 480), (1024, 1024), (1920, 1080)]
+
+
+def batch_workload(G, reps=3):
+    """Batched vs plain rendering of 64 concurrent views (sizes uniform among
+    256^2, 640x480, 1024^2, 1920x1080, seeded cameras, 8 seeded 256-entry
+    transfer functions) over one 1024^3 u8 blob volume synthesised on demand
+    (16^3 bricks, covering 1, 4 GiB pool).  The batch is first rendered cold
+    (loading its bricks), then the autotuner picks tile x samples-per-check
+    on the warm batch, then: batched = one gc_batch_render per batch (hipGraph
+    replay), plain = gc_render_frame per view, one view per launch, in request
+    order.  All 64 requests arrive together, so a view's latency is the time
+    until its own frame is ready."""
+    import autotune
+
+    rng = np.random.default_rng(SEED + 3)
+    vc = G.volume_cfg(BATCH_DIMS, BATCH_BRICK, 1, 1)
+    params = G.synth_params(SEED + 3, 32, sigma=(10.0, 60.0), amp=(60.0, 255.0), noise=8, threads=0)
+    src = G.Source.synthetic(vc, params)
+    cs = G.pool_shape(4 << 30, BATCH_BRICK ** 3)
+    cache = G.Cache(vc, G.cache_cfg(cs, BATCH_BRICK, max_requests=16384, sync=False), src)
+    tables = [G.tf_samples(G.tf_seeded(SEED + 100 + i, 6), 256) for i in range(8)]
+    d_tfs = [G.DeviceBuffer.from_array(t) for t in tables]
+    views, vt = [], []
+    for i in range(BATCH_VIEWS):
+        w, h = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+        cam = G.camera(float(rng.uniform(-40.0, 40.0)), float(rng.uniform(0.0, 360.0)), float(rng.uniform(2.2, 3.6)))
+        views.append(G.make_view(w, h, cam, alpha_stop=ALPHA_STOP, precision=32))
+        vt.append(d_tfs[i % len(d_tfs)])
+    sizes = [256] * BATCH_VIEWS
+    npix = sum(v.width * v.height for v in views)
+    b = G.Batch(cache, BATCH_VIEWS, npix, 32)
+    t0 = time.perf_counter()
+    cold = b.render(views, vt, sizes)
+    cold_ms = (time.perf_counter() - t0) * 1e3
+    print("batch: cold render %.0f ms, %d launches, %d bricks" % (cold_ms, cold.launches, cold.bricks_loaded),
+          file=sys.stderr, flush=True)
+    tune = autotune.tune_launch(cache, views, vt, sizes, marches=(1, 2, 4), reps=2)
+    print("batch: autotuned tile %d march %d" % (tune["tile"], tune["march"]), file=sys.stderr, flush=True)
+    for v in views:
+        v.tile, v.march = tune["tile"], tune["march"]
+    b.set_graph(True)
+    b.render(views, vt, sizes)                      # capture
+    bt, bs = [], 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = b.render(views, vt, sizes)
+        bt.append((time.perf_counter() - t0) * 1e3)
+        bs = r.stats.mapped_samples
+    b.close()
+    frames = [G.Frame(v.width, v.height, 32) for v in views]
+    pt, plat, ps = [], [], 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        lat = []
+        ps = 0
+        for v, d, n, f in zip(views, vt, sizes, frames):
+            f.reset()
+            r = G.render_frame(cache, v, d, n, f)
+            ps += r.stats.mapped_samples
+            lat.append((time.perf_counter() - t0) * 1e3)
+        pt.append(lat[-1])
+        plat.extend(lat)
+    for f in frames:
+        f.close()
+    cache.close()
+    bmed, pmed = float(np.median(bt)), float(np.median(pt))
+    return {
+        "workload": "64 uneven views (256^2/640x480/1024^2/1920x1080), 8 TFs, 1024^3 u8 blobs, 16^3 bricks, "
+                    "4 GiB pool, warm; batched = one hipGraph-replayed launch per batch, plain = one view per launch",
+        "views": BATCH_VIEWS, "pixels": npix,
+        "batched_frames_per_s": BATCH_VIEWS / (bmed / 1e3), "plain_frames_per_s": BATCH_VIEWS / (pmed / 1e3),
+        "speedup": pmed / bmed,
+        "batched_samples_per_s": bs / (bmed / 1e3), "plain_samples_per_s": ps / (pmed / 1e3),
+        "batched_p50_ms": bmed, "batched_p99_ms": percentile(bt, 99),
+        "plain_p50_ms": percentile(plat, 50), "plain_p99_ms": percentile(plat, 99),
+        "tile": ["8x8", "16x4", "32x2", "64x1"][tune["tile"]], "march": tune["march"],
+        "autotune": [{"tile": t["tile"], "march": t["march"], "kernel_ms": round(t["kernel_ms"], 2)}
+                     for t in tune["table"]],
+        "cold_ms": cold_ms,
+    }
+
 
 def _pmc_pass(counter, outdir, args):
     """One child run of this benchmark under rocprofv3 collecting `counter`;
     returns (per-k_dvr-dispatch values in KiB after the first frame, copy
     kernel value in KiB)."""
     cmd = ["rocprofv3", "--pmc", counter, "-d", outdir, "-o", counter, "--", sys.executable,
-           os.path.abspath(__file__), "--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-traffic",
+           os.path.abspath(__file__), "--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-traffic", "--no-batch",
            "--calib-copy-mib", str(CALIB_MIB), "--max-requests", str(args.max_requests), "--tile", str(args.tile),
            "--precision", str(args.precision)]
     env = dict(os.environ, TMPDIR="/tmp")
@@ -199,6 +285,7 @@ def main():
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain workload")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known traffic "
@@ -305,6 +392,9 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(G, cache, vc, cams[-1], tf, seconds=args.cpu_seconds)
     cache.close()
+    frame.close()
+    if not args.no_batch:
+        out["batched_vs_plain"] = batch_workload(G)
     print(json.dumps(out))
 
 
