@@ -25,6 +25,13 @@ int fail(int code, const std::string &msg);
                                               hipGetErrorString(e_));               \
     } while (0)
 
+// propagate a non-GC_OK status
+#define GC_TRY(expr)                          \
+    do {                                      \
+        int rc_ = (expr);                     \
+        if (rc_ != GC_OK) return rc_;         \
+    } while (0)
+
 // ------------------------------------------------- FP32 bookkeeping formulas
 // Evaluated identically on host and device (the library is compiled with
 // -ffp-contract=off and IEEE-rounded divide/sqrt).

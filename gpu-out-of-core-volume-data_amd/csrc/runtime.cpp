@@ -41,12 +41,6 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
-#define GC_TRY(expr)                          \
-    do {                                      \
-        int rc_ = (expr);                     \
-        if (rc_ != GC_OK) return rc_;         \
-    } while (0)
-
 static void parallel_for(uint32_t n, uint32_t threads, const std::function<void(uint32_t)> &fn)
 {
     if (threads <= 1 || n <= 1) {
@@ -1531,6 +1525,13 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
                                    first ? L.n_packets : (F.w * F.h + 63) / 64, dvr_lds_bytes(L.nb_lods, 1));
     GC_HIP(launch_dvr(L, C.vc.encoded_bytes, view.precision, view.tile, grid, s));
     cur = out;
+    return GC_OK;
+}
+
+int gc_cache_busy(gc_cache *c, int *busy)
+{
+    if (!c || !busy) return fail(GC_ERR_ARG, "null argument");
+    *busy = c->impl.handler.new_request.load() ? 1 : 0;
     return GC_OK;
 }
 

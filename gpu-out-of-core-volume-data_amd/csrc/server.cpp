@@ -1,0 +1,447 @@
+// server.cpp -- the serving loop (gc_server_*): continuous batching of view
+// requests over several resident volumes, admission by brick-pool memory.
+//
+// One scheduler thread owns every device call.  Each iteration it
+//   1. admits queued requests: a volume's cache (page directory + pool) and
+//      view batch are created when its first request arrives and the pools
+//      of the resident volumes plus its own fit the budget, evicting volumes
+//      with nothing in flight (least recently used first) to make room; free
+//      slots of a resident volume take its queued requests in arrival order;
+//   2. launches every resident volume that has rays to trace (one launch per
+//      volume on the volume's own stream, so volumes overlap on the GPU);
+//   3. updates each launched volume's cache (CacheManager::update), reads the
+//      per-slot miss counters and retires the views with no ray left waiting
+//      (pixels copied to the host, latency recorded).
+// A volume whose rays all wait for the request handler is not relaunched
+// until the handler is done; the update that then loads the bricks runs
+// before the next launch.  The bricks and the LRU of each volume evolve by
+// the reference's update rules; only the interleaving of views differs.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "gc_internal.hpp"
+
+namespace gc {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t id = 0;
+    uint32_t volume = 0;
+    gc_view view{};
+    void *d_tf = nullptr;
+    uint32_t tf_size = 0;
+    double t_submit = 0.0, t_done = 0.0;
+    uint32_t launches = 0;
+    std::vector<uint32_t> pixels;
+};
+
+struct Volume {
+    gc_volume_cfg vc{};
+    gc_source *src = nullptr;
+    uint64_t pool_bytes = 0, footprint = 0;
+    uint32_t max_requests = 0;
+    gc_cache *cache = nullptr;
+    gc_batch *batch = nullptr;
+    std::deque<Request *> queue;
+    std::vector<Request *> slots;
+    std::vector<bool> fresh;            // slot placed, not launched yet
+    bool load_pending = false;
+    uint64_t last_use = 0, samples_seen = 0;
+    bool resident() const { return cache != nullptr; }
+    bool busy_slots() const
+    {
+        for (auto *r : slots)
+            if (r) return true;
+        return false;
+    }
+};
+
+} // namespace
+
+struct Server {
+    gc_server_cfg cfg{};
+    int device = 0;
+    std::vector<Volume> vols;
+    std::mutex m;
+    std::condition_variable cv, done_cv;
+    std::thread worker;
+    bool stop = false;
+    std::string error;
+    std::map<uint64_t, Request *> reqs;
+    std::deque<Request *> finished;     // completed, not yet polled
+    uint64_t next_id = 1, in_flight = 0, tick = 0;
+    uint64_t resident_bytes = 0;
+    gc_server_stats st{};
+    Clock::time_point t0 = Clock::now();
+
+    double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+    uint64_t slot_bytes() const
+    {
+        // pixels + t + colour state + two ray-list entries per pixel
+        return (uint64_t)cfg.max_views * cfg.slot_pixels * (4 + 4 + (cfg.precision == 64 ? 32 : 16) + 8);
+    }
+    struct Place {
+        Volume *v;
+        uint32_t slot;
+        Request *r;
+    };
+    void loop();
+    int iterate(std::unique_lock<std::mutex> &lk);
+    int device_phase(const std::vector<Volume *> &evict, const std::vector<Volume *> &create,
+                     const std::vector<Place> &place, std::vector<Request *> &done, uint64_t &samples,
+                     uint64_t &launches);
+    int make_resident(Volume &v);
+    void release(Volume &v);
+};
+
+void Server::release(Volume &v)
+{
+    if (v.batch) gc_batch_destroy(v.batch);
+    if (v.cache) gc_cache_destroy(v.cache);
+    v.batch = nullptr;
+    v.cache = nullptr;
+    v.load_pending = false;
+    v.samples_seen = 0;
+}
+
+int Server::make_resident(Volume &v)
+{
+    gc_cache_cfg cc{};
+    const uint32_t bs = v.vc.brick_size[0];
+    const uint64_t brick_bytes = (uint64_t)bs * bs * bs * v.vc.encoded_bytes;
+    GC_TRY(gc_pool_shape(v.pool_bytes, brick_bytes, cc.cache_size));
+    for (int a = 0; a < 3; ++a) cc.block_size[a] = bs;
+    cc.data_cache_flags = 1;
+    cc.max_requests = v.max_requests;
+    cc.sync_requests = 0;
+    GC_TRY(gc_cache_create(&v.vc, &cc, v.src, device, &v.cache));
+    const int rc = gc_batch_create(v.cache, cfg.max_views, (uint64_t)cfg.max_views * cfg.slot_pixels, cfg.precision,
+                                   &v.batch);
+    if (rc != GC_OK) {
+        gc_cache_destroy(v.cache);
+        v.cache = nullptr;
+        return rc;
+    }
+    GC_TRY(gc_batch_set_launch(v.batch, cfg.tile, cfg.march, 0));
+    GC_TRY(gc_batch_set_graph(v.batch, cfg.graph ? 1 : 0));
+    return GC_OK;
+}
+
+// Device work of one iteration (no lock held): evictions, new pools, new
+// views, launches, updates, retirement of finished views.
+int Server::device_phase(const std::vector<Volume *> &evict, const std::vector<Volume *> &create,
+                         const std::vector<Place> &place, std::vector<Request *> &done, uint64_t &samples,
+                         uint64_t &launches)
+{
+    for (Volume *v : evict) release(*v);
+    for (Volume *v : create) GC_TRY(make_resident(*v));
+    for (const Place &p : place)
+        GC_TRY(gc_batch_set_view(p.v->batch, p.slot, &p.r->view, static_cast<const float *>(p.r->d_tf),
+                                 p.r->tf_size));
+    // launches, asynchronous, one per volume with rays to trace
+    std::vector<Volume *> launched;
+    for (auto &v : vols) {
+        if (!v.resident() || !v.busy_slots()) continue;
+        bool fresh = false;
+        for (uint32_t s = 0; s < cfg.max_views; ++s) fresh = fresh || (v.slots[s] && v.fresh[s]);
+        int busy = 0;
+        GC_TRY(gc_cache_busy(v.cache, &busy));
+        if (busy && !fresh) continue;                   // every ray waits for the handler
+        if (!busy && v.load_pending) {
+            int d = 0;
+            GC_TRY(gc_cache_update(v.cache, &d));          // load what the handler fetched
+            v.load_pending = false;
+        }
+        GC_TRY(gc_batch_launch(v.batch, nullptr));
+        launched.push_back(&v);
+    }
+    // updates and retirement
+    std::vector<uint32_t> pend(cfg.max_views);
+    for (Volume *v : launched) {
+        int d = 0;
+        GC_TRY(gc_cache_update(v->cache, &d));
+        if (!d) v->load_pending = true;
+        GC_TRY(gc_batch_pending(v->batch, pend.data()));
+        for (uint32_t s = 0; s < cfg.max_views; ++s) {
+            Request *r = v->slots[s];
+            if (!r) continue;
+            ++r->launches;
+            v->fresh[s] = false;
+            if (pend[s] != 0) continue;
+            r->pixels.resize((size_t)r->view.width * r->view.height);
+            GC_TRY(gc_batch_read_pixels(v->batch, s, r->pixels.data()));
+            GC_TRY(gc_batch_clear_view(v->batch, s));
+            r->t_done = now_ms();
+            v->slots[s] = nullptr;
+            done.push_back(r);
+        }
+        gc_render_stats rs{};
+        GC_TRY(gc_batch_stats(v->batch, &rs));
+        samples += rs.mapped_samples - v->samples_seen;
+        v->samples_seen = rs.mapped_samples;
+        v->last_use = ++tick;
+        ++launches;
+    }
+    return GC_OK;
+}
+
+// One scheduler iteration; called and returns with the lock held.
+int Server::iterate(std::unique_lock<std::mutex> &lk)
+{
+    // admission decisions
+    std::vector<Volume *> evict, create;
+    std::vector<Place> place;
+    uint64_t projected = resident_bytes;
+    for (auto &v : vols) {
+        if (v.queue.empty()) continue;
+        if (cfg.plain && in_flight > 0) break;
+        const bool now_resident = v.resident() || std::find(create.begin(), create.end(), &v) != create.end();
+        if (!now_resident) {
+            while (projected + v.footprint > cfg.pool_budget) {
+                Volume *victim = nullptr;
+                for (auto &u : vols)
+                    if (&u != &v && u.resident() && !u.busy_slots() &&
+                        std::find(evict.begin(), evict.end(), &u) == evict.end() &&
+                        (!victim || u.last_use < victim->last_use))
+                        victim = &u;
+                if (!victim) break;
+                evict.push_back(victim);
+                projected -= victim->footprint;
+            }
+            if (projected + v.footprint > cfg.pool_budget) continue;   // waits for memory
+            create.push_back(&v);
+            projected += v.footprint;
+        }
+        for (uint32_t s = 0; s < cfg.max_views && !v.queue.empty(); ++s) {
+            if (v.slots[s]) continue;
+            if (cfg.plain && in_flight > 0) break;
+            Request *r = v.queue.front();
+            v.queue.pop_front();
+            v.slots[s] = r;
+            v.fresh[s] = true;
+            ++in_flight;
+            place.push_back({&v, s, r});
+        }
+    }
+    resident_bytes = projected;
+    st.pools_evicted += evict.size();
+    st.pools_created += create.size();
+    lk.unlock();
+    std::vector<Request *> done;
+    uint64_t samples = 0, launches = 0;
+    const int rc = device_phase(evict, create, place, done, samples, launches);
+    lk.lock();
+    st.samples += samples;
+    st.launches += launches;
+    for (Request *r : done) {
+        if (r->d_tf) gc_free(r->d_tf);
+        r->d_tf = nullptr;
+        finished.push_back(r);
+        --in_flight;
+        ++st.completed;
+    }
+    if (!done.empty()) done_cv.notify_all();
+    return rc;
+}
+
+void Server::loop()
+{
+    hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+        cv.wait(lk, [&] {
+            if (stop) return true;
+            for (auto &v : vols)
+                if (!v.queue.empty() || v.busy_slots()) return true;
+            return false;
+        });
+        if (stop) break;
+        const int rc = iterate(lk);
+        if (rc != GC_OK) {
+            error = gc_last_error();
+            done_cv.notify_all();
+            break;
+        }
+    }
+}
+
+} // namespace gc
+
+struct gc_server {
+    gc::Server impl;
+};
+
+using namespace gc;
+
+extern "C" {
+
+int gc_server_create(const gc_server_cfg *cfg, int device, gc_server **out)
+{
+    if (!cfg || !out) return fail(GC_ERR_ARG, "null argument");
+    if (cfg->max_views == 0 || cfg->max_views > DVR_MAX_VIEWS) return fail(GC_ERR_ARG, "max_views must be 1..64");
+    if (cfg->slot_pixels == 0) return fail(GC_ERR_ARG, "slot_pixels must be positive");
+    if (cfg->precision != 32 && cfg->precision != 64) return fail(GC_ERR_ARG, "precision must be 32 or 64");
+    if (cfg->tile > 3 || cfg->march > 8) return fail(GC_ERR_ARG, "bad launch shape");
+    auto s = std::make_unique<gc_server>();
+    s->impl.cfg = *cfg;
+    s->impl.device = device;
+    GC_HIP(hipSetDevice(device));
+    s->impl.worker = std::thread([p = &s->impl] { p->loop(); });
+    *out = s.release();
+    return GC_OK;
+}
+
+int gc_server_destroy(gc_server *s)
+{
+    if (!s) return GC_OK;
+    Server &S = s->impl;
+    {
+        std::lock_guard<std::mutex> lk(S.m);
+        S.stop = true;
+    }
+    S.cv.notify_all();
+    if (S.worker.joinable()) S.worker.join();
+    hipSetDevice(S.device);
+    for (auto &v : S.vols)
+        if (v.resident()) S.release(v);
+    for (auto &kv : S.reqs) {
+        if (kv.second->d_tf) gc_free(kv.second->d_tf);
+        delete kv.second;
+    }
+    delete s;
+    return GC_OK;
+}
+
+int gc_server_add_volume(gc_server *s, const gc_volume_cfg *vc, gc_source *src, uint64_t pool_bytes,
+                         uint32_t max_requests, uint32_t *volume)
+{
+    if (!s || !vc || !src || !volume) return fail(GC_ERR_ARG, "null argument");
+    Server &S = s->impl;
+    const uint64_t bs = vc->brick_size[0];
+    const uint64_t brick_bytes = bs * bs * bs * vc->encoded_bytes;
+    if (pool_bytes < brick_bytes) return fail(GC_ERR_ARG, "pool smaller than one brick");
+    if (max_requests == 0) return fail(GC_ERR_ARG, "max_requests must be positive");
+    std::lock_guard<std::mutex> lk(S.m);
+    Volume v;
+    v.vc = *vc;
+    v.src = src;
+    v.pool_bytes = pool_bytes;
+    v.max_requests = max_requests;
+    v.footprint = pool_bytes + S.slot_bytes();
+    if (v.footprint > S.cfg.pool_budget) return fail(GC_ERR_ARG, "volume's pool and view slots exceed the budget");
+    v.slots.assign(S.cfg.max_views, nullptr);
+    v.fresh.assign(S.cfg.max_views, false);
+    S.vols.push_back(std::move(v));
+    *volume = (uint32_t)(S.vols.size() - 1);
+    return GC_OK;
+}
+
+int gc_server_submit(gc_server *s, uint32_t volume, const gc_view *view, const float *tf, uint32_t tf_size,
+                     uint64_t *id)
+{
+    if (!s || !view || !tf || !id || tf_size == 0) return fail(GC_ERR_ARG, "null argument");
+    Server &S = s->impl;
+    if ((uint64_t)view->width * view->height > S.cfg.slot_pixels || view->width == 0 || view->height == 0)
+        return fail(GC_ERR_ARG, "view larger than slot_pixels");
+    if (view->precision != S.cfg.precision) return fail(GC_ERR_ARG, "view precision differs from the server's");
+    auto r = std::make_unique<Request>();
+    r->view = *view;
+    r->tf_size = tf_size;
+    GC_HIP(hipSetDevice(S.device));
+    GC_TRY(gc_malloc(&r->d_tf, (size_t)tf_size * 16));
+    const int rc = gc_memcpy_h2d(r->d_tf, tf, (size_t)tf_size * 16);
+    if (rc != GC_OK) {
+        gc_free(r->d_tf);
+        return rc;
+    }
+    {
+        std::lock_guard<std::mutex> lk(S.m);
+        if (!S.error.empty()) {
+            gc_free(r->d_tf);
+            return fail(GC_ERR_STATE, "server stopped: " + S.error);
+        }
+        if (volume >= S.vols.size()) {
+            gc_free(r->d_tf);
+            return fail(GC_ERR_ARG, "no such volume");
+        }
+        r->id = S.next_id++;
+        r->volume = volume;
+        r->t_submit = S.now_ms();
+        *id = r->id;
+        S.vols[volume].queue.push_back(r.get());
+        S.reqs[r->id] = r.get();
+        ++S.st.submitted;
+        r.release();
+    }
+    S.cv.notify_all();
+    return GC_OK;
+}
+
+int gc_server_poll(gc_server *s, gc_server_result *out, uint32_t max, uint32_t *n)
+{
+    if (!s || !n || (!out && max)) return fail(GC_ERR_ARG, "null argument");
+    Server &S = s->impl;
+    std::lock_guard<std::mutex> lk(S.m);
+    uint32_t k = 0;
+    while (k < max && !S.finished.empty()) {
+        Request *r = S.finished.front();
+        S.finished.pop_front();
+        gc_server_result &o = out[k++];
+        o.id = r->id;
+        o.volume = r->volume;
+        o.width = r->view.width;
+        o.height = r->view.height;
+        o.launches = r->launches;
+        o.submit_ms = r->t_submit;
+        o.latency_ms = r->t_done - r->t_submit;
+    }
+    *n = k;
+    if (!S.error.empty() && k == 0) return fail(GC_ERR_STATE, "server stopped: " + S.error);
+    return GC_OK;
+}
+
+int gc_server_take_pixels(gc_server *s, uint64_t id, uint32_t *host)
+{
+    if (!s || !host) return fail(GC_ERR_ARG, "null argument");
+    Server &S = s->impl;
+    std::lock_guard<std::mutex> lk(S.m);
+    auto it = S.reqs.find(id);
+    if (it == S.reqs.end() || it->second->t_done == 0.0) return fail(GC_ERR_ARG, "request not completed");
+    Request *r = it->second;
+    std::memcpy(host, r->pixels.data(), r->pixels.size() * 4);
+    delete r;
+    S.reqs.erase(it);
+    return GC_OK;
+}
+
+int gc_server_drain(gc_server *s)
+{
+    if (!s) return fail(GC_ERR_ARG, "null server");
+    Server &S = s->impl;
+    std::unique_lock<std::mutex> lk(S.m);
+    S.done_cv.wait(lk, [&] { return S.st.completed == S.st.submitted || !S.error.empty(); });
+    if (!S.error.empty()) return fail(GC_ERR_STATE, "server stopped: " + S.error);
+    return GC_OK;
+}
+
+int gc_server_stats_get(gc_server *s, gc_server_stats *out)
+{
+    if (!s || !out) return fail(GC_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->impl.m);
+    *out = s->impl.st;
+    return GC_OK;
+}
+
+} // extern "C"

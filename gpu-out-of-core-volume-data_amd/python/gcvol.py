@@ -97,6 +97,21 @@ class CacheView(C.Structure):
     ]
 
 
+class ServerCfg(C.Structure):
+    _fields_ = [("pool_budget", u64), ("max_views", u32), ("slot_pixels", u32), ("precision", u32), ("tile", u32),
+                ("march", u32), ("plain", u32), ("graph", u32), ("reserved", u32)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("id", u64), ("volume", u32), ("width", u32), ("height", u32), ("launches", u32),
+                ("submit_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("launches", u64), ("samples", u64),
+                ("pools_created", u64), ("pools_evicted", u64)]
+
+
 EXPORTS = [
     "gc_last_error", "gc_version", "gc_volume_cfg_init", "gc_pool_shape", "gc_cfg_get", "gc_volume_cfg_load",
     "gc_device_count", "gc_set_device", "gc_malloc", "gc_free", "gc_memcpy_h2d", "gc_memcpy_d2h", "gc_memcpy_d2d",
@@ -113,6 +128,8 @@ EXPORTS = [
     "gc_batch_create", "gc_batch_destroy", "gc_batch_set_graph", "gc_batch_render", "gc_batch_read_pixels",
     "gc_batch_set_launch", "gc_batch_begin", "gc_batch_set_view", "gc_batch_clear_view", "gc_batch_launch",
     "gc_batch_pending", "gc_batch_stats", "gc_batch_read_state", "gc_cache_set_max_requests",
+    "gc_cache_busy", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
+    "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get",
 ]
 
 _lib = None
@@ -172,6 +189,13 @@ def lib():
             "gc_batch_set_view": [P, u32, C.POINTER(View), P, u32], "gc_batch_clear_view": [P, u32],
             "gc_batch_launch": [P, C.POINTER(C.c_double)], "gc_batch_pending": [P, P],
             "gc_batch_stats": [P, C.POINTER(RenderStats)], "gc_batch_read_state": [P, u32, P, P],
+            "gc_cache_busy": [P, C.POINTER(C.c_int)],
+            "gc_server_create": [C.POINTER(ServerCfg), C.c_int, C.POINTER(vp)], "gc_server_destroy": [P],
+            "gc_server_add_volume": [P, C.POINTER(VolumeCfg), P, u64, u32, C.POINTER(u32)],
+            "gc_server_submit": [P, u32, C.POINTER(View), P, u32, C.POINTER(u64)],
+            "gc_server_poll": [P, C.POINTER(ServerResult), u32, C.POINTER(u32)],
+            "gc_server_take_pixels": [P, u64, P], "gc_server_drain": [P],
+            "gc_server_stats_get": [P, C.POINTER(ServerStats)],
         }
         missing = [n for n in EXPORTS if n not in sig]
         if missing:
@@ -573,6 +597,60 @@ class Batch:
     def close(self):
         if self.h:
             lib().gc_batch_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (gc_server_*)."""
+
+    def __init__(self, pool_budget, max_views=64, slot_pixels=2048 * 2048, precision=32, tile=0, march=0,
+                 plain=False, graph=False, device=0):
+        cfg = ServerCfg(pool_budget, max_views, slot_pixels, precision, tile, march, 1 if plain else 0,
+                        1 if graph else 0, 0)
+        self.h = vp()
+        self.keep = []
+        check(lib().gc_server_create(C.byref(cfg), device, C.byref(self.h)))
+
+    def add_volume(self, vc, source, pool_bytes, max_requests=1024):
+        v = u32(0)
+        check(lib().gc_server_add_volume(self.h, C.byref(vc), source.h, pool_bytes, max_requests, C.byref(v)))
+        self.keep.append(source)
+        return v.value
+
+    def submit(self, volume, view, tf):
+        tf = np.ascontiguousarray(tf, np.float32)
+        i = u64(0)
+        check(lib().gc_server_submit(self.h, volume, C.byref(view), ptr(tf), len(tf), C.byref(i)))
+        return i.value
+
+    def poll(self, max_results=256):
+        out = (ServerResult * max_results)()
+        n = u32(0)
+        check(lib().gc_server_poll(self.h, out, max_results, C.byref(n)))
+        return [out[k] for k in range(n.value)]
+
+    def take_pixels(self, rid, width, height):
+        px = np.zeros(width * height, np.uint32)
+        check(lib().gc_server_take_pixels(self.h, rid, ptr(px)))
+        return px
+
+    def drain(self):
+        check(lib().gc_server_drain(self.h))
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().gc_server_stats_get(self.h, C.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            lib().gc_server_destroy(self.h)
             self.h = vp()
 
     def __del__(self):

@@ -352,6 +352,56 @@ int gc_batch_read_state(gc_batch *b, uint32_t slot, float *tbuf, void *cstate);
 /* Bricks per update (the upload batch), at most the max_requests the cache
  * was created with. */
 int gc_cache_set_max_requests(gc_cache *c, uint32_t max_requests);
+/* busy = 1 while the request handler still works on the last request. */
+int gc_cache_busy(gc_cache *c, int *busy);
+
+/* ----------------------------------------------------------------- serving */
+
+/* A serving loop over several volumes: view requests of any size, camera and
+ * transfer function arrive at any time; each volume keeps its page directory
+ * and brick pool resident on the device while the pools fit pool_budget
+ * (admission by brick-pool memory; idle volumes are evicted to make room);
+ * a scheduler thread launches every volume's in-flight views together each
+ * iteration (continuous batching: views join as slots free up, finished
+ * views leave) and updates each cache as the reference's loop does.  No
+ * reference counterpart.  plain = 1 serves one request at a time, one view
+ * per launch (the plain renderer under the same arrivals). */
+typedef struct gc_server gc_server;
+typedef struct gc_server_cfg {
+    uint64_t pool_budget;        /* device bytes of resident pools + view slots */
+    uint32_t max_views;          /* concurrent views per volume, 1..64          */
+    uint32_t slot_pixels;        /* pixels of the largest view                  */
+    uint32_t precision;          /* 32 or 64                                    */
+    uint32_t tile, march;        /* launch shape (gc_batch_set_launch)          */
+    uint32_t plain;              /* 1: one request at a time                    */
+    uint32_t graph;              /* 1: hipGraph replay of the launches          */
+    uint32_t reserved;
+} gc_server_cfg;
+typedef struct gc_server_result {
+    uint64_t id;
+    uint32_t volume, width, height, launches;
+    double   submit_ms;          /* server clock at submission                  */
+    double   latency_ms;         /* submission -> pixels on the host            */
+} gc_server_result;
+typedef struct gc_server_stats {
+    uint64_t submitted, completed, launches, samples, pools_created, pools_evicted;
+} gc_server_stats;
+int gc_server_create(const gc_server_cfg *cfg, int device, gc_server **out);
+int gc_server_destroy(gc_server *s);
+/* Register a volume served from src with a pool of pool_bytes and
+ * max_requests bricks per update (its device state is created on first use). */
+int gc_server_add_volume(gc_server *s, const gc_volume_cfg *vc, gc_source *src, uint64_t pool_bytes,
+                         uint32_t max_requests, uint32_t *volume);
+/* Queue a request; tf is a host pointer to tf_size float4 (copied). */
+int gc_server_submit(gc_server *s, uint32_t volume, const gc_view *view, const float *tf,
+                     uint32_t tf_size, uint64_t *id);
+/* Requests completed since the last poll (at most max). */
+int gc_server_poll(gc_server *s, gc_server_result *out, uint32_t max, uint32_t *n);
+/* The pixels of a completed request (W*H ABGR); releases them. */
+int gc_server_take_pixels(gc_server *s, uint64_t id, uint32_t *host);
+/* Block until every submitted request has completed. */
+int gc_server_drain(gc_server *s);
+int gc_server_stats_get(gc_server *s, gc_server_stats *out);
 
 /* ------------------------------------------------------------------ helpers */
 
