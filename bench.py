@@ -138,6 +138,100 @@ BATCH_VIEWS = 64
 BATCH_SIZES = [(256, 256), (640, 480), (1024, 1024), (1920, 1080)]
 
 
+# serving workload: Poisson arrivals over 6 resident volumes
+SERVE_VOLUMES = [((512, 512, 512), 1), ((1024, 1024, 1024), 1), ((1024, 1024, 1024), 2), ((2048, 2048, 2048), 1),
+                 ((2048, 2048, 1024), 2), ((4096, 4096, 4096), 1)]
+SERVE_POOL = 12 << 30          # per volume; 6 pools + view slots within the 96 GiB budget
+SERVE_BUDGET = 96 << 30
+
+
+def serving_workload(G, rates=(25, 50, 100, 200), seconds=3.0, max_views=16):
+    """Requests arrive as a Poisson process (seeded) over 6 volumes (512^3 u8,
+    1024^3 u8, 1024^3 u16, 2048^3 u8, 2048x2048x1024 u16, 4096^3 u8, each
+    synthesised brick by brick on the host on demand, 32^3 bricks, 12 GiB
+    pool), each with its own square resolution (256..2048), camera and one of
+    8 transfer functions.  For every rate the same arrival sequence is served
+    twice: by the serving loop (continuous batching, up to 16 views per
+    volume per launch) and by the plain renderer (one request at a time, one
+    view per launch).  Latency is submission to pixels on the host."""
+    import threading
+
+    tables = [G.tf_samples(G.tf_seeded(SEED + 200 + i, 6), 256) for i in range(8)]
+    sources, vcs = [], []
+    for i, (dims, nb) in enumerate(SERVE_VOLUMES):
+        vc = G.volume_cfg(dims, 32, 1, nb)
+        amp = (60.0, 255.0) if nb == 1 else (15000.0, 65535.0)
+        params = G.synth_params(SEED + 10 + i, 32, sigma=(dims[0] / 40.0, dims[0] / 8.0), amp=amp,
+                                noise=8 if nb == 1 else 2048, threads=0)
+        sources.append(G.Source.synthetic(vc, params))
+        vcs.append(vc)
+
+    def make_requests(rate, seed):
+        rng = np.random.default_rng(seed)
+        t, out = 0.0, []
+        while True:
+            t += float(rng.exponential(1.0 / rate))
+            if t >= seconds:
+                return out
+            side = int(rng.integers(4, 33)) * 64
+            cam = G.camera(float(rng.uniform(-30.0, 30.0)), float(rng.uniform(0.0, 360.0)),
+                           float(rng.uniform(2.2, 3.6)))
+            out.append((t, int(rng.integers(len(SERVE_VOLUMES))), side, cam, int(rng.integers(len(tables)))))
+
+    def serve(reqs, plain):
+        srv = G.Server(SERVE_BUDGET, max_views=1 if plain else max_views, slot_pixels=2048 * 2048, precision=32,
+                       plain=plain)
+        vids = [srv.add_volume(vc, src, SERVE_POOL, max_requests=4096) for vc, src in zip(vcs, sources)]
+        # warm: one far view per volume brings each volume's coarse levels in
+        for v in vids:
+            srv.submit(v, G.make_view(512, 512, G.camera(20.0, 30.0, 3.4), alpha_stop=ALPHA_STOP), tables[0])
+        srv.drain()
+        for r in srv.poll(64):
+            srv.take_pixels(r.id, r.width, r.height)
+        s0 = srv.stats().samples
+        sizes, lat = {}, []
+        t_start = time.perf_counter()
+
+        def feeder():
+            for (t, vol, side, cam, tfi) in reqs:
+                d = t_start + t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                rid = srv.submit(vids[vol], G.make_view(side, side, cam, alpha_stop=ALPHA_STOP), tables[tfi])
+                sizes[rid] = (side, side)
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        done = 0
+        while done < len(reqs):
+            for r in srv.poll(256):
+                srv.take_pixels(r.id, r.width, r.height)
+                lat.append(r.latency_ms)
+                done += 1
+            time.sleep(0.001)
+        th.join()
+        wall = time.perf_counter() - t_start
+        samples = srv.stats().samples - s0
+        srv.close()
+        return {"completed": done, "wall_s": wall, "frames_per_s": done / wall, "samples_per_s": samples / wall,
+                "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99)}
+
+    out = []
+    for i, rate in enumerate(rates):
+        reqs = make_requests(rate, SEED + 300 + i)
+        b = serve(reqs, False)
+        pl = serve(reqs, True)
+        print("serving %d req/s: %d requests; batched p50 %.0f p99 %.0f ms %.1f fps | plain p50 %.0f p99 %.0f ms %.1f fps"
+              % (rate, len(reqs), b["p50_ms"], b["p99_ms"], b["frames_per_s"], pl["p50_ms"], pl["p99_ms"],
+                 pl["frames_per_s"]), file=sys.stderr, flush=True)
+        out.append({"rate": rate, "requests": len(reqs), "batched": b, "plain": pl})
+    return {"workload": "Poisson arrivals for %.0f s per rate over 6 volumes synthesised on demand (512^3 u8, "
+                        "1024^3 u8, 1024^3 u16, 2048^3 u8, 2048x2048x1024 u16, 4096^3 u8; 32^3 bricks, 12 GiB "
+                        "pool each, 96 GiB budget), square views 256..2048, own camera and TF per request; "
+                        "batched = serving loop (<= %d views per volume per launch), plain = one request at a time"
+                        % (seconds, max_views), "rates": out}
+
+
 def batch_workload(G, reps=3):
     """Batched vs plain rendering of 64 concurrent views (sizes uniform among
     256^2, 640x480, 1024^2, 1920x1080, seeded cameras, 8 seeded 256-entry
@@ -286,6 +380,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain workload")
+    ap.add_argument("--serving", action="store_true", help="also run the serving workload (several minutes)")
+    ap.add_argument("--serving-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known traffic "
@@ -295,13 +391,16 @@ def main():
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    traffic, traffic_note = (None, "not measured") if args.no_traffic else measure_traffic(args)
+    traffic, traffic_note = (None, "not measured") if (args.no_traffic or args.serving_only) else measure_traffic(args)
 
     import gcvol as G
 
     if G.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.serving_only:
+        print(json.dumps({"serving": serving_workload(G)}))
+        return
 
     if args.calib_copy_mib > 0:
         nbytes = args.calib_copy_mib << 20
@@ -395,6 +494,8 @@ def main():
     frame.close()
     if not args.no_batch:
         out["batched_vs_plain"] = batch_workload(G)
+    if args.serving:
+        out["serving"] = serving_workload(G)
     print(json.dumps(out))
 
 

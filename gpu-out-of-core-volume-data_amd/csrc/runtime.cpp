@@ -41,13 +41,29 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
+static uint32_t hw_threads();
+
+// Extra worker threads available to parallel_for across the whole process,
+// so that several request handlers working at once share the host cores
+// instead of each starting a full set of threads.
+static std::atomic<int> g_thread_tokens{-1};
+
 static void parallel_for(uint32_t n, uint32_t threads, const std::function<void(uint32_t)> &fn)
 {
     if (threads <= 1 || n <= 1) {
         for (uint32_t i = 0; i < n; ++i) fn(i);
         return;
     }
-    threads = std::min(threads, n);
+    int expect = -1;
+    g_thread_tokens.compare_exchange_strong(expect, (int)hw_threads() - 1);
+    int want = (int)std::min(threads, n) - 1, have = g_thread_tokens.load();
+    int take = 0;
+    while (want > 0 && have > 0) {
+        take = std::min(want, have);
+        if (g_thread_tokens.compare_exchange_weak(have, have - take)) break;
+        take = 0;
+    }
+    threads = (uint32_t)take + 1;
     std::atomic<uint32_t> next{0};
     std::vector<std::thread> pool;
     auto work = [&]() {
@@ -60,6 +76,7 @@ static void parallel_for(uint32_t n, uint32_t threads, const std::function<void(
     for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work);
     work();
     for (auto &t : pool) t.join();
+    if (take) g_thread_tokens.fetch_add(take);
 }
 
 static uint32_t hw_threads()
@@ -811,7 +828,7 @@ static int batch_place(Batch &B, uint32_t slot, const gc_view &v, const float *d
 
 // One launch: the previous launch's stopped rays, then the starting slots'
 // tiles.  No cache update.
-static int batch_launch(Batch &B, gc_frame_report *rep)
+static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
 {
     Cache &C = *B.C;
     hipStream_t s = C.stream;
@@ -832,7 +849,10 @@ static int batch_launch(Batch &B, gc_frame_report *rep)
         D.tiles_x = (D.width + tw - 1) / tw;
         if (S.starting) packets += D.tiles_x * ((D.height + th - 1) / th);
     }
-    const bool has_list = B.cur >= 0;
+    // skip_list: leave the waiting rays in their list (their bricks are not
+    // loaded yet); this launch's stopped rays are appended to it
+    const bool has_list = B.cur >= 0 && !skip_list;
+    const bool append = B.cur >= 0 && skip_list;
     if (packets == 0 && !has_list) return GC_OK;
     GC_HIP(hipMemcpyAsync(B.d_views, B.h_views, (size_t)B.table * sizeof(DvrView), hipMemcpyHostToDevice, s));
     GC_HIP(hipEventRecord(B.views_free, s));
@@ -846,7 +866,7 @@ static int batch_launch(Batch &B, gc_frame_report *rep)
     L.refill_min = B.refill_min;
     L.march = B.march;
     L.view_stopped = B.ctl + 16;
-    const int out = has_list ? 1 - B.cur : 0;
+    const int out = has_list ? 1 - B.cur : (append ? B.cur : 0);
     if (has_list) {
         L.list = B.lists[B.cur];
         L.list_count = B.ctl + 8 + B.cur;
@@ -854,8 +874,9 @@ static int batch_launch(Batch &B, gc_frame_report *rep)
     L.out_list = B.lists[out];
     L.out_count = B.ctl + 8 + out;
     uint64_t listed = 0;
-    for (uint32_t i = 0; i < B.table; ++i)
-        if (B.slots[i].used && !B.slots[i].starting) listed += B.slots[i].npix;
+    if (has_list)
+        for (uint32_t i = 0; i < B.table; ++i)
+            if (B.slots[i].used && !B.slots[i].starting) listed += B.slots[i].npix;
     const uint32_t lds = dvr_lds_bytes(L.nb_lods, B.table);
     const uint32_t grid = dvr_grid(C.vc.encoded_bytes, B.precision, B.tile, packets + (uint32_t)((listed + 63) / 64), lds);
     GC_HIP(hipEventRecord(B.ev0, s));
@@ -865,14 +886,14 @@ static int batch_launch(Batch &B, gc_frame_report *rep)
         GC_HIP(launch_fill32(B.ctl + 10, 1, C.ts, s));
         L.ts_dev = B.ctl + 10;
         const std::array<uint32_t, 8> key{B.table, B.tile, B.march, B.refill_min, packets, grid,
-                                          has_list ? (uint32_t)B.cur : 2u, (uint32_t)out};
+                                          has_list ? (uint32_t)B.cur : 2u, (uint32_t)out + (append ? 4u : 0u)};
         auto it = B.graphs.find(key);
         if (it == B.graphs.end()) {
             hipGraph_t g = nullptr;
             GC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             hipError_t e = launch_fill32(B.ctl, 8, 0, s);
-            if (e == hipSuccess) e = launch_fill32(B.ctl + 8 + out, 1, 0, s);
-            if (e == hipSuccess) e = launch_fill32(B.ctl + 16, B.max_views, 0, s);
+            if (e == hipSuccess && !append) e = launch_fill32(B.ctl + 8 + out, 1, 0, s);
+            if (e == hipSuccess && !append) e = launch_fill32(B.ctl + 16, B.max_views, 0, s);
             if (e == hipSuccess) e = launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
             const hipError_t e_end = hipStreamEndCapture(s, &g);
             GC_HIP(e);
@@ -886,8 +907,11 @@ static int batch_launch(Batch &B, gc_frame_report *rep)
         GC_HIP(hipGraphLaunch(it->second, s));
     } else {
         GC_HIP(launch_fill32(B.ctl, 8, 0, s));
-        GC_HIP(launch_fill32(B.ctl + 8 + out, 1, 0, s));
-        GC_HIP(launch_fill32(B.ctl + 16, B.max_views, 0, s));
+        if (!append) {
+            // the per-slot counters count the rays in the output list
+            GC_HIP(launch_fill32(B.ctl + 8 + out, 1, 0, s));
+            GC_HIP(launch_fill32(B.ctl + 16, B.max_views, 0, s));
+        }
         GC_HIP(launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s));
     }
     GC_HIP(hipEventRecord(B.ev1, s));
@@ -1725,12 +1749,13 @@ int gc_batch_clear_view(gc_batch *b, uint32_t slot)
     return GC_OK;
 }
 
-int gc_batch_launch(gc_batch *b, double *kernel_ms)
+int gc_batch_launch(gc_batch *b, uint32_t flags, double *kernel_ms)
 {
     if (!b) return fail(GC_ERR_ARG, "null batch");
+    if (flags & ~(uint32_t)GC_LAUNCH_STARTING_ONLY) return fail(GC_ERR_ARG, "unknown launch flag");
     Batch &B = b->impl;
     GC_HIP(hipSetDevice(B.C->device));
-    GC_TRY(batch_launch(B, nullptr));
+    GC_TRY(batch_launch(B, nullptr, (flags & GC_LAUNCH_STARTING_ONLY) != 0));
     if (kernel_ms) GC_TRY(batch_kernel_ms(B, *kernel_ms));
     return GC_OK;
 }

@@ -164,7 +164,8 @@ int Server::device_phase(const std::vector<Volume *> &evict, const std::vector<V
             GC_TRY(gc_cache_update(v.cache, &d));          // load what the handler fetched
             v.load_pending = false;
         }
-        GC_TRY(gc_batch_launch(v.batch, nullptr));
+        // while the handler fetches, only the starting views are traced
+        GC_TRY(gc_batch_launch(v.batch, busy ? GC_LAUNCH_STARTING_ONLY : 0u, nullptr));
         launched.push_back(&v);
     }
     // updates and retirement

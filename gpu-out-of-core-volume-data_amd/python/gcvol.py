@@ -187,7 +187,7 @@ def lib():
             "gc_batch_set_launch": [P, u32, u32, u32],
             "gc_batch_begin": [P, C.POINTER(View), C.POINTER(vp), P, u32],
             "gc_batch_set_view": [P, u32, C.POINTER(View), P, u32], "gc_batch_clear_view": [P, u32],
-            "gc_batch_launch": [P, C.POINTER(C.c_double)], "gc_batch_pending": [P, P],
+            "gc_batch_launch": [P, u32, C.POINTER(C.c_double)], "gc_batch_pending": [P, P],
             "gc_batch_stats": [P, C.POINTER(RenderStats)], "gc_batch_read_state": [P, u32, P, P],
             "gc_cache_busy": [P, C.POINTER(C.c_int)],
             "gc_server_create": [C.POINTER(ServerCfg), C.c_int, C.POINTER(vp)], "gc_server_destroy": [P],
@@ -566,9 +566,9 @@ class Batch:
         check(lib().gc_batch_clear_view(self.h, slot))
         self.sizes.pop(slot, None)
 
-    def launch(self):
+    def launch(self, starting_only=False):
         ms = C.c_double(0)
-        check(lib().gc_batch_launch(self.h, C.byref(ms)))
+        check(lib().gc_batch_launch(self.h, 1 if starting_only else 0, C.byref(ms)))
         return ms.value
 
     def pending(self):

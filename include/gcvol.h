@@ -334,8 +334,11 @@ int gc_batch_set_view(gc_batch *b, uint32_t slot, const gc_view *view, const flo
 int gc_batch_clear_view(gc_batch *b, uint32_t slot);
 /* One launch over the batch, without a cache update (the caller updates, as
  * in the reference's loop).  kernel_ms (optional) waits for the launch and
- * returns its time. */
-int gc_batch_launch(gc_batch *b, double *kernel_ms);
+ * returns its time.  GC_LAUNCH_STARTING_ONLY: trace only the starting slots'
+ * tiles and leave the rays waiting for bricks in their list (for launches
+ * while the request handler is still fetching); their stopped rays join it. */
+#define GC_LAUNCH_STARTING_ONLY 1u
+int gc_batch_launch(gc_batch *b, uint32_t flags, double *kernel_ms);
 /* Per slot (max_views entries): rays of the last launch that stopped at a
  * miss -- 0 for a complete view, 0xffffffff for a view not launched yet. */
 int gc_batch_pending(gc_batch *b, uint32_t *stopped);

@@ -160,3 +160,34 @@ def test_upload_batch_limit(gpu):
     assert rep.complete == 1
     asked, _ = gc.asked()
     assert 0 < len(asked) <= 7
+
+
+def test_starting_only_launch_keeps_waiting_rays(gpu):
+    """A launch that traces only the starting views leaves the rays waiting
+    for bricks in their list (their per-view counts unchanged) and appends
+    the new view's stopped rays; the views then complete as usual, within the
+    restart tolerance of the alone renders (pixels of rays resumed at other
+    points may differ by a few LSB)."""
+    sc = Scene((64, 64, 64), 8, 1, 1, max_requests=50)
+    gc = sc.gpu_cache()
+    views = views_for(64)[:2]
+    d_tfs, sizes = tfs(2)
+    b = G.Batch(gc, 2, 2 * 128 * 80, 64)
+    b.set_view(0, views[0], d_tfs[0], sizes[0])
+    b.launch()
+    p0 = b.pending()
+    assert p0[0] > 0 and p0[1] == 0
+    b.set_view(1, views[1], d_tfs[1], sizes[1])
+    b.launch(starting_only=True)
+    p1 = b.pending()
+    assert p1[0] == p0[0] and p1[1] > 0
+    for it in range(200):
+        gc.update()
+        b.launch()
+        if not b.pending().any():
+            break
+    assert not b.pending().any()
+    ref, _ = singles(sc.gpu_cache(), views, d_tfs, sizes)
+    for i in range(2):
+        err = np.abs(G.unpack_rgba(b.pixels(i)) - G.unpack_rgba(ref[i]))
+        assert err.max() <= 4 and (err > 1).mean() <= 0.005, i
