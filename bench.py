@@ -145,7 +145,7 @@ SERVE_POOL = 12 << 30          # per volume; 6 pools + view slots within the 96 
 SERVE_BUDGET = 96 << 30
 
 
-def serving_workload(G, rates=(25, 50, 100, 200), seconds=3.0, max_views=16):
+def serving_workload(G, rates=(4, 8, 16, 32), seconds=4.0, max_views=16, warm_requests=24):
     """Requests arrive as a Poisson process (seeded) over 6 volumes (512^3 u8,
     1024^3 u8, 1024^3 u16, 2048^3 u8, 2048x2048x1024 u16, 4096^3 u8, each
     synthesised brick by brick on the host on demand, 32^3 bricks, 12 GiB
@@ -182,12 +182,16 @@ def serving_workload(G, rates=(25, 50, 100, 200), seconds=3.0, max_views=16):
         srv = G.Server(SERVE_BUDGET, max_views=1 if plain else max_views, slot_pixels=2048 * 2048, precision=32,
                        plain=plain)
         vids = [srv.add_volume(vc, src, SERVE_POOL, max_requests=4096) for vc, src in zip(vcs, sources)]
-        # warm: one far view per volume brings each volume's coarse levels in
-        for v in vids:
-            srv.submit(v, G.make_view(512, 512, G.camera(20.0, 30.0, 3.4), alpha_stop=ALPHA_STOP), tables[0])
+        # warm: an untimed burst from the same request distribution
+        for (_, vol, side, cam, tfi) in warm:
+            srv.submit(vids[vol], G.make_view(side, side, cam, alpha_stop=ALPHA_STOP), tables[tfi])
         srv.drain()
-        for r in srv.poll(64):
-            srv.take_pixels(r.id, r.width, r.height)
+        while True:
+            rs = srv.poll(256)
+            if not rs:
+                break
+            for r in rs:
+                srv.take_pixels(r.id, r.width, r.height)
         s0 = srv.stats().samples
         sizes, lat = {}, []
         t_start = time.perf_counter()
@@ -216,6 +220,10 @@ def serving_workload(G, rates=(25, 50, 100, 200), seconds=3.0, max_views=16):
         return {"completed": done, "wall_s": wall, "frames_per_s": done / wall, "samples_per_s": samples / wall,
                 "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99)}
 
+    warm = []
+    while len(warm) < warm_requests:
+        warm += make_requests(8.0, SEED + 299 + len(warm))
+    warm = warm[:warm_requests]
     out = []
     for i, rate in enumerate(rates):
         reqs = make_requests(rate, SEED + 300 + i)

@@ -131,6 +131,17 @@ hipError_t launch_occupancy(const SlotMeta *meta, uint32_t nslots, uint32_t *cou
 hipError_t launch_mip(const void *in, const uint32_t dims[3], uint32_t bytes, void *out,
                       hipStream_t stream);
 
+// One level-l brick from its level-(l-1) children (mip on miss).
+struct MipBrickArgs {
+    int64_t g0[3];        // level-l voxel at brick index (0,0,0): b * inner - cov
+    uint32_t dims[3];     // level-l extent (voxels outside are 0)
+    uint32_t c0[3];       // child-brick index of children[0]
+    uint32_t nc[3];       // child grid extent (children x-fastest)
+    uint32_t bs, cov, inner;
+};
+hipError_t launch_mip_brick(const MipBrickArgs &a, uint32_t bytes, const void *children, void *out,
+                            hipStream_t stream);
+
 hipError_t launch_brick_cut(const void *level, const uint32_t dims[3], uint32_t bytes,
                             uint32_t bs, uint32_t cov, const uint32_t *bricks_xyz, uint32_t n,
                             void *out, hipStream_t stream);

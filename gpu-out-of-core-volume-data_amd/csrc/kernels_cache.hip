@@ -359,6 +359,46 @@ __global__ __launch_bounds__(BLK) void k_mip2(const T *__restrict__ in, uint32_t
     }
 }
 
+// One brick of level l built from level l-1 bricks (mip on miss): voxel g of
+// level l (global, zero outside the level) = floor(sum of level-(l-1) voxels
+// 2g + {0,1}^3 / 8) with k_mip2's types and order; each level-(l-1) voxel p
+// is read from the inner region of the child brick p / inner that holds it.
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_mip_brick(const MipBrickArgs a, const T *__restrict__ children,
+                                                   T *__restrict__ out)
+{
+    const uint32_t bs = a.bs, n = bs * bs * bs;
+    const uint64_t cvox = (uint64_t)bs * bs * bs;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int64_t g[3] = {a.g0[0] + (int64_t)(i % bs), a.g0[1] + (int64_t)((i / bs) % bs),
+                              a.g0[2] + (int64_t)(i / (bs * bs))};
+        bool in = true;
+        for (int k = 0; k < 3; ++k) in = in && g[k] >= 0 && g[k] < (int64_t)a.dims[k];
+        if (!in) {
+            out[i] = T(0);
+            continue;
+        }
+        double ds = 0.0;
+        uint32_t is = 0;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const uint32_t p[3] = {(uint32_t)(2 * g[0] + dx), (uint32_t)(2 * g[1] + dy), (uint32_t)(2 * g[2] + dz)};
+                    uint32_t c[3], l[3];
+                    for (int k = 0; k < 3; ++k) {
+                        c[k] = p[k] / a.inner - a.c0[k];
+                        l[k] = p[k] % a.inner + a.cov;
+                    }
+                    const uint32_t ci = c[0] + a.nc[0] * (c[1] + a.nc[1] * c[2]);
+                    const T v = children[ci * cvox + l[0] + (uint64_t)bs * (l[1] + (uint64_t)bs * l[2])];
+                    if constexpr (sizeof(T) == 4) ds += (double)v;
+                    else is += (uint32_t)v;
+                }
+        if constexpr (sizeof(T) == 4) out[i] = (T)(ds / 8.0);
+        else out[i] = (T)(is >> 3);
+    }
+}
+
 template <int BYTES>
 __global__ __launch_bounds__(BLK) void k_brick_cut(const uint8_t *__restrict__ level, uint32_t X,
                                                    uint32_t Y, uint32_t Z, uint32_t bs, uint32_t cov,
@@ -485,6 +525,20 @@ hipError_t launch_mip(const void *in, const uint32_t dims[3], uint32_t bytes, vo
     case 1: k_mip2<uint8_t><<<nb, BLK, 0, stream>>>((const uint8_t *)in, dims[0], dims[1], (uint8_t *)out, OX, OY, total); break;
     case 2: k_mip2<uint16_t><<<nb, BLK, 0, stream>>>((const uint16_t *)in, dims[0], dims[1], (uint16_t *)out, OX, OY, total); break;
     case 4: k_mip2<float><<<nb, BLK, 0, stream>>>((const float *)in, dims[0], dims[1], (float *)out, OX, OY, total); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mip_brick(const MipBrickArgs &a, uint32_t bytes, const void *children, void *out,
+                            hipStream_t stream)
+{
+    const uint32_t n = a.bs * a.bs * a.bs;
+    const uint32_t nb = (n + BLK - 1) / BLK;
+    switch (bytes) {
+    case 1: k_mip_brick<uint8_t><<<nb, BLK, 0, stream>>>(a, (const uint8_t *)children, (uint8_t *)out); break;
+    case 2: k_mip_brick<uint16_t><<<nb, BLK, 0, stream>>>(a, (const uint16_t *)children, (uint16_t *)out); break;
+    case 4: k_mip_brick<float><<<nb, BLK, 0, stream>>>(a, (const float *)children, (float *)out); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -20,6 +20,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <functional>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -184,6 +185,123 @@ struct SynthSource : Source {
                         }
                 }
         return GC_BRICK_SUCCESS;
+    }
+};
+
+// Coarse bricks built on the GPU on request (mip on miss): level-0 bricks
+// come from a base source; a level-l brick is reduced on the device from the
+// level-(l-1) bricks whose inner regions hold its 2x2x2 groups, which are
+// obtained the same way, recursively, and remembered in a bounded host cache
+// (least recently used out).  The result is the brick the CPU preprocessor
+// would cut from its mip level: floor(mean) with the Mipmapper's accumulator
+// types, covering voxels included, zero outside the level.
+struct MipSource : Source {
+    Source *base = nullptr;     // borrowed
+    int device = 0;
+    uint64_t memo_cap = 0, memo_bytes = 0, built = 0;
+    std::mutex memo_m, gpu_m;
+    using Key = std::array<uint32_t, 4>;
+    std::list<Key> lru;
+    std::map<Key, std::pair<std::vector<uint8_t>, std::list<Key>::iterator>> memo;
+    hipStream_t stream = nullptr;
+    uint8_t *d_children = nullptr, *d_out = nullptr;
+    uint64_t d_cap = 0;
+    bool thread_safe() const override { return base->thread_safe(); }
+    uint32_t threads() const override { return base->threads(); }
+    ~MipSource() override
+    {
+        hipFree(d_children);
+        hipFree(d_out);
+        if (stream) hipStreamDestroy(stream);
+    }
+    bool lookup(const Key &k, void *dst)
+    {
+        std::lock_guard<std::mutex> lk(memo_m);
+        auto it = memo.find(k);
+        if (it == memo.end()) return false;
+        std::memcpy(dst, it->second.first.data(), it->second.first.size());
+        lru.splice(lru.begin(), lru, it->second.second);
+        return true;
+    }
+    void remember(const Key &k, const void *src, size_t n)
+    {
+        if (memo_cap == 0 || n > memo_cap) return;
+        std::lock_guard<std::mutex> lk(memo_m);
+        if (memo.count(k)) return;
+        lru.push_front(k);
+        auto &e = memo[k];
+        e.first.assign(static_cast<const uint8_t *>(src), static_cast<const uint8_t *>(src) + n);
+        e.second = lru.begin();
+        memo_bytes += n;
+        while (memo_bytes > memo_cap && !lru.empty()) {
+            auto old = memo.find(lru.back());
+            memo_bytes -= old->second.first.size();
+            memo.erase(old);
+            lru.pop_back();
+        }
+    }
+    int brick(uint32_t level, const uint32_t b[3], void *dst)
+    {
+        if (level == 0) {
+            const int st = base->get(0, b, dst);
+            if (st == GC_BRICK_EMPTY) std::memset(dst, 0, bb_bytes);
+            return st == GC_BRICK_ERROR ? st : GC_BRICK_SUCCESS;
+        }
+        const Key key{level, b[0], b[1], b[2]};
+        if (lookup(key, dst)) return GC_BRICK_SUCCESS;
+        const uint32_t inner = bs - 2 * cov;
+        const uint32_t *L = vc.initial_sizes[level];
+        MipBrickArgs a{};
+        a.bs = bs;
+        a.cov = cov;
+        a.inner = inner;
+        uint64_t nchild = 1;
+        for (int k = 0; k < 3; ++k) {
+            a.g0[k] = (int64_t)b[k] * inner - cov;
+            a.dims[k] = L[k];
+            const int64_t lo = std::max<int64_t>(a.g0[k], 0), hi = std::min<int64_t>(a.g0[k] + bs - 1, (int64_t)L[k] - 1);
+            if (lo > hi) {   // the brick lies outside the level
+                std::memset(dst, 0, bb_bytes);
+                return GC_BRICK_SUCCESS;
+            }
+            a.c0[k] = (uint32_t)(2 * lo / inner);
+            a.nc[k] = (uint32_t)((2 * hi + 1) / inner) - a.c0[k] + 1;
+            nchild *= a.nc[k];
+        }
+        std::vector<uint8_t> children(nchild * bb_bytes);
+        for (uint32_t z = 0; z < a.nc[2]; ++z)
+            for (uint32_t y = 0; y < a.nc[1]; ++y)
+                for (uint32_t x = 0; x < a.nc[0]; ++x) {
+                    const uint32_t c[3] = {a.c0[0] + x, a.c0[1] + y, a.c0[2] + z};
+                    const uint64_t i = x + (uint64_t)a.nc[0] * (y + (uint64_t)a.nc[1] * z);
+                    const int st = brick(level - 1, c, children.data() + i * bb_bytes);
+                    if (st == GC_BRICK_ERROR) return st;
+                }
+        {
+            std::lock_guard<std::mutex> lk(gpu_m);
+            if (hipSetDevice(device) != hipSuccess) return GC_BRICK_ERROR;
+            if (!stream && hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return GC_BRICK_ERROR;
+            if (children.size() > d_cap) {
+                hipFree(d_children);
+                d_children = nullptr;
+                if (hipMalloc((void **)&d_children, children.size()) != hipSuccess) return GC_BRICK_ERROR;
+                d_cap = children.size();
+            }
+            if (!d_out && hipMalloc((void **)&d_out, bb_bytes) != hipSuccess) return GC_BRICK_ERROR;
+            if (hipMemcpyAsync(d_children, children.data(), children.size(), hipMemcpyHostToDevice, stream) != hipSuccess ||
+                launch_mip_brick(a, bytes, d_children, d_out, stream) != hipSuccess ||
+                hipMemcpyAsync(dst, d_out, bb_bytes, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+                hipStreamSynchronize(stream) != hipSuccess)
+                return GC_BRICK_ERROR;
+            ++built;
+        }
+        remember(key, dst, bb_bytes);
+        return GC_BRICK_SUCCESS;
+    }
+    int get(uint32_t level, const uint32_t bbc[3], void *dst) override
+    {
+        if (level >= vc.nb_levels) return GC_BRICK_ERROR;
+        return brick(level, bbc, dst);
     }
 };
 
@@ -1169,6 +1287,31 @@ int gc_source_create_callback(const gc_volume_cfg *vc, gc_brick_fn fn, void *use
     s->fn = fn;
     s->user = user;
     *out = new gc_source{std::move(s)};
+    return GC_OK;
+}
+
+int gc_source_create_gpu_mip(const gc_volume_cfg *vc, gc_source *level0, int device, uint64_t memo_bytes,
+                             gc_source **out)
+{
+    if (!vc || !level0 || !out) return fail(GC_ERR_ARG, "null argument");
+    if (vc->nb_levels == 0 || vc->nb_levels > GC_MAX_LEVELS || vc->brick_size[0] <= 2 * vc->covering[0])
+        return fail(GC_ERR_CONFIG, "bad volume configuration");
+    if (vc->big_brick_size[0] != 1 || vc->big_brick_size[1] != 1 || vc->big_brick_size[2] != 1)
+        return fail(GC_ERR_CONFIG, "mip on miss needs BigBrickSize 1");
+    const Source &B = *level0->impl;
+    if (B.bs != vc->brick_size[0] || B.cov != vc->covering[0] || B.bytes != vc->encoded_bytes ||
+        B.BB[0] != 1 || B.BB[1] != 1 || B.BB[2] != 1)
+        return fail(GC_ERR_CONFIG, "level-0 source has another brick layout");
+    for (uint32_t l = 1; l < vc->nb_levels; ++l)
+        for (int a = 0; a < 3; ++a)
+            if (vc->initial_sizes[l][a] != vc->initial_sizes[l - 1][a] / 2)
+                return fail(GC_ERR_CONFIG, "level sizes do not follow the 2x2x2 reduction");
+    auto m = std::make_unique<MipSource>();
+    m->init(*vc);
+    m->base = level0->impl.get();
+    m->device = device;
+    m->memo_cap = memo_bytes;
+    *out = new gc_source{std::move(m)};
     return GC_OK;
 }
 

@@ -128,7 +128,7 @@ EXPORTS = [
     "gc_batch_create", "gc_batch_destroy", "gc_batch_set_graph", "gc_batch_render", "gc_batch_read_pixels",
     "gc_batch_set_launch", "gc_batch_begin", "gc_batch_set_view", "gc_batch_clear_view", "gc_batch_launch",
     "gc_batch_pending", "gc_batch_stats", "gc_batch_read_state", "gc_cache_set_max_requests",
-    "gc_cache_busy", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
+    "gc_cache_busy", "gc_source_create_gpu_mip", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
     "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get",
 ]
 
@@ -190,6 +190,7 @@ def lib():
             "gc_batch_launch": [P, u32, C.POINTER(C.c_double)], "gc_batch_pending": [P, P],
             "gc_batch_stats": [P, C.POINTER(RenderStats)], "gc_batch_read_state": [P, u32, P, P],
             "gc_cache_busy": [P, C.POINTER(C.c_int)],
+            "gc_source_create_gpu_mip": [C.POINTER(VolumeCfg), P, C.c_int, u64, C.POINTER(vp)],
             "gc_server_create": [C.POINTER(ServerCfg), C.c_int, C.POINTER(vp)], "gc_server_destroy": [P],
             "gc_server_add_volume": [P, C.POINTER(VolumeCfg), P, u64, u32, C.POINTER(u32)],
             "gc_server_submit": [P, u32, C.POINTER(View), P, u32, C.POINTER(u64)],
@@ -355,6 +356,13 @@ class Source:
         h = vp()
         check(lib().gc_source_create_synthetic(C.byref(vc), C.byref(params), C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def gpu_mip(cls, vc, level0, device=0, memo_bytes=256 << 20):
+        """Mip on miss over a level-0 source (kept alive with this source)."""
+        h = vp()
+        check(lib().gc_source_create_gpu_mip(C.byref(vc), level0.h, device, memo_bytes, C.byref(h)))
+        return cls(h, keep=level0)
 
     def get(self, vc, level, bb_coord):
         bb = [vc.big_brick_size[a] for a in range(3)]

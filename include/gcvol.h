@@ -155,6 +155,15 @@ int gc_source_create_synthetic(const gc_volume_cfg *vc, const gc_synth_params *p
                                gc_source **out);
 int gc_source_create_callback(const gc_volume_cfg *vc, gc_brick_fn fn, void *user,
                               gc_source **out);
+/* Mip on miss: level-0 bricks from `level0` (same brick size, covering and
+ * voxel type, BigBrickSize 1); every coarser brick is built on `device` by
+ * 2x2x2 reduction of the finer bricks that cover its region, recursively,
+ * bit-exact to the CPU preprocessor (Mipmapper.cu:30-105 + Bricker_v2 cut,
+ * covering voxels included).  Built bricks are kept in a host cache of at
+ * most memo_bytes.  The level sizes must halve (floor) from level to level.
+ * No reference counterpart (the reference preprocesses every level offline). */
+int gc_source_create_gpu_mip(const gc_volume_cfg *vc, gc_source *level0, int device,
+                             uint64_t memo_bytes, gc_source **out);
 int gc_source_destroy(gc_source *src);
 /* Fetch one big brick from a source (for tests / tools). */
 int gc_source_get(gc_source *src, uint32_t level, const uint32_t big_brick[3], void *dst,
