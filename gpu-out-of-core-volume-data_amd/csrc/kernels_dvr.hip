@@ -508,11 +508,13 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
                 const uint32_t mcx = lcoord + bx;
                 if (mcx != pcx || by != pcy || bz != pcz) {
                     pcx = mcx; pcy = by; pcz = bz;
-                    const uint4 e = p.mrpd[mcx + (uint64_t)MX * (by + (uint64_t)MY * bz)];
+                    // the page directory holds fewer than 2^32 cells (checked at creation)
+                    const uint4 e = p.mrpd[mcx + MX * (by + MY * bz)];
                     ew = e.w;
                     ++n_visits;
+                    // pool shapes hold fewer than 2^24 slots (checked at creation)
                     slot = bs_shift != NONE
-                               ? (e.x >> bs_shift) + cx_ * ((e.y >> bs_shift) + cy_ * (e.z >> bs_shift))
+                               ? (e.x >> bs_shift) + __umul24(cx_, (e.y >> bs_shift) + __umul24(cy_, e.z >> bs_shift))
                                : e.x / bs + cx_ * (e.y / bs + cy_ * (e.z / bs));
                     bbase = pool + (uint64_t)slot * bbytes + corner;
                     vkey = NONE;
@@ -540,7 +542,8 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
                 const uint32_t vx = (uint32_t)(pv.x * vvf.x);
                 const uint32_t vy = (uint32_t)(pv.y * vvf.y);
                 const uint32_t vz = (uint32_t)(pv.z * vvf.z);
-                uint32_t lx = vx - bx * inner, ly = vy - by * inner, lz = vz - bz * inner;
+                // 24-bit products: voxel and brick indices stay far below 2^24
+                uint32_t lx = vx - __umul24(bx, inner), ly = vy - __umul24(by, inner), lz = vz - __umul24(bz, inner);
                 if (__builtin_expect(lx >= inner || ly >= inner || lz >= inner, 0)) {
                     lx = vx % inner;
                     ly = vy % inner;
@@ -548,11 +551,11 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
                 }
                 ACC s;
                 if (fast_fetch) {
-                    const uint32_t key = (lz * bs + ly) * bs + lx + (lod << 24);
+                    const uint32_t vi = __umul24(__umul24(lz, bs) + ly, bs) + lx;   // voxel in the brick
+                    const uint32_t key = vi + (lod << 24);
                     if (key != vkey) {
                         vkey = key;
-                        const uint32_t sum = box_sum<BYTES == 4 ? 1 : BYTES>(
-                            bbase + ((lz * bs + ly) * bs + lx) * BYTES, row, plane);
+                        const uint32_t sum = box_sum<BYTES == 4 ? 1 : BYTES>(bbase + vi * BYTES, row, plane);
                         if constexpr (sizeof(ACC) == 8) vs = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
                         else vs = (float)sum * inv_norm;
                     }

@@ -495,6 +495,7 @@ int Cache::create()
     GC_HIP(hipSetDevice(device));
     compute_geometry(vc, cc, g);
     if (g.mrpd_cells >= (1ull << 32)) return fail(GC_ERR_CONFIG, "page directory exceeds 2^32 cells");
+    if (g.nslots >= (1u << 24)) return fail(GC_ERR_CONFIG, "pool exceeds 2^24 brick slots");
     cap = cc.max_requests ? cc.max_requests : 50;
     cap = std::min(cap, g.nslots);
     limit = cap;
