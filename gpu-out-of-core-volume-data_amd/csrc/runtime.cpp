@@ -80,6 +80,21 @@ static void parallel_for(uint32_t n, uint32_t threads, const std::function<void(
     if (take) g_thread_tokens.fetch_add(take);
 }
 
+// parallel_for over n items in chunks of `chunk` consecutive items:
+// done(c) runs once per chunk, on the thread that finishes its last item.
+static void parallel_for_chunks(uint32_t n, uint32_t threads, uint32_t chunk,
+                                const std::function<void(uint32_t)> &fn,
+                                const std::function<void(uint32_t)> &done)
+{
+    const uint32_t nc = (n + chunk - 1) / chunk;
+    std::unique_ptr<std::atomic<uint32_t>[]> left(new std::atomic<uint32_t>[nc]);
+    for (uint32_t c = 0; c < nc; ++c) left[c] = std::min(chunk, n - c * chunk);
+    parallel_for(n, threads, [&](uint32_t i) {
+        fn(i);
+        if (left[i / chunk].fetch_sub(1) == 1) done(i / chunk);
+    });
+}
+
 static uint32_t hw_threads()
 {
     const uint32_t h = std::thread::hardware_concurrency();
@@ -753,7 +768,22 @@ int Handler::handle(const std::vector<uint32_t> &list)
         st[u] = s;
     };
     const uint32_t nth = S.thread_safe() ? std::max(1u, S.threads()) : 1u;
-    parallel_for(nu, nth, fetch);
+    // Bricks go up on the side stream in chunks as soon as a chunk is
+    // fetched, so the upload overlaps the fetches of the later chunks.  The
+    // device staging buffer is rewritten only after the last load consumed it.
+    GC_HIP(hipStreamWaitEvent(c->upload, c->consumed, 0));
+    const uint32_t chunk = 64;
+    std::atomic<int> up_err{(int)hipSuccess};
+    parallel_for_chunks(nu, nth, chunk, fetch, [&](uint32_t k) {
+        const uint64_t u0 = (uint64_t)k * chunk, nb = std::min<uint64_t>(chunk, nu - u0);
+        hipSetDevice(c->device);
+        const hipError_t e = hipMemcpyAsync(c->d_staging + u0 * S.bb_bytes, c->h_staging + u0 * S.bb_bytes,
+                                            nb * S.bb_bytes, hipMemcpyHostToDevice, c->upload);
+        if (e != hipSuccess) up_err = (int)e;
+    });
+    GC_HIP(hipEventRecord(c->uploaded, c->upload));
+    upload_pending = true;
+    if (up_err != (int)hipSuccess) GC_HIP((hipError_t)up_err.load());
     for (uint32_t u = 0; u < nu; ++u)
         if (st[u] == GC_BRICK_ERROR)
             return fail(GC_ERR_SOURCE, "brick source failed for level " + std::to_string(uniq[u][3]));
@@ -772,11 +802,7 @@ int Handler::handle(const std::vector<uint32_t> &list)
     for (uint32_t i = 0; i < nreq; ++i)
         if (st[uid[i]] == GC_BRICK_EMPTY) std::memcpy(asked + 4 * k++, &list[4 * i], 16);
     for (uint32_t u = 0; u < nu; ++u) std::memcpy(bbc + 4 * u, uniq[u].data(), 16);
-    // upload on the side stream once the previous load has consumed the
-    // device staging buffer
-    GC_HIP(hipStreamWaitEvent(c->upload, c->consumed, 0));
     GC_HIP(hipMemcpyAsync(c->d_meta, c->h_meta, (size_t)cap * 9 * 4, hipMemcpyHostToDevice, c->upload));
-    GC_HIP(hipMemcpyAsync(c->d_staging, c->h_staging, (size_t)nu * S.bb_bytes, hipMemcpyHostToDevice, c->upload));
     GC_HIP(hipEventRecord(c->uploaded, c->upload));
     upload_pending = true;
     n = nreq;
