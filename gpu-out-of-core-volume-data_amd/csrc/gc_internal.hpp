@@ -191,6 +191,8 @@ struct DvrLaunch {
     gc_render_stats *stats;
     uint32_t refill_min;        // refill idle lanes once this many are idle
     uint32_t march;             // samples per lane between refill checks (1..8)
+    uint32_t pipelined;         // 1: software-pipelined march (requires realDivVirtual == 1 on every
+                                //    level and MRPD y*z cells, x cells below 2^24; see launch_base)
 };
 
 // Tile shape of a ray packet: 0 8x8, 1 16x4, 2 32x2, 3 64x1.

@@ -32,6 +32,16 @@
 //    the current level) reuse the box sum: those lanes issue no loads, which
 //    is what bounds close-up views, where the memory pipeline saturates.
 //
+// Software pipelining (u8/u16 voxels with covering >= 1, realDivVirtual == 1,
+// the configurations every bricked level gives).  Each ray keeps three
+// samples in flight: the page entry of sample j+2 and the footprint of sample
+// j+1 are loading while sample j is classified and composited.  Every
+// half-step issues exactly one entry load and four footprint loads, for every
+// lane (lanes that need none load one wave-uniform address), so the
+// compiler's in-order waits land one half-step after each load instead of
+// right after it.  Side effects (usage/request stamps, counters, stored
+// state) happen only when a sample is consumed, in the reference's order.
+//
 // Exactness of the sample trace.  Ray set-up, positions, LOD choice and every
 // page-table / voxel index are FP32 in the reference's operation order (no
 // contraction, IEEE divide and square root).  The LOD rewrite keeps that trace
@@ -163,6 +173,62 @@ __device__ inline uint32_t box_sum(const uint8_t *a, uint32_t row, uint32_t plan
     }
 }
 
+// The same footprint as four raw words, loaded without waiting for them (the
+// pipelined march issues them one sample ahead of their use).
+struct Quad {
+    uint32_t a, b, c, d;
+};
+template <int BYTES>
+__device__ inline Quad box_load(const uint8_t *a, uint32_t row, uint32_t plane)
+{
+    Quad q;
+    if constexpr (BYTES == 1) {
+        uint16_t h0, h1, h2, h3;
+        __builtin_memcpy(&h0, a, 2);
+        __builtin_memcpy(&h1, a + row, 2);
+        __builtin_memcpy(&h2, a + plane, 2);
+        __builtin_memcpy(&h3, a + plane + row, 2);
+        q = Quad{h0, h1, h2, h3};
+    } else {
+        __builtin_memcpy(&q.a, a, 4);
+        __builtin_memcpy(&q.b, a + row, 4);
+        __builtin_memcpy(&q.c, a + plane, 4);
+        __builtin_memcpy(&q.d, a + plane + row, 4);
+    }
+    return q;
+}
+template <int BYTES>
+__device__ inline uint32_t box_add(const Quad &q)
+{
+    if constexpr (BYTES == 1) {
+        uint32_t s = __builtin_amdgcn_sad_u8(q.a, 0u, 0u);
+        s = __builtin_amdgcn_sad_u8(q.b, 0u, s);
+        s = __builtin_amdgcn_sad_u8(q.c, 0u, s);
+        return __builtin_amdgcn_sad_u8(q.d, 0u, s);
+    } else {
+        uint32_t s = __builtin_amdgcn_sad_u16(q.a, 0u, 0u);
+        s = __builtin_amdgcn_sad_u16(q.b, 0u, s);
+        s = __builtin_amdgcn_sad_u16(q.c, 0u, s);
+        return __builtin_amdgcn_sad_u16(q.d, 0u, s);
+    }
+}
+
+// Views keep their buffer pointers in LDS; accesses through them are made
+// explicitly global (a flat access would make every later wait on vector
+// memory wait for everything in flight).
+typedef float gf4 __attribute__((ext_vector_type(4)));
+typedef double gd4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T *gp(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T *gp(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
 // General texel read in the reference's 3-D pool coordinates (covering 0, or
 // float voxels): x, y clamp, z wraps (Texture3dDevice.hpp:104-122).
 template <int BYTES>
@@ -208,8 +274,8 @@ __device__ inline uint32_t wave_sum(uint32_t v)
 
 // Opacity correction and premultiplication of an interpolated TF sample
 // (RayCasters.cu:163-175).  skip: alpha <= 0.001.
-template <typename ACC>
-__device__ inline Acc4<ACC> shade(ACC a, float4 T0, float4 T1, float kexp, bool &skip)
+template <typename ACC, typename T4>
+__device__ inline Acc4<ACC> shade(ACC a, T4 T0, T4 T1, float kexp, bool &skip)
 {
     Acc4<ACC> col;
     if constexpr (sizeof(ACC) == 8) {
@@ -230,8 +296,8 @@ __device__ inline Acc4<ACC> shade(ACC a, float4 T0, float4 T1, float kexp, bool 
 }
 
 // tex1D with linear filtering and clamp addressing over N entries.
-template <typename ACC>
-__device__ inline Acc4<ACC> classify(ACC s, const float4 *TF, uint32_t N, float kexp, bool &skip)
+template <typename ACC, typename TFP>
+__device__ inline Acc4<ACC> classify(ACC s, TFP TF, uint32_t N, float kexp, bool &skip)
 {
     const ACC xf = s * (ACC)N - (ACC)0.5;
     const ACC fl = floor(xf);
@@ -271,7 +337,7 @@ __device__ inline uint32_t fetch_packet(const DvrLaunch &p, uint32_t n_packets, 
 }
 
 template <int BYTES, typename ACC, int TW, int TH>
-__global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
+__global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : 4))) void k_dvr(const DvrLaunch p)
 {
     static_assert(TW * TH == 64, "one wave per ray packet");
     __shared__ LevelLds lv[GC_MAX_LEVELS];
@@ -311,7 +377,9 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
     }
     __syncthreads();
     // packets: first the ray list (64 entries each), then the views' tiles
-    const uint32_t list_count = p.list ? *p.list_count : 0u;
+    // wave-uniform values read from memory go to scalar registers, waited for
+    // here rather than inside the march loop
+    const uint32_t list_count = __builtin_amdgcn_readfirstlane(p.list ? *p.list_count : 0u);
     const uint32_t list_packets = (list_count + 63u) / 64u;
     const uint32_t n_packets = list_packets + p.n_packets;
 
@@ -330,7 +398,7 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
     const bool fast_fetch = BYTES != 4 && cov >= 1;
     const ACC inv_norm = BYTES == 1 ? (ACC)(1.0 / 2040.0) : (ACC)(1.0 / 524280.0);
     const float tstep = p.tstep;
-    const uint32_t ts = p.ts_dev ? *p.ts_dev : p.ts;
+    const uint32_t ts = __builtin_amdgcn_readfirstlane(p.ts_dev ? *p.ts_dev : p.ts);
     const int march = (int)p.march;
 
     // wave-uniform packet stream: the current packet and its next free slot
@@ -365,6 +433,93 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
         kexp = L.kexp;
         dstep = dir * step;
         tnext = l + 1 < nl ? tsw[(l + 1) * LANES + tid] : FINF;
+    };
+
+    // ------------------------------------------------- pipelined march state
+    // Three samples of a ray are in flight: the front sample j+2 (its page
+    // entry is being loaded), sample j+1 (its box footprint is being loaded)
+    // and sample j (consumed: side effects, classification, compositing).
+    // Every half-step issues exactly one page-entry load and four footprint
+    // loads, so each value is waited for one half-step after it was asked
+    // for, with the five newer loads still in flight.  Side effects (usage and
+    // request stamps, counters, stored state) happen only when a sample is
+    // consumed, in the reference's order, so the trace is the one-sample-at-
+    // a-time trace.  The page cell doubles as the request cell
+    // (realDivVirtual == 1).
+    const bool piped = fast_fetch && p.pipelined != 0;
+    const float inv_bs = 1.0f / (float)bs;
+    // hb: stage B holds a sample (the front placed last half-step); hc: stage
+    // C holds one; newray: the next sample placed is the ray's first
+    bool hb = false, hc = false, newray = false;
+    uint32_t cellB = 0, viB = 0, cellC = 0, slotC = 0, ewC = 0, viC = 0, sum_prev = 0;
+    float kB = 0.f, tC = 0.f, kC = 0.f;
+    bool freshB = false, freshC = false, reuseC = false;
+    uint4 eX = make_uint4(0u, 0u, 0u, 0u), eY = eX;
+    Quad wX{0u, 0u, 0u, 0u}, wY = wX;
+
+    // page cell and in-brick voxel of the front sample (get_normalized,
+    // K_CacheManager.hpp:267-361, with realDivVirtual == 1)
+    auto locate = [&](uint32_t &cell, uint32_t &vi) {
+        F3 tp = pos * 0.495f + f3(0.495f, 0.495f, 0.495f);
+        tp.z = 1.0f - tp.z;
+        const F3 np = f3(clamp099(tp.x), clamp099(tp.y), clamp099(tp.z));
+        const F3 pr = np * ior;
+        const uint32_t bx = (uint32_t)(pr.x * ldf.x);
+        const uint32_t by = (uint32_t)(pr.y * ldf.y);
+        const uint32_t bz = (uint32_t)(pr.z * ldf.z);
+        // MRPD x, y*z extents below 2^24 (checked on the host)
+        cell = lcoord + bx + __umul24(MX, by + __umul24(MY, bz));
+        const uint32_t vx = (uint32_t)(pr.x * vvf.x);
+        const uint32_t vy = (uint32_t)(pr.y * vvf.y);
+        const uint32_t vz = (uint32_t)(pr.z * vvf.z);
+        uint32_t lx = vx - __umul24(bx, inner), ly = vy - __umul24(by, inner), lz = vz - __umul24(bz, inner);
+        if (__builtin_expect(lx >= inner || ly >= inner || lz >= inner, 0)) {
+            lx = vx % inner;
+            ly = vy % inner;
+            lz = vz % inner;
+        }
+        vi = __umul24(__umul24(lz, bs) + ly, bs) + lx;
+    };
+    // pool slot of a page entry (entry xyz = the slot's voxel origin in the
+    // reference's 3-D pool; coordinates are multiples of bs below 2^24)
+    auto slot_of = [&](const uint4 &e) -> uint32_t {
+        if (bs_shift != NONE)
+            return (e.x >> bs_shift) + __umul24(cx_, (e.y >> bs_shift) + __umul24(cy_, e.z >> bs_shift));
+        const uint32_t qx = (uint32_t)__builtin_rintf((float)e.x * inv_bs);
+        const uint32_t qy = (uint32_t)__builtin_rintf((float)e.y * inv_bs);
+        const uint32_t qz = (uint32_t)__builtin_rintf((float)e.z * inv_bs);
+        return qx + __umul24(cx_, qy + __umul24(cy_, qz));
+    };
+    // the progressive state and the pixel of a ray that ends or stops
+    // (RayCasters.cu:138-147,189-197)
+    auto store_state = [&](float tt) {
+        const DvrView &V = vw[v];
+        gp(V.tbuf)[pix] = tt;
+        if constexpr (sizeof(ACC) == 8)
+            gp(reinterpret_cast<gd4 *>(V.cstate))[pix] = gd4{fc.x, fc.y, fc.z, fc.w};
+        else
+            gp(reinterpret_cast<gf4 *>(V.cstate))[pix] = gf4{fc.x, fc.y, fc.z, fc.w};
+        gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
+    };
+    // Classification and front-to-back blending of one mapped sample
+    // (RayCasters.cu:161-178).
+    auto composite = [&](ACC s, float kx) {
+        bool cskip;
+        const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(s, tfl, tf_n0, kx, cskip)
+                                    : classify<ACC>(s, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size,
+                                                    kx, cskip);
+        if (!cskip) {
+            const ACC wgt = (ACC)1 - fc.w;
+            if constexpr (sizeof(ACC) == 8) {
+                const ACC tx = cc.x * wgt, ty = cc.y * wgt, tz = cc.z * wgt, tw = cc.w * wgt;
+                fc.x += tx; fc.y += ty; fc.z += tz; fc.w += tw;
+            } else {
+                fc.x = fmaf(cc.x, wgt, fc.x);
+                fc.y = fmaf(cc.y, wgt, fc.y);
+                fc.z = fmaf(cc.z, wgt, fc.z);
+                fc.w = fmaf(cc.w, wgt, fc.w);
+            }
+        }
     };
 
     for (;;) {
@@ -446,18 +601,18 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
                     tfar = fminf(fminf(tmax.x, tmax.y), fminf(tmax.x, tmax.z));
                     const bool hit = tfar > tnear;
                     pix = y * W + x;
-                    tnear = fmaxf(tnear, V.tbuf[pix]);
+                    tnear = fmaxf(tnear, gp(V.tbuf)[pix]);
                     if constexpr (sizeof(ACC) == 8) {
-                        const double4 c = reinterpret_cast<const double4 *>(V.cstate)[pix];
+                        const gd4 c = gp(reinterpret_cast<const gd4 *>(V.cstate))[pix];
                         fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
                     } else {
-                        const float4 c = reinterpret_cast<const float4 *>(V.cstate)[pix];
+                        const gf4 c = gp(reinterpret_cast<const gf4 *>(V.cstate))[pix];
                         fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
                     }
                     counted = true;
                     if (!hit) {
                         fc.x = fc.y = fc.z = (ACC)0;
-                        V.pixels[pix] = pack(fc.x, fc.y, fc.z, fc.w);
+                        gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
                     } else {
                         if (tnear < 0.f) tnear = 0.f;
                         t = tnear;
@@ -479,6 +634,8 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
                         set_level(l0);
                         pcx = pcy = pcz = NONE;
                         vkey = NONE;
+                        hb = hc = false;
+                        newray = true;
                         have = true;
                     }
                 }
@@ -492,7 +649,119 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
 
         // ------------------------------------------------------------- march
         bool stopped = false;   // stopped at an Unmapped brick in this round
-        if (have) {
+        if constexpr (BYTES != 4) {
+            if (piped) {
+                // One half-step: issue the page entry of the new front sample
+                // into e_new and the footprint of the previous front sample
+                // into w_new; consume the sample whose footprint is w_old.
+                // Halves alternate the X/Y registers, so nothing in flight is
+                // ever copied.
+                auto half = [&](uint4 &e_new, const uint4 &e_old, Quad &w_new, const Quad &w_old) {
+                    // A: place the front sample (the reference's advance by
+                    // the step of the level of the sample before it)
+                    const float t_b = t;
+                    const F3 pos_b = pos;
+                    const uint32_t lod_b = lod;
+                    if (hb) {
+                        t += step;
+                        pos = pos + dstep;
+                    }
+                    while (t >= tnext) set_level(++lod);
+                    uint32_t cellA, viA;
+                    locate(cellA, viA);
+                    const bool freshA = newray || cellA != cellB;
+                    // A lane that stays in its brick needs no new entry: it
+                    // loads the wave's first cell instead (one shared line)
+                    // and B keeps the entry it already has.
+                    e_new = p.mrpd[freshA ? cellA : (uint32_t)__builtin_amdgcn_readfirstlane(cellA)];
+                    // B: footprint of the sample placed last time.  Every
+                    // lane issues the loads (so the waits stay one half-step
+                    // behind); lanes that need none (not mapped, or the same
+                    // voxel as sample j, whose box sum is kept) load the
+                    // wave's first footprint instead.
+                    const uint32_t ewB = freshB ? e_old.w : ewC;
+                    const uint32_t slotB = freshB ? slot_of(e_old) : slotC;
+                    const bool reuseB = !freshB && viB == viC;
+                    const uint64_t fr = (uint64_t)slotB * bbytes + corner + (uint64_t)viB * BYTES;
+                    // (readfirstlane returns int: widen through uint32_t,
+                    // never by sign extension, for offsets of 2 GiB and up)
+                    const uint64_t fu =
+                        ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(fr >> 32)) << 32) |
+                        (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)fr);
+                    const bool needB = hb && ewB == 1 && !reuseB;
+                    w_new = box_load<BYTES>(pool + (needB ? fr : fu), row, plane);
+                    // Empty brick: the ray advances by tstep from it
+                    // (RayCasters.cu:154-159), not by the level step.  The
+                    // front sample just placed is dropped and placed again,
+                    // from here, in the next half-step.
+                    const bool discard = hb && ewB == 2;
+                    if (discard) {
+                        t = t_b + tstep;
+                        pos = pos_b + dir * tstep;
+                        if (lod != lod_b) {
+                            lod = lod_b;
+                            set_level(lod);
+                        }
+                    }
+                    // C: consume sample j (RayCasters.cu:118-192)
+                    bool end = false;
+                    if (hc) {
+                        if (!((ACC)fc.w < (ACC)astop && tC < tfar)) {
+                            end = true;
+                        } else {
+                            if (freshC) ++n_visits;
+                            if (ewC == 0) {
+                                // raise_brick_request, then stop (Unmapped)
+                                p.request[cellC] = ts;
+                                stopped = true;
+                                end = true;
+                            } else if (ewC == 1) {
+                                if (freshC) p.usage[slotC] = ts;
+                                const uint32_t sum = reuseC ? sum_prev : box_add<BYTES>(w_old);
+                                sum_prev = sum;
+                                ACC s;
+                                if constexpr (sizeof(ACC) == 8) s = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
+                                else s = (float)sum * inv_norm;
+                                composite(s, kC);
+                                ++n_samples;
+                            }
+                        }
+                        if (end) store_state(tC);
+                    }
+                    // w_old is waited for on every path here (the main path
+                    // already has), so its registers are free when the next
+                    // half issues loads into them
+                    asm volatile("" ::"v"(w_old.a), "v"(w_old.b), "v"(w_old.c), "v"(w_old.d));
+                    // shift: B's sample moves to C (C keeps the last sample
+                    // when B had none, as the reference of "same brick")
+                    if (hb) {
+                        cellC = cellB; slotC = slotB; ewC = ewB; tC = t_b; kC = kB; freshC = freshB;
+                        viC = viB; reuseC = reuseB;
+                    }
+                    hc = hb;
+                    hb = have && !discard;
+                    if (!discard) {
+                        cellB = cellA; viB = viA; kB = kexp; freshB = freshA;
+                        newray = false;
+                    }
+                    if (end) {
+                        have = false;
+                        hb = hc = false;
+                    }
+                };
+                // Every lane runs every half-step: a lane without a ray
+                // places and loads harmlessly (its page cell and footprint
+                // stay in range; all side effects need hb/hc), and the loop
+                // exits on a wave-uniform test after whole pairs, so the
+                // waits above hold on every path.
+                for (int k = 0; k < march; k += 2) {
+                    half(eX, eY, wX, wY);
+                    half(eY, eX, wY, wX);
+                    if (__ballot(have) == 0) break;
+                }
+            }
+        }
+        if (!piped && have) {
             bool done = false;
             for (int k = 0; k < march; ++k) {
                 if (!((ACC)fc.w < (ACC)astop && t < tfar)) { done = true; break; }
@@ -581,35 +850,13 @@ __global__ __launch_bounds__(LANES) void k_dvr(const DvrLaunch p)
                         else s = (float)dsum * inv_norm;
                     }
                 }
-                bool cskip;
-                const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(s, tfl, tf_n0, kexp, cskip)
-                                            : classify<ACC>(s, vw[v].tf, vw[v].tf_size, kexp, cskip);
+                composite(s, kexp);
                 ++n_samples;
-                if (!cskip) {
-                    // front-to-back blending (RayCasters.cu:176-178)
-                    const ACC wgt = (ACC)1 - fc.w;
-                    if constexpr (sizeof(ACC) == 8) {
-                        const ACC tx = cc.x * wgt, ty = cc.y * wgt, tz = cc.z * wgt, tw = cc.w * wgt;
-                        fc.x += tx; fc.y += ty; fc.z += tz; fc.w += tw;
-                    } else {
-                        fc.x = fmaf(cc.x, wgt, fc.x);
-                        fc.y = fmaf(cc.y, wgt, fc.y);
-                        fc.z = fmaf(cc.z, wgt, fc.z);
-                        fc.w = fmaf(cc.w, wgt, fc.w);
-                    }
-                }
                 t += step;
                 pos = pos + dstep;
             }
             if (done || stopped) {
-                // store the progressive state and the pixel (RayCasters.cu:138-147,189-197)
-                const DvrView &V = vw[v];
-                V.tbuf[pix] = t;
-                if constexpr (sizeof(ACC) == 8)
-                    reinterpret_cast<double4 *>(V.cstate)[pix] = make_double4(fc.x, fc.y, fc.z, fc.w);
-                else
-                    reinterpret_cast<float4 *>(V.cstate)[pix] = make_float4(fc.x, fc.y, fc.z, fc.w);
-                V.pixels[pix] = pack(fc.x, fc.y, fc.z, fc.w);
+                store_state(t);
                 have = false;
             }
         }

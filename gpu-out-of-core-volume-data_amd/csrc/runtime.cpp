@@ -852,6 +852,14 @@ static void launch_base(const Cache &C, DvrLaunch &L)
         L.lod_step[i] = (smin / (float)m) / 2.f;
         L.kexp[i] = L.lod_step[i] / L.tstep;
     }
+    // The pipelined march takes the request cell to be the page cell (true when
+    // realDivVirtual is exactly 1, which the bricker's padded level sizes give)
+    // and forms page indices with 24-bit multiplies.
+    bool rdv_one = true;
+    for (uint32_t l = 0; l < C.vc.nb_levels; ++l)
+        for (int a = 0; a < 3; ++a) rdv_one = rdv_one && C.g.rdv[l][a] == 1.0f;
+    L.pipelined = rdv_one && C.g.mrpd_dim[0] < (1u << 24) &&
+                  (uint64_t)C.g.mrpd_dim[1] * C.g.mrpd_dim[2] < (1u << 24);
 }
 
 static void view_of(const gc_view &v, const float *d_tf, uint32_t tf_size, const Frame &F, DvrView &out)

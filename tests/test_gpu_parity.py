@@ -147,3 +147,50 @@ def test_out_of_core_sequence(gpu, bytes_, tile):
             if gd:
                 break
     assert steps > 3
+
+
+def _max_mapped_offset(gc, brick_bytes):
+    """Largest pool byte offset of a mapped brick (MRPD entry w == 1 holds the
+    slot's voxel origin in the 3-D pool)."""
+    m = gc.mrpd()
+    e = m[m[:, 3] == 1]
+    bs = int(gc.g.brick_size)
+    cx, cy = int(gc.g.cache_size[0]), int(gc.g.cache_size[1])
+    slots = e[:, 0] // bs + cx * (e[:, 1] // bs + cy * (e[:, 2] // bs))
+    return int(slots.max()) * brick_bytes
+
+
+@pytest.mark.parametrize("pool_shape", [(80, 80, 72), (96, 96, 93)])
+def test_large_pool_offsets_match_small_pool(gpu, pool_shape):
+    """Pools of 3.5 and 6.5 GiB: the first bricks loaded take the last slots
+    (victims are the tail of the LRU), at byte offsets whose low 32 bits have
+    bit 31 set, below and above 4 GiB.  Frames rendered through them must be
+    bit-identical (pixels, t buffer, counters) to the same frames through an
+    exactly sized pool, at both compositing precisions."""
+    sc = Scene((64, 64, 64), 16, 1, 2, max_requests=64)
+    brick_bytes = 16 ** 3 * 2
+    small = sc.gpu_cache()
+    src = G.Source.pyramid(sc.vc, sc.levels)
+    big = G.Cache(sc.vc, G.cache_cfg(pool_shape, 16, max_requests=64, sync=True), src)
+    tf = tf_table(seed=17)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 96, 64
+    for prec in (32, 64):
+        for k in (1, 4):
+            out = []
+            for c in (small, big):
+                f = G.Frame(W, H, prec)
+                r = G.render_frame(c, G.make_view(W, H, orbit(k), alpha_stop=0.99, precision=prec), d_tf, len(tf), f)
+                tb, cs = f.state()
+                out.append((f.pixels().copy(), tb.copy(), cs.copy(), r.as_dict()))
+                f.close()
+            (p0, t0, c0, r0), (p1, t1, c1, r1) = out
+            assert r0["complete"] and r1["complete"]
+            for key in ("mapped_samples", "brick_visits", "pixels"):
+                assert r0[key] == r1[key], key
+            assert np.array_equal(p0, p1), "pixels differ (precision %d, view %d)" % (prec, k)
+            assert np.array_equal(t0, t1) and np.array_equal(c0.view(np.uint8), c1.view(np.uint8))
+    off = _max_mapped_offset(big, brick_bytes)
+    assert (off & 0x80000000) and off >= (1 << 31), "bricks did not reach the intended offsets (%d)" % off
+    big.close()
+    small.close()
