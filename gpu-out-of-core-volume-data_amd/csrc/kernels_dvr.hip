@@ -502,7 +502,25 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
     };
     // Classification and front-to-back blending of one mapped sample
-    // (RayCasters.cu:161-178).
+    // (RayCasters.cu:161-178); composite_if blends only where `on`.
+    auto composite_if = [&](bool on, ACC s, float kx) {
+        bool cskip;
+        const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(s, tfl, tf_n0, kx, cskip)
+                                    : classify<ACC>(s, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size,
+                                                    kx, cskip);
+        const bool b = on && !cskip;
+        const ACC wgt = (ACC)1 - fc.w;
+        Acc4<ACC> n;
+        if constexpr (sizeof(ACC) == 8) {
+            n = Acc4<ACC>{fc.x + cc.x * wgt, fc.y + cc.y * wgt, fc.z + cc.z * wgt, fc.w + cc.w * wgt};
+        } else {
+            n = Acc4<ACC>{fmaf(cc.x, wgt, fc.x), fmaf(cc.y, wgt, fc.y), fmaf(cc.z, wgt, fc.z), fmaf(cc.w, wgt, fc.w)};
+        }
+        fc.x = b ? n.x : fc.x;
+        fc.y = b ? n.y : fc.y;
+        fc.z = b ? n.z : fc.z;
+        fc.w = b ? n.w : fc.w;
+    };
     auto composite = [&](ACC s, float kx) {
         bool cskip;
         const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(s, tfl, tf_n0, kx, cskip)
@@ -657,32 +675,40 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 // Halves alternate the X/Y registers, so nothing in flight is
                 // ever copied.
                 auto half = [&](uint4 &e_new, const uint4 &e_old, Quad &w_new, const Quad &w_old) {
+                    // The common path is branch-free (selects): most lanes
+                    // place, fetch and consume a sample every half-step.
                     // A: place the front sample (the reference's advance by
                     // the step of the level of the sample before it)
                     const float t_b = t;
                     const F3 pos_b = pos;
                     const uint32_t lod_b = lod;
-                    if (hb) {
-                        t += step;
-                        pos = pos + dstep;
+                    {
+                        const float ta = t + step;
+                        const F3 pa = pos + dstep;
+                        t = hb ? ta : t;
+                        pos = f3(hb ? pa.x : pos.x, hb ? pa.y : pos.y, hb ? pa.z : pos.z);
                     }
-                    while (t >= tnext) set_level(++lod);
+                    if (__ballot(t >= tnext)) {
+                        while (t >= tnext) set_level(++lod);
+                    }
                     uint32_t cellA, viA;
                     locate(cellA, viA);
                     const bool freshA = newray || cellA != cellB;
                     // A lane that stays in its brick needs no new entry: it
                     // loads the wave's first cell instead (one shared line)
                     // and B keeps the entry it already has.
-                    e_new = p.mrpd[freshA ? cellA : (uint32_t)__builtin_amdgcn_readfirstlane(cellA)];
+                    const uint32_t cell_u = (uint32_t)__builtin_amdgcn_readfirstlane(cellA);
+                    e_new = p.mrpd[freshA ? cellA : cell_u];
                     // B: footprint of the sample placed last time.  Every
                     // lane issues the loads (so the waits stay one half-step
                     // behind); lanes that need none (not mapped, or the same
                     // voxel as sample j, whose box sum is kept) load the
                     // wave's first footprint instead.
+                    const uint32_t slot_e = slot_of(e_old);
                     const uint32_t ewB = freshB ? e_old.w : ewC;
-                    const uint32_t slotB = freshB ? slot_of(e_old) : slotC;
+                    const uint32_t slotB = freshB ? slot_e : slotC;
                     const bool reuseB = !freshB && viB == viC;
-                    const uint64_t fr = (uint64_t)slotB * bbytes + corner + (uint64_t)viB * BYTES;
+                    const uint64_t fr = (uint64_t)slotB * (uint32_t)bbytes + corner + viB * (uint32_t)BYTES;
                     // (readfirstlane returns int: widen through uint32_t,
                     // never by sign extension, for offsets of 2 GiB and up)
                     const uint64_t fu =
@@ -695,59 +721,60 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     // front sample just placed is dropped and placed again,
                     // from here, in the next half-step.
                     const bool discard = hb && ewB == 2;
-                    if (discard) {
-                        t = t_b + tstep;
-                        pos = pos_b + dir * tstep;
-                        if (lod != lod_b) {
-                            lod = lod_b;
-                            set_level(lod);
+                    if (__ballot(discard)) {
+                        if (discard) {
+                            t = t_b + tstep;
+                            pos = pos_b + dir * tstep;
+                            if (lod != lod_b) {
+                                lod = lod_b;
+                                set_level(lod);
+                            }
                         }
                     }
                     // C: consume sample j (RayCasters.cu:118-192)
-                    bool end = false;
-                    if (hc) {
-                        if (!((ACC)fc.w < (ACC)astop && tC < tfar)) {
-                            end = true;
-                        } else {
-                            if (freshC) ++n_visits;
-                            if (ewC == 0) {
-                                // raise_brick_request, then stop (Unmapped)
-                                p.request[cellC] = ts;
-                                stopped = true;
-                                end = true;
-                            } else if (ewC == 1) {
-                                if (freshC) p.usage[slotC] = ts;
-                                const uint32_t sum = reuseC ? sum_prev : box_add<BYTES>(w_old);
-                                sum_prev = sum;
-                                ACC s;
-                                if constexpr (sizeof(ACC) == 8) s = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
-                                else s = (float)sum * inv_norm;
-                                composite(s, kC);
-                                ++n_samples;
-                            }
-                        }
-                        if (end) store_state(tC);
+                    const bool live = hc && (ACC)fc.w < (ACC)astop && tC < tfar;
+                    const bool unmapped = live && ewC == 0;
+                    const bool cons = live && ewC == 1;
+                    n_visits += (live && freshC) ? 1u : 0u;
+                    if (unmapped) {
+                        // raise_brick_request, then stop (Unmapped)
+                        p.request[cellC] = ts;
+                        stopped = true;
                     }
+                    if (cons && freshC) p.usage[slotC] = ts;
+                    // every lane classifies (a footprint sum is always a
+                    // valid sample value); only consumed samples blend
+                    const uint32_t sum = reuseC ? sum_prev : box_add<BYTES>(w_old);
+                    sum_prev = cons ? sum : sum_prev;
+                    ACC sv;
+                    if constexpr (sizeof(ACC) == 8) sv = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
+                    else sv = (float)sum * inv_norm;
+                    composite_if(cons, sv, kC);
+                    n_samples += cons ? 1u : 0u;
+                    const bool end = (hc && !live) || unmapped;
+                    if (end) store_state(tC);
                     // w_old is waited for on every path here (the main path
                     // already has), so its registers are free when the next
                     // half issues loads into them
                     asm volatile("" ::"v"(w_old.a), "v"(w_old.b), "v"(w_old.c), "v"(w_old.d));
                     // shift: B's sample moves to C (C keeps the last sample
                     // when B had none, as the reference of "same brick")
-                    if (hb) {
-                        cellC = cellB; slotC = slotB; ewC = ewB; tC = t_b; kC = kB; freshC = freshB;
-                        viC = viB; reuseC = reuseB;
-                    }
-                    hc = hb;
+                    cellC = hb ? cellB : cellC;
+                    slotC = hb ? slotB : slotC;
+                    ewC = hb ? ewB : ewC;
+                    tC = hb ? t_b : tC;
+                    kC = hb ? kB : kC;
+                    freshC = hb ? freshB : freshC;
+                    viC = hb ? viB : viC;
+                    reuseC = hb ? reuseB : reuseC;
+                    hc = hb && !end;
+                    cellB = discard ? cellB : cellA;
+                    viB = discard ? viB : viA;
+                    kB = discard ? kB : kexp;
+                    freshB = discard ? freshB : freshA;
+                    newray = newray && discard;
+                    have = have && !end;
                     hb = have && !discard;
-                    if (!discard) {
-                        cellB = cellA; viB = viA; kB = kexp; freshB = freshA;
-                        newray = false;
-                    }
-                    if (end) {
-                        have = false;
-                        hb = hc = false;
-                    }
                 };
                 // Every lane runs every half-step: a lane without a ray
                 // places and loads harmlessly (its page cell and footprint
