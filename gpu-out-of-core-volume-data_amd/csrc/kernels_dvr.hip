@@ -336,6 +336,81 @@ __device__ inline uint32_t fetch_packet(const DvrLaunch &p, uint32_t n_packets, 
     return NONE;
 }
 
+// Ray set-up of pixel (x, y) of a view (RayCasters.cu:57-120): eye ray, the
+// neighbouring rays that give the pixel footprint, and the box interval.
+struct RaySetup {
+    F3 origin, dir, dxn, dyn, dleft, dup;
+    float tnear, tfar;
+    bool hit;
+};
+__device__ inline RaySetup setup_ray(const DvrView &V, uint32_t x, uint32_t y)
+{
+    RaySetup r;
+    const uint32_t W = V.width, H = V.height;
+    uint32_t xn = x + 1;
+    if (xn > W) xn = x - 1;
+    uint32_t yn = y + 1;
+    if (yn > H) yn = y - 1;
+    const float u = (((float)x + 0.5f) / (float)W) * 2.0f - 1.0f;
+    const float un = (((float)xn + 0.5f) / (float)W) * 2.0f - 1.0f;
+    const float vy = (((float)y + 0.5f) / (float)H) * 2.0f - 1.0f;
+    const float vn = (((float)yn + 0.5f) / (float)H) * 2.0f - 1.0f;
+    const float *M = V.m;
+    r.origin = f3(M[3], M[7], M[11]);
+    r.dir = mulm(M, normalize(f3(u, vy, -2.f)));
+    r.dxn = mulm(M, normalize(f3(un, vy, -2.f)));
+    r.dyn = mulm(M, normalize(f3(u, vn, -2.f)));
+    r.dleft = normalize(cross(r.dir, r.dyn));
+    r.dup = normalize(cross(r.dir, r.dxn));
+    // intersectBox (RayCasters.cu:506-525)
+    const F3 invr = f3(1.0f / r.dir.x, 1.0f / r.dir.y, 1.0f / r.dir.z);
+    const F3 tbot = invr * (f3(V.bmin[0], V.bmin[1], V.bmin[2]) - r.origin);
+    const F3 ttop = invr * (f3(V.bmax[0], V.bmax[1], V.bmax[2]) - r.origin);
+    const F3 tmin = f3(fminf(ttop.x, tbot.x), fminf(ttop.y, tbot.y), fminf(ttop.z, tbot.z));
+    const F3 tmax = f3(fmaxf(ttop.x, tbot.x), fmaxf(ttop.y, tbot.y), fmaxf(ttop.z, tbot.z));
+    r.tnear = fmaxf(fmaxf(tmin.x, tmin.y), fmaxf(tmin.x, tmin.z));
+    r.tfar = fminf(fminf(tmax.x, tmax.y), fminf(tmax.x, tmax.z));
+    r.hit = r.tfar > r.tnear;
+    return r;
+}
+
+// Page cell and in-brick voxel of a sample position at one level
+// (get_normalized, K_CacheManager.hpp:267-361, with realDivVirtual == 1; the
+// MRPD x and y*z extents are below 2^24, checked on the host).
+__device__ inline void locate_cell(F3 pos, F3 ior, F3 ldf, F3 vvf, uint32_t lcoord, uint32_t MX, uint32_t MY,
+                                   uint32_t inner, uint32_t bs, uint32_t &cell, uint32_t &vi)
+{
+    F3 tp = pos * 0.495f + f3(0.495f, 0.495f, 0.495f);
+    tp.z = 1.0f - tp.z;
+    const F3 np = f3(clamp099(tp.x), clamp099(tp.y), clamp099(tp.z));
+    const F3 pr = np * ior;
+    const uint32_t bx = (uint32_t)(pr.x * ldf.x);
+    const uint32_t by = (uint32_t)(pr.y * ldf.y);
+    const uint32_t bz = (uint32_t)(pr.z * ldf.z);
+    cell = lcoord + bx + __umul24(MX, by + __umul24(MY, bz));
+    const uint32_t vx = (uint32_t)(pr.x * vvf.x);
+    const uint32_t vy = (uint32_t)(pr.y * vvf.y);
+    const uint32_t vz = (uint32_t)(pr.z * vvf.z);
+    uint32_t lx = vx - __umul24(bx, inner), ly = vy - __umul24(by, inner), lz = vz - __umul24(bz, inner);
+    if (__builtin_expect(lx >= inner || ly >= inner || lz >= inner, 0)) {
+        lx = vx % inner;
+        ly = vy % inner;
+        lz = vz % inner;
+    }
+    vi = __umul24(__umul24(lz, bs) + ly, bs) + lx;
+}
+
+// Pool slot of a page entry (entry xyz = the slot's voxel origin in the
+// reference's 3-D pool; coordinates are multiples of bs below 2^24).
+__device__ inline uint32_t entry_slot(const uint4 &e, uint32_t bs_shift, float inv_bs, uint32_t cx, uint32_t cy)
+{
+    if (bs_shift != NONE) return (e.x >> bs_shift) + __umul24(cx, (e.y >> bs_shift) + __umul24(cy, e.z >> bs_shift));
+    const uint32_t qx = (uint32_t)__builtin_rintf((float)e.x * inv_bs);
+    const uint32_t qy = (uint32_t)__builtin_rintf((float)e.y * inv_bs);
+    const uint32_t qz = (uint32_t)__builtin_rintf((float)e.z * inv_bs);
+    return qx + __umul24(cx, qy + __umul24(cy, qz));
+}
+
 template <int BYTES, typename ACC, int TW, int TH>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : 4))) void k_dvr(const DvrLaunch p)
 {
@@ -460,36 +535,9 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // page cell and in-brick voxel of the front sample (get_normalized,
     // K_CacheManager.hpp:267-361, with realDivVirtual == 1)
     auto locate = [&](uint32_t &cell, uint32_t &vi) {
-        F3 tp = pos * 0.495f + f3(0.495f, 0.495f, 0.495f);
-        tp.z = 1.0f - tp.z;
-        const F3 np = f3(clamp099(tp.x), clamp099(tp.y), clamp099(tp.z));
-        const F3 pr = np * ior;
-        const uint32_t bx = (uint32_t)(pr.x * ldf.x);
-        const uint32_t by = (uint32_t)(pr.y * ldf.y);
-        const uint32_t bz = (uint32_t)(pr.z * ldf.z);
-        // MRPD x, y*z extents below 2^24 (checked on the host)
-        cell = lcoord + bx + __umul24(MX, by + __umul24(MY, bz));
-        const uint32_t vx = (uint32_t)(pr.x * vvf.x);
-        const uint32_t vy = (uint32_t)(pr.y * vvf.y);
-        const uint32_t vz = (uint32_t)(pr.z * vvf.z);
-        uint32_t lx = vx - __umul24(bx, inner), ly = vy - __umul24(by, inner), lz = vz - __umul24(bz, inner);
-        if (__builtin_expect(lx >= inner || ly >= inner || lz >= inner, 0)) {
-            lx = vx % inner;
-            ly = vy % inner;
-            lz = vz % inner;
-        }
-        vi = __umul24(__umul24(lz, bs) + ly, bs) + lx;
+        locate_cell(pos, ior, ldf, vvf, lcoord, MX, MY, inner, bs, cell, vi);
     };
-    // pool slot of a page entry (entry xyz = the slot's voxel origin in the
-    // reference's 3-D pool; coordinates are multiples of bs below 2^24)
-    auto slot_of = [&](const uint4 &e) -> uint32_t {
-        if (bs_shift != NONE)
-            return (e.x >> bs_shift) + __umul24(cx_, (e.y >> bs_shift) + __umul24(cy_, e.z >> bs_shift));
-        const uint32_t qx = (uint32_t)__builtin_rintf((float)e.x * inv_bs);
-        const uint32_t qy = (uint32_t)__builtin_rintf((float)e.y * inv_bs);
-        const uint32_t qz = (uint32_t)__builtin_rintf((float)e.z * inv_bs);
-        return qx + __umul24(cx_, qy + __umul24(cy_, qz));
-    };
+    auto slot_of = [&](const uint4 &e) -> uint32_t { return entry_slot(e, bs_shift, inv_bs, cx_, cy_); };
     // the progressive state and the pixel of a ray that ends or stops
     // (RayCasters.cu:138-147,189-197)
     auto store_state = [&](float tt) {
@@ -593,31 +641,13 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 if (valid) {
                     // ray set-up (RayCasters.cu:57-120)
                     const DvrView &V = vw[v];
-                    const uint32_t W = V.width, H = V.height;
-                    uint32_t xn = x + 1;
-                    if (xn > W) xn = x - 1;
-                    uint32_t yn = y + 1;
-                    if (yn > H) yn = y - 1;
-                    const float u = (((float)x + 0.5f) / (float)W) * 2.0f - 1.0f;
-                    const float un = (((float)xn + 0.5f) / (float)W) * 2.0f - 1.0f;
-                    const float vy = (((float)y + 0.5f) / (float)H) * 2.0f - 1.0f;
-                    const float vn = (((float)yn + 0.5f) / (float)H) * 2.0f - 1.0f;
-                    const float *M = V.m;
-                    const F3 origin = f3(M[3], M[7], M[11]);
-                    dir = mulm(M, normalize(f3(u, vy, -2.f)));
-                    const F3 dxn = mulm(M, normalize(f3(un, vy, -2.f)));
-                    const F3 dyn = mulm(M, normalize(f3(u, vn, -2.f)));
-                    const F3 dleft = normalize(cross(dir, dyn));
-                    const F3 dup = normalize(cross(dir, dxn));
-                    // intersectBox (RayCasters.cu:506-525)
-                    const F3 invr = f3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
-                    const F3 tbot = invr * (f3(V.bmin[0], V.bmin[1], V.bmin[2]) - origin);
-                    const F3 ttop = invr * (f3(V.bmax[0], V.bmax[1], V.bmax[2]) - origin);
-                    const F3 tmin = f3(fminf(ttop.x, tbot.x), fminf(ttop.y, tbot.y), fminf(ttop.z, tbot.z));
-                    const F3 tmax = f3(fmaxf(ttop.x, tbot.x), fmaxf(ttop.y, tbot.y), fmaxf(ttop.z, tbot.z));
-                    float tnear = fmaxf(fmaxf(tmin.x, tmin.y), fmaxf(tmin.x, tmin.z));
-                    tfar = fminf(fminf(tmax.x, tmax.y), fminf(tmax.x, tmax.z));
-                    const bool hit = tfar > tnear;
+                    const uint32_t W = V.width;
+                    const RaySetup R = setup_ray(V, x, y);
+                    const F3 origin = R.origin, dxn = R.dxn, dyn = R.dyn, dleft = R.dleft, dup = R.dup;
+                    dir = R.dir;
+                    float tnear = R.tnear;
+                    tfar = R.tfar;
+                    const bool hit = R.hit;
                     pix = y * W + x;
                     tnear = fmaxf(tnear, gp(V.tbuf)[pix]);
                     if constexpr (sizeof(ACC) == 8) {
@@ -912,6 +942,264 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     }
 }
 
+// ------------------------------------------------------------------------
+// Wave-per-ray march, for launches that resume a short list of rays (the
+// rays an earlier launch stopped at a miss).  Such launches hold too few rays
+// to fill the machine and last as long as their longest ray, which the
+// lane-per-ray march walks one sample per step at memory latency.  Here one
+// wave walks one ray, 64 samples per super-step:
+//  * the 64 sample positions are placed sequentially, in the reference's
+//    FP32 order (t += step, position += direction * step, level switches),
+//    and lane k keeps sample k;
+//  * every lane locates, loads and classifies its own sample in parallel;
+//  * the samples are then consumed in order, front to back, by a scan that
+//    applies the reference's termination tests before each sample; page
+//    visits, usage and request stamps and counters cover exactly the
+//    consumed samples, so the trace equals the one-sample-at-a-time trace.
+// Same configurations as the pipelined march (u8/u16 voxels, covering >= 1,
+// realDivVirtual == 1).  Empty bricks end a super-step (the reference steps
+// over them by tstep, which moves every later sample).
+template <typename ACC>
+__device__ inline ACC lane_read(ACC x, uint32_t k)
+{
+    if constexpr (sizeof(ACC) == 8) {
+        const uint64_t b = __builtin_bit_cast(uint64_t, x);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)k);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)k);
+        return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    } else {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), (int)k));
+    }
+}
+
+template <int BYTES, typename ACC>
+__global__ __launch_bounds__(LANES) void k_dvr_ray(const DvrLaunch p)
+{
+    __shared__ LevelLds lv[GC_MAX_LEVELS];
+    __shared__ float4 tfl[TF_LDS + 2];
+    __shared__ float tswv[WAVES][GC_MAX_LEVELS + 1];
+    extern __shared__ __align__(16) uint8_t dyn_lds[];
+    DvrView *vw = reinterpret_cast<DvrView *>(dyn_lds);
+    const uint32_t nl = p.nb_lods;
+    const int tid = threadIdx.x;
+    const uint32_t lane = tid & 63, wv = (uint32_t)tid >> 6;
+    const gc_geometry *g = p.geom;
+    if (tid < (int)nl) {
+        LevelLds &L = lv[tid];
+        for (int a = 0; a < 3; ++a) {
+            L.ior[a] = g->ior[tid][a];
+            L.rdv[a] = g->rdv[tid][a];
+            L.ldf[a] = (float)g->level_dim[tid][a];
+            L.vvf[a] = (float)g->virt_vox[tid][a];
+            L.inv[a] = p.inv_levels[tid][a];
+        }
+        L.lcoord = g->level_coord[tid];
+        L.step = p.lod_step[tid];
+        L.kexp = p.kexp[tid];
+    }
+    const uint32_t n_views = p.views ? p.n_views : 1u;
+    if (p.views) {
+        for (uint32_t i = tid; i < n_views; i += LANES) vw[i] = p.views[i];
+    } else if (tid == 0) {
+        vw[0] = p.view0;
+    }
+    __syncthreads();
+    const uint32_t tf_n0 = __builtin_amdgcn_readfirstlane(vw[0].tf_size);
+    const bool tf_lds = n_views == 1 && tf_n0 <= (uint32_t)TF_LDS;
+    if (tf_lds) {
+        const int N = (int)vw[0].tf_size;
+        for (int i = tid; i < N + 2; i += LANES) tfl[i] = vw[0].tf[i < 1 ? 0 : (i > N ? N - 1 : i - 1)];
+    }
+    __syncthreads();
+
+    const uint32_t list_count = __builtin_amdgcn_readfirstlane(*p.list_count);
+    const uint32_t bs = g->brick_size, cov = g->covering, inner = g->inner;
+    const uint32_t cx_ = g->cache_size[0], cy_ = g->cache_size[1];
+    const uint32_t MX = g->mrpd_dim[0], MY = g->mrpd_dim[1];
+    const uint32_t bs_shift = (bs & (bs - 1)) == 0 ? (uint32_t)__builtin_ctz(bs) : NONE;
+    const float inv_bs = 1.0f / (float)bs;
+    const uint32_t bbytes = bs * bs * bs * BYTES;
+    const uint32_t row = bs * BYTES, plane = bs * bs * BYTES;
+    const uint8_t *pool = static_cast<const uint8_t *>(p.pool);
+    const uint32_t corner = (cov - 1) * (plane + row + BYTES);
+    const ACC inv_norm = BYTES == 1 ? (ACC)(1.0 / 2040.0) : (ACC)(1.0 / 524280.0);
+    const float tstep = p.tstep;
+    const uint32_t ts = __builtin_amdgcn_readfirstlane(p.ts_dev ? *p.ts_dev : p.ts);
+    float *tsw = tswv[wv];
+    uint32_t n_samples = 0, n_visits = 0, n_rays = 0, n_unmapped = 0;   // wave-uniform
+
+    for (;;) {
+        uint32_t r = 0;
+        if (lane == 0) r = atomicAdd(&p.queue[0], 1u);
+        r = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(r, 0, 64));
+        if (r >= list_count) break;
+        const uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)p.list[r]);
+        const uint32_t v = e >> DVR_PIXEL_BITS;
+        const uint32_t pix = e & ((1u << DVR_PIXEL_BITS) - 1u);
+        const DvrView &V = vw[v];
+        const uint32_t W = V.width;
+        const RaySetup R = setup_ray(V, pix % W, pix / W);
+        float tnear = fmaxf(R.tnear, gp(V.tbuf)[pix]);
+        Acc4<ACC> fc;
+        if constexpr (sizeof(ACC) == 8) {
+            const gd4 c = gp(reinterpret_cast<const gd4 *>(V.cstate))[pix];
+            fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
+        } else {
+            const gf4 c = gp(reinterpret_cast<const gf4 *>(V.cstate))[pix];
+            fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
+        }
+        ++n_rays;
+        auto store_state = [&](float tt) {
+            if (lane == 0) {
+                gp(V.tbuf)[pix] = tt;
+                if constexpr (sizeof(ACC) == 8)
+                    gp(reinterpret_cast<gd4 *>(V.cstate))[pix] = gd4{fc.x, fc.y, fc.z, fc.w};
+                else
+                    gp(reinterpret_cast<gf4 *>(V.cstate))[pix] = gf4{fc.x, fc.y, fc.z, fc.w};
+                gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
+            }
+        };
+        if (!R.hit) {
+            fc.x = fc.y = fc.z = (ACC)0;
+            if (lane == 0) gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
+            continue;
+        }
+        if (tnear < 0.f) tnear = 0.f;
+        const F3 dir = R.dir;
+        const float tfar = R.tfar, astop = V.alpha_stop;
+        float t = tnear;
+        F3 pos = R.origin + dir * tnear;
+        // level-switch distances, one level per lane; the level at t is the
+        // last one already reached (compute_LOD, RayCasters.cu:551-567)
+        {
+            const float epx = fabsf(dot(dir - R.dxn, R.dleft) * tfar);
+            const float epy = fabsf(dot(dir - R.dyn, R.dup) * tfar);
+            float T = FINF;
+            if (lane >= 1 && lane < nl)
+                T = level_switch(t, tfar, epx, epy, spacing(R.dleft, lv[lane].inv), spacing(R.dup, lv[lane].inv));
+            if (lane < nl) tsw[lane] = T;
+            __builtin_amdgcn_wave_barrier();
+        }
+        uint32_t lod = 0;
+        for (uint32_t l = 1; l < nl && tsw[l] <= t; ++l) lod = l;
+        // the front sample's level parameters (uniform)
+        float step = 0.f, tnext = FINF;
+        F3 dstep = f3(0, 0, 0);
+        auto set_level = [&](uint32_t l) {
+            step = lv[l].step;
+            dstep = dir * step;
+            tnext = l + 1 < nl ? tsw[l + 1] : FINF;
+        };
+        set_level(lod);
+        uint32_t cell_last = NONE;
+        for (;;) {
+            // place samples 0..63 of this super-step; lane k keeps sample k
+            float my_t = t;
+            F3 my_pos = pos;
+            uint32_t my_lod = lod;
+            for (uint32_t k = 0; k < 64; ++k) {
+                const bool mine = lane == k;
+                my_t = mine ? t : my_t;
+                my_pos = f3(mine ? pos.x : my_pos.x, mine ? pos.y : my_pos.y, mine ? pos.z : my_pos.z);
+                my_lod = mine ? lod : my_lod;
+                t += step;
+                pos = pos + dstep;
+                if (__builtin_amdgcn_readfirstlane((int)(t >= tnext))) {
+                    while (t >= tnext) set_level(++lod);
+                }
+            }
+            // locate, load and classify sample `lane`
+            const LevelLds &L = lv[my_lod];
+            uint32_t cell, vi;
+            locate_cell(my_pos, f3(L.ior[0], L.ior[1], L.ior[2]), f3(L.ldf[0], L.ldf[1], L.ldf[2]),
+                        f3(L.vvf[0], L.vvf[1], L.vvf[2]), L.lcoord, MX, MY, inner, bs, cell, vi);
+            const uint4 en = p.mrpd[cell];
+            const uint32_t ew = en.w;
+            const uint32_t slot = entry_slot(en, bs_shift, inv_bs, cx_, cy_);
+            const uint32_t prev = (uint32_t)__shfl_up((int)cell, 1, 64);
+            const bool fresh = cell != (lane == 0 ? cell_last : prev);
+            const uint64_t off = ew == 1 ? (uint64_t)slot * bbytes + corner + vi * (uint32_t)BYTES : 0ull;
+            const uint32_t sum = box_add<BYTES>(box_load<BYTES>(pool + off, row, plane));
+            ACC sv;
+            if constexpr (sizeof(ACC) == 8) sv = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
+            else sv = (float)sum * inv_norm;
+            bool skip;
+            const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(sv, tfl, tf_n0, L.kexp, skip)
+                                        : classify<ACC>(sv, gp(reinterpret_cast<const gf4 *>(V.tf)), V.tf_size,
+                                                        L.kexp, skip);
+            const uint64_t m_mapped = __ballot(ew == 1), m_fresh = __ballot(fresh), m_skip = __ballot(skip);
+            const uint64_t m_tok = __ballot(my_t < tfar);
+            const uint32_t u = ~m_mapped ? (uint32_t)__builtin_ctzll(~m_mapped) : 64u;   // first sample not mapped
+            const uint32_t f = ~m_tok ? (uint32_t)__builtin_ctzll(~m_tok) : 64u;         // first with t >= tfar
+            // consume in order (RayCasters.cu:118-178)
+            uint32_t k = 0;
+            bool done = false;
+            for (; k < 64; ++k) {
+                if (k >= f || !__builtin_amdgcn_readfirstlane((int)((ACC)fc.w < (ACC)astop))) {
+                    done = true;
+                    break;
+                }
+                if (k >= u) break;
+                if (!((m_skip >> k) & 1ull)) {
+                    const ACC wgt = (ACC)1 - fc.w;
+                    const ACC cx = lane_read(cc.x, k), cy = lane_read(cc.y, k), cz = lane_read(cc.z, k),
+                              cw = lane_read(cc.w, k);
+                    if constexpr (sizeof(ACC) == 8) {
+                        const ACC tx = cx * wgt, ty = cy * wgt, tz = cz * wgt, tw = cw * wgt;
+                        fc.x += tx; fc.y += ty; fc.z += tz; fc.w += tw;
+                    } else {
+                        fc.x = fmaf(cx, wgt, fc.x);
+                        fc.y = fmaf(cy, wgt, fc.y);
+                        fc.z = fmaf(cz, wgt, fc.z);
+                        fc.w = fmaf(cw, wgt, fc.w);
+                    }
+                }
+            }
+            // samples 0..k-1 were consumed (all mapped); sample k ends the ray
+            // (done), is not mapped (k == u), or the super-step is complete
+            n_samples += k;
+            const uint64_t below = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
+            n_visits += (uint32_t)__popcll(m_fresh & below);
+            if (lane < k && fresh) p.usage[slot] = ts;
+            if (done) {
+                store_state(lane_read(my_t, k));
+                break;
+            }
+            if (k == 64) {
+                cell_last = (uint32_t)__builtin_amdgcn_readlane((int)cell, 63);
+                continue;
+            }
+            // sample u = k is reached and not mapped
+            if ((m_fresh >> k) & 1ull) ++n_visits;
+            const uint32_t ewk = (uint32_t)__builtin_amdgcn_readlane((int)ew, (int)k);
+            if (ewk == 0) {
+                // raise_brick_request, then stop (Unmapped)
+                if (lane == k) p.request[cell] = ts;
+                store_state(lane_read(my_t, k));
+                ++n_unmapped;
+                if (lane == 0) {
+                    if (p.view_stopped) atomicAdd(&p.view_stopped[v], 1u);
+                    if (p.out_list) p.out_list[atomicAdd(p.out_count, 1u)] = e;
+                }
+                break;
+            }
+            // Empty brick: advance by tstep from it (RayCasters.cu:154-159)
+            cell_last = (uint32_t)__builtin_amdgcn_readlane((int)cell, (int)k);
+            t = lane_read(my_t, k) + tstep;
+            pos = f3(lane_read(my_pos.x, k), lane_read(my_pos.y, k), lane_read(my_pos.z, k)) + dir * tstep;
+            lod = (uint32_t)__builtin_amdgcn_readlane((int)my_lod, (int)k);
+            set_level(lod);
+            while (t >= tnext) set_level(++lod);
+        }
+    }
+    if (p.stats && lane == 0) {
+        if (n_samples) atomicAdd((unsigned long long *)&p.stats->mapped_samples, (unsigned long long)n_samples);
+        if (n_visits) atomicAdd((unsigned long long *)&p.stats->brick_visits, (unsigned long long)n_visits);
+        if (n_rays) atomicAdd((unsigned long long *)&p.stats->pixels, (unsigned long long)n_rays);
+        if (n_unmapped) atomicAdd((unsigned long long *)&p.stats->unmapped_rays, (unsigned long long)n_unmapped);
+    }
+}
+
 template <int BYTES, typename ACC>
 const void *kernel_of(uint32_t tile)
 {
@@ -979,6 +1267,48 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
     void *args[] = {const_cast<DvrLaunch *>(&p)};
     return hipLaunchKernel(dvr_kernel(bytes, precision, tile), dim3(grid), dim3(LANES), args,
                            dvr_lds_bytes(p.nb_lods, p.views ? p.n_views : 1), stream);
+}
+
+uint32_t dvr_ray_lds_bytes(uint32_t n_views) { return std::max(1u, n_views) * (uint32_t)sizeof(DvrView); }
+
+uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t lds_bytes)
+{
+    // resident workgroups of 4 waves, never more than one wave per ray
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    const bool d = precision == 64;
+    const void *k = bytes == 1 ? (d ? reinterpret_cast<const void *>(&k_dvr_ray<1, double>)
+                                    : reinterpret_cast<const void *>(&k_dvr_ray<1, float>))
+                               : (d ? reinterpret_cast<const void *>(&k_dvr_ray<2, double>)
+                                    : reinterpret_cast<const void *>(&k_dvr_ray<2, float>));
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, LANES, lds_bytes) != hipSuccess || per_cu <= 0)
+        per_cu = 2;
+    const uint32_t want = (uint32_t)cus * (uint32_t)per_cu;
+    const uint32_t need = (rays + WAVES - 1) / WAVES;
+    return std::max(1u, std::min(want, need));
+}
+
+bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes) { return p.pipelined != 0 && (bytes == 1 || bytes == 2); }
+
+hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t grid, hipStream_t stream)
+{
+    if (!p.list || !p.list_count || p.n_packets != 0 || grid == 0) return hipErrorInvalidValue;
+    if (!dvr_ray_supported(p, bytes)) return hipErrorInvalidValue;
+    if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
+    if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
+    const bool d = precision == 64;
+    const void *k = bytes == 1 ? (d ? reinterpret_cast<const void *>(&k_dvr_ray<1, double>)
+                                    : reinterpret_cast<const void *>(&k_dvr_ray<1, float>))
+                               : (d ? reinterpret_cast<const void *>(&k_dvr_ray<2, double>)
+                                    : reinterpret_cast<const void *>(&k_dvr_ray<2, float>));
+    void *args[] = {const_cast<DvrLaunch *>(&p)};
+    return hipLaunchKernel(k, dim3(grid), dim3(LANES), args, dvr_ray_lds_bytes(p.views ? p.n_views : 1), stream);
 }
 
 hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream)

@@ -426,6 +426,7 @@ struct Cache {
     hipEvent_t uploaded = nullptr, consumed = nullptr;
     uint32_t ts = 1;
     uint32_t cap = 50;                  // request buffers' capacity (max_requests at create)
+    uint32_t ray_list_max = 1u << 16;   // resumed-ray launches up to this many rays use the wave-per-ray march
     uint32_t limit = 50;                // bricks per update, <= cap
     // device state
     uint4 *mrpd = nullptr;
@@ -917,7 +918,7 @@ struct Batch {
     };
     std::vector<Slot> slots;
     uint32_t table = 0;                 // slots in the device table (highest used + 1)
-    uint32_t tile = 0, march = 4, refill_min = 16;
+    uint32_t tile = 0, march = 8, refill_min = 16;
     int cur = -1;                       // list holding the last launch's stopped rays (-1: none)
     bool use_graph = false;
     std::map<std::array<uint32_t, 8>, hipGraphExec_t> graphs;
@@ -1030,15 +1031,30 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
     if (has_list)
         for (uint32_t i = 0; i < B.table; ++i)
             if (B.slots[i].used && !B.slots[i].starting) listed += B.slots[i].npix;
+    // a launch that only resumes a short list of rays: one wave per ray
+    uint32_t list_rays = 0;
+    if (has_list && packets == 0) {
+        GC_HIP(hipEventSynchronize(B.launched));
+        for (uint32_t i = 0; i < B.table; ++i) list_rays += B.h_stopped[i];
+    }
+    const bool ray_kernel = has_list && packets == 0 && list_rays <= C.ray_list_max &&
+                            dvr_ray_supported(L, C.vc.encoded_bytes);
     const uint32_t lds = dvr_lds_bytes(L.nb_lods, B.table);
-    const uint32_t grid = dvr_grid(C.vc.encoded_bytes, B.precision, B.tile, packets + (uint32_t)((listed + 63) / 64), lds);
+    // (a captured graph keeps one grid for every list length)
+    const uint32_t grid = ray_kernel ? dvr_ray_grid(C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays,
+                                                    dvr_ray_lds_bytes(B.table))
+                                     : dvr_grid(C.vc.encoded_bytes, B.precision, B.tile, packets + (uint32_t)((listed + 63) / 64), lds);
+    auto launch_k = [&]() -> hipError_t {
+        return ray_kernel ? launch_dvr_ray(L, C.vc.encoded_bytes, B.precision, grid, s)
+                          : launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
+    };
     GC_HIP(hipEventRecord(B.ev0, s));
     if (B.use_graph) {
         // the timestamp is read on the device so that a captured launch
         // stays valid across updates
         GC_HIP(launch_fill32(B.ctl + 10, 1, C.ts, s));
         L.ts_dev = B.ctl + 10;
-        const std::array<uint32_t, 8> key{B.table, B.tile, B.march, B.refill_min, packets, grid,
+        const std::array<uint32_t, 8> key{B.table, B.tile + (ray_kernel ? 8u : 0u), B.march, B.refill_min, packets, grid,
                                           has_list ? (uint32_t)B.cur : 2u, (uint32_t)out + (append ? 4u : 0u)};
         auto it = B.graphs.find(key);
         if (it == B.graphs.end()) {
@@ -1047,7 +1063,7 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
             hipError_t e = launch_fill32(B.ctl, 8, 0, s);
             if (e == hipSuccess && !append) e = launch_fill32(B.ctl + 8 + out, 1, 0, s);
             if (e == hipSuccess && !append) e = launch_fill32(B.ctl + 16, B.max_views, 0, s);
-            if (e == hipSuccess) e = launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
+            if (e == hipSuccess) e = launch_k();
             const hipError_t e_end = hipStreamEndCapture(s, &g);
             GC_HIP(e);
             GC_HIP(e_end);
@@ -1065,7 +1081,7 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
             GC_HIP(launch_fill32(B.ctl + 8 + out, 1, 0, s));
             GC_HIP(launch_fill32(B.ctl + 16, B.max_views, 0, s));
         }
-        GC_HIP(launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s));
+        GC_HIP(launch_k());
     }
     GC_HIP(hipEventRecord(B.ev1, s));
     GC_HIP(hipMemcpyAsync(B.h_stopped, B.ctl + 16, (size_t)B.max_views * 4, hipMemcpyDeviceToHost, s));
@@ -1687,7 +1703,7 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
     L.stats = d_stats;
     L.queue = F.ctl;
     L.refill_min = view->refill_min ? view->refill_min : 16;
-    L.march = view->march ? view->march : 4;
+    L.march = view->march ? view->march : 8;
     if (L.march > 8) return fail(GC_ERR_ARG, "march must be 1..8");
     hipStream_t s = stream ? (hipStream_t)stream : C.stream;
     GC_HIP(launch_fill32(F.ctl, 8, 0, s));
@@ -1702,7 +1718,7 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
 // (the other rays are finished and would neither sample nor write anything
 // new).  Returns the index of the list the launch fills.
 static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_tf, uint32_t tf_size,
-                        gc_render_stats *d_stats, int first, int &cur, hipStream_t s)
+                        gc_render_stats *d_stats, int first, int &cur, hipStream_t s, uint32_t list_rays = 0)
 {
     DvrLaunch L{};
     launch_base(C, L);
@@ -1712,7 +1728,7 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
     L.stats = d_stats;
     L.queue = F.ctl;
     L.refill_min = view.refill_min ? view.refill_min : 16;
-    L.march = view.march ? view.march : 4;
+    L.march = view.march ? view.march : 8;
     if (L.march > 8) return fail(GC_ERR_ARG, "march must be 1..8");
     const int out = first ? 0 : 1 - cur;
     if (!first) {
@@ -1723,9 +1739,15 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
     L.out_count = F.ctl + 8 + out;
     GC_HIP(launch_fill32(F.ctl, 8, 0, s));
     GC_HIP(launch_fill32(F.ctl + 8 + out, 1, 0, s));
-    const uint32_t grid = dvr_grid(C.vc.encoded_bytes, view.precision, view.tile,
-                                   first ? L.n_packets : (F.w * F.h + 63) / 64, dvr_lds_bytes(L.nb_lods, 1));
-    GC_HIP(launch_dvr(L, C.vc.encoded_bytes, view.precision, view.tile, grid, s));
+    if (!first && list_rays <= C.ray_list_max && dvr_ray_supported(L, C.vc.encoded_bytes)) {
+        // a short list of resumed rays: one wave per ray
+        GC_HIP(launch_dvr_ray(L, C.vc.encoded_bytes, view.precision,
+                              dvr_ray_grid(C.vc.encoded_bytes, view.precision, list_rays, dvr_ray_lds_bytes(1)), s));
+    } else {
+        const uint32_t grid = dvr_grid(C.vc.encoded_bytes, view.precision, view.tile,
+                                       first ? L.n_packets : (F.w * F.h + 63) / 64, dvr_lds_bytes(L.nb_lods, 1));
+        GC_HIP(launch_dvr(L, C.vc.encoded_bytes, view.precision, view.tile, grid, s));
+    }
     cur = out;
     return GC_OK;
 }
@@ -1762,8 +1784,12 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
     GC_HIP(hipMemsetAsync(C.d_frame_stats, 0, sizeof(gc_render_stats), C.stream));
     int cur = 0;
     for (uint32_t it = 0; it < max_launches; ++it) {
+        uint32_t list_rays = 0;
+        if (it > 0)
+            GC_HIP(hipMemcpyAsync(&list_rays, f->impl.ctl + 8 + cur, 4, hipMemcpyDeviceToHost, C.stream));
+        GC_HIP(hipStreamSynchronize(C.stream));
         GC_HIP(hipEventRecord(C.ev0, C.stream));
-        GC_TRY(frame_launch(C, f->impl, *view, d_tf, tf_size, C.d_frame_stats, it == 0, cur, C.stream));
+        GC_TRY(frame_launch(C, f->impl, *view, d_tf, tf_size, C.d_frame_stats, it == 0, cur, C.stream, list_rays));
         GC_HIP(hipEventRecord(C.ev1, C.stream));
         ++report->launches;
         int done = 0;
@@ -1789,6 +1815,13 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
     }
     GC_HIP(hipMemcpyAsync(&report->stats, C.d_frame_stats, sizeof(gc_render_stats), hipMemcpyDeviceToHost, C.stream));
     GC_HIP(hipStreamSynchronize(C.stream));
+    return GC_OK;
+}
+
+int gc_cache_set_ray_list_max(gc_cache *c, uint32_t max_rays)
+{
+    if (!c) return fail(GC_ERR_ARG, "null cache");
+    c->impl.ray_list_max = max_rays;
     return GC_OK;
 }
 
@@ -1871,7 +1904,7 @@ int gc_batch_set_launch(gc_batch *b, uint32_t tile, uint32_t march, uint32_t ref
     if (refill_min > 64) return fail(GC_ERR_ARG, "refill_min must be 0..64");
     Batch &B = b->impl;
     B.tile = tile;
-    B.march = march ? march : 4;
+    B.march = march ? march : 8;
     B.refill_min = refill_min ? refill_min : 16;
     return GC_OK;
 }

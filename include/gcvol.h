@@ -258,7 +258,7 @@ typedef struct gc_view {
     uint32_t refill_min;         /* refill a wave's idle lanes once this many are
                                     idle (0: default 16)                      */
     uint32_t march;              /* samples per lane between refill checks,
-                                    1..8 (0: default 4)                       */
+                                    1..8 (0: default 8)                       */
 } gc_view;
 
 /* Per-view progressive state (tBuffer, colorStateBuffer, pixel buffer) on the
@@ -360,6 +360,13 @@ int gc_batch_render(gc_batch *b, const gc_view *views, const float *const *d_tfs
                     gc_frame_report *report);
 int gc_batch_read_pixels(gc_batch *b, uint32_t slot, uint32_t *host);   /* W*H ABGR */
 int gc_batch_read_state(gc_batch *b, uint32_t slot, float *tbuf, void *cstate);
+
+/* Launches that resume at most max_rays rays (the rays an earlier launch of
+ * the frame stopped at a miss) march them one wave per ray, 64 samples per
+ * step, instead of one lane per ray; 0 never does.  The results are the same
+ * either way.  No reference counterpart (a launch-shape knob, like
+ * gc_batch_set_launch). */
+int gc_cache_set_ray_list_max(gc_cache *c, uint32_t max_rays);
 
 /* Bricks per update (the upload batch), at most the max_requests the cache
  * was created with. */

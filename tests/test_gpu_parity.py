@@ -194,3 +194,41 @@ def test_large_pool_offsets_match_small_pool(gpu, pool_shape):
     assert (off & 0x80000000) and off >= (1 << 31), "bricks did not reach the intended offsets (%d)" % off
     big.close()
     small.close()
+
+
+@pytest.mark.parametrize("bytes_,prec", [(1, 32), (2, 64), (2, 32)])
+def test_ray_list_kernel_matches_lane_kernel(gpu, bytes_, prec):
+    """Launches that resume rays stopped at a miss run either one lane per ray
+    or one wave per ray (64 samples per step).  The same out-of-core frame
+    sequence rendered both ways, on two fresh caches, must give bit-identical
+    pixels, t buffers, colour state and render counters, and leave the page
+    directory, LRU, usage and request buffers bit-identical after every
+    frame."""
+    sc = Scene((96, 80, 64), 8, 1, bytes_, cache_size=(9, 9, 8), max_requests=64)
+    caches = [sc.gpu_cache(), sc.gpu_cache()]
+    G.set_ray_list_max(caches[0], 0)            # lane per ray throughout
+    G.set_ray_list_max(caches[1], 1 << 30)      # every resumed launch one wave per ray
+    tf = tf_table(seed=23)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 80, 64
+    for k in range(4):
+        out = []
+        for c in caches:
+            f = G.Frame(W, H, prec)
+            r = G.render_frame(c, G.make_view(W, H, orbit(2 * k, distance=2.2), alpha_stop=0.95 if k % 2 else 0.99,
+                                              precision=prec), d_tf, len(tf), f)
+            tb, cs = f.state()
+            out.append((f.pixels().copy(), tb.copy(), cs.copy(), r.as_dict()))
+            f.close()
+        (p0, t0, c0, r0), (p1, t1, c1, r1) = out
+        print("frame", k, "launches", r0["launches"], "samples", r0["mapped_samples"], flush=True)
+        assert r0["complete"] and r1["complete"] and r0["launches"] > 1
+        for key in ("launches", "mapped_samples", "brick_visits", "pixels", "unmapped_rays", "bricks_loaded"):
+            assert r0[key] == r1[key], (k, key, r0[key], r1[key])
+        assert np.array_equal(p0, p1), "pixels differ at frame %d" % k
+        assert np.array_equal(t0, t1) and np.array_equal(c0.view(np.uint8), c1.view(np.uint8)), k
+        a, b = caches
+        assert np.array_equal(a.mrpd(), b.mrpd()) and np.array_equal(a.lru(), b.lru()), k
+        assert np.array_equal(a.usage(), b.usage()) and np.array_equal(a.request(), b.request()), k
+    for c in caches:
+        c.close()
