@@ -207,12 +207,14 @@ inline void tile_shape(uint32_t tile, uint32_t &tw, uint32_t &th)
 // and the persistent grid for it (workgroups of 4 waves).
 uint32_t dvr_lds_bytes(uint32_t nb_lods, uint32_t n_views);
 uint32_t dvr_grid(uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t n_packets, uint32_t lds_bytes);
-// Wave-per-ray march of a ray list (list launches only; see kernels_dvr.hip):
-// persistent workgroups of 4 waves, each wave taking one ray at a time.
+// Group-per-ray march of a ray list (list launches only; see kernels_dvr.hip):
+// persistent workgroups of 4 waves, each group of 16 or 64 lanes taking one
+// ray at a time (the group size follows `rays`, the list length).
 bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes);
 uint32_t dvr_ray_lds_bytes(uint32_t n_views);
 uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t lds_bytes);
-hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t grid, hipStream_t stream);
+hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t grid,
+                          hipStream_t stream);
 hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t tile,
                       uint32_t grid, hipStream_t stream);
 hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream);

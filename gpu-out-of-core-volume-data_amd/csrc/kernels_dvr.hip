@@ -972,17 +972,18 @@ __device__ inline ACC lane_read(ACC x, uint32_t k)
     }
 }
 
-template <int BYTES, typename ACC>
-__global__ __launch_bounds__(LANES) void k_dvr_ray(const DvrLaunch p)
+template <int BYTES, typename ACC, int G>
+__global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : 4))) void k_dvr_ray(const DvrLaunch p)
 {
+    static_assert(G == 16 || G == 32 || G == 64, "16, 32 or 64 lanes per ray");
+    constexpr uint32_t R = 64 / G;   // rays per wave
     __shared__ LevelLds lv[GC_MAX_LEVELS];
     __shared__ float4 tfl[TF_LDS + 2];
-    __shared__ float tswv[WAVES][GC_MAX_LEVELS + 1];
+    __shared__ float tswv[WAVES * R][GC_MAX_LEVELS + 1];
     extern __shared__ __align__(16) uint8_t dyn_lds[];
     DvrView *vw = reinterpret_cast<DvrView *>(dyn_lds);
     const uint32_t nl = p.nb_lods;
     const int tid = threadIdx.x;
-    const uint32_t lane = tid & 63, wv = (uint32_t)tid >> 6;
     const gc_geometry *g = p.geom;
     if (tid < (int)nl) {
         LevelLds &L = lv[tid];
@@ -1025,125 +1026,142 @@ __global__ __launch_bounds__(LANES) void k_dvr_ray(const DvrLaunch p)
     const ACC inv_norm = BYTES == 1 ? (ACC)(1.0 / 2040.0) : (ACC)(1.0 / 524280.0);
     const float tstep = p.tstep;
     const uint32_t ts = __builtin_amdgcn_readfirstlane(p.ts_dev ? *p.ts_dev : p.ts);
-    float *tsw = tswv[wv];
-    uint32_t n_samples = 0, n_visits = 0, n_rays = 0, n_unmapped = 0;   // wave-uniform
+
+    const uint32_t lane = tid & 63, gl = lane % G, gsh = lane - gl;
+    float *tsw = tswv[((uint32_t)tid >> 6) * R + lane / G];
+    const uint64_t glow = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    auto gbits = [&](uint64_t m) -> uint64_t { return G == 64 ? m : ((m >> gsh) & glow); };
+    auto first_clear = [&](uint64_t m) -> uint32_t {   // index of the first 0 bit of the group's mask, or G
+        const uint64_t z = ~m & glow;
+        return z ? (uint32_t)__builtin_ctzll(z) : (uint32_t)G;
+    };
+
+    // the group's ray (every lane of the group holds the same values)
+    bool active = false, exhausted = false;
+    uint32_t e = 0, v = 0, pix = 0;
+    Acc4<ACC> fc{0, 0, 0, 0};
+    float t = 0.f, tfar = 0.f, astop = 0.f, step = 0.f, tnext = FINF;
+    F3 pos = f3(0, 0, 0), dir = f3(0, 0, 0), dstep = f3(0, 0, 0);
+    uint32_t lod = 0, cell_last = NONE;
+    uint32_t n_samples = 0, n_visits = 0, n_rays = 0, n_unmapped = 0;   // on each group's first lane
+    auto set_level = [&](uint32_t l) {
+        step = lv[l].step;
+        dstep = dir * step;
+        tnext = l + 1 < nl ? tsw[l + 1] : FINF;
+    };
 
     for (;;) {
-        uint32_t r = 0;
-        if (lane == 0) r = atomicAdd(&p.queue[0], 1u);
-        r = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(r, 0, 64));
-        if (r >= list_count) break;
-        const uint32_t e = (uint32_t)__builtin_amdgcn_readfirstlane((int)p.list[r]);
-        const uint32_t v = e >> DVR_PIXEL_BITS;
-        const uint32_t pix = e & ((1u << DVR_PIXEL_BITS) - 1u);
-        const DvrView &V = vw[v];
-        const uint32_t W = V.width;
-        const RaySetup R = setup_ray(V, pix % W, pix / W);
-        float tnear = fmaxf(R.tnear, gp(V.tbuf)[pix]);
-        Acc4<ACC> fc;
-        if constexpr (sizeof(ACC) == 8) {
-            const gd4 c = gp(reinterpret_cast<const gd4 *>(V.cstate))[pix];
-            fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
-        } else {
-            const gf4 c = gp(reinterpret_cast<const gf4 *>(V.cstate))[pix];
-            fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
-        }
-        ++n_rays;
-        auto store_state = [&](float tt) {
-            if (lane == 0) {
-                gp(V.tbuf)[pix] = tt;
-                if constexpr (sizeof(ACC) == 8)
-                    gp(reinterpret_cast<gd4 *>(V.cstate))[pix] = gd4{fc.x, fc.y, fc.z, fc.w};
-                else
-                    gp(reinterpret_cast<gf4 *>(V.cstate))[pix] = gf4{fc.x, fc.y, fc.z, fc.w};
-                gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
+        // a group without a ray takes the next one (the whole group together)
+        if (!active && !exhausted) {
+            uint32_t r = 0;
+            if (gl == 0) r = atomicAdd(&p.queue[0], 1u);
+            r = __shfl(r, 0, G);
+            if (r >= list_count) {
+                exhausted = true;
+            } else {
+                e = p.list[r];
+                v = e >> DVR_PIXEL_BITS;
+                pix = e & ((1u << DVR_PIXEL_BITS) - 1u);
+                const DvrView &V = vw[v];
+                const RaySetup Rs = setup_ray(V, pix % V.width, pix / V.width);
+                float tnear = fmaxf(Rs.tnear, gp(V.tbuf)[pix]);
+                if constexpr (sizeof(ACC) == 8) {
+                    const gd4 c = gp(reinterpret_cast<const gd4 *>(V.cstate))[pix];
+                    fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
+                } else {
+                    const gf4 c = gp(reinterpret_cast<const gf4 *>(V.cstate))[pix];
+                    fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
+                }
+                if (gl == 0) ++n_rays;
+                if (!Rs.hit) {
+                    fc.x = fc.y = fc.z = (ACC)0;
+                    if (gl == 0) gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
+                } else {
+                    if (tnear < 0.f) tnear = 0.f;
+                    dir = Rs.dir;
+                    tfar = Rs.tfar;
+                    astop = V.alpha_stop;
+                    t = tnear;
+                    pos = Rs.origin + dir * tnear;
+                    // level-switch distances, one level per lane; the level
+                    // at t is the last one already reached (compute_LOD,
+                    // RayCasters.cu:551-567)
+                    const float epx = fabsf(dot(dir - Rs.dxn, Rs.dleft) * tfar);
+                    const float epy = fabsf(dot(dir - Rs.dyn, Rs.dup) * tfar);
+                    for (uint32_t l = gl; l < nl; l += G)
+                        tsw[l] = l == 0 ? FINF
+                                        : level_switch(t, tfar, epx, epy, spacing(Rs.dleft, lv[l].inv),
+                                                       spacing(Rs.dup, lv[l].inv));
+                    __builtin_amdgcn_wave_barrier();
+                    lod = 0;
+                    for (uint32_t l = 1; l < nl && tsw[l] <= t; ++l) lod = l;
+                    set_level(lod);
+                    cell_last = NONE;
+                    active = true;
+                }
             }
-        };
-        if (!R.hit) {
-            fc.x = fc.y = fc.z = (ACC)0;
-            if (lane == 0) gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
+        }
+        if (__ballot(active) == 0) {
+            if (__ballot(!exhausted) == 0) break;
             continue;
         }
-        if (tnear < 0.f) tnear = 0.f;
-        const F3 dir = R.dir;
-        const float tfar = R.tfar, astop = V.alpha_stop;
-        float t = tnear;
-        F3 pos = R.origin + dir * tnear;
-        // level-switch distances, one level per lane; the level at t is the
-        // last one already reached (compute_LOD, RayCasters.cu:551-567)
-        {
-            const float epx = fabsf(dot(dir - R.dxn, R.dleft) * tfar);
-            const float epy = fabsf(dot(dir - R.dyn, R.dup) * tfar);
-            float T = FINF;
-            if (lane >= 1 && lane < nl)
-                T = level_switch(t, tfar, epx, epy, spacing(R.dleft, lv[lane].inv), spacing(R.dup, lv[lane].inv));
-            if (lane < nl) tsw[lane] = T;
-            __builtin_amdgcn_wave_barrier();
-        }
-        uint32_t lod = 0;
-        for (uint32_t l = 1; l < nl && tsw[l] <= t; ++l) lod = l;
-        // the front sample's level parameters (uniform)
-        float step = 0.f, tnext = FINF;
-        F3 dstep = f3(0, 0, 0);
-        auto set_level = [&](uint32_t l) {
-            step = lv[l].step;
-            dstep = dir * step;
-            tnext = l + 1 < nl ? tsw[l + 1] : FINF;
-        };
-        set_level(lod);
-        uint32_t cell_last = NONE;
-        for (;;) {
-            // place samples 0..63 of this super-step; lane k keeps sample k
-            float my_t = t;
-            F3 my_pos = pos;
-            uint32_t my_lod = lod;
-            for (uint32_t k = 0; k < 64; ++k) {
-                const bool mine = lane == k;
-                my_t = mine ? t : my_t;
-                my_pos = f3(mine ? pos.x : my_pos.x, mine ? pos.y : my_pos.y, mine ? pos.z : my_pos.z);
-                my_lod = mine ? lod : my_lod;
-                t += step;
-                pos = pos + dstep;
-                if (__builtin_amdgcn_readfirstlane((int)(t >= tnext))) {
-                    while (t >= tnext) set_level(++lod);
-                }
+
+        // place samples 0..G-1 of this super-step; lane k of a group keeps
+        // sample k of its ray (groups without a ray run along harmlessly)
+        float my_t = t;
+        F3 my_pos = pos;
+        uint32_t my_lod = lod;
+        for (uint32_t k = 0; k < (uint32_t)G; ++k) {
+            const bool mine = gl == k;
+            my_t = mine ? t : my_t;
+            my_pos = f3(mine ? pos.x : my_pos.x, mine ? pos.y : my_pos.y, mine ? pos.z : my_pos.z);
+            my_lod = mine ? lod : my_lod;
+            t += step;
+            pos = pos + dstep;
+            if (__ballot(t >= tnext)) {
+                while (t >= tnext) set_level(++lod);
             }
-            // locate, load and classify sample `lane`
-            const LevelLds &L = lv[my_lod];
-            uint32_t cell, vi;
-            locate_cell(my_pos, f3(L.ior[0], L.ior[1], L.ior[2]), f3(L.ldf[0], L.ldf[1], L.ldf[2]),
-                        f3(L.vvf[0], L.vvf[1], L.vvf[2]), L.lcoord, MX, MY, inner, bs, cell, vi);
-            const uint4 en = p.mrpd[cell];
-            const uint32_t ew = en.w;
-            const uint32_t slot = entry_slot(en, bs_shift, inv_bs, cx_, cy_);
-            const uint32_t prev = (uint32_t)__shfl_up((int)cell, 1, 64);
-            const bool fresh = cell != (lane == 0 ? cell_last : prev);
-            const uint64_t off = ew == 1 ? (uint64_t)slot * bbytes + corner + vi * (uint32_t)BYTES : 0ull;
-            const uint32_t sum = box_add<BYTES>(box_load<BYTES>(pool + off, row, plane));
-            ACC sv;
-            if constexpr (sizeof(ACC) == 8) sv = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
-            else sv = (float)sum * inv_norm;
-            bool skip;
-            const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(sv, tfl, tf_n0, L.kexp, skip)
-                                        : classify<ACC>(sv, gp(reinterpret_cast<const gf4 *>(V.tf)), V.tf_size,
-                                                        L.kexp, skip);
-            const uint64_t m_mapped = __ballot(ew == 1), m_fresh = __ballot(fresh), m_skip = __ballot(skip);
-            const uint64_t m_tok = __ballot(my_t < tfar);
-            const uint32_t u = ~m_mapped ? (uint32_t)__builtin_ctzll(~m_mapped) : 64u;   // first sample not mapped
-            const uint32_t f = ~m_tok ? (uint32_t)__builtin_ctzll(~m_tok) : 64u;         // first with t >= tfar
-            // consume in order (RayCasters.cu:118-178)
-            uint32_t k = 0;
-            bool done = false;
-            for (; k < 64; ++k) {
-                if (k >= f || !__builtin_amdgcn_readfirstlane((int)((ACC)fc.w < (ACC)astop))) {
+        }
+        // locate, load and classify each lane's sample
+        const LevelLds &L = lv[my_lod];
+        uint32_t cell, vi;
+        locate_cell(my_pos, f3(L.ior[0], L.ior[1], L.ior[2]), f3(L.ldf[0], L.ldf[1], L.ldf[2]),
+                    f3(L.vvf[0], L.vvf[1], L.vvf[2]), L.lcoord, MX, MY, inner, bs, cell, vi);
+        const uint4 en = p.mrpd[cell];
+        const uint32_t ew = en.w;
+        const uint32_t slot = entry_slot(en, bs_shift, inv_bs, cx_, cy_);
+        const uint32_t prev = __shfl_up(cell, 1u, G);
+        const bool fresh = cell != (gl == 0 ? cell_last : prev);
+        const uint64_t off = (active && ew == 1) ? (uint64_t)slot * bbytes + corner + vi * (uint32_t)BYTES : 0ull;
+        const uint32_t sum = box_add<BYTES>(box_load<BYTES>(pool + off, row, plane));
+        ACC sv;
+        if constexpr (sizeof(ACC) == 8) sv = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
+        else sv = (float)sum * inv_norm;
+        bool skip;
+        const DvrView &V = vw[v];
+        const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(sv, tfl, tf_n0, L.kexp, skip)
+                                    : classify<ACC>(sv, gp(reinterpret_cast<const gf4 *>(V.tf)), V.tf_size, L.kexp,
+                                                    skip);
+        const uint64_t m_fresh = gbits(__ballot(fresh)), m_skip = gbits(__ballot(skip));
+        const uint32_t u = first_clear(gbits(__ballot(ew == 1)));   // first sample not mapped
+        const uint32_t f = first_clear(gbits(__ballot(my_t < tfar)));   // first sample with t >= tfar
+        // consume in order (RayCasters.cu:118-178); kend: first sample not
+        // consumed (the one that ends the ray, the one not mapped, or G)
+        uint32_t kend = G;
+        bool done = false, scanning = active;
+        for (uint32_t k = 0; k < (uint32_t)G; ++k) {
+            const ACC cx = __shfl(cc.x, (int)k, G), cy = __shfl(cc.y, (int)k, G), cz = __shfl(cc.z, (int)k, G),
+                      cw = __shfl(cc.w, (int)k, G);
+            if (scanning) {
+                if (k >= f || !((ACC)fc.w < (ACC)astop)) {
                     done = true;
-                    break;
-                }
-                if (k >= u) break;
-                if (!((m_skip >> k) & 1ull)) {
+                    scanning = false;
+                    kend = k;
+                } else if (k >= u) {
+                    scanning = false;
+                    kend = k;
+                } else if (!((m_skip >> k) & 1ull)) {
                     const ACC wgt = (ACC)1 - fc.w;
-                    const ACC cx = lane_read(cc.x, k), cy = lane_read(cc.y, k), cz = lane_read(cc.z, k),
-                              cw = lane_read(cc.w, k);
                     if constexpr (sizeof(ACC) == 8) {
                         const ACC tx = cx * wgt, ty = cy * wgt, tz = cz * wgt, tw = cw * wgt;
                         fc.x += tx; fc.y += ty; fc.z += tz; fc.w += tw;
@@ -1155,48 +1173,71 @@ __global__ __launch_bounds__(LANES) void k_dvr_ray(const DvrLaunch p)
                     }
                 }
             }
-            // samples 0..k-1 were consumed (all mapped); sample k ends the ray
-            // (done), is not mapped (k == u), or the super-step is complete
-            n_samples += k;
-            const uint64_t below = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
-            n_visits += (uint32_t)__popcll(m_fresh & below);
-            if (lane < k && fresh) p.usage[slot] = ts;
-            if (done) {
-                store_state(lane_read(my_t, k));
-                break;
+            if ((k & 3u) == 3u && __ballot(scanning) == 0) break;
+        }
+        // the sample at kend, fetched with every lane active
+        const uint32_t ks = kend < (uint32_t)G ? kend : (uint32_t)G - 1u;
+        const float t_k = __shfl(my_t, (int)ks, G);
+        const float px_k = __shfl(my_pos.x, (int)ks, G), py_k = __shfl(my_pos.y, (int)ks, G),
+                    pz_k = __shfl(my_pos.z, (int)ks, G);
+        const uint32_t lod_k = __shfl(my_lod, (int)ks, G), cell_k = __shfl(cell, (int)ks, G),
+                       ew_k = __shfl(ew, (int)ks, G);
+        if (active) {
+            // samples 0..kend-1 were consumed (all mapped)
+            const uint64_t below = kend >= 64 ? ~0ull : ((1ull << kend) - 1ull);
+            if (gl == 0) {
+                n_samples += kend;
+                n_visits += (uint32_t)__popcll(m_fresh & below);
             }
-            if (k == 64) {
-                cell_last = (uint32_t)__builtin_amdgcn_readlane((int)cell, 63);
-                continue;
-            }
-            // sample u = k is reached and not mapped
-            if ((m_fresh >> k) & 1ull) ++n_visits;
-            const uint32_t ewk = (uint32_t)__builtin_amdgcn_readlane((int)ew, (int)k);
-            if (ewk == 0) {
-                // raise_brick_request, then stop (Unmapped)
-                if (lane == k) p.request[cell] = ts;
-                store_state(lane_read(my_t, k));
-                ++n_unmapped;
-                if (lane == 0) {
-                    if (p.view_stopped) atomicAdd(&p.view_stopped[v], 1u);
-                    if (p.out_list) p.out_list[atomicAdd(p.out_count, 1u)] = e;
+            if (gl < kend && fresh) p.usage[slot] = ts;
+            auto store_state = [&](float tt) {
+                if (gl == 0) {
+                    gp(V.tbuf)[pix] = tt;
+                    if constexpr (sizeof(ACC) == 8)
+                        gp(reinterpret_cast<gd4 *>(V.cstate))[pix] = gd4{fc.x, fc.y, fc.z, fc.w};
+                    else
+                        gp(reinterpret_cast<gf4 *>(V.cstate))[pix] = gf4{fc.x, fc.y, fc.z, fc.w};
+                    gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
                 }
-                break;
+            };
+            if (done) {
+                store_state(t_k);
+                active = false;
+            } else if (kend == (uint32_t)G) {
+                cell_last = cell_k;
+            } else {
+                // sample kend is reached and not mapped
+                if (gl == 0 && ((m_fresh >> kend) & 1ull)) ++n_visits;
+                if (ew_k == 0) {
+                    // raise_brick_request, then stop (Unmapped)
+                    if (gl == kend) p.request[cell] = ts;
+                    store_state(t_k);
+                    if (gl == 0) {
+                        ++n_unmapped;
+                        if (p.view_stopped) atomicAdd(&p.view_stopped[v], 1u);
+                        if (p.out_list) p.out_list[atomicAdd(p.out_count, 1u)] = e;
+                    }
+                    active = false;
+                } else {
+                    // Empty brick: advance by tstep from it (RayCasters.cu:154-159)
+                    cell_last = cell_k;
+                    t = t_k + tstep;
+                    pos = f3(px_k, py_k, pz_k) + dir * tstep;
+                    lod = lod_k;
+                    set_level(lod);
+                    while (t >= tnext) set_level(++lod);
+                }
             }
-            // Empty brick: advance by tstep from it (RayCasters.cu:154-159)
-            cell_last = (uint32_t)__builtin_amdgcn_readlane((int)cell, (int)k);
-            t = lane_read(my_t, k) + tstep;
-            pos = f3(lane_read(my_pos.x, k), lane_read(my_pos.y, k), lane_read(my_pos.z, k)) + dir * tstep;
-            lod = (uint32_t)__builtin_amdgcn_readlane((int)my_lod, (int)k);
-            set_level(lod);
-            while (t >= tnext) set_level(++lod);
         }
     }
-    if (p.stats && lane == 0) {
-        if (n_samples) atomicAdd((unsigned long long *)&p.stats->mapped_samples, (unsigned long long)n_samples);
-        if (n_visits) atomicAdd((unsigned long long *)&p.stats->brick_visits, (unsigned long long)n_visits);
-        if (n_rays) atomicAdd((unsigned long long *)&p.stats->pixels, (unsigned long long)n_rays);
-        if (n_unmapped) atomicAdd((unsigned long long *)&p.stats->unmapped_rays, (unsigned long long)n_unmapped);
+    if (p.stats) {
+        const uint32_t s0 = wave_sum(n_samples), s1 = wave_sum(n_visits), s2 = wave_sum(n_rays), s3 = wave_sum(n_unmapped);
+        if (lane == 0) {
+            if (s0) atomicAdd((unsigned long long *)&p.stats->mapped_samples, (unsigned long long)s0);
+            if (s1) atomicAdd((unsigned long long *)&p.stats->brick_visits, (unsigned long long)s1);
+            if (s2) atomicAdd((unsigned long long *)&p.stats->pixels, (unsigned long long)s2);
+            if (s3) atomicAdd((unsigned long long *)&p.stats->unmapped_rays, (unsigned long long)s3);
+        }
     }
 }
 
@@ -1271,9 +1312,29 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
 
 uint32_t dvr_ray_lds_bytes(uint32_t n_views) { return std::max(1u, n_views) * (uint32_t)sizeof(DvrView); }
 
+bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes) { return p.pipelined != 0 && (bytes == 1 || bytes == 2); }
+
+// Lanes per ray: a whole wave for the shortest lists (lowest latency per
+// ray), 16 lanes (4 rays per wave) otherwise.
+static uint32_t ray_group(uint32_t rays) { return rays <= 256u ? 64u : 16u; }
+
+static const void *ray_kernel(uint32_t bytes, uint32_t precision, uint32_t group)
+{
+    const bool d = precision == 64;
+    if (group == 64)
+        return bytes == 1 ? (d ? reinterpret_cast<const void *>(&k_dvr_ray<1, double, 64>)
+                               : reinterpret_cast<const void *>(&k_dvr_ray<1, float, 64>))
+                          : (d ? reinterpret_cast<const void *>(&k_dvr_ray<2, double, 64>)
+                               : reinterpret_cast<const void *>(&k_dvr_ray<2, float, 64>));
+    return bytes == 1 ? (d ? reinterpret_cast<const void *>(&k_dvr_ray<1, double, 16>)
+                           : reinterpret_cast<const void *>(&k_dvr_ray<1, float, 16>))
+                      : (d ? reinterpret_cast<const void *>(&k_dvr_ray<2, double, 16>)
+                           : reinterpret_cast<const void *>(&k_dvr_ray<2, float, 16>));
+}
+
 uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t lds_bytes)
 {
-    // resident workgroups of 4 waves, never more than one wave per ray
+    // resident workgroups of 4 waves, never more groups than rays
     static int cus = 0;
     if (cus == 0) {
         int dev = 0;
@@ -1281,34 +1342,28 @@ uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
     }
-    const bool d = precision == 64;
-    const void *k = bytes == 1 ? (d ? reinterpret_cast<const void *>(&k_dvr_ray<1, double>)
-                                    : reinterpret_cast<const void *>(&k_dvr_ray<1, float>))
-                               : (d ? reinterpret_cast<const void *>(&k_dvr_ray<2, double>)
-                                    : reinterpret_cast<const void *>(&k_dvr_ray<2, float>));
+    const uint32_t group = ray_group(rays);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, LANES, lds_bytes) != hipSuccess || per_cu <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ray_kernel(bytes, precision, group), LANES, lds_bytes) !=
+            hipSuccess ||
+        per_cu <= 0)
         per_cu = 2;
     const uint32_t want = (uint32_t)cus * (uint32_t)per_cu;
-    const uint32_t need = (rays + WAVES - 1) / WAVES;
-    return std::max(1u, std::min(want, need));
+    const uint64_t rays_per_block = (uint64_t)WAVES * (64u / group);
+    const uint32_t need = (uint32_t)std::min<uint64_t>(((uint64_t)rays + rays_per_block - 1) / rays_per_block, want);
+    return std::max(1u, need);
 }
 
-bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes) { return p.pipelined != 0 && (bytes == 1 || bytes == 2); }
-
-hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t grid, hipStream_t stream)
+hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t grid,
+                          hipStream_t stream)
 {
     if (!p.list || !p.list_count || p.n_packets != 0 || grid == 0) return hipErrorInvalidValue;
     if (!dvr_ray_supported(p, bytes)) return hipErrorInvalidValue;
     if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
-    const bool d = precision == 64;
-    const void *k = bytes == 1 ? (d ? reinterpret_cast<const void *>(&k_dvr_ray<1, double>)
-                                    : reinterpret_cast<const void *>(&k_dvr_ray<1, float>))
-                               : (d ? reinterpret_cast<const void *>(&k_dvr_ray<2, double>)
-                                    : reinterpret_cast<const void *>(&k_dvr_ray<2, float>));
     void *args[] = {const_cast<DvrLaunch *>(&p)};
-    return hipLaunchKernel(k, dim3(grid), dim3(LANES), args, dvr_ray_lds_bytes(p.views ? p.n_views : 1), stream);
+    return hipLaunchKernel(ray_kernel(bytes, precision, ray_group(rays)), dim3(grid), dim3(LANES), args,
+                           dvr_ray_lds_bytes(p.views ? p.n_views : 1), stream);
 }
 
 hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream)

@@ -1045,7 +1045,7 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
                                                     dvr_ray_lds_bytes(B.table))
                                      : dvr_grid(C.vc.encoded_bytes, B.precision, B.tile, packets + (uint32_t)((listed + 63) / 64), lds);
     auto launch_k = [&]() -> hipError_t {
-        return ray_kernel ? launch_dvr_ray(L, C.vc.encoded_bytes, B.precision, grid, s)
+        return ray_kernel ? launch_dvr_ray(L, C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays, grid, s)
                           : launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
     };
     GC_HIP(hipEventRecord(B.ev0, s));
@@ -1741,7 +1741,7 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
     GC_HIP(launch_fill32(F.ctl + 8 + out, 1, 0, s));
     if (!first && list_rays <= C.ray_list_max && dvr_ray_supported(L, C.vc.encoded_bytes)) {
         // a short list of resumed rays: one wave per ray
-        GC_HIP(launch_dvr_ray(L, C.vc.encoded_bytes, view.precision,
+        GC_HIP(launch_dvr_ray(L, C.vc.encoded_bytes, view.precision, list_rays,
                               dvr_ray_grid(C.vc.encoded_bytes, view.precision, list_rays, dvr_ray_lds_bytes(1)), s));
     } else {
         const uint32_t grid = dvr_grid(C.vc.encoded_bytes, view.precision, view.tile,
