@@ -320,59 +320,77 @@ def batch_workload(G, reps=3):
     }
 
 
-def _pmc_pass(counter, outdir, args):
-    """One child run of this benchmark under rocprofv3 collecting `counter`;
-    returns (per-k_dvr-dispatch values in KiB after the first frame, copy
-    kernel value in KiB)."""
-    cmd = ["rocprofv3", "--pmc", counter, "-d", outdir, "-o", counter, "--", sys.executable,
+def _pmc_pass(counters, outdir, args):
+    """One child run of this benchmark under rocprofv3 collecting
+    `counters` (one rocprofv3 pass); returns ({counter: per-k_dvr-dispatch
+    values after the first frame}, {counter: copy kernel value})."""
+    tag = counters[0]
+    cmd = ["rocprofv3", "--pmc"] + list(counters) + ["-d", outdir, "-o", tag, "--", sys.executable,
            os.path.abspath(__file__), "--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-traffic", "--no-batch",
            "--calib-copy-mib", str(CALIB_MIB), "--max-requests", str(args.max_requests), "--tile", str(args.tile),
            "--precision", str(args.precision)]
     env = dict(os.environ, TMPDIR="/tmp")
     r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=300)
     if r.returncode != 0:
-        raise RuntimeError("rocprofv3 %s pass failed (%d): %s" % (counter, r.returncode, r.stderr[-400:]))
-    first_launches = None
+        raise RuntimeError("rocprofv3 %s pass failed (%d): %s" % (tag, r.returncode, r.stderr[-400:]))
+    first_launches = None   # k_dvr launches of the cold first frame
     for line in r.stderr.splitlines():
         if line.startswith("frame   0 "):
-            first_launches = int(line.split("launches")[1].split()[0])
-    dbs = [os.path.join(dp, f) for dp, _, fs in os.walk(outdir) for f in fs if f.endswith(".db") and counter in f]
+            first_launches = int(line.split(" lane ")[1].split()[0])
+    dbs = [os.path.join(dp, f) for dp, _, fs in os.walk(outdir) for f in fs if f.endswith(".db") and tag in f]
     if not dbs or first_launches is None:
-        raise RuntimeError("rocprofv3 %s pass left no results" % counter)
+        raise RuntimeError("rocprofv3 %s pass left no results" % tag)
     con = sqlite3.connect(dbs[0])
-    rows = con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection where counter_name = ? "
-                       "group by dispatch_id order by dispatch_id", (counter,)).fetchall()
+    vals, copies = {}, {}
+    for counter in counters:
+        rows = con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection where counter_name = ? "
+                           "group by dispatch_id order by dispatch_id", (counter,)).fetchall()
+        dvr = [v for _, k, v in rows if "k_dvr<" in k]
+        copy = [v for _, k, v in rows if "copy" in k.lower()]
+        if len(dvr) <= first_launches or not copy:
+            con.close()
+            raise RuntimeError("rocprofv3 %s pass: no ray-caster or copy dispatches" % counter)
+        vals[counter] = dvr[first_launches:]
+        copies[counter] = max(copy)
     con.close()
-    dvr = [v for _, k, v in rows if "k_dvr" in k]
-    copy = [v for _, k, v in rows if "copy" in k.lower()]
-    if len(dvr) <= first_launches or not copy:
-        raise RuntimeError("rocprofv3 %s pass: no ray-caster or copy dispatches" % counter)
-    return dvr[first_launches:], max(copy)
+    return vals, copies
 
 
 def measure_traffic(args):
     """HBM bytes per ray-caster launch from FETCH_SIZE + WRITE_SIZE (KiB per
     dispatch), each scaled by (bytes the calibration copy moves) / (what the
-    counter reads for it)."""
+    counter reads for it), and the ray caster's L2 hit rate
+    TCC_HIT / (TCC_HIT + TCC_MISS) over the same launches.  Three rocprofv3
+    passes, one counter group each."""
     if not shutil.which("rocprofv3"):
-        return None, "rocprofv3 not found"
+        return None, "rocprofv3 not found", None
     tmp = tempfile.mkdtemp(prefix="gcvol_pmc_", dir="/tmp")
     try:
         out = {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             t0 = time.perf_counter()
-            vals, copy_kib = _pmc_pass(counter, os.path.join(tmp, counter), args)
+            vals, copies = _pmc_pass([counter], os.path.join(tmp, counter), args)
+            v, copy_kib = vals[counter], copies[counter]
             scale = (CALIB_MIB * 1024.0) / copy_kib
-            print("traffic pass %s: %d launches, copy %.0f KiB, %.1f s" % (counter, len(vals), copy_kib,
+            print("traffic pass %s: %d launches, copy %.0f KiB, %.1f s" % (counter, len(v), copy_kib,
                                                                            time.perf_counter() - t0),
                   file=sys.stderr, flush=True)
-            out[counter] = (sum(vals) / len(vals) * scale * 1024.0, scale, len(vals))
+            out[counter] = (sum(v) / len(v) * scale * 1024.0, scale, len(v))
         total = out["FETCH_SIZE"][0] + out["WRITE_SIZE"][0]
         note = ("FETCH_SIZE x%.3f + WRITE_SIZE x%.3f (copy calibration, %d MiB), mean over %d launches of frames 1-3"
                 % (out["FETCH_SIZE"][1], out["WRITE_SIZE"][1], CALIB_MIB, out["FETCH_SIZE"][2]))
-        return total, note
+        t0 = time.perf_counter()
+        vals, _ = _pmc_pass(["TCC_HIT_sum", "TCC_MISS_sum"], os.path.join(tmp, "TCC"), args)
+        hit, miss = sum(vals["TCC_HIT_sum"]), sum(vals["TCC_MISS_sum"])
+        l2 = {"hit_rate": hit / (hit + miss) if hit + miss > 0 else None,
+              "hits_per_launch": hit / len(vals["TCC_HIT_sum"]), "misses_per_launch": miss / len(vals["TCC_MISS_sum"]),
+              "note": "TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum) over the ray-caster launches of frames 1-3"}
+        print("traffic pass TCC: %d launches, L2 hit rate %.3f, %.1f s" % (len(vals["TCC_HIT_sum"]), l2["hit_rate"] or 0,
+                                                                         time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
+        return total, note, l2
     except (RuntimeError, subprocess.TimeoutExpired, sqlite3.Error) as e:
-        return None, str(e)[:300]
+        return None, str(e)[:300], None
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
@@ -399,7 +417,8 @@ def main():
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    traffic, traffic_note = (None, "not measured") if (args.no_traffic or args.serving_only) else measure_traffic(args)
+    traffic, traffic_note, l2 = ((None, "not measured", None) if (args.no_traffic or args.serving_only)
+                                 else measure_traffic(args))
 
     import gcvol as G
 
@@ -444,9 +463,9 @@ def main():
         dt = time.perf_counter() - t0
         d = rep.as_dict()
         d["wall_ms"] = dt * 1e3
-        print("frame %3d %s wall %.1f ms launches %d bricks %d samples %.3g kernel %.1f ms" % (
-            k, "timed" if k >= args.warmup else "warmup", dt * 1e3, d["launches"], d["bricks_loaded"],
-            d["mapped_samples"], d["kernel_ms"]), file=sys.stderr, flush=True)
+        print("frame %3d %s wall %.1f ms launches %d lane %d bricks %d samples %.3g kernel %.1f ms" % (
+            k, "timed" if k >= args.warmup else "warmup", dt * 1e3, d["launches"], d["lane_launches"],
+            d["bricks_loaded"], d["mapped_samples"], d["kernel_ms"]), file=sys.stderr, flush=True)
         if k >= args.warmup:
             lat.append(dt * 1e3)
             reports.append(d)
@@ -456,7 +475,12 @@ def main():
     launches = sum(r["launches"] for r in reports)
     tot = {k: sum(r[k] for r in reports) for k in ("mapped_samples", "brick_visits", "pixels")}
     B = algorithmic_bytes(tot, BYTES)
-    achieved = B / (kms / 1e3) if kms > 0 else 0.0
+    # the dominant kernel: k_dvr (tile / lane-per-ray launches), timed alone
+    lane = {k: sum(r["lane_" + k] for r in reports) for k in ("mapped_samples", "brick_visits", "pixels")}
+    lane_launches = sum(r["lane_launches"] for r in reports)
+    lane_ms = sum(r["lane_kernel_ms"] for r in reports)
+    B_lane = algorithmic_bytes(lane, BYTES)
+    achieved = B_lane / (lane_ms / 1e3) if lane_ms > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": S / total_s if total_s > 0 else 0.0,
@@ -491,9 +515,17 @@ def main():
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
             "traffic_note": traffic_note,
+            "l2": l2,
             "kernel": "k_dvr<2,%s,%s>" % ("double" if args.precision == 64 else "float", ["8,8", "16,4", "32,2", "64,1"][args.tile]),
-            "avg_launch_ms": kms / launches if launches else 0.0,
-            "algorithmic_bytes_per_launch": B / launches if launches else 0.0,
+            "launches": lane_launches,
+            "avg_launch_ms": lane_ms / lane_launches if lane_launches else 0.0,
+            "algorithmic_bytes_per_launch": B_lane / lane_launches if lane_launches else 0.0,
+            "all_ray_caster_launches": {
+                "kernels": "k_dvr + k_dvr_ray", "launches": launches,
+                "avg_launch_ms": kms / launches if launches else 0.0,
+                "algorithmic_bytes_per_launch": B / launches if launches else 0.0,
+                "achieved": (B / (kms / 1e3) if kms > 0 else 0.0) / 1e9,
+            },
         },
     }
     if not args.no_cpu_baseline:

@@ -1717,8 +1717,11 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
 // the view, the next ones only the rays the previous launch stopped at a miss
 // (the other rays are finished and would neither sample nor write anything
 // new).  Returns the index of the list the launch fills.
+// One launch of a frame.  d_stats[0] gathers the counters of tile / lane-
+// per-ray launches (k_dvr), d_stats[1] those of resumed-ray launches
+// (k_dvr_ray); *lane tells which one ran.
 static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_tf, uint32_t tf_size,
-                        gc_render_stats *d_stats, int first, int &cur, hipStream_t s, uint32_t list_rays = 0)
+                        gc_render_stats *d_stats, int first, int &cur, hipStream_t s, uint32_t list_rays, int *lane)
 {
     DvrLaunch L{};
     launch_base(C, L);
@@ -1739,8 +1742,10 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
     L.out_count = F.ctl + 8 + out;
     GC_HIP(launch_fill32(F.ctl, 8, 0, s));
     GC_HIP(launch_fill32(F.ctl + 8 + out, 1, 0, s));
-    if (!first && list_rays <= C.ray_list_max && dvr_ray_supported(L, C.vc.encoded_bytes)) {
-        // a short list of resumed rays: one wave per ray
+    *lane = !(!first && list_rays <= C.ray_list_max && dvr_ray_supported(L, C.vc.encoded_bytes));
+    if (!*lane) {
+        // a short list of resumed rays: a group of lanes per ray
+        L.stats = d_stats + 1;
         GC_HIP(launch_dvr_ray(L, C.vc.encoded_bytes, view.precision, list_rays,
                               dvr_ray_grid(C.vc.encoded_bytes, view.precision, list_rays, dvr_ray_lds_bytes(1)), s));
     } else {
@@ -1775,13 +1780,13 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
     GC_HIP(hipSetDevice(C.device));
     std::memset(report, 0, sizeof(*report));
     if (!C.d_frame_stats) {
-        GC_TRY(dalloc(&C.d_frame_stats, sizeof(gc_render_stats)));
+        GC_TRY(dalloc(&C.d_frame_stats, 2 * sizeof(gc_render_stats)));
         GC_HIP(hipEventCreate(&C.ev0));
         GC_HIP(hipEventCreate(&C.ev1));
     }
     if (view->width != f->impl.w || view->height != f->impl.h || view->precision != f->impl.precision || view->tile > 3)
         return fail(GC_ERR_ARG, "view does not match the frame");
-    GC_HIP(hipMemsetAsync(C.d_frame_stats, 0, sizeof(gc_render_stats), C.stream));
+    GC_HIP(hipMemsetAsync(C.d_frame_stats, 0, 2 * sizeof(gc_render_stats), C.stream));
     int cur = 0;
     for (uint32_t it = 0; it < max_launches; ++it) {
         uint32_t list_rays = 0;
@@ -1789,9 +1794,12 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
             GC_HIP(hipMemcpyAsync(&list_rays, f->impl.ctl + 8 + cur, 4, hipMemcpyDeviceToHost, C.stream));
         GC_HIP(hipStreamSynchronize(C.stream));
         GC_HIP(hipEventRecord(C.ev0, C.stream));
-        GC_TRY(frame_launch(C, f->impl, *view, d_tf, tf_size, C.d_frame_stats, it == 0, cur, C.stream, list_rays));
+        int lane = 1;
+        GC_TRY(frame_launch(C, f->impl, *view, d_tf, tf_size, C.d_frame_stats, it == 0, cur, C.stream, list_rays,
+                            &lane));
         GC_HIP(hipEventRecord(C.ev1, C.stream));
         ++report->launches;
+        report->lane_launches += (uint32_t)lane;
         int done = 0;
         const auto t0 = std::chrono::steady_clock::now();
         GC_TRY(C.update(&done));
@@ -1800,6 +1808,7 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
         float ms = 0.f;
         GC_HIP(hipEventElapsedTime(&ms, C.ev0, C.ev1));
         report->kernel_ms += ms;
+        if (lane) report->lane_kernel_ms += ms;
         if (done) {
             report->update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             report->complete = 1;
@@ -1813,8 +1822,14 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
         report->bricks_loaded += before;
         report->update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    GC_HIP(hipMemcpyAsync(&report->stats, C.d_frame_stats, sizeof(gc_render_stats), hipMemcpyDeviceToHost, C.stream));
+    gc_render_stats st[2];
+    GC_HIP(hipMemcpyAsync(st, C.d_frame_stats, sizeof(st), hipMemcpyDeviceToHost, C.stream));
     GC_HIP(hipStreamSynchronize(C.stream));
+    report->lane_stats = st[0];
+    report->stats.mapped_samples = st[0].mapped_samples + st[1].mapped_samples;
+    report->stats.brick_visits = st[0].brick_visits + st[1].brick_visits;
+    report->stats.pixels = st[0].pixels + st[1].pixels;
+    report->stats.unmapped_rays = st[0].unmapped_rays + st[1].unmapped_rays;
     return GC_OK;
 }
 

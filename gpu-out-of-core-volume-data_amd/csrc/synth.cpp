@@ -104,45 +104,28 @@ void Synth::fill_box(uint32_t level, const int64_t o[3], const uint32_t n[3], co
     // per-row x keys of the noise hash
     std::vector<uint32_t> xkey(n[0]);
     for (uint32_t i = 0; i < n[0]; ++i) xkey[i] = (uint32_t)(o[0] + i) * 0x9e3779b1u;
-    const uint32_t span = 2u * prm.noise + 1u;
-    const float fmax = (float)maxval;
-    std::vector<float> row(n[0]);
+    SynthRows a;
+    a.level = level;
+    a.seed = (uint32_t)prm.seed;
+    a.noise = prm.noise;
+    a.bytes = bytes;
+    a.fmax = (float)maxval;
+    a.nb = nb;
+    a.o = o;
+    a.n = n;
+    a.ext = ext;
+    a.fx = fx.data();
+    a.fy = fy.data();
+    a.fz = fz.data();
+    a.xkey = xkey.data();
+    std::vector<float> row(4 * (size_t)n[0]);
     std::vector<int32_t> nz(n[0]);
-    // x range inside the level
-    const int64_t x0 = std::max<int64_t>(0, -o[0]), x1 = std::min<int64_t>((int64_t)n[0], (int64_t)ext[0] - o[0]);
-    for (uint32_t kz = 0; kz < n[2]; ++kz) {
-        const int64_t gz = o[2] + kz;
-        for (uint32_t ky = 0; ky < n[1]; ++ky) {
-            const int64_t gy = o[1] + ky;
-            const bool row_in = gz >= 0 && gy >= 0 && gz < (int64_t)ext[2] && gy < (int64_t)ext[1];
-            if (!row_in || x1 <= x0) continue;   // stays 0
-            const uint64_t rbase = (uint64_t)n[0] * (ky + (uint64_t)n[1] * kz);
-            std::fill(row.begin(), row.end(), 0.f);
-            for (size_t k = 0; k < nb; ++k) {
-                const float w = fz[k * n[2] + kz] * fy[k * n[1] + ky];
-                if (w == 0.f) continue;
-                const float *f = &fx[k * n[0]];
-                float *r = row.data();
-                for (uint32_t i = 0; i < n[0]; ++i) r[i] += w * f[i];
-            }
-            if (prm.noise) {
-                const uint32_t rk = (uint32_t)prm.seed ^ (level * 0x27d4eb2fu) ^ ((uint32_t)gz * 0x165667b1u) ^
-                                    ((uint32_t)gy * 0xc2b2ae3du);
-                for (uint32_t i = 0; i < n[0]; ++i)
-                    nz[i] = (int32_t)(((uint64_t)hash32(rk ^ xkey[i]) * span) >> 32) - (int32_t)prm.noise;
-            } else {
-                std::fill(nz.begin(), nz.end(), 0);
-            }
-            for (int64_t kx = x0; kx < x1; ++kx) {
-                float v = (float)(int32_t)(row[kx] + 0.5f) + (float)nz[kx];   // row >= 0: round half up
-                v = v < 0.f ? 0.f : (v > fmax ? fmax : v);
-                const uint64_t di = rbase + kx;
-                if (bytes == 1) static_cast<uint8_t *>(dst)[di] = (uint8_t)v;
-                else if (bytes == 2) static_cast<uint16_t *>(dst)[di] = (uint16_t)v;
-                else static_cast<float *>(dst)[di] = v;
-            }
-        }
-    }
+    a.row = row.data();
+    a.nz = nz.data();
+    std::vector<float> wz(nb * n[1]);
+    a.wz = wz.data();
+    a.dst = dst;
+    synth_rows(a);
 }
 
 // -------------------------------------------------------- transfer function

@@ -33,6 +33,30 @@ struct Synth {
                   void *dst) const;
 };
 
+// Arguments of the voxel loop of Synth::fill_box: separable blob factors
+// fx[k][i], fy[k][j], amp*fz[k][l] of the nb blobs that reach the box, the
+// x keys of the noise hash and scratch for 4 rows of sums (row, 4*n[0]
+// floats) and one row of noise (nz, n[0] entries).
+struct SynthRows {
+    uint32_t level, seed, noise, bytes;
+    float fmax;
+    size_t nb;
+    const int64_t *o;
+    const uint32_t *n, *ext;
+    const float *fx, *fy, *fz;
+    const uint32_t *xkey;
+    float *row;
+    int32_t *nz;
+    float *wz;   // scratch, nb * n[1] floats
+    void *dst;
+};
+// synth_rows.cpp is compiled three times (AVX-512, AVX2, baseline x86-64);
+// synth_rows() picks the widest one the host CPU runs.
+void synth_rows(const SynthRows &a);
+void synth_rows_avx512(const SynthRows &a);
+void synth_rows_avx2(const SynthRows &a);
+void synth_rows_base(const SynthRows &a);
+
 void tf_samples(const float *knots, uint32_t nk, uint32_t n, float *out);
 void tf_seeded(uint64_t seed, uint32_t nk, float *knots);
 void camera_matrix(float rx, float ry, const float t[3], float out[12]);

@@ -78,6 +78,7 @@ class FrameReport(C.Structure):
     _fields_ = [
         ("launches", u32), ("updates", u32), ("complete", u32), ("reserved", u32), ("bricks_loaded", u64),
         ("stats", RenderStats), ("kernel_ms", C.c_double), ("update_ms", C.c_double),
+        ("lane_launches", u32), ("reserved2", u32), ("lane_stats", RenderStats), ("lane_kernel_ms", C.c_double),
     ]
 
     def as_dict(self):
@@ -87,6 +88,9 @@ class FrameReport(C.Structure):
             "brick_visits": int(self.stats.brick_visits), "pixels": int(self.stats.pixels),
             "unmapped_rays": int(self.stats.unmapped_rays), "kernel_ms": self.kernel_ms,
             "update_ms": self.update_ms,
+            "lane_launches": self.lane_launches, "lane_kernel_ms": self.lane_kernel_ms,
+            "lane_mapped_samples": int(self.lane_stats.mapped_samples),
+            "lane_brick_visits": int(self.lane_stats.brick_visits), "lane_pixels": int(self.lane_stats.pixels),
         }
 
 

@@ -300,6 +300,13 @@ typedef struct gc_frame_report {
     gc_render_stats stats;       /* summed over the launches                   */
     double   kernel_ms;          /* summed ray-caster time (HIP events)        */
     double   update_ms;          /* summed host time inside update()           */
+    /* The share of the launches above that ran the tile / lane-per-ray march
+     * (k_dvr); the others resumed a short ray list with the group-per-ray
+     * march (k_dvr_ray).  Filled by gc_render_frame, zero from batches. */
+    uint32_t lane_launches;
+    uint32_t reserved2;
+    gc_render_stats lane_stats;
+    double   lane_kernel_ms;
 } gc_frame_report;
 
 /* The plain single-view renderer's progressive loop (VolumeRayCaster.cu:
