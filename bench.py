@@ -145,7 +145,8 @@ SERVE_POOL = 12 << 30          # per volume; 6 pools + view slots within the 96 
 SERVE_BUDGET = 96 << 30
 
 
-def serving_workload(G, rates=(4, 8, 16, 32), seconds=4.0, max_views=16, warm_requests=24):
+def serving_workload(G, rates=(4, 8, 16, 32), seconds=4.0, max_views=16, warm_requests=24, fixed_ab=False,
+                     volumes=None):
     """Requests arrive as a Poisson process (seeded) over 6 volumes (512^3 u8,
     1024^3 u8, 1024^3 u16, 2048^3 u8, 2048x2048x1024 u16, 4096^3 u8, each
     synthesised brick by brick on the host on demand, 32^3 bricks, 12 GiB
@@ -176,11 +177,14 @@ def serving_workload(G, rates=(4, 8, 16, 32), seconds=4.0, max_views=16, warm_re
             side = int(rng.integers(4, 33)) * 64
             cam = G.camera(float(rng.uniform(-30.0, 30.0)), float(rng.uniform(0.0, 360.0)),
                            float(rng.uniform(2.2, 3.6)))
-            out.append((t, int(rng.integers(len(SERVE_VOLUMES))), side, cam, int(rng.integers(len(tables)))))
+            vol = int(rng.integers(len(SERVE_VOLUMES)))
+            if volumes is not None:
+                vol = volumes[vol % len(volumes)]
+            out.append((t, vol, side, cam, int(rng.integers(len(tables)))))
 
-    def serve(reqs, plain):
+    def serve(reqs, plain, fixed=False):
         srv = G.Server(SERVE_BUDGET, max_views=1 if plain else max_views, slot_pixels=2048 * 2048, precision=32,
-                       plain=plain)
+                       plain=plain, fixed_views=fixed)
         vids = [srv.add_volume(vc, src, SERVE_POOL, max_requests=4096) for vc, src in zip(vcs, sources)]
         # warm: an untimed burst from the same request distribution
         for (_, vol, side, cam, tfi) in warm:
@@ -192,7 +196,8 @@ def serving_workload(G, rates=(4, 8, 16, 32), seconds=4.0, max_views=16, warm_re
                 break
             for r in rs:
                 srv.take_pixels(r.id, r.width, r.height)
-        s0 = srv.stats().samples
+        st0 = srv.stats()
+        s0 = st0.samples
         sizes, lat = {}, []
         t_start = time.perf_counter()
 
@@ -215,10 +220,17 @@ def serving_workload(G, rates=(4, 8, 16, 32), seconds=4.0, max_views=16, warm_re
             time.sleep(0.001)
         th.join()
         wall = time.perf_counter() - t_start
-        samples = srv.stats().samples - s0
+        st1 = srv.stats()
+        samples = st1.samples - s0
         srv.close()
         return {"completed": done, "wall_s": wall, "frames_per_s": done / wall, "samples_per_s": samples / wall,
-                "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99)}
+                "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+                "bricks_loaded": st1.bricks_loaded - st0.bricks_loaded,
+                "pressure_events": st1.pressure_events - st0.pressure_events,
+                "scheduler": {"iterations": st1.iterations - st0.iterations,
+                              "idle_iterations": st1.idle_iterations - st0.idle_iterations,
+                              "launch_ms": st1.launch_ms - st0.launch_ms, "update_ms": st1.update_ms - st0.update_ms,
+                              "fetch_ms": st1.fetch_ms - st0.fetch_ms}}
 
     warm = []
     while len(warm) < warm_requests:
@@ -228,11 +240,20 @@ def serving_workload(G, rates=(4, 8, 16, 32), seconds=4.0, max_views=16, warm_re
     for i, rate in enumerate(rates):
         reqs = make_requests(rate, SEED + 300 + i)
         b = serve(reqs, False)
+        bf = serve(reqs, False, fixed=True) if fixed_ab else None
         pl = serve(reqs, True)
-        print("serving %d req/s: %d requests; batched p50 %.0f p99 %.0f ms %.1f fps | plain p50 %.0f p99 %.0f ms %.1f fps"
-              % (rate, len(reqs), b["p50_ms"], b["p99_ms"], b["frames_per_s"], pl["p50_ms"], pl["p99_ms"],
-                 pl["frames_per_s"]), file=sys.stderr, flush=True)
+        print("serving %d req/s: %d requests; batched p50 %.0f p99 %.0f ms %.1f fps %d bricks %d pressure | plain p50 "
+              "%.0f p99 %.0f ms %.1f fps %d bricks" % (rate, len(reqs), b["p50_ms"], b["p99_ms"], b["frames_per_s"],
+                                                       b["bricks_loaded"], b["pressure_events"], pl["p50_ms"],
+                                                       pl["p99_ms"], pl["frames_per_s"], pl["bricks_loaded"]),
+              file=sys.stderr, flush=True)
+        if bf:
+            print("serving %d req/s: batched, fixed %d views: p50 %.0f p99 %.0f ms %.1f fps %d bricks"
+                  % (rate, max_views, bf["p50_ms"], bf["p99_ms"], bf["frames_per_s"], bf["bricks_loaded"]),
+                  file=sys.stderr, flush=True)
         out.append({"rate": rate, "requests": len(reqs), "batched": b, "plain": pl})
+        if bf:
+            out[-1]["batched_fixed_views"] = bf
     return {"workload": "Poisson arrivals for %.0f s per rate over 6 volumes synthesised on demand (512^3 u8, "
                         "1024^3 u8, 1024^3 u16, 2048^3 u8, 2048x2048x1024 u16, 4096^3 u8; 32^3 bricks, 12 GiB "
                         "pool each, 96 GiB budget), square views 256..2048, own camera and TF per request; "
@@ -408,6 +429,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain workload")
     ap.add_argument("--serving", action="store_true", help="also run the serving workload (several minutes)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving workload")
+    ap.add_argument("--serving-rates", default="4,8,16,32", help="request rates (req/s) of the serving workload")
+    ap.add_argument("--serving-volumes", default="", help="serve only these volumes (indices into the six)")
+    ap.add_argument("--serving-ab", action="store_true",
+                    help="also serve each arrival sequence with a fixed number of views per volume")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known traffic "
@@ -426,7 +451,9 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
     if args.serving_only:
-        print(json.dumps({"serving": serving_workload(G)}))
+        rates = tuple(int(r) for r in args.serving_rates.split(","))
+        vols = [int(v) for v in args.serving_volumes.split(",")] if args.serving_volumes else None
+        print(json.dumps({"serving": serving_workload(G, rates=rates, fixed_ab=args.serving_ab, volumes=vols)}))
         return
 
     if args.calib_copy_mib > 0:

@@ -212,9 +212,12 @@ uint32_t dvr_grid(uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t n_
 // ray at a time (the group size follows `rays`, the list length).
 bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes);
 uint32_t dvr_ray_lds_bytes(uint32_t n_views);
-uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t lds_bytes);
+// group-per-ray march: group = lanes per ray (16 or 64; 0 = by the list
+// length: 64 up to 256 rays, else 16)
+uint32_t dvr_ray_group(uint32_t rays);
+uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t lds_bytes, uint32_t group = 0);
 hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t grid,
-                          hipStream_t stream);
+                          hipStream_t stream, uint32_t group = 0);
 hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t tile,
                       uint32_t grid, hipStream_t stream);
 hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream);

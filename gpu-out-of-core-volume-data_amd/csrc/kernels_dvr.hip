@@ -1050,11 +1050,17 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         tnext = l + 1 < nl ? tsw[l + 1] : FINF;
     };
 
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
     for (;;) {
-        // a group without a ray takes the next one (the whole group together)
+        // a group without a ray takes the next one (the whole group
+        // together); the wave's groups take consecutive list entries with
+        // one atomic
+        const uint64_t want = __ballot(!active && !exhausted && gl == 0);
         if (!active && !exhausted) {
-            uint32_t r = 0;
-            if (gl == 0) r = atomicAdd(&p.queue[0], 1u);
+            uint32_t base = 0;
+            if (lane == (uint32_t)__builtin_ctzll(want)) base = atomicAdd(&p.queue[0], (uint32_t)__popcll(want));
+            base = __shfl(base, __builtin_ctzll(want), 64);
+            uint32_t r = base + (uint32_t)__popcll(want & lt_mask);
             r = __shfl(r, 0, G);
             if (r >= list_count) {
                 exhausted = true;
@@ -1316,23 +1322,26 @@ bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes) { return p.pipelined 
 
 // Lanes per ray: a whole wave for the shortest lists (lowest latency per
 // ray), 16 lanes (4 rays per wave) otherwise.
-static uint32_t ray_group(uint32_t rays) { return rays <= 256u ? 64u : 16u; }
+uint32_t dvr_ray_group(uint32_t rays) { return rays <= 256u ? 64u : 16u; }
+
+template <int BYTES, typename ACC>
+static const void *ray_kernel_of(uint32_t group)
+{
+    switch (group) {
+    case 16: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 16>);
+    case 64: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 64>);
+    default: return nullptr;
+    }
+}
 
 static const void *ray_kernel(uint32_t bytes, uint32_t precision, uint32_t group)
 {
     const bool d = precision == 64;
-    if (group == 64)
-        return bytes == 1 ? (d ? reinterpret_cast<const void *>(&k_dvr_ray<1, double, 64>)
-                               : reinterpret_cast<const void *>(&k_dvr_ray<1, float, 64>))
-                          : (d ? reinterpret_cast<const void *>(&k_dvr_ray<2, double, 64>)
-                               : reinterpret_cast<const void *>(&k_dvr_ray<2, float, 64>));
-    return bytes == 1 ? (d ? reinterpret_cast<const void *>(&k_dvr_ray<1, double, 16>)
-                           : reinterpret_cast<const void *>(&k_dvr_ray<1, float, 16>))
-                      : (d ? reinterpret_cast<const void *>(&k_dvr_ray<2, double, 16>)
-                           : reinterpret_cast<const void *>(&k_dvr_ray<2, float, 16>));
+    if (bytes == 1) return d ? ray_kernel_of<1, double>(group) : ray_kernel_of<1, float>(group);
+    return d ? ray_kernel_of<2, double>(group) : ray_kernel_of<2, float>(group);
 }
 
-uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t lds_bytes)
+uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t lds_bytes, uint32_t group)
 {
     // resident workgroups of 4 waves, never more groups than rays
     static int cus = 0;
@@ -1342,11 +1351,11 @@ uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
     }
-    const uint32_t group = ray_group(rays);
+    if (group == 0) group = dvr_ray_group(rays);
+    const void *k = ray_kernel(bytes, precision, group);
+    if (!k) return 0;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ray_kernel(bytes, precision, group), LANES, lds_bytes) !=
-            hipSuccess ||
-        per_cu <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, LANES, lds_bytes) != hipSuccess || per_cu <= 0)
         per_cu = 2;
     const uint32_t want = (uint32_t)cus * (uint32_t)per_cu;
     const uint64_t rays_per_block = (uint64_t)WAVES * (64u / group);
@@ -1355,15 +1364,17 @@ uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_
 }
 
 hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t grid,
-                          hipStream_t stream)
+                          hipStream_t stream, uint32_t group)
 {
     if (!p.list || !p.list_count || p.n_packets != 0 || grid == 0) return hipErrorInvalidValue;
     if (!dvr_ray_supported(p, bytes)) return hipErrorInvalidValue;
     if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
+    if (group == 0) group = dvr_ray_group(rays);
+    const void *k = ray_kernel(bytes, precision, group);
+    if (!k) return hipErrorInvalidValue;
     void *args[] = {const_cast<DvrLaunch *>(&p)};
-    return hipLaunchKernel(ray_kernel(bytes, precision, ray_group(rays)), dim3(grid), dim3(LANES), args,
-                           dvr_ray_lds_bytes(p.views ? p.n_views : 1), stream);
+    return hipLaunchKernel(k, dim3(grid), dim3(LANES), args, dvr_ray_lds_bytes(p.views ? p.n_views : 1), stream);
 }
 
 hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream)

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <atomic>
+#include <deque>
 #include <chrono>
 #include <condition_variable>
 #include <functional>
@@ -44,10 +45,113 @@ int fail(int code, const std::string &msg)
 
 static uint32_t hw_threads();
 
-// Extra worker threads available to parallel_for across the whole process,
-// so that several request handlers working at once share the host cores
-// instead of each starting a full set of threads.
-static std::atomic<int> g_thread_tokens{-1};
+// One process-wide pool of host worker threads for parallel_for.  Several
+// request handlers (one per cache) can fetch bricks at once: every job sits
+// in one queue that the workers serve round-robin, a few items at a time, so
+// concurrent jobs share all the cores for as long as they run (a job that
+// starts while another holds the cores is not left with one thread).  The
+// calling thread works on its own job too, so a job always progresses.
+namespace {
+struct PoolJob {
+    const std::function<void(uint32_t)> *fn = nullptr;
+    uint32_t n = 0;
+    uint32_t cap = 1;                    // most threads at once, the caller included
+    std::atomic<uint32_t> next{0};       // next unclaimed item
+    std::atomic<uint32_t> left{0};       // items not finished
+    uint32_t workers = 0;                // pool threads inside the job (pool lock)
+    bool queued = true;                  // still in the pool's queue (pool lock)
+};
+
+class WorkPool {
+public:
+    static WorkPool &get()
+    {
+        // never destroyed: its threads wait for work until the process ends,
+        // so a handler still running at exit never meets a destroyed pool
+        static WorkPool *pool = new WorkPool();
+        return *pool;
+    }
+
+    void run(uint32_t n, uint32_t cap, const std::function<void(uint32_t)> &fn)
+    {
+        PoolJob j;
+        j.fn = &fn;
+        j.n = n;
+        j.cap = cap;
+        j.left = n;
+        {
+            std::lock_guard<std::mutex> lk(m);
+            jobs.push_back(&j);
+        }
+        cv.notify_all();
+        for (;;) {
+            const uint32_t i = j.next.fetch_add(1);
+            if (i >= n) break;
+            fn(i);
+            j.left.fetch_sub(1);
+        }
+        std::unique_lock<std::mutex> lk(m);
+        dequeue(&j);
+        done_cv.wait(lk, [&] { return j.left.load() == 0 && j.workers == 0; });
+    }
+
+private:
+    static constexpr uint32_t BATCH = 4;   // items per visit to a job
+    std::mutex m;
+    std::condition_variable cv, done_cv;
+    std::deque<PoolJob *> jobs;            // jobs with unclaimed items
+    std::vector<std::thread> threads;
+
+    WorkPool()
+    {
+        const uint32_t n = hw_threads();
+        for (uint32_t t = 0; t < n; ++t) threads.emplace_back([this] { loop(); });
+    }
+    void dequeue(PoolJob *j)
+    {
+        if (!j->queued) return;
+        jobs.erase(std::find(jobs.begin(), jobs.end(), j));
+        j->queued = false;
+    }
+    void loop()
+    {
+        std::unique_lock<std::mutex> lk(m);
+        for (;;) {
+            PoolJob *j = nullptr;
+            cv.wait(lk, [&] {
+                // the next job in round-robin order that has room for a thread
+                for (size_t k = 0; k < jobs.size(); ++k) {
+                    PoolJob *c = jobs.front();
+                    jobs.pop_front();
+                    jobs.push_back(c);
+                    if (c->workers + 1 < c->cap) {
+                        j = c;
+                        return true;
+                    }
+                }
+                return false;
+            });
+            ++j->workers;
+            lk.unlock();
+            bool exhausted = false;
+            for (uint32_t k = 0; k < BATCH; ++k) {
+                const uint32_t i = j->next.fetch_add(1);
+                if (i >= j->n) {
+                    exhausted = true;
+                    break;
+                }
+                (*j->fn)(i);
+                j->left.fetch_sub(1);
+            }
+            lk.lock();
+            --j->workers;
+            if (exhausted) dequeue(j);
+            done_cv.notify_all();
+            cv.notify_all();   // a slot in this job may have freed up
+        }
+    }
+};
+} // namespace
 
 static void parallel_for(uint32_t n, uint32_t threads, const std::function<void(uint32_t)> &fn)
 {
@@ -55,29 +159,7 @@ static void parallel_for(uint32_t n, uint32_t threads, const std::function<void(
         for (uint32_t i = 0; i < n; ++i) fn(i);
         return;
     }
-    int expect = -1;
-    g_thread_tokens.compare_exchange_strong(expect, (int)hw_threads() - 1);
-    int want = (int)std::min(threads, n) - 1, have = g_thread_tokens.load();
-    int take = 0;
-    while (want > 0 && have > 0) {
-        take = std::min(want, have);
-        if (g_thread_tokens.compare_exchange_weak(have, have - take)) break;
-        take = 0;
-    }
-    threads = (uint32_t)take + 1;
-    std::atomic<uint32_t> next{0};
-    std::vector<std::thread> pool;
-    auto work = [&]() {
-        for (;;) {
-            const uint32_t i = next.fetch_add(1);
-            if (i >= n) break;
-            fn(i);
-        }
-    };
-    for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work);
-    work();
-    for (auto &t : pool) t.join();
-    if (take) g_thread_tokens.fetch_add(take);
+    WorkPool::get().run(n, std::min(threads, n), fn);
 }
 
 // parallel_for over n items in chunks of `chunk` consecutive items:
@@ -447,6 +529,8 @@ struct Cache {
     Handler handler;
     std::mutex done_m;
     bool request_done = false;
+    gc_cache_counters counters{};
+    std::atomic<uint64_t> fetch_ns{0};   // the handler thread's fetch time
     // frame driver
     gc_render_stats *d_frame_stats = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -583,6 +667,7 @@ int Cache::prepare_bricks(bool &done)
     // prepare_bricks + create_request_list (CacheManager.hpp:410-491)
     GC_HIP(launch_miss_compact(request, g.mrpd_cells, ts, d_geom, limit, d_misses, miss_s, stream));
     GC_HIP(hipMemcpyAsync(h_misses, miss_s.total, 4, hipMemcpyDeviceToHost, stream));
+    GC_HIP(hipMemcpyAsync(h_misses + 1, lru_s.total, 4, hipMemcpyDeviceToHost, stream));   // slots used
     GC_HIP(hipMemcpyAsync(h_misses + 4, d_misses, (size_t)cap * 16, hipMemcpyDeviceToHost, stream));
     GC_HIP(hipStreamSynchronize(stream));
     const uint32_t n = std::min(h_misses[0], limit);
@@ -591,6 +676,9 @@ int Cache::prepare_bricks(bool &done)
         return GC_OK;
     }
     done = false;
+    ++counters.requests;
+    counters.last_used = h_misses[1];
+    counters.last_requested = n;
     std::vector<uint32_t> list(h_misses + 4, h_misses + 4 + 4 * (size_t)n);
     if (cc.sync_requests) {
         GC_TRY(handler.handle(list));
@@ -622,6 +710,8 @@ int Cache::load_new_bricks()
     // CacheManager.hpp:503-632
     const uint32_t n = handler.n, nne = handler.nne;
     if (n == 0) return GC_OK;
+    ++counters.loads;
+    counters.bricks_loaded += nne;
     GC_HIP(hipStreamWaitEvent(stream, uploaded, 0));
     const uint32_t *d_asked = d_meta;
     const uint32_t *d_bbi = d_meta + 4 * (size_t)cap;
@@ -639,6 +729,7 @@ int Cache::load_new_bricks()
 int Cache::update(int *done)
 {
     // CacheManager.hpp:319-354
+    ++counters.updates;
     GC_TRY(lru_update());
     if (handler.new_request.load()) {
         ++ts;
@@ -709,7 +800,10 @@ void Handler::loop()
             if (!run) return;
             list = incoming;
         }
+        const auto t0 = std::chrono::steady_clock::now();
         const int rc = handle(list);
+        c->fetch_ns += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                           std::chrono::steady_clock::now() - t0).count();
         if (rc != GC_OK) {
             status = rc;
             error = gc_last_error();
@@ -1486,6 +1580,15 @@ int gc_cache_timestamp(gc_cache *c, uint32_t *ts)
 {
     if (!c || !ts) return fail(GC_ERR_ARG, "null argument");
     *ts = c->impl.ts;
+    return GC_OK;
+}
+
+int gc_cache_counters_get(gc_cache *c, gc_cache_counters *out)
+{
+    if (!c || !out) return fail(GC_ERR_ARG, "null argument");
+    *out = c->impl.counters;
+    out->slots = c->impl.g.nslots;
+    out->fetch_ms = (double)c->impl.fetch_ns.load() / 1e6;
     return GC_OK;
 }
 

@@ -58,7 +58,18 @@ struct Volume {
     std::vector<Request *> slots;
     std::vector<bool> fresh;            // slot placed, not launched yet
     bool load_pending = false;
-    uint64_t last_use = 0, samples_seen = 0;
+    uint64_t last_use = 0, samples_seen = 0, bricks_seen = 0;
+    double fetch_seen = 0.0;
+    // concurrent views admitted (limit by pool pressure); pressure seen
+    // since the last view of this volume completed
+    uint32_t cap = 0;
+    bool pressure = false;
+    uint32_t active() const
+    {
+        uint32_t k = 0;
+        for (auto *r : slots) k += r ? 1u : 0u;
+        return k;
+    }
     bool resident() const { return cache != nullptr; }
     bool busy_slots() const
     {
@@ -100,8 +111,7 @@ struct Server {
     void loop();
     int iterate(std::unique_lock<std::mutex> &lk);
     int device_phase(const std::vector<Volume *> &evict, const std::vector<Volume *> &create,
-                     const std::vector<Place> &place, std::vector<Request *> &done, uint64_t &samples,
-                     uint64_t &launches);
+                     const std::vector<Place> &place, std::vector<Request *> &done, gc_server_stats &d);
     int make_resident(Volume &v);
     void release(Volume &v);
 };
@@ -114,6 +124,8 @@ void Server::release(Volume &v)
     v.cache = nullptr;
     v.load_pending = false;
     v.samples_seen = 0;
+    v.bricks_seen = 0;
+    v.fetch_seen = 0.0;
 }
 
 int Server::make_resident(Volume &v)
@@ -142,9 +154,9 @@ int Server::make_resident(Volume &v)
 // Device work of one iteration (no lock held): evictions, new pools, new
 // views, launches, updates, retirement of finished views.
 int Server::device_phase(const std::vector<Volume *> &evict, const std::vector<Volume *> &create,
-                         const std::vector<Place> &place, std::vector<Request *> &done, uint64_t &samples,
-                         uint64_t &launches)
+                         const std::vector<Place> &place, std::vector<Request *> &done, gc_server_stats &d)
 {
+    const auto t0 = Clock::now();
     for (Volume *v : evict) release(*v);
     for (Volume *v : create) GC_TRY(make_resident(*v));
     for (const Place &p : place)
@@ -160,20 +172,38 @@ int Server::device_phase(const std::vector<Volume *> &evict, const std::vector<V
         GC_TRY(gc_cache_busy(v.cache, &busy));
         if (busy && !fresh) continue;                   // every ray waits for the handler
         if (!busy && v.load_pending) {
-            int d = 0;
-            GC_TRY(gc_cache_update(v.cache, &d));          // load what the handler fetched
+            int complete = 0;
+            GC_TRY(gc_cache_update(v.cache, &complete));   // load what the handler fetched
             v.load_pending = false;
         }
         // while the handler fetches, only the starting views are traced
         GC_TRY(gc_batch_launch(v.batch, busy ? GC_LAUNCH_STARTING_ONLY : 0u, nullptr));
         launched.push_back(&v);
     }
+    const auto t1 = Clock::now();
+    d.launch_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (launched.empty()) ++d.idle_iterations;
     // updates and retirement
     std::vector<uint32_t> pend(cfg.max_views);
     for (Volume *v : launched) {
-        int d = 0;
-        GC_TRY(gc_cache_update(v->cache, &d));
-        if (!d) v->load_pending = true;
+        int complete = 0;
+        gc_cache_counters before{}, after{};
+        GC_TRY(gc_cache_counters_get(v->cache, &before));
+        GC_TRY(gc_cache_update(v->cache, &complete));
+        GC_TRY(gc_cache_counters_get(v->cache, &after));
+        if (!complete) v->load_pending = true;
+        if (after.requests != before.requests &&
+            (uint64_t)after.last_used + after.last_requested > after.slots) {
+            // the working set of the views in flight no longer fits the pool:
+            // admit no more than half of them until views complete
+            ++d.pressure_events;
+            v->pressure = true;
+            v->cap = std::max(1u, v->active() / 2);
+        }
+        d.bricks_loaded += after.bricks_loaded - v->bricks_seen;
+        v->bricks_seen = after.bricks_loaded;
+        d.fetch_ms += after.fetch_ms - v->fetch_seen;
+        v->fetch_seen = after.fetch_ms;
         GC_TRY(gc_batch_pending(v->batch, pend.data()));
         for (uint32_t s = 0; s < cfg.max_views; ++s) {
             Request *r = v->slots[s];
@@ -187,14 +217,17 @@ int Server::device_phase(const std::vector<Volume *> &evict, const std::vector<V
             r->t_done = now_ms();
             v->slots[s] = nullptr;
             done.push_back(r);
+            if (!v->pressure) v->cap = std::min(cfg.max_views, v->cap + 1);
+            v->pressure = false;
         }
         gc_render_stats rs{};
         GC_TRY(gc_batch_stats(v->batch, &rs));
-        samples += rs.mapped_samples - v->samples_seen;
+        d.samples += rs.mapped_samples - v->samples_seen;
         v->samples_seen = rs.mapped_samples;
         v->last_use = ++tick;
-        ++launches;
+        ++d.launches;
     }
+    d.update_ms += std::chrono::duration<double, std::milli>(Clock::now() - t1).count();
     return GC_OK;
 }
 
@@ -225,9 +258,13 @@ int Server::iterate(std::unique_lock<std::mutex> &lk)
             create.push_back(&v);
             projected += v.footprint;
         }
+        if (v.cap == 0 || cfg.fixed_views) v.cap = cfg.max_views;
+        uint32_t active = v.active();
         for (uint32_t s = 0; s < cfg.max_views && !v.queue.empty(); ++s) {
             if (v.slots[s]) continue;
             if (cfg.plain && in_flight > 0) break;
+            if (active >= v.cap) break;   // pool pressure: wait for views to complete
+            ++active;
             Request *r = v.queue.front();
             v.queue.pop_front();
             v.slots[s] = r;
@@ -241,11 +278,18 @@ int Server::iterate(std::unique_lock<std::mutex> &lk)
     st.pools_created += create.size();
     lk.unlock();
     std::vector<Request *> done;
-    uint64_t samples = 0, launches = 0;
-    const int rc = device_phase(evict, create, place, done, samples, launches);
+    gc_server_stats d{};
+    const int rc = device_phase(evict, create, place, done, d);
     lk.lock();
-    st.samples += samples;
-    st.launches += launches;
+    st.samples += d.samples;
+    st.launches += d.launches;
+    st.bricks_loaded += d.bricks_loaded;
+    st.pressure_events += d.pressure_events;
+    ++st.iterations;
+    st.idle_iterations += d.idle_iterations;
+    st.launch_ms += d.launch_ms;
+    st.update_ms += d.update_ms;
+    st.fetch_ms += d.fetch_ms;
     for (Request *r : done) {
         if (r->d_tf) gc_free(r->d_tf);
         r->d_tf = nullptr;

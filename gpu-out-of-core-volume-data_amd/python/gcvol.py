@@ -103,7 +103,7 @@ class CacheView(C.Structure):
 
 class ServerCfg(C.Structure):
     _fields_ = [("pool_budget", u64), ("max_views", u32), ("slot_pixels", u32), ("precision", u32), ("tile", u32),
-                ("march", u32), ("plain", u32), ("graph", u32), ("reserved", u32)]
+                ("march", u32), ("plain", u32), ("graph", u32), ("fixed_views", u32)]
 
 
 class ServerResult(C.Structure):
@@ -113,7 +113,14 @@ class ServerResult(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("launches", u64), ("samples", u64),
-                ("pools_created", u64), ("pools_evicted", u64)]
+                ("pools_created", u64), ("pools_evicted", u64), ("bricks_loaded", u64), ("pressure_events", u64),
+                ("iterations", u64), ("idle_iterations", u64), ("launch_ms", C.c_double), ("update_ms", C.c_double),
+                ("fetch_ms", C.c_double)]
+
+
+class CacheCounters(C.Structure):
+    _fields_ = [("updates", u64), ("loads", u64), ("bricks_loaded", u64), ("requests", u64), ("slots", u32),
+                ("last_used", u32), ("last_requested", u32), ("reserved", u32), ("fetch_ms", C.c_double)]
 
 
 EXPORTS = [
@@ -134,7 +141,7 @@ EXPORTS = [
     "gc_batch_pending", "gc_batch_stats", "gc_batch_read_state", "gc_cache_set_max_requests",
     "gc_cache_set_ray_list_max",
     "gc_cache_busy", "gc_source_create_gpu_mip", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
-    "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get",
+    "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get", "gc_cache_counters_get",
 ]
 
 _lib = None
@@ -173,6 +180,7 @@ def lib():
             "gc_cache_destroy": [P], "gc_cache_update": [P, C.POINTER(I)],
             "gc_cache_completude": [P, C.POINTER(f32)], "gc_cache_load_all": [P, P],
             "gc_cache_timestamp": [P, C.POINTER(u32)], "gc_cache_stream": [P, C.POINTER(vp)],
+            "gc_cache_counters_get": [P, C.POINTER(CacheCounters)],
             "gc_cache_view_get": [P, C.POINTER(CacheView)], "gc_cache_geometry": [P, C.POINTER(Geometry)],
             "gc_geometry_compute": [C.POINTER(VolumeCfg), C.POINTER(CacheCfg), C.POINTER(Geometry)],
             "gc_cache_read_mrpd": [P, P], "gc_cache_read_request": [P, P], "gc_cache_write_request": [P, P],
@@ -423,6 +431,11 @@ class Cache:
         check(lib().gc_cache_stream(self.h, C.byref(s)))
         return s
 
+    def counters(self):
+        c = CacheCounters()
+        check(lib().gc_cache_counters_get(self.h, C.byref(c)))
+        return {f: getattr(c, f) for f, _ in CacheCounters._fields_ if f != "reserved"}
+
     def update(self):
         d = C.c_int(0)
         check(lib().gc_cache_update(self.h, C.byref(d)))
@@ -624,9 +637,9 @@ class Server:
     """The serving loop (gc_server_*)."""
 
     def __init__(self, pool_budget, max_views=64, slot_pixels=2048 * 2048, precision=32, tile=0, march=0,
-                 plain=False, graph=False, device=0):
+                 plain=False, graph=False, device=0, fixed_views=False):
         cfg = ServerCfg(pool_budget, max_views, slot_pixels, precision, tile, march, 1 if plain else 0,
-                        1 if graph else 0, 0)
+                        1 if graph else 0, 1 if fixed_views else 0)
         self.h = vp()
         self.keep = []
         check(lib().gc_server_create(C.byref(cfg), device, C.byref(self.h)))

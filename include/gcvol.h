@@ -193,6 +193,26 @@ int gc_cache_load_all(gc_cache *c, const void *const *d_levels);
 int gc_cache_timestamp(gc_cache *c, uint32_t *ts);
 int gc_cache_stream(gc_cache *c, void **stream);
 
+/* Running counters of a cache (no reference counterpart; for schedulers and
+ * benchmarks).  last_used and last_requested come from the last update that
+ * issued a request list: the pool slots the launches since the update before
+ * it used, and the bricks it asked for.  When last_used + last_requested
+ * exceeds slots, loading that request evicts bricks the in-flight rays used
+ * in their last launch. */
+typedef struct gc_cache_counters {
+    uint64_t updates;          /* gc_cache_update calls                       */
+    uint64_t loads;            /* updates that loaded a request               */
+    uint64_t bricks_loaded;    /* non-empty bricks written into the pool      */
+    uint64_t requests;         /* request lists handed to the handler         */
+    uint32_t slots;            /* pool slots                                  */
+    uint32_t last_used;
+    uint32_t last_requested;
+    uint32_t reserved;
+    double   fetch_ms;         /* host time the request handler spent fetching
+                                  and queueing the uploads of its requests     */
+} gc_cache_counters;
+int gc_cache_counters_get(gc_cache *c, gc_cache_counters *out);
+
 /* Device view handed to kernels: K_CacheManager<T> (K_CacheManager.hpp:34-133)
  * as plain pointers.  Valid until the next update() or destroy(). */
 typedef struct gc_cache_view {
@@ -391,7 +411,13 @@ int gc_cache_busy(gc_cache *c, int *busy);
  * iteration (continuous batching: views join as slots free up, finished
  * views leave) and updates each cache as the reference's loop does.  No
  * reference counterpart.  plain = 1 serves one request at a time, one view
- * per launch (the plain renderer under the same arrivals). */
+ * per launch (the plain renderer under the same arrivals).
+ * Within a volume, views are also admitted by brick-pool memory: when a
+ * request list would evict bricks the in-flight rays used in their last
+ * launch (gc_cache_counters: last_used + last_requested > slots), the
+ * volume's limit of concurrent views drops to half the views in flight; it
+ * grows back by one per view that completes without such pressure.
+ * fixed_views = 1 turns this off (max_views concurrent views always). */
 typedef struct gc_server gc_server;
 typedef struct gc_server_cfg {
     uint64_t pool_budget;        /* device bytes of resident pools + view slots */
@@ -401,7 +427,7 @@ typedef struct gc_server_cfg {
     uint32_t tile, march;        /* launch shape (gc_batch_set_launch)          */
     uint32_t plain;              /* 1: one request at a time                    */
     uint32_t graph;              /* 1: hipGraph replay of the launches          */
-    uint32_t reserved;
+    uint32_t fixed_views;        /* 1: no pool-pressure limit on views          */
 } gc_server_cfg;
 typedef struct gc_server_result {
     uint64_t id;
@@ -411,6 +437,14 @@ typedef struct gc_server_result {
 } gc_server_result;
 typedef struct gc_server_stats {
     uint64_t submitted, completed, launches, samples, pools_created, pools_evicted;
+    uint64_t bricks_loaded;      /* non-empty bricks loaded, all volumes        */
+    uint64_t pressure_events;    /* request lists that met pool pressure        */
+    uint64_t iterations;         /* scheduler iterations                        */
+    uint64_t idle_iterations;    /* iterations that launched nothing            */
+    double   launch_ms;          /* scheduler time in loads + launches          */
+    double   update_ms;          /* scheduler time in post-launch updates and
+                                    retiring views (waits for the launches)    */
+    double   fetch_ms;           /* request handlers' fetch time, all volumes  */
 } gc_server_stats;
 int gc_server_create(const gc_server_cfg *cfg, int device, gc_server **out);
 int gc_server_destroy(gc_server *s);
