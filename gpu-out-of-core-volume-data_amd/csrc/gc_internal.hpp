@@ -100,7 +100,12 @@ hipError_t launch_miss_compact(const uint32_t *req, uint64_t cells, uint32_t ts,
                                const gc_geometry *d_geom, uint32_t cap, uint32_t *out4,
                                const MissScratch &s, hipStream_t stream);
 
-hipError_t launch_deref(uint4 *mrpd, const uint32_t *order, uint32_t nslots, uint32_t nne,
+// The cell table is a compact copy of the page directory that the ray caster
+// reads: one word per MRPD cell, slot << 2 | w (w = 0 unmapped, 1 mapped,
+// 2 empty, as the entry's w).  Every kernel that writes the MRPD writes it.
+__host__ __device__ inline uint32_t cell_word(uint32_t slot, uint32_t w) { return slot << 2 | w; }
+
+hipError_t launch_deref(uint4 *mrpd, uint32_t *cells, const uint32_t *order, uint32_t nslots, uint32_t nne,
                         const SlotMeta *meta, const gc_geometry *d_geom, hipStream_t stream);
 
 // Victims take the new level / position and receive the brick voxels from
@@ -121,7 +126,7 @@ hipError_t launch_assign_cut(const uint32_t *order, uint32_t nslots, uint32_t n,
 // MRPD insert of the asked bricks (reference_new_bricks, MRPD only): the first
 // nne map to their victim slot (w = 1), the rest are Empty (w = 2).  serial=1
 // replays the reference's in-order single-thread insert.
-hipError_t launch_insert(uint4 *mrpd, const uint32_t *order, uint32_t nslots, uint32_t n,
+hipError_t launch_insert(uint4 *mrpd, uint32_t *cells, const uint32_t *order, uint32_t nslots, uint32_t n,
                          uint32_t nne, const uint32_t *asked4, const gc_geometry *d_geom,
                          int serial, hipStream_t stream);
 
@@ -167,6 +172,7 @@ struct DvrView {
 
 struct DvrLaunch {
     const uint4 *mrpd;
+    const uint32_t *cells;      // cell table (cell_word per MRPD cell)
     uint32_t *request;
     uint32_t *usage;
     const void *pool;

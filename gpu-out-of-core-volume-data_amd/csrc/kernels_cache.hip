@@ -190,8 +190,8 @@ __global__ __launch_bounds__(BLK) void k_miss_scatter(uint32_t n, const uint64_t
 }
 
 // ------------------------------------------------------------------ loading
-__global__ void k_deref(uint4 *__restrict__ mrpd, const uint32_t *__restrict__ order, uint32_t N,
-                        uint32_t nne, const SlotMeta *__restrict__ meta,
+__global__ void k_deref(uint4 *__restrict__ mrpd, uint32_t *__restrict__ cells, const uint32_t *__restrict__ order,
+                        uint32_t N, uint32_t nne, const SlotMeta *__restrict__ meta,
                         const gc_geometry *__restrict__ g)
 {
     // dereference_old_bricks (CacheManagerKernels.hpp:115-145), bound j < nne.
@@ -199,7 +199,9 @@ __global__ void k_deref(uint4 *__restrict__ mrpd, const uint32_t *__restrict__ o
     if (j >= nne) return;
     const SlotMeta m = meta[order[N - nne + j]];
     if (m.vol[0] < 0.f) return;   // never-filled slot
-    mrpd[mrpd_index(*g, m.level, m.vol)] = make_uint4(0, 0, 0, 0);
+    const uint64_t idx = mrpd_index(*g, m.level, m.vol);
+    mrpd[idx] = make_uint4(0, 0, 0, 0);
+    cells[idx] = 0u;
 }
 
 template <int BYTES>
@@ -285,7 +287,7 @@ __global__ __launch_bounds__(BLK) void k_assign_cut(const uint32_t *__restrict__
     }
 }
 
-__device__ inline void insert_one(uint4 *mrpd, const uint32_t *order, uint32_t N, uint32_t nne,
+__device__ inline void insert_one(uint4 *mrpd, uint32_t *cells, const uint32_t *order, uint32_t N, uint32_t nne,
                                   const uint32_t *a, uint32_t k, const gc_geometry &g)
 {
     // reference_new_bricks / upward for the MRPD-only hierarchy
@@ -295,30 +297,35 @@ __device__ inline void insert_one(uint4 *mrpd, const uint32_t *order, uint32_t N
     const uint64_t idx = mrpd_index(g, a[3], vol);
     if (mrpd[idx].w >= 1) return;
     uint4 e;
+    uint32_t c;
     if (k < nne) {
+        const uint32_t slot = order[N - nne + k];
         uint32_t o[3];
-        slot_origin(g, order[N - nne + k], o);
+        slot_origin(g, slot, o);
         e = make_uint4(o[0], o[1], o[2], 1);
+        c = cell_word(slot, 1);
     } else {
         e = make_uint4(0, 0, 0, 2);   // zero-initialised draft position (CacheManager.hpp:546)
+        c = cell_word(0, 2);
     }
     mrpd[idx] = e;
+    cells[idx] = c;
 }
 
-__global__ void k_insert(uint4 *__restrict__ mrpd, const uint32_t *__restrict__ order, uint32_t N,
-                         uint32_t n, uint32_t nne, const uint32_t *__restrict__ asked4,
+__global__ void k_insert(uint4 *__restrict__ mrpd, uint32_t *__restrict__ cells, const uint32_t *__restrict__ order,
+                         uint32_t N, uint32_t n, uint32_t nne, const uint32_t *__restrict__ asked4,
                          const gc_geometry *__restrict__ g)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    insert_one(mrpd, order, N, nne, asked4 + 4 * k, k, *g);
+    insert_one(mrpd, cells, order, N, nne, asked4 + 4 * k, k, *g);
 }
 
-__global__ void k_insert_serial(uint4 *mrpd, const uint32_t *order, uint32_t N, uint32_t n,
+__global__ void k_insert_serial(uint4 *mrpd, uint32_t *cells, const uint32_t *order, uint32_t N, uint32_t n,
                                 uint32_t nne, const uint32_t *asked4, const gc_geometry *g)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (uint32_t k = 0; k < n; ++k) insert_one(mrpd, order, N, nne, asked4 + 4 * k, k, *g);
+    for (uint32_t k = 0; k < n; ++k) insert_one(mrpd, cells, order, N, nne, asked4 + 4 * k, k, *g);
 }
 
 __global__ __launch_bounds__(BLK) void k_occupancy(const SlotMeta *__restrict__ meta, uint32_t n,
@@ -452,11 +459,11 @@ hipError_t launch_miss_compact(const uint32_t *req, uint64_t cells, uint32_t ts,
     return hipGetLastError();
 }
 
-hipError_t launch_deref(uint4 *mrpd, const uint32_t *order, uint32_t nslots, uint32_t nne,
+hipError_t launch_deref(uint4 *mrpd, uint32_t *cells, const uint32_t *order, uint32_t nslots, uint32_t nne,
                         const SlotMeta *meta, const gc_geometry *d_geom, hipStream_t stream)
 {
     if (nne == 0) return hipSuccess;
-    k_deref<<<blocks_for(nne, BLK), BLK, 0, stream>>>(mrpd, order, nslots, nne, meta, d_geom);
+    k_deref<<<blocks_for(nne, BLK), BLK, 0, stream>>>(mrpd, cells, order, nslots, nne, meta, d_geom);
     return hipGetLastError();
 }
 
@@ -495,13 +502,13 @@ hipError_t launch_assign_cut(const uint32_t *order, uint32_t nslots, uint32_t n,
     return hipGetLastError();
 }
 
-hipError_t launch_insert(uint4 *mrpd, const uint32_t *order, uint32_t nslots, uint32_t n,
+hipError_t launch_insert(uint4 *mrpd, uint32_t *cells, const uint32_t *order, uint32_t nslots, uint32_t n,
                          uint32_t nne, const uint32_t *asked4, const gc_geometry *d_geom,
                          int serial, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
-    if (serial) k_insert_serial<<<1, 64, 0, stream>>>(mrpd, order, nslots, n, nne, asked4, d_geom);
-    else k_insert<<<blocks_for(n, BLK), BLK, 0, stream>>>(mrpd, order, nslots, n, nne, asked4, d_geom);
+    if (serial) k_insert_serial<<<1, 64, 0, stream>>>(mrpd, cells, order, nslots, n, nne, asked4, d_geom);
+    else k_insert<<<blocks_for(n, BLK), BLK, 0, stream>>>(mrpd, cells, order, nslots, n, nne, asked4, d_geom);
     return hipGetLastError();
 }
 

@@ -67,6 +67,10 @@ constexpr int WAVES = 4;
 constexpr int LANES = 64 * WAVES;
 constexpr float FINF = __builtin_huge_valf();
 constexpr uint32_t NONE = 0xffffffffu;
+// Waves per SIMD the FP32 tile march is compiled for (its register budget).
+#ifndef DVR_WAVES_PER_EU
+#define DVR_WAVES_PER_EU 4
+#endif
 
 struct LevelLds {
     float ior[3], rdv[3], ldf[3], vvf[3], inv[3];
@@ -402,17 +406,12 @@ __device__ inline void locate_cell(F3 pos, F3 ior, F3 ldf, F3 vvf, uint32_t lcoo
 
 // Pool slot of a page entry (entry xyz = the slot's voxel origin in the
 // reference's 3-D pool; coordinates are multiples of bs below 2^24).
-__device__ inline uint32_t entry_slot(const uint4 &e, uint32_t bs_shift, float inv_bs, uint32_t cx, uint32_t cy)
-{
-    if (bs_shift != NONE) return (e.x >> bs_shift) + __umul24(cx, (e.y >> bs_shift) + __umul24(cy, e.z >> bs_shift));
-    const uint32_t qx = (uint32_t)__builtin_rintf((float)e.x * inv_bs);
-    const uint32_t qy = (uint32_t)__builtin_rintf((float)e.y * inv_bs);
-    const uint32_t qz = (uint32_t)__builtin_rintf((float)e.z * inv_bs);
-    return qx + __umul24(cx, qy + __umul24(cy, qz));
-}
+// Slot and state of a cell-table word (cell_word: slot << 2 | w).
+__device__ inline uint32_t word_slot(uint32_t c) { return c >> 2; }
+__device__ inline uint32_t word_state(uint32_t c) { return c & 3u; }
 
 template <int BYTES, typename ACC, int TW, int TH>
-__global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : 4))) void k_dvr(const DvrLaunch p)
+__global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : DVR_WAVES_PER_EU))) void k_dvr(const DvrLaunch p)
 {
     static_assert(TW * TH == 64, "one wave per ray packet");
     __shared__ LevelLds lv[GC_MAX_LEVELS];
@@ -462,9 +461,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t xcd = blockIdx.x & 7u;
     const uint32_t bs = g->brick_size, cov = g->covering, inner = g->inner;
-    const uint32_t cx_ = g->cache_size[0], cy_ = g->cache_size[1];
     const uint32_t MX = g->mrpd_dim[0], MY = g->mrpd_dim[1];
-    const uint32_t bs_shift = (bs & (bs - 1)) == 0 ? (uint32_t)__builtin_ctz(bs) : NONE;
     const uint64_t bbytes = (uint64_t)bs * bs * bs * BYTES;
     const uint32_t row = bs * BYTES, plane = bs * bs * BYTES;
     const uint8_t *pool = static_cast<const uint8_t *>(p.pool);
@@ -522,14 +519,13 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // a-time trace.  The page cell doubles as the request cell
     // (realDivVirtual == 1).
     const bool piped = fast_fetch && p.pipelined != 0;
-    const float inv_bs = 1.0f / (float)bs;
     // hb: stage B holds a sample (the front placed last half-step); hc: stage
     // C holds one; newray: the next sample placed is the ray's first
     bool hb = false, hc = false, newray = false;
     uint32_t cellB = 0, viB = 0, cellC = 0, slotC = 0, ewC = 0, viC = 0, sum_prev = 0;
     float kB = 0.f, tC = 0.f, kC = 0.f;
     bool freshB = false, freshC = false, reuseC = false;
-    uint4 eX = make_uint4(0u, 0u, 0u, 0u), eY = eX;
+    uint32_t eX = 0u, eY = 0u;   // cell-table words in flight
     Quad wX{0u, 0u, 0u, 0u}, wY = wX;
 
     // page cell and in-brick voxel of the front sample (get_normalized,
@@ -537,7 +533,6 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     auto locate = [&](uint32_t &cell, uint32_t &vi) {
         locate_cell(pos, ior, ldf, vvf, lcoord, MX, MY, inner, bs, cell, vi);
     };
-    auto slot_of = [&](const uint4 &e) -> uint32_t { return entry_slot(e, bs_shift, inv_bs, cx_, cy_); };
     // the progressive state and the pixel of a ray that ends or stops
     // (RayCasters.cu:138-147,189-197)
     auto store_state = [&](float tt) {
@@ -704,7 +699,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 // into w_new; consume the sample whose footprint is w_old.
                 // Halves alternate the X/Y registers, so nothing in flight is
                 // ever copied.
-                auto half = [&](uint4 &e_new, const uint4 &e_old, Quad &w_new, const Quad &w_old) {
+                auto half = [&](uint32_t &e_new, const uint32_t &e_old, Quad &w_new, const Quad &w_old) {
                     // The common path is branch-free (selects): most lanes
                     // place, fetch and consume a sample every half-step.
                     // A: place the front sample (the reference's advance by
@@ -728,14 +723,14 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     // loads the wave's first cell instead (one shared line)
                     // and B keeps the entry it already has.
                     const uint32_t cell_u = (uint32_t)__builtin_amdgcn_readfirstlane(cellA);
-                    e_new = p.mrpd[freshA ? cellA : cell_u];
+                    e_new = p.cells[freshA ? cellA : cell_u];
                     // B: footprint of the sample placed last time.  Every
                     // lane issues the loads (so the waits stay one half-step
                     // behind); lanes that need none (not mapped, or the same
                     // voxel as sample j, whose box sum is kept) load the
                     // wave's first footprint instead.
-                    const uint32_t slot_e = slot_of(e_old);
-                    const uint32_t ewB = freshB ? e_old.w : ewC;
+                    const uint32_t slot_e = word_slot(e_old);
+                    const uint32_t ewB = freshB ? word_state(e_old) : ewC;
                     const uint32_t slotB = freshB ? slot_e : slotC;
                     const bool reuseB = !freshB && viB == viC;
                     const uint64_t fr = (uint64_t)slotB * (uint32_t)bbytes + corner + viB * (uint32_t)BYTES;
@@ -835,13 +830,10 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 if (mcx != pcx || by != pcy || bz != pcz) {
                     pcx = mcx; pcy = by; pcz = bz;
                     // the page directory holds fewer than 2^32 cells (checked at creation)
-                    const uint4 e = p.mrpd[mcx + MX * (by + MY * bz)];
-                    ew = e.w;
+                    const uint32_t e = p.cells[mcx + MX * (by + MY * bz)];
+                    ew = word_state(e);
                     ++n_visits;
-                    // pool shapes hold fewer than 2^24 slots (checked at creation)
-                    slot = bs_shift != NONE
-                               ? (e.x >> bs_shift) + __umul24(cx_, (e.y >> bs_shift) + __umul24(cy_, e.z >> bs_shift))
-                               : e.x / bs + cx_ * (e.y / bs + cy_ * (e.z / bs));
+                    slot = word_slot(e);
                     bbase = pool + (uint64_t)slot * bbytes + corner;
                     vkey = NONE;
                     if (ew == 1) p.usage[slot] = ts;
@@ -1015,10 +1007,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
 
     const uint32_t list_count = __builtin_amdgcn_readfirstlane(*p.list_count);
     const uint32_t bs = g->brick_size, cov = g->covering, inner = g->inner;
-    const uint32_t cx_ = g->cache_size[0], cy_ = g->cache_size[1];
     const uint32_t MX = g->mrpd_dim[0], MY = g->mrpd_dim[1];
-    const uint32_t bs_shift = (bs & (bs - 1)) == 0 ? (uint32_t)__builtin_ctz(bs) : NONE;
-    const float inv_bs = 1.0f / (float)bs;
     const uint32_t bbytes = bs * bs * bs * BYTES;
     const uint32_t row = bs * BYTES, plane = bs * bs * BYTES;
     const uint8_t *pool = static_cast<const uint8_t *>(p.pool);
@@ -1133,9 +1122,9 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         uint32_t cell, vi;
         locate_cell(my_pos, f3(L.ior[0], L.ior[1], L.ior[2]), f3(L.ldf[0], L.ldf[1], L.ldf[2]),
                     f3(L.vvf[0], L.vvf[1], L.vvf[2]), L.lcoord, MX, MY, inner, bs, cell, vi);
-        const uint4 en = p.mrpd[cell];
-        const uint32_t ew = en.w;
-        const uint32_t slot = entry_slot(en, bs_shift, inv_bs, cx_, cy_);
+        const uint32_t en = p.cells[cell];
+        const uint32_t ew = word_state(en);
+        const uint32_t slot = word_slot(en);
         const uint32_t prev = __shfl_up(cell, 1u, G);
         const bool fresh = cell != (gl == 0 ? cell_last : prev);
         const uint64_t off = (active && ew == 1) ? (uint64_t)slot * bbytes + corner + vi * (uint32_t)BYTES : 0ull;

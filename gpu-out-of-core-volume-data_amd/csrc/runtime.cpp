@@ -512,6 +512,7 @@ struct Cache {
     uint32_t limit = 50;                // bricks per update, <= cap
     // device state
     uint4 *mrpd = nullptr;
+    uint32_t *cells = nullptr;          // cell table: slot << 2 | w per MRPD cell (the ray caster's copy)
     uint32_t *request = nullptr, *usage = nullptr, *order = nullptr, *order_tmp = nullptr;
     SlotMeta *meta = nullptr;
     uint8_t *pool = nullptr;
@@ -550,7 +551,7 @@ void Cache::release()
     handler.stop();
     if (stream) hipStreamSynchronize(stream);
     if (upload) hipStreamSynchronize(upload);
-    hipFree(mrpd); hipFree(request); hipFree(usage); hipFree(order); hipFree(order_tmp);
+    hipFree(mrpd); hipFree(cells); hipFree(request); hipFree(usage); hipFree(order); hipFree(order_tmp);
     hipFree(meta); hipFree(pool); hipFree(d_geom);
     hipFree(lru_s.masks); hipFree(lru_s.counts); hipFree(lru_s.total);
     hipFree(miss_s.masks); hipFree(miss_s.counts); hipFree(miss_s.total);
@@ -567,7 +568,7 @@ void Cache::release()
     ev0 = ev1 = nullptr;
     if (stream) hipStreamDestroy(stream);
     if (upload) hipStreamDestroy(upload);
-    mrpd = nullptr; request = usage = order = order_tmp = nullptr; meta = nullptr; pool = nullptr;
+    mrpd = nullptr; cells = nullptr; request = usage = order = order_tmp = nullptr; meta = nullptr; pool = nullptr;
     d_geom = nullptr; h_misses = nullptr; h_staging = nullptr; h_meta = nullptr;
     uploaded = consumed = nullptr; stream = upload = nullptr;
 }
@@ -605,6 +606,7 @@ int Cache::create()
     GC_HIP(hipEventCreateWithFlags(&consumed, hipEventDisableTiming));
     const uint64_t bvox = (uint64_t)g.brick_size * g.brick_size * g.brick_size;
     GC_TRY(dalloc(&mrpd, g.mrpd_cells * sizeof(uint4)));
+    GC_TRY(dalloc(&cells, g.mrpd_cells * sizeof(uint32_t)));
     GC_TRY(dalloc(&request, g.mrpd_cells * sizeof(uint32_t)));
     GC_TRY(dalloc(&usage, (size_t)g.nslots * sizeof(uint32_t)));
     GC_TRY(dalloc(&order, (size_t)g.nslots * sizeof(uint32_t)));
@@ -617,6 +619,7 @@ int Cache::create()
     // Surfaces and thrust vectors of the reference start zeroed
     // (Surface3dDevice.hpp:97-109, DynamicArray3dDevice).
     GC_HIP(hipMemsetAsync(mrpd, 0, g.mrpd_cells * sizeof(uint4), stream));
+    GC_HIP(hipMemsetAsync(cells, 0, g.mrpd_cells * sizeof(uint32_t), stream));
     GC_HIP(hipMemsetAsync(request, 0, g.mrpd_cells * sizeof(uint32_t), stream));
     GC_HIP(hipMemsetAsync(usage, 0, (size_t)g.nslots * sizeof(uint32_t), stream));
     GC_HIP(hipMemsetAsync(pool, 0, (size_t)g.nslots * bvox * vc.encoded_bytes + 64, stream));
@@ -716,12 +719,12 @@ int Cache::load_new_bricks()
     const uint32_t *d_asked = d_meta;
     const uint32_t *d_bbi = d_meta + 4 * (size_t)cap;
     const uint32_t *d_bbc = d_meta + 5 * (size_t)cap;
-    GC_HIP(launch_deref(mrpd, order, g.nslots, nne, meta, d_geom, stream));
+    GC_HIP(launch_deref(mrpd, cells, order, g.nslots, nne, meta, d_geom, stream));
     GC_HIP(launch_assign_copy(order, g.nslots, nne, d_asked, d_bbi, d_bbc, meta, pool, d_staging, d_geom,
                               vc.big_brick_size, vc.encoded_bytes, stream));
     int serial = 0;
     GC_TRY(insert_serial_needed(h_meta, n, serial));
-    GC_HIP(launch_insert(mrpd, order, g.nslots, n, nne, d_asked, d_geom, serial, stream));
+    GC_HIP(launch_insert(mrpd, cells, order, g.nslots, n, nne, d_asked, d_geom, serial, stream));
     GC_HIP(hipEventRecord(consumed, stream));
     return GC_OK;
 }
@@ -932,6 +935,7 @@ struct Frame {
 static void launch_base(const Cache &C, DvrLaunch &L)
 {
     L.mrpd = C.mrpd;
+    L.cells = C.cells;
     L.request = C.request;
     L.usage = C.usage;
     L.pool = C.pool;
@@ -1563,12 +1567,12 @@ int gc_cache_load_all(gc_cache *c, const void *const *d_levels)
     GC_HIP(hipMemcpyAsync(d_asked, asked.data(), asked.size() * 4, hipMemcpyHostToDevice, C.stream));
     GC_HIP(hipMemcpyAsync(d_dims, C.vc.initial_sizes, (size_t)g.nb_levels * 12, hipMemcpyHostToDevice, C.stream));
     GC_HIP(hipMemcpyAsync(d_lv, d_levels, (size_t)g.nb_levels * sizeof(void *), hipMemcpyHostToDevice, C.stream));
-    GC_HIP(launch_deref(C.mrpd, C.order, g.nslots, n, C.meta, C.d_geom, C.stream));
+    GC_HIP(launch_deref(C.mrpd, C.cells, C.order, g.nslots, n, C.meta, C.d_geom, C.stream));
     GC_HIP(launch_assign_cut(C.order, g.nslots, n, d_asked, C.meta, C.pool, d_lv, d_dims, C.d_geom,
                              C.vc.encoded_bytes, C.stream));
     int serial = 0;
     GC_TRY(C.insert_serial_needed(asked.data(), n, serial));
-    GC_HIP(launch_insert(C.mrpd, C.order, g.nslots, n, n, d_asked, C.d_geom, serial, C.stream));
+    GC_HIP(launch_insert(C.mrpd, C.cells, C.order, g.nslots, n, n, d_asked, C.d_geom, serial, C.stream));
     GC_HIP(hipStreamSynchronize(C.stream));
     hipFree(d_asked);
     hipFree(d_dims);
@@ -1648,6 +1652,12 @@ int gc_cache_read_mrpd(gc_cache *c, uint32_t *host)
 {
     if (!c || !host) return fail(GC_ERR_ARG, "null argument");
     return d2h_sync(c->impl, host, c->impl.mrpd, c->impl.g.mrpd_cells * 16);
+}
+
+int gc_cache_read_cells(gc_cache *c, uint32_t *host)
+{
+    if (!c || !host) return fail(GC_ERR_ARG, "null argument");
+    return d2h_sync(c->impl, host, c->impl.cells, c->impl.g.mrpd_cells * 4);
 }
 int gc_cache_read_request(gc_cache *c, uint32_t *host)
 {

@@ -131,7 +131,7 @@ EXPORTS = [
     "gc_source_create_callback", "gc_source_destroy", "gc_source_get", "gc_synth_volume",
     "gc_cache_create", "gc_cache_destroy", "gc_cache_update", "gc_cache_completude", "gc_cache_load_all",
     "gc_cache_timestamp", "gc_cache_stream", "gc_cache_view_get", "gc_cache_geometry", "gc_geometry_compute",
-    "gc_cache_read_mrpd",
+    "gc_cache_read_mrpd", "gc_cache_read_cells",
     "gc_cache_read_request", "gc_cache_write_request", "gc_cache_read_usage", "gc_cache_write_usage",
     "gc_cache_read_lru", "gc_cache_read_pool", "gc_cache_read_asked", "gc_frame_create", "gc_frame_destroy",
     "gc_frame_reset", "gc_frame_read_pixels", "gc_frame_read_state", "gc_frame_buffers", "gc_render_dvr",
@@ -183,7 +183,7 @@ def lib():
             "gc_cache_counters_get": [P, C.POINTER(CacheCounters)],
             "gc_cache_view_get": [P, C.POINTER(CacheView)], "gc_cache_geometry": [P, C.POINTER(Geometry)],
             "gc_geometry_compute": [C.POINTER(VolumeCfg), C.POINTER(CacheCfg), C.POINTER(Geometry)],
-            "gc_cache_read_mrpd": [P, P], "gc_cache_read_request": [P, P], "gc_cache_write_request": [P, P],
+            "gc_cache_read_mrpd": [P, P], "gc_cache_read_cells": [P, P], "gc_cache_read_request": [P, P], "gc_cache_write_request": [P, P],
             "gc_cache_read_usage": [P, P], "gc_cache_write_usage": [P, P], "gc_cache_read_lru": [P, P],
             "gc_cache_read_pool": [P, P], "gc_cache_read_asked": [P, P, C.POINTER(u32), C.POINTER(u32)],
             "gc_frame_create": [u32, u32, u32, C.POINTER(vp)], "gc_frame_destroy": [P],
@@ -453,6 +453,11 @@ class Cache:
     def mrpd(self):
         out = np.zeros((self.cells, 4), np.uint32)
         check(lib().gc_cache_read_mrpd(self.h, ptr(out)))
+        return out
+
+    def cell_table(self):
+        out = np.zeros(self.cells, np.uint32)
+        check(lib().gc_cache_read_cells(self.h, ptr(out)))
         return out
 
     def request(self):

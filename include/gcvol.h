@@ -253,6 +253,9 @@ typedef struct gc_lru_entry {
     float    vol_pos[3];
 } gc_lru_entry;
 int gc_cache_read_mrpd(gc_cache *c, uint32_t *host);     /* mrpd_cells x 4  */
+/* The ray caster's compact copy of the page directory: one word per cell,
+ * slot << 2 | w (no reference counterpart; kept equal to the MRPD). */
+int gc_cache_read_cells(gc_cache *c, uint32_t *host);    /* mrpd_cells      */
 int gc_cache_read_request(gc_cache *c, uint32_t *host);  /* mrpd_cells      */
 int gc_cache_write_request(gc_cache *c, const uint32_t *host);
 int gc_cache_read_usage(gc_cache *c, uint32_t *host);    /* nslots          */

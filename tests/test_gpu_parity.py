@@ -64,6 +64,7 @@ def _compare_state(gc, oc, pool=True):
     assert np.array_equal(gc.lru(), oc.lru()), "LRU differs"
     assert np.array_equal(gc.usage(), oc.usage()), "usage differs"
     assert np.array_equal(gc.request(), oc.request()), "request buffer differs"
+    _check_cell_table(gc)
     if pool:
         assert np.array_equal(gc.pool(), oc.pool()), "pool contents differ"
 
@@ -147,6 +148,17 @@ def test_out_of_core_sequence(gpu, bytes_, tile):
             if gd:
                 break
     assert steps > 3
+
+
+def _check_cell_table(gc):
+    """The ray caster's cell table equals slot << 2 | w derived from the MRPD
+    (slot from the entry's pool voxel origin)."""
+    m = gc.mrpd().astype(np.uint64)
+    bs = int(gc.g.brick_size)
+    cx, cy = int(gc.g.cache_size[0]), int(gc.g.cache_size[1])
+    slots = m[:, 0] // bs + cx * (m[:, 1] // bs + cy * (m[:, 2] // bs))
+    want = np.where(m[:, 3] == 1, slots * 4 + 1, m[:, 3]).astype(np.uint32)
+    assert np.array_equal(gc.cell_table(), want), "cell table differs from the page directory"
 
 
 def _max_mapped_offset(gc, brick_bytes):
