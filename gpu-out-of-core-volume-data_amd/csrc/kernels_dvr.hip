@@ -23,7 +23,9 @@
 //    reloaded from LDS only when the level changes;
 //  * the distances at which a ray changes level are computed once, at ray
 //    set-up, into a per-lane LDS table (no search inside the march);
-//  * the page entry is read with one 16-byte load per brick visit, which also
+//  * the page entry is read with one 4-byte load per brick visit from the
+//    cell table (slot << 2 | state per page-directory cell, kept equal to
+//    the 16-byte MRPD entries by the kernels that write them), which also
 //    yields the brick's base address in the brick-major pool;
 //  * the 2x2x2 footprint of a sample is four texel pairs (x-1, x) in one
 //    brick: four 4-byte (u16) or 2-byte (u8) loads at 2- or 1-byte alignment,
