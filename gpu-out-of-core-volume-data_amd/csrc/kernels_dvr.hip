@@ -122,12 +122,20 @@ __device__ float first_above(float ep, float s, float tfar, float lo, float hi)
     uint32_t lb = __float_as_uint(lo) & 0x7fffffffu, hb = __float_as_uint(hi);
     const float est = (s / ep) * tfar;
     if (est > lo && est < hi) {
+        // brackets of +-4 ulp, then +-64 ulp, around the closed-form estimate;
+        // each is used only when the predicate confirms it, so the result is
+        // the bisection's over the full range
         const uint32_t eb = __float_as_uint(est);
-        const uint32_t w = 64;
-        const uint32_t a = eb - lb > w ? eb - w : lb, b = hb - eb > w ? eb + w : hb;
-        const bool a_ok = a == lb || pixel_dist(ep, __uint_as_float(a), tfar) <= s;
-        const bool b_ok = b == hb || pixel_dist(ep, __uint_as_float(b), tfar) > s;
-        if (a_ok && b_ok) { lb = a; hb = b; }
+        for (uint32_t w = 4; w <= 64; w *= 16) {
+            const uint32_t a = eb - lb > w ? eb - w : lb, b = hb - eb > w ? eb + w : hb;
+            const bool a_ok = a == lb || pixel_dist(ep, __uint_as_float(a), tfar) <= s;
+            const bool b_ok = b == hb || pixel_dist(ep, __uint_as_float(b), tfar) > s;
+            if (a_ok && b_ok) {
+                lb = a;
+                hb = b;
+                break;
+            }
+        }
     }
     while (hb - lb > 1) {
         const uint32_t mb = lb + ((hb - lb) >> 1);
