@@ -146,7 +146,7 @@ SERVE_BUDGET = 96 << 30
 
 
 def serving_workload(G, rates=(4, 8, 16, 32), seconds=4.0, max_views=16, warm_requests=24, fixed_ab=False,
-                     volumes=None):
+                     volumes=None, max_requests=4096):
     """Requests arrive as a Poisson process (seeded) over 6 volumes (512^3 u8,
     1024^3 u8, 1024^3 u16, 2048^3 u8, 2048x2048x1024 u16, 4096^3 u8, each
     synthesised brick by brick on the host on demand, 32^3 bricks, 12 GiB
@@ -185,7 +185,7 @@ def serving_workload(G, rates=(4, 8, 16, 32), seconds=4.0, max_views=16, warm_re
     def serve(reqs, plain, fixed=False):
         srv = G.Server(SERVE_BUDGET, max_views=1 if plain else max_views, slot_pixels=2048 * 2048, precision=32,
                        plain=plain, fixed_views=fixed)
-        vids = [srv.add_volume(vc, src, SERVE_POOL, max_requests=4096) for vc, src in zip(vcs, sources)]
+        vids = [srv.add_volume(vc, src, SERVE_POOL, max_requests=max_requests) for vc, src in zip(vcs, sources)]
         # warm: an untimed burst from the same request distribution
         for (_, vol, side, cam, tfi) in warm:
             srv.submit(vids[vol], G.make_view(side, side, cam, alpha_stop=ALPHA_STOP), tables[tfi])
@@ -230,7 +230,8 @@ def serving_workload(G, rates=(4, 8, 16, 32), seconds=4.0, max_views=16, warm_re
                 "scheduler": {"iterations": st1.iterations - st0.iterations,
                               "idle_iterations": st1.idle_iterations - st0.idle_iterations,
                               "launch_ms": st1.launch_ms - st0.launch_ms, "update_ms": st1.update_ms - st0.update_ms,
-                              "fetch_ms": st1.fetch_ms - st0.fetch_ms}}
+                              "fetch_ms": st1.fetch_ms - st0.fetch_ms},
+                "ray_starts": st1.ray_starts - st0.ray_starts}
 
     warm = []
     while len(warm) < warm_requests:
@@ -431,9 +432,12 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--serving-rates", default="4,8,16,32", help="request rates (req/s) of the serving workload")
     ap.add_argument("--serving-volumes", default="", help="serve only these volumes (indices into the six)")
+    ap.add_argument("--serving-max-views", type=int, default=16, help="views per volume per launch when batched")
+    ap.add_argument("--serving-max-requests", type=int, default=4096, help="bricks per update of each served volume")
     ap.add_argument("--serving-ab", action="store_true",
                     help="also serve each arrival sequence with a fixed number of views per volume")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--pool-gib", type=float, default=0.0, help="brick pool size (default: the workload's 4 GiB)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known traffic "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -453,7 +457,9 @@ def main():
     if args.serving_only:
         rates = tuple(int(r) for r in args.serving_rates.split(","))
         vols = [int(v) for v in args.serving_volumes.split(",")] if args.serving_volumes else None
-        print(json.dumps({"serving": serving_workload(G, rates=rates, fixed_ab=args.serving_ab, volumes=vols)}))
+        print(json.dumps({"serving": serving_workload(G, rates=rates, fixed_ab=args.serving_ab, volumes=vols,
+                                                      max_views=args.serving_max_views,
+                                                      max_requests=args.serving_max_requests)}))
         return
 
     if args.calib_copy_mib > 0:
@@ -469,7 +475,7 @@ def main():
     params = G.synth_params(SEED, 64, sigma=(40.0, 200.0), amp=(15000.0, 65535.0), noise=2048, threads=0)
     src = G.Source.synthetic(vc, params)
     brick_bytes = BRICK ** 3 * BYTES
-    cs = G.pool_shape(POOL_BYTES, brick_bytes)
+    cs = G.pool_shape(POOL_BYTES if args.pool_gib <= 0 else int(args.pool_gib * (1 << 30)), brick_bytes)
     cc = G.cache_cfg(cs, BRICK, max_requests=args.max_requests, sync=False)
     cache = G.Cache(vc, cc, src)
     tf = G.tf_samples(G.tf_seeded(SEED, 6), 256)
@@ -490,9 +496,9 @@ def main():
         dt = time.perf_counter() - t0
         d = rep.as_dict()
         d["wall_ms"] = dt * 1e3
-        print("frame %3d %s wall %.1f ms launches %d lane %d bricks %d samples %.3g kernel %.1f ms" % (
+        print("frame %3d %s wall %.1f ms launches %d lane %d bricks %d samples %.3g rays %d kernel %.1f ms" % (
             k, "timed" if k >= args.warmup else "warmup", dt * 1e3, d["launches"], d["lane_launches"],
-            d["bricks_loaded"], d["mapped_samples"], d["kernel_ms"]), file=sys.stderr, flush=True)
+            d["bricks_loaded"], d["mapped_samples"], d["pixels"], d["kernel_ms"]), file=sys.stderr, flush=True)
         if k >= args.warmup:
             lat.append(dt * 1e3)
             reports.append(d)

@@ -157,13 +157,21 @@ hipError_t launch_brick_cut(const void *level, const uint32_t dims[3], uint32_t 
 // (view << 26 | pixel), then the TWxTH pixel tiles of the starting views.
 constexpr uint32_t DVR_MAX_VIEWS = 64;
 constexpr uint32_t DVR_PIXEL_BITS = 26;
+// Transfer functions in LDS: every distinct table of a launch gets a copy
+// padded with one clamped entry on each side, when at most DVR_TF_LDS_TABLES
+// tables of at most DVR_TF_LDS_MAX entries are used (u8/u16 voxels).
+constexpr uint32_t DVR_TF_LDS_TABLES = 8;
+constexpr uint32_t DVR_TF_LDS_MAX = 256;
+constexpr uint32_t DVR_TF_OWNER = 0x80000000u;
 
 struct DvrView {
     uint32_t width, height, tiles_x, packet_off;   // packet_off: first tile packet
     float m[12];                                   // d_invModelViewMatrix rows
     float bmin[3], bmax[3];
-    float alpha_stop;
+    uint32_t tf_lds_off;                           // entry offset of the LDS copy (| DVR_TF_OWNER: this
+                                                   // view copies it in), when the launch has LDS tables
     uint32_t tf_size;
+    float alpha_stop;
     const float4 *tf;
     uint32_t *pixels;
     float *tbuf;
@@ -197,6 +205,8 @@ struct DvrLaunch {
     gc_render_stats *stats;
     uint32_t refill_min;        // refill idle lanes once this many are idle
     uint32_t march;             // samples per lane between refill checks (1..8)
+    uint32_t tf_lds_entries;    // float4 entries of the LDS transfer-function tables (0: read from HBM)
+    uint32_t tf_lds_tables;     // distinct tables among them
     uint32_t pipelined;         // 1: software-pipelined march (requires realDivVirtual == 1 on every
                                 //    level and MRPD y*z cells, x cells below 2^24; see launch_base)
 };
@@ -211,13 +221,18 @@ inline void tile_shape(uint32_t tile, uint32_t &tw, uint32_t &th)
 
 // Dynamic LDS of a DVR launch (per-lane level-switch table and view table)
 // and the persistent grid for it (workgroups of 4 waves).
-uint32_t dvr_lds_bytes(uint32_t nb_lods, uint32_t n_views);
-uint32_t dvr_grid(uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t n_packets, uint32_t lds_bytes);
+uint32_t dvr_lds_bytes(uint32_t nb_lods, uint32_t n_views, uint32_t tf_entries);
+uint32_t dvr_grid(uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t n_packets, uint32_t lds_bytes,
+                  uint32_t tf_tables);
 // Group-per-ray march of a ray list (list launches only; see kernels_dvr.hip):
 // persistent workgroups of 4 waves, each group of 16 or 64 lanes taking one
 // ray at a time (the group size follows `rays`, the list length).
 bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes);
-uint32_t dvr_ray_lds_bytes(uint32_t n_views);
+uint32_t dvr_ray_lds_bytes(uint32_t n_views, uint32_t tf_entries);
+// Assign LDS copies to the views' transfer functions (views[i].tf_lds_off);
+// returns the table entries, or 0 when they do not fit (then tf_lds_off is
+// unused and the kernels read the tables from HBM).
+uint32_t dvr_assign_tf_lds(DvrView *views, uint32_t n, uint32_t bytes, uint32_t *tables);
 // group-per-ray march: group = lanes per ray (16 or 64; 0 = by the list
 // length: 64 up to 256 rays, else 16)
 uint32_t dvr_ray_group(uint32_t rays);

@@ -64,7 +64,6 @@ namespace gc {
 
 namespace {
 
-constexpr int TF_LDS = 256;
 constexpr int WAVES = 4;
 constexpr int LANES = 64 * WAVES;
 constexpr float FINF = __builtin_huge_valf();
@@ -420,17 +419,42 @@ __device__ inline void locate_cell(F3 pos, F3 ior, F3 ldf, F3 vvf, uint32_t lcoo
 __device__ inline uint32_t word_slot(uint32_t c) { return c >> 2; }
 __device__ inline uint32_t word_state(uint32_t c) { return c & 3u; }
 
-template <int BYTES, typename ACC, int TW, int TH>
+// Byte offset of the transfer-function tables in a launch's dynamic LDS:
+// after `head` bytes (level-switch table, if any) and the view table.
+__host__ __device__ inline uint32_t tf_lds_base(uint32_t head, uint32_t n_views)
+{
+    return (head + n_views * (uint32_t)sizeof(DvrView) + 15u) & ~15u;
+}
+
+// Each distinct table of the launch is copied into LDS once (by the view that
+// owns it), padded with one clamped entry on each side: dst[j] = tf[clamp(j-1)].
+__device__ inline void load_tf_tables(const DvrView *vw, uint32_t n_views, float4 *tfl, int tid)
+{
+    for (uint32_t v = 0; v < n_views; ++v) {
+        const uint32_t o = vw[v].tf_lds_off;
+        if (!(o & DVR_TF_OWNER)) continue;
+        const int N = (int)vw[v].tf_size;
+        float4 *dst = tfl + (o & ~DVR_TF_OWNER);
+        for (int i = tid; i < N + 2; i += LANES) dst[i] = vw[v].tf[i < 1 ? 0 : (i > N ? N - 1 : i - 1)];
+    }
+}
+
+// MT: the launch's views use more than one LDS transfer-function table (each
+// ray keeps its table's offset and size); otherwise there is one table, at
+// offset 0, whose size the whole launch shares.
+template <int BYTES, typename ACC, int TW, int TH, bool MT>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : DVR_WAVES_PER_EU))) void k_dvr(const DvrLaunch p)
 {
     static_assert(TW * TH == 64, "one wave per ray packet");
     __shared__ LevelLds lv[GC_MAX_LEVELS];
-    __shared__ float4 tfl[TF_LDS + 2];
-    // dynamic LDS: level-switch distances [nb_lods][LANES], then the views
+    // dynamic LDS: level-switch distances [nb_lods][LANES], the views, then
+    // the launch's transfer functions (padded copies), when they fit
     extern __shared__ __align__(16) uint8_t dyn_lds[];
     const uint32_t nl = p.nb_lods;
+    const uint32_t n_views = p.views ? p.n_views : 1u;
     float *tsw = reinterpret_cast<float *>(dyn_lds);
     DvrView *vw = reinterpret_cast<DvrView *>(dyn_lds + (size_t)nl * LANES * sizeof(float));
+    float4 *tfl = reinterpret_cast<float4 *>(dyn_lds + tf_lds_base(nl * LANES * (uint32_t)sizeof(float), n_views));
     const int tid = threadIdx.x;
     const gc_geometry *g = p.geom;
     if (tid < (int)nl) {
@@ -446,20 +470,16 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         L.step = p.lod_step[tid];
         L.kexp = p.kexp[tid];
     }
-    const uint32_t n_views = p.views ? p.n_views : 1u;
     if (p.views) {
         for (uint32_t i = tid; i < n_views; i += LANES) vw[i] = p.views[i];
     } else if (tid == 0) {
         vw[0] = p.view0;
     }
     __syncthreads();
-    const uint32_t tf_n0 = __builtin_amdgcn_readfirstlane(vw[0].tf_size);
-    const bool tf_lds = n_views == 1 && tf_n0 <= (uint32_t)TF_LDS && BYTES != 4;
-    if (tf_lds) {
-        const int N = (int)vw[0].tf_size;
-        for (int i = tid; i < N + 2; i += LANES) tfl[i] = vw[0].tf[i < 1 ? 0 : (i > N ? N - 1 : i - 1)];
-    }
+    const bool tf_lds = BYTES != 4 && p.tf_lds_entries != 0;
+    if (tf_lds) load_tf_tables(vw, n_views, tfl, tid);
     __syncthreads();
+    const uint32_t tf_n1 = p.tf_lds_entries - 2u;   // the single table's size (!MT)
     // packets: first the ray list (64 entries each), then the views' tiles
     // wave-uniform values read from memory go to scalar registers, waited for
     // here rather than inside the march loop
@@ -490,6 +510,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // per-lane ray state
     bool have = false;
     uint32_t v = 0, pix = 0;
+    uint32_t tfw = 0;   // the ray's LDS table: entry offset | entries << 16
     float t = 0.f, tfar = 0.f, astop = 0.f;
     F3 pos = f3(0, 0, 0), dir = f3(0, 0, 0), dstep = f3(0, 0, 0);
     // current level: index, next switch distance and its parameters
@@ -558,7 +579,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // (RayCasters.cu:161-178); composite_if blends only where `on`.
     auto composite_if = [&](bool on, ACC s, float kx) {
         bool cskip;
-        const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(s, tfl, tf_n0, kx, cskip)
+        const Acc4<ACC> cc = tf_lds ? (MT ? classify_padded<ACC>(s, tfl + (tfw & 0xffffu), tfw >> 16, kx, cskip)
+                                          : classify_padded<ACC>(s, tfl, tf_n1, kx, cskip))
                                     : classify<ACC>(s, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size,
                                                     kx, cskip);
         const bool b = on && !cskip;
@@ -576,7 +598,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     };
     auto composite = [&](ACC s, float kx) {
         bool cskip;
-        const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(s, tfl, tf_n0, kx, cskip)
+        const Acc4<ACC> cc = tf_lds ? (MT ? classify_padded<ACC>(s, tfl + (tfw & 0xffffu), tfw >> 16, kx, cskip)
+                                          : classify_padded<ACC>(s, tfl, tf_n1, kx, cskip))
                                     : classify<ACC>(s, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size,
                                                     kx, cskip);
         if (!cskip) {
@@ -647,6 +670,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     // ray set-up (RayCasters.cu:57-120)
                     const DvrView &V = vw[v];
                     const uint32_t W = V.width;
+                    if constexpr (MT) tfw = (V.tf_lds_off & ~DVR_TF_OWNER) | (V.tf_size << 16);
                     const RaySetup R = setup_ray(V, x, y);
                     const F3 origin = R.origin, dxn = R.dxn, dyn = R.dyn, dleft = R.dleft, dup = R.dup;
                     dir = R.dir;
@@ -980,8 +1004,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     static_assert(G == 16 || G == 32 || G == 64, "16, 32 or 64 lanes per ray");
     constexpr uint32_t R = 64 / G;   // rays per wave
     __shared__ LevelLds lv[GC_MAX_LEVELS];
-    __shared__ float4 tfl[TF_LDS + 2];
     __shared__ float tswv[WAVES * R][GC_MAX_LEVELS + 1];
+    // dynamic LDS: the views, then the launch's transfer functions
     extern __shared__ __align__(16) uint8_t dyn_lds[];
     DvrView *vw = reinterpret_cast<DvrView *>(dyn_lds);
     const uint32_t nl = p.nb_lods;
@@ -1007,12 +1031,9 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         vw[0] = p.view0;
     }
     __syncthreads();
-    const uint32_t tf_n0 = __builtin_amdgcn_readfirstlane(vw[0].tf_size);
-    const bool tf_lds = n_views == 1 && tf_n0 <= (uint32_t)TF_LDS;
-    if (tf_lds) {
-        const int N = (int)vw[0].tf_size;
-        for (int i = tid; i < N + 2; i += LANES) tfl[i] = vw[0].tf[i < 1 ? 0 : (i > N ? N - 1 : i - 1)];
-    }
+    float4 *tfl = reinterpret_cast<float4 *>(dyn_lds + tf_lds_base(0, n_views));
+    const bool tf_lds = p.tf_lds_entries != 0;
+    if (tf_lds) load_tf_tables(vw, n_views, tfl, tid);
     __syncthreads();
 
     const uint32_t list_count = __builtin_amdgcn_readfirstlane(*p.list_count);
@@ -1144,7 +1165,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         else sv = (float)sum * inv_norm;
         bool skip;
         const DvrView &V = vw[v];
-        const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(sv, tfl, tf_n0, L.kexp, skip)
+        const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(sv, tfl + (V.tf_lds_off & ~DVR_TF_OWNER), V.tf_size, L.kexp, skip)
                                     : classify<ACC>(sv, gp(reinterpret_cast<const gf4 *>(V.tf)), V.tf_size, L.kexp,
                                                     skip);
         const uint64_t m_fresh = gbits(__ballot(fresh)), m_skip = gbits(__ballot(skip));
@@ -1247,23 +1268,32 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
 }
 
 template <int BYTES, typename ACC>
-const void *kernel_of(uint32_t tile)
+const void *kernel_of(uint32_t tile, bool mt)
 {
+    // float volumes never use LDS tables: one variant
+    if (BYTES == 4 || !mt) {
+        switch (tile & 3) {
+        case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false>);
+        case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4, false>);
+        case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2, false>);
+        default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1, false>);
+        }
+    }
     switch (tile & 3) {
-    case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8>);
-    case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4>);
-    case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2>);
-    default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1>);
+    case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, true>);
+    case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4, true>);
+    case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2, true>);
+    default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1, true>);
     }
 }
 
-const void *dvr_kernel(uint32_t bytes, uint32_t precision, uint32_t tile)
+const void *dvr_kernel(uint32_t bytes, uint32_t precision, uint32_t tile, bool mt)
 {
     const bool d = precision == 64;
     switch (bytes) {
-    case 1: return d ? kernel_of<1, double>(tile) : kernel_of<1, float>(tile);
-    case 2: return d ? kernel_of<2, double>(tile) : kernel_of<2, float>(tile);
-    default: return d ? kernel_of<4, double>(tile) : kernel_of<4, float>(tile);
+    case 1: return d ? kernel_of<1, double>(tile, mt) : kernel_of<1, float>(tile, mt);
+    case 2: return d ? kernel_of<2, double>(tile, mt) : kernel_of<2, float>(tile, mt);
+    default: return d ? kernel_of<4, double>(tile, mt) : kernel_of<4, float>(tile, mt);
     }
 }
 
@@ -1275,12 +1305,50 @@ __global__ void k_fill32(uint32_t *p, uint32_t n, uint32_t value)
 
 } // namespace
 
-uint32_t dvr_lds_bytes(uint32_t nb_lods, uint32_t n_views)
+uint32_t dvr_lds_bytes(uint32_t nb_lods, uint32_t n_views, uint32_t tf_entries)
 {
-    return nb_lods * LANES * (uint32_t)sizeof(float) + std::max(1u, n_views) * (uint32_t)sizeof(DvrView);
+    return tf_lds_base(nb_lods * LANES * (uint32_t)sizeof(float), std::max(1u, n_views)) + tf_entries * 16u;
 }
 
-uint32_t dvr_grid(uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t n_packets, uint32_t lds_bytes)
+uint32_t dvr_assign_tf_lds(DvrView *views, uint32_t n, uint32_t bytes, uint32_t *tables)
+{
+    *tables = 0;
+    // distinct tables in order of first use
+    const float4 *tabs[DVR_TF_LDS_TABLES];
+    uint32_t size[DVR_TF_LDS_TABLES], off[DVR_TF_LDS_TABLES];
+    uint32_t nt = 0, entries = 0;
+    bool fits = bytes != 4;
+    for (uint32_t i = 0; i < n && fits; ++i) {
+        DvrView &V = views[i];
+        V.tf_lds_off = 0;
+        if (V.width == 0 || !V.tf) continue;   // unused slot
+        uint32_t k = 0;
+        while (k < nt && !(tabs[k] == V.tf && size[k] == V.tf_size)) ++k;
+        if (k == nt) {
+            if (nt == DVR_TF_LDS_TABLES || V.tf_size == 0 || V.tf_size > DVR_TF_LDS_MAX) {
+                fits = false;
+                break;
+            }
+            tabs[nt] = V.tf;
+            size[nt] = V.tf_size;
+            off[nt] = entries;
+            entries += V.tf_size + 2;
+            ++nt;
+            V.tf_lds_off = off[k] | DVR_TF_OWNER;
+        } else {
+            V.tf_lds_off = off[k];
+        }
+    }
+    if (!fits) {
+        for (uint32_t i = 0; i < n; ++i) views[i].tf_lds_off = 0;
+        return 0;
+    }
+    *tables = nt;
+    return entries;
+}
+
+uint32_t dvr_grid(uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t n_packets, uint32_t lds_bytes,
+                  uint32_t tf_tables)
 {
     // enough resident workgroups to fill every CU, never more than the packets
     static int cus = 0;
@@ -1291,7 +1359,7 @@ uint32_t dvr_grid(uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t n_
             cus = 256;
     }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dvr_kernel(bytes, precision, tile), LANES,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dvr_kernel(bytes, precision, tile, tf_tables > 1), LANES,
                                                      lds_bytes) != hipSuccess ||
         per_cu <= 0)
         per_cu = 2;
@@ -1311,11 +1379,14 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
     if (p.list && !p.list_count) return hipErrorInvalidValue;
     if (p.march < 1 || p.march > 8) return hipErrorInvalidValue;
     void *args[] = {const_cast<DvrLaunch *>(&p)};
-    return hipLaunchKernel(dvr_kernel(bytes, precision, tile), dim3(grid), dim3(LANES), args,
-                           dvr_lds_bytes(p.nb_lods, p.views ? p.n_views : 1), stream);
+    return hipLaunchKernel(dvr_kernel(bytes, precision, tile, p.tf_lds_tables > 1), dim3(grid), dim3(LANES), args,
+                           dvr_lds_bytes(p.nb_lods, p.views ? p.n_views : 1, p.tf_lds_entries), stream);
 }
 
-uint32_t dvr_ray_lds_bytes(uint32_t n_views) { return std::max(1u, n_views) * (uint32_t)sizeof(DvrView); }
+uint32_t dvr_ray_lds_bytes(uint32_t n_views, uint32_t tf_entries)
+{
+    return tf_lds_base(0, std::max(1u, n_views)) + tf_entries * 16u;
+}
 
 bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes) { return p.pipelined != 0 && (bytes == 1 || bytes == 2); }
 
@@ -1373,7 +1444,8 @@ hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision
     const void *k = ray_kernel(bytes, precision, group);
     if (!k) return hipErrorInvalidValue;
     void *args[] = {const_cast<DvrLaunch *>(&p)};
-    return hipLaunchKernel(k, dim3(grid), dim3(LANES), args, dvr_ray_lds_bytes(p.views ? p.n_views : 1), stream);
+    return hipLaunchKernel(k, dim3(grid), dim3(LANES), args, dvr_ray_lds_bytes(p.views ? p.n_views : 1, p.tf_lds_entries),
+                           stream);
 }
 
 hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream)

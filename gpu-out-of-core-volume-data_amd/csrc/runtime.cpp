@@ -1019,7 +1019,7 @@ struct Batch {
     uint32_t tile = 0, march = 8, refill_min = 16;
     int cur = -1;                       // list holding the last launch's stopped rays (-1: none)
     bool use_graph = false;
-    std::map<std::array<uint32_t, 8>, hipGraphExec_t> graphs;
+    std::map<std::array<uint32_t, 9>, hipGraphExec_t> graphs;
     size_t cbytes_per_pixel() const { return precision == 64 ? 32 : 16; }
     uint64_t slot_pixels() const { return max_pixels / max_views; }
     void drop_graphs()
@@ -1106,10 +1106,14 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
     const bool has_list = B.cur >= 0 && !skip_list;
     const bool append = B.cur >= 0 && skip_list;
     if (packets == 0 && !has_list) return GC_OK;
+    uint32_t tf_tables = 0;
+    const uint32_t tf_entries = dvr_assign_tf_lds(B.h_views, B.table, C.vc.encoded_bytes, &tf_tables);
     GC_HIP(hipMemcpyAsync(B.d_views, B.h_views, (size_t)B.table * sizeof(DvrView), hipMemcpyHostToDevice, s));
     GC_HIP(hipEventRecord(B.views_free, s));
     DvrLaunch L{};
     launch_base(C, L);
+    L.tf_lds_entries = tf_entries;
+    L.tf_lds_tables = tf_tables;
     L.views = B.d_views;
     L.n_views = B.table;
     L.n_packets = packets;
@@ -1137,11 +1141,12 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
     }
     const bool ray_kernel = has_list && packets == 0 && list_rays <= C.ray_list_max &&
                             dvr_ray_supported(L, C.vc.encoded_bytes);
-    const uint32_t lds = dvr_lds_bytes(L.nb_lods, B.table);
+    const uint32_t lds = dvr_lds_bytes(L.nb_lods, B.table, tf_entries);
     // (a captured graph keeps one grid for every list length)
     const uint32_t grid = ray_kernel ? dvr_ray_grid(C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays,
-                                                    dvr_ray_lds_bytes(B.table))
-                                     : dvr_grid(C.vc.encoded_bytes, B.precision, B.tile, packets + (uint32_t)((listed + 63) / 64), lds);
+                                                    dvr_ray_lds_bytes(B.table, tf_entries))
+                                     : dvr_grid(C.vc.encoded_bytes, B.precision, B.tile, packets + (uint32_t)((listed + 63) / 64), lds,
+                                                tf_tables);
     auto launch_k = [&]() -> hipError_t {
         return ray_kernel ? launch_dvr_ray(L, C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays, grid, s)
                           : launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
@@ -1152,8 +1157,9 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
         // stays valid across updates
         GC_HIP(launch_fill32(B.ctl + 10, 1, C.ts, s));
         L.ts_dev = B.ctl + 10;
-        const std::array<uint32_t, 8> key{B.table, B.tile + (ray_kernel ? 8u : 0u), B.march, B.refill_min, packets, grid,
-                                          has_list ? (uint32_t)B.cur : 2u, (uint32_t)out + (append ? 4u : 0u)};
+        const std::array<uint32_t, 9> key{B.table, B.tile + (ray_kernel ? 8u : 0u), B.march, B.refill_min, packets, grid,
+                                          has_list ? (uint32_t)B.cur : 2u, (uint32_t)out + (append ? 4u : 0u),
+                                          tf_entries};
         auto it = B.graphs.find(key);
         if (it == B.graphs.end()) {
             hipGraph_t g = nullptr;
@@ -1811,6 +1817,7 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
     DvrLaunch L{};
     launch_base(C, L);
     view_of(*view, d_tf, tf_size, F, L.view0);
+    L.tf_lds_entries = dvr_assign_tf_lds(&L.view0, 1, C.vc.encoded_bytes, &L.tf_lds_tables);
     L.n_views = 1;
     L.n_packets = packets_of(*view);
     L.stats = d_stats;
@@ -1821,7 +1828,7 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
     hipStream_t s = stream ? (hipStream_t)stream : C.stream;
     GC_HIP(launch_fill32(F.ctl, 8, 0, s));
     const uint32_t grid = dvr_grid(C.vc.encoded_bytes, view->precision, view->tile, L.n_packets,
-                                   dvr_lds_bytes(L.nb_lods, 1));
+                                   dvr_lds_bytes(L.nb_lods, 1, L.tf_lds_entries), L.tf_lds_tables);
     GC_HIP(launch_dvr(L, C.vc.encoded_bytes, view->precision, view->tile, grid, s));
     return GC_OK;
 }
@@ -1839,6 +1846,7 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
     DvrLaunch L{};
     launch_base(C, L);
     view_of(view, d_tf, tf_size, F, L.view0);
+    L.tf_lds_entries = dvr_assign_tf_lds(&L.view0, 1, C.vc.encoded_bytes, &L.tf_lds_tables);
     L.n_views = 1;
     L.n_packets = first ? packets_of(view) : 0;
     L.stats = d_stats;
@@ -1860,10 +1868,12 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
         // a short list of resumed rays: a group of lanes per ray
         L.stats = d_stats + 1;
         GC_HIP(launch_dvr_ray(L, C.vc.encoded_bytes, view.precision, list_rays,
-                              dvr_ray_grid(C.vc.encoded_bytes, view.precision, list_rays, dvr_ray_lds_bytes(1)), s));
+                              dvr_ray_grid(C.vc.encoded_bytes, view.precision, list_rays,
+                                           dvr_ray_lds_bytes(1, L.tf_lds_entries)), s));
     } else {
         const uint32_t grid = dvr_grid(C.vc.encoded_bytes, view.precision, view.tile,
-                                       first ? L.n_packets : (F.w * F.h + 63) / 64, dvr_lds_bytes(L.nb_lods, 1));
+                                       first ? L.n_packets : (F.w * F.h + 63) / 64,
+                                       dvr_lds_bytes(L.nb_lods, 1, L.tf_lds_entries), L.tf_lds_tables);
         GC_HIP(launch_dvr(L, C.vc.encoded_bytes, view.precision, view.tile, grid, s));
     }
     cur = out;

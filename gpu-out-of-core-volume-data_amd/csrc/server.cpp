@@ -58,7 +58,7 @@ struct Volume {
     std::vector<Request *> slots;
     std::vector<bool> fresh;            // slot placed, not launched yet
     bool load_pending = false;
-    uint64_t last_use = 0, samples_seen = 0, bricks_seen = 0;
+    uint64_t last_use = 0, samples_seen = 0, bricks_seen = 0, starts_seen = 0;
     double fetch_seen = 0.0;
     // concurrent views admitted (limit by pool pressure); pressure seen
     // since the last view of this volume completed
@@ -124,6 +124,7 @@ void Server::release(Volume &v)
     v.cache = nullptr;
     v.load_pending = false;
     v.samples_seen = 0;
+    v.starts_seen = 0;
     v.bricks_seen = 0;
     v.fetch_seen = 0.0;
 }
@@ -224,6 +225,8 @@ int Server::device_phase(const std::vector<Volume *> &evict, const std::vector<V
         GC_TRY(gc_batch_stats(v->batch, &rs));
         d.samples += rs.mapped_samples - v->samples_seen;
         v->samples_seen = rs.mapped_samples;
+        d.ray_starts += rs.pixels - v->starts_seen;
+        v->starts_seen = rs.pixels;
         v->last_use = ++tick;
         ++d.launches;
     }
@@ -282,6 +285,7 @@ int Server::iterate(std::unique_lock<std::mutex> &lk)
     const int rc = device_phase(evict, create, place, done, d);
     lk.lock();
     st.samples += d.samples;
+    st.ray_starts += d.ray_starts;
     st.launches += d.launches;
     st.bricks_loaded += d.bricks_loaded;
     st.pressure_events += d.pressure_events;

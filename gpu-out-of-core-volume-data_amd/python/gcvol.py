@@ -115,7 +115,7 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("launches", u64), ("samples", u64),
                 ("pools_created", u64), ("pools_evicted", u64), ("bricks_loaded", u64), ("pressure_events", u64),
                 ("iterations", u64), ("idle_iterations", u64), ("launch_ms", C.c_double), ("update_ms", C.c_double),
-                ("fetch_ms", C.c_double)]
+                ("fetch_ms", C.c_double), ("ray_starts", u64)]
 
 
 class CacheCounters(C.Structure):

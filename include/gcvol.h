@@ -448,6 +448,7 @@ typedef struct gc_server_stats {
     double   update_ms;          /* scheduler time in post-launch updates and
                                     retiring views (waits for the launches)    */
     double   fetch_ms;           /* request handlers' fetch time, all volumes  */
+    uint64_t ray_starts;         /* rays set up, first starts and resumes      */
 } gc_server_stats;
 int gc_server_create(const gc_server_cfg *cfg, int device, gc_server **out);
 int gc_server_destroy(gc_server *s);
