@@ -490,15 +490,18 @@ def main():
         cams.append(cam)
         view = G.make_view(WIDTH, HEIGHT, cam, alpha_stop=ALPHA_STOP, precision=args.precision, tile=args.tile)
         G.synchronize()
+        fetch0 = cache.counters()["fetch_ms"]
         t0 = time.perf_counter()
         frame.reset()
         rep = G.render_frame(cache, view, d_tf, len(tf), frame)
         dt = time.perf_counter() - t0
         d = rep.as_dict()
         d["wall_ms"] = dt * 1e3
-        print("frame %3d %s wall %.1f ms launches %d lane %d bricks %d samples %.3g rays %d kernel %.1f ms" % (
-            k, "timed" if k >= args.warmup else "warmup", dt * 1e3, d["launches"], d["lane_launches"],
-            d["bricks_loaded"], d["mapped_samples"], d["pixels"], d["kernel_ms"]), file=sys.stderr, flush=True)
+        d["fetch_ms"] = cache.counters()["fetch_ms"] - fetch0
+        print("frame %3d %s wall %.1f ms launches %d lane %d bricks %d samples %.3g rays %d kernel %.1f ms "
+              "fetch %.1f ms" % (k, "timed" if k >= args.warmup else "warmup", dt * 1e3, d["launches"],
+                                 d["lane_launches"], d["bricks_loaded"], d["mapped_samples"], d["pixels"],
+                                 d["kernel_ms"], d["fetch_ms"]), file=sys.stderr, flush=True)
         if k >= args.warmup:
             lat.append(dt * 1e3)
             reports.append(d)
@@ -540,6 +543,9 @@ def main():
         "kernel_samples_per_s": S / (kms / 1e3) if kms > 0 else 0.0,
         "launches": launches,
         "bricks_loaded": sum(r["bricks_loaded"] for r in reports),
+        "frame_split_ms": {"kernel": kms / len(reports) if reports else 0.0,
+                           "brick_fetch": sum(r["fetch_ms"] for r in reports) / len(reports) if reports else 0.0,
+                           "wall": sum(lat) / len(lat) if lat else 0.0},
         "roofline": {
             "bound": "hbm",
             "achieved": achieved / 1e9,

@@ -43,6 +43,7 @@ Synth::Synth(const gc_synth_params &p, const uint32_t dims0[3], uint32_t bytes) 
         b.k = 1.0 / (2.0 * sigma * sigma);
         b.radius = 4.0 * sigma;   // contributions beyond 4 sigma are exactly 0
     }
+    build_tables();
 }
 
 // Separable Gaussian factor along one axis for a level-l voxel coordinate:
@@ -54,6 +55,48 @@ static inline float axis_factor(const Synth::Blob &b, int a, uint32_t level, int
     const double d = c - b.c[a];
     if (std::fabs(d) > b.radius) return 0.f;
     return (float)std::exp(-d * d * b.k);
+}
+
+void Synth::build_tables()
+{
+    const size_t nb = blobs.size();
+    for (uint32_t l = 0; l < 16; ++l) {
+        bool any = false;
+        std::array<AxisTable, 3> t;
+        for (int a = 0; a < 3; ++a) {
+            const int64_t ext = ((int64_t)dims[a] + ((int64_t)1 << l) - 1) >> l;
+            any = any || ext > 1 || l == 0;
+            t[a].lo = -TABLE_PAD;
+            t[a].len = ext + 2 * TABLE_PAD;
+            t[a].f.resize(nb * (size_t)t[a].len);
+            for (size_t k = 0; k < nb; ++k)
+                for (int64_t i = 0; i < t[a].len; ++i)
+                    t[a].f[k * (size_t)t[a].len + (size_t)i] = axis_factor(blobs[k], a, l, t[a].lo + i);
+        }
+        table.push_back(std::move(t));
+        if (!any) break;   // every axis is down to one voxel
+    }
+}
+
+// Factors of blob k along axis a for level-l coordinates [o, o + n): from
+// the level's table when the range lies inside it.
+void Synth::axis_factors(size_t k, int a, uint32_t level, int64_t o, uint32_t n, float scale, float *out) const
+{
+    if (level < table.size()) {
+        const AxisTable &t = table[level][a];
+        if (o >= t.lo && o + (int64_t)n <= t.lo + t.len) {
+            const float *src = &t.f[k * (size_t)t.len + (size_t)(o - t.lo)];
+            if (scale == 1.f)
+                for (uint32_t i = 0; i < n; ++i) out[i] = src[i];
+            else
+                for (uint32_t i = 0; i < n; ++i) out[i] = scale * src[i];
+            return;
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const float f = axis_factor(blobs[k], a, level, o + i);
+        out[i] = scale == 1.f ? f : scale * f;
+    }
 }
 
 // 32-bit avalanche hash (murmur3 finaliser) of a voxel key.
@@ -97,9 +140,10 @@ void Synth::fill_box(uint32_t level, const int64_t o[3], const uint32_t n[3], co
     const size_t nb = near.size();
     std::vector<float> fx(nb * n[0]), fy(nb * n[1]), fz(nb * n[2]);
     for (size_t k = 0; k < nb; ++k) {
-        for (uint32_t i = 0; i < n[0]; ++i) fx[k * n[0] + i] = axis_factor(*near[k], 0, level, o[0] + i);
-        for (uint32_t i = 0; i < n[1]; ++i) fy[k * n[1] + i] = axis_factor(*near[k], 1, level, o[1] + i);
-        for (uint32_t i = 0; i < n[2]; ++i) fz[k * n[2] + i] = (float)near[k]->amp * axis_factor(*near[k], 2, level, o[2] + i);
+        const size_t bk = (size_t)(near[k] - blobs.data());
+        axis_factors(bk, 0, level, o[0], n[0], 1.f, &fx[k * n[0]]);
+        axis_factors(bk, 1, level, o[1], n[1], 1.f, &fy[k * n[1]]);
+        axis_factors(bk, 2, level, o[2], n[2], (float)near[k]->amp, &fz[k * n[2]]);
     }
     // per-row x keys of the noise hash
     std::vector<uint32_t> xkey(n[0]);

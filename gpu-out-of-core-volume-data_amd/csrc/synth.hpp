@@ -1,6 +1,7 @@
 // synth.hpp -- host utilities of libgcvol.so (see synth.cpp).
 #pragma once
 
+#include <array>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -26,9 +27,22 @@ struct Synth {
     uint32_t bytes;
     double maxval;
     std::vector<Blob> blobs;
+    // Separable factors of every blob at every voxel coordinate of a level,
+    // computed once: table[l][a] holds blob k's factor along axis a at level-l
+    // coordinate i at [k * len + i - lo], for i in [lo, lo + len) (the level
+    // extent plus TABLE_PAD voxels on each side).  Coordinates outside it are
+    // evaluated directly; both give the same float.
+    static constexpr int64_t TABLE_PAD = 64;
+    struct AxisTable {
+        int64_t lo = 0, len = 0;
+        std::vector<float> f;
+    };
+    std::vector<std::array<AxisTable, 3>> table;
 
     Synth(const gc_synth_params &p, const uint32_t dims0[3], uint32_t bytes);
     int noise_at(uint32_t level, int64_t x, int64_t y, int64_t z) const;
+    void build_tables();
+    void axis_factors(size_t k, int a, uint32_t level, int64_t o, uint32_t n, float scale, float *out) const;
     void fill_box(uint32_t level, const int64_t o[3], const uint32_t n[3], const uint32_t ext[3],
                   void *dst) const;
 };

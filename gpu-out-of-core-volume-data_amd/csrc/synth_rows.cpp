@@ -90,9 +90,13 @@ static inline void sum_rows32(const SynthRows &a, uint32_t ky0, uint32_t xb, flo
 #undef GC_OUT
 }
 
-void SYNTH_ROWS_NAME(const SynthRows &a)
+// NXC: the row width when it is known at compile time (32, the usual brick
+// width), else 0; fixed trip counts let the short per-row loops run without
+// run-time alias checks and remainders.
+template <uint32_t NXC>
+static void rows_impl(const SynthRows &a)
 {
-    const uint32_t nx = a.n[0], ny = a.n[1], nzz = a.n[2];
+    const uint32_t nx = NXC ? NXC : a.n[0], ny = a.n[1], nzz = a.n[2];
     const uint32_t span = 2u * a.noise + 1u;
     const int32_t noise = (int32_t)a.noise;
     float *__restrict row = a.row;
@@ -156,12 +160,24 @@ void SYNTH_ROWS_NAME(const SynthRows &a)
                     for (uint32_t i = 0; i < nx; ++i) nz[i] = 0;
                 }
                 const float *rr = row + r * nx;
-                if (a.bytes == 1) store_row(rr, nz, x0, x1, a.fmax, static_cast<uint8_t *>(a.dst) + rbase);
-                else if (a.bytes == 2) store_row(rr, nz, x0, x1, a.fmax, static_cast<uint16_t *>(a.dst) + rbase);
-                else store_row(rr, nz, x0, x1, a.fmax, static_cast<float *>(a.dst) + rbase);
+                if (NXC && x0 == 0 && x1 == (int64_t)NXC) {
+                    if (a.bytes == 1) store_row(rr, nz, 0, NXC, a.fmax, static_cast<uint8_t *>(a.dst) + rbase);
+                    else if (a.bytes == 2) store_row(rr, nz, 0, NXC, a.fmax, static_cast<uint16_t *>(a.dst) + rbase);
+                    else store_row(rr, nz, 0, NXC, a.fmax, static_cast<float *>(a.dst) + rbase);
+                } else {
+                    if (a.bytes == 1) store_row(rr, nz, x0, x1, a.fmax, static_cast<uint8_t *>(a.dst) + rbase);
+                    else if (a.bytes == 2) store_row(rr, nz, x0, x1, a.fmax, static_cast<uint16_t *>(a.dst) + rbase);
+                    else store_row(rr, nz, x0, x1, a.fmax, static_cast<float *>(a.dst) + rbase);
+                }
             }
         }
     }
+}
+
+void SYNTH_ROWS_NAME(const SynthRows &a)
+{
+    if (a.n[0] == 32) rows_impl<32>(a);
+    else rows_impl<0>(a);
 }
 
 #if defined(SYNTH_ROWS_DISPATCH)
