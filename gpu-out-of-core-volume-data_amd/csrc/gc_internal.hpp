@@ -207,6 +207,8 @@ struct DvrLaunch {
     uint32_t march;             // samples per lane between refill checks (1..8)
     uint32_t tf_lds_entries;    // float4 entries of the LDS transfer-function tables (0: read from HBM)
     uint32_t tf_lds_tables;     // distinct tables among them
+    uint32_t brick_size;        // the geometry's brick size and covering, for the host's choice of kernel
+    uint32_t covering;
     uint32_t pipelined;         // 1: software-pipelined march (requires realDivVirtual == 1 on every
                                 //    level and MRPD y*z cells, x cells below 2^24; see launch_base)
 };

@@ -442,7 +442,11 @@ __device__ inline void load_tf_tables(const DvrView *vw, uint32_t n_views, float
 // MT: the launch's views use more than one LDS transfer-function table (each
 // ray keeps its table's offset and size); otherwise there is one table, at
 // offset 0, whose size the whole launch shares.
-template <int BYTES, typename ACC, int TW, int TH, bool MT>
+// BSC: the brick size when it is fixed at compile time (32 with covering 1,
+// the bricks of the out-of-core configurations), else 0: constant row and
+// plane strides turn the footprint's four loads into one address and three
+// immediate offsets.
+template <int BYTES, typename ACC, int TW, int TH, bool MT, int BSC = 0>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : DVR_WAVES_PER_EU))) void k_dvr(const DvrLaunch p)
 {
     static_assert(TW * TH == 64, "one wave per ray packet");
@@ -490,7 +494,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     const uint32_t lane = tid & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t xcd = blockIdx.x & 7u;
-    const uint32_t bs = g->brick_size, cov = g->covering, inner = g->inner;
+    const uint32_t bs = BSC ? (uint32_t)BSC : g->brick_size, cov = BSC ? 1u : g->covering,
+                   inner = BSC ? (uint32_t)BSC - 2u : g->inner;
     const uint32_t MX = g->mrpd_dim[0], MY = g->mrpd_dim[1];
     const uint64_t bbytes = (uint64_t)bs * bs * bs * BYTES;
     const uint32_t row = bs * BYTES, plane = bs * bs * BYTES;
@@ -1268,8 +1273,12 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
 }
 
 template <int BYTES, typename ACC>
-const void *kernel_of(uint32_t tile, bool mt)
+const void *kernel_of(uint32_t tile, bool mt, bool bs32)
 {
+    // 8x8 tiles of u8/u16 bricks of 32 with covering 1 at FP32: constant strides
+    if (BYTES != 4 && sizeof(ACC) == 4 && (tile & 3) == 0 && bs32)
+        return mt ? reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, true, 32>)
+                  : reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 32>);
     // float volumes never use LDS tables: one variant
     if (BYTES == 4 || !mt) {
         switch (tile & 3) {
@@ -1287,13 +1296,13 @@ const void *kernel_of(uint32_t tile, bool mt)
     }
 }
 
-const void *dvr_kernel(uint32_t bytes, uint32_t precision, uint32_t tile, bool mt)
+const void *dvr_kernel(uint32_t bytes, uint32_t precision, uint32_t tile, bool mt, bool bs32)
 {
     const bool d = precision == 64;
     switch (bytes) {
-    case 1: return d ? kernel_of<1, double>(tile, mt) : kernel_of<1, float>(tile, mt);
-    case 2: return d ? kernel_of<2, double>(tile, mt) : kernel_of<2, float>(tile, mt);
-    default: return d ? kernel_of<4, double>(tile, mt) : kernel_of<4, float>(tile, mt);
+    case 1: return d ? kernel_of<1, double>(tile, mt, bs32) : kernel_of<1, float>(tile, mt, bs32);
+    case 2: return d ? kernel_of<2, double>(tile, mt, bs32) : kernel_of<2, float>(tile, mt, bs32);
+    default: return d ? kernel_of<4, double>(tile, mt, bs32) : kernel_of<4, float>(tile, mt, bs32);
     }
 }
 
@@ -1359,8 +1368,8 @@ uint32_t dvr_grid(uint32_t bytes, uint32_t precision, uint32_t tile, uint32_t n_
             cus = 256;
     }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dvr_kernel(bytes, precision, tile, tf_tables > 1), LANES,
-                                                     lds_bytes) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dvr_kernel(bytes, precision, tile, tf_tables > 1, false),
+                                                     LANES, lds_bytes) != hipSuccess ||
         per_cu <= 0)
         per_cu = 2;
     const uint32_t want = (uint32_t)cus * (uint32_t)per_cu;
@@ -1379,7 +1388,8 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
     if (p.list && !p.list_count) return hipErrorInvalidValue;
     if (p.march < 1 || p.march > 8) return hipErrorInvalidValue;
     void *args[] = {const_cast<DvrLaunch *>(&p)};
-    return hipLaunchKernel(dvr_kernel(bytes, precision, tile, p.tf_lds_tables > 1), dim3(grid), dim3(LANES), args,
+    const bool bs32 = p.brick_size == 32 && p.covering == 1;
+    return hipLaunchKernel(dvr_kernel(bytes, precision, tile, p.tf_lds_tables > 1, bs32), dim3(grid), dim3(LANES), args,
                            dvr_lds_bytes(p.nb_lods, p.views ? p.n_views : 1, p.tf_lds_entries), stream);
 }
 

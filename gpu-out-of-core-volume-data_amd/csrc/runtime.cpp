@@ -936,6 +936,8 @@ static void launch_base(const Cache &C, DvrLaunch &L)
 {
     L.mrpd = C.mrpd;
     L.cells = C.cells;
+    L.brick_size = C.g.brick_size;
+    L.covering = C.g.covering;
     L.request = C.request;
     L.usage = C.usage;
     L.pool = C.pool;

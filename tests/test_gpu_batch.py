@@ -61,6 +61,34 @@ def test_batch_equals_single_views_resident(gpu, prec, tile, march):
     assert np.array_equal(got, ref_st)
 
 
+@pytest.mark.parametrize("n_tables,entries", [(1, 256), (8, 256), (10, 256), (3, 300), (4, 64)])
+def test_batch_transfer_function_tables(gpu, n_tables, entries):
+    """Launches whose views share one table, use several LDS tables (up to 8
+    of up to 256 entries) or fall back to reading the tables from HBM (more
+    than 8 tables, or tables above 256 entries) render every view as it
+    renders alone."""
+    sc = Scene((64, 64, 64), 8, 1, 2)
+    gc = sc.gpu_cache()
+    gc.load_all(sc.device_levels())
+    n_views = 12
+    views = []
+    for i in range(n_views):
+        w, h = SIZES[i % len(SIZES)]
+        cam = G.camera(10.0 + 5.0 * i, 31.0 * i, 2.5 + 0.1 * (i % 4))
+        views.append(G.make_view(w, h, cam, alpha_stop=0.99, precision=32))
+    tables = [tf_table(seed=40 + s, n=entries) for s in range(n_tables)]
+    d = [G.DeviceBuffer.from_array(t) for t in tables]
+    d_tfs = [d[i % n_tables] for i in range(n_views)]
+    sizes = [entries] * n_views
+    print("batch tf tables", n_tables, "entries", entries, flush=True)
+    ref, ref_st = singles(gc, views, d_tfs, sizes)
+    b = G.Batch(gc, 16, 100000, 32)
+    rep = b.render(views, d_tfs, sizes)
+    assert rep.complete == 1
+    for i in range(n_views):
+        assert np.array_equal(b.pixels(i), ref[i]), "view %d differs" % i
+
+
 def _oracle_views(sc, views):
     tables = [tf_table(seed=s) for s in (5, 6, 7)]
     out = []

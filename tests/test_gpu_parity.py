@@ -244,3 +244,65 @@ def test_ray_list_kernel_matches_lane_kernel(gpu, bytes_, prec):
         assert np.array_equal(a.usage(), b.usage()) and np.array_equal(a.request(), b.request()), k
     for c in caches:
         c.close()
+
+
+@pytest.mark.parametrize("bytes_", [1, 2])
+def test_brick32_kernel_matches_generic(gpu, bytes_):
+    """Bricks of 32 with covering 1 at FP32 and 8x8 tiles run a variant of the
+    tile march compiled for those strides; 16x4 tiles run the generic one.
+    The same out-of-core frame sequence (pool smaller than the volume) must
+    give bit-identical pixels, t buffers, colour state, counters and cache
+    state either way, after every frame."""
+    sc = Scene((160, 128, 96), 32, 1, bytes_, cache_size=(5, 4, 4), max_requests=64)
+    caches = [sc.gpu_cache(), sc.gpu_cache()]
+    for c in caches:
+        G.set_ray_list_max(c, 0)                # every launch is a tile / lane-per-ray march
+    tf = tf_table(seed=31)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 96, 80
+    for k in range(4):
+        out = []
+        for c, tile in zip(caches, (0, 1)):
+            f = G.Frame(W, H, 32)
+            view = G.make_view(W, H, orbit(3 * k, distance=2.0 + 0.3 * k), alpha_stop=0.99, precision=32, tile=tile)
+            r = G.render_frame(c, view, d_tf, len(tf), f)
+            tb, cs = f.state()
+            out.append((f.pixels().copy(), tb.copy(), cs.copy(), r.as_dict()))
+            f.close()
+        (p0, t0, c0, r0), (p1, t1, c1, r1) = out
+        print("brick32 frame", k, "launches", r0["launches"], "samples", r0["mapped_samples"], flush=True)
+        assert r0["complete"] and r1["complete"]
+        for key in ("launches", "mapped_samples", "brick_visits", "pixels", "unmapped_rays", "bricks_loaded"):
+            assert r0[key] == r1[key], (k, key, r0[key], r1[key])
+        assert np.array_equal(p0, p1), "pixels differ at frame %d" % k
+        assert np.array_equal(t0, t1) and np.array_equal(c0.view(np.uint8), c1.view(np.uint8)), k
+        a, b = caches
+        assert np.array_equal(a.mrpd(), b.mrpd()) and np.array_equal(a.lru(), b.lru()), k
+        assert np.array_equal(a.usage(), b.usage()) and np.array_equal(a.request(), b.request()), k
+    for c in caches:
+        c.close()
+
+
+@pytest.mark.parametrize("bytes_", [1, 2])
+def test_brick32_fp32_against_oracle(gpu, bytes_):
+    """The bricks-of-32 variant at FP32 against the FP64 oracle, fully
+    resident: RGBA within the FP32 tolerance."""
+    sc = Scene((96, 96, 80), 32, 1, bytes_)
+    gc, oc = sc.gpu_cache(), sc.oracle_cache()
+    gc.load_all(sc.device_levels())
+    oc.load_all()
+    _compare_state(gc, oc)
+    tf = tf_table(seed=7)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 96, 80
+    for k in (0, 2):
+        frame = G.Frame(W, H, 32)
+        otb = np.zeros(W * H, np.float32)
+        ocs = np.zeros(W * H * 4)
+        gst, g, ost, o = _render_pair(sc, gc, oc, orbit(k), 32, W, H, tf, d_tf, frame, otb, ocs, alpha_stop=0.99)
+        err = np.abs(g - o)
+        print("brick32 fp32 frame", k, "stats", gst.tolist(), "max err", err.max(), flush=True)
+        assert gst[2] == W * H
+        assert err.max() <= 2
+        assert (err > 1).mean() < 1e-3
+        frame.close()
