@@ -1003,7 +1003,8 @@ __device__ inline ACC lane_read(ACC x, uint32_t k)
     }
 }
 
-template <int BYTES, typename ACC, int G>
+// BSC as for k_dvr: 32 when the bricks are 32 with covering 1, else 0.
+template <int BYTES, typename ACC, int G, int BSC = 0>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : 4))) void k_dvr_ray(const DvrLaunch p)
 {
     static_assert(G == 16 || G == 32 || G == 64, "16, 32 or 64 lanes per ray");
@@ -1042,7 +1043,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     __syncthreads();
 
     const uint32_t list_count = __builtin_amdgcn_readfirstlane(*p.list_count);
-    const uint32_t bs = g->brick_size, cov = g->covering, inner = g->inner;
+    const uint32_t bs = BSC ? (uint32_t)BSC : g->brick_size, cov = BSC ? 1u : g->covering,
+                   inner = BSC ? (uint32_t)BSC - 2u : g->inner;
     const uint32_t MX = g->mrpd_dim[0], MY = g->mrpd_dim[1];
     const uint32_t bbytes = bs * bs * bs * BYTES;
     const uint32_t row = bs * BYTES, plane = bs * bs * BYTES;
@@ -1405,8 +1407,15 @@ bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes) { return p.pipelined 
 uint32_t dvr_ray_group(uint32_t rays) { return rays <= 256u ? 64u : 16u; }
 
 template <int BYTES, typename ACC>
-static const void *ray_kernel_of(uint32_t group)
+static const void *ray_kernel_of(uint32_t group, bool bs32)
 {
+    if (sizeof(ACC) == 4 && bs32) {
+        switch (group) {
+        case 16: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 16, 32>);
+        case 64: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 64, 32>);
+        default: return nullptr;
+        }
+    }
     switch (group) {
     case 16: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 16>);
     case 64: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 64>);
@@ -1414,11 +1423,11 @@ static const void *ray_kernel_of(uint32_t group)
     }
 }
 
-static const void *ray_kernel(uint32_t bytes, uint32_t precision, uint32_t group)
+static const void *ray_kernel(uint32_t bytes, uint32_t precision, uint32_t group, bool bs32 = false)
 {
     const bool d = precision == 64;
-    if (bytes == 1) return d ? ray_kernel_of<1, double>(group) : ray_kernel_of<1, float>(group);
-    return d ? ray_kernel_of<2, double>(group) : ray_kernel_of<2, float>(group);
+    if (bytes == 1) return d ? ray_kernel_of<1, double>(group, bs32) : ray_kernel_of<1, float>(group, bs32);
+    return d ? ray_kernel_of<2, double>(group, bs32) : ray_kernel_of<2, float>(group, bs32);
 }
 
 uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t lds_bytes, uint32_t group)
@@ -1451,7 +1460,7 @@ hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision
     if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
     if (group == 0) group = dvr_ray_group(rays);
-    const void *k = ray_kernel(bytes, precision, group);
+    const void *k = ray_kernel(bytes, precision, group, p.brick_size == 32 && p.covering == 1);
     if (!k) return hipErrorInvalidValue;
     void *args[] = {const_cast<DvrLaunch *>(&p)};
     return hipLaunchKernel(k, dim3(grid), dim3(LANES), args, dvr_ray_lds_bytes(p.views ? p.n_views : 1, p.tf_lds_entries),

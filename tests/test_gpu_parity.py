@@ -246,17 +246,20 @@ def test_ray_list_kernel_matches_lane_kernel(gpu, bytes_, prec):
         c.close()
 
 
-@pytest.mark.parametrize("bytes_", [1, 2])
-def test_brick32_kernel_matches_generic(gpu, bytes_):
-    """Bricks of 32 with covering 1 at FP32 and 8x8 tiles run a variant of the
-    tile march compiled for those strides; 16x4 tiles run the generic one.
-    The same out-of-core frame sequence (pool smaller than the volume) must
-    give bit-identical pixels, t buffers, colour state, counters and cache
-    state either way, after every frame."""
+@pytest.mark.parametrize("bytes_,resume", [(1, "lane"), (2, "lane"), (2, "group")])
+def test_brick32_kernel_matches_generic(gpu, bytes_, resume):
+    """Bricks of 32 with covering 1 at FP32 run variants of the marches
+    compiled for those strides (the tile march with 8x8 tiles, the
+    group-per-ray march for resumed rays); 16x4 tiles run the generic tile
+    march.  The same out-of-core frame sequence (pool smaller than the
+    volume) must give bit-identical pixels, t buffers, colour state, counters
+    and cache state either way, after every frame.  resume="lane": both
+    caches resume rays with the tile march; "group": the first cache resumes
+    them with the group-per-ray march."""
     sc = Scene((160, 128, 96), 32, 1, bytes_, cache_size=(5, 4, 4), max_requests=64)
     caches = [sc.gpu_cache(), sc.gpu_cache()]
-    for c in caches:
-        G.set_ray_list_max(c, 0)                # every launch is a tile / lane-per-ray march
+    G.set_ray_list_max(caches[0], 1 << 30 if resume == "group" else 0)
+    G.set_ray_list_max(caches[1], 0)            # every launch of the second cache is a generic tile march
     tf = tf_table(seed=31)
     d_tf = G.DeviceBuffer.from_array(tf)
     W, H = 96, 80
