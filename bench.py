@@ -265,6 +265,62 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
                         "%d bricks per update" % (seconds, max_views, max_requests), "rates": out}
 
 
+# on-demand processing: a virtual volume larger than HBM, coarse levels built
+# on the GPU on miss
+ONDEMAND_DIMS = (8192, 8192, 4096)
+ONDEMAND_POOL = 32 << 30
+
+
+def ondemand_workload(G, sessions=8, steps=6, width=1280, height=720, half=0.06):
+    """A virtual 8192x8192x4096 u8 volume (256 GiB at level 0, 9 levels, 32^3
+    bricks, covering 1): level-0 bricks synthesised on the host from seeded
+    hash noise + blobs, every coarser brick built on the GPU on miss by 2x2x2
+    reduction of its finer bricks (gc_source_create_gpu_mip, bit-exact to the
+    CPU preprocessor), 32 GiB pool.  `sessions` interactive sessions of
+    1280x720 views (each its own orbit angle and transfer function) zoom in
+    on a region of interest at the volume's centre (a box of +-`half` in
+    normalised coordinates, 2.6 K level-0 bricks), from 1.6 to 0.25 in
+    `steps` steps, so that the views span the coarse to the fine levels of
+    the region.  Each step renders the sessions' views together (one batch);
+    the region bounds the work: mip on miss for the full volume's coarsest
+    levels would reduce all 256 GiB of level 0."""
+    vc = G.volume_cfg(ONDEMAND_DIMS, 32, 1, 1)
+    params = G.synth_params(SEED + 5, 48, sigma=(100.0, 900.0), amp=(60.0, 255.0), noise=8, threads=0)
+    base = G.Source.synthetic(vc, params)
+    src = G.Source.gpu_mip(vc, base, memo_bytes=2 << 30)
+    cs = G.pool_shape(ONDEMAND_POOL, 32 ** 3)
+    cache = G.Cache(vc, G.cache_cfg(cs, 32, max_requests=16384, sync=False), src)
+    tables = [G.tf_samples(G.tf_seeded(SEED + 400 + i, 6), 256) for i in range(sessions)]
+    d_tfs = [G.DeviceBuffer.from_array(t) for t in tables]
+    box = ((-half, -half, -half), (half, half, half))
+    b = G.Batch(cache, sessions, sessions * width * height, 32)
+    out = []
+    for k in range(steps):
+        dist = 1.6 * (0.25 / 1.6) ** (k / max(1, steps - 1))
+        views = [G.make_view(width, height, G.camera(15.0 + 10.0 * (i % 3), 45.0 * i, dist), alpha_stop=ALPHA_STOP,
+                             precision=32, bbox=box) for i in range(sessions)]
+        c0 = cache.counters()
+        t0 = time.perf_counter()
+        r = b.render(views, d_tfs, [256] * sessions)
+        dt = time.perf_counter() - t0
+        c1 = cache.counters()
+        step = {"distance": dist, "ms": dt * 1e3, "frames_per_s": sessions / dt, "launches": r.launches,
+                "bricks_loaded": c1["bricks_loaded"] - c0["bricks_loaded"],
+                "fetch_ms": c1["fetch_ms"] - c0["fetch_ms"], "samples": r.stats.mapped_samples}
+        print("ondemand step %d distance %.2f: %.0f ms, %d launches, %d bricks, fetch %.0f ms, %.3g samples"
+              % (k, dist, dt * 1e3, r.launches, step["bricks_loaded"], step["fetch_ms"], step["samples"]),
+              file=sys.stderr, flush=True)
+        out.append(step)
+    b.close()
+    cache.close()
+    lat = [s["ms"] for s in out]
+    return {"workload": "virtual 8192x8192x4096 u8 (9 levels, 32^3 bricks, 32 GiB pool), level 0 synthesised on the "
+                        "host, coarser levels built on the GPU on miss; %d sessions of %dx%d zooming 1.6 -> 0.25 on a "
+                        "+-%.2f region in %d steps, one batch per step" % (sessions, width, height, half, steps),
+            "frames_per_s": sessions * len(out) / (sum(lat) / 1e3), "step_p50_ms": percentile(lat, 50),
+            "step_p99_ms": percentile(lat, 99), "steps": out}
+
+
 def batch_workload(G, reps=3):
     """Batched vs plain rendering of 64 concurrent views (sizes uniform among
     256^2, 640x480, 1024^2, 1920x1080, seeded cameras, 8 seeded 256-entry
@@ -433,6 +489,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain workload")
     ap.add_argument("--serving", action="store_true", help="also run the serving workload (several minutes)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving workload")
+    ap.add_argument("--ondemand-only", action="store_true",
+                    help="run only the on-demand workload (virtual 8192x8192x4096 u8, mip on miss)")
     ap.add_argument("--serving-rates", default="4,8,16,32,64", help="request rates (req/s) of the serving workload")
     ap.add_argument("--serving-volumes", default="", help="serve only these volumes (indices into the six)")
     ap.add_argument("--serving-max-views", type=int, default=4, help="views per volume per launch when batched")
@@ -449,7 +507,8 @@ def main():
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    traffic, traffic_note, l2 = ((None, "not measured", None) if (args.no_traffic or args.serving_only)
+    traffic, traffic_note, l2 = ((None, "not measured", None) if (args.no_traffic or args.serving_only or
+                                                                 args.ondemand_only)
                                  else measure_traffic(args))
 
     import gcvol as G
@@ -457,6 +516,9 @@ def main():
     if G.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.ondemand_only:
+        print(json.dumps({"ondemand": ondemand_workload(G)}))
+        return
     if args.serving_only:
         rates = tuple(int(r) for r in args.serving_rates.split(","))
         vols = [int(v) for v in args.serving_volumes.split(",")] if args.serving_volumes else None
