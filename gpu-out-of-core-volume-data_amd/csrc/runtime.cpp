@@ -482,6 +482,7 @@ struct Handler {
     std::thread worker;
     std::mutex m;
     std::condition_variable cv;
+    std::condition_variable idle_cv;   // signalled when a request has been handled
     bool run = false;
     std::atomic<bool> new_request{false};
     std::vector<uint32_t> incoming;   // uint4 list as compacted
@@ -815,7 +816,11 @@ void Handler::loop()
             std::lock_guard<std::mutex> lk(c->done_m);
             c->request_done = rc == GC_OK;
         }
-        new_request = false;
+        {
+            std::lock_guard<std::mutex> lk(m);
+            new_request = false;
+        }
+        idle_cv.notify_all();
     }
 }
 
@@ -1892,8 +1897,9 @@ int gc_cache_busy(gc_cache *c, int *busy)
 int gc_cache_wait_idle(gc_cache *c)
 {
     if (!c) return fail(GC_ERR_ARG, "null cache");
-    Cache &C = c->impl;
-    while (C.handler.new_request.load()) std::this_thread::sleep_for(std::chrono::microseconds(50));
+    Handler &H = c->impl.handler;
+    std::unique_lock<std::mutex> lk(H.m);
+    H.idle_cv.wait(lk, [&] { return !H.new_request.load(); });
     return GC_OK;
 }
 
