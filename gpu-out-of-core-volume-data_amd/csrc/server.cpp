@@ -7,11 +7,12 @@
 //      of the resident volumes plus its own fit the budget, evicting volumes
 //      with nothing in flight (least recently used first) to make room; free
 //      slots of a resident volume take its queued requests in arrival order;
-//   2. launches every resident volume that has rays to trace (one launch per
-//      volume on the volume's own stream, so volumes overlap on the GPU);
-//   3. updates each launched volume's cache (CacheManager::update), reads the
-//      per-slot miss counters and retires the views with no ray left waiting
-//      (pixels copied to the host, latency recorded).
+//   2. visits every resident volume without waiting for the others: when the
+//      volume's launch (on the volume's own stream) has finished, it updates
+//      the cache (CacheManager::update), reads the per-slot miss counters and
+//      retires the views with no ray left waiting (pixels copied to the host,
+//      latency recorded); a volume with no launch in flight sets its newly
+//      admitted views and launches again if it has rays to trace.
 // A volume whose rays all wait for the request handler is not relaunched
 // until the handler is done; the update that then loads the bricks runs
 // before the next launch.  The bricks and the LRU of each volume evolve by
@@ -57,7 +58,11 @@ struct Volume {
     std::deque<Request *> queue;
     std::vector<Request *> slots;
     std::vector<bool> fresh;            // slot placed, not launched yet
+    std::vector<bool> unplaced;         // slot taken, its view not yet set in the batch
+    std::vector<std::pair<uint32_t, Request *>> pending;   // (slot, request) to set before the next launch
     bool load_pending = false;
+    bool in_flight = false;             // a launch is running; `done` is recorded after it
+    hipEvent_t done = nullptr;
     uint64_t last_use = 0, samples_seen = 0, bricks_seen = 0, starts_seen = 0;
     double fetch_seen = 0.0;
     // concurrent views admitted (limit by pool pressure); pressure seen
@@ -111,17 +116,23 @@ struct Server {
     void loop();
     int iterate(std::unique_lock<std::mutex> &lk);
     int device_phase(const std::vector<Volume *> &evict, const std::vector<Volume *> &create,
-                     const std::vector<Place> &place, std::vector<Request *> &done, gc_server_stats &d);
+                     const std::vector<Place> &place, std::vector<Request *> &done, gc_server_stats &d,
+                     bool &progressed);
+    bool progressed = false;
     int make_resident(Volume &v);
     void release(Volume &v);
 };
 
 void Server::release(Volume &v)
 {
+    if (v.done) hipEventSynchronize(v.done);
     if (v.batch) gc_batch_destroy(v.batch);
     if (v.cache) gc_cache_destroy(v.cache);
+    if (v.done) hipEventDestroy(v.done);
     v.batch = nullptr;
     v.cache = nullptr;
+    v.done = nullptr;
+    v.in_flight = false;
     v.load_pending = false;
     v.samples_seen = 0;
     v.starts_seen = 0;
@@ -149,24 +160,87 @@ int Server::make_resident(Volume &v)
     }
     GC_TRY(gc_batch_set_launch(v.batch, cfg.tile, cfg.march, 0));
     GC_TRY(gc_batch_set_graph(v.batch, cfg.graph ? 1 : 0));
+    GC_HIP(hipEventCreateWithFlags(&v.done, hipEventDisableTiming));
     return GC_OK;
 }
 
-// Device work of one iteration (no lock held): evictions, new pools, new
-// views, launches, updates, retirement of finished views.
+// Device work of one iteration (no lock held): evictions, new pools, then
+// per volume, without waiting for the others: a finished launch is followed
+// by its cache update and the retirement of completed views; a volume with
+// no launch in flight takes its new views and launches again.
 int Server::device_phase(const std::vector<Volume *> &evict, const std::vector<Volume *> &create,
-                         const std::vector<Place> &place, std::vector<Request *> &done, gc_server_stats &d)
+                         const std::vector<Place> &place, std::vector<Request *> &done, gc_server_stats &d,
+                         bool &progressed)
 {
-    const auto t0 = Clock::now();
+    progressed = !evict.empty() || !create.empty();
     for (Volume *v : evict) release(*v);
     for (Volume *v : create) GC_TRY(make_resident(*v));
-    for (const Place &p : place)
-        GC_TRY(gc_batch_set_view(p.v->batch, p.slot, &p.r->view, static_cast<const float *>(p.r->d_tf),
-                                 p.r->tf_size));
-    // launches, asynchronous, one per volume with rays to trace
-    std::vector<Volume *> launched;
+    for (const Place &p : place) p.v->pending.emplace_back(p.slot, p.r);
+    std::vector<uint32_t> pend(cfg.max_views);
     for (auto &v : vols) {
-        if (!v.resident() || !v.busy_slots()) continue;
+        if (!v.resident()) continue;
+        if (v.in_flight) {
+            const hipError_t q = hipEventQuery(v.done);
+            if (q == hipErrorNotReady) continue;   // still running
+            GC_HIP(q);
+            v.in_flight = false;
+            progressed = true;
+            // update and retirement
+            const auto t1 = Clock::now();
+            int complete = 0;
+            gc_cache_counters before{}, after{};
+            GC_TRY(gc_cache_counters_get(v.cache, &before));
+            GC_TRY(gc_cache_update(v.cache, &complete));
+            GC_TRY(gc_cache_counters_get(v.cache, &after));
+            if (!complete) v.load_pending = true;
+            if (after.requests != before.requests &&
+                (uint64_t)after.last_used + after.last_requested > after.slots) {
+                // the working set of the views in flight no longer fits the pool:
+                // admit no more than half of them until views complete
+                ++d.pressure_events;
+                v.pressure = true;
+                v.cap = std::max(1u, v.active() / 2);
+            }
+            d.bricks_loaded += after.bricks_loaded - v.bricks_seen;
+            v.bricks_seen = after.bricks_loaded;
+            d.fetch_ms += after.fetch_ms - v.fetch_seen;
+            v.fetch_seen = after.fetch_ms;
+            GC_TRY(gc_batch_pending(v.batch, pend.data()));
+            for (uint32_t s = 0; s < cfg.max_views; ++s) {
+                Request *r = v.slots[s];
+                if (!r || v.unplaced[s]) continue;   // taken after the launch was issued
+                ++r->launches;
+                v.fresh[s] = false;
+                if (pend[s] != 0) continue;
+                r->pixels.resize((size_t)r->view.width * r->view.height);
+                GC_TRY(gc_batch_read_pixels(v.batch, s, r->pixels.data()));
+                GC_TRY(gc_batch_clear_view(v.batch, s));
+                r->t_done = now_ms();
+                v.slots[s] = nullptr;
+                done.push_back(r);
+                if (!v.pressure) v.cap = std::min(cfg.max_views, v.cap + 1);
+                v.pressure = false;
+            }
+            gc_render_stats rs{};
+            GC_TRY(gc_batch_stats(v.batch, &rs));
+            d.samples += rs.mapped_samples - v.samples_seen;
+            v.samples_seen = rs.mapped_samples;
+            d.ray_starts += rs.pixels - v.starts_seen;
+            v.starts_seen = rs.pixels;
+            v.last_use = ++tick;
+            ++d.launches;
+            d.update_ms += std::chrono::duration<double, std::milli>(Clock::now() - t1).count();
+        }
+        // new views join before the next launch
+        const auto t0 = Clock::now();
+        for (const auto &ps : v.pending) {
+            Request *r = ps.second;
+            GC_TRY(gc_batch_set_view(v.batch, ps.first, &r->view, static_cast<const float *>(r->d_tf), r->tf_size));
+            v.unplaced[ps.first] = false;
+            progressed = true;
+        }
+        v.pending.clear();
+        if (!v.busy_slots()) continue;
         bool fresh = false;
         for (uint32_t s = 0; s < cfg.max_views; ++s) fresh = fresh || (v.slots[s] && v.fresh[s]);
         int busy = 0;
@@ -179,58 +253,14 @@ int Server::device_phase(const std::vector<Volume *> &evict, const std::vector<V
         }
         // while the handler fetches, only the starting views are traced
         GC_TRY(gc_batch_launch(v.batch, busy ? GC_LAUNCH_STARTING_ONLY : 0u, nullptr));
-        launched.push_back(&v);
+        void *stream = nullptr;
+        GC_TRY(gc_cache_stream(v.cache, &stream));
+        GC_HIP(hipEventRecord(v.done, static_cast<hipStream_t>(stream)));
+        v.in_flight = true;
+        progressed = true;
+        d.launch_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     }
-    const auto t1 = Clock::now();
-    d.launch_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
-    if (launched.empty()) ++d.idle_iterations;
-    // updates and retirement
-    std::vector<uint32_t> pend(cfg.max_views);
-    for (Volume *v : launched) {
-        int complete = 0;
-        gc_cache_counters before{}, after{};
-        GC_TRY(gc_cache_counters_get(v->cache, &before));
-        GC_TRY(gc_cache_update(v->cache, &complete));
-        GC_TRY(gc_cache_counters_get(v->cache, &after));
-        if (!complete) v->load_pending = true;
-        if (after.requests != before.requests &&
-            (uint64_t)after.last_used + after.last_requested > after.slots) {
-            // the working set of the views in flight no longer fits the pool:
-            // admit no more than half of them until views complete
-            ++d.pressure_events;
-            v->pressure = true;
-            v->cap = std::max(1u, v->active() / 2);
-        }
-        d.bricks_loaded += after.bricks_loaded - v->bricks_seen;
-        v->bricks_seen = after.bricks_loaded;
-        d.fetch_ms += after.fetch_ms - v->fetch_seen;
-        v->fetch_seen = after.fetch_ms;
-        GC_TRY(gc_batch_pending(v->batch, pend.data()));
-        for (uint32_t s = 0; s < cfg.max_views; ++s) {
-            Request *r = v->slots[s];
-            if (!r) continue;
-            ++r->launches;
-            v->fresh[s] = false;
-            if (pend[s] != 0) continue;
-            r->pixels.resize((size_t)r->view.width * r->view.height);
-            GC_TRY(gc_batch_read_pixels(v->batch, s, r->pixels.data()));
-            GC_TRY(gc_batch_clear_view(v->batch, s));
-            r->t_done = now_ms();
-            v->slots[s] = nullptr;
-            done.push_back(r);
-            if (!v->pressure) v->cap = std::min(cfg.max_views, v->cap + 1);
-            v->pressure = false;
-        }
-        gc_render_stats rs{};
-        GC_TRY(gc_batch_stats(v->batch, &rs));
-        d.samples += rs.mapped_samples - v->samples_seen;
-        v->samples_seen = rs.mapped_samples;
-        d.ray_starts += rs.pixels - v->starts_seen;
-        v->starts_seen = rs.pixels;
-        v->last_use = ++tick;
-        ++d.launches;
-    }
-    d.update_ms += std::chrono::duration<double, std::milli>(Clock::now() - t1).count();
+    if (!progressed) ++d.idle_iterations;
     return GC_OK;
 }
 
@@ -272,6 +302,7 @@ int Server::iterate(std::unique_lock<std::mutex> &lk)
             v.queue.pop_front();
             v.slots[s] = r;
             v.fresh[s] = true;
+            v.unplaced[s] = true;
             ++in_flight;
             place.push_back({&v, s, r});
         }
@@ -282,7 +313,7 @@ int Server::iterate(std::unique_lock<std::mutex> &lk)
     lk.unlock();
     std::vector<Request *> done;
     gc_server_stats d{};
-    const int rc = device_phase(evict, create, place, done, d);
+    const int rc = device_phase(evict, create, place, done, d, progressed);
     lk.lock();
     st.samples += d.samples;
     st.ray_starts += d.ray_starts;
@@ -322,6 +353,13 @@ void Server::loop()
             error = gc_last_error();
             done_cv.notify_all();
             break;
+        }
+        if (!progressed) {
+            // launches still running and handlers still fetching: give the
+            // host cores to the handlers for a moment
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            lk.lock();
         }
     }
 }
@@ -392,6 +430,7 @@ int gc_server_add_volume(gc_server *s, const gc_volume_cfg *vc, gc_source *src, 
     if (v.footprint > S.cfg.pool_budget) return fail(GC_ERR_ARG, "volume's pool and view slots exceed the budget");
     v.slots.assign(S.cfg.max_views, nullptr);
     v.fresh.assign(S.cfg.max_views, false);
+    v.unplaced.assign(S.cfg.max_views, false);
     S.vols.push_back(std::move(v));
     *volume = (uint32_t)(S.vols.size() - 1);
     return GC_OK;
