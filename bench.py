@@ -268,15 +268,15 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
 # on-demand processing: a virtual volume larger than HBM, coarse levels built
 # on the GPU on miss
 ONDEMAND_DIMS = (8192, 8192, 4096)
-ONDEMAND_POOL = 32 << 30
+ONDEMAND_POOL = 128 << 30
 
 
-def ondemand_workload(G, sessions=8, steps=6, width=1280, height=720, half=0.06):
+def ondemand_workload(G, sessions=32, steps=6, width=1280, height=720, half=0.06):
     """A virtual 8192x8192x4096 u8 volume (256 GiB at level 0, 9 levels, 32^3
     bricks, covering 1): level-0 bricks synthesised on the host from seeded
     hash noise + blobs, every coarser brick built on the GPU on miss by 2x2x2
     reduction of its finer bricks (gc_source_create_gpu_mip, bit-exact to the
-    CPU preprocessor), 32 GiB pool.  `sessions` interactive sessions of
+    CPU preprocessor), 128 GiB pool.  `sessions` interactive sessions of
     1280x720 views (each its own orbit angle and transfer function) zoom in
     on a region of interest at the volume's centre (a box of +-`half` in
     normalised coordinates, 2.6 K level-0 bricks), from 1.6 to 0.25 in
@@ -314,7 +314,7 @@ def ondemand_workload(G, sessions=8, steps=6, width=1280, height=720, half=0.06)
     b.close()
     cache.close()
     lat = [s["ms"] for s in out]
-    return {"workload": "virtual 8192x8192x4096 u8 (9 levels, 32^3 bricks, 32 GiB pool), level 0 synthesised on the "
+    return {"workload": "virtual 8192x8192x4096 u8 (9 levels, 32^3 bricks, 128 GiB pool), level 0 synthesised on the "
                         "host, coarser levels built on the GPU on miss; %d sessions of %dx%d zooming 1.6 -> 0.25 on a "
                         "+-%.2f region in %d steps, one batch per step" % (sessions, width, height, half, steps),
             "frames_per_s": sessions * len(out) / (sum(lat) / 1e3), "step_p50_ms": percentile(lat, 50),
