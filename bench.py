@@ -146,7 +146,7 @@ SERVE_BUDGET = 96 << 30
 
 
 def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm_requests=24, fixed_ab=False,
-                     volumes=None, max_requests=16384):
+                     volumes=None, max_requests=16384, device_source=False):
     """Requests arrive as a Poisson process (seeded) over 6 volumes (512^3 u8,
     1024^3 u8, 1024^3 u16, 2048^3 u8, 2048x2048x1024 u16, 4096^3 u8, each
     synthesised brick by brick on the host on demand, 32^3 bricks, 12 GiB
@@ -157,7 +157,9 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
     streams) and by the plain renderer (one request at a time, one view per
     launch).  Each cache loads up to `max_requests` bricks per update (the
     upload batch; 16384 measured best against 4096 for both loops).  Latency is
-    submission to pixels on the host."""
+    submission to pixels on the host.  With `device_source`, the bricks are
+    synthesised on the GPU straight into each cache's staging buffer (same
+    bytes, no host arithmetic, no PCIe upload)."""
     import threading
 
     tables = [G.tf_samples(G.tf_seeded(SEED + 200 + i, 6), 256) for i in range(8)]
@@ -167,7 +169,7 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
         amp = (60.0, 255.0) if nb == 1 else (15000.0, 65535.0)
         params = G.synth_params(SEED + 10 + i, 32, sigma=(dims[0] / 40.0, dims[0] / 8.0), amp=amp,
                                 noise=8 if nb == 1 else 2048, threads=0)
-        sources.append(G.Source.synthetic(vc, params))
+        sources.append(G.Source.synthetic(vc, params, device=0 if device_source else None))
         vcs.append(vc)
 
     def make_requests(rate, seed):
@@ -258,11 +260,12 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
         out.append({"rate": rate, "requests": len(reqs), "batched": b, "plain": pl})
         if bf:
             out[-1]["batched_fixed_views"] = bf
-    return {"workload": "Poisson arrivals for %.0f s per rate over 6 volumes synthesised on demand (512^3 u8, "
+    return {"workload": "Poisson arrivals for %.0f s per rate over 6 volumes synthesised on demand %s(512^3 u8, "
                         "1024^3 u8, 1024^3 u16, 2048^3 u8, 2048x2048x1024 u16, 4096^3 u8; 32^3 bricks, 12 GiB "
                         "pool each, 96 GiB budget), square views 256..2048, own camera and TF per request; "
                         "batched = serving loop (<= %d views per volume per launch), plain = one request at a time; "
-                        "%d bricks per update" % (seconds, max_views, max_requests), "rates": out}
+                        "%d bricks per update" % (seconds, "on the GPU " if device_source else "on the host ",
+                                                  max_views, max_requests), "rates": out}
 
 
 # on-demand processing: a virtual volume larger than HBM, coarse levels built
@@ -495,6 +498,8 @@ def main():
     ap.add_argument("--serving-volumes", default="", help="serve only these volumes (indices into the six)")
     ap.add_argument("--serving-max-views", type=int, default=4, help="views per volume per launch when batched")
     ap.add_argument("--serving-max-requests", type=int, default=16384, help="bricks per update of each served volume")
+    ap.add_argument("--serving-source", choices=("host", "gpu"), default="host",
+                    help="where the served volumes' bricks are synthesised")
     ap.add_argument("--serving-ab", action="store_true",
                     help="also serve each arrival sequence with a fixed number of views per volume")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -524,7 +529,8 @@ def main():
         vols = [int(v) for v in args.serving_volumes.split(",")] if args.serving_volumes else None
         print(json.dumps({"serving": serving_workload(G, rates=rates, fixed_ab=args.serving_ab, volumes=vols,
                                                       max_views=args.serving_max_views,
-                                                      max_requests=args.serving_max_requests)}))
+                                                      max_requests=args.serving_max_requests,
+                                                      device_source=args.serving_source == "gpu")}))
         return
 
     if args.calib_copy_mib > 0:

@@ -147,6 +147,27 @@ struct MipBrickArgs {
 hipError_t launch_mip_brick(const MipBrickArgs &a, uint32_t bytes, const void *children, void *out,
                             hipStream_t stream);
 
+// Synthetic bricks on the device (k_synth_bricks): blob k's factor along axis
+// a at level-l coordinate c is tab[axis[l][a].off + k * len + c - lo] for c in
+// [lo, lo + len).
+struct SynthAxis {
+    uint64_t off;
+    int64_t lo;
+    uint32_t len, pad;
+};
+struct SynthLaunch {
+    const float *tab;          // axis tables of every level
+    const float *amp;          // (float) amplitude of every blob
+    const uint32_t *keys;      // n x (bx, by, bz, level)
+    uint8_t *dst;              // n bricks of bs^3 voxels
+    uint32_t *below;           // n: 1 when every voxel is below thr
+    uint32_t n, nb, bs, cov, bytes, seed, noise;
+    float fmax, thr;
+    SynthAxis axis[GC_MAX_LEVELS][3];
+    uint32_t ext[GC_MAX_LEVELS][3];   // level extents (voxels outside are 0)
+};
+hipError_t launch_synth_bricks(const SynthLaunch &p, hipStream_t s);
+
 hipError_t launch_brick_cut(const void *level, const uint32_t dims[3], uint32_t bytes,
                             uint32_t bs, uint32_t cov, const uint32_t *bricks_xyz, uint32_t n,
                             void *out, hipStream_t stream);

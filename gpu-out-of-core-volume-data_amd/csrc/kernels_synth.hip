@@ -1,0 +1,111 @@
+// kernels_synth.hip -- seeded synthetic bricks written on the device.
+//
+// The device form of Synth::fill_box (synth.cpp / synth_rows.cpp): one
+// workgroup per requested brick writes the brick straight into device memory
+// (the cache's staging buffer), so a volume synthesised on demand costs no
+// host arithmetic and no PCIe upload.  Per voxel inside the level:
+//   sum  = sum over blobs k, in blob order, of (amp_k fz_k(z) * fy_k(y)) * fx_k(x)
+//   value = clamp((float)(int)(sum + 0.5f) + noise(x, y, z), 0, max)
+// with the separable factors read from the host's per-level axis tables
+// (copied to the device once), each product and sum a single IEEE float
+// operation (the library is built with -ffp-contract=off) and the same
+// integer noise hash, so every brick is bit-identical to the host's.  Blobs
+// that do not reach the brick have a zero factor on some axis and add +0,
+// which leaves the (non-negative) sum unchanged: the host skips them, the
+// device adds them.  Voxels outside the level are 0.
+#include <hip/hip_runtime.h>
+
+#include "gc_internal.hpp"
+
+namespace gc {
+
+__device__ __forceinline__ uint32_t synth_hash32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+template <typename T>
+__device__ __forceinline__ void synth_store(uint8_t *out, uint64_t i, float v)
+{
+    if constexpr (sizeof(T) < 4)
+        reinterpret_cast<T *>(out)[i] = (T)(int32_t)v;
+    else
+        reinterpret_cast<float *>(out)[i] = v;
+}
+
+// Grid: one workgroup of 256 lanes per brick (p.n workgroups).  Dynamic LDS:
+// 3 * nb * bs floats (the brick's x, y and amp-scaled z factors of every
+// blob).
+template <typename T>
+__global__ __launch_bounds__(256) void k_synth_bricks(SynthLaunch p)
+{
+    extern __shared__ float sf[];
+    const uint32_t u = blockIdx.x;
+    const uint32_t *key = p.keys + 4ull * u;
+    const uint32_t level = key[3], bs = p.bs, nb = p.nb, inner = bs - 2 * p.cov;
+    const int64_t o0 = (int64_t)key[0] * inner - p.cov, o1 = (int64_t)key[1] * inner - p.cov,
+                  o2 = (int64_t)key[2] * inner - p.cov;
+    const uint32_t nf = nb * bs;
+    for (uint32_t i = threadIdx.x; i < 3 * nf; i += blockDim.x) {
+        const uint32_t a = i / nf, r = i - a * nf, k = r / bs, j = r - k * bs;
+        const SynthAxis ax = p.axis[level][a];
+        const int64_t c = (a == 0 ? o0 : (a == 1 ? o1 : o2)) + j;
+        float f = 0.f;
+        if (c >= ax.lo && c < ax.lo + (int64_t)ax.len) f = p.tab[ax.off + (uint64_t)k * ax.len + (uint64_t)(c - ax.lo)];
+        if (a == 2) f = p.amp[k] * f;
+        sf[i] = f;
+    }
+    __syncthreads();
+    const float *fx = sf, *fy = sf + nf, *fz = sf + 2 * nf;
+    const uint32_t e0 = p.ext[level][0], e1 = p.ext[level][1], e2 = p.ext[level][2];
+    const uint32_t span = 2u * p.noise + 1u;
+    const uint64_t nvox = (uint64_t)bs * bs * bs;
+    uint8_t *out = p.dst + (uint64_t)u * nvox * sizeof(T);
+    int above = 0;
+    for (uint32_t v = threadIdx.x; v < nvox; v += blockDim.x) {
+        const uint32_t x = v % bs, y = (v / bs) % bs, z = v / (bs * bs);
+        const int64_t gx = o0 + x, gy = o1 + y, gz = o2 + z;
+        float val = 0.f;
+        if (gx >= 0 && gx < (int64_t)e0 && gy >= 0 && gy < (int64_t)e1 && gz >= 0 && gz < (int64_t)e2) {
+            float c = 0.f;
+            for (uint32_t k = 0; k < nb; ++k) {
+                const float w = fz[k * bs + z] * fy[k * bs + y];
+                c += w * fx[k * bs + x];
+            }
+            int32_t nz = 0;
+            if (p.noise) {
+                const uint32_t rk = p.seed ^ (level * 0x27d4eb2fu) ^ ((uint32_t)gz * 0x165667b1u) ^
+                                    ((uint32_t)gy * 0xc2b2ae3du);
+                const uint32_t h = synth_hash32(rk ^ ((uint32_t)gx * 0x9e3779b1u));
+                nz = (int32_t)(uint32_t)(((uint64_t)h * span) >> 32) - (int32_t)p.noise;
+            }
+            float r = (float)(int32_t)(c + 0.5f) + (float)nz;
+            r = r < 0.f ? 0.f : r;
+            r = r > p.fmax ? p.fmax : r;
+            val = r;
+        }
+        synth_store<T>(out, v, val);
+        above |= !(val < p.thr);
+    }
+    // empty classification: every voxel below the threshold
+    // (BrickProcessorPredicate.hpp:11-45)
+    above = __syncthreads_or(above);
+    if (threadIdx.x == 0) p.below[u] = above ? 0u : 1u;
+}
+
+hipError_t launch_synth_bricks(const SynthLaunch &p, hipStream_t s)
+{
+    if (p.n == 0) return hipSuccess;
+    const size_t lds = (size_t)3 * p.nb * p.bs * sizeof(float);
+    if (p.bytes == 1) k_synth_bricks<uint8_t><<<p.n, 256, lds, s>>>(p);
+    else if (p.bytes == 2) k_synth_bricks<uint16_t><<<p.n, 256, lds, s>>>(p);
+    else k_synth_bricks<float><<<p.n, 256, lds, s>>>(p);
+    return hipGetLastError();
+}
+
+} // namespace gc

@@ -192,6 +192,16 @@ struct Source {
     virtual int get(uint32_t level, const uint32_t bbc[3], void *dst) = 0;   // GC_BRICK_*
     virtual bool thread_safe() const { return true; }
     virtual uint32_t threads() const { return 1; }
+    // Sources that write bricks on the device: n bricks (keys: bx, by, bz,
+    // level) into d_dst on stream s, below[i] = every voxel of brick i below
+    // thr; returns when the bricks are written.
+    virtual bool on_device() const { return false; }
+    virtual int get_device(const uint32_t *keys, uint32_t n, uint8_t *d_dst, float thr, uint8_t *below,
+                           hipStream_t s)
+    {
+        (void)keys; (void)n; (void)d_dst; (void)thr; (void)below; (void)s;
+        return fail(GC_ERR_STATE, "source does not write bricks on the device");
+    }
     void init(const gc_volume_cfg &v)
     {
         vc = v;
@@ -282,6 +292,99 @@ struct SynthSource : Source {
                         }
                 }
         return GC_BRICK_SUCCESS;
+    }
+};
+
+// Synthetic volume written on the device (k_synth_bricks): the host's axis
+// tables of every level go to the device once; requested bricks are
+// synthesised straight into device memory, bit-identical to SynthSource's.
+// get() (host bricks) stays available.
+struct DeviceSynthSource : SynthSource {
+    int device = 0;
+    float *d_tab = nullptr, *d_amp = nullptr;
+    uint32_t *d_keys = nullptr, *d_below = nullptr;
+    uint32_t cap_n = 0;
+    SynthLaunch base{};
+    std::mutex m;
+    bool on_device() const override { return true; }
+    ~DeviceSynthSource() override
+    {
+        hipFree(d_tab); hipFree(d_amp); hipFree(d_keys); hipFree(d_below);
+    }
+    int setup()
+    {
+        const Synth &S = *synth;
+        const uint32_t nb = (uint32_t)S.blobs.size();
+        if (BB[0] != 1 || BB[1] != 1 || BB[2] != 1)
+            return fail(GC_ERR_CONFIG, "bricks synthesised on the device need BigBrickSize 1");
+        if ((uint64_t)3 * nb * bs * sizeof(float) > 65536)
+            return fail(GC_ERR_CONFIG, "3 x blobs x brick size factors exceed 64 KiB of LDS");
+        if (vc.nb_levels == 0 || vc.nb_levels > GC_MAX_LEVELS) return fail(GC_ERR_CONFIG, "bad level count");
+        // factors of every level over its extent plus a margin on each side
+        // (covering voxels of border bricks lie outside the extent and are 0)
+        const int64_t pad = Synth::TABLE_PAD;
+        std::vector<float> tab;
+        for (uint32_t l = 0; l < vc.nb_levels; ++l)
+            for (int a = 0; a < 3; ++a) {
+                SynthAxis &ax = base.axis[l][a];
+                ax.off = tab.size();
+                ax.lo = -pad;
+                ax.len = vc.initial_sizes[l][a] + 2 * (uint32_t)pad;
+                tab.resize(tab.size() + (size_t)nb * ax.len);
+                for (uint32_t k = 0; k < nb; ++k)
+                    S.axis_factors(k, a, l, ax.lo, ax.len, 1.f, &tab[ax.off + (size_t)k * ax.len]);
+                base.ext[l][a] = vc.initial_sizes[l][a];
+            }
+        std::vector<float> amp(nb);
+        for (uint32_t k = 0; k < nb; ++k) amp[k] = (float)S.blobs[k].amp;
+        GC_HIP(hipSetDevice(device));
+        GC_HIP(hipMalloc((void **)&d_tab, std::max<size_t>(tab.size(), 1) * sizeof(float)));
+        GC_HIP(hipMalloc((void **)&d_amp, std::max<size_t>(nb, 1) * sizeof(float)));
+        GC_HIP(hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+        GC_HIP(hipMemcpy(d_amp, amp.data(), nb * sizeof(float), hipMemcpyHostToDevice));
+        base.tab = d_tab;
+        base.amp = d_amp;
+        base.nb = nb;
+        base.bs = bs;
+        base.cov = cov;
+        base.bytes = bytes;
+        base.seed = (uint32_t)S.prm.seed;
+        base.noise = S.prm.noise;
+        base.fmax = (float)S.maxval;
+        return GC_OK;
+    }
+    int get_device(const uint32_t *keys, uint32_t n, uint8_t *d_dst, float thr, uint8_t *below,
+                   hipStream_t s) override
+    {
+        if (n == 0) return GC_OK;
+        int cur = -1;
+        GC_HIP(hipGetDevice(&cur));
+        if (cur != device) return fail(GC_ERR_ARG, "brick source lives on another device");
+        for (uint32_t i = 0; i < n; ++i)
+            if (keys[4 * i + 3] >= vc.nb_levels) return fail(GC_ERR_SOURCE, "brick level out of range");
+        std::lock_guard<std::mutex> lk(m);
+        if (n > cap_n) {
+            hipFree(d_keys);
+            hipFree(d_below);
+            d_keys = d_below = nullptr;
+            cap_n = 0;
+            GC_HIP(hipMalloc((void **)&d_keys, (size_t)n * 16));
+            GC_HIP(hipMalloc((void **)&d_below, (size_t)n * 4));
+            cap_n = n;
+        }
+        std::vector<uint32_t> flags(n);
+        SynthLaunch p = base;
+        p.keys = d_keys;
+        p.dst = d_dst;
+        p.below = d_below;
+        p.n = n;
+        p.thr = thr;
+        GC_HIP(hipMemcpyAsync(d_keys, keys, (size_t)n * 16, hipMemcpyHostToDevice, s));
+        GC_HIP(launch_synth_bricks(p, s));
+        GC_HIP(hipMemcpyAsync(flags.data(), d_below, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        GC_HIP(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < n; ++i) below[i] = (uint8_t)flags[i];
+        return GC_OK;
     }
 };
 
@@ -871,22 +974,34 @@ int Handler::handle(const std::vector<uint32_t> &list)
         st[u] = s;
     };
     const uint32_t nth = S.thread_safe() ? std::max(1u, S.threads()) : 1u;
-    // Bricks go up on the side stream in chunks as soon as a chunk is
-    // fetched, so the upload overlaps the fetches of the later chunks.  The
-    // device staging buffer is rewritten only after the last load consumed it.
+    // The device staging buffer is rewritten only after the last load
+    // consumed it.
     GC_HIP(hipStreamWaitEvent(c->upload, c->consumed, 0));
-    const uint32_t chunk = 64;
-    std::atomic<int> up_err{(int)hipSuccess};
-    parallel_for_chunks(nu, nth, chunk, fetch, [&](uint32_t k) {
-        const uint64_t u0 = (uint64_t)k * chunk, nb = std::min<uint64_t>(chunk, nu - u0);
-        hipSetDevice(c->device);
-        const hipError_t e = hipMemcpyAsync(c->d_staging + u0 * S.bb_bytes, c->h_staging + u0 * S.bb_bytes,
-                                            nb * S.bb_bytes, hipMemcpyHostToDevice, c->upload);
-        if (e != hipSuccess) up_err = (int)e;
-    });
-    GC_HIP(hipEventRecord(c->uploaded, c->upload));
-    upload_pending = true;
-    if (up_err != (int)hipSuccess) GC_HIP((hipError_t)up_err.load());
+    if (S.on_device()) {
+        // the source writes the bricks straight into the staging buffer
+        std::vector<uint32_t> keys(4 * (size_t)nu);
+        for (uint32_t u = 0; u < nu; ++u) std::memcpy(&keys[4 * (size_t)u], uniq[u].data(), 16);
+        std::vector<uint8_t> below(nu);
+        GC_TRY(S.get_device(keys.data(), nu, c->d_staging, thr, below.data(), c->upload));
+        for (uint32_t u = 0; u < nu; ++u) st[u] = below[u] ? GC_BRICK_EMPTY : GC_BRICK_SUCCESS;
+        GC_HIP(hipEventRecord(c->uploaded, c->upload));
+        upload_pending = true;
+    } else {
+        // Bricks go up on the side stream in chunks as soon as a chunk is
+        // fetched, so the upload overlaps the fetches of the later chunks.
+        const uint32_t chunk = 64;
+        std::atomic<int> up_err{(int)hipSuccess};
+        parallel_for_chunks(nu, nth, chunk, fetch, [&](uint32_t k) {
+            const uint64_t u0 = (uint64_t)k * chunk, nb = std::min<uint64_t>(chunk, nu - u0);
+            hipSetDevice(c->device);
+            const hipError_t e = hipMemcpyAsync(c->d_staging + u0 * S.bb_bytes, c->h_staging + u0 * S.bb_bytes,
+                                                nb * S.bb_bytes, hipMemcpyHostToDevice, c->upload);
+            if (e != hipSuccess) up_err = (int)e;
+        });
+        GC_HIP(hipEventRecord(c->uploaded, c->upload));
+        upload_pending = true;
+        if (up_err != (int)hipSuccess) GC_HIP((hipError_t)up_err.load());
+    }
     for (uint32_t u = 0; u < nu; ++u)
         if (st[u] == GC_BRICK_ERROR)
             return fail(GC_ERR_SOURCE, "brick source failed for level " + std::to_string(uniq[u][3]));
@@ -1439,6 +1554,32 @@ int gc_source_create_synthetic(const gc_volume_cfg *vc, const gc_synth_params *p
     s->nthreads = p->threads ? p->threads : hw_threads();
     *out = new gc_source{std::move(s)};
     return GC_OK;
+}
+
+int gc_source_create_synthetic_gpu(const gc_volume_cfg *vc, const gc_synth_params *p, int device, gc_source **out)
+{
+    if (!vc || !p || !out) return fail(GC_ERR_ARG, "null argument");
+    auto s = std::make_unique<DeviceSynthSource>();
+    s->init(*vc);
+    s->synth = std::make_unique<Synth>(*p, vc->initial_sizes[0], vc->encoded_bytes);
+    s->nthreads = p->threads ? p->threads : hw_threads();
+    s->device = device;
+    GC_TRY(s->setup());
+    *out = new gc_source{std::move(s)};
+    return GC_OK;
+}
+
+int gc_source_get_device(gc_source *src, const uint32_t *keys, uint32_t n, void *d_dst, float threshold,
+                         uint8_t *empty)
+{
+    if (!src || (n && (!keys || !d_dst || !empty))) return fail(GC_ERR_ARG, "null argument");
+    Source &S = *src->impl;
+    if (!S.on_device()) return fail(GC_ERR_ARG, "source does not write bricks on the device");
+    hipStream_t s = nullptr;
+    GC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const int rc = S.get_device(keys, n, static_cast<uint8_t *>(d_dst), threshold, empty, s);
+    hipStreamDestroy(s);
+    return rc;
 }
 
 int gc_source_create_callback(const gc_volume_cfg *vc, gc_brick_fn fn, void *user, gc_source **out)

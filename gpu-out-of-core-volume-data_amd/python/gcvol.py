@@ -142,6 +142,7 @@ EXPORTS = [
     "gc_cache_set_ray_list_max",
     "gc_cache_busy", "gc_source_create_gpu_mip", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
     "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get", "gc_cache_counters_get",
+    "gc_source_create_synthetic_gpu", "gc_source_get_device",
 ]
 
 _lib = None
@@ -205,6 +206,8 @@ def lib():
             "gc_batch_stats": [P, C.POINTER(RenderStats)], "gc_batch_read_state": [P, u32, P, P],
             "gc_cache_busy": [P, C.POINTER(C.c_int)],
             "gc_source_create_gpu_mip": [C.POINTER(VolumeCfg), P, C.c_int, u64, C.POINTER(vp)],
+            "gc_source_create_synthetic_gpu": [C.POINTER(VolumeCfg), C.POINTER(SynthParams), C.c_int, C.POINTER(vp)],
+            "gc_source_get_device": [P, P, u32, P, C.c_float, P],
             "gc_server_create": [C.POINTER(ServerCfg), C.c_int, C.POINTER(vp)], "gc_server_destroy": [P],
             "gc_server_add_volume": [P, C.POINTER(VolumeCfg), P, u64, u32, C.POINTER(u32)],
             "gc_server_submit": [P, u32, C.POINTER(View), P, u32, C.POINTER(u64)],
@@ -366,10 +369,27 @@ class Source:
         return cls(h, (levels, arr))
 
     @classmethod
-    def synthetic(cls, vc, params):
+    def synthetic(cls, vc, params, device=None):
+        """Seeded synthetic volume; with `device`, bricks are synthesised on
+        that GPU straight into the cache's staging buffer."""
         h = vp()
-        check(lib().gc_source_create_synthetic(C.byref(vc), C.byref(params), C.byref(h)))
+        if device is None:
+            check(lib().gc_source_create_synthetic(C.byref(vc), C.byref(params), C.byref(h)))
+        else:
+            check(lib().gc_source_create_synthetic_gpu(C.byref(vc), C.byref(params), int(device), C.byref(h)))
         return cls(h)
+
+    def get_device(self, vc, keys, threshold=0.0):
+        """Bricks (bx, by, bz, level) of a device source, synthesised on the
+        GPU: (n, bs, bs, bs) voxels and the per-brick empty flags."""
+        keys = np.ascontiguousarray(keys, np.uint32).reshape(-1, 4)
+        n, bs = len(keys), vc.brick_size[0]
+        d = DeviceBuffer(max(1, n * bs ** 3 * vc.encoded_bytes))
+        empty = np.zeros(max(1, n), np.uint8)
+        check(lib().gc_source_get_device(self.h, ptr(keys), n, d.p, float(threshold), ptr(empty)))
+        out = d.to_array(DT[vc.encoded_bytes], (n, bs, bs, bs))
+        d.free()
+        return out, empty[:n]
 
     @classmethod
     def gpu_mip(cls, vc, level0, device=0, memo_bytes=256 << 20):

@@ -153,6 +153,20 @@ typedef struct gc_synth_params {
 } gc_synth_params;
 int gc_source_create_synthetic(const gc_volume_cfg *vc, const gc_synth_params *p,
                                gc_source **out);
+/* The same seeded synthetic volume, synthesised on `device` when requested:
+ * a cache's request handler has the bricks written straight into its device
+ * staging buffer (no host arithmetic, no PCIe upload), bit-identical to
+ * gc_source_create_synthetic's bricks.  BigBrickSize 1; 3 x nb_blobs x
+ * BrickSize factors must fit 64 KiB.  gc_source_get still returns host bricks.
+ * No reference counterpart (the reference reads bricks from its LZ4 store). */
+int gc_source_create_synthetic_gpu(const gc_volume_cfg *vc, const gc_synth_params *p, int device,
+                                   gc_source **out);
+/* Device bricks of such a source (for tests / tools): n bricks, keys[4i..4i+3]
+ * = (bx, by, bz, level), written to d_dst (n x BrickSize^3 voxels) on the
+ * current device; empty[i] = 1 when every voxel of brick i is below
+ * threshold (the EmptyThreshold rule).  Synchronous. */
+int gc_source_get_device(gc_source *src, const uint32_t *keys, uint32_t n, void *d_dst,
+                         float threshold, uint8_t *empty);
 int gc_source_create_callback(const gc_volume_cfg *vc, gc_brick_fn fn, void *user,
                               gc_source **out);
 /* Mip on miss: level-0 bricks from `level0` (same brick size, covering and
