@@ -4,10 +4,11 @@
 // workgroup per requested brick writes the brick straight into device memory
 // (the cache's staging buffer), so a volume synthesised on demand costs no
 // host arithmetic and no PCIe upload.  Per voxel inside the level:
-//   sum  = sum over blobs k, in blob order, of (amp_k fz_k(z) * fy_k(y)) * fx_k(x)
+//   sum  = sum over blobs k, in blob order, of (amp_k fz_k(z) * fy_k(y)) * fx_k(x),
+//          one fused multiply-add per blob: sum = fma(w_k, fx_k(x), sum)
 //   value = clamp((float)(int)(sum + 0.5f) + noise(x, y, z), 0, max)
 // with the separable factors read from the host's per-level axis tables
-// (copied to the device once), each product and sum a single IEEE float
+// (copied to the device once), each product and fma a single IEEE float
 // operation (the library is built with -ffp-contract=off) and the same
 // integer noise hash, so every brick is bit-identical to the host's.  Blobs
 // that do not reach the brick have a zero factor on some axis and add +0,
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(256) void k_synth_bricks(SynthLaunch p)
             float c = 0.f;
             for (uint32_t k = 0; k < nb; ++k) {
                 const float w = fz[k * bs + z] * fy[k * bs + y];
-                c += w * fx[k * bs + x];
+                c = __builtin_fmaf(w, fx[k * bs + x], c);
             }
             int32_t nz = 0;
             if (p.noise) {

@@ -122,7 +122,8 @@ int Synth::noise_at(uint32_t level, int64_t x, int64_t y, int64_t z) const
 
 // Fill a box [o, o+n) of level-`level` voxels (x-fastest into dst, row pitch
 // n[0]); voxels outside the level extent `ext` are 0.  value = clamp(
-// round(sum of blobs) + noise), with the blob sum in float in blob order.
+// round(sum of blobs) + noise), with the blob sum in float in blob order, one
+// fused multiply-add per blob.
 void Synth::fill_box(uint32_t level, const int64_t o[3], const uint32_t n[3], const uint32_t ext[3],
                      void *dst) const
 {

@@ -1,15 +1,23 @@
 // synth_rows.cpp -- the voxel loop of Synth::fill_box (synth.cpp), host only.
 //
-// Per row: the blob sum in blob order, the noise, then rounding and
-// clamping.  Every operation is element-wise IEEE single precision (built
-// with -ffp-contract=off, so no fused multiply-add) or 32/64-bit integer
-// arithmetic, so the builds of this file for AVX-512, AVX2 and baseline
-// x86-64 (SYNTH_ROWS_NAME names each one) write the same bytes.  The
-// baseline build (SYNTH_ROWS_DISPATCH) also holds synth_rows(), which picks
-// the widest build the host CPU runs.
+// Per row: the blob sum in blob order (one fused multiply-add per blob:
+// sum = fma(w, fx, sum)), the noise, then rounding and clamping.  Every
+// operation is element-wise IEEE single precision, each fma rounded once
+// (built with -ffp-contract=off, so no other contraction), or 32/64-bit
+// integer arithmetic, so the builds of this file for AVX-512, AVX2 and
+// baseline x86-64 (SYNTH_ROWS_NAME names each one; the baseline one calls
+// the C library's fmaf) write the same bytes.  The baseline build
+// (SYNTH_ROWS_DISPATCH) also holds synth_rows(), which picks the widest build
+// the host CPU runs (GC_SYNTH_ISA=base|avx2|avx512 in the environment picks
+// one by name, for tests).
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#if SYNTH_VEC_BYTES >= 32
+#include <immintrin.h>
+#endif
 
 #include "synth.hpp"
 
@@ -63,11 +71,25 @@ static inline vf vload(const float *p)
     return v;
 }
 
+// c + w * f per element, rounded once
+static inline vf vfma(float w, vf f, vf c)
+{
+#if SYNTH_VEC_BYTES == 64
+    return (vf)_mm512_fmadd_ps(_mm512_set1_ps(w), (__m512)f, (__m512)c);
+#elif SYNTH_VEC_BYTES == 32
+    return (vf)_mm256_fmadd_ps(_mm256_set1_ps(w), (__m256)f, (__m256)c);
+#else
+    vf r;
+    for (int i = 0; i < VL; ++i) r[i] = std::fma(w, f[i], c[i]);
+    return r;
+#endif
+}
+
 // Blob sums of R consecutive rows (ky0 ..), voxels [xb, xb+32) of each, into
 // row[r * nx + xb ..]: 32 x R independent accumulators held in registers,
 // each summing its blobs in blob order.  Weights and factors are >= 0, so a
-// blob whose weight is 0 for a row adds +0 there, which leaves the sum (never
-// -0) unchanged: no blob needs to be skipped.
+// blob whose weight is 0 for a row gives fma(0, f, s) = s + 0 = s (s is never
+// -0): no blob needs to be skipped.
 template <int R>
 static inline void sum_rows32(const SynthRows &a, uint32_t ky0, uint32_t xb, float *__restrict row)
 {
@@ -80,7 +102,7 @@ static inline void sum_rows32(const SynthRows &a, uint32_t ky0, uint32_t xb, flo
         const float *w = &a.wz[k * ny + ky0];
         const float *f = &a.fx[k * nx + xb];
 #define GC_ACC(j)                                                          \
-        if constexpr ((j) < R * NV) c##j += w[(j) / NV] * vload(f + ((j) % NV) * VL);
+        if constexpr ((j) < R * NV) c##j = vfma(w[(j) / NV], vload(f + ((j) % NV) * VL), c##j);
         GC_ACC(0) GC_ACC(1) GC_ACC(2) GC_ACC(3) GC_ACC(4) GC_ACC(5) GC_ACC(6) GC_ACC(7)
 #undef GC_ACC
     }
@@ -130,7 +152,7 @@ static void rows_impl(const SynthRows &a)
                         const float w = a.wz[k * ny + ky0 + r];
                         if (w == 0.f) continue;
                         const float *__restrict f = &a.fx[k * nx];
-                        for (uint32_t i = xb; i < nx; ++i) rr[i] += w * f[i];
+                        for (uint32_t i = xb; i < nx; ++i) rr[i] = std::fma(w, f[i], rr[i]);
                     }
                 }
             }
@@ -185,10 +207,16 @@ void synth_rows(const SynthRows &a)
 {
     static const auto fn = []() -> void (*)(const SynthRows &) {
         __builtin_cpu_init();
+        if (const char *isa = std::getenv("GC_SYNTH_ISA")) {
+            if (!std::strcmp(isa, "base")) return synth_rows_base;
+            if (!std::strcmp(isa, "avx2") && __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma"))
+                return synth_rows_avx2;
+            if (!std::strcmp(isa, "avx512") && __builtin_cpu_supports("avx512f")) return synth_rows_avx512;
+        }
         if (__builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") &&
             __builtin_cpu_supports("avx512dq") && __builtin_cpu_supports("avx512vl"))
             return synth_rows_avx512;
-        if (__builtin_cpu_supports("avx2")) return synth_rows_avx2;
+        if (__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma")) return synth_rows_avx2;
         return synth_rows_base;
     }();
     fn(a);
