@@ -95,3 +95,39 @@ def test_server_results_alone_and_in_mix(gpu, plain, budget_vols):
         print("request", k, "vol", vol, (w, h), "mix vs alone max", e1, "frac>1", round(f1, 5),
               "| mix vs oracle max", e2, "frac>1", round(f2, 5), flush=True)
         assert ok1 and ok2, k
+
+
+def test_server_over_device_synthesised_volumes(gpu):
+    """Volumes whose bricks are synthesised on the GPU serve the same pixels
+    as the host-synthesised volume rendered alone."""
+    vols = [((72, 64, 56), 1), ((64, 56, 48), 2)]
+    params = [G.synth_params(4242 + i, 16, sigma=(4.0, 16.0), amp=(60.0, 255.0) if b == 1 else (15000.0, 65535.0),
+                             noise=8 if b == 1 else 2048, threads=2) for i, (_, b) in enumerate(vols)]
+    vcs = [G.volume_cfg(d, 8, 1, b) for d, b in vols]
+    pool_bytes = [2048 * 8 ** 3 * b for _, b in vols]
+    srv = G.Server(1 << 30, max_views=4, slot_pixels=128 * 96, precision=64)
+    srcs = [G.Source.synthetic(vc, p, device=0) for vc, p in zip(vcs, params)]
+    vids = [srv.add_volume(vc, src, pb, max_requests=64) for vc, src, pb in zip(vcs, srcs, pool_bytes)]
+    reqs = [(k % 2,) + r[1:] for k, r in enumerate(_requests(8, seed=11))]
+    ids = {}
+    print("server over device-synthesised volumes", flush=True)
+    for k, (vol, w, h, cam, tseed, astop) in enumerate(reqs):
+        view = G.make_view(w, h, cam, alpha_stop=astop, precision=64)
+        ids[srv.submit(vids[vol], view, tf_table(seed=tseed))] = k
+    srv.drain()
+    res = srv.poll(64)
+    assert len(res) == len(reqs)
+    got = {ids[r.id]: srv.take_pixels(r.id, r.width, r.height) for r in res}
+    srv.close()
+    for k, (vol, w, h, cam, tseed, astop) in enumerate(reqs):
+        tf = tf_table(seed=tseed)
+        d_tf = G.DeviceBuffer.from_array(tf)
+        cc = G.cache_cfg(G.pool_shape(pool_bytes[vol], 8 ** 3 * vols[vol][1]), 8, max_requests=64)
+        gc = G.Cache(vcs[vol], cc, G.Source.synthetic(vcs[vol], params[vol]))
+        f = G.Frame(w, h, 64)
+        rep = G.render_frame(gc, G.make_view(w, h, cam, alpha_stop=astop, precision=64), d_tf, len(tf), f)
+        assert rep.complete == 1
+        ok, e, fr = _within(got[k], f.pixels().reshape(-1))
+        print("request", k, "vol", vol, (w, h), "mix vs alone (host bricks) max", e, "frac>1", round(fr, 5), flush=True)
+        assert ok, k
+        gc.close()
