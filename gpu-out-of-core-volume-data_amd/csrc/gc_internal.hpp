@@ -197,6 +197,9 @@ struct DvrView {
     uint32_t *pixels;
     float *tbuf;
     void *cstate;                                  // float4 (fp32) or double4 (fp64)
+    float4 *rpos;                                  // exact continuation: per pixel the position (xyz) and
+                                                   // t (w) of the sample a stopped ray resumes at; null:
+                                                   // resume at pos = origin + dir * t (the reference)
 };
 
 struct DvrLaunch {
@@ -230,6 +233,7 @@ struct DvrLaunch {
     uint32_t tf_lds_tables;     // distinct tables among them
     uint32_t brick_size;        // the geometry's brick size and covering, for the host's choice of kernel
     uint32_t covering;
+    uint32_t exact_resume;      // 1: some view has rpos (exact continuation of stopped rays)
     uint32_t pipelined;         // 1: software-pipelined march (requires realDivVirtual == 1 on every
                                 //    level and MRPD y*z cells, x cells below 2^24; see launch_base)
 };

@@ -334,6 +334,62 @@ __device__ inline Acc4<ACC> classify_padded(ACC s, const float4 *TFp, uint32_t N
     return shade<ACC>(a, TFp[j], TFp[j + 1], kexp, skip);
 }
 
+// Texture coordinate of an integer box sum.  The sample is s = sum / D
+// (D = 8 * 255 for u8, 8 * 65535 for u16) and the coordinate xf = s * N - 0.5
+// (RayCasters.cu:163); xf + 1 = (2 sum N + D) / (2 D), so q = floor(xf) + 1
+// and the lerp weight frac(xf) follow from integer arithmetic, with one
+// rounding in the weight.  In FP32, forming s and xf as floats would round
+// xf to 2^-15 at N = 256, an error in the weight that the lerp carries into
+// every colour and alpha.  WIDE: 64-bit numerator (tables above 4095
+// entries); else 2 sum N + D < 2^32.
+template <int BYTES, bool WIDE>
+__device__ inline void tf_coord(uint32_t sum, uint32_t N, uint32_t &q, float &a)
+{
+    constexpr uint32_t D = BYTES == 1 ? 2040u : 524280u;
+    constexpr float inv2d = 1.0f / (float)(2u * D);
+    if constexpr (WIDE) {
+        const uint64_t num = 2ull * sum * N + D;
+        const uint64_t qq = num / (2u * D);
+        q = (uint32_t)qq;
+        a = (float)(uint32_t)(num - qq * (2u * D)) * inv2d;
+    } else {
+        const uint32_t num = 2u * sum * N + D;
+        q = num / (2u * D);
+        a = (float)(num - q * (2u * D)) * inv2d;
+    }
+}
+
+// Classification of an integer box sum: FP32 from the exact coordinate,
+// FP64 as the FP64 caster computes it (s = sum / D, then s * N - 0.5).
+// Padded LDS table (N <= DVR_TF_LDS_MAX).
+template <int BYTES, typename ACC>
+__device__ inline Acc4<ACC> classify_sum_padded(uint32_t sum, const float4 *TFp, uint32_t N, float kexp, bool &skip)
+{
+    if constexpr (sizeof(ACC) == 8) {
+        return classify_padded<double>((double)sum / (BYTES == 1 ? 2040.0 : 524280.0), TFp, N, kexp, skip);
+    } else {
+        uint32_t q;
+        float a;
+        tf_coord<BYTES, false>(sum, N, q, a);
+        return shade<float>(a, TFp[q], TFp[q + 1], kexp, skip);
+    }
+}
+// Table in HBM (any N): entries clamp(q - 1) and clamp(q).
+template <int BYTES, typename ACC, typename TFP>
+__device__ inline Acc4<ACC> classify_sum(uint32_t sum, TFP TF, uint32_t N, float kexp, bool &skip)
+{
+    if constexpr (sizeof(ACC) == 8) {
+        return classify<double>((double)sum / (BYTES == 1 ? 2040.0 : 524280.0), TF, N, kexp, skip);
+    } else {
+        uint32_t q;
+        float a;
+        tf_coord<BYTES, true>(sum, N, q, a);
+        const uint32_t i0 = q == 0 ? 0u : (q - 1u > N - 1u ? N - 1u : q - 1u);
+        const uint32_t i1 = q > N - 1u ? N - 1u : q;
+        return shade<float>(a, TF[i0], TF[i1], kexp, skip);
+    }
+}
+
 // Next packet for a wave on XCD `xcd`: its own contiguous share first, then
 // the other XCDs' shares.  NONE when every queue is drained.
 __device__ inline uint32_t fetch_packet(const DvrLaunch &p, uint32_t n_packets, uint32_t xcd)
@@ -503,7 +559,6 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // byte offset of texel (cov-1, cov-1, cov-1) inside a brick
     const uint32_t corner = cov >= 1 ? (cov - 1) * (plane + row + BYTES) : 0u;
     const bool fast_fetch = BYTES != 4 && cov >= 1;
-    const ACC inv_norm = BYTES == 1 ? (ACC)(1.0 / 2040.0) : (ACC)(1.0 / 524280.0);
     const float tstep = p.tstep;
     const uint32_t ts = __builtin_amdgcn_readfirstlane(p.ts_dev ? *p.ts_dev : p.ts);
     const int march = (int)p.march;
@@ -524,7 +579,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     F3 ior = f3(0, 0, 0), rdv = f3(0, 0, 0), ldf = f3(0, 0, 0), vvf = f3(0, 0, 0);
     // current brick: page-table cell, entry state, slot and base address
     uint32_t pcx = NONE, pcy = NONE, pcz = NONE, ew = 0, slot = 0, vkey = NONE;
-    ACC vs = 0;
+    uint32_t vsum = 0;   // box sum of the voxel vkey (non-pipelined march)
     const uint8_t *bbase = pool;
     Acc4<ACC> fc{0, 0, 0, 0};
     uint32_t n_samples = 0, n_visits = 0;      // per lane
@@ -560,6 +615,10 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     bool hb = false, hc = false, newray = false;
     uint32_t cellB = 0, viB = 0, cellC = 0, slotC = 0, ewC = 0, viC = 0, sum_prev = 0;
     float kB = 0.f, tC = 0.f, kC = 0.f;
+    // position of sample C, kept (in LDS, off the register budget) only when
+    // a view of the launch asks for exact continuation
+    __shared__ F3 pos_c[LANES];
+    const bool keep_pos = p.exact_resume != 0;
     bool freshB = false, freshC = false, reuseC = false;
     uint32_t eX = 0u, eY = 0u;   // cell-table words in flight
     Quad wX{0u, 0u, 0u, 0u}, wY = wX;
@@ -571,9 +630,10 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     };
     // the progressive state and the pixel of a ray that ends or stops
     // (RayCasters.cu:138-147,189-197)
-    auto store_state = [&](float tt) {
+    auto store_state = [&](float tt, F3 pp) {
         const DvrView &V = vw[v];
         gp(V.tbuf)[pix] = tt;
+        if (V.rpos) gp(reinterpret_cast<gf4 *>(V.rpos))[pix] = gf4{pp.x, pp.y, pp.z, tt};
         if constexpr (sizeof(ACC) == 8)
             gp(reinterpret_cast<gd4 *>(V.cstate))[pix] = gd4{fc.x, fc.y, fc.z, fc.w};
         else
@@ -582,12 +642,21 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     };
     // Classification and front-to-back blending of one mapped sample
     // (RayCasters.cu:161-178); composite_if blends only where `on`.
-    auto composite_if = [&](bool on, ACC s, float kx) {
+    // Integer voxels are classified from their box sum (exact texture
+    // coordinate), float voxels from the sample value s.
+    auto classify_any = [&](uint32_t sum, ACC s, float kx, bool &cskip) -> Acc4<ACC> {
+        if constexpr (BYTES != 4) {
+            return tf_lds ? (MT ? classify_sum_padded<BYTES, ACC>(sum, tfl + (tfw & 0xffffu), tfw >> 16, kx, cskip)
+                                : classify_sum_padded<BYTES, ACC>(sum, tfl, tf_n1, kx, cskip))
+                          : classify_sum<BYTES, ACC>(sum, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size,
+                                                     kx, cskip);
+        } else {
+            return classify<ACC>(s, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size, kx, cskip);
+        }
+    };
+    auto composite_if = [&](bool on, uint32_t sum, float kx) {
         bool cskip;
-        const Acc4<ACC> cc = tf_lds ? (MT ? classify_padded<ACC>(s, tfl + (tfw & 0xffffu), tfw >> 16, kx, cskip)
-                                          : classify_padded<ACC>(s, tfl, tf_n1, kx, cskip))
-                                    : classify<ACC>(s, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size,
-                                                    kx, cskip);
+        const Acc4<ACC> cc = classify_any(sum, (ACC)0, kx, cskip);
         const bool b = on && !cskip;
         const ACC wgt = (ACC)1 - fc.w;
         Acc4<ACC> n;
@@ -601,12 +670,9 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         fc.z = b ? n.z : fc.z;
         fc.w = b ? n.w : fc.w;
     };
-    auto composite = [&](ACC s, float kx) {
+    auto composite = [&](uint32_t sum, ACC s, float kx) {
         bool cskip;
-        const Acc4<ACC> cc = tf_lds ? (MT ? classify_padded<ACC>(s, tfl + (tfw & 0xffffu), tfw >> 16, kx, cskip)
-                                          : classify_padded<ACC>(s, tfl, tf_n1, kx, cskip))
-                                    : classify<ACC>(s, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size,
-                                                    kx, cskip);
+        const Acc4<ACC> cc = classify_any(sum, s, kx, cskip);
         if (!cskip) {
             const ACC wgt = (ACC)1 - fc.w;
             if constexpr (sizeof(ACC) == 8) {
@@ -683,7 +749,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     tfar = R.tfar;
                     const bool hit = R.hit;
                     pix = y * W + x;
-                    tnear = fmaxf(tnear, gp(V.tbuf)[pix]);
+                    const float tb = gp(V.tbuf)[pix];
+                    tnear = fmaxf(tnear, tb);
                     if constexpr (sizeof(ACC) == 8) {
                         const gd4 c = gp(reinterpret_cast<const gd4 *>(V.cstate))[pix];
                         fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
@@ -699,6 +766,11 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                         if (tnear < 0.f) tnear = 0.f;
                         t = tnear;
                         pos = origin + dir * tnear;
+                        if (V.rpos && tb > 0.f && tb == tnear) {
+                            // exact continuation of a stopped ray
+                            const gf4 q = gp(reinterpret_cast<const gf4 *>(V.rpos))[pix];
+                            pos = f3(q.x, q.y, q.z);
+                        }
                         const float epx = fabsf(dot(dir - dxn, dleft) * tfar);
                         const float epy = fabsf(dot(dir - dyn, dup) * tfar);
                         astop = V.alpha_stop;
@@ -810,13 +882,10 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     // valid sample value); only consumed samples blend
                     const uint32_t sum = reuseC ? sum_prev : box_add<BYTES>(w_old);
                     sum_prev = cons ? sum : sum_prev;
-                    ACC sv;
-                    if constexpr (sizeof(ACC) == 8) sv = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
-                    else sv = (float)sum * inv_norm;
-                    composite_if(cons, sv, kC);
+                    composite_if(cons, sum, kC);
                     n_samples += cons ? 1u : 0u;
                     const bool end = (hc && !live) || unmapped;
-                    if (end) store_state(tC);
+                    if (end) store_state(tC, keep_pos ? pos_c[tid] : pos_b);
                     // w_old is waited for on every path here (the main path
                     // already has), so its registers are free when the next
                     // half issues loads into them
@@ -827,6 +896,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     slotC = hb ? slotB : slotC;
                     ewC = hb ? ewB : ewC;
                     tC = hb ? t_b : tC;
+                    if (keep_pos && hb) pos_c[tid] = pos_b;
                     kC = hb ? kB : kC;
                     freshC = hb ? freshB : freshC;
                     viC = hb ? viB : viC;
@@ -906,17 +976,16 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     ly = vy % inner;
                     lz = vz % inner;
                 }
-                ACC s;
+                ACC s = 0;
+                uint32_t isum = 0;
                 if (fast_fetch) {
                     const uint32_t vi = __umul24(__umul24(lz, bs) + ly, bs) + lx;   // voxel in the brick
                     const uint32_t key = vi + (lod << 24);
                     if (key != vkey) {
                         vkey = key;
-                        const uint32_t sum = box_sum<BYTES == 4 ? 1 : BYTES>(bbase + vi * BYTES, row, plane);
-                        if constexpr (sizeof(ACC) == 8) vs = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
-                        else vs = (float)sum * inv_norm;
+                        vsum = box_sum<BYTES == 4 ? 1 : BYTES>(bbase + vi * BYTES, row, plane);
                     }
-                    s = vs;
+                    isum = vsum;
                 } else {
                     uint32_t o[3];
                     slot_origin(*g, slot, o);
@@ -934,17 +1003,16 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                         if constexpr (sizeof(ACC) == 8) s = dsum / 8.0;
                         else s = fsum / 8.0f;
                     } else {
-                        if constexpr (sizeof(ACC) == 8) s = dsum / (BYTES == 1 ? 2040.0 : 524280.0);
-                        else s = (float)dsum * inv_norm;
+                        isum = (uint32_t)dsum;   // a sum of 8 integer voxels, exact
                     }
                 }
-                composite(s, kexp);
+                composite(isum, s, kexp);
                 ++n_samples;
                 t += step;
                 pos = pos + dstep;
             }
             if (done || stopped) {
-                store_state(t);
+                store_state(t, pos);
                 have = false;
             }
         }
@@ -1050,7 +1118,6 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     const uint32_t row = bs * BYTES, plane = bs * bs * BYTES;
     const uint8_t *pool = static_cast<const uint8_t *>(p.pool);
     const uint32_t corner = (cov - 1) * (plane + row + BYTES);
-    const ACC inv_norm = BYTES == 1 ? (ACC)(1.0 / 2040.0) : (ACC)(1.0 / 524280.0);
     const float tstep = p.tstep;
     const uint32_t ts = __builtin_amdgcn_readfirstlane(p.ts_dev ? *p.ts_dev : p.ts);
 
@@ -1097,7 +1164,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 pix = e & ((1u << DVR_PIXEL_BITS) - 1u);
                 const DvrView &V = vw[v];
                 const RaySetup Rs = setup_ray(V, pix % V.width, pix / V.width);
-                float tnear = fmaxf(Rs.tnear, gp(V.tbuf)[pix]);
+                const float tb = gp(V.tbuf)[pix];
+                float tnear = fmaxf(Rs.tnear, tb);
                 if constexpr (sizeof(ACC) == 8) {
                     const gd4 c = gp(reinterpret_cast<const gd4 *>(V.cstate))[pix];
                     fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
@@ -1116,6 +1184,11 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     astop = V.alpha_stop;
                     t = tnear;
                     pos = Rs.origin + dir * tnear;
+                    if (V.rpos && tb > 0.f && tb == tnear) {
+                        // exact continuation of a stopped ray
+                        const gf4 q = gp(reinterpret_cast<const gf4 *>(V.rpos))[pix];
+                        pos = f3(q.x, q.y, q.z);
+                    }
                     // level-switch distances, one level per lane; the level
                     // at t is the last one already reached (compute_LOD,
                     // RayCasters.cu:551-567)
@@ -1167,14 +1240,11 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         const bool fresh = cell != (gl == 0 ? cell_last : prev);
         const uint64_t off = (active && ew == 1) ? (uint64_t)slot * bbytes + corner + vi * (uint32_t)BYTES : 0ull;
         const uint32_t sum = box_add<BYTES>(box_load<BYTES>(pool + off, row, plane));
-        ACC sv;
-        if constexpr (sizeof(ACC) == 8) sv = (double)sum / (BYTES == 1 ? 2040.0 : 524280.0);
-        else sv = (float)sum * inv_norm;
         bool skip;
         const DvrView &V = vw[v];
-        const Acc4<ACC> cc = tf_lds ? classify_padded<ACC>(sv, tfl + (V.tf_lds_off & ~DVR_TF_OWNER), V.tf_size, L.kexp, skip)
-                                    : classify<ACC>(sv, gp(reinterpret_cast<const gf4 *>(V.tf)), V.tf_size, L.kexp,
-                                                    skip);
+        const Acc4<ACC> cc =
+            tf_lds ? classify_sum_padded<BYTES, ACC>(sum, tfl + (V.tf_lds_off & ~DVR_TF_OWNER), V.tf_size, L.kexp, skip)
+                   : classify_sum<BYTES, ACC>(sum, gp(reinterpret_cast<const gf4 *>(V.tf)), V.tf_size, L.kexp, skip);
         const uint64_t m_fresh = gbits(__ballot(fresh)), m_skip = gbits(__ballot(skip));
         const uint32_t u = first_clear(gbits(__ballot(ew == 1)));   // first sample not mapped
         const uint32_t f = first_clear(gbits(__ballot(my_t < tfar)));   // first sample with t >= tfar
@@ -1226,6 +1296,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
             auto store_state = [&](float tt) {
                 if (gl == 0) {
                     gp(V.tbuf)[pix] = tt;
+                    if (V.rpos) gp(reinterpret_cast<gf4 *>(V.rpos))[pix] = gf4{px_k, py_k, pz_k, tt};
                     if constexpr (sizeof(ACC) == 8)
                         gp(reinterpret_cast<gd4 *>(V.cstate))[pix] = gd4{fc.x, fc.y, fc.z, fc.w};
                     else

@@ -1035,6 +1035,7 @@ struct Frame {
     uint32_t *pixels = nullptr;
     float *tbuf = nullptr;
     void *cstate = nullptr;
+    float4 *rpos = nullptr;                    // resume positions (views with resume = 1)
     // launch scratch: [0..7] per-XCD packet cursors, [8], [9] ray-list counts
     uint32_t *ctl = nullptr;
     uint32_t *lists[2] = {nullptr, nullptr};   // rays stopped at a miss
@@ -1044,6 +1045,7 @@ struct Frame {
         hipFree(pixels);
         hipFree(tbuf);
         hipFree(cstate);
+        hipFree(rpos);
         hipFree(ctl);
         hipFree(lists[0]);
         hipFree(lists[1]);
@@ -1100,6 +1102,7 @@ static void view_of(const gc_view &v, const float *d_tf, uint32_t tf_size, const
     out.pixels = F.pixels;
     out.tbuf = F.tbuf;
     out.cstate = F.cstate;
+    out.rpos = v.resume ? F.rpos : nullptr;
 }
 
 static uint32_t packets_of(const gc_view &v)
@@ -1123,6 +1126,7 @@ struct Batch {
     uint32_t *pixels = nullptr;
     float *tbuf = nullptr;
     void *cstate = nullptr;
+    float4 *rpos = nullptr;             // resume positions (views with resume = 1)
     uint32_t *lists[2] = {nullptr, nullptr};
     // [0..7] per-XCD packet cursors, [8], [9] ray-list counts, [10] timestamp,
     // [16..16+max_views) rays stopped per slot in the last launch
@@ -1155,6 +1159,7 @@ struct Batch {
         hipFree(pixels);
         hipFree(tbuf);
         hipFree(cstate);
+        hipFree(rpos);
         hipFree(lists[0]);
         hipFree(lists[1]);
         hipFree(ctl);
@@ -1192,6 +1197,7 @@ static int batch_place(Batch &B, uint32_t slot, const gc_view &v, const float *d
     D.pixels = B.pixels + off;
     D.tbuf = B.tbuf + off;
     D.cstate = static_cast<uint8_t *>(B.cstate) + off * B.cbytes_per_pixel();
+    D.rpos = v.resume ? B.rpos + off : nullptr;
     B.table = std::max(B.table, slot + 1);
     // a starting view begins from an empty progressive state
     hipStream_t s = B.C->stream;
@@ -1244,6 +1250,8 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
     L.refill_min = B.refill_min;
     L.march = B.march;
     L.view_stopped = B.ctl + 16;
+    for (uint32_t i = 0; i < B.table; ++i)
+        if (B.slots[i].used && B.slots[i].v.rpos) L.exact_resume = 1;
     const int out = has_list ? 1 - B.cur : (append ? B.cur : 0);
     if (has_list) {
         L.list = B.lists[B.cur];
@@ -1894,6 +1902,7 @@ int gc_frame_create(uint32_t width, uint32_t height, uint32_t precision, gc_fram
     GC_TRY(dalloc(&F.pixels, (size_t)width * height * 4));
     GC_TRY(dalloc(&F.tbuf, (size_t)width * height * 4));
     GC_TRY(dalloc(&F.cstate, F.cbytes()));
+    GC_TRY(dalloc(&F.rpos, (size_t)width * height * sizeof(float4)));
     GC_TRY(dalloc(&F.ctl, 64));
     GC_TRY(dalloc(&F.lists[0], (size_t)width * height * 4));
     GC_TRY(dalloc(&F.lists[1], (size_t)width * height * 4));
@@ -1922,8 +1931,9 @@ int gc_frame_reset(gc_frame *f, void *stream)
         GC_HIP(hipDeviceSynchronize());
         return GC_OK;
     }
-    GC_HIP(hipMemsetAsync(F.tbuf, 0, (size_t)F.w * F.h * 4, (hipStream_t)stream));
-    GC_HIP(hipMemsetAsync(F.cstate, 0, F.cbytes(), (hipStream_t)stream));
+    // fill kernels rather than memsets: a caller may capture this stream
+    GC_HIP(launch_fill32(reinterpret_cast<uint32_t *>(F.tbuf), F.w * F.h, 0, (hipStream_t)stream));
+    GC_HIP(launch_fill32(static_cast<uint32_t *>(F.cstate), (uint32_t)(F.cbytes() / 4), 0, (hipStream_t)stream));
     return GC_OK;
 }
 
@@ -1965,6 +1975,7 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
     DvrLaunch L{};
     launch_base(C, L);
     view_of(*view, d_tf, tf_size, F, L.view0);
+    L.exact_resume = L.view0.rpos ? 1u : 0u;
     L.tf_lds_entries = dvr_assign_tf_lds(&L.view0, 1, C.vc.encoded_bytes, &L.tf_lds_tables);
     L.n_views = 1;
     L.n_packets = packets_of(*view);
@@ -1994,6 +2005,7 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
     DvrLaunch L{};
     launch_base(C, L);
     view_of(view, d_tf, tf_size, F, L.view0);
+    L.exact_resume = L.view0.rpos ? 1u : 0u;
     L.tf_lds_entries = dvr_assign_tf_lds(&L.view0, 1, C.vc.encoded_bytes, &L.tf_lds_tables);
     L.n_views = 1;
     L.n_packets = first ? packets_of(view) : 0;
@@ -2141,6 +2153,7 @@ int gc_batch_create(gc_cache *c, uint32_t max_views, uint64_t max_pixels, uint32
     GC_TRY(dalloc(&B.pixels, max_pixels * 4));
     GC_TRY(dalloc(&B.tbuf, max_pixels * 4));
     GC_TRY(dalloc(&B.cstate, max_pixels * B.cbytes_per_pixel()));
+    GC_TRY(dalloc(&B.rpos, max_pixels * sizeof(float4)));
     GC_TRY(dalloc(&B.lists[0], max_pixels * 4));
     GC_TRY(dalloc(&B.lists[1], max_pixels * 4));
     GC_TRY(dalloc(&B.ctl, (16 + DVR_MAX_VIEWS) * 4));

@@ -89,7 +89,9 @@ struct Volume {
 struct Server {
     gc_server_cfg cfg{};
     int device = 0;
-    std::vector<Volume> vols;
+    // volumes live at stable addresses: the scheduler works on them without
+    // the lock while gc_server_add_volume may append to the list
+    std::vector<std::unique_ptr<Volume>> vols;
     std::mutex m;
     std::condition_variable cv, done_cv;
     std::thread worker;
@@ -105,8 +107,8 @@ struct Server {
     double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
     uint64_t slot_bytes() const
     {
-        // pixels + t + colour state + two ray-list entries per pixel
-        return (uint64_t)cfg.max_views * cfg.slot_pixels * (4 + 4 + (cfg.precision == 64 ? 32 : 16) + 8);
+        // pixels + t + colour state + resume position + two ray-list entries per pixel
+        return (uint64_t)cfg.max_views * cfg.slot_pixels * (4 + 4 + (cfg.precision == 64 ? 32 : 16) + 16 + 8);
     }
     struct Place {
         Volume *v;
@@ -115,9 +117,9 @@ struct Server {
     };
     void loop();
     int iterate(std::unique_lock<std::mutex> &lk);
-    int device_phase(const std::vector<Volume *> &evict, const std::vector<Volume *> &create,
-                     const std::vector<Place> &place, std::vector<Request *> &done, gc_server_stats &d,
-                     bool &progressed);
+    int device_phase(const std::vector<Volume *> &all, const std::vector<Volume *> &evict,
+                     const std::vector<Volume *> &create, const std::vector<Place> &place,
+                     std::vector<Request *> &done, gc_server_stats &d, bool &progressed);
     bool progressed = false;
     int make_resident(Volume &v);
     void release(Volume &v);
@@ -168,16 +170,17 @@ int Server::make_resident(Volume &v)
 // per volume, without waiting for the others: a finished launch is followed
 // by its cache update and the retirement of completed views; a volume with
 // no launch in flight takes its new views and launches again.
-int Server::device_phase(const std::vector<Volume *> &evict, const std::vector<Volume *> &create,
-                         const std::vector<Place> &place, std::vector<Request *> &done, gc_server_stats &d,
-                         bool &progressed)
+int Server::device_phase(const std::vector<Volume *> &all, const std::vector<Volume *> &evict,
+                         const std::vector<Volume *> &create, const std::vector<Place> &place,
+                         std::vector<Request *> &done, gc_server_stats &d, bool &progressed)
 {
     progressed = !evict.empty() || !create.empty();
     for (Volume *v : evict) release(*v);
     for (Volume *v : create) GC_TRY(make_resident(*v));
     for (const Place &p : place) p.v->pending.emplace_back(p.slot, p.r);
     std::vector<uint32_t> pend(cfg.max_views);
-    for (auto &v : vols) {
+    for (Volume *vp : all) {
+        Volume &v = *vp;
         if (!v.resident()) continue;
         if (v.in_flight) {
             const hipError_t q = hipEventQuery(v.done);
@@ -271,18 +274,27 @@ int Server::iterate(std::unique_lock<std::mutex> &lk)
     std::vector<Volume *> evict, create;
     std::vector<Place> place;
     uint64_t projected = resident_bytes;
-    for (auto &v : vols) {
+    std::vector<Volume *> all;
+    for (auto &vp : vols) all.push_back(vp.get());
+    for (Volume *vp : all) {
+        Volume &v = *vp;
         if (v.queue.empty()) continue;
         if (cfg.plain && in_flight > 0) break;
-        const bool now_resident = v.resident() || std::find(create.begin(), create.end(), &v) != create.end();
+        // a volume picked for eviction in this pass is released before any
+        // placement takes effect, so it counts as not resident here
+        const bool evicted = std::find(evict.begin(), evict.end(), &v) != evict.end();
+        const bool now_resident = (v.resident() && !evicted) ||
+                                  std::find(create.begin(), create.end(), &v) != create.end();
         if (!now_resident) {
             while (projected + v.footprint > cfg.pool_budget) {
+                // victims: resident, nothing in flight and nothing queued (a
+                // volume with queued requests is about to be used again)
                 Volume *victim = nullptr;
-                for (auto &u : vols)
-                    if (&u != &v && u.resident() && !u.busy_slots() &&
-                        std::find(evict.begin(), evict.end(), &u) == evict.end() &&
-                        (!victim || u.last_use < victim->last_use))
-                        victim = &u;
+                for (Volume *u : all)
+                    if (u != &v && u->resident() && !u->busy_slots() && u->queue.empty() &&
+                        std::find(evict.begin(), evict.end(), u) == evict.end() &&
+                        (!victim || u->last_use < victim->last_use))
+                        victim = u;
                 if (!victim) break;
                 evict.push_back(victim);
                 projected -= victim->footprint;
@@ -313,7 +325,7 @@ int Server::iterate(std::unique_lock<std::mutex> &lk)
     lk.unlock();
     std::vector<Request *> done;
     gc_server_stats d{};
-    const int rc = device_phase(evict, create, place, done, d, progressed);
+    const int rc = device_phase(all, evict, create, place, done, d, progressed);
     lk.lock();
     st.samples += d.samples;
     st.ray_starts += d.ray_starts;
@@ -344,7 +356,7 @@ void Server::loop()
         cv.wait(lk, [&] {
             if (stop) return true;
             for (auto &v : vols)
-                if (!v.queue.empty() || v.busy_slots()) return true;
+                if (!v->queue.empty() || v->busy_slots()) return true;
             return false;
         });
         if (stop) break;
@@ -402,7 +414,7 @@ int gc_server_destroy(gc_server *s)
     if (S.worker.joinable()) S.worker.join();
     hipSetDevice(S.device);
     for (auto &v : S.vols)
-        if (v.resident()) S.release(v);
+        if (v->resident()) S.release(*v);
     for (auto &kv : S.reqs) {
         if (kv.second->d_tf) gc_free(kv.second->d_tf);
         delete kv.second;
@@ -421,16 +433,16 @@ int gc_server_add_volume(gc_server *s, const gc_volume_cfg *vc, gc_source *src, 
     if (pool_bytes < brick_bytes) return fail(GC_ERR_ARG, "pool smaller than one brick");
     if (max_requests == 0) return fail(GC_ERR_ARG, "max_requests must be positive");
     std::lock_guard<std::mutex> lk(S.m);
-    Volume v;
-    v.vc = *vc;
-    v.src = src;
-    v.pool_bytes = pool_bytes;
-    v.max_requests = max_requests;
-    v.footprint = pool_bytes + S.slot_bytes();
-    if (v.footprint > S.cfg.pool_budget) return fail(GC_ERR_ARG, "volume's pool and view slots exceed the budget");
-    v.slots.assign(S.cfg.max_views, nullptr);
-    v.fresh.assign(S.cfg.max_views, false);
-    v.unplaced.assign(S.cfg.max_views, false);
+    auto v = std::make_unique<Volume>();
+    v->vc = *vc;
+    v->src = src;
+    v->pool_bytes = pool_bytes;
+    v->max_requests = max_requests;
+    v->footprint = pool_bytes + S.slot_bytes();
+    if (v->footprint > S.cfg.pool_budget) return fail(GC_ERR_ARG, "volume's pool and view slots exceed the budget");
+    v->slots.assign(S.cfg.max_views, nullptr);
+    v->fresh.assign(S.cfg.max_views, false);
+    v->unplaced.assign(S.cfg.max_views, false);
     S.vols.push_back(std::move(v));
     *volume = (uint32_t)(S.vols.size() - 1);
     return GC_OK;
@@ -468,7 +480,7 @@ int gc_server_submit(gc_server *s, uint32_t volume, const gc_view *view, const f
         r->volume = volume;
         r->t_submit = S.now_ms();
         *id = r->id;
-        S.vols[volume].queue.push_back(r.get());
+        S.vols[volume]->queue.push_back(r.get());
         S.reqs[r->id] = r.get();
         ++S.st.submitted;
         r.release();

@@ -67,6 +67,7 @@ class View(C.Structure):
     _fields_ = [
         ("width", u32), ("height", u32), ("inv_mv", f32 * 12), ("bbox_min", f32 * 3), ("bbox_max", f32 * 3),
         ("alpha_stop", f32), ("precision", u32), ("tile", u32), ("refill_min", u32), ("march", u32),
+        ("resume", u32),
     ]
 
 
@@ -551,7 +552,7 @@ class Frame:
 
 
 def make_view(width, height, inv_mv, alpha_stop=0.95, precision=32, tile=0,
-              bbox=((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)), refill_min=0, march=0):
+              bbox=((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)), refill_min=0, march=0, resume=0):
     v = View()
     v.width, v.height = width, height
     v.inv_mv[:] = [float(x) for x in np.asarray(inv_mv, np.float32).reshape(12)]
@@ -562,6 +563,7 @@ def make_view(width, height, inv_mv, alpha_stop=0.95, precision=32, tile=0,
     v.tile = tile
     v.refill_min = refill_min
     v.march = march
+    v.resume = resume
     return v
 
 

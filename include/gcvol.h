@@ -296,6 +296,15 @@ typedef struct gc_view {
                                     idle (0: default 16)                      */
     uint32_t march;              /* samples per lane between refill checks,
                                     1..8 (0: default 8)                       */
+    uint32_t resume;             /* how a ray stopped at a missing brick resumes
+                                    in the next launch.  0: at t from the t
+                                    buffer with pos = origin + dir * t
+                                    (RayCasters.cu:85-120), so a frame rendered
+                                    in several launches can differ from a one-
+                                    pass render where rays resume.  1: exact
+                                    continuation at the stored sample position
+                                    (no reference counterpart), so every launch
+                                    sequence gives the one-pass render.       */
 } gc_view;
 
 /* Per-view progressive state (tBuffer, colorStateBuffer, pixel buffer) on the

@@ -613,6 +613,7 @@ typedef struct {
     const or_view *v;
     float *tbuf;
     double *cstate;
+    float *rpos;
     uint32_t *pixels;
     int precision;
     uint32_t next_row, end_row;
@@ -731,7 +732,8 @@ static void dvr_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
     const int hit = tfar > tnear;
 
     const uint64_t i = (uint64_t)y * W + x;
-    tnear = fmaxf(tnear, job->tbuf[i]);
+    const float tb = job->tbuf[i];
+    tnear = fmaxf(tnear, tb);
     int unmapped = 0;
     double fc[4];
     float ff[4];
@@ -747,6 +749,9 @@ static void dvr_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
         if (tnear < 0.f) tnear = 0.f;
         float t = tnear;
         f3 pos = add(origin, muls(dir, tnear));
+        /* exact continuation (resume = 1, no reference counterpart): a ray
+         * stopped at a missing brick resumes at the stored sample position */
+        if (job->rpos && tb > 0.f && tb == t) pos = mk(job->rpos[4 * i], job->rpos[4 * i + 1], job->rpos[4 * i + 2]);
         const float epx = fabsf(dot3(sub(dir, dxn), dleft) * tfar);
         const float epy = fabsf(dot3(sub(dir, dyn), dup) * tfar);
         uint32_t prev[3] = {UINT32_MAX, UINT32_MAX, UINT32_MAX};
@@ -800,6 +805,10 @@ static void dvr_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
                 job->tbuf[i] = t;
                 if (p64) memcpy(job->cstate + 4 * i, fc, sizeof(fc));
                 else for (int k = 0; k < 4; ++k) job->cstate[4 * i + k] = ff[k];
+                if (job->rpos) {
+                    job->rpos[4 * i] = pos.x; job->rpos[4 * i + 1] = pos.y; job->rpos[4 * i + 2] = pos.z;
+                    job->rpos[4 * i + 3] = t;
+                }
                 unmapped = 1;
                 st[3] += 1;
                 break;
@@ -873,6 +882,10 @@ static void dvr_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
             pos = add(pos, muls(dir, V->lod_step[lod]));
         }
         if (!unmapped) {
+            if (job->rpos) {
+                job->rpos[4 * i] = pos.x; job->rpos[4 * i + 1] = pos.y; job->rpos[4 * i + 2] = pos.z;
+                job->rpos[4 * i + 3] = t;
+            }
             job->tbuf[i] = t;
             if (p64) memcpy(job->cstate + 4 * i, fc, sizeof(fc));
             else for (int k = 0; k < 4; ++k) job->cstate[4 * i + k] = ff[k];
@@ -908,9 +921,16 @@ void or_render_dvr_rows(or_cache *c, const or_view *v, float *tbuf, double *csta
                         uint32_t *pixels, int precision, int nthreads, uint32_t y0, uint32_t rows,
                         uint64_t stats[4])
 {
+    or_render_dvr_resume(c, v, tbuf, cstate, NULL, pixels, precision, nthreads, y0, rows, stats);
+}
+
+void or_render_dvr_resume(or_cache *c, const or_view *v, float *tbuf, double *cstate, float *rpos,
+                          uint32_t *pixels, int precision, int nthreads, uint32_t y0, uint32_t rows,
+                          uint64_t stats[4])
+{
     dvr_job job;
     memset(&job, 0, sizeof(job));
-    job.c = c; job.v = v; job.tbuf = tbuf; job.cstate = cstate; job.pixels = pixels;
+    job.c = c; job.v = v; job.tbuf = tbuf; job.cstate = cstate; job.rpos = rpos; job.pixels = pixels;
     job.precision = precision;
     job.next_row = y0;
     job.end_row = y0 + rows > v->height ? v->height : y0 + rows;

@@ -184,6 +184,14 @@ void or_render_dvr(or_cache *c, const or_view *v, float *tbuf, double *cstate,
 void or_render_dvr_rows(or_cache *c, const or_view *v, float *tbuf, double *cstate,
                         uint32_t *pixels, int precision, int nthreads, uint32_t y0, uint32_t rows,
                         uint64_t stats[4]);
+/* The same with exact continuation (the product's resume = 1 mode, which has
+ * no reference counterpart): rpos (W*H*4 floats) receives the position and t
+ * of the sample each ray stores its state at, and a ray whose t buffer holds
+ * t > 0 resumes at that stored position instead of origin + dir * t.
+ * rpos == NULL is or_render_dvr_rows. */
+void or_render_dvr_resume(or_cache *c, const or_view *v, float *tbuf, double *cstate, float *rpos,
+                          uint32_t *pixels, int precision, int nthreads, uint32_t y0, uint32_t rows,
+                          uint64_t stats[4]);
 
 #ifdef __cplusplus
 }

@@ -3,9 +3,11 @@
 GPU tests need libgcvol.so built in-tree (python3 __graft_entry__.py build)
 and a visible HIP device; they fail loudly if the library is missing.
 """
+import json
 import os
 import sys
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -36,3 +38,29 @@ def gpu(gcv):
     if gcv.device_count() < 1:
         pytest.fail("no HIP device visible to a gpu-marked test")
     return gcv
+
+
+# RGBA error records of the GPU parity tests (max error, RMSE per test and
+# precision), written as JSON to $GC_PARITY_LOG at the end of the session.
+_PARITY = []
+
+
+@pytest.fixture
+def parity_log(request):
+    def record(precision, err, **extra):
+        err = np.asarray(err)
+        d = {"test": request.node.nodeid, "precision": int(precision), "channels": int(err.size),
+             "max_err": int(err.max()) if err.size else 0,
+             "rmse": float(np.sqrt((err.astype(np.float64) ** 2).mean())) if err.size else 0.0,
+             "frac_gt1": float((err > 1).mean()) if err.size else 0.0}
+        d.update(extra)
+        _PARITY.append(d)
+        return d
+    return record
+
+
+def pytest_sessionfinish(session, exitstatus):
+    path = os.environ.get("GC_PARITY_LOG")
+    if path and _PARITY:
+        with open(path, "w") as f:
+            json.dump(_PARITY, f, indent=1)

@@ -102,6 +102,7 @@ def lib():
         L.or_tf_samples.argtypes = [P, u32, u32, P]
         L.or_render_dvr.argtypes = [P, C.POINTER(OrView), P, P, P, C.c_int, C.c_int, P]
         L.or_render_dvr_rows.argtypes = [P, C.POINTER(OrView), P, P, P, C.c_int, C.c_int, u32, u32, P]
+        L.or_render_dvr_resume.argtypes = [P, C.POINTER(OrView), P, P, P, P, C.c_int, C.c_int, u32, u32, P]
         L.or_sample.restype = C.c_int
         L.or_sample.argtypes = [P, u32, P, C.POINTER(C.c_double), P, P]
         _lib = L
@@ -263,10 +264,16 @@ class Cache:
         st = lib().or_sample(self.h, level, ptr(np.asarray(pos, np.float32)), C.byref(s), ptr(b), ptr(v))
         return st, s.value, b, v
 
-    def render(self, view, tbuf, cstate, pixels, precision=64, nthreads=1):
+    def render(self, view, tbuf, cstate, pixels, precision=64, nthreads=1, rpos=None):
+        """One launch; rpos (W*H*4 float32) selects exact continuation."""
         st = np.zeros(4, np.uint64)
-        lib().or_render_dvr(self.h, C.byref(view), ptr(tbuf), ptr(cstate), ptr(pixels), precision,
-                            nthreads, ptr(st))
+        if rpos is None:
+            lib().or_render_dvr(self.h, C.byref(view), ptr(tbuf), ptr(cstate), ptr(pixels), precision,
+                                nthreads, ptr(st))
+        else:
+            assert rpos.dtype == np.float32 and rpos.size == view.width * view.height * 4
+            lib().or_render_dvr_resume(self.h, C.byref(view), ptr(tbuf), ptr(cstate), ptr(rpos), ptr(pixels),
+                                       precision, nthreads, 0, view.height, ptr(st))
         return st
 
 

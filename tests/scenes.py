@@ -51,6 +51,15 @@ class Scene:
         c._src = src
         return c
 
+    def resident_gpu_cache(self):
+        """A cache whose pool holds every brick, fully loaded."""
+        shape = O.pool_shape(self.total_bricks * self.bs ** 3 * self.bytes, self.bs ** 3 * self.bytes)
+        src = G.Source.pyramid(self.vc, self.levels)
+        c = G.Cache(self.vc, G.cache_cfg(shape, self.bs, max_requests=self.max_requests, sync=True), src)
+        c._src = src
+        c.load_all(self.device_levels())
+        return c
+
     def device_levels(self):
         return [G.DeviceBuffer.from_array(l) for l in self.levels]
 

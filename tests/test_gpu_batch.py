@@ -16,11 +16,25 @@ pytestmark = pytest.mark.gpu
 SIZES = [(40, 30), (64, 64), (96, 80), (17, 23), (128, 72), (33, 65)]
 
 
-def views_for(prec, tile=0, march=0, k=0):
+def views_for(prec, tile=0, march=0, k=0, resume=0):
     out = []
     for i, (w, h) in enumerate(SIZES):
         cam = G.camera(15.0 + 7.0 * i, 25.0 * i + 11.0 * k, 2.6 + 0.15 * i)
-        out.append(G.make_view(w, h, cam, alpha_stop=0.95 if i % 2 else 0.99, precision=prec, tile=tile, march=march))
+        out.append(G.make_view(w, h, cam, alpha_stop=0.95 if i % 2 else 0.99, precision=prec, tile=tile, march=march,
+                               resume=resume))
+    return out
+
+
+def one_pass(sc, views, d_tfs, sizes):
+    """Each view in one launch over a fully loaded cache."""
+    gc = sc.resident_gpu_cache()
+    out = []
+    for v, t, n in zip(views, d_tfs, sizes):
+        f = G.Frame(v.width, v.height, v.precision)
+        G.render(gc, v, t, n, f)
+        out.append(f.pixels().reshape(-1).copy())
+        f.close()
+    gc.close()
     return out
 
 
@@ -99,7 +113,7 @@ def _oracle_views(sc, views):
 
 
 @pytest.mark.parametrize("bytes_,graph,tile", [(1, False, 0), (2, True, 1)])
-def test_batch_out_of_core_lockstep(gpu, bytes_, graph, tile):
+def test_batch_out_of_core_lockstep(gpu, bytes_, graph, tile, parity_log):
     """Batched progressive launches with a pool smaller than the working set.
     After every launch, each view's pixels (<= 1 LSB), t buffer (exact) and
     FP64 colour state match the oracle rendering that view over the same cache
@@ -120,6 +134,7 @@ def test_batch_out_of_core_lockstep(gpu, bytes_, graph, tile):
     ocs = [np.zeros(v.width * v.height * 4) for v in views]
     prev = np.zeros(4, np.uint64)
     launches = 0
+    errs = []
     for it in range(80):
         print("lockstep launch", it, flush=True)
         b.launch()
@@ -134,6 +149,7 @@ def test_batch_out_of_core_lockstep(gpu, bytes_, graph, tile):
             ost = oc.render(oviews[i], otb[i], ocs[i], opx, precision=64, nthreads=8)
             ototal += ost
             err = np.abs(G.unpack_rgba(b.pixels(i)) - G.unpack_rgba(opx))
+            errs.append(err.reshape(-1))
             assert err.max() <= 1, (it, i)
             tb, cs = b.state(i)
             assert np.array_equal(tb, otb[i]), (it, i)
@@ -148,6 +164,7 @@ def test_batch_out_of_core_lockstep(gpu, bytes_, graph, tile):
         if gd:
             break
     assert gd and launches > 3
+    parity_log(64, np.concatenate(errs), launches=launches)
 
 
 def test_batch_graph_replay_matches_eager(gpu):
@@ -193,14 +210,13 @@ def test_upload_batch_limit(gpu):
 def test_starting_only_launch_keeps_waiting_rays(gpu):
     """A launch that traces only the starting views leaves the rays waiting
     for bricks in their list (their per-view counts unchanged) and appends
-    the new view's stopped rays; the views then complete as usual, within the
-    restart tolerance of the alone renders (pixels of rays resumed at other
-    points may differ by a few LSB)."""
+    the new view's stopped rays; the views then complete as usual.  With
+    exact continuation each view equals its one-pass render bit for bit."""
     sc = Scene((64, 64, 64), 8, 1, 1, max_requests=50)
     gc = sc.gpu_cache()
-    views = views_for(64)[:2]
+    views = views_for(32, resume=1)[:2]
     d_tfs, sizes = tfs(2)
-    b = G.Batch(gc, 2, 2 * 128 * 80, 64)
+    b = G.Batch(gc, 2, 2 * 128 * 80, 32)
     b.set_view(0, views[0], d_tfs[0], sizes[0])
     b.launch()
     p0 = b.pending()
@@ -215,7 +231,58 @@ def test_starting_only_launch_keeps_waiting_rays(gpu):
         if not b.pending().any():
             break
     assert not b.pending().any()
-    ref, _ = singles(sc.gpu_cache(), views, d_tfs, sizes)
+    ref = one_pass(sc, views, d_tfs, sizes)
     for i in range(2):
-        err = np.abs(G.unpack_rgba(b.pixels(i)) - G.unpack_rgba(ref[i]))
-        assert err.max() <= 4 and (err > 1).mean() <= 0.005, i
+        assert np.array_equal(b.pixels(i), ref[i]), i
+
+
+@pytest.mark.parametrize("prec,graph", [(32, False), (64, True)])
+def test_batch_exact_continuation_lockstep(gpu, prec, graph, parity_log):
+    """Progressive batched launches with exact continuation (resume = 1) and
+    a pool smaller than the working set, in lockstep with the oracle's
+    exact-continuation mode: after every launch the pixels are within 1/255
+    of the FP64 oracle, the t buffers and counters are equal, and every
+    update leaves the cache state bit-identical; at the end every view equals
+    its one-pass render bit for bit."""
+    sc = Scene((64, 64, 64), 8, 1, 2, cache_size=(9, 8, 8), max_requests=50)
+    gc, oc = sc.gpu_cache(), sc.oracle_cache()
+    views = views_for(prec, resume=1)[:4]
+    d_tfs, sizes = tfs(len(views))
+    oviews = _oracle_views(sc, views)
+    b = G.Batch(gc, 8, 8 * 128 * 80, prec)
+    b.set_graph(graph)
+    b.begin(views, d_tfs, sizes)
+    otb = [np.zeros(v.width * v.height, np.float32) for v in views]
+    ocs = [np.zeros(v.width * v.height * 4) for v in views]
+    orp = [np.zeros(v.width * v.height * 4, np.float32) for v in views]
+    prev = np.zeros(4, np.uint64)
+    errs = []
+    for it in range(80):
+        print("exact continuation lockstep launch", it, flush=True)
+        b.launch()
+        st = b.stats()
+        now = np.array([st.mapped_samples, st.brick_visits, st.pixels, st.unmapped_rays], np.uint64)
+        delta, prev = now - prev, now
+        pend = b.pending()
+        ototal = np.zeros(4, np.uint64)
+        for i, v in enumerate(views):
+            opx = np.zeros(v.width * v.height, np.uint32)
+            ost = oc.render(oviews[i], otb[i], ocs[i], opx, precision=64, nthreads=8, rpos=orp[i])
+            ototal += ost
+            err = np.abs(G.unpack_rgba(b.pixels(i)) - G.unpack_rgba(opx))
+            errs.append(err.reshape(-1))
+            assert err.max() <= 1, (it, i)
+            tb, cs = b.state(i)
+            assert np.array_equal(tb, otb[i]), (it, i)
+            assert pend[i] == ost[3], (it, i)
+        assert delta[[0, 1, 3]].tolist() == ototal[[0, 1, 3]].tolist(), it
+        gd, od = gc.update(), oc.update()
+        assert gd == od
+        _compare_state(gc, oc, pool=(it % 4 == 0))
+        if gd:
+            break
+    assert gd and it > 3
+    ref = one_pass(sc, views, d_tfs, sizes)
+    for i in range(len(views)):
+        assert np.array_equal(b.pixels(i), ref[i]), i
+    parity_log(prec, np.concatenate(errs), launches=it + 1)

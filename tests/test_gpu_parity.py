@@ -2,10 +2,12 @@
 
 Integer and index results (mips, bricks, page directory, LRU, usage and
 request buffers, pool contents, render counters) must be bit-exact.  Rendered
-RGBA is compared with the FP64-compositing oracle: at precision 64 the device
-follows the same sample trace and must agree within 1/255 per channel; at
-precision 32 (the reference's arithmetic) the maximum error and RMSE are
-reported and bounded.
+RGBA is compared with the FP64-compositing oracle within 1/255 per channel at
+both compositing precisions: at precision 64 the device follows the same
+sample trace and arithmetic; at precision 32 (the precision the benchmarks
+run) it follows the same sample trace with FP32 classification and blending.
+Every RGBA comparison's maximum error and RMSE go to the parity log
+($GC_PARITY_LOG).
 """
 import numpy as np
 import pytest
@@ -83,7 +85,7 @@ def _render_pair(scene, gc, oc, cam, prec, W, H, tf, d_tf, frame, otb, ocs, alph
 
 
 @pytest.mark.parametrize("bytes_", [1, 2])
-def test_resident_load_and_render(gpu, bytes_):
+def test_resident_load_and_render(gpu, bytes_, parity_log):
     sc = Scene((64, 64, 64), 8, 1, bytes_)
     gc, oc = sc.gpu_cache(), sc.oracle_cache()
     d_levels = sc.device_levels()
@@ -101,30 +103,31 @@ def test_resident_load_and_render(gpu, bytes_):
             ocs = np.zeros(W * H * 4)
             gst, g, ost, o = _render_pair(sc, gc, oc, orbit(k), prec, W, H, tf, d_tf, frame, otb, ocs)
             err = np.abs(g - o)
-            print("prec", prec, "frame", k, "stats", gst.tolist(), "max err", err.max(),
-                  "rmse %.4f" % np.sqrt((err.astype(np.float64) ** 2).mean()))
+            rec = parity_log(prec, err, frame=k, bytes=bytes_)
+            print("prec", prec, "frame", k, "stats", gst.tolist(), "max err", rec["max_err"], "rmse %.4f" % rec["rmse"])
             assert gst[2] == W * H
-            if prec == 64:
-                assert gst.tolist() == ost.tolist()
-                assert err.max() <= 1
-                assert np.array_equal(frame.state()[0].reshape(-1), otb)
-                assert np.array_equal(gc.usage(), oc.usage())
-                assert np.array_equal(gc.request(), oc.request())
-            else:
-                assert err.max() <= 2
-                assert (err > 1).mean() < 1e-3
+            assert err.max() <= 1
+            # the sample trace (counters, t buffer, usage, requests) is the
+            # oracle's at either precision
+            assert gst.tolist() == ost.tolist()
+            assert np.array_equal(frame.state()[0].reshape(-1), otb)
+            assert np.array_equal(gc.usage(), oc.usage())
+            assert np.array_equal(gc.request(), oc.request())
 
 
-@pytest.mark.parametrize("bytes_,tile", [(1, 0), (2, 2)])
-def test_out_of_core_sequence(gpu, bytes_, tile):
+@pytest.mark.parametrize("bytes_,tile,prec", [(1, 0, 64), (2, 2, 64), (1, 0, 32), (2, 1, 32)])
+def test_out_of_core_sequence(gpu, bytes_, tile, prec, parity_log):
     """Progressive frames with a pool smaller than the volume: every update's
-    LRU, page directory, miss list, pool and buffers match the oracle."""
+    LRU, page directory, miss list, pool and buffers match the oracle, and
+    every launch's pixels are within 1/255 of the FP64 oracle over the same
+    cache state (FP32 or FP64 compositing on the device)."""
     sc = Scene((64, 64, 64), 8, 1, bytes_, cache_size=(6, 5, 5), max_requests=50)
     gc, oc = sc.gpu_cache(), sc.oracle_cache()
     tf = tf_table(seed=11)
     d_tf = G.DeviceBuffer.from_array(tf)
     W, H = 64, 48
-    frame = G.Frame(W, H, 64)
+    frame = G.Frame(W, H, prec)
+    errs = []
     otb = np.zeros(W * H, np.float32)
     ocs = np.zeros(W * H * 4)
     steps = 0
@@ -134,9 +137,10 @@ def test_out_of_core_sequence(gpu, bytes_, tile):
         otb[:] = 0
         ocs[:] = 0
         for it in range(40):
-            gst, g, ost, o = _render_pair(sc, gc, oc, cam, 64, W, H, tf, d_tf, frame, otb, ocs, tile=tile)
+            gst, g, ost, o = _render_pair(sc, gc, oc, cam, prec, W, H, tf, d_tf, frame, otb, ocs, tile=tile)
             assert gst.tolist() == ost.tolist(), "render counters differ at view %d step %d" % (k, it)
-            assert np.abs(g - o).max() <= 1
+            errs.append(np.abs(g - o))
+            assert errs[-1].max() <= 1
             assert np.array_equal(frame.state()[0].reshape(-1), otb), "t buffer differs"   # FP32 ray geometry
             gd, od = gc.update(), oc.update()
             assert gd == od
@@ -147,6 +151,7 @@ def test_out_of_core_sequence(gpu, bytes_, tile):
             steps += 1
             if gd:
                 break
+    parity_log(prec, np.concatenate(errs), launches=steps, bytes=bytes_)
     assert steps > 3
 
 
@@ -287,9 +292,9 @@ def test_brick32_kernel_matches_generic(gpu, bytes_, resume):
 
 
 @pytest.mark.parametrize("bytes_", [1, 2])
-def test_brick32_fp32_against_oracle(gpu, bytes_):
+def test_brick32_fp32_against_oracle(gpu, bytes_, parity_log):
     """The bricks-of-32 variant at FP32 against the FP64 oracle, fully
-    resident: RGBA within the FP32 tolerance."""
+    resident: RGBA within 1/255, the same sample trace."""
     sc = Scene((96, 96, 80), 32, 1, bytes_)
     gc, oc = sc.gpu_cache(), sc.oracle_cache()
     gc.load_all(sc.device_levels())
@@ -304,8 +309,11 @@ def test_brick32_fp32_against_oracle(gpu, bytes_):
         ocs = np.zeros(W * H * 4)
         gst, g, ost, o = _render_pair(sc, gc, oc, orbit(k), 32, W, H, tf, d_tf, frame, otb, ocs, alpha_stop=0.99)
         err = np.abs(g - o)
-        print("brick32 fp32 frame", k, "stats", gst.tolist(), "max err", err.max(), flush=True)
+        rec = parity_log(32, err, frame=k, bytes=bytes_)
+        print("brick32 fp32 frame", k, "stats", gst.tolist(), "max err", rec["max_err"], "rmse %.4f" % rec["rmse"],
+              flush=True)
         assert gst[2] == W * H
-        assert err.max() <= 2
-        assert (err > 1).mean() < 1e-3
+        assert err.max() <= 1
+        assert gst.tolist() == ost.tolist()
+        assert np.array_equal(frame.state()[0].reshape(-1), otb)
         frame.close()
