@@ -200,6 +200,10 @@ struct DvrView {
     float4 *rpos;                                  // exact continuation: per pixel the position (xyz) and
                                                    // t (w) of the sample a stopped ray resumes at; null:
                                                    // resume at pos = origin + dir * t (the reference)
+    double *qbuf;                                  // FP32: per pixel the exact opacity product Q of the
+                                                   // samples blended so far (read when tbuf > 0)
+    double thr;                                    // FP32: a ray continues while Q > thr =
+                                                   // (1 - alpha_stop)^(1/k0)
 };
 
 struct DvrLaunch {
@@ -216,6 +220,13 @@ struct DvrLaunch {
     float inv_levels[GC_MAX_LEVELS][3];
     float lod_step[GC_MAX_LEVELS];
     float kexp[GC_MAX_LEVELS];
+    // exact opacity decisions of FP32 compositing (exact_alpha = 1): y =
+    // x^(kexp[l] / kexp[0]) by esq[l] squarings (kexp[l] / kexp[0] =
+    // 2^esq[l] on every level); a sample is skipped when y >= qskip =
+    // (1 - 0.001)^(1/kexp[0])
+    int32_t esq[GC_MAX_LEVELS];
+    double qskip;
+    uint32_t exact_alpha;
     DvrView view0;              // the view when `views` is null
     const DvrView *views;       // device table of n_views views, or null
     uint32_t n_views;

@@ -77,6 +77,7 @@ struct LevelLds {
     float ior[3], rdv[3], ldf[3], vvf[3], inv[3];
     uint32_t lcoord;
     float step, kexp;
+    int32_t esq;        // exact opacity factor: squarings (see opacity_factor)
 };
 
 struct F3 {
@@ -337,57 +338,100 @@ __device__ inline Acc4<ACC> classify_padded(ACC s, const float4 *TFp, uint32_t N
 // Texture coordinate of an integer box sum.  The sample is s = sum / D
 // (D = 8 * 255 for u8, 8 * 65535 for u16) and the coordinate xf = s * N - 0.5
 // (RayCasters.cu:163); xf + 1 = (2 sum N + D) / (2 D), so q = floor(xf) + 1
-// and the lerp weight frac(xf) follow from integer arithmetic, with one
-// rounding in the weight.  In FP32, forming s and xf as floats would round
-// xf to 2^-15 at N = 256, an error in the weight that the lerp carries into
-// every colour and alpha.  WIDE: 64-bit numerator (tables above 4095
-// entries); else 2 sum N + D < 2^32.
+// and the lerp weight frac(xf) = rem / (2 D) follow from integer arithmetic.
+// In FP32, forming s and xf as floats would round xf to 2^-15 at N = 256, an
+// error in the weight that the lerp carries into every colour and alpha.
+// WIDE: 64-bit numerator (tables above 4095 entries); else 2 sum N + D < 2^32.
 template <int BYTES, bool WIDE>
-__device__ inline void tf_coord(uint32_t sum, uint32_t N, uint32_t &q, float &a)
+__device__ inline void tf_coord(uint32_t sum, uint32_t N, uint32_t &q, uint32_t &rem)
 {
     constexpr uint32_t D = BYTES == 1 ? 2040u : 524280u;
-    constexpr float inv2d = 1.0f / (float)(2u * D);
     if constexpr (WIDE) {
         const uint64_t num = 2ull * sum * N + D;
         const uint64_t qq = num / (2u * D);
         q = (uint32_t)qq;
-        a = (float)(uint32_t)(num - qq * (2u * D)) * inv2d;
+        rem = (uint32_t)(num - qq * (2u * D));
     } else {
         const uint32_t num = 2u * sum * N + D;
         q = num / (2u * D);
-        a = (float)(num - q * (2u * D)) * inv2d;
+        rem = num - q * (2u * D);
     }
 }
 
-// Classification of an integer box sum: FP32 from the exact coordinate,
-// FP64 as the FP64 caster computes it (s = sum / D, then s * N - 0.5).
-// Padded LDS table (N <= DVR_TF_LDS_MAX).
+// FP32 classification of an integer box sum at the exact coordinate.  It
+// also returns x = 1 - a in double (a: the interpolated transfer-function
+// alpha, from the double weight), the input of the exact opacity decisions.
+template <int BYTES, typename T4>
+__device__ inline Acc4<float> shade_exact(uint32_t rem, T4 T0, T4 T1, float kexp, bool &skip, double &x)
+{
+    constexpr uint32_t D2 = 2u * (BYTES == 1 ? 2040u : 524280u);
+    const float a = (float)rem * (1.0f / (float)D2);
+    const double wd = (double)rem * (1.0 / (double)D2);
+    x = 1.0 - fma(wd, (double)T1.w - (double)T0.w, (double)T0.w);
+    return shade<float>(a, T0, T1, kexp, skip);
+}
+
+// Classification of an integer box sum.  FP64: as the FP64 caster computes
+// it (s = sum / D, then s * N - 0.5), skip from the FP64 alpha.  FP32: the
+// exact coordinate, skip from the FP32 alpha, and x as above.  Padded LDS
+// table (N <= DVR_TF_LDS_MAX).
 template <int BYTES, typename ACC>
-__device__ inline Acc4<ACC> classify_sum_padded(uint32_t sum, const float4 *TFp, uint32_t N, float kexp, bool &skip)
+__device__ inline Acc4<ACC> classify_sum_padded(uint32_t sum, const float4 *TFp, uint32_t N, float kexp, bool &skip,
+                                                double &x)
 {
     if constexpr (sizeof(ACC) == 8) {
         return classify_padded<double>((double)sum / (BYTES == 1 ? 2040.0 : 524280.0), TFp, N, kexp, skip);
     } else {
-        uint32_t q;
-        float a;
-        tf_coord<BYTES, false>(sum, N, q, a);
-        return shade<float>(a, TFp[q], TFp[q + 1], kexp, skip);
+        uint32_t q, rem;
+        tf_coord<BYTES, false>(sum, N, q, rem);
+        return shade_exact<BYTES>(rem, TFp[q], TFp[q + 1], kexp, skip, x);
     }
 }
 // Table in HBM (any N): entries clamp(q - 1) and clamp(q).
 template <int BYTES, typename ACC, typename TFP>
-__device__ inline Acc4<ACC> classify_sum(uint32_t sum, TFP TF, uint32_t N, float kexp, bool &skip)
+__device__ inline Acc4<ACC> classify_sum(uint32_t sum, TFP TF, uint32_t N, float kexp, bool &skip, double &x)
 {
     if constexpr (sizeof(ACC) == 8) {
         return classify<double>((double)sum / (BYTES == 1 ? 2040.0 : 524280.0), TF, N, kexp, skip);
     } else {
-        uint32_t q;
-        float a;
-        tf_coord<BYTES, true>(sum, N, q, a);
+        uint32_t q, rem;
+        tf_coord<BYTES, true>(sum, N, q, rem);
         const uint32_t i0 = q == 0 ? 0u : (q - 1u > N - 1u ? N - 1u : q - 1u);
         const uint32_t i1 = q > N - 1u ? N - 1u : q;
-        return shade<float>(a, TF[i0], TF[i1], kexp, skip);
+        return shade_exact<BYTES>(rem, TF[i0], TF[i1], kexp, skip, x);
     }
+}
+// Float voxels (FP32 colour from the float sample, x from the double one).
+template <typename TFP>
+__device__ inline Acc4<float> classify_float_x(float s, double sd, TFP TF, uint32_t N, float kexp, bool &skip,
+                                               double &x)
+{
+    const double xf = sd * (double)N - 0.5;
+    const double fl = floor(xf);
+    const double ad = xf - fl;
+    int i0 = (int)fl, i1 = i0 + 1;
+    i0 = i0 < 0 ? 0 : (i0 > (int)N - 1 ? (int)N - 1 : i0);
+    i1 = i1 < 0 ? 0 : (i1 > (int)N - 1 ? (int)N - 1 : i1);
+    x = 1.0 - ((1.0 - ad) * (double)TF[i0].w + ad * (double)TF[i1].w);
+    return classify<float>(s, TF, N, kexp, skip);
+}
+
+// Exact opacity decisions of FP32 compositing.  The FP64 caster keeps alpha
+// as fc.w += a'(1 - fc.w) with a' = 1 - x^k (k the level's opacity-correction
+// exponent), skips samples with a' <= 0.001 and stops once fc.w >= the alpha
+// stop (RayCasters.cu:122,165-178).  The transmittance 1 - fc.w is the
+// product of x^k over the blended samples, i.e. Q^(k0) with Q the product of
+// y = x^(k / k0) (k0: level 0's exponent).  Accumulating Q in double makes
+// every skip and stop decision the FP64 caster's (their thresholds become
+// (1 - 0.001)^(1/k0) and (1 - alpha stop)^(1/k0)), so FP32 compositing
+// follows the FP64 sample trace; only the colours are FP32.  y is x squared
+// esq times, k / k0 = 2^esq: every level of a volume whose smallest extent
+// halves exactly from level to level (power-of-two volumes).  Other volumes
+// (DvrLaunch::exact_alpha == 0) take the FP32 decisions.
+__device__ inline double opacity_factor(double x, int esq)
+{
+    for (int i = 0; i < esq; ++i) x *= x;
+    return x;
 }
 
 // Next packet for a wave on XCD `xcd`: its own contiguous share first, then
@@ -529,6 +573,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         L.lcoord = g->level_coord[tid];
         L.step = p.lod_step[tid];
         L.kexp = p.kexp[tid];
+        L.esq = p.esq[tid];
     }
     if (p.views) {
         for (uint32_t i = tid; i < n_views; i += LANES) vw[i] = p.views[i];
@@ -537,6 +582,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     }
     __syncthreads();
     const bool tf_lds = BYTES != 4 && p.tf_lds_entries != 0;
+    const double qskip = p.qskip;
+    const bool exact = p.exact_alpha != 0;
     if (tf_lds) load_tf_tables(vw, n_views, tfl, tid);
     __syncthreads();
     const uint32_t tf_n1 = p.tf_lds_entries - 2u;   // the single table's size (!MT)
@@ -575,13 +622,16 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     F3 pos = f3(0, 0, 0), dir = f3(0, 0, 0), dstep = f3(0, 0, 0);
     // current level: index, next switch distance and its parameters
     uint32_t lod = 0, lcoord = 0;
-    float tnext = FINF, step = 0.f, kexp = 0.f;
+    float tnext = FINF, step = 0.f;
     F3 ior = f3(0, 0, 0), rdv = f3(0, 0, 0), ldf = f3(0, 0, 0), vvf = f3(0, 0, 0);
     // current brick: page-table cell, entry state, slot and base address
     uint32_t pcx = NONE, pcy = NONE, pcz = NONE, ew = 0, slot = 0, vkey = NONE;
     uint32_t vsum = 0;   // box sum of the voxel vkey (non-pipelined march)
     const uint8_t *bbase = pool;
     Acc4<ACC> fc{0, 0, 0, 0};
+    // FP32: exact opacity product of the blended samples and its stop
+    // threshold (see opacity_factor)
+    double Q = 1.0, thr = 0.0;
     uint32_t n_samples = 0, n_visits = 0;      // per lane
     uint32_t n_rays = 0, n_unmapped = 0;       // per wave
 
@@ -593,7 +643,6 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         vvf = f3(L.vvf[0], L.vvf[1], L.vvf[2]);
         lcoord = L.lcoord;
         step = L.step;
-        kexp = L.kexp;
         dstep = dir * step;
         tnext = l + 1 < nl ? tsw[(l + 1) * LANES + tid] : FINF;
     };
@@ -614,7 +663,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // C holds one; newray: the next sample placed is the ray's first
     bool hb = false, hc = false, newray = false;
     uint32_t cellB = 0, viB = 0, cellC = 0, slotC = 0, ewC = 0, viC = 0, sum_prev = 0;
-    float kB = 0.f, tC = 0.f, kC = 0.f;
+    uint32_t lodB = 0, lodC = 0;   // levels of samples B and C
+    float tC = 0.f;
     // position of sample C, kept (in LDS, off the register budget) only when
     // a view of the launch asks for exact continuation
     __shared__ F3 pos_c[LANES];
@@ -634,30 +684,50 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         const DvrView &V = vw[v];
         gp(V.tbuf)[pix] = tt;
         if (V.rpos) gp(reinterpret_cast<gf4 *>(V.rpos))[pix] = gf4{pp.x, pp.y, pp.z, tt};
-        if constexpr (sizeof(ACC) == 8)
+        if constexpr (sizeof(ACC) == 8) {
             gp(reinterpret_cast<gd4 *>(V.cstate))[pix] = gd4{fc.x, fc.y, fc.z, fc.w};
-        else
+        } else {
             gp(reinterpret_cast<gf4 *>(V.cstate))[pix] = gf4{fc.x, fc.y, fc.z, fc.w};
+            if (V.qbuf) gp(V.qbuf)[pix] = Q;
+        }
         gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
     };
     // Classification and front-to-back blending of one mapped sample
     // (RayCasters.cu:161-178); composite_if blends only where `on`.
     // Integer voxels are classified from their box sum (exact texture
     // coordinate), float voxels from the sample value s.
-    auto classify_any = [&](uint32_t sum, ACC s, float kx, bool &cskip) -> Acc4<ACC> {
+    // At FP32 the skip decision comes from the exact opacity factor and the
+    // blended sample's factor joins Q.
+    auto classify_any = [&](uint32_t sum, ACC s, double sd, uint32_t lx, bool &cskip, double &y) -> Acc4<ACC> {
+        const LevelLds &L = lv[lx];
+        const float kx = L.kexp;
+        double x = 1.0;
+        Acc4<ACC> cc;
         if constexpr (BYTES != 4) {
-            return tf_lds ? (MT ? classify_sum_padded<BYTES, ACC>(sum, tfl + (tfw & 0xffffu), tfw >> 16, kx, cskip)
-                                : classify_sum_padded<BYTES, ACC>(sum, tfl, tf_n1, kx, cskip))
-                          : classify_sum<BYTES, ACC>(sum, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size,
-                                                     kx, cskip);
+            cc = tf_lds ? (MT ? classify_sum_padded<BYTES, ACC>(sum, tfl + (tfw & 0xffffu), tfw >> 16, kx, cskip, x)
+                              : classify_sum_padded<BYTES, ACC>(sum, tfl, tf_n1, kx, cskip, x))
+                        : classify_sum<BYTES, ACC>(sum, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size, kx,
+                                                   cskip, x);
         } else {
-            return classify<ACC>(s, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size, kx, cskip);
+            if constexpr (sizeof(ACC) == 8)
+                cc = classify<ACC>(s, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size, kx, cskip);
+            else
+                cc = classify_float_x(s, sd, gp(reinterpret_cast<const gf4 *>(vw[v].tf)), vw[v].tf_size, kx, cskip, x);
         }
+        if constexpr (sizeof(ACC) == 4) {
+            if (exact) {
+                y = opacity_factor(x, L.esq);
+                cskip = !(y < qskip);
+            }
+        }
+        return cc;
     };
-    auto composite_if = [&](bool on, uint32_t sum, float kx) {
+    auto composite_if = [&](bool on, uint32_t sum, uint32_t lx) {
         bool cskip;
-        const Acc4<ACC> cc = classify_any(sum, (ACC)0, kx, cskip);
+        double y = 1.0;
+        const Acc4<ACC> cc = classify_any(sum, (ACC)0, 0.0, lx, cskip, y);
         const bool b = on && !cskip;
+        if constexpr (sizeof(ACC) == 4) Q = b ? Q * y : Q;
         const ACC wgt = (ACC)1 - fc.w;
         Acc4<ACC> n;
         if constexpr (sizeof(ACC) == 8) {
@@ -670,10 +740,12 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         fc.z = b ? n.z : fc.z;
         fc.w = b ? n.w : fc.w;
     };
-    auto composite = [&](uint32_t sum, ACC s, float kx) {
+    auto composite = [&](uint32_t sum, ACC s, double sd, uint32_t lx) {
         bool cskip;
-        const Acc4<ACC> cc = classify_any(sum, s, kx, cskip);
+        double y = 1.0;
+        const Acc4<ACC> cc = classify_any(sum, s, sd, lx, cskip, y);
         if (!cskip) {
+            if constexpr (sizeof(ACC) == 4) Q *= y;
             const ACC wgt = (ACC)1 - fc.w;
             if constexpr (sizeof(ACC) == 8) {
                 const ACC tx = cc.x * wgt, ty = cc.y * wgt, tz = cc.z * wgt, tw = cc.w * wgt;
@@ -774,6 +846,10 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                         const float epx = fabsf(dot(dir - dxn, dleft) * tfar);
                         const float epy = fabsf(dot(dir - dyn, dup) * tfar);
                         astop = V.alpha_stop;
+                        if constexpr (sizeof(ACC) == 4) {
+                            thr = V.thr;
+                            Q = (tb > 0.f && V.qbuf) ? gp(V.qbuf)[pix] : 1.0;
+                        }
                         // level-switch distances from t on; the level at t
                         // is the last one already reached (compute_LOD)
                         uint32_t l0 = 0;
@@ -868,7 +944,9 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                         }
                     }
                     // C: consume sample j (RayCasters.cu:118-192)
-                    const bool live = hc && (ACC)fc.w < (ACC)astop && tC < tfar;
+                    bool live;
+                    if constexpr (sizeof(ACC) == 4) live = hc && (exact ? Q > thr : fc.w < astop) && tC < tfar;
+                    else live = hc && fc.w < (ACC)astop && tC < tfar;
                     const bool unmapped = live && ewC == 0;
                     const bool cons = live && ewC == 1;
                     n_visits += (live && freshC) ? 1u : 0u;
@@ -882,7 +960,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     // valid sample value); only consumed samples blend
                     const uint32_t sum = reuseC ? sum_prev : box_add<BYTES>(w_old);
                     sum_prev = cons ? sum : sum_prev;
-                    composite_if(cons, sum, kC);
+                    composite_if(cons, sum, lodC);
                     n_samples += cons ? 1u : 0u;
                     const bool end = (hc && !live) || unmapped;
                     if (end) store_state(tC, keep_pos ? pos_c[tid] : pos_b);
@@ -897,14 +975,14 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     ewC = hb ? ewB : ewC;
                     tC = hb ? t_b : tC;
                     if (keep_pos && hb) pos_c[tid] = pos_b;
-                    kC = hb ? kB : kC;
+                    lodC = hb ? lodB : lodC;
                     freshC = hb ? freshB : freshC;
                     viC = hb ? viB : viC;
                     reuseC = hb ? reuseB : reuseC;
                     hc = hb && !end;
                     cellB = discard ? cellB : cellA;
                     viB = discard ? viB : viA;
-                    kB = discard ? kB : kexp;
+                    lodB = discard ? lodB : lod;
                     freshB = discard ? freshB : freshA;
                     newray = newray && discard;
                     have = have && !end;
@@ -925,7 +1003,10 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         if (!piped && have) {
             bool done = false;
             for (int k = 0; k < march; ++k) {
-                if (!((ACC)fc.w < (ACC)astop && t < tfar)) { done = true; break; }
+                bool go;
+                if constexpr (sizeof(ACC) == 4) go = exact ? Q > thr : fc.w < astop;
+                else go = fc.w < (ACC)astop;
+                if (!(go && t < tfar)) { done = true; break; }
                 while (t >= tnext) set_level(++lod);
                 F3 tp = pos * 0.495f + f3(0.495f, 0.495f, 0.495f);
                 tp.z = 1.0f - tp.z;
@@ -977,6 +1058,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     lz = vz % inner;
                 }
                 ACC s = 0;
+                double sd = 0.0;
                 uint32_t isum = 0;
                 if (fast_fetch) {
                     const uint32_t vi = __umul24(__umul24(lz, bs) + ly, bs) + lx;   // voxel in the brick
@@ -1000,13 +1082,14 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                                 dsum += tv;
                             }
                     if constexpr (BYTES == 4) {
-                        if constexpr (sizeof(ACC) == 8) s = dsum / 8.0;
+                        sd = dsum / 8.0;
+                        if constexpr (sizeof(ACC) == 8) s = sd;
                         else s = fsum / 8.0f;
                     } else {
                         isum = (uint32_t)dsum;   // a sum of 8 integer voxels, exact
                     }
                 }
-                composite(isum, s, kexp);
+                composite(isum, s, sd, lod);
                 ++n_samples;
                 t += step;
                 pos = pos + dstep;
@@ -1097,6 +1180,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         L.lcoord = g->level_coord[tid];
         L.step = p.lod_step[tid];
         L.kexp = p.kexp[tid];
+        L.esq = p.esq[tid];
     }
     const uint32_t n_views = p.views ? p.n_views : 1u;
     if (p.views) {
@@ -1111,6 +1195,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     __syncthreads();
 
     const uint32_t list_count = __builtin_amdgcn_readfirstlane(*p.list_count);
+    const double qskip = p.qskip;
+    const bool exact = p.exact_alpha != 0;
     const uint32_t bs = BSC ? (uint32_t)BSC : g->brick_size, cov = BSC ? 1u : g->covering,
                    inner = BSC ? (uint32_t)BSC - 2u : g->inner;
     const uint32_t MX = g->mrpd_dim[0], MY = g->mrpd_dim[1];
@@ -1137,6 +1223,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     float t = 0.f, tfar = 0.f, astop = 0.f, step = 0.f, tnext = FINF;
     F3 pos = f3(0, 0, 0), dir = f3(0, 0, 0), dstep = f3(0, 0, 0);
     uint32_t lod = 0, cell_last = NONE;
+    double Q = 1.0, thr = 0.0;   // FP32: exact opacity product and its stop threshold
     uint32_t n_samples = 0, n_visits = 0, n_rays = 0, n_unmapped = 0;   // on each group's first lane
     auto set_level = [&](uint32_t l) {
         step = lv[l].step;
@@ -1182,6 +1269,10 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     dir = Rs.dir;
                     tfar = Rs.tfar;
                     astop = V.alpha_stop;
+                    if constexpr (sizeof(ACC) == 4) {
+                        thr = V.thr;
+                        Q = (tb > 0.f && V.qbuf) ? gp(V.qbuf)[pix] : 1.0;
+                    }
                     t = tnear;
                     pos = Rs.origin + dir * tnear;
                     if (V.rpos && tb > 0.f && tb == tnear) {
@@ -1241,10 +1332,18 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         const uint64_t off = (active && ew == 1) ? (uint64_t)slot * bbytes + corner + vi * (uint32_t)BYTES : 0ull;
         const uint32_t sum = box_add<BYTES>(box_load<BYTES>(pool + off, row, plane));
         bool skip;
+        double x = 1.0, y = 1.0;
         const DvrView &V = vw[v];
         const Acc4<ACC> cc =
-            tf_lds ? classify_sum_padded<BYTES, ACC>(sum, tfl + (V.tf_lds_off & ~DVR_TF_OWNER), V.tf_size, L.kexp, skip)
-                   : classify_sum<BYTES, ACC>(sum, gp(reinterpret_cast<const gf4 *>(V.tf)), V.tf_size, L.kexp, skip);
+            tf_lds ? classify_sum_padded<BYTES, ACC>(sum, tfl + (V.tf_lds_off & ~DVR_TF_OWNER), V.tf_size, L.kexp, skip, x)
+                   : classify_sum<BYTES, ACC>(sum, gp(reinterpret_cast<const gf4 *>(V.tf)), V.tf_size, L.kexp, skip, x);
+        if constexpr (sizeof(ACC) == 4) {
+            // exact opacity decisions (opacity_factor)
+            if (exact) {
+                y = opacity_factor(x, L.esq);
+                skip = !(y < qskip);
+            }
+        }
         const uint64_t m_fresh = gbits(__ballot(fresh)), m_skip = gbits(__ballot(skip));
         const uint32_t u = first_clear(gbits(__ballot(ew == 1)));   // first sample not mapped
         const uint32_t f = first_clear(gbits(__ballot(my_t < tfar)));   // first sample with t >= tfar
@@ -1255,8 +1354,13 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         for (uint32_t k = 0; k < (uint32_t)G; ++k) {
             const ACC cx = __shfl(cc.x, (int)k, G), cy = __shfl(cc.y, (int)k, G), cz = __shfl(cc.z, (int)k, G),
                       cw = __shfl(cc.w, (int)k, G);
+            double yk = 1.0;
+            if constexpr (sizeof(ACC) == 4) yk = __shfl(y, (int)k, G);
+            bool go;
+            if constexpr (sizeof(ACC) == 4) go = exact ? Q > thr : fc.w < astop;
+            else go = fc.w < (ACC)astop;
             if (scanning) {
-                if (k >= f || !((ACC)fc.w < (ACC)astop)) {
+                if (k >= f || !go) {
                     done = true;
                     scanning = false;
                     kend = k;
@@ -1273,6 +1377,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                         fc.y = fmaf(cy, wgt, fc.y);
                         fc.z = fmaf(cz, wgt, fc.z);
                         fc.w = fmaf(cw, wgt, fc.w);
+                        Q *= yk;
                     }
                 }
             }
@@ -1297,10 +1402,12 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 if (gl == 0) {
                     gp(V.tbuf)[pix] = tt;
                     if (V.rpos) gp(reinterpret_cast<gf4 *>(V.rpos))[pix] = gf4{px_k, py_k, pz_k, tt};
-                    if constexpr (sizeof(ACC) == 8)
+                    if constexpr (sizeof(ACC) == 8) {
                         gp(reinterpret_cast<gd4 *>(V.cstate))[pix] = gd4{fc.x, fc.y, fc.z, fc.w};
-                    else
+                    } else {
                         gp(reinterpret_cast<gf4 *>(V.cstate))[pix] = gf4{fc.x, fc.y, fc.z, fc.w};
+                        if (V.qbuf) gp(V.qbuf)[pix] = Q;
+                    }
                     gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
                 }
             };

@@ -1036,6 +1036,7 @@ struct Frame {
     float *tbuf = nullptr;
     void *cstate = nullptr;
     float4 *rpos = nullptr;                    // resume positions (views with resume = 1)
+    double *qbuf = nullptr;                    // FP32: exact opacity products
     // launch scratch: [0..7] per-XCD packet cursors, [8], [9] ray-list counts
     uint32_t *ctl = nullptr;
     uint32_t *lists[2] = {nullptr, nullptr};   // rays stopped at a miss
@@ -1046,11 +1047,31 @@ struct Frame {
         hipFree(tbuf);
         hipFree(cstate);
         hipFree(rpos);
+        hipFree(qbuf);
         hipFree(ctl);
         hipFree(lists[0]);
         hipFree(lists[1]);
     }
 };
+
+// Exact opacity threshold of a view for FP32 compositing: a ray continues
+// while its opacity product Q > (1 - alpha_stop)^(1/k0) (see kernels_dvr.hip,
+// opacity_factor); an alpha stop of 1 continues while Q > 0, above 1 always.
+static double opacity_threshold(float alpha_stop, float k0)
+{
+    const double a = (double)alpha_stop;
+    if (a > 1.0) return -1.0;
+    if (a == 1.0) return 0.0;
+    return std::pow(1.0 - a, 1.0 / (double)k0);
+}
+
+// Exact opacity decisions hold when every level's exponent is a power-of-two
+// multiple of level 0's and every threshold stays a normal double.
+static void finish_exact(DvrLaunch &L, const DvrView *views, uint32_t n)
+{
+    for (uint32_t i = 0; i < n; ++i)
+        if (views[i].width && views[i].thr >= 0.0 && views[i].thr < 1e-290) L.exact_alpha = 0;
+}
 
 // Shared part of a ray-caster launch over cache C: device state, timestamp
 // and the LOD tables of VolumeRayCaster.cu:136-161 (start-up box [-1,1]^3).
@@ -1075,6 +1096,20 @@ static void launch_base(const Cache &C, DvrLaunch &L)
         L.lod_step[i] = (smin / (float)m) / 2.f;
         L.kexp[i] = L.lod_step[i] / L.tstep;
     }
+    // exact opacity decisions: kexp[l] / kexp[0] = 2^esq[l] on every level
+    L.exact_alpha = 1;
+    for (uint32_t i = 0; i < C.vc.nb_levels; ++i) {
+        const double r = (double)L.kexp[i] / (double)L.kexp[0];
+        int e = 0;
+        const double m = std::frexp(r, &e);   // r = m * 2^e, m in [0.5, 1)
+        if (m != 0.5 || e < 1 || e > 31) {
+            L.exact_alpha = 0;
+            L.esq[i] = 0;
+        } else {
+            L.esq[i] = e - 1;
+        }
+    }
+    L.qskip = std::pow(1.0 - (double)0.001f, 1.0 / (double)L.kexp[0]);
     // The pipelined march takes the request cell to be the page cell (true when
     // realDivVirtual is exactly 1, which the bricker's padded level sizes give)
     // and forms page indices with 24-bit multiplies.
@@ -1085,7 +1120,7 @@ static void launch_base(const Cache &C, DvrLaunch &L)
                   (uint64_t)C.g.mrpd_dim[1] * C.g.mrpd_dim[2] < (1u << 24);
 }
 
-static void view_of(const gc_view &v, const float *d_tf, uint32_t tf_size, const Frame &F, DvrView &out)
+static void view_of(const gc_view &v, const float *d_tf, uint32_t tf_size, const Frame &F, float k0, DvrView &out)
 {
     uint32_t tw, th;
     tile_shape(v.tile, tw, th);
@@ -1103,6 +1138,8 @@ static void view_of(const gc_view &v, const float *d_tf, uint32_t tf_size, const
     out.tbuf = F.tbuf;
     out.cstate = F.cstate;
     out.rpos = v.resume ? F.rpos : nullptr;
+    out.qbuf = F.precision == 32 ? F.qbuf : nullptr;
+    out.thr = opacity_threshold(v.alpha_stop, k0);
 }
 
 static uint32_t packets_of(const gc_view &v)
@@ -1127,6 +1164,7 @@ struct Batch {
     float *tbuf = nullptr;
     void *cstate = nullptr;
     float4 *rpos = nullptr;             // resume positions (views with resume = 1)
+    double *qbuf = nullptr;             // FP32: exact opacity products
     uint32_t *lists[2] = {nullptr, nullptr};
     // [0..7] per-XCD packet cursors, [8], [9] ray-list counts, [10] timestamp,
     // [16..16+max_views) rays stopped per slot in the last launch
@@ -1160,6 +1198,7 @@ struct Batch {
         hipFree(tbuf);
         hipFree(cstate);
         hipFree(rpos);
+        hipFree(qbuf);
         hipFree(lists[0]);
         hipFree(lists[1]);
         hipFree(ctl);
@@ -1198,6 +1237,12 @@ static int batch_place(Batch &B, uint32_t slot, const gc_view &v, const float *d
     D.tbuf = B.tbuf + off;
     D.cstate = static_cast<uint8_t *>(B.cstate) + off * B.cbytes_per_pixel();
     D.rpos = v.resume ? B.rpos + off : nullptr;
+    D.qbuf = B.precision == 32 ? B.qbuf + off : nullptr;
+    {
+        DvrLaunch K{};
+        launch_base(*B.C, K);
+        D.thr = opacity_threshold(v.alpha_stop, K.kexp[0]);
+    }
     B.table = std::max(B.table, slot + 1);
     // a starting view begins from an empty progressive state
     hipStream_t s = B.C->stream;
@@ -1252,6 +1297,7 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
     L.view_stopped = B.ctl + 16;
     for (uint32_t i = 0; i < B.table; ++i)
         if (B.slots[i].used && B.slots[i].v.rpos) L.exact_resume = 1;
+    finish_exact(L, B.h_views, B.table);
     const int out = has_list ? 1 - B.cur : (append ? B.cur : 0);
     if (has_list) {
         L.list = B.lists[B.cur];
@@ -1903,6 +1949,7 @@ int gc_frame_create(uint32_t width, uint32_t height, uint32_t precision, gc_fram
     GC_TRY(dalloc(&F.tbuf, (size_t)width * height * 4));
     GC_TRY(dalloc(&F.cstate, F.cbytes()));
     GC_TRY(dalloc(&F.rpos, (size_t)width * height * sizeof(float4)));
+    GC_TRY(dalloc(&F.qbuf, (size_t)width * height * sizeof(double)));
     GC_TRY(dalloc(&F.ctl, 64));
     GC_TRY(dalloc(&F.lists[0], (size_t)width * height * 4));
     GC_TRY(dalloc(&F.lists[1], (size_t)width * height * 4));
@@ -1974,8 +2021,9 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
     if (view->tile > 3) return fail(GC_ERR_ARG, "tile must be 0..3");
     DvrLaunch L{};
     launch_base(C, L);
-    view_of(*view, d_tf, tf_size, F, L.view0);
+    view_of(*view, d_tf, tf_size, F, L.kexp[0], L.view0);
     L.exact_resume = L.view0.rpos ? 1u : 0u;
+    finish_exact(L, &L.view0, 1);
     L.tf_lds_entries = dvr_assign_tf_lds(&L.view0, 1, C.vc.encoded_bytes, &L.tf_lds_tables);
     L.n_views = 1;
     L.n_packets = packets_of(*view);
@@ -2004,8 +2052,9 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
 {
     DvrLaunch L{};
     launch_base(C, L);
-    view_of(view, d_tf, tf_size, F, L.view0);
+    view_of(view, d_tf, tf_size, F, L.kexp[0], L.view0);
     L.exact_resume = L.view0.rpos ? 1u : 0u;
+    finish_exact(L, &L.view0, 1);
     L.tf_lds_entries = dvr_assign_tf_lds(&L.view0, 1, C.vc.encoded_bytes, &L.tf_lds_tables);
     L.n_views = 1;
     L.n_packets = first ? packets_of(view) : 0;
@@ -2154,6 +2203,7 @@ int gc_batch_create(gc_cache *c, uint32_t max_views, uint64_t max_pixels, uint32
     GC_TRY(dalloc(&B.tbuf, max_pixels * 4));
     GC_TRY(dalloc(&B.cstate, max_pixels * B.cbytes_per_pixel()));
     GC_TRY(dalloc(&B.rpos, max_pixels * sizeof(float4)));
+    GC_TRY(dalloc(&B.qbuf, max_pixels * sizeof(double)));
     GC_TRY(dalloc(&B.lists[0], max_pixels * 4));
     GC_TRY(dalloc(&B.lists[1], max_pixels * 4));
     GC_TRY(dalloc(&B.ctl, (16 + DVR_MAX_VIEWS) * 4));

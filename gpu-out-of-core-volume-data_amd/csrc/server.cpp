@@ -107,8 +107,9 @@ struct Server {
     double now_ms() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
     uint64_t slot_bytes() const
     {
-        // pixels + t + colour state + resume position + two ray-list entries per pixel
-        return (uint64_t)cfg.max_views * cfg.slot_pixels * (4 + 4 + (cfg.precision == 64 ? 32 : 16) + 16 + 8);
+        // pixels + t + colour state + resume position + opacity product + two
+        // ray-list entries per pixel
+        return (uint64_t)cfg.max_views * cfg.slot_pixels * (4 + 4 + (cfg.precision == 64 ? 32 : 16) + 16 + 8 + 8);
     }
     struct Place {
         Volume *v;

@@ -75,7 +75,7 @@ def test_server_results_alone_and_in_mix(gpu, plain, budget_vols, prec, parity_l
     scenes = [Scene(d, 8, 1, b, max_requests=64) for d, b in VOLS]
     pool_bytes = [2048 * 8 ** 3 * b for _, b in VOLS]
     max_views, slot_pixels = 8, 128 * 96
-    slot_bytes = max_views * slot_pixels * (4 + 4 + (32 if prec == 64 else 16) + 16 + 8)
+    slot_bytes = max_views * slot_pixels * (4 + 4 + (32 if prec == 64 else 16) + 16 + 8 + 8)
     budget = sum(sorted(p + slot_bytes for p in pool_bytes)[-budget_vols:]) + 1
     srv = G.Server(budget, max_views=max_views, slot_pixels=slot_pixels, precision=prec, plain=plain)
     srcs = []
@@ -167,7 +167,7 @@ def test_server_evicts_only_idle_volumes_with_empty_queues(gpu):
     scenes = [Scene(d, 8, 1, b, max_requests=64) for d, b in VOLS[:2]]
     pool_bytes = [512 * 8 ** 3 * b for _, b in VOLS[:2]]
     max_views, slot_pixels = 2, 64 * 48
-    slot_bytes = max_views * slot_pixels * (4 + 4 + 16 + 16 + 8)
+    slot_bytes = max_views * slot_pixels * (4 + 4 + 16 + 16 + 8 + 8)
     budget = max(p + slot_bytes for p in pool_bytes) + 1
     srv = G.Server(budget, max_views=max_views, slot_pixels=slot_pixels, precision=32)
     srcs = [G.Source.pyramid(sc.vc, sc.levels) for sc in scenes]
