@@ -608,6 +608,24 @@ static inline f3 mulm(const float *m, f3 v)
               v.x * m[8] + v.y * m[9] + v.z * m[10]);
 }
 
+uint32_t or_compute_lod(const or_view *V, const float dleft[3], const float dup[3], float pdx, float pdy)
+{
+    /* compute_LOD (RayCasters.cu:551-567): level k is kept as soon as the
+     * next level's voxel spacing along either screen axis reaches the pixel
+     * footprint on that axis. */
+    const f3 l = mk(dleft[0], dleft[1], dleft[2]), u = mk(dup[0], dup[1], dup[2]);
+    int32_t k;
+    for (k = 0; (uint32_t)k < V->nb_lods - 1; ++k) {
+        const float *ns = V->inv_levels_size[k + 1];
+        const f3 a = mul(l, mk(ns[0], ns[1], ns[2]));
+        const f3 b = mul(u, mk(ns[0], ns[1], ns[2]));
+        const float sl = sqrtf(dot3(a, a));
+        const float su = sqrtf(dot3(b, b));
+        if (sl >= pdx * 1.0f || su >= pdy * 1.0f) break;
+    }
+    return (uint32_t)k;
+}
+
 typedef struct {
     or_cache *c;
     const or_view *v;
@@ -767,20 +785,8 @@ static void dvr_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
             tp.z = 1.0f - tp.z;
             const f3 np = mk(fmaxf(0.f, fminf(tp.x, 0.99f)), fmaxf(0.f, fminf(tp.y, 0.99f)),
                              fmaxf(0.f, fminf(tp.z, 0.99f)));
-            /* compute_LOD (RayCasters.cu:551-567) */
-            uint32_t lod;
-            {
-                int32_t k;
-                for (k = 0; (uint32_t)k < V->nb_lods - 1; ++k) {
-                    const float *ns = V->inv_levels_size[k + 1];
-                    const f3 a = mul(dleft, mk(ns[0], ns[1], ns[2]));
-                    const f3 b = mul(dup, mk(ns[0], ns[1], ns[2]));
-                    const float sl = sqrtf(dot3(a, a));
-                    const float su = sqrtf(dot3(b, b));
-                    if (sl >= pdx * 1.0f || su >= pdy * 1.0f) break;
-                }
-                lod = (uint32_t)k;
-            }
+            const float dl[3] = {dleft.x, dleft.y, dleft.z}, du[3] = {dup.x, dup.y, dup.z};
+            const uint32_t lod = or_compute_lod(V, dl, du, pdx, pdy);
             /* K_CacheManager::get_normalized (K_CacheManager.hpp:267-361) */
             const f3 pr = mul(np, mk(g->ior[lod][0], g->ior[lod][1], g->ior[lod][2]));
             const f3 pv = mul(pr, mk(g->rdv[lod][0], g->rdv[lod][1], g->rdv[lod][2]));

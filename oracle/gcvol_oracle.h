@@ -164,6 +164,9 @@ int or_sample(const or_cache *c, uint32_t level, const float pos[3], double *sum
 
 /* Host-side LOD tables (VolumeRayCaster.cu:136-161) for a [-1,1]^3 box. */
 void or_view_tables(or_view *v, const or_cfg *cfg);
+/* compute_LOD (RayCasters.cu:551-567): the level of a sample whose pixel
+ * footprint is (pdx, pdy) along the screen axes dleft, dup. */
+uint32_t or_compute_lod(const or_view *v, const float dleft[3], const float dup[3], float pdx, float pdy);
 
 /* Transfer-function samples at x = i/N, piecewise linear between control
  * points with implicit end points (TransferFunction.cpp:121-157).

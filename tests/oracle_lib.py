@@ -99,6 +99,8 @@ def lib():
         L.or_mrpd_index.restype = C.c_uint64
         L.or_mrpd_index.argtypes = [C.POINTER(OrGeom), u32, P]
         L.or_view_tables.argtypes = [C.POINTER(OrView), C.POINTER(OrCfg)]
+        L.or_compute_lod.restype = u32
+        L.or_compute_lod.argtypes = [C.POINTER(OrView), P, P, f32, f32]
         L.or_tf_samples.argtypes = [P, u32, u32, P]
         L.or_render_dvr.argtypes = [P, C.POINTER(OrView), P, P, P, C.c_int, C.c_int, P]
         L.or_render_dvr_rows.argtypes = [P, C.POINTER(OrView), P, P, P, C.c_int, C.c_int, u32, u32, P]

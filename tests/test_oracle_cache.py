@@ -10,9 +10,12 @@ Pins:
     permutation of the slots, Empty bricks map to (0,0,0,2), victims are the
     last non-empty-count LRU entries.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
+import oracle_lib as O
 from scenes import Scene
 
 
@@ -118,7 +121,10 @@ def test_random_trace_invariants(oracle, seed):
     c = sc.oracle_cache()
     rng = np.random.default_rng(seed)
     g = c.g
+    bs = sc.bs
+    cx, cy = sc.cache_size[0], sc.cache_size[1]
     loaded = 0
+    will_load = False     # the last update handed a request to the handler
     for step in range(40):
         ts = c.ts
         # a frame: random requests on valid cells, usage on some mapped slots
@@ -133,15 +139,38 @@ def test_random_trace_invariants(oracle, seed):
         u[rng.random(c.nslots) < 0.2] = ts
         pending = c.request_list(100000)
         lru_before = c.lru().copy()
+        asked_before, nne_before = c.asked()
+        usage = c.usage().copy()
         done = c.update()
         assert c.ts == ts + 1
-        asked, nne = c.asked()
         if done:
             assert len(pending) == 0
         _check_invariants(sc, c, sc.levels)
-        if len(asked) and not np.array_equal(c.lru()[:, :3], lru_before[:, :3]):
-            pass
-        loaded += nne
+        # LRU update: the slots used in this frame first, each group in its
+        # old order (CacheDevice.hpp:202-219)
+        slot = lru_before[:, 0] // bs + cx * (lru_before[:, 1] // bs + cy * (lru_before[:, 2] // bs))
+        used = usage[slot] == ts
+        part = np.concatenate([lru_before[used], lru_before[~used]])
+        after = c.lru()
+        assert np.array_equal(after[:, :3], part[:, :3]), step
+        if will_load and nne_before:
+            # load: the victims are the last nne entries, in order, and take
+            # the asked bricks' level and volume position
+            # (CacheManager.hpp:511-512,554-569)
+            N = c.nslots
+            assert np.array_equal(after[:N - nne_before], part[:N - nne_before]), step
+            for j in range(nne_before):
+                a = asked_before[j]
+                vp = np.zeros(3, np.float32)
+                O.lib().or_volume_position(C.byref(c.g), int(a[3]), O.ptr(np.ascontiguousarray(a[:3], np.uint32)),
+                                           O.ptr(vp))
+                assert after[N - nne_before + j, 3] == a[3], (step, j)
+                assert np.array_equal(after[N - nne_before + j, 4:7], vp.view(np.uint32)), (step, j)
+            loaded += nne_before
+            will_load = False
+        else:
+            assert np.array_equal(after, part), step
+            will_load = not done
     assert loaded > 0
 
 
