@@ -281,4 +281,12 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
                       uint32_t grid, hipStream_t stream);
 hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream);
 
+// Exact opacity threshold of a view (FP32 compositing) and the check that a
+// launch's thresholds are normal doubles (runtime.cpp).
+double opacity_threshold(float alpha_stop, float k0);
+void finish_exact(DvrLaunch &L, const DvrView *views, uint32_t n);
+// The shared fields of a ray-caster launch over a cache (device state,
+// timestamp, level tables), with the cache's stream, device and voxel bytes.
+int cache_launch_base(gc_cache *c, DvrLaunch &L, hipStream_t &stream, int &device, uint32_t &bytes);
+
 } // namespace gc

@@ -1057,7 +1057,7 @@ struct Frame {
 // Exact opacity threshold of a view for FP32 compositing: a ray continues
 // while its opacity product Q > (1 - alpha_stop)^(1/k0) (see kernels_dvr.hip,
 // opacity_factor); an alpha stop of 1 continues while Q > 0, above 1 always.
-static double opacity_threshold(float alpha_stop, float k0)
+double opacity_threshold(float alpha_stop, float k0)
 {
     const double a = (double)alpha_stop;
     if (a > 1.0) return -1.0;
@@ -1067,7 +1067,7 @@ static double opacity_threshold(float alpha_stop, float k0)
 
 // Exact opacity decisions hold when every level's exponent is a power-of-two
 // multiple of level 0's and every threshold stays a normal double.
-static void finish_exact(DvrLaunch &L, const DvrView *views, uint32_t n)
+void finish_exact(DvrLaunch &L, const DvrView *views, uint32_t n)
 {
     for (uint32_t i = 0; i < n; ++i)
         if (views[i].width && views[i].thr >= 0.0 && views[i].thr < 1e-290) L.exact_alpha = 0;
@@ -1119,6 +1119,8 @@ static void launch_base(const Cache &C, DvrLaunch &L)
     L.pipelined = rdv_one && C.g.mrpd_dim[0] < (1u << 24) &&
                   (uint64_t)C.g.mrpd_dim[1] * C.g.mrpd_dim[2] < (1u << 24);
 }
+
+int cache_launch_base(gc_cache *c, DvrLaunch &L, hipStream_t &stream, int &device, uint32_t &bytes);
 
 static void view_of(const gc_view &v, const float *d_tf, uint32_t tf_size, const Frame &F, float k0, DvrView &out)
 {
@@ -1399,6 +1401,19 @@ static int batch_kernel_ms(Batch &B, double &ms)
 struct gc_cache {
     gc::Cache impl;
 };
+
+namespace gc {
+int cache_launch_base(gc_cache *c, DvrLaunch &L, hipStream_t &stream, int &device, uint32_t &bytes)
+{
+    if (!c) return fail(GC_ERR_ARG, "null cache");
+    Cache &C = c->impl;
+    launch_base(C, L);
+    stream = C.stream;
+    device = C.device;
+    bytes = C.vc.encoded_bytes;
+    return GC_OK;
+}
+} // namespace gc
 struct gc_frame {
     gc::Frame impl;
 };

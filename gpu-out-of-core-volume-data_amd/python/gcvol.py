@@ -143,7 +143,7 @@ EXPORTS = [
     "gc_cache_set_ray_list_max",
     "gc_cache_busy", "gc_source_create_gpu_mip", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
     "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get", "gc_cache_counters_get",
-    "gc_source_create_synthetic_gpu", "gc_source_get_device",
+    "gc_source_create_synthetic_gpu", "gc_source_get_device", "gc_render_views_noalloc",
 ]
 
 _lib = None
@@ -215,6 +215,8 @@ def lib():
             "gc_server_poll": [P, C.POINTER(ServerResult), u32, C.POINTER(u32)],
             "gc_server_take_pixels": [P, u64, P], "gc_server_drain": [P],
             "gc_server_stats_get": [P, C.POINTER(ServerStats)],
+            "gc_render_views_noalloc": [P, C.POINTER(View), C.POINTER(vp), P, u32, C.POINTER(vp), C.POINTER(vp),
+                                        C.POINTER(vp), P, P],
         }
         missing = [n for n in EXPORTS if n not in sig]
         if missing:
@@ -712,6 +714,16 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def render_views_noalloc(cache, views, d_tfs, tf_sizes, d_pixels, d_tbufs, d_cstates, d_stats=None, stream=None):
+    """One launch over the views into caller device buffers (DeviceBuffer
+    lists), no device allocation (gc_render_views_noalloc)."""
+    n = len(views)
+    arr = lambda bufs: (vp * n)(*[b.p for b in bufs])
+    check(lib().gc_render_views_noalloc(cache.h, (View * n)(*views), arr(d_tfs), ptr(np.asarray(tf_sizes, np.uint32)),
+                                        n, arr(d_pixels), arr(d_tbufs), arr(d_cstates),
+                                        d_stats.p if d_stats is not None else None, stream))
 
 
 def set_ray_list_max(cache, n):

@@ -414,6 +414,25 @@ int gc_batch_render(gc_batch *b, const gc_view *views, const float *const *d_tfs
 int gc_batch_read_pixels(gc_batch *b, uint32_t slot, uint32_t *host);   /* W*H ABGR */
 int gc_batch_read_state(gc_batch *b, uint32_t slot, float *tbuf, void *cstate);
 
+/* One ray-caster launch over n_views (at most 64) views of a cache, written
+ * into caller buffers, with no device allocation.  No reference counterpart:
+ * it is for callers that cannot allocate device memory -- code under hipGraph
+ * capture, tight serving loops.  Per view i: d_pixels[i] (W*H uint32 ABGR),
+ * d_tbufs[i] (W*H float) and d_cstates[i] (W*H float4 at precision 32,
+ * double4 at 64) receive what gc_batch_begin + gc_batch_launch write for the
+ * same views: every view starts from an empty progressive state (the call
+ * clears t buffer and colour state first), rays that reach a missing brick
+ * stop and raise their request, no ray list is kept.  Precision, tile, march
+ * and refill threshold come from views[0].  d_stats (optional) accumulates.
+ * The launch's packet cursors and view table live in a per-device static
+ * scratch array of 256 B + 64 view records; more than 64 views is GC_ERR_ARG.
+ * Every call on a GPU shares that array: calls that overlap on one GPU (other
+ * host threads, other streams) race. */
+int gc_render_views_noalloc(gc_cache *c, const gc_view *views, const float *const *d_tfs,
+                            const uint32_t *tf_sizes, uint32_t n_views, uint32_t *const *d_pixels,
+                            float *const *d_tbufs, void *const *d_cstates, gc_render_stats *d_stats,
+                            void *stream);
+
 /* Launches that resume at most max_rays rays (the rays an earlier launch of
  * the frame stopped at a miss) march them one wave per ray, 64 samples per
  * step, instead of one lane per ray; 0 never does.  The results are the same
