@@ -143,7 +143,7 @@ EXPORTS = [
     "gc_cache_set_ray_list_max",
     "gc_cache_busy", "gc_source_create_gpu_mip", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
     "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get", "gc_cache_counters_get",
-    "gc_source_create_synthetic_gpu", "gc_source_get_device", "gc_render_views_noalloc",
+    "gc_source_create_synthetic_gpu", "gc_source_get_device", "gc_render_views_noalloc", "gc_cache_probe",
 ]
 
 _lib = None
@@ -217,6 +217,7 @@ def lib():
             "gc_server_stats_get": [P, C.POINTER(ServerStats)],
             "gc_render_views_noalloc": [P, C.POINTER(View), C.POINTER(vp), P, u32, C.POINTER(vp), C.POINTER(vp),
                                         C.POINTER(vp), P, P],
+            "gc_cache_probe": [P, u32, P, P, u32, P, P],
         }
         missing = [n for n in EXPORTS if n not in sig]
         if missing:
@@ -512,6 +513,18 @@ class Cache:
         out = np.zeros((d[2], d[1], d[0]), DT[self.vc.encoded_bytes])
         check(lib().gc_cache_read_pool(self.h, ptr(out)))
         return out
+
+    def probe(self, mode, pos, levels, init=0.0):
+        """Device API probes (mode 0 gc_get_normalized, 1 gc_get):
+        (values, statuses)."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        n = len(pos)
+        d_pos = DeviceBuffer.from_array(pos)
+        d_lv = DeviceBuffer.from_array(np.ascontiguousarray(levels, np.uint32))
+        d_val = DeviceBuffer.from_array(np.full(n, init, np.float32))
+        d_st = DeviceBuffer(4 * n)
+        check(lib().gc_cache_probe(self.h, mode, d_pos.p, d_lv.p, n, d_val.p, d_st.p))
+        return d_val.to_array(np.float32, (n,)), d_st.to_array(np.uint32, (n,))
 
     def asked(self):
         n, nne = u32(0), u32(0)

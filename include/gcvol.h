@@ -281,6 +281,17 @@ int gc_cache_read_pool(gc_cache *c, void *host);
 int gc_cache_read_asked(gc_cache *c, uint32_t *host4, uint32_t *nb_asked,
                         uint32_t *nb_non_empty);
 
+/* The device page-table API of include/gc_cache_device.h run over n probes
+ * (for tests and tools): probe i asks level d_levels[i] at position
+ * d_pos[3i..3i+2] (in [0,1)^3 of the level) with mode 0 gc_get_normalized
+ * (K_CacheManager::get_normalized, K_CacheManager.hpp:267-361) or mode 1
+ * gc_get (K_CacheManager::get, :164-262, coarser-level fallback), with their
+ * usage and request side effects at the cache's timestamp.  d_status[i]
+ * receives the VoxelStatus; d_values[i] the sample (mode 0) or the voxel
+ * (mode 1) as a float, left unchanged for Empty.  Synchronous. */
+int gc_cache_probe(gc_cache *c, uint32_t mode, const float *d_pos, const uint32_t *d_levels, uint32_t n,
+                   float *d_values, uint32_t *d_status);
+
 /* --------------------------------------------------------------------- DVR */
 
 /* Host-side per-view parameters of RayCastDVR (RayCasters.cu:352-370,

@@ -689,6 +689,51 @@ int or_sample(const or_cache *c, uint32_t lod, const float np[3], double *sum8,
     return 0;
 }
 
+int or_probe(or_cache *c, int mode, uint32_t level, const float pos[3], double *raw)
+{
+    /* mode 0: K_CacheManager::get_normalized (K_CacheManager.hpp:267-361);
+     * mode 1: K_CacheManager::get (:164-262), which on an Unmapped entry
+     * raises the request only for the asked level and tries the next coarser
+     * levels.  Mapped stamps the slot's usage (compute_element_position,
+     * K_CacheDevice.hpp:316-334); raw is the 2x2x2 texel sum (mode 0) or the
+     * texel at the element position (mode 1). */
+    const or_geom *g = &c->g;
+    const uint32_t asked = level, last = c->cfg.nb_levels - 1;
+    const uint32_t bs = c->cfg.brick_size, cov = c->cfg.covering, inner = bs - 2 * cov;
+    for (;;) {
+        float real[3], virt[3];
+        for (int a = 0; a < 3; ++a) {
+            real[a] = pos[a] * g->ior[level][a];
+            virt[a] = real[a] * g->rdv[level][a];
+        }
+        const uint32_t *e = c->mrpd + 4 * or_mrpd_index(g, level, virt);
+        if (e[3] == 1) {
+            const uint64_t ui = e[0] / bs + (uint64_t)c->cfg.cache_size[0] * (e[1] / bs + (uint64_t)c->cfg.cache_size[1] * (e[2] / bs));
+            __atomic_store_n(&c->usage[ui], c->ts, __ATOMIC_RELAXED);
+            uint32_t el[3];
+            for (int a = 0; a < 3; ++a) el[a] = cov + e[a] + ((uint32_t)(virt[a] * (float)g->virt_vox[level][a])) % inner;
+            if (mode == 0) {
+                *raw = sample_box(c, el[0], el[1], el[2], NULL);
+            } else {
+                const uint64_t p = el[0] + (uint64_t)g->pool_dim[0] * (el[1] + (uint64_t)g->pool_dim[1] * el[2]);
+                if (c->cfg.bytes == 1) *raw = c->pool[p];
+                else if (c->cfg.bytes == 2) *raw = ((const uint16_t *)c->pool)[p];
+                else *raw = ((const float *)c->pool)[p];
+            }
+            return 0;
+        }
+        if (e[3] == 2) return 2;
+        if (mode == 0 || level == asked)
+            __atomic_store_n(&c->req[or_mrpd_index(g, level, real)], c->ts, __ATOMIC_RELAXED);
+        if (mode == 1 && level < last) {
+            ++level;
+            continue;
+        }
+        *raw = 0.0;
+        return 1;
+    }
+}
+
 static inline uint32_t pack_abgr(double r, double g, double b, double a)
 {
     /* rgbaFloatToInt (RayCasters.cu:528-536) */

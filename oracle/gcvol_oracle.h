@@ -163,6 +163,12 @@ int or_sample(const or_cache *c, uint32_t level, const float pos[3], double *sum
               uint32_t brick[3], uint32_t voxel[3]);
 
 /* Host-side LOD tables (VolumeRayCaster.cu:136-161) for a [-1,1]^3 box. */
+/* The device page-table API with its side effects (usage stamp, request at
+ * the cache's timestamp): mode 0 get_normalized (K_CacheManager.hpp:267-361),
+ * mode 1 get with the coarser-level fallback (:164-262).  Returns 0 Mapped,
+ * 1 Unmapped, 2 Empty; raw: the 2x2x2 texel sum (mode 0) or the texel at the
+ * element position (mode 1), 0 for Unmapped. */
+int or_probe(or_cache *c, int mode, uint32_t level, const float pos[3], double *raw);
 void or_view_tables(or_view *v, const or_cfg *cfg);
 /* compute_LOD (RayCasters.cu:551-567): the level of a sample whose pixel
  * footprint is (pdx, pdy) along the screen axes dleft, dup. */

@@ -99,6 +99,8 @@ def lib():
         L.or_mrpd_index.restype = C.c_uint64
         L.or_mrpd_index.argtypes = [C.POINTER(OrGeom), u32, P]
         L.or_view_tables.argtypes = [C.POINTER(OrView), C.POINTER(OrCfg)]
+        L.or_probe.restype = C.c_int
+        L.or_probe.argtypes = [P, C.c_int, u32, P, C.POINTER(C.c_double)]
         L.or_compute_lod.restype = u32
         L.or_compute_lod.argtypes = [C.POINTER(OrView), P, P, f32, f32]
         L.or_tf_samples.argtypes = [P, u32, u32, P]
@@ -265,6 +267,12 @@ class Cache:
         v = np.zeros(3, np.uint32)
         st = lib().or_sample(self.h, level, ptr(np.asarray(pos, np.float32)), C.byref(s), ptr(b), ptr(v))
         return st, s.value, b, v
+
+    def probe(self, mode, level, pos):
+        """Device API probe with side effects: (status, raw value)."""
+        raw = C.c_double(0.0)
+        st = lib().or_probe(self.h, mode, level, ptr(np.ascontiguousarray(pos, np.float32)), C.byref(raw))
+        return int(st), raw.value
 
     def render(self, view, tbuf, cstate, pixels, precision=64, nthreads=1, rpos=None):
         """One launch; rpos (W*H*4 float32) selects exact continuation."""

@@ -24,7 +24,7 @@ class Scene:
     """A volume, its pyramid, matching oracle and product configurations."""
 
     def __init__(self, dims, bs, cov, bytes_, cache_size=None, max_requests=50, seed=99, big_brick=(1, 1, 1),
-                 **blob_kw):
+                 empty_threshold=0.0, **blob_kw):
         self.dims, self.bs, self.cov, self.bytes = dims, bs, cov, bytes_
         self.vc = G.volume_cfg(dims, bs, cov, bytes_, big_brick)
         self.initial = self.vc.initial()
@@ -38,9 +38,11 @@ class Scene:
         self.cache_size = tuple(int(c) for c in cache_size)
         self.max_requests = max_requests
         self.big_brick = big_brick
+        self.empty_threshold = empty_threshold
         self.ocfg = O.make_cfg(self.initial, self.real, bs, cov, bytes_, self.cache_size, bb=big_brick,
-                               max_requests=max_requests)
-        self.cc = G.cache_cfg(self.cache_size, bs, max_requests=max_requests, sync=True)
+                               max_requests=max_requests, empty_threshold=empty_threshold)
+        self.cc = G.cache_cfg(self.cache_size, bs, max_requests=max_requests, sync=True,
+                              empty_threshold=empty_threshold)
 
     def oracle_cache(self):
         return O.Cache(self.ocfg, self.levels)
