@@ -11,8 +11,17 @@ termination at alpha 0.99.
 
 A step is one frame: the plain single-view renderer's progressive loop
 (launch, update the cache, load the requested bricks, relaunch) until the
-view is complete.  `value` is ray samples per second over the timed frames,
-wall clock, including the on-demand brick synthesis and upload.
+view is complete.  The K timed frames are spread evenly over the 120-frame
+path (frame j of K is path frame j*120/K), so they cover the far start and
+the close-up end; the W warm-up frames are the path's first W frames.
+`value` is ray samples per second over the timed frames, wall clock,
+including the on-demand brick synthesis and upload.
+
+Beside the headline line, the same JSON carries: the plain-method
+configuration (256^3 u8 blobs, resident, 36-frame 512^2 orbit) on the GPU
+and on the host cores (bricker + integer mip + FP64 caster), the batched vs
+plain renderer on 64 uneven views, and a bounded serving run at 50, 200 and
+800 requests/s (batched serving loop vs one request at a time).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-traffic]
 
@@ -63,6 +72,14 @@ def fly_through(k):
     return G.camera(rx, ry, dist)
 
 
+def path_frame(k, warmup, steps):
+    """Path frame of run frame k: warm-up frames are the path's first
+    frames, timed frame j of `steps` is path frame j * 120 / steps."""
+    if k < warmup:
+        return k % PATH_FRAMES
+    return ((k - warmup) * PATH_FRAMES // max(1, steps)) % PATH_FRAMES
+
+
 def algorithmic_bytes(stats, bytes_per_voxel):
     """B = S*8*sizeof(T) + V*(16+4) + P*44 (box footprint per mapped sample,
     page entry + usage stamp per brick visit, RGBA8 + t read/write + colour
@@ -79,11 +96,14 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-def cpu_baseline(G, cache, vc, cam, tf, seconds=12.0):
+def cpu_baseline(G, cache, vc, cam, tf, seconds=12.0, device_pixels=None):
     """The oracle's FP64 host ray caster (test infrastructure) on the host
-    cores, over the device cache state of the last timed frame: the same
-    page directory and pool, the same 1024x1024 view.  Rows are rendered in
-    bands until about `seconds` of CPU work; samples/s is reported."""
+    cores, over the device cache state right after the first timed frame
+    (path frame 0, whose bricks all fit in the pool, so the state holds every
+    brick the view needs): the same page directory and pool, the same
+    1024x1024 view.  Rows are rendered in bands until about `seconds` of CPU
+    work; samples/s is reported, and the rendered rows' largest RGBA
+    difference from the device frame."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
 
@@ -97,7 +117,8 @@ def cpu_baseline(G, cache, vc, cam, tf, seconds=12.0):
     G.check(G.lib().gc_cache_read_pool(cache.h, O.lib().or_cache_pool(oc.h)))
     band = 64
     rows_done = 0
-    samples = 0
+    samples = unmapped = 0
+    maxdiff = 0
     t0 = time.perf_counter()
     while rows_done < HEIGHT and time.perf_counter() - t0 < seconds:
         h = min(band, HEIGHT - rows_done)
@@ -110,12 +131,19 @@ def cpu_baseline(G, cache, vc, cam, tf, seconds=12.0):
         px = np.zeros(WIDTH * HEIGHT, np.uint32)
         st = render_rows(O, oc, view, tb, cs, px, rows_done, h, cores)
         samples += int(st[0])
+        unmapped += int(st[3])
+        if device_pixels is not None:
+            a = px[rows_done * WIDTH:(rows_done + h) * WIDTH]
+            b = device_pixels[rows_done * WIDTH:(rows_done + h) * WIDTH]
+            maxdiff = max(maxdiff, int(np.abs(G.unpack_rgba(a) - G.unpack_rgba(b)).max()))
         rows_done += h
     dt = time.perf_counter() - t0
     oc.close()
     return {"value": samples / dt, "unit": "samples/s", "cores": cores, "kind": "port",
-            "sample": "FP64 oracle ray caster, rows 0-%d of the last timed 1024x1024 view over the device "
-                      "cache state (%.1f s, %d samples)" % (rows_done, dt, samples)}
+            "sample": "FP64 oracle ray caster, rows 0-%d of the first timed 1024x1024 view (path frame 0) over the "
+                      "device cache state after it (%.1f s, %d samples, %d rays stopped at a missing brick)"
+                      % (rows_done, dt, samples, unmapped),
+            "max_rgba_diff_vs_device_frame": maxdiff if device_pixels is not None else None}
 
 
 def render_rows(O, oc, view, tb, cs, px, y0, h, threads):
@@ -130,6 +158,164 @@ def render_rows(O, oc, view, tb, cs, px, y0, h, threads):
 
 
 CALIB_MIB = 256
+TILE_NAMES = ["8x8", "16x4", "32x2", "64x1", "16x8", "32x4"]
+
+# plain-method configuration: 256^3 u8, 12 blobs + noise, 16^3 bricks, resident
+C1_DIMS = (256, 256, 256)
+C1_BRICK = 16
+C1_SIZE = 512
+C1_FRAMES = 36
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    return O
+
+
+def cpu_mip_and_bricks(O, vol, nlev, bs, cov, cores):
+    """The CPU preprocessor on `cores` threads: integer 2x2x2 mips level after
+    level (z-slabs of a level in parallel) and the bricker's cut of every level
+    (brick ranges in parallel).  Returns (levels, mip seconds, brick seconds,
+    bricks)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    levels = [vol]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        for _ in range(1, nlev):
+            src = levels[-1]
+            Z, Y, X = src.shape
+            oz = Z // 2
+            out = np.zeros((oz, Y // 2, X // 2), src.dtype)
+            step = max(1, (oz + cores - 1) // cores)
+
+            def slab(z0, src=src, out=out):
+                z1 = min(oz, z0 + step)
+                out[z0:z1] = O.mip_reduce(src[2 * z0:2 * z1])
+
+            list(ex.map(slab, range(0, oz, step)))
+            levels.append(out)
+        t_mip = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        inner = bs - 2 * cov
+        bricks = 0
+        jobs = []
+        for lv in levels:
+            nb = [(d + inner - 1) // inner for d in (lv.shape[2], lv.shape[1], lv.shape[0])]
+            n = nb[0] * nb[1] * nb[2]
+            bricks += n
+            step = max(1, (n + cores - 1) // cores)
+            jobs += [(lv, b0, min(n, b0 + step)) for b0 in range(0, n, step)]
+        list(ex.map(lambda j: O.brick_level(j[0], bs, cov, j[1], j[2]), jobs))
+        t_brick = time.perf_counter() - t0
+    return levels, t_mip, t_brick, bricks
+
+
+def plain_config_workload(G, cpu_frames=2):
+    """The plain-method configuration: a 256^3 u8 volume of 12 seeded blobs
+    (sigma 8-40, amplitude 60-255) plus noise +-8, 16^3 bricks with covering
+    1, 5 levels, fully resident, 512x512 views on the 36-frame orbit (10
+    degrees per frame, distance 3), alpha stop 0.99.  GPU: mips and brick
+    cuts on the device, the resident load, 36 one-launch frames (FP32
+    compositing).  CPU (the oracle, test infrastructure, on all host cores):
+    the integer mips, the bricker's cut of every level and the FP64 caster on
+    `cpu_frames` of the frames, whose pixels are also compared with the GPU's."""
+    O = _oracle()
+    cores = os.cpu_count() or 1
+    params = G.synth_params(SEED + 1, 12, sigma=(8.0, 40.0), amp=(60.0, 255.0), noise=8, threads=0)
+    vol = G.synth_volume(params, C1_DIMS, 1)
+    vc = G.volume_cfg(C1_DIMS, C1_BRICK, 1, 1)
+    nl = vc.nb_levels
+    # GPU preprocessor: mips, then every level's bricks
+    d_levels, dims = [G.DeviceBuffer.from_array(vol)], [C1_DIMS]
+    G.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(1, nl):
+        nd = tuple(d // 2 for d in dims[-1])
+        d_out = G.DeviceBuffer(max(16, nd[0] * nd[1] * nd[2]))
+        G.mip_reduce(d_levels[-1], dims[-1], 1, d_out)
+        d_levels.append(d_out)
+        dims.append(nd)
+    G.synchronize()
+    gpu_mip_ms = (time.perf_counter() - t0) * 1e3
+    inner = C1_BRICK - 2
+    lists, outs, counts = [], [], []
+    for d in dims:
+        nb = [(x + inner - 1) // inner for x in d]
+        br = np.array([[x, y, z] for z in range(nb[2]) for y in range(nb[1]) for x in range(nb[0])], np.uint32)
+        lists.append(G.DeviceBuffer.from_array(br))
+        outs.append(G.DeviceBuffer(len(br) * C1_BRICK ** 3))
+        counts.append(len(br))
+    total = sum(counts)
+    G.synchronize()
+    t0 = time.perf_counter()
+    for l, d in enumerate(dims):
+        G.check(G.lib().gc_brick_cut(d_levels[l].p, G.ptr(np.asarray(d, np.uint32)), 1, C1_BRICK, 1, lists[l].p,
+                                     counts[l], outs[l].p, None))
+    G.synchronize()
+    gpu_brick_ms = (time.perf_counter() - t0) * 1e3
+    for b in lists + outs:
+        b.free()
+    host_levels = [d_levels[l].to_array(np.uint8, (dims[l][2], dims[l][1], dims[l][0])) for l in range(nl)]
+    cs = G.pool_shape(total * C1_BRICK ** 3, C1_BRICK ** 3)
+    cache = G.Cache(vc, G.cache_cfg(cs, C1_BRICK, max_requests=64, sync=True), G.Source.pyramid(vc, host_levels))
+    G.synchronize()
+    t0 = time.perf_counter()
+    cache.load_all(d_levels)
+    G.synchronize()
+    load_ms = (time.perf_counter() - t0) * 1e3
+    tf = G.tf_samples(G.tf_seeded(SEED + 1, 6), 256)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    frame = G.Frame(C1_SIZE, C1_SIZE, 32)
+    d_st = G.DeviceBuffer.from_array(np.zeros(4, np.uint64))
+    lat, gpx = [], {}
+    for k in range(C1_FRAMES):
+        cam = G.camera(20.0, 10.0 * k, 3.0)
+        view = G.make_view(C1_SIZE, C1_SIZE, cam, alpha_stop=ALPHA_STOP, precision=32)
+        G.synchronize()
+        t0 = time.perf_counter()
+        frame.reset()
+        G.render(cache, view, d_tf, len(tf), frame, d_st)
+        G.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+        if k < cpu_frames:
+            gpx[k] = frame.pixels().reshape(-1).copy()
+    samples = int(d_st.to_array(np.uint64, (4,))[0])
+    frame.close()
+    cache.close()
+    for b in d_levels:
+        b.free()
+    gpu = {"mip_ms": gpu_mip_ms, "brick_cut_ms": gpu_brick_ms, "load_ms": load_ms,
+           "frames_per_s": C1_FRAMES / (sum(lat) / 1e3), "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+           "samples_per_s": samples / (sum(lat) / 1e3)}
+    # CPU: preprocessor and FP64 caster
+    levels, t_mip, t_brick, bricks = cpu_mip_and_bricks(O, vol, nl, C1_BRICK, 1, cores)
+    ocfg = O.make_cfg(vc.initial(), vc.real(), C1_BRICK, 1, 1, cs)
+    oc = O.Cache(ocfg, levels)
+    oc.load_all()
+    errs, csamp = [], 0
+    t0 = time.perf_counter()
+    for k in range(cpu_frames):
+        view = O.make_view(ocfg, C1_SIZE, C1_SIZE, G.camera(20.0, 10.0 * k, 3.0), tf, alpha_stop=ALPHA_STOP)
+        px = np.zeros(C1_SIZE * C1_SIZE, np.uint32)
+        st = oc.render(view, np.zeros(C1_SIZE * C1_SIZE, np.float32), np.zeros(C1_SIZE * C1_SIZE * 4), px,
+                       precision=64, nthreads=cores)
+        csamp += int(st[0])
+        errs.append(int(np.abs(G.unpack_rgba(px) - G.unpack_rgba(gpx[k])).max()))
+    t_cast = time.perf_counter() - t0
+    oc.close()
+    mip_vox = sum(l.size for l in levels[1:])
+    cpu = {"cores": cores, "mip_ms": t_mip * 1e3, "bricker_ms": t_brick * 1e3, "bricks": bricks,
+           "frames_per_s": cpu_frames / t_cast, "samples_per_s": csamp / t_cast, "frames_timed": cpu_frames,
+           "mip_voxels_per_s": mip_vox / t_mip, "bricker_bricks_per_s": bricks / t_brick}
+    print("plain config: GPU %.1f frames/s (mip %.1f ms, cut %.1f ms, load %.1f ms); CPU %d cores %.2f frames/s "
+          "(mip %.0f ms, bricker %.0f ms); max RGBA diff %s" % (gpu["frames_per_s"], gpu_mip_ms, gpu_brick_ms, load_ms,
+                                                                cores, cpu["frames_per_s"], cpu["mip_ms"],
+                                                                cpu["bricker_ms"], errs), file=sys.stderr, flush=True)
+    return {"workload": "256^3 u8, 12 blobs + noise, 16^3 bricks, covering 1, 5 levels, resident; 512x512, 36-frame "
+                        "orbit, alpha stop 0.99; GPU FP32 compositing, CPU FP64 caster on %d frames" % cpu_frames,
+            "gpu": gpu, "cpu": cpu, "max_rgba_diff_gpu_vs_cpu": max(errs) if errs else None}
 
 # batching workload: 64 uneven views of one 1024^3 u8 volume
 BATCH_DIMS = (1024, 1024, 1024)
@@ -146,7 +332,7 @@ SERVE_BUDGET = 96 << 30
 
 
 def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm_requests=24, fixed_ab=False,
-                     volumes=None, max_requests=16384, device_source=False):
+                     volumes=None, max_requests=16384, device_source=False, count=0, tune_upload=False):
     """Requests arrive as a Poisson process (seeded) over 6 volumes (512^3 u8,
     1024^3 u8, 1024^3 u16, 2048^3 u8, 2048x2048x1024 u16, 4096^3 u8, each
     synthesised brick by brick on the host on demand, 32^3 bricks, 12 GiB
@@ -159,7 +345,11 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
     upload batch; 16384 measured best against 4096 for both loops).  Latency is
     submission to pixels on the host.  With `device_source`, the bricks are
     synthesised on the GPU straight into each cache's staging buffer (same
-    bytes, no host arithmetic, no PCIe upload)."""
+    bytes, no host arithmetic, no PCIe upload).  With `count`, every rate
+    serves that many requests (arrivals over count / rate seconds) instead of
+    `seconds` of arrivals, which bounds the run at rates the GPU cannot
+    sustain.  Requests resume stopped rays by exact continuation, so every
+    result equals its one-pass render."""
     import threading
 
     tables = [G.tf_samples(G.tf_seeded(SEED + 200 + i, 6), 256) for i in range(8)]
@@ -171,13 +361,30 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
                                 noise=8 if nb == 1 else 2048, threads=0)
         sources.append(G.Source.synthetic(vc, params, device=0 if device_source else None))
         vcs.append(vc)
+    d_tf0 = G.DeviceBuffer.from_array(tables[0])
 
-    def make_requests(rate, seed):
+    upload_table = None
+    if tune_upload:
+        # the upload batch, autotuned on cold frames of the 1024^3 u8 volume
+        import autotune
+
+        def make_cache():
+            vc1 = vcs[1]
+            cc = G.cache_cfg(G.pool_shape(SERVE_POOL, 32 ** 3), 32, max_requests=16384, sync=False)
+            return G.Cache(vc1, cc, sources[1])
+
+        fr = [(G.make_view(side, side, G.camera(15.0, a, 2.6), alpha_stop=ALPHA_STOP), d_tf0, 256)
+              for side, a in ((1024, 30.0), (512, 160.0))]
+        up = autotune.tune_upload(make_cache, fr, candidates=(4096, 16384))
+        max_requests, upload_table = up["max_requests"], up["table"]
+        print("serving: autotuned upload batch %d bricks" % max_requests, file=sys.stderr, flush=True)
+
+    def make_requests(rate, seed, n=0):
         rng = np.random.default_rng(seed)
         t, out = 0.0, []
         while True:
             t += float(rng.exponential(1.0 / rate))
-            if t >= seconds:
+            if (n and len(out) >= n) or (not n and t >= seconds):
                 return out
             side = int(rng.integers(4, 33)) * 64
             cam = G.camera(float(rng.uniform(-30.0, 30.0)), float(rng.uniform(0.0, 360.0)),
@@ -193,7 +400,7 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
         vids = [srv.add_volume(vc, src, SERVE_POOL, max_requests=max_requests) for vc, src in zip(vcs, sources)]
         # warm: an untimed burst from the same request distribution
         for (_, vol, side, cam, tfi) in warm:
-            srv.submit(vids[vol], G.make_view(side, side, cam, alpha_stop=ALPHA_STOP), tables[tfi])
+            srv.submit(vids[vol], G.make_view(side, side, cam, alpha_stop=ALPHA_STOP, resume=1), tables[tfi])
         srv.drain()
         while True:
             rs = srv.poll(256)
@@ -211,7 +418,7 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
                 d = t_start + t - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                rid = srv.submit(vids[vol], G.make_view(side, side, cam, alpha_stop=ALPHA_STOP), tables[tfi])
+                rid = srv.submit(vids[vol], G.make_view(side, side, cam, alpha_stop=ALPHA_STOP, resume=1), tables[tfi])
                 sizes[rid] = (side, side)
 
         th = threading.Thread(target=feeder)
@@ -244,7 +451,7 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
     warm = warm[:warm_requests]
     out = []
     for i, rate in enumerate(rates):
-        reqs = make_requests(rate, SEED + 300 + i)
+        reqs = make_requests(rate, SEED + 300 + i, count)
         b = serve(reqs, False)
         bf = serve(reqs, False, fixed=True) if fixed_ab else None
         pl = serve(reqs, True)
@@ -260,12 +467,25 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
         out.append({"rate": rate, "requests": len(reqs), "batched": b, "plain": pl})
         if bf:
             out[-1]["batched_fixed_views"] = bf
-    return {"workload": "Poisson arrivals for %.0f s per rate over 6 volumes synthesised on demand %s(512^3 u8, "
-                        "1024^3 u8, 1024^3 u16, 2048^3 u8, 2048x2048x1024 u16, 4096^3 u8; 32^3 bricks, 12 GiB "
-                        "pool each, 96 GiB budget), square views 256..2048, own camera and TF per request; "
-                        "batched = serving loop (<= %d views per volume per launch), plain = one request at a time; "
-                        "%d bricks per update" % (seconds, "on the GPU " if device_source else "on the host ",
-                                                  max_views, max_requests), "rates": out}
+    arrivals = ("%d requests per rate" % count) if count else ("Poisson arrivals for %.0f s per rate" % seconds)
+    res = {"workload": "%s over 6 volumes synthesised on demand %s(512^3 u8, 1024^3 u8, 1024^3 u16, 2048^3 u8, "
+                       "2048x2048x1024 u16, 4096^3 u8; 32^3 bricks, 12 GiB pool each within one 96 GiB device budget "
+                       "that admits a volume's pool while it fits), square views 256..2048, own camera and TF per "
+                       "request; batched = serving loop (<= %d views per volume per launch), plain = one request at "
+                       "a time; %d bricks per update" % (arrivals, "on the GPU " if device_source else "on the host ",
+                                                         max_views, max_requests), "rates": out}
+    # frames/s of each loop at equal p99: the batched loop's throughput at the
+    # highest rate whose p99 does not exceed the plain loop's p99 at that rate
+    ratios = [r["batched"]["frames_per_s"] / r["plain"]["frames_per_s"] for r in out if r["plain"]["frames_per_s"] > 0]
+    res["batched_over_plain_frames_per_s"] = ratios
+    if upload_table:
+        res["upload_autotune"] = upload_table
+    eq = [r for r in out if r["batched"]["p99_ms"] <= r["plain"]["p99_ms"]]
+    if eq:
+        best = max(eq, key=lambda r: r["batched"]["frames_per_s"] / r["plain"]["frames_per_s"])
+        res["at_equal_or_lower_p99"] = {"rate": best["rate"],
+                                        "speedup": best["batched"]["frames_per_s"] / best["plain"]["frames_per_s"]}
+    return res
 
 
 # on-demand processing: a virtual volume larger than HBM, coarse levels built
@@ -328,9 +548,11 @@ def batch_workload(G, reps=3):
     """Batched vs plain rendering of 64 concurrent views (sizes uniform among
     256^2, 640x480, 1024^2, 1920x1080, seeded cameras, 8 seeded 256-entry
     transfer functions) over one 1024^3 u8 blob volume synthesised on demand
-    (16^3 bricks, covering 1, 4 GiB pool).  The batch is first rendered cold
-    (loading its bricks), then the autotuner picks tile x samples-per-check
-    on the warm batch, then: batched = one gc_batch_render per batch (hipGraph
+    (16^3 bricks, covering 1, 4 GiB pool).  The autotuner first picks the
+    upload batch (bricks per update) on cold renders of 8 of the views, then
+    the batch is rendered cold (loading its bricks), then the autotuner picks
+    the ray tile (8x8, 16x8, 32x4, 64x1) x samples-per-check (1-4) on the
+    warm batch, then: batched = one gc_batch_render per batch (hipGraph
     replay), plain = gc_render_frame per view, one view per launch, in request
     order.  All 64 requests arrive together, so a view's latency is the time
     until its own frame is ready."""
@@ -341,7 +563,10 @@ def batch_workload(G, reps=3):
     params = G.synth_params(SEED + 3, 32, sigma=(10.0, 60.0), amp=(60.0, 255.0), noise=8, threads=0)
     src = G.Source.synthetic(vc, params)
     cs = G.pool_shape(4 << 30, BATCH_BRICK ** 3)
-    cache = G.Cache(vc, G.cache_cfg(cs, BATCH_BRICK, max_requests=16384, sync=False), src)
+
+    def make_cache():
+        return G.Cache(vc, G.cache_cfg(cs, BATCH_BRICK, max_requests=16384, sync=False), src)
+
     tables = [G.tf_samples(G.tf_seeded(SEED + 100 + i, 6), 256) for i in range(8)]
     d_tfs = [G.DeviceBuffer.from_array(t) for t in tables]
     views, vt = [], []
@@ -352,13 +577,18 @@ def batch_workload(G, reps=3):
         vt.append(d_tfs[i % len(d_tfs)])
     sizes = [256] * BATCH_VIEWS
     npix = sum(v.width * v.height for v in views)
+    upload = autotune.tune_upload(make_cache, [(views[i], vt[i], 256) for i in range(0, BATCH_VIEWS, 8)],
+                                  candidates=(2048, 4096, 8192, 16384))
+    print("batch: autotuned upload batch %d bricks" % upload["max_requests"], file=sys.stderr, flush=True)
+    cache = make_cache()
+    G.set_max_requests(cache, upload["max_requests"])
     b = G.Batch(cache, BATCH_VIEWS, npix, 32)
     t0 = time.perf_counter()
     cold = b.render(views, vt, sizes)
     cold_ms = (time.perf_counter() - t0) * 1e3
     print("batch: cold render %.0f ms, %d launches, %d bricks" % (cold_ms, cold.launches, cold.bricks_loaded),
           file=sys.stderr, flush=True)
-    tune = autotune.tune_launch(cache, views, vt, sizes, marches=(1, 2, 4), reps=2)
+    tune = autotune.tune_launch(cache, views, vt, sizes, tiles=(0, 4, 5, 3), marches=(1, 2, 3, 4), reps=2)
     print("batch: autotuned tile %d march %d" % (tune["tile"], tune["march"]), file=sys.stderr, flush=True)
     for v in views:
         v.tile, v.march = tune["tile"], tune["march"]
@@ -397,7 +627,8 @@ def batch_workload(G, reps=3):
         "batched_samples_per_s": bs / (bmed / 1e3), "plain_samples_per_s": ps / (pmed / 1e3),
         "batched_p50_ms": bmed, "batched_p99_ms": percentile(bt, 99),
         "plain_p50_ms": percentile(plat, 50), "plain_p99_ms": percentile(plat, 99),
-        "tile": ["8x8", "16x4", "32x2", "64x1"][tune["tile"]], "march": tune["march"],
+        "tile": TILE_NAMES[tune["tile"]], "march": tune["march"], "upload_batch": upload["max_requests"],
+        "upload_autotune": upload["table"],
         "autotune": [{"tile": t["tile"], "march": t["march"], "kernel_ms": round(t["kernel_ms"], 2)}
                      for t in tune["table"]],
         "cold_ms": cold_ms,
@@ -410,17 +641,17 @@ def _pmc_pass(counters, outdir, args):
     values after the first frame}, {counter: copy kernel value})."""
     tag = counters[0]
     cmd = ["rocprofv3", "--pmc"] + list(counters) + ["-d", outdir, "-o", tag, "--", sys.executable,
-           os.path.abspath(__file__), "--steps", "3", "--warmup", "1", "--no-cpu-baseline", "--no-traffic", "--no-batch",
-           "--calib-copy-mib", str(CALIB_MIB), "--max-requests", str(args.max_requests), "--tile", str(args.tile),
-           "--precision", str(args.precision)]
+           os.path.abspath(__file__), "--steps", str(args.steps), "--warmup", str(args.warmup), "--no-cpu-baseline",
+           "--no-traffic", "--no-extras", "--calib-copy-mib", str(CALIB_MIB), "--max-requests", str(args.max_requests),
+           "--tile", str(args.tile), "--precision", str(args.precision)]
     env = dict(os.environ, TMPDIR="/tmp")
     r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=300)
     if r.returncode != 0:
         raise RuntimeError("rocprofv3 %s pass failed (%d): %s" % (tag, r.returncode, r.stderr[-400:]))
-    first_launches = None   # k_dvr launches of the cold first frame
+    first_launches = None   # k_dvr launches of the warm-up frames (the timed frames' follow)
     for line in r.stderr.splitlines():
-        if line.startswith("frame   0 "):
-            first_launches = int(line.split(" lane ")[1].split()[0])
+        if line.startswith("frame ") and " warmup " in line:
+            first_launches = (first_launches or 0) + int(line.split(" lane ")[1].split()[0])
     dbs = [os.path.join(dp, f) for dp, _, fs in os.walk(outdir) for f in fs if f.endswith(".db") and tag in f]
     if not dbs or first_launches is None:
         raise RuntimeError("rocprofv3 %s pass left no results" % tag)
@@ -461,14 +692,14 @@ def measure_traffic(args):
                   file=sys.stderr, flush=True)
             out[counter] = (sum(v) / len(v) * scale * 1024.0, scale, len(v))
         total = out["FETCH_SIZE"][0] + out["WRITE_SIZE"][0]
-        note = ("FETCH_SIZE x%.3f + WRITE_SIZE x%.3f (copy calibration, %d MiB), mean over %d launches of frames 1-3"
-                % (out["FETCH_SIZE"][1], out["WRITE_SIZE"][1], CALIB_MIB, out["FETCH_SIZE"][2]))
+        note = ("FETCH_SIZE x%.3f + WRITE_SIZE x%.3f (copy calibration, %d MiB), mean over the %d k_dvr launches of "
+                "the timed frames" % (out["FETCH_SIZE"][1], out["WRITE_SIZE"][1], CALIB_MIB, out["FETCH_SIZE"][2]))
         t0 = time.perf_counter()
         vals, _ = _pmc_pass(["TCC_HIT_sum", "TCC_MISS_sum"], os.path.join(tmp, "TCC"), args)
         hit, miss = sum(vals["TCC_HIT_sum"]), sum(vals["TCC_MISS_sum"])
         l2 = {"hit_rate": hit / (hit + miss) if hit + miss > 0 else None,
               "hits_per_launch": hit / len(vals["TCC_HIT_sum"]), "misses_per_launch": miss / len(vals["TCC_MISS_sum"]),
-              "note": "TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum) over the ray-caster launches of frames 1-3"}
+              "note": "TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum) over the k_dvr launches of the timed frames"}
         print("traffic pass TCC: %d launches, L2 hit rate %.3f, %.1f s" % (len(vals["TCC_HIT_sum"]), l2["hit_rate"] or 0,
                                                                          time.perf_counter() - t0),
               file=sys.stderr, flush=True)
@@ -484,13 +715,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--max-requests", type=int, default=4096)
+    ap.add_argument("--max-requests", type=int, default=0,
+                    help="bricks per update (0: picked by the autotuner on cold frames of the path)")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain workload")
-    ap.add_argument("--serving", action="store_true", help="also run the serving workload (several minutes)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the bounded serving leg")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline workload (no plain-config, batch or serving legs)")
+    ap.add_argument("--serving", action="store_true",
+                    help="also run the long serving workload (host-synthesised bricks, several minutes)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--ondemand-only", action="store_true",
                     help="run only the on-demand workload (virtual 8192x8192x4096 u8, mip on miss)")
@@ -511,10 +747,6 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
-
-    traffic, traffic_note, l2 = ((None, "not measured", None) if (args.no_traffic or args.serving_only or
-                                                                 args.ondemand_only)
-                                 else measure_traffic(args))
 
     import gcvol as G
 
@@ -547,17 +779,35 @@ def main():
     src = G.Source.synthetic(vc, params)
     brick_bytes = BRICK ** 3 * BYTES
     cs = G.pool_shape(POOL_BYTES if args.pool_gib <= 0 else int(args.pool_gib * (1 << 30)), brick_bytes)
-    cc = G.cache_cfg(cs, BRICK, max_requests=args.max_requests, sync=False)
-    cache = G.Cache(vc, cc, src)
     tf = G.tf_samples(G.tf_seeded(SEED, 6), 256)
     d_tf = G.DeviceBuffer.from_array(tf)
+    upload_tune = None
+    if args.max_requests <= 0:
+        # the upload batch, picked by the autotuner on three cold frames of
+        # the path (fresh caches), before anything is timed
+        import autotune
+
+        def make_cache():
+            return G.Cache(vc, G.cache_cfg(cs, BRICK, max_requests=16384, sync=False), src)
+
+        fr = [(G.make_view(WIDTH, HEIGHT, fly_through(pf), alpha_stop=ALPHA_STOP, precision=args.precision,
+                           tile=args.tile), d_tf, len(tf)) for pf in (0, 54, 108)]
+        upload_tune = autotune.tune_upload(make_cache, fr, candidates=(2048, 4096, 8192, 16384))
+        args.max_requests = upload_tune["max_requests"]
+        print("autotuned upload batch: %d bricks per update %s" % (args.max_requests, upload_tune["table"]),
+              file=sys.stderr, flush=True)
+    # counters of the timed frames' ray-caster launches, from child runs of
+    # this benchmark with the same frames and upload batch
+    traffic, traffic_note, l2 = (None, "not measured", None) if args.no_traffic else measure_traffic(args)
+    cc = G.cache_cfg(cs, BRICK, max_requests=args.max_requests, sync=False)
+    cache = G.Cache(vc, cc, src)
     frame = G.Frame(WIDTH, HEIGHT, args.precision)
 
     nframes = args.warmup + args.steps
     lat, reports = [], []
     cams = []
     for k in range(nframes):
-        cam = fly_through(k % PATH_FRAMES)
+        cam = fly_through(path_frame(k, args.warmup, args.steps))
         cams.append(cam)
         view = G.make_view(WIDTH, HEIGHT, cam, alpha_stop=ALPHA_STOP, precision=args.precision, tile=args.tile)
         G.synchronize()
@@ -569,13 +819,18 @@ def main():
         d = rep.as_dict()
         d["wall_ms"] = dt * 1e3
         d["fetch_ms"] = cache.counters()["fetch_ms"] - fetch0
-        print("frame %3d %s wall %.1f ms launches %d lane %d bricks %d samples %.3g rays %d kernel %.1f ms "
-              "fetch %.1f ms" % (k, "timed" if k >= args.warmup else "warmup", dt * 1e3, d["launches"],
+        print("frame %3d path %3d %s wall %.1f ms launches %d lane %d bricks %d samples %.3g rays %d kernel %.1f ms "
+              "fetch %.1f ms" % (k, path_frame(k, args.warmup, args.steps), "timed" if k >= args.warmup else "warmup",
+                                 dt * 1e3, d["launches"],
                                  d["lane_launches"], d["bricks_loaded"], d["mapped_samples"], d["pixels"],
                                  d["kernel_ms"], d["fetch_ms"]), file=sys.stderr, flush=True)
         if k >= args.warmup:
             lat.append(dt * 1e3)
             reports.append(d)
+        if k == args.warmup and not args.no_cpu_baseline:
+            # untimed: the CPU baseline over this frame's cache state
+            cpu = cpu_baseline(G, cache, vc, cam, tf, seconds=args.cpu_seconds,
+                               device_pixels=frame.pixels().reshape(-1))
     total_s = sum(lat) / 1e3
     S = sum(r["mapped_samples"] for r in reports)
     kms = sum(r["kernel_ms"] for r in reports)
@@ -606,7 +861,9 @@ def main():
                         "32^3 bricks, covering 1, 7 levels, 4 GiB pool (%dx%dx%d slots), 1024x1024, 120-frame "
                         "fly-through, step 0.5 voxel, alpha stop 0.99, %d requests/update" % (cs + (args.max_requests,)),
             "precision": "fp%d compositing" % args.precision,
-            "tile": ["8x8", "16x4", "32x2", "64x1"][args.tile],
+            "tile": TILE_NAMES[args.tile],
+            "upload_batch": args.max_requests,
+            "upload_autotune": upload_tune["table"] if upload_tune else None,
         },
         "frames_per_s": len(lat) / total_s if total_s > 0 else 0.0,
         "p50_ms": percentile(lat, 50),
@@ -626,7 +883,8 @@ def main():
             "traffic": traffic,
             "traffic_note": traffic_note,
             "l2": l2,
-            "kernel": "k_dvr<2,%s,%s>" % ("double" if args.precision == 64 else "float", ["8,8", "16,4", "32,2", "64,1"][args.tile]),
+            "kernel": "k_dvr<2,%s,%s>" % ("double" if args.precision == 64 else "float",
+                                          ["8,8", "16,4", "32,2", "64,1", "16,4", "32,2"][args.tile]),
             "launches": lane_launches,
             "avg_launch_ms": lane_ms / lane_launches if lane_launches else 0.0,
             "algorithmic_bytes_per_launch": B_lane / lane_launches if lane_launches else 0.0,
@@ -639,13 +897,25 @@ def main():
         },
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(G, cache, vc, cams[-1], tf, seconds=args.cpu_seconds)
+        out["cpu_baseline"] = cpu
     cache.close()
     frame.close()
-    if not args.no_batch:
-        out["batched_vs_plain"] = batch_workload(G)
+    if not args.no_extras:
+        pc = plain_config_workload(G)
+        out["plain_config"] = pc
+        if "cpu_baseline" in out:
+            c = pc["cpu"]
+            out["cpu_baseline"]["preprocess"] = {
+                "volume": "256^3 u8, 16^3 bricks, covering 1, 5 levels", "cores": c["cores"],
+                "mip_voxels_per_s": c["mip_voxels_per_s"], "bricker_bricks_per_s": c["bricker_bricks_per_s"],
+                "mip_ms": c["mip_ms"], "bricker_ms": c["bricker_ms"], "bricks": c["bricks"]}
+        if not args.no_batch:
+            out["batched_vs_plain"] = batch_workload(G)
+        if not args.no_serving:
+            out["serving"] = serving_workload(G, rates=(50, 200, 800), count=48, max_views=8, warm_requests=8,
+                                              device_source=True, tune_upload=True)
     if args.serving:
-        out["serving"] = serving_workload(G)
+        out["serving_long"] = serving_workload(G)
     print(json.dumps(out))
 
 

@@ -192,6 +192,7 @@ struct DvrView {
     uint32_t tf_lds_off;                           // entry offset of the LDS copy (| DVR_TF_OWNER: this
                                                    // view copies it in), when the launch has LDS tables
     uint32_t tf_size;
+    uint32_t tile_pair;                            // 1: tiles of two packets (top, bottom half)
     float alpha_stop;
     const float4 *tf;
     uint32_t *pixels;
@@ -249,12 +250,33 @@ struct DvrLaunch {
                                 //    level and MRPD y*z cells, x cells below 2^24; see launch_base)
 };
 
-// Tile shape of a ray packet: 0 8x8, 1 16x4, 2 32x2, 3 64x1.
+// Ray tiles: 0 8x8, 1 16x4, 2 32x2, 3 64x1 are one 64-ray packet each; 4
+// 16x8 and 5 32x4 are two packets, the tile's top and bottom halves (16x4,
+// 32x2), taken one after the other.  packet_kind: the packet shape (the ray
+// caster's variant, 0..3); tile_pair: 1 for two-packet tiles.
+constexpr uint32_t GC_TILES = 6;
+inline uint32_t packet_kind(uint32_t tile) { return tile == 4 ? 1u : (tile == 5 ? 2u : tile & 3u); }
+inline uint32_t tile_pair(uint32_t tile) { return tile >= 4 ? 1u : 0u; }
+// packet shape (tw x th = 64)
 inline void tile_shape(uint32_t tile, uint32_t &tw, uint32_t &th)
 {
     static const uint32_t TW[4] = {8, 16, 32, 64};
-    tw = TW[tile & 3];
+    tw = TW[packet_kind(tile)];
     th = 64 / tw;
+}
+// tile columns of a view of width w, and its packets
+inline uint32_t tile_cols(uint32_t tile, uint32_t w)
+{
+    uint32_t tw, th;
+    tile_shape(tile, tw, th);
+    return (w + tw - 1) / tw;
+}
+inline uint32_t tile_packets(uint32_t tile, uint32_t w, uint32_t h)
+{
+    uint32_t tw, th;
+    tile_shape(tile, tw, th);
+    const uint32_t rows = th << tile_pair(tile);
+    return (tile_cols(tile, w) * ((h + rows - 1) / rows)) << tile_pair(tile);
 }
 
 // Dynamic LDS of a DVR launch (per-lane level-switch table and view table)

@@ -805,8 +805,9 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     while (vv + 1 < n_views && vw[vv + 1].packet_off <= tp) ++vv;
                     v = vv;
                     const uint32_t tt = tp - vw[v].packet_off;
-                    x = (tt % vw[v].tiles_x) * TW + my_j % TW;
-                    y = (tt / vw[v].tiles_x) * TH + my_j / TW;
+                    const uint32_t pr = vw[v].tile_pair, ti = tt >> pr, half = tt & pr;
+                    x = (ti % vw[v].tiles_x) * TW + my_j % TW;
+                    y = (ti / vw[v].tiles_x) * (TH << pr) + half * TH + my_j / TW;
                     valid = x < vw[v].width && y < vw[v].height;
                 }
                 if (valid) {
@@ -1479,6 +1480,7 @@ const void *kernel_of(uint32_t tile, bool mt, bool bs32)
 const void *dvr_kernel(uint32_t bytes, uint32_t precision, uint32_t tile, bool mt, bool bs32)
 {
     const bool d = precision == 64;
+    tile = packet_kind(tile);
     switch (bytes) {
     case 1: return d ? kernel_of<1, double>(tile, mt, bs32) : kernel_of<1, float>(tile, mt, bs32);
     case 2: return d ? kernel_of<2, double>(tile, mt, bs32) : kernel_of<2, float>(tile, mt, bs32);

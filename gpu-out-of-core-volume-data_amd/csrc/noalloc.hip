@@ -81,7 +81,7 @@ extern "C" int gc_render_views_noalloc(gc_cache *c, const gc_view *views, const 
                                     " views exceed its scratch limit of " + std::to_string(NOALLOC_MAX_VIEWS));
     const uint32_t prec = views[0].precision;
     if (prec != 32 && prec != 64) return fail(GC_ERR_ARG, "precision must be 32 or 64");
-    if (views[0].tile > 3 || views[0].march > 8) return fail(GC_ERR_ARG, "bad launch shape");
+    if (views[0].tile >= GC_TILES || views[0].march > 8) return fail(GC_ERR_ARG, "bad launch shape");
     DvrLaunch L{};
     hipStream_t cs = nullptr;
     int device = 0;
@@ -93,8 +93,6 @@ extern "C" int gc_render_views_noalloc(gc_cache *c, const gc_view *views, const 
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : cs;
 
     // the view table (batch-slot semantics, every view starting)
-    uint32_t tw, th;
-    tile_shape(views[0].tile, tw, th);
     DvrView h[NOALLOC_MAX_VIEWS];
     uint32_t packets = 0;
     for (uint32_t i = 0; i < n_views; ++i) {
@@ -108,9 +106,10 @@ extern "C" int gc_render_views_noalloc(gc_cache *c, const gc_view *views, const 
         D = DvrView{};
         D.width = v.width;
         D.height = v.height;
-        D.tiles_x = (v.width + tw - 1) / tw;
+        D.tiles_x = tile_cols(views[0].tile, v.width);
+        D.tile_pair = tile_pair(views[0].tile);
         D.packet_off = packets;
-        packets += D.tiles_x * ((v.height + th - 1) / th);
+        packets += tile_packets(views[0].tile, v.width, v.height);
         for (int k = 0; k < 12; ++k) D.m[k] = v.inv_mv[k];
         for (int k = 0; k < 3; ++k) {
             D.bmin[k] = v.bbox_min[k];

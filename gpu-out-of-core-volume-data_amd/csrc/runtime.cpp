@@ -1124,11 +1124,10 @@ int cache_launch_base(gc_cache *c, DvrLaunch &L, hipStream_t &stream, int &devic
 
 static void view_of(const gc_view &v, const float *d_tf, uint32_t tf_size, const Frame &F, float k0, DvrView &out)
 {
-    uint32_t tw, th;
-    tile_shape(v.tile, tw, th);
     out.width = v.width;
     out.height = v.height;
-    out.tiles_x = (v.width + tw - 1) / tw;
+    out.tiles_x = tile_cols(v.tile, v.width);
+    out.tile_pair = tile_pair(v.tile);
     out.packet_off = 0;
     std::memcpy(out.m, v.inv_mv, sizeof(out.m));
     std::memcpy(out.bmin, v.bbox_min, sizeof(out.bmin));
@@ -1144,12 +1143,7 @@ static void view_of(const gc_view &v, const float *d_tf, uint32_t tf_size, const
     out.thr = opacity_threshold(v.alpha_stop, k0);
 }
 
-static uint32_t packets_of(const gc_view &v)
-{
-    uint32_t tw, th;
-    tile_shape(v.tile, tw, th);
-    return ((v.width + tw - 1) / tw) * ((v.height + th - 1) / th);
-}
+static uint32_t packets_of(const gc_view &v) { return tile_packets(v.tile, v.width, v.height); }
 
 
 
@@ -1259,8 +1253,6 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
 {
     Cache &C = *B.C;
     hipStream_t s = C.stream;
-    uint32_t tw, th;
-    tile_shape(B.tile, tw, th);
     // view table with this launch's tile ranges
     GC_HIP(hipEventSynchronize(B.views_free));
     uint32_t packets = 0;
@@ -1273,8 +1265,9 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
             D.width = D.height = D.tiles_x = 0;
             continue;
         }
-        D.tiles_x = (D.width + tw - 1) / tw;
-        if (S.starting) packets += D.tiles_x * ((D.height + th - 1) / th);
+        D.tiles_x = tile_cols(B.tile, D.width);
+        D.tile_pair = tile_pair(B.tile);
+        if (S.starting) packets += tile_packets(B.tile, D.width, D.height);
     }
     // skip_list: leave the waiting rays in their list (their bricks are not
     // loaded yet); this launch's stopped rays are appended to it
@@ -2033,7 +2026,7 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
     Frame &F = f->impl;
     if (view->width != F.w || view->height != F.h) return fail(GC_ERR_ARG, "view and frame sizes differ");
     if (view->precision != F.precision) return fail(GC_ERR_ARG, "view and frame precision differ");
-    if (view->tile > 3) return fail(GC_ERR_ARG, "tile must be 0..3");
+    if (view->tile >= GC_TILES) return fail(GC_ERR_ARG, "tile must be 0..5");
     DvrLaunch L{};
     launch_base(C, L);
     view_of(*view, d_tf, tf_size, F, L.kexp[0], L.view0);
@@ -2132,7 +2125,8 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
         GC_HIP(hipEventCreate(&C.ev0));
         GC_HIP(hipEventCreate(&C.ev1));
     }
-    if (view->width != f->impl.w || view->height != f->impl.h || view->precision != f->impl.precision || view->tile > 3)
+    if (view->width != f->impl.w || view->height != f->impl.h || view->precision != f->impl.precision ||
+        view->tile >= GC_TILES)
         return fail(GC_ERR_ARG, "view does not match the frame");
     GC_HIP(hipMemsetAsync(C.d_frame_stats, 0, 2 * sizeof(gc_render_stats), C.stream));
     int cur = 0;
@@ -2264,7 +2258,7 @@ int gc_batch_set_graph(gc_batch *b, int enable)
 int gc_batch_set_launch(gc_batch *b, uint32_t tile, uint32_t march, uint32_t refill_min)
 {
     if (!b) return fail(GC_ERR_ARG, "null batch");
-    if (tile > 3) return fail(GC_ERR_ARG, "tile must be 0..3");
+    if (tile >= GC_TILES) return fail(GC_ERR_ARG, "tile must be 0..5");
     if (march > 8) return fail(GC_ERR_ARG, "march must be 1..8");
     if (refill_min > 64) return fail(GC_ERR_ARG, "refill_min must be 0..64");
     Batch &B = b->impl;

@@ -393,7 +393,7 @@ int gc_server_create(const gc_server_cfg *cfg, int device, gc_server **out)
     if (cfg->max_views == 0 || cfg->max_views > DVR_MAX_VIEWS) return fail(GC_ERR_ARG, "max_views must be 1..64");
     if (cfg->slot_pixels == 0) return fail(GC_ERR_ARG, "slot_pixels must be positive");
     if (cfg->precision != 32 && cfg->precision != 64) return fail(GC_ERR_ARG, "precision must be 32 or 64");
-    if (cfg->tile > 3 || cfg->march > 8) return fail(GC_ERR_ARG, "bad launch shape");
+    if (cfg->tile >= GC_TILES || cfg->march > 8) return fail(GC_ERR_ARG, "bad launch shape");
     auto s = std::make_unique<gc_server>();
     s->impl.cfg = *cfg;
     s->impl.device = device;

@@ -302,7 +302,8 @@ typedef struct gc_view {
     float    bbox_min[3], bbox_max[3];
     float    alpha_stop;         /* early termination; 0.95 in the reference   */
     uint32_t precision;          /* 32: FP32 compositing, 64: FP64 compositing */
-    uint32_t tile;               /* 64-ray packet: 0 8x8, 1 16x4, 2 32x2, 3 64x1 */
+    uint32_t tile;               /* ray tile: 0 8x8, 1 16x4, 2 32x2, 3 64x1 (one
+                                    64-ray packet), 4 16x8, 5 32x4 (two)     */
     uint32_t refill_min;         /* refill a wave's idle lanes once this many are
                                     idle (0: default 16)                      */
     uint32_t march;              /* samples per lane between refill checks,
@@ -390,7 +391,7 @@ typedef struct gc_batch gc_batch;
 int gc_batch_create(gc_cache *c, uint32_t max_views, uint64_t max_pixels, uint32_t precision,
                     gc_batch **out);
 int gc_batch_destroy(gc_batch *b);
-/* Launch shape: tile 0..3 (8x8, 16x4, 32x2, 64x1), march 1..8 samples per
+/* Launch shape: tile 0..5 (8x8, 16x4, 32x2, 64x1, 16x8, 32x4), march 1..8 samples per
  * lane between refill checks (0: 4), refill_min (0: 16). */
 int gc_batch_set_launch(gc_batch *b, uint32_t tile, uint32_t march, uint32_t refill_min);
 /* 1: launches are replays of captured hipGraphs (counter fills + the ray

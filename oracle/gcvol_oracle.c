@@ -199,6 +199,22 @@ void or_brick_extract(const void *level, const uint32_t dims[3], uint32_t bytes,
             }
 }
 
+void or_brick_level(const void *level, const uint32_t dims[3], uint32_t bytes, uint32_t bs, uint32_t cov,
+                    uint32_t b0, uint32_t b1, void *out)
+{
+    /* process_bricking's per-level loop (Bricker_v2.cpp:79-252) over bricks
+     * [b0, b1) of the level in x-fastest brick order, one brick of bs^3
+     * voxels after the other in `out`. */
+    const uint32_t inner = bs - 2 * cov;
+    const uint32_t nb[3] = {(dims[0] + inner - 1) / inner, (dims[1] + inner - 1) / inner, (dims[2] + inner - 1) / inner};
+    const uint32_t one[3] = {1, 1, 1};
+    const uint64_t bvox = (uint64_t)bs * bs * bs;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t c[3] = {b % nb[0], (b / nb[0]) % nb[1], b / (nb[0] * nb[1])};
+        or_brick_extract(level, dims, bytes, bs, cov, one, c, (uint8_t *)out + (uint64_t)(b - b0) * bvox * bytes);
+    }
+}
+
 /* =================================================================== cache */
 
 struct or_cache {

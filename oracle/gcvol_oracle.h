@@ -66,6 +66,10 @@ void or_mip_reduce(const void *in, const uint32_t dims[3], uint32_t bytes,
 void or_brick_extract(const void *level, const uint32_t dims[3], uint32_t bytes,
                       uint32_t brick_size, uint32_t covering, const uint32_t big_brick[3],
                       const uint32_t big_brick_coord[3], void *out);
+/* Bricks [b0, b1) of a level (x-fastest brick order, BigBrickSize 1), one
+ * after the other: the CPU bricker's per-level loop (Bricker_v2.cpp:79-252). */
+void or_brick_level(const void *level, const uint32_t dims[3], uint32_t bytes, uint32_t bs, uint32_t cov,
+                    uint32_t b0, uint32_t b1, void *out);
 
 /* ------------------------------------------------------------------- cache */
 

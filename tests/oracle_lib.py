@@ -71,6 +71,7 @@ def lib():
         L.or_geometry.argtypes = [C.POINTER(OrCfg), C.POINTER(OrGeom)]
         L.or_mip_reduce.argtypes = [P, P, u32, P, P]
         L.or_brick_extract.argtypes = [P, P, u32, u32, u32, P, P, P]
+        L.or_brick_level.argtypes = [P, P, u32, u32, u32, u32, u32, P]
         L.or_cache_create.restype = P
         L.or_cache_create.argtypes = [C.POINTER(OrCfg), P]
         L.or_cache_destroy.argtypes = [P]
@@ -164,6 +165,15 @@ def brick_extract(level, bs, cov, bbc, bb=(1, 1, 1)):
     out = np.zeros((bs * bb[2], bs * bb[1], bs * bb[0]), level.dtype)
     lib().or_brick_extract(ptr(level), ptr(dims), level.dtype.itemsize, bs, cov, ptr(bb),
                            ptr(np.asarray(bbc, np.uint32)), ptr(out))
+    return out
+
+
+def brick_level(level, bs, cov, b0, b1):
+    """Bricks [b0, b1) of a level ([z,y,x] array), x-fastest brick order."""
+    level = np.ascontiguousarray(level)
+    dims = np.array([level.shape[2], level.shape[1], level.shape[0]], np.uint32)
+    out = np.zeros((b1 - b0, bs, bs, bs), level.dtype)
+    lib().or_brick_level(ptr(level), ptr(dims), level.dtype.itemsize, bs, cov, b0, b1, ptr(out))
     return out
 
 

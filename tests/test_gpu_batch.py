@@ -56,7 +56,7 @@ def singles(cache, views, d_tfs, sizes):
     return pix, st
 
 
-@pytest.mark.parametrize("prec,tile,march", [(32, 0, 0), (64, 2, 2), (32, 3, 1)])
+@pytest.mark.parametrize("prec,tile,march", [(32, 0, 0), (64, 2, 2), (32, 3, 1), (32, 4, 3), (64, 5, 4)])
 def test_batch_equals_single_views_resident(gpu, prec, tile, march):
     sc = Scene((64, 64, 64), 8, 1, 1)
     gc = sc.gpu_cache()

@@ -162,3 +162,19 @@ def test_tf_samples_reference_ramps(oracle, gcv):
     assert np.allclose(s[0], [0, 0, 1, 0], atol=1e-6)
     assert np.allclose(s[10], [0, 0, 1, 0.05], atol=1e-6)
     assert np.array_equal(gcv.tf_samples(colorful, 100), s)
+
+
+def test_brick_level_loop_is_per_brick_cut(oracle):
+    """or_brick_level (the bricker's per-level loop, used by the CPU
+    baseline) cuts brick b of the level at (b % nx, b / nx % ny, b / nx / ny)
+    exactly as the per-brick cut does."""
+    import oracle_lib as O
+    rng = np.random.default_rng(3)
+    lv = rng.integers(0, 65536, size=(21, 30, 37), dtype=np.uint16)
+    bs, cov = 8, 1
+    inner = bs - 2 * cov
+    nb = [(d + inner - 1) // inner for d in (37, 30, 21)]
+    out = O.brick_level(lv, bs, cov, 3, nb[0] * nb[1] * nb[2])
+    for i, b in enumerate(range(3, nb[0] * nb[1] * nb[2])):
+        c = (b % nb[0], (b // nb[0]) % nb[1], b // (nb[0] * nb[1]))
+        assert np.array_equal(out[i], O.brick_extract(lv, bs, cov, c)), b
