@@ -205,6 +205,10 @@ struct DvrView {
                                                    // samples blended so far (read when tbuf > 0)
     double thr;                                    // FP32: a ray continues while Q > thr =
                                                    // (1 - alpha_stop)^(1/k0)
+    uint8_t *stop;                                 // per pixel: 1 = the ray stopped at an Unmapped
+                                                   // brick in the last launch (or null: not recorded)
+    uint32_t stop_tile_off;                        // first 8x8 tile of this view in the stopped-ray
+                                                   // collection after a launch
 };
 
 struct DvrLaunch {
@@ -235,9 +239,6 @@ struct DvrLaunch {
     const uint32_t *list;       // explicit ray list, or null for all tiles
     const uint32_t *list_count; // device count of `list`
     uint32_t *queue;            // 8 per-XCD packet cursors, zero at launch
-    uint32_t *out_list;         // rays stopped at an Unmapped brick (or null)
-    uint32_t *out_count;
-    uint32_t *view_stopped;     // per view: rays stopped at an Unmapped brick (or null)
     gc_render_stats *stats;
     uint32_t refill_min;        // refill idle lanes once this many are idle
     uint32_t march;             // samples per lane between refill checks (1..8)
@@ -302,6 +303,23 @@ hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision
 hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t tile,
                       uint32_t grid, hipStream_t stream);
 hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream);
+
+// The rays a launch stopped at an Unmapped brick (the views' stop flags),
+// gathered into the next launch's ray list in screen order: 8x8 tile by 8x8
+// tile over the views, each tile's rays contiguous, so that a 64-ray packet
+// of the list holds neighbouring rays (coherent brick accesses), as a tile
+// of the first launch does.  The flags are cleared as they are read.
+struct CollectArgs {
+    const DvrView *views;       // device table of n_views views, or null: `one`
+    DvrView one;
+    uint32_t n_views;
+    uint32_t n_tiles;           // 8x8 tiles over the views (stop_tile_off ranges)
+    uint32_t *out_list;         // entries view << DVR_PIXEL_BITS | pixel, appended
+    uint32_t *out_count;
+    uint32_t *view_stopped;     // per view: rays appended (or null)
+};
+inline uint32_t stop_tiles(uint32_t w, uint32_t h) { return ((w + 7) / 8) * ((h + 7) / 8); }
+hipError_t launch_collect_stopped(const CollectArgs &a, hipStream_t stream);
 
 // Exact opacity threshold of a view (FP32 compositing) and the check that a
 // launch's thresholds are normal doubles (runtime.cpp).

@@ -1100,17 +1100,12 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 have = false;
             }
         }
-        // rays stopped at a miss go to the next launch's ray list
+        // rays stopped at a miss: flagged for the next launch's ray list
         const uint64_t st = __ballot(stopped);
         n_unmapped += (uint32_t)__popcll(st);
-        if (stopped && p.view_stopped) atomicAdd(&p.view_stopped[v], 1u);
-        if (p.out_list) {
-            if (st) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(p.out_count, (uint32_t)__popcll(st));
-                base = __shfl(base, 0, 64);
-                if (stopped) p.out_list[base + (uint32_t)__popcll(st & lt_mask)] = (v << DVR_PIXEL_BITS) | pix;
-            }
+        if (stopped) {
+            uint8_t *const sf = vw[v].stop;
+            if (sf) gp(sf)[pix] = 1;
         }
     }
 
@@ -1426,8 +1421,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     store_state(t_k);
                     if (gl == 0) {
                         ++n_unmapped;
-                        if (p.view_stopped) atomicAdd(&p.view_stopped[v], 1u);
-                        if (p.out_list) p.out_list[atomicAdd(p.out_count, 1u)] = e;
+                        if (V.stop) gp(V.stop)[pix] = 1;
                     }
                     active = false;
                 } else {
@@ -1651,6 +1645,58 @@ hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t st
 {
     if (n == 0) return hipSuccess;
     k_fill32<<<(n + 255) / 256, 256, 0, stream>>>(p, n, value);
+    return hipGetLastError();
+}
+
+// One wave per 8x8 tile of the views' stop flags (four tiles per block, in
+// tile order, so consecutive blocks append neighbouring tiles): the wave
+// reads and clears its tile's flags and appends the flagged rays with one
+// atomic, the tile's rays contiguous in the list.
+__global__ __launch_bounds__(256) void k_collect_stopped(const CollectArgs a)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (tile >= a.n_tiles) return;   // whole waves
+    // the view holding the tile: the last one whose range starts at or
+    // before it (views without tiles share the next view's start)
+    uint32_t v = 0;
+    if (a.views) {
+        uint32_t lo = 0, hi = a.n_views;   // stop_tile_off[lo] <= tile < stop_tile_off[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.views[mid].stop_tile_off <= tile) lo = mid;
+            else hi = mid;
+        }
+        v = lo;
+    }
+    const DvrView &V = a.views ? a.views[v] : a.one;
+    uint8_t *const sf = V.stop;
+    if (!sf) return;
+    const uint32_t t = tile - V.stop_tile_off, tx = (V.width + 7u) / 8u;
+    const uint32_t x = (t % tx) * 8u + (lane & 7u), y = (t / tx) * 8u + (lane >> 3);
+    bool s = false;
+    uint32_t pix = 0;
+    if (x < V.width && y < V.height) {
+        pix = y * V.width + x;
+        s = sf[pix] != 0;
+        if (s) sf[pix] = 0;
+    }
+    const uint64_t m = __ballot(s);
+    if (!m) return;
+    const uint32_t n = (uint32_t)__popcll(m);
+    uint32_t base = 0;
+    if (lane == 0) {
+        base = atomicAdd(a.out_count, n);
+        if (a.view_stopped) atomicAdd(&a.view_stopped[v], n);
+    }
+    base = (uint32_t)__shfl((int)base, 0, 64);
+    if (s) a.out_list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (v << DVR_PIXEL_BITS) | pix;
+}
+
+hipError_t launch_collect_stopped(const CollectArgs &a, hipStream_t stream)
+{
+    if (a.n_tiles == 0) return hipSuccess;
+    k_collect_stopped<<<(a.n_tiles + 3u) / 4u, 256, 0, stream>>>(a);
     return hipGetLastError();
 }
 

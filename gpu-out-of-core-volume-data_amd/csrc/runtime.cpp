@@ -1040,6 +1040,7 @@ struct Frame {
     // launch scratch: [0..7] per-XCD packet cursors, [8], [9] ray-list counts
     uint32_t *ctl = nullptr;
     uint32_t *lists[2] = {nullptr, nullptr};   // rays stopped at a miss
+    uint8_t *stop = nullptr;                   // per pixel: stopped at a miss in the last launch
     size_t cbytes() const { return (size_t)w * h * (precision == 64 ? 32 : 16); }
     ~Frame()
     {
@@ -1051,6 +1052,7 @@ struct Frame {
         hipFree(ctl);
         hipFree(lists[0]);
         hipFree(lists[1]);
+        hipFree(stop);
     }
 };
 
@@ -1162,6 +1164,7 @@ struct Batch {
     float4 *rpos = nullptr;             // resume positions (views with resume = 1)
     double *qbuf = nullptr;             // FP32: exact opacity products
     uint32_t *lists[2] = {nullptr, nullptr};
+    uint8_t *stop = nullptr;            // per pixel: stopped at a miss in the last launch
     // [0..7] per-XCD packet cursors, [8], [9] ray-list counts, [10] timestamp,
     // [16..16+max_views) rays stopped per slot in the last launch
     uint32_t *ctl = nullptr;
@@ -1179,7 +1182,7 @@ struct Batch {
     uint32_t tile = 0, march = 8, refill_min = 16;
     int cur = -1;                       // list holding the last launch's stopped rays (-1: none)
     bool use_graph = false;
-    std::map<std::array<uint32_t, 9>, hipGraphExec_t> graphs;
+    std::map<std::array<uint32_t, 10>, hipGraphExec_t> graphs;
     size_t cbytes_per_pixel() const { return precision == 64 ? 32 : 16; }
     uint64_t slot_pixels() const { return max_pixels / max_views; }
     void drop_graphs()
@@ -1197,6 +1200,7 @@ struct Batch {
         hipFree(qbuf);
         hipFree(lists[0]);
         hipFree(lists[1]);
+        hipFree(stop);
         hipFree(ctl);
         hipFree(d_views);
         hipFree(d_stats);
@@ -1234,6 +1238,7 @@ static int batch_place(Batch &B, uint32_t slot, const gc_view &v, const float *d
     D.cstate = static_cast<uint8_t *>(B.cstate) + off * B.cbytes_per_pixel();
     D.rpos = v.resume ? B.rpos + off : nullptr;
     D.qbuf = B.precision == 32 ? B.qbuf + off : nullptr;
+    D.stop = B.stop + off;
     {
         DvrLaunch K{};
         launch_base(*B.C, K);
@@ -1255,16 +1260,19 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
     hipStream_t s = C.stream;
     // view table with this launch's tile ranges
     GC_HIP(hipEventSynchronize(B.views_free));
-    uint32_t packets = 0;
+    uint32_t packets = 0, stop_tiles_n = 0;
     for (uint32_t i = 0; i < B.table; ++i) {
         Batch::Slot &S = B.slots[i];
         DvrView &D = B.h_views[i];
         D = S.v;
         D.packet_off = packets;
+        D.stop_tile_off = stop_tiles_n;
         if (!S.used) {
             D.width = D.height = D.tiles_x = 0;
+            D.stop = nullptr;
             continue;
         }
+        stop_tiles_n += stop_tiles(D.width, D.height);
         D.tiles_x = tile_cols(B.tile, D.width);
         D.tile_pair = tile_pair(B.tile);
         if (S.starting) packets += tile_packets(B.tile, D.width, D.height);
@@ -1289,7 +1297,6 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
     L.queue = B.ctl;
     L.refill_min = B.refill_min;
     L.march = B.march;
-    L.view_stopped = B.ctl + 16;
     for (uint32_t i = 0; i < B.table; ++i)
         if (B.slots[i].used && B.slots[i].v.rpos) L.exact_resume = 1;
     finish_exact(L, B.h_views, B.table);
@@ -1298,8 +1305,15 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
         L.list = B.lists[B.cur];
         L.list_count = B.ctl + 8 + B.cur;
     }
-    L.out_list = B.lists[out];
-    L.out_count = B.ctl + 8 + out;
+    // the stopped rays, in screen order, form the next launch's list (appended
+    // to the waiting rays when the list was skipped)
+    CollectArgs ca{};
+    ca.views = B.d_views;
+    ca.n_views = B.table;
+    ca.n_tiles = stop_tiles_n;
+    ca.out_list = B.lists[out];
+    ca.out_count = B.ctl + 8 + out;
+    ca.view_stopped = B.ctl + 16;
     uint64_t listed = 0;
     if (has_list)
         for (uint32_t i = 0; i < B.table; ++i)
@@ -1319,8 +1333,10 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
                                      : dvr_grid(C.vc.encoded_bytes, B.precision, B.tile, packets + (uint32_t)((listed + 63) / 64), lds,
                                                 tf_tables);
     auto launch_k = [&]() -> hipError_t {
-        return ray_kernel ? launch_dvr_ray(L, C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays, grid, s)
-                          : launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
+        const hipError_t e =
+            ray_kernel ? launch_dvr_ray(L, C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays, grid, s)
+                       : launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
+        return e == hipSuccess ? launch_collect_stopped(ca, s) : e;
     };
     GC_HIP(hipEventRecord(B.ev0, s));
     if (B.use_graph) {
@@ -1328,9 +1344,9 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
         // stays valid across updates
         GC_HIP(launch_fill32(B.ctl + 10, 1, C.ts, s));
         L.ts_dev = B.ctl + 10;
-        const std::array<uint32_t, 9> key{B.table, B.tile + (ray_kernel ? 8u : 0u), B.march, B.refill_min, packets, grid,
-                                          has_list ? (uint32_t)B.cur : 2u, (uint32_t)out + (append ? 4u : 0u),
-                                          tf_entries};
+        const std::array<uint32_t, 10> key{B.table, B.tile + (ray_kernel ? 8u : 0u), B.march, B.refill_min, packets, grid,
+                                           has_list ? (uint32_t)B.cur : 2u, (uint32_t)out + (append ? 4u : 0u),
+                                           tf_entries, stop_tiles_n};
         auto it = B.graphs.find(key);
         if (it == B.graphs.end()) {
             hipGraph_t g = nullptr;
@@ -1961,6 +1977,8 @@ int gc_frame_create(uint32_t width, uint32_t height, uint32_t precision, gc_fram
     GC_TRY(dalloc(&F.ctl, 64));
     GC_TRY(dalloc(&F.lists[0], (size_t)width * height * 4));
     GC_TRY(dalloc(&F.lists[1], (size_t)width * height * 4));
+    GC_TRY(dalloc(&F.stop, ((size_t)width * height + 3) & ~(size_t)3));
+    GC_HIP(hipMemset(F.stop, 0, ((size_t)width * height + 3) & ~(size_t)3));
     GC_HIP(hipMemset(F.ctl, 0, 64));
     GC_HIP(hipMemset(F.pixels, 0, (size_t)width * height * 4));
     GC_HIP(hipMemset(F.tbuf, 0, (size_t)width * height * 4));
@@ -2061,6 +2079,8 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
     DvrLaunch L{};
     launch_base(C, L);
     view_of(view, d_tf, tf_size, F, L.kexp[0], L.view0);
+    L.view0.stop = F.stop;
+    L.view0.stop_tile_off = 0;
     L.exact_resume = L.view0.rpos ? 1u : 0u;
     finish_exact(L, &L.view0, 1);
     L.tf_lds_entries = dvr_assign_tf_lds(&L.view0, 1, C.vc.encoded_bytes, &L.tf_lds_tables);
@@ -2076,8 +2096,6 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
         L.list = F.lists[cur];
         L.list_count = F.ctl + 8 + cur;
     }
-    L.out_list = F.lists[out];
-    L.out_count = F.ctl + 8 + out;
     GC_HIP(launch_fill32(F.ctl, 8, 0, s));
     GC_HIP(launch_fill32(F.ctl + 8 + out, 1, 0, s));
     *lane = !(!first && list_rays <= C.ray_list_max && dvr_ray_supported(L, C.vc.encoded_bytes));
@@ -2093,6 +2111,14 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
                                        dvr_lds_bytes(L.nb_lods, 1, L.tf_lds_entries), L.tf_lds_tables);
         GC_HIP(launch_dvr(L, C.vc.encoded_bytes, view.precision, view.tile, grid, s));
     }
+    // the stopped rays, in screen order, form the next launch's list
+    CollectArgs ca{};
+    ca.one = L.view0;
+    ca.n_views = 1;
+    ca.n_tiles = stop_tiles(F.w, F.h);
+    ca.out_list = F.lists[out];
+    ca.out_count = F.ctl + 8 + out;
+    GC_HIP(launch_collect_stopped(ca, s));
     cur = out;
     return GC_OK;
 }
@@ -2215,6 +2241,8 @@ int gc_batch_create(gc_cache *c, uint32_t max_views, uint64_t max_pixels, uint32
     GC_TRY(dalloc(&B.qbuf, max_pixels * sizeof(double)));
     GC_TRY(dalloc(&B.lists[0], max_pixels * 4));
     GC_TRY(dalloc(&B.lists[1], max_pixels * 4));
+    GC_TRY(dalloc(&B.stop, (max_pixels + 3) & ~3ull));
+    GC_HIP(hipMemset(B.stop, 0, (max_pixels + 3) & ~3ull));
     GC_TRY(dalloc(&B.ctl, (16 + DVR_MAX_VIEWS) * 4));
     GC_TRY(dalloc(&B.d_views, (size_t)max_views * sizeof(DvrView)));
     GC_TRY(dalloc(&B.d_stats, sizeof(gc_render_stats)));
