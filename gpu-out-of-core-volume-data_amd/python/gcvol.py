@@ -144,6 +144,8 @@ EXPORTS = [
     "gc_cache_busy", "gc_source_create_gpu_mip", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
     "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get", "gc_cache_counters_get",
     "gc_source_create_synthetic_gpu", "gc_source_get_device", "gc_render_views_noalloc", "gc_cache_probe",
+    "gc_brick_key", "gc_brick_key_unpack", "gc_store_brick_path", "gc_lz4_bound", "gc_lz4_compress",
+    "gc_lz4_decompress", "gc_store_write", "gc_source_create_store", "gc_store_read_meta", "gc_store_stats",
 ]
 
 _lib = None
@@ -159,6 +161,9 @@ def lib():
             getattr(L, name).restype = C.c_int
         L.gc_last_error.restype = C.c_char_p
         L.gc_version.restype = C.c_char_p
+        L.gc_brick_key.restype = u64
+        L.gc_brick_key_unpack.restype = None
+        L.gc_lz4_bound.restype = u64
         # Every export gets explicit argument types: without them ctypes passes
         # Python ints as 32-bit C ints and silently truncates pointers.
         P, I, SZ = vp, C.c_int, C.c_size_t
@@ -218,6 +223,14 @@ def lib():
             "gc_render_views_noalloc": [P, C.POINTER(View), C.POINTER(vp), P, u32, C.POINTER(vp), C.POINTER(vp),
                                         C.POINTER(vp), P, P],
             "gc_cache_probe": [P, u32, P, P, u32, P, P],
+            "gc_brick_key": [u32, u32, u32, u32], "gc_brick_key_unpack": [u64, C.POINTER(u32), P],
+            "gc_store_brick_path": [C.c_char_p, u32, u32, P, C.c_char_p, SZ],
+            "gc_lz4_bound": [u64], "gc_lz4_compress": [P, u64, P, u64, C.POINTER(u64)],
+            "gc_lz4_decompress": [P, u64, P, u64],
+            "gc_store_write": [C.POINTER(VolumeCfg), P, C.c_char_p, C.c_int, C.c_float, P, u32, u32],
+            "gc_source_create_store": [C.POINTER(VolumeCfg), C.c_char_p, u32, C.POINTER(vp)],
+            "gc_store_read_meta": [C.c_char_p, u32, P, C.POINTER(u32), P, C.POINTER(u32)],
+            "gc_store_stats": [P, P],
         }
         missing = [n for n in EXPORTS if n not in sig]
         if missing:
@@ -401,6 +414,20 @@ class Source:
         h = vp()
         check(lib().gc_source_create_gpu_mip(C.byref(vc), level0.h, device, memo_bytes, C.byref(h)))
         return cls(h, keep=level0)
+
+    @classmethod
+    def store(cls, vc, volume_dir, cache_bricks=0):
+        """The on-disk brick store under volume_dir (bricks_B_B_B/...), read
+        through a host LRU of cache_bricks big bricks (0: 32768)."""
+        h = vp()
+        check(lib().gc_source_create_store(C.byref(vc), os.fsencode(volume_dir), cache_bricks, C.byref(h)))
+        return cls(h)
+
+    def store_stats(self):
+        """(host-cache hits, file loads, big bricks held, Empty answers)."""
+        out = np.zeros(4, np.uint64)
+        check(lib().gc_store_stats(self.h, ptr(out)))
+        return tuple(int(x) for x in out)
 
     def get(self, vc, level, bb_coord):
         bb = [vc.big_brick_size[a] for a in range(3)]
@@ -737,6 +764,61 @@ def render_views_noalloc(cache, views, d_tfs, tf_sizes, d_pixels, d_tbufs, d_cst
     check(lib().gc_render_views_noalloc(cache.h, (View * n)(*views), arr(d_tfs), ptr(np.asarray(tf_sizes, np.uint32)),
                                         n, arr(d_pixels), arr(d_tbufs), arr(d_cstates),
                                         d_stats.p if d_stats is not None else None, stream))
+
+
+# ----------------------------------------------------------- brick store
+def brick_key(level, x, y, z):
+    """Bkey: (level, x, y, z) as four 16-bit fields, level lowest."""
+    return int(lib().gc_brick_key(level, x, y, z))
+
+
+def brick_key_unpack(key):
+    lv = u32(0)
+    xyz = np.zeros(3, np.uint32)
+    lib().gc_brick_key_unpack(int(key), C.byref(lv), ptr(xyz))
+    return int(lv.value), int(xyz[0]), int(xyz[1]), int(xyz[2])
+
+
+def store_brick_path(volume_dir, brick_size, level, xyz):
+    buf = C.create_string_buffer(4096)
+    check(lib().gc_store_brick_path(os.fsencode(volume_dir), brick_size, level,
+                                    ptr(np.asarray(xyz, np.uint32)), buf, 4096))
+    return os.fsdecode(buf.value)
+
+
+def lz4_compress(data):
+    src = np.frombuffer(bytes(data), np.uint8)
+    cap = int(lib().gc_lz4_bound(len(src)))
+    dst = np.zeros(max(cap, 1), np.uint8)
+    n = u64(0)
+    check(lib().gc_lz4_compress(ptr(src) if len(src) else None, len(src), ptr(dst), cap, C.byref(n)))
+    return dst[:n.value].tobytes()
+
+
+def lz4_decompress(data, size):
+    src = np.frombuffer(bytes(data), np.uint8)
+    dst = np.zeros(max(size, 1), np.uint8)
+    check(lib().gc_lz4_decompress(ptr(src), len(src), ptr(dst), size))
+    return dst[:size].tobytes()
+
+
+def store_write(vc, source, volume_dir, compression=True, empty_threshold=0.0, histogram=None, threads=0):
+    """Write the source's bricks and bricks.xml as the reference's store."""
+    h = None if histogram is None else np.ascontiguousarray(histogram, np.float32)
+    check(lib().gc_store_write(C.byref(vc), source.h, os.fsencode(volume_dir), 1 if compression else 0,
+                               float(empty_threshold), None if h is None else ptr(h), 0 if h is None else len(h),
+                               threads))
+
+
+def store_read_meta(volume_dir, brick_size):
+    """(empty keys, histogram) of a store's bricks.xml."""
+    ne, nh = u32(0), u32(0)
+    check(lib().gc_store_read_meta(os.fsencode(volume_dir), brick_size, None, C.byref(ne), None, C.byref(nh)))
+    keys = np.zeros(max(1, ne.value), np.uint64)
+    histo = np.zeros(max(1, nh.value), np.float32)
+    check(lib().gc_store_read_meta(os.fsencode(volume_dir), brick_size, ptr(keys), C.byref(ne), ptr(histo),
+                                   C.byref(nh)))
+    return keys[:ne.value], histo[:nh.value]
 
 
 def set_ray_list_max(cache, n):

@@ -183,6 +183,53 @@ int gc_source_destroy(gc_source *src);
 int gc_source_get(gc_source *src, uint32_t level, const uint32_t big_brick[3], void *dst,
                   int *status);
 
+/* ------------------------------------------------------ on-disk brick store */
+
+/* Bkey (BrickKey.cpp:15-25): level, x, y, z as four 16-bit fields, level in
+ * the lowest (the reference's uint16_t[4] read as a little-endian uint64). */
+uint64_t gc_brick_key(uint32_t level, uint32_t x, uint32_t y, uint32_t z);
+void gc_brick_key_unpack(uint64_t key, uint32_t *level, uint32_t xyz[3]);
+/* Brick file of big brick (level, x, y, z) under volume_dir
+ * (BricksManager::get_brick_path, BricksManager.cpp:175-188):
+ * volume_dir "bricks_B_B_B/L<l>/L<l>_<x>_<y>_<z>.raw". */
+int gc_store_brick_path(const char *volume_dir, uint32_t brick_size, uint32_t level, const uint32_t xyz[3],
+                        char *out, size_t out_size);
+
+/* LZ4 block format, the codec of the brick files (BricksManager.cpp:93-114
+ * writes with LZ4_compress_HC, :150-158 reads with LZ4_decompress_fast).
+ * Compress: any valid block (not byte-identical to LZ4-HC's); *out_n =
+ * compressed size, at most gc_lz4_bound(n).  Decompress: the block must
+ * decode to exactly dst_n bytes, else GC_ERR_SOURCE. */
+uint64_t gc_lz4_bound(uint64_t n);
+int gc_lz4_compress(const void *src, uint64_t n, void *dst, uint64_t cap, uint64_t *out_n);
+int gc_lz4_decompress(const void *src, uint64_t n, void *dst, uint64_t dst_n);
+
+/* Write every big brick of every level of `src` as the reference's bricker
+ * lays them out (Bricker_v2.cpp:79-252, BricksManager::write_brick) under
+ * volume_dir (created as needed), LZ4-compressed when `compression`, and
+ * bricks.xml (MetaData::write_bricks_xml, MetaData.cpp:36-103): the levels'
+ * initial and real sizes, the empty list and the histogram.  Big bricks the
+ * source reports Empty, or (empty_threshold > 0) whose voxels are all below
+ * empty_threshold, go into the empty list (their files are written too).
+ * histogram: histo_n values, or NULL for the reference's default of zeros
+ * (256 for 1-byte, 65535 for 2-byte voxels, none for floats). */
+int gc_store_write(const gc_volume_cfg *vc, gc_source *src, const char *volume_dir, int compression,
+                   float empty_threshold, const float *histogram, uint32_t histo_n, uint32_t threads);
+/* A brick source over such a store (BricksManager, BricksManager.cpp:29-227):
+ * bricks.xml's empty list gives GC_BRICK_EMPTY; other big bricks are read
+ * from their files, LZ4-decoded (a file of exactly a big brick's size that
+ * is not a valid block is taken as raw voxels) and kept in a host LRU of at
+ * most cache_bricks big bricks (0: 32768, the reference's default). */
+int gc_source_create_store(const gc_volume_cfg *vc, const char *volume_dir, uint32_t cache_bricks,
+                           gc_source **out);
+/* The store's bricks.xml: empty keys and histogram (either may be NULL;
+ * *n_empty / *histo_n in: capacity, out: count). */
+int gc_store_read_meta(const char *volume_dir, uint32_t brick_size, uint64_t *empty_keys, uint32_t *n_empty,
+                       float *histogram, uint32_t *histo_n);
+/* Counters of a store source: out[0] host-cache hits, out[1] file loads,
+ * out[2] big bricks held, out[3] Empty answers. */
+int gc_store_stats(gc_source *src, uint64_t out[4]);
+
 /* Synthesise a whole level-0 volume (x-fastest) with the synthetic rule. */
 int gc_synth_volume(const gc_synth_params *p, const uint32_t dims[3], uint32_t encoded_bytes,
                     void *host_out);

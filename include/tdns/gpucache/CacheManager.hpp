@@ -11,8 +11,11 @@
 //    BricksManager (LZ4 brick files).  load_raw_volume() builds the mip
 //    pyramid of a raw volume on the GPU in memory (the Mipmapper + Bricker
 //    preprocessing, Mipmapper.cu:401-597, Bricker_v2.cpp:45-252) and serves
-//    bricks from it; set_source() attaches any gc_source (synthetic, callback).
-//    The MetaData must outlive the CacheManager built from it.
+//    bricks from it; load() opens a preprocessed store (bricks_B_B_B/L*/ LZ4
+//    brick files and bricks.xml, read through a host LRU as BricksManager
+//    does) with its empty list and histogram; set_source() attaches any
+//    gc_source (synthetic, callback).  The MetaData must outlive the
+//    CacheManager built from it.
 //  * to_kernel_object() returns a K_CacheManager<T> whose device API
 //    (get / get_normalized) is declared in gc_cache_device.h; it is available
 //    in code compiled by hipcc.
@@ -52,6 +55,29 @@ inline void check(int rc)
 } // namespace detail
 
 namespace data {
+
+// Bkey (BrickKey.hpp, BrickKey.cpp:15-55): level, x, y, z as 16-bit fields.
+typedef uint64_t Bkey;
+inline Bkey get_key(uint32_t level, const math::Vector3ui &position)
+{
+    return gc_brick_key(level, position[0], position[1], position[2]);
+}
+inline void get_brick_level_position(Bkey key, uint32_t &level, math::Vector3ui &position)
+{
+    gc_brick_key_unpack(key, &level, position.v);
+}
+inline std::string get_brick_name(uint32_t level, const math::Vector3ui &position)
+{
+    return "L" + std::to_string(level) + "_" + std::to_string(position[0]) + "_" + std::to_string(position[1]) + "_" +
+           std::to_string(position[2]);
+}
+inline std::string get_brick_name(Bkey key)
+{
+    uint32_t level = 0;
+    math::Vector3ui p;
+    get_brick_level_position(key, level, p);
+    return get_brick_name(level, p);
+}
 
 struct VolumeConfiguration {
     std::string VolumeDirectory;
@@ -199,6 +225,35 @@ public:
         _levels = std::move(levels);
         _source = src;
         _owned = true;
+        return true;
+    }
+
+    // MetaData::load (MetaData.cpp:106-213) plus the BricksManager over the
+    // store: the store of the volume under `volumeDirectory` (default: the
+    // configuration's VolumeDirectory + the volume file's stem, where
+    // load_volume_configuration found bricks.xml), read through a host LRU
+    // of cacheSize big bricks.
+    bool load(const VolumeConfiguration &vc, std::string volumeDirectory = std::string(), size_t cacheSize = 32768)
+    {
+        if (volumeDirectory.empty()) {
+            const std::string f = vc.VolumeFileName;
+            const size_t dot = f.find_last_of('.');
+            volumeDirectory = vc.VolumeDirectory + f.substr(0, dot) + "/";
+        }
+        const gc_volume_cfg c = to_gc(vc);
+        uint32_t ne = 0, nh = 0;
+        if (gc_store_read_meta(volumeDirectory.c_str(), c.brick_size[0], nullptr, &ne, nullptr, &nh) != GC_OK)
+            return false;
+        std::vector<uint64_t> empty(ne);
+        std::vector<float> histo(nh);
+        detail::check(gc_store_read_meta(volumeDirectory.c_str(), c.brick_size[0], empty.data(), &ne, histo.data(), &nh));
+        gc_source *src = nullptr;
+        detail::check(gc_source_create_store(&c, volumeDirectory.c_str(), (uint32_t)cacheSize, &src));
+        reset();
+        _source = src;
+        _owned = true;
+        _empty = std::move(empty);
+        _histo = std::move(histo);
         return true;
     }
 
