@@ -303,6 +303,9 @@ hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision
 hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t tile,
                       uint32_t grid, hipStream_t stream);
 hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t stream);
+// Maximum projection over the launch's ray list (list_rays entries, host-known
+// count) and the views' 8x8 tile packets (packet_off, tiles_x of tile 0).
+hipError_t launch_mop(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t list_rays, hipStream_t stream);
 
 // The rays a launch stopped at an Unmapped brick (the views' stop flags),
 // gathered into the next launch's ray list in screen order: 8x8 tile by 8x8

@@ -1447,6 +1447,283 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     }
 }
 
+// ------------------------------------------------------------------------
+// Maximum projection (RayCastMOPImpl, RayCasters.cu:201-349): the ray keeps
+// the largest sample value, with the DVR march's stepping (level steps,
+// tstep over Empty bricks, stop at the first Unmapped brick, no early
+// termination), and classifies it once: pixel = TF(max), no opacity
+// correction or blending.  The colour state's x holds the normalised maximum
+// (colorStateBuffer[i].x).  One lane per ray, one wave per packet of 64 rays:
+// first the launch's ray list, then the views' 8x8 tiles.
+//
+// Integer voxels keep the maximum box sum (exact; the normalised value
+// sum / D is monotone in it and converts back exactly) and classify it at
+// the exact texture coordinate of tf_coord; float voxels keep the maximum
+// sample.
+
+// TF(value) with linear filtering and clamp addressing, no opacity correction.
+template <int BYTES, typename ACC, typename TFP>
+__device__ inline Acc4<ACC> tf_lookup_sum(uint32_t sum, TFP TF, uint32_t N)
+{
+    constexpr uint32_t D = BYTES == 1 ? 2040u : 524280u;
+    if constexpr (sizeof(ACC) == 8) {
+        const double xf = ((double)sum / (double)D) * (double)N - 0.5;
+        const double fl = floor(xf), a = xf - fl;
+        int i0 = (int)fl, i1 = i0 + 1;
+        i0 = i0 < 0 ? 0 : (i0 > (int)N - 1 ? (int)N - 1 : i0);
+        i1 = i1 < 0 ? 0 : (i1 > (int)N - 1 ? (int)N - 1 : i1);
+        const gf4 T0 = TF[i0], T1 = TF[i1];
+        const double b = 1.0 - a;
+        return Acc4<ACC>{b * (double)T0.x + a * (double)T1.x, b * (double)T0.y + a * (double)T1.y,
+                         b * (double)T0.z + a * (double)T1.z, b * (double)T0.w + a * (double)T1.w};
+    } else {
+        uint32_t q, rem;
+        tf_coord<BYTES, true>(sum, N, q, rem);
+        const uint32_t i0 = q == 0 ? 0u : (q - 1u > N - 1u ? N - 1u : q - 1u);
+        const uint32_t i1 = q > N - 1u ? N - 1u : q;
+        const gf4 T0 = TF[i0], T1 = TF[i1];
+        const float a = (float)rem * (1.0f / (float)(2u * D));
+        return Acc4<ACC>{fmaf(a, T1.x - T0.x, T0.x), fmaf(a, T1.y - T0.y, T0.y), fmaf(a, T1.z - T0.z, T0.z),
+                         fmaf(a, T1.w - T0.w, T0.w)};
+    }
+}
+template <typename ACC, typename TFP>
+__device__ inline Acc4<ACC> tf_lookup_value(ACC s, TFP TF, uint32_t N)
+{
+    const ACC xf = s * (ACC)N - (ACC)0.5;
+    const ACC fl = floor(xf), a = xf - fl;
+    int i0 = (int)fl, i1 = i0 + 1;
+    i0 = i0 < 0 ? 0 : (i0 > (int)N - 1 ? (int)N - 1 : i0);
+    i1 = i1 < 0 ? 0 : (i1 > (int)N - 1 ? (int)N - 1 : i1);
+    const gf4 T0 = TF[i0], T1 = TF[i1];
+    const ACC b = (ACC)1 - a;
+    return Acc4<ACC>{b * (ACC)T0.x + a * (ACC)T1.x, b * (ACC)T0.y + a * (ACC)T1.y, b * (ACC)T0.z + a * (ACC)T1.z,
+                     b * (ACC)T0.w + a * (ACC)T1.w};
+}
+
+template <int BYTES, typename ACC>
+__global__ __launch_bounds__(256) void k_mop(const DvrLaunch p)
+{
+    __shared__ LevelLds lv[GC_MAX_LEVELS];
+    const int tid = threadIdx.x;
+    const gc_geometry *g = p.geom;
+    const uint32_t nl = p.nb_lods;
+    if (tid < (int)nl) {
+        LevelLds &L = lv[tid];
+        for (int a = 0; a < 3; ++a) {
+            L.ior[a] = g->ior[tid][a];
+            L.rdv[a] = g->rdv[tid][a];
+            L.ldf[a] = (float)g->level_dim[tid][a];
+            L.vvf[a] = (float)g->virt_vox[tid][a];
+            L.inv[a] = p.inv_levels[tid][a];
+        }
+        L.lcoord = g->level_coord[tid];
+        L.step = p.lod_step[tid];
+    }
+    __syncthreads();
+    const uint32_t lane = tid & 63u;
+    const uint32_t list_count = p.list ? *p.list_count : 0u;
+    const uint32_t list_packets = (list_count + 63u) / 64u;
+    const uint32_t pk = blockIdx.x * 4u + (uint32_t)(tid >> 6);
+    const uint32_t n_views = p.views ? p.n_views : 1u;
+    if (pk >= list_packets + p.n_packets) return;   // whole waves
+    // the lane's ray: (view, pixel)
+    bool valid = false;
+    uint32_t v = 0, x = 0, y = 0;
+    if (pk < list_packets) {
+        const uint64_t idx = (uint64_t)pk * 64u + lane;
+        if (idx < list_count) {
+            const uint32_t e = p.list[idx];
+            v = e >> DVR_PIXEL_BITS;
+            const uint32_t px = e & ((1u << DVR_PIXEL_BITS) - 1u);
+            const uint32_t W = p.views ? p.views[v].width : p.view0.width;
+            x = px % W;
+            y = px / W;
+            valid = true;
+        }
+    } else {
+        const uint32_t tp = pk - list_packets;
+        if (p.views)
+            while (v + 1 < n_views && p.views[v + 1].packet_off <= tp) ++v;
+        const DvrView &V = p.views ? p.views[v] : p.view0;
+        const uint32_t tt = tp - V.packet_off;
+        x = (tt % V.tiles_x) * 8u + lane % 8u;
+        y = (tt / V.tiles_x) * 8u + lane / 8u;
+        valid = x < V.width && y < V.height;
+    }
+    uint32_t n_samples = 0, n_visits = 0;
+    bool stopped = false;
+    if (valid) {
+        const DvrView &V = p.views ? p.views[v] : p.view0;
+        const RaySetup R = setup_ray(V, x, y);
+        const uint32_t pix = y * V.width + x;
+        const float tb = gp(V.tbuf)[pix];
+        float tnear = fmaxf(R.tnear, tb);
+        // the stored maximum
+        constexpr double D = BYTES == 1 ? 2040.0 : (BYTES == 2 ? 524280.0 : 1.0);
+        ACC cs[4];
+        if constexpr (sizeof(ACC) == 8) {
+            const gd4 c = gp(reinterpret_cast<const gd4 *>(V.cstate))[pix];
+            cs[0] = c.x; cs[1] = c.y; cs[2] = c.z; cs[3] = c.w;
+        } else {
+            const gf4 c = gp(reinterpret_cast<const gf4 *>(V.cstate))[pix];
+            cs[0] = c.x; cs[1] = c.y; cs[2] = c.z; cs[3] = c.w;
+        }
+        uint32_t msum = 0;                 // integer voxels: maximum box sum
+        ACC mval = cs[0];                  // float voxels: maximum sample
+        double mvd = (double)cs[0];
+        if constexpr (BYTES != 4) {
+            const double r = rint((double)cs[0] * D);
+            msum = r <= 0.0 ? 0u : (uint32_t)r;
+        }
+        if (!R.hit) {
+            gp(V.pixels)[pix] = 0u;
+        } else {
+            const uint32_t bs = g->brick_size, cov = g->covering, inner = g->inner;
+            const uint32_t MX = g->mrpd_dim[0], MY = g->mrpd_dim[1];
+            const uint64_t bbytes = (uint64_t)bs * bs * bs * BYTES;
+            const uint32_t row = bs * BYTES, plane = bs * bs * BYTES;
+            const uint32_t corner = cov >= 1 ? (cov - 1) * (plane + row + BYTES) : 0u;
+            const uint8_t *pool = static_cast<const uint8_t *>(p.pool);
+            const uint32_t ts = p.ts_dev ? *p.ts_dev : p.ts;
+            if (tnear < 0.f) tnear = 0.f;
+            float t = tnear;
+            const float tfar = R.tfar;
+            F3 pos = R.origin + R.dir * tnear;
+            if (V.rpos && tb > 0.f && tb == tnear) {
+                const gf4 q = gp(reinterpret_cast<const gf4 *>(V.rpos))[pix];
+                pos = f3(q.x, q.y, q.z);
+            }
+            const F3 dir = R.dir;
+            const float epx = fabsf(dot(dir - R.dxn, R.dleft) * tfar);
+            const float epy = fabsf(dot(dir - R.dyn, R.dup) * tfar);
+            // level at t: the last one of the consecutive levels already
+            // reached (compute_LOD); the next switch computed when needed
+            uint32_t lod = 0;
+            float tnext = FINF;
+            for (uint32_t l = 1; l < nl; ++l) {
+                const float T = level_switch(t, tfar, epx, epy, spacing(R.dleft, lv[l].inv), spacing(R.dup, lv[l].inv));
+                if (T <= t) {
+                    lod = l;
+                } else {
+                    tnext = T;
+                    break;
+                }
+            }
+            uint32_t pcell = NONE, ew = 0, slot = 0;
+            for (;;) {
+                if (!(t < tfar)) break;
+                while (t >= tnext) {
+                    ++lod;
+                    tnext = lod + 1 < nl ? level_switch(t, tfar, epx, epy, spacing(R.dleft, lv[lod + 1].inv),
+                                                        spacing(R.dup, lv[lod + 1].inv))
+                                         : FINF;
+                }
+                const LevelLds &L = lv[lod];
+                F3 tp = pos * 0.495f + f3(0.495f, 0.495f, 0.495f);
+                tp.z = 1.0f - tp.z;
+                const F3 np = f3(clamp099(tp.x), clamp099(tp.y), clamp099(tp.z));
+                const F3 pr = np * f3(L.ior[0], L.ior[1], L.ior[2]);
+                const F3 pv = pr * f3(L.rdv[0], L.rdv[1], L.rdv[2]);
+                const uint32_t bx = (uint32_t)(pv.x * L.ldf[0]), by = (uint32_t)(pv.y * L.ldf[1]),
+                               bz = (uint32_t)(pv.z * L.ldf[2]);
+                const uint32_t cell = L.lcoord + bx + MX * (by + MY * bz);
+                if (cell != pcell) {
+                    pcell = cell;
+                    const uint32_t w = p.cells[cell];
+                    ew = word_state(w);
+                    slot = word_slot(w);
+                    ++n_visits;
+                    if (ew == 1) p.usage[slot] = ts;
+                }
+                if (ew == 0) {
+                    // raise_brick_request with the real-volume position, stop
+                    const uint32_t rx = L.lcoord + (uint32_t)(pr.x * L.ldf[0]);
+                    const uint32_t ry = (uint32_t)(pr.y * L.ldf[1]);
+                    const uint32_t rz = (uint32_t)(pr.z * L.ldf[2]);
+                    p.request[rx + (uint64_t)MX * (ry + (uint64_t)MY * rz)] = ts;
+                    stopped = true;
+                    break;
+                }
+                if (ew != 1) {
+                    // Empty brick: tstep (RayCasters.cu:308-316)
+                    t += p.tstep;
+                    pos = pos + dir * p.tstep;
+                    continue;
+                }
+                const uint32_t vx = (uint32_t)(pv.x * L.vvf[0]), vy = (uint32_t)(pv.y * L.vvf[1]),
+                               vz = (uint32_t)(pv.z * L.vvf[2]);
+                uint32_t lx = vx - bx * inner, ly = vy - by * inner, lz = vz - bz * inner;
+                if (lx >= inner || ly >= inner || lz >= inner) {
+                    lx = vx % inner;
+                    ly = vy % inner;
+                    lz = vz % inner;
+                }
+                if (BYTES != 4 && cov >= 1) {
+                    const uint32_t vi = (lz * bs + ly) * bs + lx;
+                    const uint32_t sum =
+                        box_sum<BYTES == 4 ? 1 : BYTES>(pool + (uint64_t)slot * bbytes + corner + (uint64_t)vi * BYTES, row, plane);
+                    msum = sum > msum ? sum : msum;
+                } else {
+                    uint32_t o[3];
+                    slot_origin(*g, slot, o);
+                    const int ex = (int)(cov + o[0] + lx), ey = (int)(cov + o[1] + ly), ez = (int)(cov + o[2] + lz);
+                    float fsum = 0.f;
+                    double dsum = 0.0;
+                    for (int kk = 0; kk < 2; ++kk)
+                        for (int j = 0; j < 2; ++j)
+                            for (int i = 0; i < 2; ++i) {
+                                const float tv = texel<BYTES>(pool, g, ex - 1 + i, ey - 1 + j, ez - 1 + kk);
+                                fsum += tv;
+                                dsum += tv;
+                            }
+                    if constexpr (BYTES == 4) {
+                        const double sd = dsum / 8.0;
+                        const ACC sv = sizeof(ACC) == 8 ? (ACC)sd : (ACC)(fsum / 8.0f);
+                        if (sd > mvd) {
+                            mvd = sd;
+                            mval = sv;
+                        }
+                    } else {
+                        const uint32_t sum = (uint32_t)dsum;
+                        msum = sum > msum ? sum : msum;
+                    }
+                }
+                ++n_samples;
+                t += L.step;
+                pos = pos + dir * L.step;
+            }
+            // state (RayCasters.cu:288-329) and pixel = TF(max)
+            ACC mx;
+            Acc4<ACC> c;
+            const auto TF = gp(reinterpret_cast<const gf4 *>(V.tf));
+            if constexpr (BYTES != 4) {
+                mx = (ACC)((double)msum / D);
+                c = tf_lookup_sum<BYTES, ACC>(msum, TF, V.tf_size);
+            } else {
+                mx = mval;
+                c = tf_lookup_value<ACC>(mval, TF, V.tf_size);
+            }
+            gp(V.tbuf)[pix] = t;
+            if (V.rpos) gp(reinterpret_cast<gf4 *>(V.rpos))[pix] = gf4{pos.x, pos.y, pos.z, t};
+            if constexpr (sizeof(ACC) == 8) gp(reinterpret_cast<gd4 *>(V.cstate))[pix] = gd4{mx, cs[1], cs[2], cs[3]};
+            else gp(reinterpret_cast<gf4 *>(V.cstate))[pix] = gf4{mx, cs[1], cs[2], cs[3]};
+            gp(V.pixels)[pix] = pack(c.x, c.y, c.z, c.w);
+            if (stopped && V.stop) gp(V.stop)[pix] = 1;
+        }
+    }
+    if (p.stats) {
+        const uint32_t s0 = wave_sum(n_samples), s1 = wave_sum(n_visits);
+        const uint32_t s2 = (uint32_t)__popcll(__ballot(valid)), s3 = (uint32_t)__popcll(__ballot(stopped));
+        if (lane == 0) {
+            if (s0) atomicAdd((unsigned long long *)&p.stats->mapped_samples, (unsigned long long)s0);
+            if (s1) atomicAdd((unsigned long long *)&p.stats->brick_visits, (unsigned long long)s1);
+            if (s2) atomicAdd((unsigned long long *)&p.stats->pixels, (unsigned long long)s2);
+            if (s3) atomicAdd((unsigned long long *)&p.stats->unmapped_rays, (unsigned long long)s3);
+        }
+    }
+}
+
 template <int BYTES, typename ACC>
 const void *kernel_of(uint32_t tile, bool mt, bool bs32)
 {
@@ -1626,6 +1903,24 @@ uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_
     return std::max(1u, need);
 }
 
+hipError_t launch_mop(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t list_rays, hipStream_t stream)
+{
+    if (bytes != 1 && bytes != 2 && bytes != 4) return hipErrorInvalidValue;
+    if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
+    if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
+    if (p.list && !p.list_count) return hipErrorInvalidValue;
+    const uint64_t packets = (uint64_t)(p.list ? (list_rays + 63u) / 64u : 0u) + p.n_packets;
+    if (packets == 0) return hipSuccess;
+    const void *k = nullptr;
+    if (precision == 64) k = bytes == 1 ? reinterpret_cast<const void *>(&k_mop<1, double>)
+                             : bytes == 2 ? reinterpret_cast<const void *>(&k_mop<2, double>)
+                                          : reinterpret_cast<const void *>(&k_mop<4, double>);
+    else k = bytes == 1 ? reinterpret_cast<const void *>(&k_mop<1, float>)
+             : bytes == 2 ? reinterpret_cast<const void *>(&k_mop<2, float>)
+                          : reinterpret_cast<const void *>(&k_mop<4, float>);
+    void *args[] = {const_cast<DvrLaunch *>(&p)};
+    return hipLaunchKernel(k, dim3((uint32_t)((packets + 3u) / 4u)), dim3(256), args, 0, stream);
+}
 hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t grid,
                           hipStream_t stream, uint32_t group)
 {

@@ -100,6 +100,7 @@ extern "C" int gc_render_views_noalloc(gc_cache *c, const gc_view *views, const 
         if (v.width == 0 || v.height == 0 || (uint64_t)v.width * v.height >= (1ull << DVR_PIXEL_BITS))
             return fail(GC_ERR_ARG, "view " + std::to_string(i) + ": bad size");
         if (v.precision != prec) return fail(GC_ERR_ARG, "view " + std::to_string(i) + ": precision differs");
+        if (v.compositing != 0) return fail(GC_ERR_ARG, "view " + std::to_string(i) + ": DVR only (compositing 0)");
         if (!d_tfs[i] || tf_sizes[i] == 0 || !d_pixels[i] || !d_tbufs[i] || !d_cstates[i])
             return fail(GC_ERR_ARG, "view " + std::to_string(i) + ": missing buffer");
         DvrView &D = h[i];

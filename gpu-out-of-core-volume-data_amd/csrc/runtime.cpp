@@ -1088,8 +1088,9 @@ static void view_of(const gc_view &v, const float *d_tf, uint32_t tf_size, const
 {
     out.width = v.width;
     out.height = v.height;
-    out.tiles_x = tile_cols(v.tile, v.width);
-    out.tile_pair = tile_pair(v.tile);
+    const uint32_t tile = v.compositing == 1 ? 0u : v.tile;   // maximum projection: 8x8 tiles
+    out.tiles_x = tile_cols(tile, v.width);
+    out.tile_pair = tile_pair(tile);
     out.packet_off = 0;
     std::memcpy(out.m, v.inv_mv, sizeof(out.m));
     std::memcpy(out.bmin, v.bbox_min, sizeof(out.bmin));
@@ -1179,6 +1180,8 @@ static int batch_place(Batch &B, uint32_t slot, const gc_view &v, const float *d
         return fail(GC_ERR_ARG, "slot " + std::to_string(slot) + ": bad view size");
     if (v.precision != B.precision) return fail(GC_ERR_ARG, "slot " + std::to_string(slot) + ": precision differs from the batch");
     if (!d_tf || tf_size == 0) return fail(GC_ERR_ARG, "slot " + std::to_string(slot) + ": no transfer function");
+    if (v.compositing != 0)
+        return fail(GC_ERR_ARG, "slot " + std::to_string(slot) + ": batches render DVR only (compositing 0)");
     Batch::Slot &S = B.slots[slot];
     S.used = true;
     S.starting = true;
@@ -2005,9 +2008,19 @@ int gc_render_dvr(gc_cache *c, const gc_view *view, const float *d_tf, uint32_t 
     if (view->width != F.w || view->height != F.h) return fail(GC_ERR_ARG, "view and frame sizes differ");
     if (view->precision != F.precision) return fail(GC_ERR_ARG, "view and frame precision differ");
     if (view->tile >= GC_TILES) return fail(GC_ERR_ARG, "tile must be 0..5");
+    if (view->compositing > 1) return fail(GC_ERR_ARG, "compositing must be 0 (DVR) or 1 (MOP)");
     DvrLaunch L{};
     launch_base(C, L);
     view_of(*view, d_tf, tf_size, F, L.kexp[0], L.view0);
+    if (view->compositing == 1) {
+        // maximum projection: one launch over the 8x8 tiles
+        L.n_views = 1;
+        L.n_packets = tile_packets(0, view->width, view->height);
+        L.stats = d_stats;
+        hipStream_t s = stream ? (hipStream_t)stream : C.stream;
+        GC_HIP(launch_mop(L, C.vc.encoded_bytes, view->precision, 0, s));
+        return GC_OK;
+    }
     L.exact_resume = L.view0.rpos ? 1u : 0u;
     finish_exact(L, &L.view0, 1);
     L.tf_lds_entries = dvr_assign_tf_lds(&L.view0, 1, C.vc.encoded_bytes, &L.tf_lds_tables);
@@ -2058,8 +2071,11 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
     }
     GC_HIP(launch_fill32(F.ctl, 8, 0, s));
     GC_HIP(launch_fill32(F.ctl + 8 + out, 1, 0, s));
-    *lane = !(!first && list_rays <= C.ray_list_max && dvr_ray_supported(L, C.vc.encoded_bytes));
-    if (!*lane) {
+    *lane = view.compositing == 1 || !(!first && list_rays <= C.ray_list_max && dvr_ray_supported(L, C.vc.encoded_bytes));
+    if (view.compositing == 1) {
+        if (first) L.n_packets = tile_packets(0, view.width, view.height);
+        GC_HIP(launch_mop(L, C.vc.encoded_bytes, view.precision, first ? 0u : list_rays, s));
+    } else if (!*lane) {
         // a short list of resumed rays: a group of lanes per ray
         L.stats = d_stats + 1;
         GC_HIP(launch_dvr_ray(L, C.vc.encoded_bytes, view.precision, list_rays,
@@ -2112,7 +2128,7 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
         GC_HIP(hipEventCreate(&C.ev1));
     }
     if (view->width != f->impl.w || view->height != f->impl.h || view->precision != f->impl.precision ||
-        view->tile >= GC_TILES)
+        view->tile >= GC_TILES || view->compositing > 1)
         return fail(GC_ERR_ARG, "view does not match the frame");
     GC_HIP(hipMemsetAsync(C.d_frame_stats, 0, 2 * sizeof(gc_render_stats), C.stream));
     int cur = 0;

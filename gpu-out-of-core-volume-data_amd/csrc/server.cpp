@@ -108,8 +108,8 @@ struct Server {
     uint64_t slot_bytes() const
     {
         // pixels + t + colour state + resume position + opacity product + two
-        // ray-list entries per pixel
-        return (uint64_t)cfg.max_views * cfg.slot_pixels * (4 + 4 + (cfg.precision == 64 ? 32 : 16) + 16 + 8 + 8);
+        // ray-list entries + the stop flag per pixel
+        return (uint64_t)cfg.max_views * cfg.slot_pixels * (4 + 4 + (cfg.precision == 64 ? 32 : 16) + 16 + 8 + 8 + 1);
     }
     struct Place {
         Volume *v;
@@ -457,6 +457,7 @@ int gc_server_submit(gc_server *s, uint32_t volume, const gc_view *view, const f
     if ((uint64_t)view->width * view->height > S.cfg.slot_pixels || view->width == 0 || view->height == 0)
         return fail(GC_ERR_ARG, "view larger than slot_pixels");
     if (view->precision != S.cfg.precision) return fail(GC_ERR_ARG, "view precision differs from the server's");
+    if (view->compositing != 0) return fail(GC_ERR_ARG, "the server renders DVR only (compositing 0)");
     auto r = std::make_unique<Request>();
     r->view = *view;
     r->tf_size = tf_size;

@@ -67,8 +67,16 @@ class View(C.Structure):
     _fields_ = [
         ("width", u32), ("height", u32), ("inv_mv", f32 * 12), ("bbox_min", f32 * 3), ("bbox_max", f32 * 3),
         ("alpha_stop", f32), ("precision", u32), ("tile", u32), ("refill_min", u32), ("march", u32),
-        ("resume", u32),
+        ("resume", u32), ("compositing", u32),
     ]
+
+
+MSG_TF_MAX_ENTRIES = 2047
+
+
+class ViewRequest(C.Structure):
+    _fields_ = [("inv_mv", f32 * 12), ("bbox_min", f32 * 3), ("bbox_max", f32 * 3), ("tf_entries", u32),
+                ("have", u32), ("tf", f32 * (MSG_TF_MAX_ENTRIES * 4))]
 
 
 class RenderStats(C.Structure):
@@ -146,6 +154,8 @@ EXPORTS = [
     "gc_source_create_synthetic_gpu", "gc_source_get_device", "gc_render_views_noalloc", "gc_cache_probe",
     "gc_brick_key", "gc_brick_key_unpack", "gc_store_brick_path", "gc_lz4_bound", "gc_lz4_compress",
     "gc_lz4_decompress", "gc_store_write", "gc_source_create_store", "gc_store_read_meta", "gc_store_stats",
+    "gc_msg_encode_camera", "gc_msg_encode_transfer_function", "gc_msg_encode_bounding_box", "gc_msg_id",
+    "gc_view_request_init", "gc_view_request_apply", "gc_view_request_view",
 ]
 
 _lib = None
@@ -231,6 +241,12 @@ def lib():
             "gc_source_create_store": [C.POINTER(VolumeCfg), C.c_char_p, u32, C.POINTER(vp)],
             "gc_store_read_meta": [C.c_char_p, u32, P, C.POINTER(u32), P, C.POINTER(u32)],
             "gc_store_stats": [P, P],
+            "gc_msg_encode_camera": [P, P, SZ, C.POINTER(SZ)],
+            "gc_msg_encode_transfer_function": [P, u32, P, SZ, C.POINTER(SZ)],
+            "gc_msg_encode_bounding_box": [P, P, P, SZ, C.POINTER(SZ)],
+            "gc_msg_id": [P, SZ, C.POINTER(u32)],
+            "gc_view_request_init": [C.POINTER(ViewRequest)], "gc_view_request_apply": [C.POINTER(ViewRequest), P, SZ],
+            "gc_view_request_view": [C.POINTER(ViewRequest), u32, u32, u32, C.POINTER(View)],
         }
         missing = [n for n in EXPORTS if n not in sig]
         if missing:
@@ -594,7 +610,7 @@ class Frame:
 
 
 def make_view(width, height, inv_mv, alpha_stop=0.95, precision=32, tile=0,
-              bbox=((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)), refill_min=0, march=0, resume=0):
+              bbox=((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)), refill_min=0, march=0, resume=0, compositing=0):
     v = View()
     v.width, v.height = width, height
     v.inv_mv[:] = [float(x) for x in np.asarray(inv_mv, np.float32).reshape(12)]
@@ -606,6 +622,7 @@ def make_view(width, height, inv_mv, alpha_stop=0.95, precision=32, tile=0,
     v.refill_min = refill_min
     v.march = march
     v.resume = resume
+    v.compositing = compositing
     return v
 
 
@@ -819,6 +836,50 @@ def store_read_meta(volume_dir, brick_size):
     check(lib().gc_store_read_meta(os.fsencode(volume_dir), brick_size, ptr(keys), C.byref(ne), ptr(histo),
                                    C.byref(nh)))
     return keys[:ne.value], histo[:nh.value]
+
+
+# ------------------------------------------------- view-request messages
+MSG_CONFIGURATION, MSG_H264_CHUNKS, MSG_CAMERA, MSG_TRANSFER_FUNCTION, MSG_BOUNDING_BOX = 1, 2, 3, 4, 5
+
+
+def _msg(fn, *args):
+    out = C.create_string_buffer(32768)
+    n = C.c_size_t(0)
+    check(fn(*args, out, 32768, C.byref(n)))
+    return out.raw[:n.value]
+
+
+def msg_camera(inv_view):
+    return _msg(lib().gc_msg_encode_camera, ptr(np.ascontiguousarray(inv_view, np.float32).reshape(12)))
+
+
+def msg_transfer_function(rgba):
+    rgba = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4)
+    return _msg(lib().gc_msg_encode_transfer_function, ptr(rgba), len(rgba))
+
+
+def msg_bounding_box(bmin, bmax):
+    return _msg(lib().gc_msg_encode_bounding_box, ptr(np.asarray(bmin, np.float32)), ptr(np.asarray(bmax, np.float32)))
+
+
+class Request:
+    """A client's view request folded from the reference's messages."""
+
+    def __init__(self):
+        self.r = ViewRequest()
+        check(lib().gc_view_request_init(C.byref(self.r)))
+
+    def apply(self, msg):
+        buf = C.create_string_buffer(bytes(msg), len(msg))
+        check(lib().gc_view_request_apply(C.byref(self.r), buf, len(msg)))
+
+    def view(self, width, height, precision=32):
+        v = View()
+        check(lib().gc_view_request_view(C.byref(self.r), width, height, precision, C.byref(v)))
+        return v
+
+    def tf(self):
+        return np.ctypeslib.as_array(self.r.tf)[:4 * self.r.tf_entries].reshape(-1, 4).copy()
 
 
 def set_ray_list_max(cache, n):

@@ -364,7 +364,56 @@ typedef struct gc_view {
                                     continuation at the stored sample position
                                     (no reference counterpart), so every launch
                                     sequence gives the one-pass render.       */
+    uint32_t compositing;        /* 0: DVR (RayCastDVRImpl, RayCasters.cu:36-198),
+                                    1: maximum projection (RayCastMOPImpl,
+                                    :201-349; gc_render_dvr and gc_render_frame
+                                    only; 8x8 ray tiles)                       */
 } gc_view;
+
+/* ------------------------------------------------- view-request messages */
+
+/* The reference's network messages (libNetwork, Message.hpp:18-60): a
+ * little-endian uint32 id, then the payload, at most 32768 bytes.  Camera
+ * (MessageCamera.cpp:24-45): 12 floats, the 3x4 inverse view matrix rows the
+ * renderer takes (update_CUDA_inv_view_model_matrix, RayCasters.cu:587-591).
+ * Transfer function (MessageTransferFunction.cpp): float4 RGBA entries, their
+ * count given by the size (the reference's message holds 256 * 4 = 1024).
+ * Bounding box (MessageBoundingBox.cpp): min and max, 3 floats each.  The
+ * configuration and H.264 messages (ids 1, 2) are recognised and ignored. */
+#define GC_MSG_CONFIGURATION 1
+#define GC_MSG_H264_CHUNKS 2
+#define GC_MSG_CAMERA 3
+#define GC_MSG_TRANSFER_FUNCTION 4
+#define GC_MSG_BOUNDING_BOX 5
+#define GC_MSG_MAX_BYTES 32768
+#define GC_MSG_TF_ENTRIES 1024          /* entries of the reference's TF message  */
+#define GC_MSG_TF_MAX_ENTRIES 2047      /* the most a 32768-byte message holds    */
+int gc_msg_encode_camera(const float inv_view[12], void *out, size_t cap, size_t *n);
+int gc_msg_encode_transfer_function(const float *rgba, uint32_t entries, void *out, size_t cap, size_t *n);
+int gc_msg_encode_bounding_box(const float bbox_min[3], const float bbox_max[3], void *out, size_t cap,
+                               size_t *n);
+/* Id of a message (GC_ERR_ARG if shorter than 4 bytes or over 32768). */
+int gc_msg_id(const void *msg, size_t n, uint32_t *id);
+
+/* A client's view request assembled from messages: the last camera, transfer
+ * function and bounding box received (have: bit 0 camera, 1 TF, 2 box). */
+typedef struct gc_view_request {
+    float    inv_mv[12];
+    float    bbox_min[3], bbox_max[3];
+    uint32_t tf_entries;
+    uint32_t have;
+    float    tf[GC_MSG_TF_MAX_ENTRIES * 4];
+} gc_view_request;
+/* Empty request: identity camera at distance 0, box [-1,1]^3, no TF. */
+int gc_view_request_init(gc_view_request *r);
+/* Apply one message (GC_ERR_ARG: malformed or of an unknown id; the request
+ * is then unchanged). */
+int gc_view_request_apply(gc_view_request *r, const void *msg, size_t n);
+/* The request as a gc_view (defaults for everything messages do not carry:
+ * alpha stop 0.95, tile 0, ...); GC_ERR_STATE until a camera arrived.  Its
+ * transfer function is r->tf, r->tf_entries entries. */
+int gc_view_request_view(const gc_view_request *r, uint32_t width, uint32_t height, uint32_t precision,
+                         gc_view *out);
 
 /* Per-view progressive state (tBuffer, colorStateBuffer, pixel buffer) on the
  * device (VolumeRayCaster.cu:225-238). */

@@ -650,6 +650,7 @@ typedef struct {
     float *rpos;
     uint32_t *pixels;
     int precision;
+    int mop;                      /* 1: maximum projection (RayCastMOPImpl) */
     uint32_t next_row, end_row;
     pthread_mutex_t lock;
     uint64_t stats[4];
@@ -961,6 +962,152 @@ static void dvr_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
     job->pixels[i] = p64 ? pack_abgr(fc[0], fc[1], fc[2], fc[3]) : pack_abgr_f(ff[0], ff[1], ff[2], ff[3]);
 }
 
+/* RayCastMOPImpl (RayCasters.cu:201-349): the ray set-up, level of detail
+ * and stepping of the DVR march (level steps, tstep over Empty bricks, stop
+ * at the first Unmapped brick), with no early termination; the ray keeps the
+ * largest normalised sample (colorStateBuffer[i].x) and the pixel is the
+ * transfer function at it, without opacity correction or blending. */
+static void mop_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
+{
+    or_cache *c = job->c;
+    const or_view *V = job->v;
+    const or_geom *g = &c->g;
+    const uint32_t W = V->width, H = V->height;
+    const uint32_t ts = c->ts;
+    const uint32_t bs = c->cfg.brick_size, cov = c->cfg.covering;
+    const uint32_t inner = bs - 2 * cov;
+    const float *M = V->inv_mv;
+    const int p64 = job->precision == 64;
+
+    uint32_t xn = x + 1;
+    if (xn > W) xn = x - 1;
+    uint32_t yn = y + 1;
+    if (yn > H) yn = y - 1;
+    const float u = (((float)x + 0.5f) / (float)W) * 2.0f - 1.0f;
+    const float un = (((float)xn + 0.5f) / (float)W) * 2.0f - 1.0f;
+    const float vv = (((float)y + 0.5f) / (float)H) * 2.0f - 1.0f;
+    const float vn = (((float)yn + 0.5f) / (float)H) * 2.0f - 1.0f;
+    const f3 origin = mk(M[3], M[7], M[11]);
+    const f3 dir = mulm(M, norm3(mk(u, vv, -2.f)));
+    const f3 dxn = mulm(M, norm3(mk(un, vv, -2.f)));
+    const f3 dyn = mulm(M, norm3(mk(u, vn, -2.f)));
+    const f3 dleft = norm3(cross3(dir, dyn));
+    const f3 dup = norm3(cross3(dir, dxn));
+    const f3 invr = mk(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+    const f3 tbot = mul(invr, sub(mk(V->bbox_min[0], V->bbox_min[1], V->bbox_min[2]), origin));
+    const f3 ttop = mul(invr, sub(mk(V->bbox_max[0], V->bbox_max[1], V->bbox_max[2]), origin));
+    const f3 tmin = mk(fminf(ttop.x, tbot.x), fminf(ttop.y, tbot.y), fminf(ttop.z, tbot.z));
+    const f3 tmax = mk(fmaxf(ttop.x, tbot.x), fmaxf(ttop.y, tbot.y), fmaxf(ttop.z, tbot.z));
+    float tnear = fmaxf(fmaxf(tmin.x, tmin.y), fmaxf(tmin.x, tmin.z));
+    const float tfar = fminf(fminf(tmax.x, tmax.y), fminf(tmax.x, tmax.z));
+    const int hit = tfar > tnear;
+
+    const uint64_t i = (uint64_t)y * W + x;
+    const float tb = job->tbuf[i];
+    tnear = fmaxf(tnear, tb);
+    st[2] += 1;
+    if (!hit) {
+        job->pixels[i] = 0u;
+        return;
+    }
+    if (tnear < 0.f) tnear = 0.f;
+    float t = tnear;
+    f3 pos = add(origin, muls(dir, tnear));
+    if (job->rpos && tb > 0.f && tb == t) pos = mk(job->rpos[4 * i], job->rpos[4 * i + 1], job->rpos[4 * i + 2]);
+    const float epx = fabsf(dot3(sub(dir, dxn), dleft) * tfar);
+    const float epy = fabsf(dot3(sub(dir, dyn), dup) * tfar);
+    uint32_t prev[3] = {UINT32_MAX, UINT32_MAX, UINT32_MAX};
+    uint32_t entry[4] = {0, 0, 0, 0};
+    /* maxOpacity: FP64 the sample s = sum / D in double, FP32 in float */
+    double mx = job->cstate[4 * i];
+    float mxf = (float)mx;
+    const double D = c->cfg.bytes == 1 ? 2040.0 : (c->cfg.bytes == 2 ? 524280.0 : 8.0);
+    int unmapped = 0;
+    while (t < tfar) {
+        const float alpha = t / tfar;
+        const float pdx = epx * alpha, pdy = epy * alpha;
+        f3 tp = add(muls(pos, 0.495f), mk(0.495f, 0.495f, 0.495f));
+        tp.z = 1.0f - tp.z;
+        const f3 np = mk(fmaxf(0.f, fminf(tp.x, 0.99f)), fmaxf(0.f, fminf(tp.y, 0.99f)),
+                         fmaxf(0.f, fminf(tp.z, 0.99f)));
+        const float dl[3] = {dleft.x, dleft.y, dleft.z}, du[3] = {dup.x, dup.y, dup.z};
+        const uint32_t lod = or_compute_lod(V, dl, du, pdx, pdy);
+        const f3 pr = mul(np, mk(g->ior[lod][0], g->ior[lod][1], g->ior[lod][2]));
+        const f3 pv = mul(pr, mk(g->rdv[lod][0], g->rdv[lod][1], g->rdv[lod][2]));
+        const uint32_t mc[3] = {g->level_coord[lod] + (uint32_t)(pv.x * (float)g->level_dim[lod][0]),
+                                (uint32_t)(pv.y * (float)g->level_dim[lod][1]),
+                                (uint32_t)(pv.z * (float)g->level_dim[lod][2])};
+        if (mc[0] != prev[0] || mc[1] != prev[1] || mc[2] != prev[2]) {
+            const uint64_t mi = mc[0] + (uint64_t)g->mrpd_dim[0] * (mc[1] + (uint64_t)g->mrpd_dim[1] * mc[2]);
+            memcpy(entry, c->mrpd + 4 * mi, sizeof(entry));
+            memcpy(prev, mc, sizeof(prev));
+            st[1] += 1;
+        }
+        if (entry[3] == 0) {
+            const uint32_t rx = g->level_coord[lod] + (uint32_t)(pr.x * (float)g->level_dim[lod][0]);
+            const uint32_t ry = (uint32_t)(pr.y * (float)g->level_dim[lod][1]);
+            const uint32_t rz = (uint32_t)(pr.z * (float)g->level_dim[lod][2]);
+            const uint64_t ri = rx + (uint64_t)g->mrpd_dim[0] * (ry + (uint64_t)g->mrpd_dim[1] * rz);
+            __atomic_store_n(&c->req[ri], ts, __ATOMIC_RELAXED);
+            unmapped = 1;
+            st[3] += 1;
+            break;
+        }
+        if (entry[3] == 2) {
+            t += V->tstep;
+            if (t > tfar) break;
+            pos = add(pos, muls(dir, V->tstep));
+            continue;
+        }
+        {
+            const uint32_t ux = entry[0] / bs, uy = entry[1] / bs, uz = entry[2] / bs;
+            const uint64_t ui = ux + (uint64_t)c->cfg.cache_size[0] * (uy + (uint64_t)c->cfg.cache_size[1] * uz);
+            __atomic_store_n(&c->usage[ui], ts, __ATOMIC_RELAXED);
+        }
+        const uint32_t ex = cov + entry[0] + ((uint32_t)(pv.x * (float)g->virt_vox[lod][0])) % inner;
+        const uint32_t ey = cov + entry[1] + ((uint32_t)(pv.y * (float)g->virt_vox[lod][1])) % inner;
+        const uint32_t ez = cov + entry[2] + ((uint32_t)(pv.z * (float)g->virt_vox[lod][2])) % inner;
+        float fsum = 0.f;
+        const double sum = sample_box(c, ex, ey, ez, &fsum);
+        st[0] += 1;
+        /* maxOpacity = max(maxOpacity, sample) */
+        const double s = sum / D;
+        if (s > mx) mx = s;
+        const float sf = c->cfg.bytes == 4 ? fsum / 8.0f : (float)sum / (float)D;
+        if (sf > mxf) mxf = sf;
+        t += V->lod_step[lod];
+        pos = add(pos, muls(dir, V->lod_step[lod]));
+    }
+    (void)unmapped;
+    job->tbuf[i] = t;
+    job->cstate[4 * i] = p64 ? mx : (double)mxf;
+    if (job->rpos) {
+        job->rpos[4 * i] = pos.x; job->rpos[4 * i + 1] = pos.y; job->rpos[4 * i + 2] = pos.z;
+        job->rpos[4 * i + 3] = t;
+    }
+    /* classification: tex1D(tfTex, maxOpacity), linear, normalised, clamp */
+    const uint32_t N = V->tf_size;
+    if (p64) {
+        const double xf = mx * (double)N - 0.5;
+        const double fl = floor(xf), a = xf - fl;
+        int64_t i0 = (int64_t)fl, i1 = i0 + 1;
+        i0 = i0 < 0 ? 0 : (i0 > (int64_t)N - 1 ? (int64_t)N - 1 : i0);
+        i1 = i1 < 0 ? 0 : (i1 > (int64_t)N - 1 ? (int64_t)N - 1 : i1);
+        double col[4];
+        for (int k = 0; k < 4; ++k) col[k] = (1.0 - a) * (double)V->tf[4 * i0 + k] + a * (double)V->tf[4 * i1 + k];
+        job->pixels[i] = pack_abgr(col[0], col[1], col[2], col[3]);
+    } else {
+        const float xf = mxf * (float)N - 0.5f;
+        const float fl = floorf(xf), a = xf - fl;
+        int64_t i0 = (int64_t)fl, i1 = i0 + 1;
+        i0 = i0 < 0 ? 0 : (i0 > (int64_t)N - 1 ? (int64_t)N - 1 : i0);
+        i1 = i1 < 0 ? 0 : (i1 > (int64_t)N - 1 ? (int64_t)N - 1 : i1);
+        float col[4];
+        for (int k = 0; k < 4; ++k) col[k] = (1.0f - a) * V->tf[4 * i0 + k] + a * V->tf[4 * i1 + k];
+        job->pixels[i] = pack_abgr_f(col[0], col[1], col[2], col[3]);
+    }
+}
+
 static void *dvr_worker(void *arg)
 {
     dvr_job *job = (dvr_job *)arg;
@@ -970,7 +1117,10 @@ static void *dvr_worker(void *arg)
         const uint32_t y = job->next_row++;
         pthread_mutex_unlock(&job->lock);
         if (y >= job->end_row) break;
-        for (uint32_t x = 0; x < job->v->width; ++x) dvr_pixel(job, x, y, st);
+        for (uint32_t x = 0; x < job->v->width; ++x) {
+            if (job->mop) mop_pixel(job, x, y, st);
+            else dvr_pixel(job, x, y, st);
+        }
     }
     pthread_mutex_lock(&job->lock);
     for (int k = 0; k < 4; ++k) job->stats[k] += st[k];
@@ -991,14 +1141,15 @@ void or_render_dvr_rows(or_cache *c, const or_view *v, float *tbuf, double *csta
     or_render_dvr_resume(c, v, tbuf, cstate, NULL, pixels, precision, nthreads, y0, rows, stats);
 }
 
-void or_render_dvr_resume(or_cache *c, const or_view *v, float *tbuf, double *cstate, float *rpos,
-                          uint32_t *pixels, int precision, int nthreads, uint32_t y0, uint32_t rows,
-                          uint64_t stats[4])
+static void render_rows(or_cache *c, const or_view *v, float *tbuf, double *cstate, float *rpos,
+                        uint32_t *pixels, int precision, int mop, int nthreads, uint32_t y0, uint32_t rows,
+                        uint64_t stats[4])
 {
     dvr_job job;
     memset(&job, 0, sizeof(job));
     job.c = c; job.v = v; job.tbuf = tbuf; job.cstate = cstate; job.rpos = rpos; job.pixels = pixels;
     job.precision = precision;
+    job.mop = mop;
     job.next_row = y0;
     job.end_row = y0 + rows > v->height ? v->height : y0 + rows;
     pthread_mutex_init(&job.lock, NULL);
@@ -1010,4 +1161,17 @@ void or_render_dvr_resume(or_cache *c, const or_view *v, float *tbuf, double *cs
     free(th);
     pthread_mutex_destroy(&job.lock);
     if (stats) memcpy(stats, job.stats, sizeof(job.stats));
+}
+
+void or_render_dvr_resume(or_cache *c, const or_view *v, float *tbuf, double *cstate, float *rpos,
+                          uint32_t *pixels, int precision, int nthreads, uint32_t y0, uint32_t rows,
+                          uint64_t stats[4])
+{
+    render_rows(c, v, tbuf, cstate, rpos, pixels, precision, 0, nthreads, y0, rows, stats);
+}
+
+void or_render_mop(or_cache *c, const or_view *v, float *tbuf, double *cstate, float *rpos, uint32_t *pixels,
+                   int precision, int nthreads, uint64_t stats[4])
+{
+    render_rows(c, v, tbuf, cstate, rpos, pixels, precision, 1, nthreads, 0, v->height, stats);
 }

@@ -205,6 +205,10 @@ void or_render_dvr_rows(or_cache *c, const or_view *v, float *tbuf, double *csta
 void or_render_dvr_resume(or_cache *c, const or_view *v, float *tbuf, double *cstate, float *rpos,
                           uint32_t *pixels, int precision, int nthreads, uint32_t y0, uint32_t rows,
                           uint64_t stats[4]);
+/* RayCastMOPImpl (RayCasters.cu:201-349): maximum projection; cstate[4i] is
+ * the normalised maximum (colorStateBuffer[i].x), the pixel TF(maximum). */
+void or_render_mop(or_cache *c, const or_view *v, float *tbuf, double *cstate, float *rpos, uint32_t *pixels,
+                   int precision, int nthreads, uint64_t stats[4]);
 
 #ifdef __cplusplus
 }

@@ -108,6 +108,7 @@ def lib():
         L.or_render_dvr.argtypes = [P, C.POINTER(OrView), P, P, P, C.c_int, C.c_int, P]
         L.or_render_dvr_rows.argtypes = [P, C.POINTER(OrView), P, P, P, C.c_int, C.c_int, u32, u32, P]
         L.or_render_dvr_resume.argtypes = [P, C.POINTER(OrView), P, P, P, P, C.c_int, C.c_int, u32, u32, P]
+        L.or_render_mop.argtypes = [P, C.POINTER(OrView), P, P, P, P, C.c_int, C.c_int, P]
         L.or_sample.restype = C.c_int
         L.or_sample.argtypes = [P, u32, P, C.POINTER(C.c_double), P, P]
         _lib = L
@@ -294,6 +295,16 @@ class Cache:
             assert rpos.dtype == np.float32 and rpos.size == view.width * view.height * 4
             lib().or_render_dvr_resume(self.h, C.byref(view), ptr(tbuf), ptr(cstate), ptr(rpos), ptr(pixels),
                                        precision, nthreads, 0, view.height, ptr(st))
+        return st
+
+    def render_mop(self, view, tbuf, cstate, pixels, precision=64, nthreads=1, rpos=None):
+        """One maximum-projection launch (RayCastMOPImpl); cstate[4i] is the
+        normalised maximum."""
+        st = np.zeros(4, np.uint64)
+        if rpos is not None:
+            assert rpos.dtype == np.float32 and rpos.size == view.width * view.height * 4
+        lib().or_render_mop(self.h, C.byref(view), ptr(tbuf), ptr(cstate), None if rpos is None else ptr(rpos),
+                            ptr(pixels), precision, nthreads, ptr(st))
         return st
 
 

@@ -75,7 +75,7 @@ def test_server_results_alone_and_in_mix(gpu, plain, budget_vols, prec, parity_l
     scenes = [Scene(d, 8, 1, b, max_requests=64) for d, b in VOLS]
     pool_bytes = [2048 * 8 ** 3 * b for _, b in VOLS]
     max_views, slot_pixels = 8, 128 * 96
-    slot_bytes = max_views * slot_pixels * (4 + 4 + (32 if prec == 64 else 16) + 16 + 8 + 8)
+    slot_bytes = max_views * slot_pixels * (4 + 4 + (32 if prec == 64 else 16) + 16 + 8 + 8 + 1)
     budget = sum(sorted(p + slot_bytes for p in pool_bytes)[-budget_vols:]) + 1
     srv = G.Server(budget, max_views=max_views, slot_pixels=slot_pixels, precision=prec, plain=plain)
     srcs = []
@@ -167,7 +167,7 @@ def test_server_evicts_only_idle_volumes_with_empty_queues(gpu):
     scenes = [Scene(d, 8, 1, b, max_requests=64) for d, b in VOLS[:2]]
     pool_bytes = [512 * 8 ** 3 * b for _, b in VOLS[:2]]
     max_views, slot_pixels = 2, 64 * 48
-    slot_bytes = max_views * slot_pixels * (4 + 4 + 16 + 16 + 8 + 8)
+    slot_bytes = max_views * slot_pixels * (4 + 4 + 16 + 16 + 8 + 8 + 1)
     budget = max(p + slot_bytes for p in pool_bytes) + 1
     srv = G.Server(budget, max_views=max_views, slot_pixels=slot_pixels, precision=32)
     srcs = [G.Source.pyramid(sc.vc, sc.levels) for sc in scenes]
@@ -190,3 +190,44 @@ def test_server_evicts_only_idle_volumes_with_empty_queues(gpu):
     assert st.completed == st.submitted == 6 * 7
     assert st.pools_evicted > 0
     srv.close()
+
+
+def test_server_fed_by_view_request_messages(gpu):
+    """Requests folded from the reference's camera / transfer-function /
+    bounding-box messages (libNetwork) render as the same views submitted
+    directly: bit-identical pixels to each request rendered alone."""
+    vc = G.volume_cfg((72, 64, 56), 8, 1, 1)
+    params = G.synth_params(77, 16, sigma=(4.0, 16.0), amp=(60.0, 255.0), noise=8, threads=2)
+    pool_bytes = 2048 * 8 ** 3
+    srv = G.Server(1 << 30, max_views=4, slot_pixels=128 * 96, precision=32)
+    vid = srv.add_volume(vc, G.Source.synthetic(vc, params), pool_bytes, max_requests=64)
+    boxes = [((-1, -1, -1), (1, 1, 1)), ((-0.8, -1, -0.6), (1, 0.7, 1))]
+    reqs = []
+    ids = {}
+    for k, (_, w, h, cam, tseed, _) in enumerate(_requests(6, seed=5)):
+        r = G.Request()
+        for msg in (G.msg_camera(cam), G.msg_transfer_function(tf_table(seed=tseed)),
+                    G.msg_bounding_box(*boxes[k % 2])):
+            r.apply(msg)
+        view = r.view(w, h, precision=32)
+        view.resume = 1
+        reqs.append((view, r.tf()))
+        ids[srv.submit(vid, view, r.tf())] = k
+    srv.drain()
+    res = srv.poll(64)
+    assert len(res) == len(reqs)
+    got = {ids[x.id]: srv.take_pixels(x.id, x.width, x.height) for x in res}
+    srv.close()
+    for k, (view, tf) in enumerate(reqs):
+        cam = np.array(list(view.inv_mv), np.float32).reshape(3, 4)
+        direct = G.make_view(view.width, view.height, cam, alpha_stop=0.95, precision=32, resume=1, bbox=boxes[k % 2])
+        assert bytes(direct) == bytes(view)
+        d_tf = G.DeviceBuffer.from_array(tf)
+        gc = G.Cache(vc, G.cache_cfg(G.pool_shape(pool_bytes, 8 ** 3), 8, max_requests=64), G.Source.synthetic(vc, params))
+        f = G.Frame(view.width, view.height, 32)
+        rep = G.render_frame(gc, direct, d_tf, len(tf), f)
+        assert rep.complete == 1
+        e = _err(got[k], f.pixels().reshape(-1))
+        print("message request", k, (view.width, view.height), "max diff", int(e.max()), flush=True)
+        assert e.max() == 0, k
+        gc.close()
