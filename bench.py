@@ -494,52 +494,76 @@ ONDEMAND_DIMS = (8192, 8192, 4096)
 ONDEMAND_POOL = 128 << 30
 
 
-def ondemand_workload(G, sessions=32, steps=6, width=1280, height=720, half=0.06):
-    """A virtual 8192x8192x4096 u8 volume (256 GiB at level 0, 9 levels, 32^3
-    bricks, covering 1): level-0 bricks synthesised on the host from seeded
-    hash noise + blobs, every coarser brick built on the GPU on miss by 2x2x2
-    reduction of its finer bricks (gc_source_create_gpu_mip, bit-exact to the
-    CPU preprocessor), 128 GiB pool.  `sessions` interactive sessions of
-    1280x720 views (each its own orbit angle and transfer function) zoom in
-    on a region of interest at the volume's centre (a box of +-`half` in
-    normalised coordinates, 2.6 K level-0 bricks), from 1.6 to 0.25 in
-    `steps` steps, so that the views span the coarse to the fine levels of
-    the region.  Each step renders the sessions' views together (one batch);
-    the region bounds the work: mip on miss for the full volume's coarsest
-    levels would reduce all 256 GiB of level 0."""
+def _lod_span(vc, width, dist):
+    """Levels a centred view at `dist` samples (compute_LOD, RayCasters.cu:
+    551-567): the level at the volume's front face and at its back, from the
+    pixel footprint dist_along_ray / width (the view's half-angle is
+    atan(1/2) over width/2 pixels)."""
+    def level_at(t):
+        pd = t / width
+        lev = 0
+        for l in range(1, vc.nb_levels):
+            sp = [2.0 / vc.initial_sizes[l][a] for a in range(3)]
+            if max(sp[0], sp[1]) < pd:
+                lev = l
+            else:
+                break
+        return lev
+    return level_at(max(dist - 1.0, 1e-6)), level_at(dist + 1.0)
+
+
+def ondemand_workload(G, sessions=32, steps=9, width=1280, height=720, far=200.0, near=1.05):
+    """The on-demand configuration: a virtual 8192x8192x4096 u8 volume
+    (256 GiB at level 0, 9 levels, 32^3 bricks, covering 1, 128 GiB pool).
+    Level 0 is synthesised on the GPU from seeded hash noise + blobs when a
+    brick is requested; the coarser levels are its mip pyramid, built once on
+    the GPU (cold build, timed apart: level 0 synthesised and reduced in
+    z-slabs, levels 1-8 kept resident) and cut into bricks on request.
+    `sessions` interactive sessions of 1280x720 views, each its own orbit
+    angle and transfer function, zoom from `far` to `near` (distance to the
+    volume centre) in `steps` geometric steps, so the path spans the coarsest
+    to the finest level; each step renders the sessions' views as one batch."""
     vc = G.volume_cfg(ONDEMAND_DIMS, 32, 1, 1)
     params = G.synth_params(SEED + 5, 48, sigma=(100.0, 900.0), amp=(60.0, 255.0), noise=8, threads=0)
-    base = G.Source.synthetic(vc, params)
-    src = G.Source.gpu_mip(vc, base, memo_bytes=2 << 30)
+    src, build_ms = G.Source.gpu_pyramid(vc, params, slab_planes=64)
+    print("ondemand: cold pyramid build %.0f ms (levels 1-%d resident)" % (build_ms, vc.nb_levels - 1),
+          file=sys.stderr, flush=True)
     cs = G.pool_shape(ONDEMAND_POOL, 32 ** 3)
     cache = G.Cache(vc, G.cache_cfg(cs, 32, max_requests=16384, sync=False), src)
     tables = [G.tf_samples(G.tf_seeded(SEED + 400 + i, 6), 256) for i in range(sessions)]
     d_tfs = [G.DeviceBuffer.from_array(t) for t in tables]
-    box = ((-half, -half, -half), (half, half, half))
     b = G.Batch(cache, sessions, sessions * width * height, 32)
     out = []
     for k in range(steps):
-        dist = 1.6 * (0.25 / 1.6) ** (k / max(1, steps - 1))
+        dist = far * (near / far) ** (k / max(1, steps - 1))
         views = [G.make_view(width, height, G.camera(15.0 + 10.0 * (i % 3), 45.0 * i, dist), alpha_stop=ALPHA_STOP,
-                             precision=32, bbox=box) for i in range(sessions)]
+                             precision=32) for i in range(sessions)]
         c0 = cache.counters()
         t0 = time.perf_counter()
         r = b.render(views, d_tfs, [256] * sessions)
         dt = time.perf_counter() - t0
         c1 = cache.counters()
-        step = {"distance": dist, "ms": dt * 1e3, "frames_per_s": sessions / dt, "launches": r.launches,
-                "bricks_loaded": c1["bricks_loaded"] - c0["bricks_loaded"],
-                "fetch_ms": c1["fetch_ms"] - c0["fetch_ms"], "samples": r.stats.mapped_samples}
-        print("ondemand step %d distance %.2f: %.0f ms, %d launches, %d bricks, fetch %.0f ms, %.3g samples"
-              % (k, dist, dt * 1e3, r.launches, step["bricks_loaded"], step["fetch_ms"], step["samples"]),
-              file=sys.stderr, flush=True)
+        lru = cache.lru()   # (slot origin xyz, level, volume position xyz as float bits; -1: free)
+        levels = lru[:, 3][lru[:, 4] != np.float32(-1.0).view(np.uint32)]
+        span = _lod_span(vc, width, dist)
+        step = {"distance": dist, "lod_front_back": list(span), "ms": dt * 1e3, "frames_per_s": sessions / dt,
+                "launches": r.launches, "bricks_loaded": c1["bricks_loaded"] - c0["bricks_loaded"],
+                "fetch_ms": c1["fetch_ms"] - c0["fetch_ms"], "samples": r.stats.mapped_samples,
+                "complete": bool(r.complete)}
+        step["resident_bricks_per_level"] = np.bincount(levels, minlength=vc.nb_levels).tolist()
+        print("ondemand step %d distance %.2f (levels %d..%d): %.0f ms, %d launches, %d bricks, fetch %.0f ms, "
+              "%.3g samples" % (k, dist, span[0], span[1], dt * 1e3, r.launches, step["bricks_loaded"],
+                                step["fetch_ms"], step["samples"]), file=sys.stderr, flush=True)
         out.append(step)
     b.close()
     cache.close()
+    src.close()
     lat = [s["ms"] for s in out]
-    return {"workload": "virtual 8192x8192x4096 u8 (9 levels, 32^3 bricks, 128 GiB pool), level 0 synthesised on the "
-                        "host, coarser levels built on the GPU on miss; %d sessions of %dx%d zooming 1.6 -> 0.25 on a "
-                        "+-%.2f region in %d steps, one batch per step" % (sessions, width, height, half, steps),
+    return {"workload": "virtual 8192x8192x4096 u8 (9 levels, 32^3 bricks, covering 1, 128 GiB pool); level 0 "
+                        "synthesised on the GPU on request, levels 1-8 its mip pyramid built cold on the GPU; %d "
+                        "sessions of %dx%d zooming %.0f -> %.2f in %d steps (levels 8 -> 0), one batch per step"
+                        % (sessions, width, height, far, near, steps),
+            "cold_pyramid_build_ms": build_ms,
             "frames_per_s": sessions * len(out) / (sum(lat) / 1e3), "step_p50_ms": percentile(lat, 50),
             "step_p99_ms": percentile(lat, 99), "steps": out}
 
@@ -729,7 +753,7 @@ def main():
                     help="also run the long serving workload (host-synthesised bricks, several minutes)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--ondemand-only", action="store_true",
-                    help="run only the on-demand workload (virtual 8192x8192x4096 u8, mip on miss)")
+                    help="run only the on-demand workload (virtual 8192x8192x4096 u8, GPU pyramid)")
     ap.add_argument("--serving-rates", default="4,8,16,32,64", help="request rates (req/s) of the serving workload")
     ap.add_argument("--serving-volumes", default="", help="serve only these volumes (indices into the six)")
     ap.add_argument("--serving-max-views", type=int, default=4, help="views per volume per launch when batched")

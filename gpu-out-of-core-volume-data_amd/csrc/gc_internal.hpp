@@ -162,11 +162,40 @@ struct SynthLaunch {
     uint8_t *dst;              // n bricks of bs^3 voxels
     uint32_t *below;           // n: 1 when every voxel is below thr
     uint32_t n, nb, bs, cov, bytes, seed, noise;
+    uint32_t max_level;        // keys of coarser levels are skipped (left to another writer)
     float fmax, thr;
     SynthAxis axis[GC_MAX_LEVELS][3];
     uint32_t ext[GC_MAX_LEVELS][3];   // level extents (voxels outside are 0)
 };
 hipError_t launch_synth_bricks(const SynthLaunch &p, hipStream_t s);
+// A box of voxels (x fastest) of one level of the synthetic volume, the
+// values the bricks of k_synth_bricks hold at the same positions.
+struct SynthBox {
+    const float *tab, *amp;
+    uint8_t *dst;
+    int64_t o[3];              // first voxel (level coordinates)
+    uint32_t n[3];             // extent
+    uint32_t level, nb, bytes, seed, noise;
+    float fmax;
+    SynthAxis axis[3];
+    uint32_t ext[3];
+};
+hipError_t launch_synth_box(const SynthBox &b, hipStream_t s);
+
+// Bricks of keys (bx, by, bz, level) cut with the covering rule from
+// resident device levels (levels[l], dims[l]; keys of levels below
+// first_level skipped), written to out + u * bs^3 voxels, with below[u] =
+// every voxel below thr.
+struct CutKeys {
+    const uint8_t *levels[GC_MAX_LEVELS];
+    uint32_t dims[GC_MAX_LEVELS][3];
+    const uint32_t *keys;
+    uint8_t *out;
+    uint32_t *below;
+    uint32_t n, bs, cov, bytes, first_level;
+    float thr;
+};
+hipError_t launch_cut_keys(const CutKeys &a, hipStream_t s);
 
 hipError_t launch_brick_cut(const void *level, const uint32_t dims[3], uint32_t bytes,
                             uint32_t bs, uint32_t cov, const uint32_t *bricks_xyz, uint32_t n,

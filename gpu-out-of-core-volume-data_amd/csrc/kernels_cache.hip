@@ -428,6 +428,48 @@ __global__ __launch_bounds__(BLK) void k_brick_cut(const uint8_t *__restrict__ l
     }
 }
 
+// One workgroup per key: the brick of a resident level with the covering
+// rule (k_brick_cut's), and the EmptyThreshold flag.
+template <int BYTES>
+__global__ __launch_bounds__(BLK) void k_cut_keys(const CutKeys a)
+{
+    const uint32_t u = blockIdx.x;
+    const uint32_t *key = a.keys + 4ull * u;
+    const uint32_t level = key[3];
+    if (level < a.first_level) return;   // the whole workgroup: another writer's brick
+    const uint32_t bs = a.bs, inner = bs - 2 * a.cov;
+    const uint32_t X = a.dims[level][0], Y = a.dims[level][1], Z = a.dims[level][2];
+    const uint8_t *src = a.levels[level];
+    const uint64_t bvox = (uint64_t)bs * bs * bs;
+    const int64_t ox = (int64_t)key[0] * inner - a.cov, oy = (int64_t)key[1] * inner - a.cov,
+                  oz = (int64_t)key[2] * inner - a.cov;
+    uint8_t *dst = a.out + (uint64_t)u * bvox * BYTES;
+    int above = 0;
+    for (uint64_t v = threadIdx.x; v < bvox; v += BLK) {
+        const int64_t gx = ox + (int64_t)(v % bs), gy = oy + (int64_t)((v / bs) % bs),
+                      gz = oz + (int64_t)(v / ((uint64_t)bs * bs));
+        const bool in = gx >= 0 && gy >= 0 && gz >= 0 && gx < X && gy < Y && gz < Z;
+        const uint64_t si = in ? (uint64_t)gx + (uint64_t)X * ((uint64_t)gy + (uint64_t)Y * gz) : 0;
+        float val;
+        if constexpr (BYTES == 1) {
+            const uint8_t t = in ? src[si] : 0;
+            dst[v] = t;
+            val = (float)t;
+        } else if constexpr (BYTES == 2) {
+            const uint16_t t = in ? reinterpret_cast<const uint16_t *>(src)[si] : 0;
+            reinterpret_cast<uint16_t *>(dst)[v] = t;
+            val = (float)t;
+        } else {
+            const float t = in ? reinterpret_cast<const float *>(src)[si] : 0.f;
+            reinterpret_cast<float *>(dst)[v] = t;
+            val = t;
+        }
+        above |= !(val < a.thr);
+    }
+    above = __syncthreads_or(above);
+    if (threadIdx.x == 0 && a.below) a.below[u] = above ? 0u : 1u;
+}
+
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 } // namespace
@@ -546,6 +588,18 @@ hipError_t launch_mip_brick(const MipBrickArgs &a, uint32_t bytes, const void *c
     case 1: k_mip_brick<uint8_t><<<nb, BLK, 0, stream>>>(a, (const uint8_t *)children, (uint8_t *)out); break;
     case 2: k_mip_brick<uint16_t><<<nb, BLK, 0, stream>>>(a, (const uint16_t *)children, (uint16_t *)out); break;
     case 4: k_mip_brick<float><<<nb, BLK, 0, stream>>>(a, (const float *)children, (float *)out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cut_keys(const CutKeys &a, hipStream_t stream)
+{
+    if (a.n == 0) return hipSuccess;
+    switch (a.bytes) {
+    case 1: k_cut_keys<1><<<a.n, BLK, 0, stream>>>(a); break;
+    case 2: k_cut_keys<2><<<a.n, BLK, 0, stream>>>(a); break;
+    case 4: k_cut_keys<4><<<a.n, BLK, 0, stream>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

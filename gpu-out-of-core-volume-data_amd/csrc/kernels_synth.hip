@@ -49,6 +49,7 @@ __global__ __launch_bounds__(256) void k_synth_bricks(SynthLaunch p)
     const uint32_t u = blockIdx.x;
     const uint32_t *key = p.keys + 4ull * u;
     const uint32_t level = key[3], bs = p.bs, nb = p.nb, inner = bs - 2 * p.cov;
+    if (level > p.max_level) return;   // the whole workgroup: another writer's brick
     const int64_t o0 = (int64_t)key[0] * inner - p.cov, o1 = (int64_t)key[1] * inner - p.cov,
                   o2 = (int64_t)key[2] * inner - p.cov;
     const uint32_t nf = nb * bs;
@@ -97,6 +98,66 @@ __global__ __launch_bounds__(256) void k_synth_bricks(SynthLaunch p)
     // (BrickProcessorPredicate.hpp:11-45)
     above = __syncthreads_or(above);
     if (threadIdx.x == 0) p.below[u] = above ? 0u : 1u;
+}
+
+// Grid: (ceil(n[0] / 256), n[1], n[2]) workgroups of 256 lanes, one row
+// segment each.  Dynamic LDS: nb x 256 x factors of the segment.
+template <typename T>
+__global__ __launch_bounds__(256) void k_synth_box(SynthBox b)
+{
+    extern __shared__ float fxs[];
+    const uint32_t nb = b.nb;
+    const uint32_t x0 = blockIdx.x * 256u, y = blockIdx.y, z = blockIdx.z;
+    const int64_t gy = b.o[1] + y, gz = b.o[2] + z;
+    for (uint32_t i = threadIdx.x; i < nb * 256u; i += blockDim.x) {
+        const uint32_t k = i / 256u, j = i % 256u;
+        const int64_t c = b.o[0] + x0 + j;
+        const SynthAxis &ax = b.axis[0];
+        float f = 0.f;
+        if (c >= ax.lo && c < ax.lo + (int64_t)ax.len) f = b.tab[ax.off + (uint64_t)k * ax.len + (uint64_t)(c - ax.lo)];
+        fxs[i] = f;
+    }
+    __syncthreads();
+    const uint32_t x = x0 + threadIdx.x;
+    if (x >= b.n[0]) return;
+    const int64_t gx = b.o[0] + x;
+    float val = 0.f;
+    if (gx >= 0 && gx < (int64_t)b.ext[0] && gy >= 0 && gy < (int64_t)b.ext[1] && gz >= 0 && gz < (int64_t)b.ext[2]) {
+        float c = 0.f;
+        for (uint32_t k = 0; k < nb; ++k) {
+            // the brick kernel's factors: fy, and fz scaled by the amplitude
+            const SynthAxis &ay = b.axis[1], &az = b.axis[2];
+            const float fy = gy >= ay.lo && gy < ay.lo + (int64_t)ay.len ? b.tab[ay.off + (uint64_t)k * ay.len + (uint64_t)(gy - ay.lo)] : 0.f;
+            const float fz0 = gz >= az.lo && gz < az.lo + (int64_t)az.len ? b.tab[az.off + (uint64_t)k * az.len + (uint64_t)(gz - az.lo)] : 0.f;
+            const float fz = b.amp[k] * fz0;
+            const float w = fz * fy;
+            c = __builtin_fmaf(w, fxs[k * 256u + threadIdx.x], c);
+        }
+        int32_t nz = 0;
+        if (b.noise) {
+            const uint32_t span = 2u * b.noise + 1u;
+            const uint32_t rk = b.seed ^ (b.level * 0x27d4eb2fu) ^ ((uint32_t)gz * 0x165667b1u) ^ ((uint32_t)gy * 0xc2b2ae3du);
+            const uint32_t h = synth_hash32(rk ^ ((uint32_t)gx * 0x9e3779b1u));
+            nz = (int32_t)(uint32_t)(((uint64_t)h * span) >> 32) - (int32_t)b.noise;
+        }
+        float r = (float)(int32_t)(c + 0.5f) + (float)nz;
+        r = r < 0.f ? 0.f : r;
+        r = r > b.fmax ? b.fmax : r;
+        val = r;
+    }
+    synth_store<T>(b.dst, (uint64_t)x + (uint64_t)b.n[0] * ((uint64_t)y + (uint64_t)b.n[1] * z), val);
+}
+
+hipError_t launch_synth_box(const SynthBox &b, hipStream_t s)
+{
+    if (b.n[0] == 0 || b.n[1] == 0 || b.n[2] == 0) return hipSuccess;
+    if (b.n[1] > 65535u || b.n[2] > 65535u) return hipErrorInvalidValue;
+    const dim3 grid((b.n[0] + 255u) / 256u, b.n[1], b.n[2]);
+    const size_t lds = (size_t)b.nb * 256u * sizeof(float);
+    if (b.bytes == 1) k_synth_box<uint8_t><<<grid, 256, lds, s>>>(b);
+    else if (b.bytes == 2) k_synth_box<uint16_t><<<grid, 256, lds, s>>>(b);
+    else k_synth_box<float><<<grid, 256, lds, s>>>(b);
+    return hipGetLastError();
 }
 
 hipError_t launch_synth_bricks(const SynthLaunch &p, hipStream_t s)

@@ -315,6 +315,14 @@ struct DeviceSynthSource : SynthSource {
         base.seed = (uint32_t)S.prm.seed;
         base.noise = S.prm.noise;
         base.fmax = (float)S.maxval;
+        base.max_level = ~0u;
+        return GC_OK;
+    }
+    // Bricks of the keys that the synthesis kernel skipped (coarser levels
+    // of a source that builds them otherwise), on stream s.
+    virtual int write_other(uint32_t n, uint8_t *d_dst, float thr, hipStream_t s)
+    {
+        (void)n; (void)d_dst; (void)thr; (void)s;
         return GC_OK;
     }
     int get_device(const uint32_t *keys, uint32_t n, uint8_t *d_dst, float thr, uint8_t *below,
@@ -345,10 +353,127 @@ struct DeviceSynthSource : SynthSource {
         p.thr = thr;
         GC_HIP(hipMemcpyAsync(d_keys, keys, (size_t)n * 16, hipMemcpyHostToDevice, s));
         GC_HIP(launch_synth_bricks(p, s));
+        GC_TRY(write_other(n, d_dst, thr, s));
         GC_HIP(hipMemcpyAsync(flags.data(), d_below, (size_t)n * 4, hipMemcpyDeviceToHost, s));
         GC_HIP(hipStreamSynchronize(s));
         for (uint32_t i = 0; i < n; ++i) below[i] = (uint8_t)flags[i];
         return GC_OK;
+    }
+};
+
+// The synthetic volume's level 0 with a resident mip pyramid on the device
+// (the preprocessing of Mipmapper.cu:401-597 for a volume that does not fit
+// anywhere at level 0): a cold build synthesises level 0 in z-slabs of
+// `slab` planes straight into device memory (k_synth_box, the voxels of the
+// brick synthesiser) and reduces each slab into level 1 (k_mip2, the
+// Mipmapper rule; slabs start on even planes, so each 2x2x2 group lies in one
+// slab); levels 2.. are reduced from the resident level 1.  Requests then
+// synthesise level-0 bricks and cut coarser bricks from the resident levels
+// with the covering rule, both on the device.  The coarse levels are the
+// CPU preprocessor's bit for bit; the level-0 volume is never stored.
+struct DevicePyramidSource : DeviceSynthSource {
+    std::vector<uint8_t *> d_levels;   // [l] for l >= 1
+    CutKeys cut{};
+    double build_ms = 0.0;
+    ~DevicePyramidSource() override
+    {
+        for (uint8_t *p : d_levels) hipFree(p);
+    }
+    int build(uint32_t slab)
+    {
+        const uint32_t nl = vc.nb_levels;
+        if (slab == 0 || slab % 2) return fail(GC_ERR_ARG, "slab planes must be even and positive");
+        for (uint32_t l = 1; l < nl; ++l)
+            for (int a = 0; a < 3; ++a)
+                if (vc.initial_sizes[l][a] != vc.initial_sizes[l - 1][a] / 2)
+                    return fail(GC_ERR_CONFIG, "level sizes do not halve (floor) from level to level");
+        const uint32_t *d0 = vc.initial_sizes[0];
+        if (d0[1] > 65535u) return fail(GC_ERR_CONFIG, "level 0 taller than 65535 voxels");
+        GC_HIP(hipSetDevice(device));
+        const auto t0 = std::chrono::steady_clock::now();
+        d_levels.assign(nl, nullptr);
+        for (uint32_t l = 1; l < nl; ++l) {
+            const uint64_t n = (uint64_t)vc.initial_sizes[l][0] * vc.initial_sizes[l][1] * vc.initial_sizes[l][2] * bytes;
+            GC_HIP(hipMalloc((void **)&d_levels[l], std::max<uint64_t>(n, 1)));
+        }
+        const uint64_t plane = (uint64_t)d0[0] * d0[1] * bytes;
+        uint8_t *d_slab = nullptr;
+        GC_HIP(hipMalloc((void **)&d_slab, plane * std::min<uint32_t>(slab, d0[2])));
+        hipStream_t s = nullptr;
+        hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        const uint64_t plane1 = nl > 1 ? (uint64_t)vc.initial_sizes[1][0] * vc.initial_sizes[1][1] * bytes : 0;
+        for (uint32_t z0 = 0; e == hipSuccess && z0 < d0[2]; z0 += slab) {
+            const uint32_t T = std::min<uint32_t>(slab, d0[2] - z0);
+            SynthBox b{};
+            b.tab = base.tab;
+            b.amp = base.amp;
+            b.dst = d_slab;
+            b.o[0] = 0; b.o[1] = 0; b.o[2] = z0;
+            b.n[0] = d0[0]; b.n[1] = d0[1]; b.n[2] = T;
+            b.level = 0;
+            b.nb = base.nb;
+            b.bytes = bytes;
+            b.seed = base.seed;
+            b.noise = base.noise;
+            b.fmax = base.fmax;
+            for (int a = 0; a < 3; ++a) {
+                b.axis[a] = base.axis[0][a];
+                b.ext[a] = base.ext[0][a];
+            }
+            e = launch_synth_box(b, s);
+            if (e == hipSuccess && nl > 1) {
+                const uint32_t dims[3] = {d0[0], d0[1], T};
+                e = launch_mip(d_slab, dims, bytes, d_levels[1] + (uint64_t)(z0 / 2) * plane1, s);
+            }
+        }
+        for (uint32_t l = 2; e == hipSuccess && l < nl; ++l)
+            e = launch_mip(d_levels[l - 1], vc.initial_sizes[l - 1], bytes, d_levels[l], s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (s) hipStreamDestroy(s);
+        hipFree(d_slab);
+        GC_HIP(e);
+        build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (uint32_t l = 1; l < nl; ++l) {
+            cut.levels[l] = d_levels[l];
+            for (int a = 0; a < 3; ++a) cut.dims[l][a] = vc.initial_sizes[l][a];
+        }
+        cut.bs = bs;
+        cut.cov = cov;
+        cut.bytes = bytes;
+        cut.first_level = 1;
+        base.max_level = 0;
+        return GC_OK;
+    }
+    int write_other(uint32_t n, uint8_t *d_dst, float thr, hipStream_t s) override
+    {
+        CutKeys c = cut;
+        c.keys = d_keys;
+        c.out = d_dst;
+        c.below = d_below;
+        c.n = n;
+        c.thr = thr;
+        GC_HIP(launch_cut_keys(c, s));
+        return GC_OK;
+    }
+    // host bricks (tests / tools): level 0 from the host synthesiser (the
+    // same voxels), coarser levels cut on the device and copied back
+    int get(uint32_t level, const uint32_t bbc[3], void *dst) override
+    {
+        if (level >= vc.nb_levels) return GC_BRICK_ERROR;
+        if (level == 0) return SynthSource::get(level, bbc, dst);
+        if (hipSetDevice(device) != hipSuccess) return GC_BRICK_ERROR;
+        const uint32_t key[4] = {bbc[0], bbc[1], bbc[2], level};
+        uint8_t *d = nullptr;
+        if (hipMalloc((void **)&d, bb_bytes) != hipSuccess) return GC_BRICK_ERROR;
+        hipStream_t s = nullptr;
+        uint8_t below = 0;
+        int rc = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess ? get_device(key, 1, d, 0.f, &below, s)
+                                                                                 : GC_ERR_HIP;
+        if (rc == GC_OK && hipMemcpyAsync(dst, d, bb_bytes, hipMemcpyDeviceToHost, s) != hipSuccess) rc = GC_ERR_HIP;
+        if (rc == GC_OK && hipStreamSynchronize(s) != hipSuccess) rc = GC_ERR_HIP;
+        if (s) hipStreamDestroy(s);
+        hipFree(d);
+        return rc == GC_OK ? GC_BRICK_SUCCESS : GC_BRICK_ERROR;
     }
 };
 
@@ -1606,6 +1731,22 @@ int gc_source_create_synthetic_gpu(const gc_volume_cfg *vc, const gc_synth_param
     s->nthreads = p->threads ? p->threads : hw_threads();
     s->device = device;
     GC_TRY(s->setup());
+    *out = new gc_source{std::move(s)};
+    return GC_OK;
+}
+
+int gc_source_create_gpu_pyramid(const gc_volume_cfg *vc, const gc_synth_params *p, int device, uint32_t slab_planes,
+                                 gc_source **out, double *build_ms)
+{
+    if (!vc || !p || !out) return fail(GC_ERR_ARG, "null argument");
+    auto s = std::make_unique<DevicePyramidSource>();
+    s->init(*vc);
+    s->synth = std::make_unique<Synth>(*p, vc->initial_sizes[0], vc->encoded_bytes);
+    s->nthreads = p->threads ? p->threads : hw_threads();
+    s->device = device;
+    GC_TRY(s->setup());
+    GC_TRY(s->build(slab_planes ? slab_planes : 64u));
+    if (build_ms) *build_ms = s->build_ms;
     *out = new gc_source{std::move(s)};
     return GC_OK;
 }

@@ -155,7 +155,7 @@ EXPORTS = [
     "gc_brick_key", "gc_brick_key_unpack", "gc_store_brick_path", "gc_lz4_bound", "gc_lz4_compress",
     "gc_lz4_decompress", "gc_store_write", "gc_source_create_store", "gc_store_read_meta", "gc_store_stats",
     "gc_msg_encode_camera", "gc_msg_encode_transfer_function", "gc_msg_encode_bounding_box", "gc_msg_id",
-    "gc_view_request_init", "gc_view_request_apply", "gc_view_request_view",
+    "gc_view_request_init", "gc_view_request_apply", "gc_view_request_view", "gc_source_create_gpu_pyramid",
 ]
 
 _lib = None
@@ -247,6 +247,8 @@ def lib():
             "gc_msg_id": [P, SZ, C.POINTER(u32)],
             "gc_view_request_init": [C.POINTER(ViewRequest)], "gc_view_request_apply": [C.POINTER(ViewRequest), P, SZ],
             "gc_view_request_view": [C.POINTER(ViewRequest), u32, u32, u32, C.POINTER(View)],
+            "gc_source_create_gpu_pyramid": [C.POINTER(VolumeCfg), C.POINTER(SynthParams), C.c_int, u32,
+                                             C.POINTER(vp), C.POINTER(C.c_double)],
         }
         missing = [n for n in EXPORTS if n not in sig]
         if missing:
@@ -423,6 +425,16 @@ class Source:
         out = d.to_array(DT[vc.encoded_bytes], (n, bs, bs, bs))
         d.free()
         return out, empty[:n]
+
+    @classmethod
+    def gpu_pyramid(cls, vc, params, device=0, slab_planes=0):
+        """Synthetic level 0 with its mip pyramid built on the GPU (cold
+        build); returns (source, build milliseconds)."""
+        h = vp()
+        ms = C.c_double(0)
+        check(lib().gc_source_create_gpu_pyramid(C.byref(vc), C.byref(params), int(device), slab_planes,
+                                                 C.byref(h), C.byref(ms)))
+        return cls(h), ms.value
 
     @classmethod
     def gpu_mip(cls, vc, level0, device=0, memo_bytes=256 << 20):

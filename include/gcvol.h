@@ -178,6 +178,18 @@ int gc_source_create_callback(const gc_volume_cfg *vc, gc_brick_fn fn, void *use
  * No reference counterpart (the reference preprocesses every level offline). */
 int gc_source_create_gpu_mip(const gc_volume_cfg *vc, gc_source *level0, int device,
                              uint64_t memo_bytes, gc_source **out);
+/* The seeded synthetic volume's level 0 (synthesised on `device` when
+ * requested, as gc_source_create_synthetic_gpu) with every coarser level the
+ * mip pyramid of it (Mipmapper.cu:30-105,401-597), built once, cold, on the
+ * device: level 0 is synthesised in z-slabs of slab_planes planes (even; 0:
+ * 64) and reduced slab by slab, the coarser levels stay resident (about 1/7
+ * of level 0's size), and coarse bricks are cut from them on request with the
+ * covering rule (Bricker_v2.cpp:138-222).  Bit-identical to the CPU
+ * preprocessor over the host-synthesised level 0.  *build_ms: the build's
+ * wall time.  BigBrickSize 1; level sizes must halve (floor).  No reference
+ * counterpart (the reference preprocesses every level offline). */
+int gc_source_create_gpu_pyramid(const gc_volume_cfg *vc, const gc_synth_params *p, int device,
+                                 uint32_t slab_planes, gc_source **out, double *build_ms);
 int gc_source_destroy(gc_source *src);
 /* Fetch one big brick from a source (for tests / tools). */
 int gc_source_get(gc_source *src, uint32_t level, const uint32_t big_brick[3], void *dst,
