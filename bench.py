@@ -752,6 +752,7 @@ def main():
     ap.add_argument("--serving", action="store_true",
                     help="also run the long serving workload (host-synthesised bricks, several minutes)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving workload")
+    ap.add_argument("--no-ondemand", action="store_true", help="skip the on-demand 8192x8192x4096 workload")
     ap.add_argument("--ondemand-only", action="store_true",
                     help="run only the on-demand workload (virtual 8192x8192x4096 u8, GPU pyramid)")
     ap.add_argument("--serving-rates", default="4,8,16,32,64", help="request rates (req/s) of the serving workload")
@@ -938,6 +939,8 @@ def main():
         if not args.no_serving:
             out["serving"] = serving_workload(G, rates=(50, 200, 800), count=48, max_views=8, warm_requests=8,
                                               device_source=True, tune_upload=True)
+        if not args.no_ondemand:
+            out["ondemand"] = ondemand_workload(G)
     if args.serving:
         out["serving_long"] = serving_workload(G)
     print(json.dumps(out))

@@ -1943,55 +1943,95 @@ hipError_t launch_fill32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t st
     return hipGetLastError();
 }
 
-// One wave per 8x8 tile of the views' stop flags (four tiles per block, in
-// tile order, so consecutive blocks append neighbouring tiles): the wave
-// reads and clears its tile's flags and appends the flagged rays with one
-// atomic, the tile's rays contiguous in the list.
+// Each workgroup gathers 16 consecutive 8x8 tiles of the views' stop flags
+// (four per wave, in tile order, so consecutive workgroups append
+// neighbouring tiles): a wave reads and clears its tiles' flags, the
+// workgroup reserves room for all its rays with one atomic, and each tile's
+// rays are written contiguously.  One atomic per workgroup rather than per
+// tile keeps the list counter off the critical path.
+constexpr uint32_t COLLECT_TILES_PER_WAVE = 4, COLLECT_TILES = 4 * COLLECT_TILES_PER_WAVE;
+
+__device__ inline uint32_t collect_view(const CollectArgs &a, uint32_t tile)
+{
+    // the last view whose range starts at or before the tile (views without
+    // tiles share the next view's start)
+    if (!a.views) return 0;
+    uint32_t lo = 0, hi = a.n_views;   // stop_tile_off[lo] <= tile < stop_tile_off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.views[mid].stop_tile_off <= tile) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(256) void k_collect_stopped(const CollectArgs a)
 {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (tile >= a.n_tiles) return;   // whole waves
-    // the view holding the tile: the last one whose range starts at or
-    // before it (views without tiles share the next view's start)
-    uint32_t v = 0;
-    if (a.views) {
-        uint32_t lo = 0, hi = a.n_views;   // stop_tile_off[lo] <= tile < stop_tile_off[hi]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (a.views[mid].stop_tile_off <= tile) lo = mid;
-            else hi = mid;
+    __shared__ uint32_t cnt[COLLECT_TILES];
+    __shared__ uint32_t vof[COLLECT_TILES];
+    __shared__ uint32_t base_s;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint64_t masks[COLLECT_TILES_PER_WAVE];
+    uint32_t pixs[COLLECT_TILES_PER_WAVE], views[COLLECT_TILES_PER_WAVE];
+    for (uint32_t k = 0; k < COLLECT_TILES_PER_WAVE; ++k) {
+        const uint32_t j = wave * COLLECT_TILES_PER_WAVE + k;
+        const uint32_t tile = blockIdx.x * COLLECT_TILES + j;
+        bool s = false;
+        uint32_t pix = 0, v = 0;
+        if (tile < a.n_tiles) {
+            v = collect_view(a, tile);
+            const DvrView &V = a.views ? a.views[v] : a.one;
+            uint8_t *const sf = V.stop;
+            const uint32_t t = tile - V.stop_tile_off, tx = (V.width + 7u) / 8u;
+            const uint32_t x = (t % tx) * 8u + (lane & 7u), y = (t / tx) * 8u + (lane >> 3);
+            if (sf && x < V.width && y < V.height) {
+                pix = y * V.width + x;
+                s = sf[pix] != 0;
+                if (s) sf[pix] = 0;
+            }
         }
-        v = lo;
+        masks[k] = __ballot(s);
+        pixs[k] = pix;
+        views[k] = v;
+        if (lane == 0) {
+            cnt[j] = (uint32_t)__popcll(masks[k]);
+            vof[j] = v;
+        }
     }
-    const DvrView &V = a.views ? a.views[v] : a.one;
-    uint8_t *const sf = V.stop;
-    if (!sf) return;
-    const uint32_t t = tile - V.stop_tile_off, tx = (V.width + 7u) / 8u;
-    const uint32_t x = (t % tx) * 8u + (lane & 7u), y = (t / tx) * 8u + (lane >> 3);
-    bool s = false;
-    uint32_t pix = 0;
-    if (x < V.width && y < V.height) {
-        pix = y * V.width + x;
-        s = sf[pix] != 0;
-        if (s) sf[pix] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t j = 0; j < COLLECT_TILES; ++j) total += cnt[j];
+        base_s = total ? atomicAdd(a.out_count, total) : 0u;
+        if (a.view_stopped && total) {
+            // one atomic per run of tiles of the same view
+            uint32_t run = 0, rv = vof[0];
+            for (uint32_t j = 0; j < COLLECT_TILES; ++j) {
+                if (vof[j] != rv) {
+                    if (run) atomicAdd(&a.view_stopped[rv], run);
+                    run = 0;
+                    rv = vof[j];
+                }
+                run += cnt[j];
+            }
+            if (run) atomicAdd(&a.view_stopped[rv], run);
+        }
     }
-    const uint64_t m = __ballot(s);
-    if (!m) return;
-    const uint32_t n = (uint32_t)__popcll(m);
-    uint32_t base = 0;
-    if (lane == 0) {
-        base = atomicAdd(a.out_count, n);
-        if (a.view_stopped) atomicAdd(&a.view_stopped[v], n);
+    __syncthreads();
+    uint32_t off = base_s;
+    for (uint32_t j = 0; j < wave * COLLECT_TILES_PER_WAVE; ++j) off += cnt[j];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t k = 0; k < COLLECT_TILES_PER_WAVE; ++k) {
+        if ((masks[k] >> lane) & 1ull)
+            a.out_list[off + (uint32_t)__popcll(masks[k] & lt)] = (views[k] << DVR_PIXEL_BITS) | pixs[k];
+        off += (uint32_t)__popcll(masks[k]);
     }
-    base = (uint32_t)__shfl((int)base, 0, 64);
-    if (s) a.out_list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (v << DVR_PIXEL_BITS) | pix;
 }
 
 hipError_t launch_collect_stopped(const CollectArgs &a, hipStream_t stream)
 {
     if (a.n_tiles == 0) return hipSuccess;
-    k_collect_stopped<<<(a.n_tiles + 3u) / 4u, 256, 0, stream>>>(a);
+    k_collect_stopped<<<(a.n_tiles + COLLECT_TILES - 1u) / COLLECT_TILES, 256, 0, stream>>>(a);
     return hipGetLastError();
 }
 

@@ -101,14 +101,24 @@ __global__ __launch_bounds__(256) void k_synth_bricks(SynthLaunch p)
 }
 
 // Grid: (ceil(n[0] / 256), n[1], n[2]) workgroups of 256 lanes, one row
-// segment each.  Dynamic LDS: nb x 256 x factors of the segment.
+// segment each.  Dynamic LDS: nb x 256 x factors of the segment, then the nb
+// row weights w_k = (amp_k fz_k(z)) * fy_k(y), the products the brick kernel
+// forms per voxel.
 template <typename T>
 __global__ __launch_bounds__(256) void k_synth_box(SynthBox b)
 {
     extern __shared__ float fxs[];
     const uint32_t nb = b.nb;
+    float *ws = fxs + nb * 256u;
     const uint32_t x0 = blockIdx.x * 256u, y = blockIdx.y, z = blockIdx.z;
     const int64_t gy = b.o[1] + y, gz = b.o[2] + z;
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+        const SynthAxis &ay = b.axis[1], &az = b.axis[2];
+        const float fy = gy >= ay.lo && gy < ay.lo + (int64_t)ay.len ? b.tab[ay.off + (uint64_t)k * ay.len + (uint64_t)(gy - ay.lo)] : 0.f;
+        const float fz0 = gz >= az.lo && gz < az.lo + (int64_t)az.len ? b.tab[az.off + (uint64_t)k * az.len + (uint64_t)(gz - az.lo)] : 0.f;
+        const float fz = b.amp[k] * fz0;
+        ws[k] = fz * fy;
+    }
     for (uint32_t i = threadIdx.x; i < nb * 256u; i += blockDim.x) {
         const uint32_t k = i / 256u, j = i % 256u;
         const int64_t c = b.o[0] + x0 + j;
@@ -124,15 +134,7 @@ __global__ __launch_bounds__(256) void k_synth_box(SynthBox b)
     float val = 0.f;
     if (gx >= 0 && gx < (int64_t)b.ext[0] && gy >= 0 && gy < (int64_t)b.ext[1] && gz >= 0 && gz < (int64_t)b.ext[2]) {
         float c = 0.f;
-        for (uint32_t k = 0; k < nb; ++k) {
-            // the brick kernel's factors: fy, and fz scaled by the amplitude
-            const SynthAxis &ay = b.axis[1], &az = b.axis[2];
-            const float fy = gy >= ay.lo && gy < ay.lo + (int64_t)ay.len ? b.tab[ay.off + (uint64_t)k * ay.len + (uint64_t)(gy - ay.lo)] : 0.f;
-            const float fz0 = gz >= az.lo && gz < az.lo + (int64_t)az.len ? b.tab[az.off + (uint64_t)k * az.len + (uint64_t)(gz - az.lo)] : 0.f;
-            const float fz = b.amp[k] * fz0;
-            const float w = fz * fy;
-            c = __builtin_fmaf(w, fxs[k * 256u + threadIdx.x], c);
-        }
+        for (uint32_t k = 0; k < nb; ++k) c = __builtin_fmaf(ws[k], fxs[k * 256u + threadIdx.x], c);
         int32_t nz = 0;
         if (b.noise) {
             const uint32_t span = 2u * b.noise + 1u;
@@ -153,7 +155,7 @@ hipError_t launch_synth_box(const SynthBox &b, hipStream_t s)
     if (b.n[0] == 0 || b.n[1] == 0 || b.n[2] == 0) return hipSuccess;
     if (b.n[1] > 65535u || b.n[2] > 65535u) return hipErrorInvalidValue;
     const dim3 grid((b.n[0] + 255u) / 256u, b.n[1], b.n[2]);
-    const size_t lds = (size_t)b.nb * 256u * sizeof(float);
+    const size_t lds = (size_t)b.nb * 257u * sizeof(float);
     if (b.bytes == 1) k_synth_box<uint8_t><<<grid, 256, lds, s>>>(b);
     else if (b.bytes == 2) k_synth_box<uint16_t><<<grid, 256, lds, s>>>(b);
     else k_synth_box<float><<<grid, 256, lds, s>>>(b);
