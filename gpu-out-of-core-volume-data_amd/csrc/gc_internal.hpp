@@ -130,6 +130,66 @@ hipError_t launch_insert(uint4 *mrpd, uint32_t *cells, const uint32_t *order, ui
                          uint32_t nne, const uint32_t *asked4, const gc_geometry *d_geom,
                          int serial, hipStream_t stream);
 
+// ------------------------------------------------------ page-table caches
+// The table caches between the MRPD and the pool (CacheManager.hpp:151-236).
+// Table i (i = 0 next to the MRPD) holds blocks of block[i] entries; its
+// entries are compact words, block-major (entry `loc` of block b at
+// b * b3 + loc, loc x-fastest inside the block), child << 2 | w, the child
+// being a block of table i + 1, or a brick slot for the last table.  The
+// MRPD's cell table holds table-0 block ids.  Each block has a usage stamp,
+// an LRU position (order: a permutation of block ids) and the level and
+// position of the brick it was taken for (meta).
+struct TableState {
+    uint32_t *words[GC_MAX_TABLES];
+    uint32_t *usage[GC_MAX_TABLES];
+    uint32_t *order[GC_MAX_TABLES];
+    SlotMeta *meta[GC_MAX_TABLES];
+};
+
+__host__ __device__ inline uint32_t word_state_of(uint32_t w) { return w & 3u; }
+__host__ __device__ inline uint32_t table_b3(const gc_table_geometry &t) { return t.block[0] * t.block[1] * t.block[2]; }
+
+// origin of block b of table t in its surface (the entry's x, y, z)
+__host__ __device__ inline void block_origin(const gc_table_geometry &t, uint32_t b, uint32_t o[3])
+{
+    o[0] = (b % t.blocks[0]) * t.block[0];
+    o[1] = ((b / t.blocks[0]) % t.blocks[1]) * t.block[1];
+    o[2] = (b / (t.blocks[0] * t.blocks[1])) * t.block[2];
+}
+
+// MRPD cell of a position (K_MultiResolutionPageDirectory.hpp:168-180 over
+// the MRPD's levels: the brick grid without table caches)
+__host__ __device__ inline uint32_t top_index(const gc_geometry &g, uint32_t level, const float p[3])
+{
+    const uint32_t x = g.top_coord[level] + (uint32_t)(p[0] * (float)g.top_dim[level][0]);
+    const uint32_t y = (uint32_t)(p[1] * (float)g.top_dim[level][1]);
+    const uint32_t z = (uint32_t)(p[2] * (float)g.top_dim[level][2]);
+    return x + g.top_cells_dim[0] * (y + g.top_cells_dim[1] * z);
+}
+
+// Entry of a position inside its table-t block: (uint)(p * virtualized) %
+// BlockSize per axis (K_CacheDevice.hpp:221-233,316-334, covering 0), as the
+// block-local index.
+__host__ __device__ inline uint32_t table_local(const gc_table_geometry &t, uint32_t level, const float p[3])
+{
+    uint32_t l[3];
+    for (int a = 0; a < 3; ++a) l[a] = ((uint32_t)(p[a] * (float)t.virt[level][a])) % t.block[a];
+    return l[0] + t.block[0] * (l[1] + t.block[1] * l[2]);
+}
+
+// dereference_old_bricks through the tables: the entry that maps a victim
+// slot is cleared if the walk still reaches it (see kernels_cache.hip).
+hipError_t launch_deref_tables(uint32_t *cells, const TableState &ts_, const uint32_t *order, uint32_t nslots,
+                               uint32_t nne, const SlotMeta *meta, const gc_geometry *d_geom, uint32_t ts,
+                               hipStream_t stream);
+// reference_new_bricks through the tables, one brick after the other (see
+// kernels_cache.hip); unreferenced[0] counts the bricks left unreferenced
+// because a table had no block left in this load.
+hipError_t launch_reference_tables(uint4 *mrpd, uint32_t *cells, const TableState &ts_, const uint32_t *order,
+                                   uint32_t nslots, uint32_t n, uint32_t nne, const uint32_t *asked4,
+                                   const SlotMeta *meta, const gc_geometry *d_geom, uint32_t ts,
+                                   uint32_t *unreferenced, hipStream_t stream);
+
 hipError_t launch_occupancy(const SlotMeta *meta, uint32_t nslots, uint32_t *count,
                             hipStream_t stream);
 
@@ -240,9 +300,20 @@ struct DvrView {
                                                    // collection after a launch
 };
 
+// The table caches as the ray casters walk them (K_CacheManager.hpp:297-339).
+struct PageTables {
+    uint32_t n;                                  // table caches; 0: the MRPD maps bricks
+    uint32_t top_x, top_y;                       // MRPD cells per row, rows per plane
+    uint32_t block[GC_MAX_TABLES][3];
+    uint32_t b3[GC_MAX_TABLES];                  // entries per block
+    const uint32_t *words[GC_MAX_TABLES];
+    uint32_t *usage[GC_MAX_TABLES];
+};
+
 struct DvrLaunch {
     const uint4 *mrpd;
     const uint32_t *cells;      // cell table (cell_word per MRPD cell)
+    PageTables pt;
     uint32_t *request;
     uint32_t *usage;
     const void *pool;
@@ -277,7 +348,8 @@ struct DvrLaunch {
     uint32_t covering;
     uint32_t exact_resume;      // 1: some view has rpos (exact continuation of stopped rays)
     uint32_t pipelined;         // 1: software-pipelined march (requires realDivVirtual == 1 on every
-                                //    level and MRPD y*z cells, x cells below 2^24; see launch_base)
+                                //    level and MRPD y*z cells, x cells below 2^24, and no table
+                                //    caches; see launch_base)
 };
 
 // Ray tiles: 0 8x8, 1 16x4, 2 32x2, 3 64x1 are one 64-ray packet each; 4

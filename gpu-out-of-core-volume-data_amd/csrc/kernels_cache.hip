@@ -554,6 +554,181 @@ hipError_t launch_insert(uint4 *mrpd, uint32_t *cells, const uint32_t *order, ui
     return hipGetLastError();
 }
 
+// ----------------------------------------------------- page-table caches
+// dereference_old_bricks (CacheManagerKernels.hpp:111-145) through the table
+// caches: the walk to a victim slot's entry reads the MRPD and tables 0 ..
+// T-2 (stamping their blocks, the reference's get calls); the entry in the
+// last table is cleared if it still names the slot.  The reference clears
+// the entry the walk reaches whatever it holds -- after the slot's table
+// block was evicted, another brick's entry, or, from an Unmapped entry,
+// one at the surface origin; here the walk stops at an entry that is not
+// Mapped and the last entry is cleared only when it names the slot.  Victims
+// are independent: an entry names one slot, so at most one thread clears it.
+__global__ void k_deref_tables(uint32_t *__restrict__ cells, TableState tab, const uint32_t *__restrict__ order,
+                               uint32_t N, uint32_t nne, const SlotMeta *__restrict__ meta,
+                               const gc_geometry *__restrict__ g, uint32_t ts)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nne) return;
+    const uint32_t s = order[N - nne + j];
+    const SlotMeta m = meta[s];
+    if (m.vol[0] < 0.f) return;   // never-filled slot
+    const uint32_t T = g->nb_tables;
+    uint32_t w = cells[top_index(*g, m.level, m.vol)];
+    for (uint32_t i = 0; i + 1 < T; ++i) {
+        if (word_state_of(w) != 1) return;
+        const uint32_t b = w >> 2;
+        tab.usage[i][b] = ts;
+        w = tab.words[i][(uint64_t)b * table_b3(g->table[i]) + table_local(g->table[i], m.level, m.vol)];
+    }
+    if (word_state_of(w) != 1) return;
+    const gc_table_geometry &t = g->table[T - 1];
+    uint32_t *e = tab.words[T - 1] + (uint64_t)(w >> 2) * table_b3(t) + table_local(t, m.level, m.vol);
+    if (*e == cell_word(s, 1)) *e = 0u;
+}
+
+// Serial state of one reference_new_bricks call.
+struct RefCtx {
+    uint4 *mrpd;
+    uint32_t *cells;
+    TableState tab;
+    const gc_geometry *g;
+    uint32_t ts;
+    uint32_t used[GC_MAX_TABLES];   // blocks taken from each table's LRU tail
+};
+
+// the MRPD cell / table entry that maps the owner of block vb of table j
+// (level and position of the brick it was taken for), if it still names vb;
+// reads above table j-1 stamp their blocks (upward's get calls,
+// CacheManagerKernels.hpp:246-263)
+__device__ inline uint32_t *owner_word(RefCtx &c, uint32_t j, const SlotMeta &m, uint32_t vb, uint32_t *top)
+{
+    const gc_geometry &g = *c.g;
+    *top = top_index(g, m.level, m.vol);
+    uint32_t *p = c.cells + *top;
+    for (uint32_t i = 0; i < j; ++i) {
+        if (word_state_of(*p) != 1) return nullptr;
+        const uint32_t b = *p >> 2;
+        if (i + 1 < j) c.tab.usage[i][b] = c.ts;
+        p = c.tab.words[i] + (uint64_t)b * table_b3(g.table[i]) + table_local(g.table[i], m.level, m.vol);
+        *top = UINT32_MAX;
+    }
+    return *p == cell_word(vb, 1) ? p : nullptr;
+}
+
+// reference_new_bricks with downward / upward (CacheManagerKernels.hpp
+// :148-357) for one brick at (level, vol): child is the brick's slot (w = 1)
+// or 0 with w = 2 for an Empty brick.  downward walks the Mapped levels; the
+// brick is referenced only when the walk ends at an Unmapped entry, and the
+// tables below the last Mapped level each take their next LRU-tail block.
+// upward, bottom table first: the block is cleared, its previous owner's
+// entry cleared where it still names the block, the new entry written and
+// the block's owner recorded; then the new chain is linked into the last
+// Mapped level.  Departures from the reference's latent multi-table path,
+// each one of its defects: its downward stack outlives the brick (a brick
+// with an Unmapped MRPD entry is linked into an earlier brick's block,
+// :322-350); fill_stack also takes a block from the table whose entry is
+// Mapped and indexes lruSizes / indexLRU of the wrong table (:130-136,
+// 176-183); an Empty brick is entered with w = 1 in its block and w = 2 in
+// the MRPD, hiding the whole block (:210-214,226-232); the owner's entry is
+// looked up one table too deep and cleared unchecked (:246-263).  Here a
+// table with no block left in this load leaves the brick unreferenced.
+__device__ void reference_one(RefCtx &c, uint32_t level, const float vol[3], uint32_t child, uint32_t cw,
+                              uint32_t *unreferenced)
+{
+    const gc_geometry &g = *c.g;
+    const uint32_t T = g.nb_tables;
+    const uint32_t top = top_index(g, level, vol);
+    uint32_t *p = c.cells + top;
+    uint32_t d = 0;
+    while (d < T && word_state_of(*p) == 1) {
+        const uint32_t b = *p >> 2;
+        c.tab.usage[d][b] = c.ts;
+        p = c.tab.words[d] + (uint64_t)b * table_b3(g.table[d]) + table_local(g.table[d], level, vol);
+        ++d;
+    }
+    if (word_state_of(*p) != 0) return;
+    for (uint32_t j = d; j < T; ++j)
+        if (c.used[j] >= g.table[j].nblocks) {
+            ++*unreferenced;
+            return;
+        }
+    uint32_t vic[GC_MAX_TABLES];
+    for (uint32_t j = d; j < T; ++j) vic[j] = c.tab.order[j][g.table[j].nblocks - 1 - c.used[j]++];
+    for (uint32_t j = T; j-- > d;) {
+        const gc_table_geometry &t = g.table[j];
+        const uint32_t b3 = table_b3(t);
+        uint32_t *blk = c.tab.words[j] + (uint64_t)vic[j] * b3;
+        for (uint32_t k = 0; k < b3; ++k) blk[k] = 0u;   // reset_entries (K_CacheDevice.hpp:279-311)
+        SlotMeta &m = c.tab.meta[j][vic[j]];
+        if (m.vol[0] > -1.f) {
+            uint32_t otop;
+            uint32_t *ow = owner_word(c, j, m, vic[j], &otop);
+            if (ow) {
+                *ow = 0u;
+                if (otop != UINT32_MAX) c.mrpd[otop] = make_uint4(0, 0, 0, 0);
+            }
+        }
+        blk[table_local(t, level, vol)] = j + 1 == T ? cell_word(child, cw) : cell_word(vic[j + 1], 1);
+        m.vol[0] = vol[0];
+        m.vol[1] = vol[1];
+        m.vol[2] = vol[2];
+        m.level = level;
+    }
+    if (d == T) {
+        *p = cell_word(child, cw);
+    } else {
+        *p = cell_word(vic[d], 1);
+        if (d == 0) {
+            uint32_t o[3];
+            block_origin(g.table[0], vic[0], o);
+            c.mrpd[top] = make_uint4(o[0], o[1], o[2], 1);
+        }
+    }
+}
+
+__global__ void k_reference_tables(uint4 *mrpd, uint32_t *cells, TableState tab, const uint32_t *order, uint32_t N,
+                                   uint32_t n, uint32_t nne, const uint32_t *asked4, const SlotMeta *meta,
+                                   const gc_geometry *g, uint32_t ts, uint32_t *unreferenced)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    RefCtx c{mrpd, cells, tab, g, ts, {0u, 0u, 0u}};
+    uint32_t unref = 0;
+    // non-empty bricks (their slots already hold level and position), then
+    // the Empty ones (CacheManagerKernels.hpp:336-353)
+    for (uint32_t k = 0; k < nne; ++k) {
+        const uint32_t s = order[N - nne + k];
+        const SlotMeta m = meta[s];
+        reference_one(c, m.level, m.vol, s, 1, &unref);
+    }
+    for (uint32_t k = nne; k < n; ++k) {
+        const uint32_t *a = asked4 + 4 * k;
+        float vol[3];
+        volume_position(*g, a[3], a, vol);
+        reference_one(c, a[3], vol, 0, 2, &unref);
+    }
+    *unreferenced = unref;
+}
+
+hipError_t launch_deref_tables(uint32_t *cells, const TableState &tab, const uint32_t *order, uint32_t nslots,
+                               uint32_t nne, const SlotMeta *meta, const gc_geometry *d_geom, uint32_t ts,
+                               hipStream_t stream)
+{
+    if (nne == 0) return hipSuccess;
+    k_deref_tables<<<blocks_for(nne, BLK), BLK, 0, stream>>>(cells, tab, order, nslots, nne, meta, d_geom, ts);
+    return hipGetLastError();
+}
+
+hipError_t launch_reference_tables(uint4 *mrpd, uint32_t *cells, const TableState &tab, const uint32_t *order,
+                                   uint32_t nslots, uint32_t n, uint32_t nne, const uint32_t *asked4,
+                                   const SlotMeta *meta, const gc_geometry *d_geom, uint32_t ts,
+                                   uint32_t *unreferenced, hipStream_t stream)
+{
+    k_reference_tables<<<1, 1, 0, stream>>>(mrpd, cells, tab, order, nslots, n, nne, asked4, meta, d_geom, ts,
+                                            unreferenced);
+    return hipGetLastError();
+}
+
 hipError_t launch_occupancy(const SlotMeta *meta, uint32_t nslots, uint32_t *count, hipStream_t stream)
 {
     hipError_t e = hipMemsetAsync(count, 0, sizeof(uint32_t), stream);

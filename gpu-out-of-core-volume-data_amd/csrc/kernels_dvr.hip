@@ -519,6 +519,75 @@ __device__ inline void locate_cell(F3 pos, F3 ior, F3 ldf, F3 vvf, uint32_t lcoo
 __device__ inline uint32_t word_slot(uint32_t c) { return c >> 2; }
 __device__ inline uint32_t word_state(uint32_t c) { return c & 3u; }
 
+// Per level, what the walk through the table caches needs (LDS copy).
+struct PtLds {
+    float tdf[3];                       // MRPD entries per axis
+    uint32_t tcoord;                    // MRPD x offset of the level
+    float tvf[GC_MAX_TABLES][3];        // per table: virtualized entries per axis
+};
+
+__device__ inline void pt_lds_fill(PtLds &Q, const gc_geometry *g, uint32_t l)
+{
+    for (int a = 0; a < 3; ++a) {
+        Q.tdf[a] = (float)g->top_dim[l][a];
+        for (int i = 0; i < GC_MAX_TABLES; ++i) Q.tvf[i][a] = (float)g->table[i].virt[l][a];
+    }
+    Q.tcoord = g->top_coord[l];
+}
+
+// The key of the entries a walk reads at the page position pv: the MRPD
+// cell, then per table the entry inside its block, (uint)(pv * virtualized)
+// % BlockSize (K_MultiResolutionPageDirectory.hpp:143-155,
+// K_CacheDevice.hpp:316-334).  Positions with equal keys walk to the same
+// entry.
+struct PtKey {
+    uint32_t top, loc[GC_MAX_TABLES];
+};
+__device__ inline PtKey pt_key(const PageTables &pt, const PtLds &Q, F3 pv)
+{
+    PtKey k;
+    k.top = Q.tcoord + (uint32_t)(pv.x * Q.tdf[0]) +
+            pt.top_x * ((uint32_t)(pv.y * Q.tdf[1]) + pt.top_y * (uint32_t)(pv.z * Q.tdf[2]));
+#pragma unroll
+    for (int i = 0; i < GC_MAX_TABLES; ++i) {
+        k.loc[i] = 0u;
+        if ((uint32_t)i < pt.n) {
+            const uint32_t lx = ((uint32_t)(pv.x * Q.tvf[i][0])) % pt.block[i][0];
+            const uint32_t ly = ((uint32_t)(pv.y * Q.tvf[i][1])) % pt.block[i][1];
+            const uint32_t lz = ((uint32_t)(pv.z * Q.tvf[i][2])) % pt.block[i][2];
+            k.loc[i] = lx + pt.block[i][0] * (ly + pt.block[i][1] * lz);
+        }
+    }
+    return k;
+}
+__device__ inline bool pt_same(const PtKey &a, const PtKey &b)
+{
+    bool e = a.top == b.top;
+#pragma unroll
+    for (int i = 0; i < GC_MAX_TABLES; ++i) e = e && a.loc[i] == b.loc[i];
+    return e;
+}
+// The walk of get_normalized (K_CacheManager.hpp:297-339): the MRPD word,
+// then one word per table while Mapped, stamping each table block read
+// (compute_element_position, K_CacheDevice.hpp:316-327).  Returns the last
+// word read: w = 1 maps the brick slot, 0 Unmapped, 2 Empty.  (The
+// reference, after an Unmapped table entry with a coarser level pending,
+// repeats the same lookup for ever, K_CacheManager.hpp:318-330; such an
+// entry is Unmapped here, as an Unmapped MRPD entry is.)
+__device__ inline uint32_t pt_walk(const DvrLaunch &p, const PtKey &k, uint32_t ts)
+{
+    uint32_t w = p.cells[k.top];
+#pragma unroll
+    for (int i = 0; i < GC_MAX_TABLES; ++i) {
+        if ((uint32_t)i < p.pt.n && word_state(w) == 1) {
+            const uint32_t b = word_slot(w);
+            p.pt.usage[i][b] = ts;
+            w = p.pt.words[i][(uint64_t)b * p.pt.b3[i] + k.loc[i]];
+        }
+    }
+    return w;
+}
+
 // Byte offset of the transfer-function tables in a launch's dynamic LDS:
 // after `head` bytes (level-switch table, if any) and the view table.
 __host__ __device__ inline uint32_t tf_lds_base(uint32_t head, uint32_t n_views)
@@ -546,11 +615,14 @@ __device__ inline void load_tf_tables(const DvrView *vw, uint32_t n_views, float
 // the bricks of the out-of-core configurations), else 0: constant row and
 // plane strides turn the footprint's four loads into one address and three
 // immediate offsets.
-template <int BYTES, typename ACC, int TW, int TH, bool MT, int BSC = 0>
+// PT: the page directory has table caches (the non-pipelined march with the
+// walk through them; otherwise the MRPD maps bricks).
+template <int BYTES, typename ACC, int TW, int TH, bool MT, int BSC = 0, bool PT = false>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : DVR_WAVES_PER_EU))) void k_dvr(const DvrLaunch p)
 {
     static_assert(TW * TH == 64, "one wave per ray packet");
     __shared__ LevelLds lv[GC_MAX_LEVELS];
+    __shared__ PtLds plv[PT ? GC_MAX_LEVELS : 1];
     // dynamic LDS: level-switch distances [nb_lods][LANES], the views, then
     // the launch's transfer functions (padded copies), when they fit
     extern __shared__ __align__(16) uint8_t dyn_lds[];
@@ -574,6 +646,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         L.step = p.lod_step[tid];
         L.kexp = p.kexp[tid];
         L.esq = p.esq[tid];
+        if constexpr (PT) pt_lds_fill(plv[tid], g, tid);
     }
     if (p.views) {
         for (uint32_t i = tid; i < n_views; i += LANES) vw[i] = p.views[i];
@@ -626,6 +699,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     F3 ior = f3(0, 0, 0), rdv = f3(0, 0, 0), ldf = f3(0, 0, 0), vvf = f3(0, 0, 0);
     // current brick: page-table cell, entry state, slot and base address
     uint32_t pcx = NONE, pcy = NONE, pcz = NONE, ew = 0, slot = 0, vkey = NONE;
+    PtKey pk_prev{NONE, {NONE, NONE, NONE}};   // PT: the walk's last key
     uint32_t vsum = 0;   // box sum of the voxel vkey (non-pipelined march)
     const uint8_t *bbase = pool;
     Acc4<ACC> fc{0, 0, 0, 0};
@@ -658,7 +732,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // consumed, in the reference's order, so the trace is the one-sample-at-
     // a-time trace.  The page cell doubles as the request cell
     // (realDivVirtual == 1).
-    const bool piped = fast_fetch && p.pipelined != 0;
+    const bool piped = !PT && fast_fetch && p.pipelined != 0;
     // hb: stage B holds a sample (the front placed last half-step); hc: stage
     // C holds one; newray: the next sample placed is the ray's first
     bool hb = false, hc = false, newray = false;
@@ -864,6 +938,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                         lod = l0;
                         set_level(l0);
                         pcx = pcy = pcz = NONE;
+                        if constexpr (PT) pk_prev.top = NONE;
                         vkey = NONE;
                         hb = hc = false;
                         newray = true;
@@ -1014,20 +1089,36 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 const F3 np = f3(clamp099(tp.x), clamp099(tp.y), clamp099(tp.z));
                 const F3 pr = np * ior;
                 const F3 pv = pr * rdv;   // exact when rdv == 1
-                const uint32_t bx = (uint32_t)(pv.x * ldf.x);
-                const uint32_t by = (uint32_t)(pv.y * ldf.y);
-                const uint32_t bz = (uint32_t)(pv.z * ldf.z);
-                const uint32_t mcx = lcoord + bx;
-                if (mcx != pcx || by != pcy || bz != pcz) {
-                    pcx = mcx; pcy = by; pcz = bz;
-                    // the page directory holds fewer than 2^32 cells (checked at creation)
-                    const uint32_t e = p.cells[mcx + MX * (by + MY * bz)];
-                    ew = word_state(e);
-                    ++n_visits;
-                    slot = word_slot(e);
-                    bbase = pool + (uint64_t)slot * bbytes + corner;
-                    vkey = NONE;
-                    if (ew == 1) p.usage[slot] = ts;
+                uint32_t bx = 0, by = 0, bz = 0;
+                if constexpr (!PT) {
+                    bx = (uint32_t)(pv.x * ldf.x);
+                    by = (uint32_t)(pv.y * ldf.y);
+                    bz = (uint32_t)(pv.z * ldf.z);
+                    const uint32_t mcx = lcoord + bx;
+                    if (mcx != pcx || by != pcy || bz != pcz) {
+                        pcx = mcx; pcy = by; pcz = bz;
+                        // the page directory holds fewer than 2^32 cells (checked at creation)
+                        const uint32_t e = p.cells[mcx + MX * (by + MY * bz)];
+                        ew = word_state(e);
+                        ++n_visits;
+                        slot = word_slot(e);
+                        bbase = pool + (uint64_t)slot * bbytes + corner;
+                        vkey = NONE;
+                        if (ew == 1) p.usage[slot] = ts;
+                    }
+                } else {
+                    // the walk's key; walk again when it changes
+                    const PtKey k = pt_key(p.pt, plv[lod], pv);
+                    if (!pt_same(k, pk_prev)) {
+                        pk_prev = k;
+                        const uint32_t e = pt_walk(p, k, ts);
+                        ew = word_state(e);
+                        ++n_visits;
+                        slot = word_slot(e);
+                        bbase = pool + (uint64_t)slot * bbytes + corner;
+                        vkey = NONE;
+                        if (ew == 1) p.usage[slot] = ts;
+                    }
                 }
                 if (ew == 0) {
                     // raise_brick_request with the real-volume position
@@ -1052,11 +1143,18 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 const uint32_t vy = (uint32_t)(pv.y * vvf.y);
                 const uint32_t vz = (uint32_t)(pv.z * vvf.z);
                 // 24-bit products: voxel and brick indices stay far below 2^24
-                uint32_t lx = vx - __umul24(bx, inner), ly = vy - __umul24(by, inner), lz = vz - __umul24(bz, inner);
-                if (__builtin_expect(lx >= inner || ly >= inner || lz >= inner, 0)) {
+                uint32_t lx, ly, lz;
+                if constexpr (PT) {
                     lx = vx % inner;
                     ly = vy % inner;
                     lz = vz % inner;
+                } else {
+                    lx = vx - __umul24(bx, inner), ly = vy - __umul24(by, inner), lz = vz - __umul24(bz, inner);
+                    if (__builtin_expect(lx >= inner || ly >= inner || lz >= inner, 0)) {
+                        lx = vx % inner;
+                        ly = vy % inner;
+                        lz = vz % inner;
+                    }
                 }
                 ACC s = 0;
                 double sd = 0.0;
@@ -1505,6 +1603,7 @@ template <int BYTES, typename ACC>
 __global__ __launch_bounds__(256) void k_mop(const DvrLaunch p)
 {
     __shared__ LevelLds lv[GC_MAX_LEVELS];
+    __shared__ PtLds plv[GC_MAX_LEVELS];
     const int tid = threadIdx.x;
     const gc_geometry *g = p.geom;
     const uint32_t nl = p.nb_lods;
@@ -1519,6 +1618,7 @@ __global__ __launch_bounds__(256) void k_mop(const DvrLaunch p)
         }
         L.lcoord = g->level_coord[tid];
         L.step = p.lod_step[tid];
+        if (p.pt.n) pt_lds_fill(plv[tid], g, tid);
     }
     __syncthreads();
     const uint32_t lane = tid & 63u;
@@ -1611,6 +1711,7 @@ __global__ __launch_bounds__(256) void k_mop(const DvrLaunch p)
                 }
             }
             uint32_t pcell = NONE, ew = 0, slot = 0;
+            PtKey pkp{NONE, {NONE, NONE, NONE}};
             for (;;) {
                 if (!(t < tfar)) break;
                 while (t >= tnext) {
@@ -1625,16 +1726,30 @@ __global__ __launch_bounds__(256) void k_mop(const DvrLaunch p)
                 const F3 np = f3(clamp099(tp.x), clamp099(tp.y), clamp099(tp.z));
                 const F3 pr = np * f3(L.ior[0], L.ior[1], L.ior[2]);
                 const F3 pv = pr * f3(L.rdv[0], L.rdv[1], L.rdv[2]);
-                const uint32_t bx = (uint32_t)(pv.x * L.ldf[0]), by = (uint32_t)(pv.y * L.ldf[1]),
-                               bz = (uint32_t)(pv.z * L.ldf[2]);
-                const uint32_t cell = L.lcoord + bx + MX * (by + MY * bz);
-                if (cell != pcell) {
-                    pcell = cell;
-                    const uint32_t w = p.cells[cell];
-                    ew = word_state(w);
-                    slot = word_slot(w);
-                    ++n_visits;
-                    if (ew == 1) p.usage[slot] = ts;
+                uint32_t bx = 0, by = 0, bz = 0;   // (with table caches the voxel below takes v mod inner)
+                if (p.pt.n) {
+                    const PtKey k = pt_key(p.pt, plv[lod], pv);
+                    if (!pt_same(k, pkp)) {
+                        pkp = k;
+                        const uint32_t w = pt_walk(p, k, ts);
+                        ew = word_state(w);
+                        slot = word_slot(w);
+                        ++n_visits;
+                        if (ew == 1) p.usage[slot] = ts;
+                    }
+                } else {
+                    bx = (uint32_t)(pv.x * L.ldf[0]);
+                    by = (uint32_t)(pv.y * L.ldf[1]);
+                    bz = (uint32_t)(pv.z * L.ldf[2]);
+                    const uint32_t cell = L.lcoord + bx + MX * (by + MY * bz);
+                    if (cell != pcell) {
+                        pcell = cell;
+                        const uint32_t w = p.cells[cell];
+                        ew = word_state(w);
+                        slot = word_slot(w);
+                        ++n_visits;
+                        if (ew == 1) p.usage[slot] = ts;
+                    }
                 }
                 if (ew == 0) {
                     // raise_brick_request with the real-volume position, stop
@@ -1725,8 +1840,18 @@ __global__ __launch_bounds__(256) void k_mop(const DvrLaunch p)
 }
 
 template <int BYTES, typename ACC>
-const void *kernel_of(uint32_t tile, bool mt, bool bs32)
+const void *kernel_of(uint32_t tile, bool mt, bool bs32, bool pt)
 {
+    // table caches: one variant per packet shape (transfer functions from
+    // HBM when a launch has several)
+    if (pt) {
+        switch (tile & 3) {
+        case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 0, true>);
+        case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4, false, 0, true>);
+        case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2, false, 0, true>);
+        default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1, false, 0, true>);
+        }
+    }
     // 8x8 tiles of u8/u16 bricks of 32 with covering 1 at FP32: constant strides
     if (BYTES != 4 && sizeof(ACC) == 4 && (tile & 3) == 0 && bs32)
         return mt ? reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, true, 32>)
@@ -1748,14 +1873,14 @@ const void *kernel_of(uint32_t tile, bool mt, bool bs32)
     }
 }
 
-const void *dvr_kernel(uint32_t bytes, uint32_t precision, uint32_t tile, bool mt, bool bs32)
+const void *dvr_kernel(uint32_t bytes, uint32_t precision, uint32_t tile, bool mt, bool bs32, bool pt = false)
 {
     const bool d = precision == 64;
     tile = packet_kind(tile);
     switch (bytes) {
-    case 1: return d ? kernel_of<1, double>(tile, mt, bs32) : kernel_of<1, float>(tile, mt, bs32);
-    case 2: return d ? kernel_of<2, double>(tile, mt, bs32) : kernel_of<2, float>(tile, mt, bs32);
-    default: return d ? kernel_of<4, double>(tile, mt, bs32) : kernel_of<4, float>(tile, mt, bs32);
+    case 1: return d ? kernel_of<1, double>(tile, mt, bs32, pt) : kernel_of<1, float>(tile, mt, bs32, pt);
+    case 2: return d ? kernel_of<2, double>(tile, mt, bs32, pt) : kernel_of<2, float>(tile, mt, bs32, pt);
+    default: return d ? kernel_of<4, double>(tile, mt, bs32, pt) : kernel_of<4, float>(tile, mt, bs32, pt);
     }
 }
 
@@ -1840,6 +1965,15 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
     if (p.list && !p.list_count) return hipErrorInvalidValue;
     if (p.march < 1 || p.march > 8) return hipErrorInvalidValue;
+    if (p.pt.n) {
+        if (p.pipelined) return hipErrorInvalidValue;
+        // the table-cache variants keep one LDS transfer function at most
+        DvrLaunch q = p;
+        if (q.tf_lds_tables > 1) q.tf_lds_entries = q.tf_lds_tables = 0;
+        void *args[] = {&q};
+        return hipLaunchKernel(dvr_kernel(bytes, precision, tile, false, false, true), dim3(grid), dim3(LANES), args,
+                               dvr_lds_bytes(q.nb_lods, q.views ? q.n_views : 1, q.tf_lds_entries), stream);
+    }
     void *args[] = {const_cast<DvrLaunch *>(&p)};
     const bool bs32 = p.brick_size == 32 && p.covering == 1;
     return hipLaunchKernel(dvr_kernel(bytes, precision, tile, p.tf_lds_tables > 1, bs32), dim3(grid), dim3(LANES), args,
@@ -1851,7 +1985,10 @@ uint32_t dvr_ray_lds_bytes(uint32_t n_views, uint32_t tf_entries)
     return tf_lds_base(0, std::max(1u, n_views)) + tf_entries * 16u;
 }
 
-bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes) { return p.pipelined != 0 && (bytes == 1 || bytes == 2); }
+bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes)
+{
+    return p.pipelined != 0 && p.pt.n == 0 && (bytes == 1 || bytes == 2);
+}
 
 // Lanes per ray: a whole wave for the shortest lists (lowest latency per
 // ray), 16 lanes (4 rays per wave) otherwise.

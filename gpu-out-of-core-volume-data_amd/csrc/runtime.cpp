@@ -625,33 +625,63 @@ static bool validate(const gc_volume_cfg &vc, const gc_cache_cfg &cc, std::strin
     for (uint32_t l = 0; l < vc.nb_levels; ++l)
         for (int a = 0; a < 3; ++a)
             if (vc.initial_sizes[l][a] == 0 || vc.real_sizes[l][a] == 0) { why = "zero level size"; return false; }
+    if (cc.nb_tables > GC_MAX_TABLES) { why = "at most GC_MAX_TABLES page-table caches"; return false; }
+    for (uint32_t k = 0; k < cc.nb_tables; ++k)
+        for (int a = 0; a < 3; ++a)
+            if (cc.table_cache_size[k][a] == 0 || cc.table_block_size[k][a] == 0) {
+                why = "page-table cache sizes and block sizes must be >= 1";
+                return false;
+            }
     return true;
 }
 
 static void compute_geometry(const gc_volume_cfg &vc, const gc_cache_cfg &cc, gc_geometry &g)
 {
-    // CacheManager.hpp:157-285 for a single pagination level
+    // CacheManager.hpp:157-285: realNumberOfEntries bottom-up and
+    // virtualizedNumberOfEntries top-down over the hierarchy (index 0 the
+    // voxels, 1 the bricks, ..., T + 1 the MRPD), with the reference's
+    // float / double rounding.
     std::memset(&g, 0, sizeof(g));
     g.nb_levels = vc.nb_levels;
     g.brick_size = vc.brick_size[0];
     g.covering = vc.covering[0];
     g.inner = g.brick_size - 2 * g.covering;
     g.encoded_bytes = vc.encoded_bytes;
+    const uint32_t T = cc.nb_tables, H = T + 2;
     const double inner = (double)g.brick_size - 2.0 * (double)g.covering;
+    uint32_t real[GC_MAX_TABLES + 2][GC_MAX_LEVELS][3] = {}, virt[GC_MAX_TABLES + 2][GC_MAX_LEVELS][3] = {};
     for (uint32_t l = 0; l < vc.nb_levels; ++l)
         for (int a = 0; a < 3; ++a) {
-            const uint32_t r1 = (uint32_t)std::ceil((double)(float)vc.real_sizes[l][a] / inner);
-            g.level_dim[l][a] = r1;
-            g.virt_vox[l][a] = (uint32_t)std::ceil((double)(float)r1 * inner);
+            real[0][l][a] = vc.real_sizes[l][a];
+            real[1][l][a] = (uint32_t)std::ceil((double)(float)vc.real_sizes[l][a] / inner);
+            for (uint32_t h = 2; h < H; ++h)
+                real[h][l][a] = (uint32_t)std::ceil((float)real[h - 1][l][a] / (float)cc.table_block_size[h - 2][a]);
+            virt[H - 1][l][a] = real[H - 1][l][a];
+            for (uint32_t h = H - 2; h >= 1; --h)
+                virt[h][l][a] = (uint32_t)std::ceil((float)virt[h + 1][l][a] * (float)cc.table_block_size[h - 1][a]);
+            virt[0][l][a] = (uint32_t)std::ceil((double)(float)virt[1][l][a] * inner);
+        }
+    for (uint32_t l = 0; l < vc.nb_levels; ++l)
+        for (int a = 0; a < 3; ++a) {
+            g.level_dim[l][a] = real[1][l][a];
+            g.virt_vox[l][a] = virt[0][l][a];
             g.rdv[l][a] = (float)vc.real_sizes[l][a] / (float)g.virt_vox[l][a];
             g.ior[l][a] = (float)vc.initial_sizes[l][a] / (float)vc.real_sizes[l][a];
+            g.top_dim[l][a] = real[H - 1][l][a];
         }
     g.level_coord[0] = 0;
-    for (uint32_t l = 1; l < vc.nb_levels; ++l) g.level_coord[l] = g.level_coord[l - 1] + g.level_dim[l - 1][0];
+    g.top_coord[0] = 0;
+    for (uint32_t l = 1; l < vc.nb_levels; ++l) {
+        g.level_coord[l] = g.level_coord[l - 1] + g.level_dim[l - 1][0];
+        g.top_coord[l] = g.top_coord[l - 1] + g.top_dim[l - 1][0];
+    }
     for (uint32_t l = 0; l < vc.nb_levels; ++l) {
         g.mrpd_dim[0] += g.level_dim[l][0];
         g.mrpd_dim[1] = std::max(g.mrpd_dim[1], g.level_dim[l][1]);
         g.mrpd_dim[2] = std::max(g.mrpd_dim[2], g.level_dim[l][2]);
+        g.top_cells_dim[0] += g.top_dim[l][0];
+        g.top_cells_dim[1] = std::max(g.top_cells_dim[1], g.top_dim[l][1]);
+        g.top_cells_dim[2] = std::max(g.top_cells_dim[2], g.top_dim[l][2]);
     }
     for (int a = 0; a < 3; ++a) {
         g.one_over[a] = 1.f / (float)(vc.brick_size[a] - 2 * vc.covering[a]);
@@ -659,6 +689,23 @@ static void compute_geometry(const gc_volume_cfg &vc, const gc_cache_cfg &cc, gc
     }
     g.nslots = cc.cache_size[0] * cc.cache_size[1] * cc.cache_size[2];
     g.mrpd_cells = (uint64_t)g.mrpd_dim[0] * g.mrpd_dim[1] * g.mrpd_dim[2];
+    g.top_cells = (uint64_t)g.top_cells_dim[0] * g.top_cells_dim[1] * g.top_cells_dim[2];
+    // _tableCaches[i] is hierarchy index H - 2 - i (CacheManager.hpp:223-233)
+    g.nb_tables = T;
+    for (uint32_t i = 0; i < T; ++i) {
+        const uint32_t h = H - 2 - i;
+        gc_table_geometry &t = g.table[i];
+        for (int a = 0; a < 3; ++a) {
+            t.block[a] = cc.table_block_size[h - 1][a];
+            t.blocks[a] = cc.table_cache_size[h - 1][a];
+            t.surf[a] = t.block[a] * t.blocks[a];
+            for (uint32_t l = 0; l < vc.nb_levels; ++l) {
+                t.virt[l][a] = virt[h][l][a];
+                t.real[l][a] = real[h][l][a];
+            }
+        }
+        t.nblocks = t.blocks[0] * t.blocks[1] * t.blocks[2];
+    }
 }
 
 struct Cache;
@@ -710,6 +757,10 @@ struct Cache {
     MissScratch miss_s{};
     uint32_t *d_misses = nullptr;       // cap x uint4
     uint32_t *d_count = nullptr;        // scratch counter
+    // page-table caches (g.nb_tables of them)
+    TableState tabs{};
+    uint32_t *tab_order_tmp[GC_MAX_TABLES] = {};
+    uint32_t *d_unref = nullptr;        // bricks the last load left unreferenced
     // host staging
     uint32_t *h_misses = nullptr;       // pinned: [total, pad x3, cap x uint4]
     uint8_t *h_staging = nullptr;       // pinned: cap big bricks
@@ -745,6 +796,14 @@ void Cache::release()
     hipFree(lru_s.masks); hipFree(lru_s.counts); hipFree(lru_s.total);
     hipFree(miss_s.masks); hipFree(miss_s.counts); hipFree(miss_s.total);
     hipFree(d_misses); hipFree(d_count); hipFree(d_staging); hipFree(d_meta);
+    for (int i = 0; i < GC_MAX_TABLES; ++i) {
+        hipFree(tabs.words[i]); hipFree(tabs.usage[i]); hipFree(tabs.order[i]); hipFree(tabs.meta[i]);
+        hipFree(tab_order_tmp[i]);
+        tabs.words[i] = tabs.usage[i] = tabs.order[i] = tab_order_tmp[i] = nullptr;
+        tabs.meta[i] = nullptr;
+    }
+    hipFree(d_unref);
+    d_unref = nullptr;
     if (h_misses) hipHostFree(h_misses);
     if (h_staging) hipHostFree(h_staging);
     if (h_meta) hipHostFree(h_meta);
@@ -786,6 +845,9 @@ int Cache::create()
     compute_geometry(vc, cc, g);
     if (g.mrpd_cells >= (1ull << 32)) return fail(GC_ERR_CONFIG, "page directory exceeds 2^32 cells");
     if (g.nslots >= (1u << 24)) return fail(GC_ERR_CONFIG, "pool exceeds 2^24 brick slots");
+    for (uint32_t i = 0; i < g.nb_tables; ++i)
+        if ((uint64_t)g.table[i].nblocks * table_b3(g.table[i]) >= (1ull << 32) || g.table[i].nblocks >= (1u << 28))
+            return fail(GC_ERR_CONFIG, "page-table cache exceeds 2^32 entries");
     cap = cc.max_requests ? cc.max_requests : 50;
     cap = std::min(cap, g.nslots);
     limit = cap;
@@ -794,8 +856,8 @@ int Cache::create()
     GC_HIP(hipEventCreateWithFlags(&uploaded, hipEventDisableTiming));
     GC_HIP(hipEventCreateWithFlags(&consumed, hipEventDisableTiming));
     const uint64_t bvox = (uint64_t)g.brick_size * g.brick_size * g.brick_size;
-    GC_TRY(dalloc(&mrpd, g.mrpd_cells * sizeof(uint4)));
-    GC_TRY(dalloc(&cells, g.mrpd_cells * sizeof(uint32_t)));
+    GC_TRY(dalloc(&mrpd, g.top_cells * sizeof(uint4)));
+    GC_TRY(dalloc(&cells, g.top_cells * sizeof(uint32_t)));
     GC_TRY(dalloc(&request, g.mrpd_cells * sizeof(uint32_t)));
     GC_TRY(dalloc(&usage, (size_t)g.nslots * sizeof(uint32_t)));
     GC_TRY(dalloc(&order, (size_t)g.nslots * sizeof(uint32_t)));
@@ -807,8 +869,35 @@ int Cache::create()
     GC_TRY(dalloc(&d_geom, sizeof(gc_geometry)));
     // Surfaces and thrust vectors of the reference start zeroed
     // (Surface3dDevice.hpp:97-109, DynamicArray3dDevice).
-    GC_HIP(hipMemsetAsync(mrpd, 0, g.mrpd_cells * sizeof(uint4), stream));
-    GC_HIP(hipMemsetAsync(cells, 0, g.mrpd_cells * sizeof(uint32_t), stream));
+    GC_HIP(hipMemsetAsync(mrpd, 0, g.top_cells * sizeof(uint4), stream));
+    GC_HIP(hipMemsetAsync(cells, 0, g.top_cells * sizeof(uint32_t), stream));
+    GC_TRY(dalloc(&d_unref, 16));
+    GC_HIP(hipMemsetAsync(d_unref, 0, 16, stream));
+    uint32_t lru_max = g.nslots;
+    for (uint32_t i = 0; i < g.nb_tables; ++i) {
+        // a table cache: zeroed entries and usage, LRU in block order with
+        // no owner (CacheDevice.hpp:145-165)
+        const gc_table_geometry &t = g.table[i];
+        const size_t ne = (size_t)t.nblocks * table_b3(t);
+        GC_TRY(dalloc(&tabs.words[i], ne * 4));
+        GC_TRY(dalloc(&tabs.usage[i], (size_t)t.nblocks * 4));
+        GC_TRY(dalloc(&tabs.order[i], (size_t)t.nblocks * 4));
+        GC_TRY(dalloc(&tab_order_tmp[i], (size_t)t.nblocks * 4));
+        GC_TRY(dalloc(&tabs.meta[i], (size_t)t.nblocks * sizeof(SlotMeta)));
+        GC_HIP(hipMemsetAsync(tabs.words[i], 0, ne * 4, stream));
+        GC_HIP(hipMemsetAsync(tabs.usage[i], 0, (size_t)t.nblocks * 4, stream));
+        std::vector<uint32_t> ord(t.nblocks);
+        std::vector<SlotMeta> mt(t.nblocks);
+        for (uint32_t b = 0; b < t.nblocks; ++b) {
+            ord[b] = b;
+            mt[b].vol[0] = mt[b].vol[1] = mt[b].vol[2] = -1.f;
+            mt[b].level = 0;
+        }
+        GC_HIP(hipMemcpyAsync(tabs.order[i], ord.data(), ord.size() * 4, hipMemcpyHostToDevice, stream));
+        GC_HIP(hipMemcpyAsync(tabs.meta[i], mt.data(), mt.size() * sizeof(SlotMeta), hipMemcpyHostToDevice, stream));
+        GC_HIP(hipStreamSynchronize(stream));
+        lru_max = std::max(lru_max, t.nblocks);
+    }
     GC_HIP(hipMemsetAsync(request, 0, g.mrpd_cells * sizeof(uint32_t), stream));
     GC_HIP(hipMemsetAsync(usage, 0, (size_t)g.nslots * sizeof(uint32_t), stream));
     GC_HIP(hipMemsetAsync(pool, 0, (size_t)g.nslots * bvox * vc.encoded_bytes + 64, stream));
@@ -826,8 +915,8 @@ int Cache::create()
         GC_HIP(hipMemcpyAsync(d_geom, &g, sizeof(g), hipMemcpyHostToDevice, stream));
         GC_HIP(hipStreamSynchronize(stream));
     }
-    lru_s.nblocks = (g.nslots + 1023) / 1024;
-    GC_TRY(dalloc(&lru_s.masks, ((size_t)g.nslots + 63) / 64 * 8 + 8));
+    lru_s.nblocks = (lru_max + 1023) / 1024;
+    GC_TRY(dalloc(&lru_s.masks, ((size_t)lru_max + 63) / 64 * 8 + 8));
     GC_TRY(dalloc(&lru_s.counts, (size_t)lru_s.nblocks * 4 + 4));
     GC_TRY(dalloc(&lru_s.total, 16));
     miss_s.nblocks = (uint32_t)((g.mrpd_cells + 1023) / 1024);
@@ -851,6 +940,13 @@ int Cache::lru_update()
     // CacheManager.hpp:387-406
     GC_HIP(launch_lru_update(order, order_tmp, usage, g.nslots, ts, lru_s, stream));
     std::swap(order, order_tmp);
+    // the table caches after the data cache (the used-slot total the
+    // counters read stays the data cache's: read it before)
+    if (g.nb_tables) GC_HIP(hipMemcpyAsync(d_count + 1, lru_s.total, 4, hipMemcpyDeviceToDevice, stream));
+    for (uint32_t i = 0; i < g.nb_tables; ++i) {
+        GC_HIP(launch_lru_update(tabs.order[i], tab_order_tmp[i], tabs.usage[i], g.table[i].nblocks, ts, lru_s, stream));
+        std::swap(tabs.order[i], tab_order_tmp[i]);
+    }
     return GC_OK;
 }
 
@@ -859,7 +955,8 @@ int Cache::prepare_bricks(bool &done)
     // prepare_bricks + create_request_list (CacheManager.hpp:410-491)
     GC_HIP(launch_miss_compact(request, g.mrpd_cells, ts, d_geom, limit, d_misses, miss_s, stream));
     GC_HIP(hipMemcpyAsync(h_misses, miss_s.total, 4, hipMemcpyDeviceToHost, stream));
-    GC_HIP(hipMemcpyAsync(h_misses + 1, lru_s.total, 4, hipMemcpyDeviceToHost, stream));   // slots used
+    GC_HIP(hipMemcpyAsync(h_misses + 1, g.nb_tables ? d_count + 1 : lru_s.total, 4, hipMemcpyDeviceToHost,
+                          stream));   // slots used
     GC_HIP(hipMemcpyAsync(h_misses + 4, d_misses, (size_t)cap * 16, hipMemcpyDeviceToHost, stream));
     GC_HIP(hipStreamSynchronize(stream));
     const uint32_t n = std::min(h_misses[0], limit);
@@ -908,12 +1005,20 @@ int Cache::load_new_bricks()
     const uint32_t *d_asked = d_meta;
     const uint32_t *d_bbi = d_meta + 4 * (size_t)cap;
     const uint32_t *d_bbc = d_meta + 5 * (size_t)cap;
-    GC_HIP(launch_deref(mrpd, cells, order, g.nslots, nne, meta, d_geom, stream));
-    GC_HIP(launch_assign_copy(order, g.nslots, nne, d_asked, d_bbi, d_bbc, meta, pool, d_staging, d_geom,
-                              vc.big_brick_size, vc.encoded_bytes, stream));
-    int serial = 0;
-    GC_TRY(insert_serial_needed(h_meta, n, serial));
-    GC_HIP(launch_insert(mrpd, cells, order, g.nslots, n, nne, d_asked, d_geom, serial, stream));
+    if (g.nb_tables) {
+        GC_HIP(launch_deref_tables(cells, tabs, order, g.nslots, nne, meta, d_geom, ts, stream));
+        GC_HIP(launch_assign_copy(order, g.nslots, nne, d_asked, d_bbi, d_bbc, meta, pool, d_staging, d_geom,
+                                  vc.big_brick_size, vc.encoded_bytes, stream));
+        GC_HIP(launch_reference_tables(mrpd, cells, tabs, order, g.nslots, n, nne, d_asked, meta, d_geom, ts, d_unref,
+                                       stream));
+    } else {
+        GC_HIP(launch_deref(mrpd, cells, order, g.nslots, nne, meta, d_geom, stream));
+        GC_HIP(launch_assign_copy(order, g.nslots, nne, d_asked, d_bbi, d_bbc, meta, pool, d_staging, d_geom,
+                                  vc.big_brick_size, vc.encoded_bytes, stream));
+        int serial = 0;
+        GC_TRY(insert_serial_needed(h_meta, n, serial));
+        GC_HIP(launch_insert(mrpd, cells, order, g.nslots, n, nne, d_asked, d_geom, serial, stream));
+    }
     GC_HIP(hipEventRecord(consumed, stream));
     return GC_OK;
 }
@@ -1205,6 +1310,18 @@ static void launch_base(const Cache &C, DvrLaunch &L)
         for (int a = 0; a < 3; ++a) rdv_one = rdv_one && C.g.rdv[l][a] == 1.0f;
     L.pipelined = rdv_one && C.g.mrpd_dim[0] < (1u << 24) &&
                   (uint64_t)C.g.mrpd_dim[1] * C.g.mrpd_dim[2] < (1u << 24);
+    // table caches: the walk through them (the non-pipelined march)
+    L.pt = PageTables{};
+    L.pt.n = C.g.nb_tables;
+    L.pt.top_x = C.g.top_cells_dim[0];
+    L.pt.top_y = C.g.top_cells_dim[1];
+    for (uint32_t i = 0; i < C.g.nb_tables; ++i) {
+        for (int a = 0; a < 3; ++a) L.pt.block[i][a] = C.g.table[i].block[a];
+        L.pt.b3[i] = table_b3(C.g.table[i]);
+        L.pt.words[i] = C.tabs.words[i];
+        L.pt.usage[i] = C.tabs.usage[i];
+    }
+    if (C.g.nb_tables) L.pipelined = 0;
 }
 
 int cache_launch_base(gc_cache *c, DvrLaunch &L, hipStream_t &stream, int &device, uint32_t &bytes);
@@ -1903,12 +2020,20 @@ int gc_cache_load_all(gc_cache *c, const void *const *d_levels)
     GC_HIP(hipMemcpyAsync(d_asked, asked.data(), asked.size() * 4, hipMemcpyHostToDevice, C.stream));
     GC_HIP(hipMemcpyAsync(d_dims, C.vc.initial_sizes, (size_t)g.nb_levels * 12, hipMemcpyHostToDevice, C.stream));
     GC_HIP(hipMemcpyAsync(d_lv, d_levels, (size_t)g.nb_levels * sizeof(void *), hipMemcpyHostToDevice, C.stream));
-    GC_HIP(launch_deref(C.mrpd, C.cells, C.order, g.nslots, n, C.meta, C.d_geom, C.stream));
-    GC_HIP(launch_assign_cut(C.order, g.nslots, n, d_asked, C.meta, C.pool, d_lv, d_dims, C.d_geom,
-                             C.vc.encoded_bytes, C.stream));
-    int serial = 0;
-    GC_TRY(C.insert_serial_needed(asked.data(), n, serial));
-    GC_HIP(launch_insert(C.mrpd, C.cells, C.order, g.nslots, n, n, d_asked, C.d_geom, serial, C.stream));
+    if (g.nb_tables) {
+        GC_HIP(launch_deref_tables(C.cells, C.tabs, C.order, g.nslots, n, C.meta, C.d_geom, C.ts, C.stream));
+        GC_HIP(launch_assign_cut(C.order, g.nslots, n, d_asked, C.meta, C.pool, d_lv, d_dims, C.d_geom,
+                                 C.vc.encoded_bytes, C.stream));
+        GC_HIP(launch_reference_tables(C.mrpd, C.cells, C.tabs, C.order, g.nslots, n, n, d_asked, C.meta, C.d_geom,
+                                       C.ts, C.d_unref, C.stream));
+    } else {
+        GC_HIP(launch_deref(C.mrpd, C.cells, C.order, g.nslots, n, C.meta, C.d_geom, C.stream));
+        GC_HIP(launch_assign_cut(C.order, g.nslots, n, d_asked, C.meta, C.pool, d_lv, d_dims, C.d_geom,
+                                 C.vc.encoded_bytes, C.stream));
+        int serial = 0;
+        GC_TRY(C.insert_serial_needed(asked.data(), n, serial));
+        GC_HIP(launch_insert(C.mrpd, C.cells, C.order, g.nslots, n, n, d_asked, C.d_geom, serial, C.stream));
+    }
     GC_HIP(hipStreamSynchronize(C.stream));
     hipFree(d_asked);
     hipFree(d_dims);
@@ -1952,6 +2077,11 @@ int gc_cache_view_get(gc_cache *c, gc_cache_view *out)
     out->brick_size = C.g.brick_size;
     out->covering = C.g.covering;
     out->encoded_bytes = C.g.encoded_bytes;
+    out->nb_tables = C.g.nb_tables;
+    for (int i = 0; i < GC_MAX_TABLES; ++i) {
+        out->table_words[i] = C.tabs.words[i];
+        out->table_usage[i] = C.tabs.usage[i];
+    }
     return GC_OK;
 }
 
@@ -1987,13 +2117,13 @@ static int h2d_sync(Cache &C, void *dst, const void *src, size_t bytes)
 int gc_cache_read_mrpd(gc_cache *c, uint32_t *host)
 {
     if (!c || !host) return fail(GC_ERR_ARG, "null argument");
-    return d2h_sync(c->impl, host, c->impl.mrpd, c->impl.g.mrpd_cells * 16);
+    return d2h_sync(c->impl, host, c->impl.mrpd, c->impl.g.top_cells * 16);
 }
 
 int gc_cache_read_cells(gc_cache *c, uint32_t *host)
 {
     if (!c || !host) return fail(GC_ERR_ARG, "null argument");
-    return d2h_sync(c->impl, host, c->impl.cells, c->impl.g.mrpd_cells * 4);
+    return d2h_sync(c->impl, host, c->impl.cells, c->impl.g.top_cells * 4);
 }
 int gc_cache_read_request(gc_cache *c, uint32_t *host)
 {
@@ -2030,6 +2160,80 @@ int gc_cache_read_lru(gc_cache *c, gc_lru_entry *host)
         std::memcpy(host[i].vol_pos, mt[ord[i]].vol, sizeof(host[i].vol_pos));
     }
     return GC_OK;
+}
+
+static int table_arg(gc_cache *c, uint32_t i, const void *host)
+{
+    if (!c || !host) return fail(GC_ERR_ARG, "null argument");
+    if (i >= c->impl.g.nb_tables) return fail(GC_ERR_ARG, "no such page-table cache");
+    return GC_OK;
+}
+
+int gc_cache_read_table(gc_cache *c, uint32_t i, uint32_t *host4)
+{
+    GC_TRY(table_arg(c, i, host4));
+    Cache &C = c->impl;
+    const gc_table_geometry &t = C.g.table[i];
+    const uint32_t b3 = table_b3(t);
+    std::vector<uint32_t> w((size_t)t.nblocks * b3);
+    GC_TRY(d2h_sync(C, w.data(), C.tabs.words[i], w.size() * 4));
+    // the reference's surface: entry (x, y, z) of block (x, y, z) / block;
+    // a word names a block of the next table or a brick slot
+    const bool last = i + 1 == C.g.nb_tables;
+    for (uint32_t b = 0; b < t.nblocks; ++b) {
+        uint32_t o[3];
+        block_origin(t, b, o);
+        for (uint32_t k = 0; k < b3; ++k) {
+            const uint32_t lx = k % t.block[0], ly = (k / t.block[0]) % t.block[1], lz = k / (t.block[0] * t.block[1]);
+            const uint64_t e = (o[0] + lx) + (uint64_t)t.surf[0] * ((o[1] + ly) + (uint64_t)t.surf[1] * (o[2] + lz));
+            const uint32_t word = w[(size_t)b * b3 + k], st = word & 3u;
+            uint32_t co[3] = {0, 0, 0};
+            if (st == 1) {
+                if (last) slot_origin(C.g, word >> 2, co);
+                else block_origin(C.g.table[i + 1], word >> 2, co);
+            }
+            host4[4 * e] = co[0];
+            host4[4 * e + 1] = co[1];
+            host4[4 * e + 2] = co[2];
+            host4[4 * e + 3] = st;
+        }
+    }
+    return GC_OK;
+}
+
+int gc_cache_read_table_usage(gc_cache *c, uint32_t i, uint32_t *host)
+{
+    GC_TRY(table_arg(c, i, host));
+    return d2h_sync(c->impl, host, c->impl.tabs.usage[i], (size_t)c->impl.g.table[i].nblocks * 4);
+}
+
+int gc_cache_write_table_usage(gc_cache *c, uint32_t i, const uint32_t *host)
+{
+    GC_TRY(table_arg(c, i, host));
+    return h2d_sync(c->impl, c->impl.tabs.usage[i], host, (size_t)c->impl.g.table[i].nblocks * 4);
+}
+
+int gc_cache_read_table_lru(gc_cache *c, uint32_t i, gc_lru_entry *host)
+{
+    GC_TRY(table_arg(c, i, host));
+    Cache &C = c->impl;
+    const gc_table_geometry &t = C.g.table[i];
+    std::vector<uint32_t> ord(t.nblocks);
+    std::vector<SlotMeta> mt(t.nblocks);
+    GC_TRY(d2h_sync(C, ord.data(), C.tabs.order[i], ord.size() * 4));
+    GC_TRY(d2h_sync(C, mt.data(), C.tabs.meta[i], mt.size() * sizeof(SlotMeta)));
+    for (uint32_t k = 0; k < t.nblocks; ++k) {
+        block_origin(t, ord[k], host[k].cache_pos);
+        host[k].level = mt[ord[k]].level;
+        std::memcpy(host[k].vol_pos, mt[ord[k]].vol, sizeof(host[k].vol_pos));
+    }
+    return GC_OK;
+}
+
+int gc_cache_unreferenced(gc_cache *c, uint32_t *out)
+{
+    if (!c || !out) return fail(GC_ERR_ARG, "null argument");
+    return d2h_sync(c->impl, out, c->impl.d_unref, 4);
 }
 
 int gc_cache_read_pool(gc_cache *c, void *host)

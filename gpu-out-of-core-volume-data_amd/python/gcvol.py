@@ -12,6 +12,7 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libgcvol.so")
 MAXL = 16
+MAXT = 3
 
 u32 = C.c_uint32
 u64 = C.c_uint64
@@ -44,6 +45,7 @@ class CacheCfg(C.Structure):
     _fields_ = [
         ("cache_size", u32 * 3), ("block_size", u32 * 3), ("data_cache_flags", u32),
         ("max_requests", u32), ("sync_requests", u32), ("empty_threshold", f32),
+        ("nb_tables", u32), ("table_cache_size", (u32 * 3) * MAXT), ("table_block_size", (u32 * 3) * MAXT),
     ]
 
 
@@ -54,12 +56,21 @@ class SynthParams(C.Structure):
     ]
 
 
+class TableGeometry(C.Structure):
+    _fields_ = [
+        ("block", u32 * 3), ("blocks", u32 * 3), ("surf", u32 * 3), ("nblocks", u32),
+        ("virt", (u32 * 3) * MAXL), ("real", (u32 * 3) * MAXL),
+    ]
+
+
 class Geometry(C.Structure):
     _fields_ = [
         ("nb_levels", u32), ("brick_size", u32), ("covering", u32), ("inner", u32), ("encoded_bytes", u32),
         ("level_dim", (u32 * 3) * MAXL), ("level_coord", u32 * MAXL), ("virt_vox", (u32 * 3) * MAXL),
         ("ior", (f32 * 3) * MAXL), ("rdv", (f32 * 3) * MAXL), ("one_over", f32 * 3),
         ("mrpd_dim", u32 * 3), ("cache_size", u32 * 3), ("nslots", u32), ("pad", u32), ("mrpd_cells", u64),
+        ("nb_tables", u32), ("top_dim", (u32 * 3) * MAXL), ("top_coord", u32 * MAXL), ("top_cells_dim", u32 * 3),
+        ("top_cells", u64), ("table", TableGeometry * MAXT),
     ]
 
 
@@ -107,6 +118,7 @@ class CacheView(C.Structure):
     _fields_ = [
         ("mrpd", vp), ("request", vp), ("usage", vp), ("pool", vp), ("geometry", vp),
         ("timestamp", u32), ("brick_size", u32), ("covering", u32), ("encoded_bytes", u32),
+        ("nb_tables", u32), ("table_words", vp * MAXT), ("table_usage", vp * MAXT),
     ]
 
 
@@ -140,7 +152,8 @@ EXPORTS = [
     "gc_source_create_callback", "gc_source_destroy", "gc_source_get", "gc_synth_volume",
     "gc_cache_create", "gc_cache_destroy", "gc_cache_update", "gc_cache_completude", "gc_cache_load_all",
     "gc_cache_timestamp", "gc_cache_stream", "gc_cache_view_get", "gc_cache_geometry", "gc_geometry_compute",
-    "gc_cache_read_mrpd", "gc_cache_read_cells",
+    "gc_cache_read_mrpd", "gc_cache_read_cells", "gc_cache_read_table", "gc_cache_read_table_usage",
+    "gc_cache_write_table_usage", "gc_cache_read_table_lru", "gc_cache_unreferenced",
     "gc_cache_read_request", "gc_cache_write_request", "gc_cache_read_usage", "gc_cache_write_usage",
     "gc_cache_read_lru", "gc_cache_read_pool", "gc_cache_read_asked", "gc_frame_create", "gc_frame_destroy",
     "gc_frame_reset", "gc_frame_read_pixels", "gc_frame_read_state", "gc_frame_buffers", "gc_render_dvr",
@@ -200,7 +213,9 @@ def lib():
             "gc_cache_counters_get": [P, C.POINTER(CacheCounters)],
             "gc_cache_view_get": [P, C.POINTER(CacheView)], "gc_cache_geometry": [P, C.POINTER(Geometry)],
             "gc_geometry_compute": [C.POINTER(VolumeCfg), C.POINTER(CacheCfg), C.POINTER(Geometry)],
-            "gc_cache_read_mrpd": [P, P], "gc_cache_read_cells": [P, P], "gc_cache_read_request": [P, P], "gc_cache_write_request": [P, P],
+            "gc_cache_read_mrpd": [P, P], "gc_cache_read_cells": [P, P], "gc_cache_read_table": [P, u32, P],
+            "gc_cache_read_table_usage": [P, u32, P], "gc_cache_write_table_usage": [P, u32, P],
+            "gc_cache_read_table_lru": [P, u32, P], "gc_cache_unreferenced": [P, P], "gc_cache_read_request": [P, P], "gc_cache_write_request": [P, P],
             "gc_cache_read_usage": [P, P], "gc_cache_write_usage": [P, P], "gc_cache_read_lru": [P, P],
             "gc_cache_read_pool": [P, P], "gc_cache_read_asked": [P, P, C.POINTER(u32), C.POINTER(u32)],
             "gc_frame_create": [u32, u32, u32, C.POINTER(vp)], "gc_frame_destroy": [P],
@@ -290,7 +305,9 @@ def volume_cfg_load(cfg_path):
     return vc, buf.value.decode()
 
 
-def cache_cfg(cache_size, brick_size, max_requests=50, sync=True, empty_threshold=0.0):
+def cache_cfg(cache_size, brick_size, max_requests=50, sync=True, empty_threshold=0.0, tables=()):
+    """tables: page-table caches as (CacheSize[k+1], BlockSize[k+1]) pairs in
+    CacheConfiguration order (k = 0 next to the brick pool)."""
     cc = CacheCfg()
     cc.cache_size[:] = list(cache_size)
     cc.block_size[:] = [brick_size] * 3
@@ -298,6 +315,10 @@ def cache_cfg(cache_size, brick_size, max_requests=50, sync=True, empty_threshol
     cc.max_requests = max_requests
     cc.sync_requests = 1 if sync else 0
     cc.empty_threshold = empty_threshold
+    cc.nb_tables = len(tables)
+    for k, (cs, bl) in enumerate(tables):
+        cc.table_cache_size[k][:] = [int(v) for v in cs]
+        cc.table_block_size[k][:] = [int(v) for v in bl]
     return cc
 
 
@@ -486,6 +507,8 @@ class Cache:
         self.g = Geometry()
         check(lib().gc_cache_geometry(self.h, C.byref(self.g)))
         self.cells = int(self.g.mrpd_cells)
+        self.top_cells = int(self.g.top_cells)
+        self.nb_tables = int(self.g.nb_tables)
         self.nslots = int(self.g.nslots)
 
     def close(self):
@@ -530,14 +553,41 @@ class Cache:
         check(lib().gc_cache_load_all(self.h, arr))
 
     def mrpd(self):
-        out = np.zeros((self.cells, 4), np.uint32)
+        out = np.zeros((self.top_cells, 4), np.uint32)
         check(lib().gc_cache_read_mrpd(self.h, ptr(out)))
         return out
 
     def cell_table(self):
-        out = np.zeros(self.cells, np.uint32)
+        out = np.zeros(self.top_cells, np.uint32)
         check(lib().gc_cache_read_cells(self.h, ptr(out)))
         return out
+
+    def table(self, i):
+        """Entries of page-table cache i (i = 0 next to the MRPD) in the
+        reference's surface layout: (entries, 4)."""
+        t = self.g.table[i]
+        out = np.zeros((int(t.surf[0]) * int(t.surf[1]) * int(t.surf[2]), 4), np.uint32)
+        check(lib().gc_cache_read_table(self.h, i, ptr(out)))
+        return out
+
+    def table_usage(self, i):
+        out = np.zeros(int(self.g.table[i].nblocks), np.uint32)
+        check(lib().gc_cache_read_table_usage(self.h, i, ptr(out)))
+        return out
+
+    def write_table_usage(self, i, a):
+        a = np.ascontiguousarray(a, np.uint32)
+        check(lib().gc_cache_write_table_usage(self.h, i, ptr(a)))
+
+    def table_lru(self, i):
+        out = np.zeros((int(self.g.table[i].nblocks), 7), np.uint32)
+        check(lib().gc_cache_read_table_lru(self.h, i, ptr(out)))
+        return out
+
+    def unreferenced(self):
+        n = u32(0)
+        check(lib().gc_cache_unreferenced(self.h, C.byref(n)))
+        return int(n.value)
 
     def request(self):
         out = np.zeros(self.cells, np.uint32)

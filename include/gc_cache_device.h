@@ -33,7 +33,7 @@ typedef enum gc_voxel_status {
 } gc_voxel_status;
 
 typedef struct gc_kcache {
-    const uint32_t *mrpd;        /* uint4 entries (x, y, z, w), mrpd_dim, x-fastest */
+    const uint32_t *mrpd;        /* uint4 entries (x, y, z, w), top_cells_dim, x-fastest */
     uint32_t *request;
     uint32_t *usage;
     const void *pool;            /* brick-major: slot s holds bs^3 voxels          */
@@ -41,6 +41,9 @@ typedef struct gc_kcache {
     uint32_t timestamp;
     uint32_t prev_coords[3];     /* one-entry page cache of the calling thread     */
     uint32_t prev_entry[4];
+    uint32_t nb_tables;          /* page-table caches (gc_cache_view)              */
+    const uint32_t *table_words[GC_MAX_TABLES];
+    uint32_t *table_usage[GC_MAX_TABLES];
 } gc_kcache;
 
 /* Kernel object of a cache view (valid until the next update or destroy). */
@@ -55,6 +58,11 @@ static inline gc_kcache gc_kcache_from_view(const gc_cache_view *v)
     c.timestamp = v->timestamp;
     c.prev_coords[0] = c.prev_coords[1] = c.prev_coords[2] = 0xffffffffu;
     c.prev_entry[0] = c.prev_entry[1] = c.prev_entry[2] = c.prev_entry[3] = 0;
+    c.nb_tables = v->nb_tables;
+    for (int i = 0; i < GC_MAX_TABLES; ++i) {
+        c.table_words[i] = v->table_words[i];
+        c.table_usage[i] = v->table_usage[i];
+    }
     return c;
 }
 
@@ -70,7 +78,8 @@ __device__ inline void positions(const gc_geometry &g, uint32_t level, const flo
     virt = make_float3(real.x * g.rdv[level][0], real.y * g.rdv[level][1], real.z * g.rdv[level][2]);
 }
 
-/* K_MultiResolutionPageDirectory::get_coords (K_MultiResolutionPageDirectory.hpp:168-180) */
+/* K_MultiResolutionPageDirectory::get_coords (K_MultiResolutionPageDirectory.hpp:168-180) over the
+ * brick grid: the request buffer's (raise_brick_request, K_CacheManager.hpp:365-396) */
 __device__ inline uint3 page_coords(const gc_geometry &g, uint32_t level, const float3 &q)
 {
     return make_uint3(g.level_coord[level] + (uint32_t)(q.x * (float)g.level_dim[level][0]),
@@ -78,14 +87,62 @@ __device__ inline uint3 page_coords(const gc_geometry &g, uint32_t level, const 
                       (uint32_t)(q.z * (float)g.level_dim[level][2]));
 }
 
+/* request-buffer cell of brick-grid coordinates */
 __device__ inline uint64_t cell(const gc_geometry &g, const uint3 &c)
 {
     return c.x + (uint64_t)g.mrpd_dim[0] * (c.y + (uint64_t)g.mrpd_dim[1] * c.z);
 }
 
+/* MRPD coordinates (the top of the hierarchy: the brick grid without
+ * page-table caches) */
+__device__ inline uint3 top_coords(const gc_geometry &g, uint32_t level, const float3 &q)
+{
+    return make_uint3(g.top_coord[level] + (uint32_t)(q.x * (float)g.top_dim[level][0]),
+                      (uint32_t)(q.y * (float)g.top_dim[level][1]),
+                      (uint32_t)(q.z * (float)g.top_dim[level][2]));
+}
+
 __device__ inline uint4 entry_at(const gc_kcache &k, const uint3 &c)
 {
-    return reinterpret_cast<const uint4 *>(k.mrpd)[cell(*k.geom, c)];
+    const gc_geometry &g = *k.geom;
+    return reinterpret_cast<const uint4 *>(k.mrpd)[c.x + (uint64_t)g.top_cells_dim[0] *
+                                                             (c.y + (uint64_t)g.top_cells_dim[1] * c.z)];
+}
+
+/* From a Mapped MRPD entry through the page-table caches
+ * (K_CacheManager.hpp:206-231,310-339): each table read stamps its block's
+ * usage (compute_element_position, K_CacheDevice.hpp:316-327) and yields the
+ * next entry, (origin of the block or brick slot it maps, w); the walk ends
+ * at the first entry that is not Mapped. */
+__device__ inline uint4 walk_tables(const gc_kcache &k, uint4 e, uint32_t level, const float3 &virt)
+{
+    const gc_geometry &g = *k.geom;
+    for (uint32_t i = 0; i < k.nb_tables && e.w == 1; ++i) {
+        const gc_table_geometry &t = g.table[i];
+        const uint32_t b = e.x / t.block[0] + t.blocks[0] * (e.y / t.block[1] + t.blocks[1] * (e.z / t.block[2]));
+        k.table_usage[i][b] = k.timestamp;
+        const uint32_t lx = (uint32_t)(virt.x * (float)t.virt[level][0]) % t.block[0];
+        const uint32_t ly = (uint32_t)(virt.y * (float)t.virt[level][1]) % t.block[1];
+        const uint32_t lz = (uint32_t)(virt.z * (float)t.virt[level][2]) % t.block[2];
+        const uint32_t w = k.table_words[i][(uint64_t)b * (t.block[0] * t.block[1] * t.block[2]) + lx +
+                                            t.block[0] * (ly + t.block[1] * lz)];
+        const uint32_t c = w >> 2;
+        uint32_t o[3] = {0, 0, 0};
+        if ((w & 3u) == 1) {
+            if (i + 1 < k.nb_tables) {
+                const gc_table_geometry &n = g.table[i + 1];
+                o[0] = (c % n.blocks[0]) * n.block[0];
+                o[1] = ((c / n.blocks[0]) % n.blocks[1]) * n.block[1];
+                o[2] = (c / (n.blocks[0] * n.blocks[1])) * n.block[2];
+            } else {
+                o[0] = (c % g.cache_size[0]) * g.brick_size;
+                o[1] = ((c / g.cache_size[0]) % g.cache_size[1]) * g.brick_size;
+                o[2] = (c / (g.cache_size[0] * g.cache_size[1])) * g.brick_size;
+            }
+        }
+        e = make_uint4(o[0], o[1], o[2], w & 3u);
+    }
+    return e;
 }
 
 /* raise_brick_request (K_CacheManager.hpp:365-396) */
@@ -143,13 +200,17 @@ __device__ inline gc_voxel_status gc_get_normalized(gc_kcache &k, uint32_t level
     using namespace gc_device;
     float3 real, virt;
     positions(*k.geom, level, p, real, virt);
-    const uint3 c = page_coords(*k.geom, level, virt);
+    const uint3 c = top_coords(*k.geom, level, virt);
     if (c.x != k.prev_coords[0] || c.y != k.prev_coords[1] || c.z != k.prev_coords[2]) {
         const uint4 e = entry_at(k, c);
         k.prev_entry[0] = e.x; k.prev_entry[1] = e.y; k.prev_entry[2] = e.z; k.prev_entry[3] = e.w;
         k.prev_coords[0] = c.x; k.prev_coords[1] = c.y; k.prev_coords[2] = c.z;
     }
-    const uint4 e = make_uint4(k.prev_entry[0], k.prev_entry[1], k.prev_entry[2], k.prev_entry[3]);
+    /* an Unmapped table entry is Unmapped (the reference, with a coarser
+     * level pending, repeats the same lookup for ever there,
+     * K_CacheManager.hpp:318-330) */
+    const uint4 e = walk_tables(k, make_uint4(k.prev_entry[0], k.prev_entry[1], k.prev_entry[2], k.prev_entry[3]),
+                                level, virt);
     if (e.w == 1) {
         out = box<T>(k, element(k, e, level, virt));
         return GC_VOXEL_MAPPED;
@@ -170,7 +231,10 @@ __device__ inline gc_voxel_status gc_get(const gc_kcache &k, uint32_t level, con
     for (;;) {
         float3 real, virt;
         positions(*k.geom, level, p, real, virt);
-        const uint4 e = entry_at(k, page_coords(*k.geom, level, virt));
+        /* an Unmapped entry at any level of the walk falls back to the
+         * coarser level (the reference does so only for the MRPD's; after a
+         * table's it repeats the lookup for ever, K_CacheManager.hpp:212-222) */
+        const uint4 e = walk_tables(k, entry_at(k, top_coords(*k.geom, level, virt)), level, virt);
         if (e.w == 1) {
             const uint3 el = element(k, e, level, virt);
             out = texel<T>(k, (int)el.x, (int)el.y, (int)el.z);

@@ -34,6 +34,7 @@ extern "C" {
 #define GC_ERR_NODEV 7     /* no usable GPU                                    */
 
 #define GC_MAX_LEVELS 16
+#define GC_MAX_TABLES 3   /* page-table caches between the MRPD and the pool */
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *gc_last_error(void);
@@ -65,6 +66,13 @@ typedef struct gc_cache_cfg {
     uint32_t max_requests;       /* bricks per update; 50 in the reference     */
     uint32_t sync_requests;      /* 1: requests are served inside update()    */
     float    empty_threshold;    /* a big brick is Empty if every voxel < this */
+    /* CacheSize[1..] and BlockSize[1..]: the page-table caches between the
+     * MRPD and the brick pool (CacheManager.hpp:151-236).  Table k is
+     * CacheSize[k + 1] blocks of BlockSize[k + 1] entries (k = 0 next to the
+     * pool).  nb_tables = 0: the MRPD maps bricks directly. */
+    uint32_t nb_tables;
+    uint32_t table_cache_size[GC_MAX_TABLES][3];
+    uint32_t table_block_size[GC_MAX_TABLES][3];
 } gc_cache_cfg;
 
 /* Fill a gc_volume_cfg from the preprocessing rules: level count and sizes
@@ -296,23 +304,53 @@ typedef struct gc_cache_view {
     const void *geometry;        /* device copy of gc_geometry                 */
     uint32_t    timestamp;
     uint32_t    brick_size, covering, encoded_bytes;
+    /* page-table caches (K_CacheDevice<uint4>, geometry table[i]): entries as
+     * words child << 2 | w, block-major (entry k of block b at b * entries
+     * per block + k, k x-fastest inside the block), the child a block of
+     * table i + 1 or, for the last table, a brick slot; usage per block */
+    uint32_t    nb_tables;
+    const uint32_t *table_words[GC_MAX_TABLES];
+    uint32_t   *table_usage[GC_MAX_TABLES];
 } gc_cache_view;
 int gc_cache_view_get(gc_cache *c, gc_cache_view *out);
+
+/* One page-table cache (CacheDevice<uint4>, CacheDevice.hpp:123-180):
+ * geometry table[i] is the reference's _tableCaches[i]
+ * (CacheManager.hpp:223-233), i = 0 next to the MRPD, which is configuration
+ * table nb_tables - 1 - i. */
+typedef struct gc_table_geometry {
+    uint32_t block[3];                      /* BlockSize: entries per block    */
+    uint32_t blocks[3];                     /* CacheSize: blocks per axis      */
+    uint32_t surf[3];                       /* entries per axis                */
+    uint32_t nblocks;
+    uint32_t virt[GC_MAX_LEVELS][3];        /* virtualizedNumberOfEntries      */
+    uint32_t real[GC_MAX_LEVELS][3];        /* realNumberOfEntries             */
+} gc_table_geometry;
 
 /* Derived geometry (CacheManager.hpp:157-285). */
 typedef struct gc_geometry {
     uint32_t nb_levels, brick_size, covering, inner, encoded_bytes;
-    uint32_t level_dim[GC_MAX_LEVELS][3];   /* bricks per axis (MRPD)          */
-    uint32_t level_coord[GC_MAX_LEVELS];    /* MRPD x offset of each level     */
+    uint32_t level_dim[GC_MAX_LEVELS][3];   /* bricks per axis                 */
+    uint32_t level_coord[GC_MAX_LEVELS];    /* brick-grid x offset of a level  */
     uint32_t virt_vox[GC_MAX_LEVELS][3];    /* virtualised voxels per level    */
     float    ior[GC_MAX_LEVELS][3];         /* initialOverReal                 */
     float    rdv[GC_MAX_LEVELS][3];         /* real / virtualised              */
     float    one_over[3];                   /* 1/(bs-2c)                       */
-    uint32_t mrpd_dim[3];                   /* MRPD and request buffer shape   */
+    uint32_t mrpd_dim[3];                   /* brick grid: the request buffer's
+                                               shape, and the MRPD's when
+                                               nb_tables = 0                   */
     uint32_t cache_size[3];                 /* slots per axis                  */
     uint32_t nslots;
     uint32_t pad;
-    uint64_t mrpd_cells;
+    uint64_t mrpd_cells;                    /* brick-grid cells                */
+    /* the MRPD (realNumberOfEntries.back()): the brick grid when
+     * nb_tables = 0, else the blocks of table 0 */
+    uint32_t nb_tables;
+    uint32_t top_dim[GC_MAX_LEVELS][3];
+    uint32_t top_coord[GC_MAX_LEVELS];
+    uint32_t top_cells_dim[3];
+    uint64_t top_cells;
+    gc_table_geometry table[GC_MAX_TABLES];
 } gc_geometry;
 int gc_cache_geometry(gc_cache *c, gc_geometry *out);
 /* The same geometry from the configurations alone (host only, no device). */
@@ -325,10 +363,23 @@ typedef struct gc_lru_entry {
     uint32_t level;
     float    vol_pos[3];
 } gc_lru_entry;
-int gc_cache_read_mrpd(gc_cache *c, uint32_t *host);     /* mrpd_cells x 4  */
+int gc_cache_read_mrpd(gc_cache *c, uint32_t *host);     /* top_cells x 4   */
 /* The ray caster's compact copy of the page directory: one word per cell,
- * slot << 2 | w (no reference counterpart; kept equal to the MRPD). */
-int gc_cache_read_cells(gc_cache *c, uint32_t *host);    /* mrpd_cells      */
+ * slot << 2 | w, the slot a table-0 block with table caches (no reference
+ * counterpart; kept equal to the MRPD). */
+int gc_cache_read_cells(gc_cache *c, uint32_t *host);    /* top_cells       */
+/* Page-table cache i (geometry table[i], i = 0 next to the MRPD): its
+ * entries in the reference's surface layout (uint32 x4 per entry, surf
+ * entries x-fastest: the origin of the block or brick slot they map, w), its
+ * per-block usage stamps and its LRU (nblocks records, LRU order; a block's
+ * level and position are those of the brick it was taken for). */
+int gc_cache_read_table(gc_cache *c, uint32_t i, uint32_t *host4);
+int gc_cache_read_table_usage(gc_cache *c, uint32_t i, uint32_t *host);
+int gc_cache_write_table_usage(gc_cache *c, uint32_t i, const uint32_t *host);
+int gc_cache_read_table_lru(gc_cache *c, uint32_t i, gc_lru_entry *host);
+/* Bricks the last load could not reference because a page-table cache had
+ * no block left for it in that load (they are requested again). */
+int gc_cache_unreferenced(gc_cache *c, uint32_t *out);
 int gc_cache_read_request(gc_cache *c, uint32_t *host);  /* mrpd_cells      */
 int gc_cache_write_request(gc_cache *c, const uint32_t *host);
 int gc_cache_read_usage(gc_cache *c, uint32_t *host);    /* nslots          */

@@ -77,37 +77,74 @@ void or_pool_shape(uint64_t budget_bytes, uint64_t brick_bytes, uint32_t out[3])
 
 void or_geometry(const or_cfg *cfg, or_geom *g)
 {
-    /* CacheManager.hpp:157-209 (one pagination level: MRPD + data cache),
-     * MultiResolutionPageDirectory.hpp:99-124, CacheDevice.hpp:171-180,
-     * CacheManager.hpp:268-285. */
+    /* CacheManager.hpp:157-209 (the hierarchy: realNumberOfEntries bottom-up,
+     * virtualizedNumberOfEntries top-down), MultiResolutionPageDirectory.hpp
+     * :99-124, CacheDevice.hpp:123-180, CacheManager.hpp:223-285. */
     memset(g, 0, sizeof(*g));
-    const uint32_t L = cfg->nb_levels;
+    const uint32_t L = cfg->nb_levels, T = cfg->nb_tables;
+    const uint32_t H = T + 2;   /* nbLevelPTHierarchy = CacheSize.size() + 1 */
     const double inner = (double)cfg->brick_size - 2.0 * (double)cfg->covering;
+    /* real[h][l][a], virt[h][l][a] for hierarchy index h (0 = voxels) */
+    uint32_t real[OR_MAX_TABLES + 2][OR_MAX_LEVELS][3], virt[OR_MAX_TABLES + 2][OR_MAX_LEVELS][3];
+    for (uint32_t l = 0; l < L; ++l)
+        for (int a = 0; a < 3; ++a) {
+            real[0][l][a] = cfg->real[l][a];
+            /* std::ceil(float(real0) / (bs - 2.0*c)) in double */
+            real[1][l][a] = (uint32_t)ceil((double)(float)cfg->real[l][a] / inner);
+            /* std::ceil(float(real[h-1]) / BlockSize[h-1]) in float */
+            for (uint32_t h = 2; h < H; ++h)
+                real[h][l][a] = (uint32_t)ceilf((float)real[h - 1][l][a] / (float)cfg->table_block[h - 2][a]);
+            virt[H - 1][l][a] = real[H - 1][l][a];
+            /* std::ceil(float(virt[h+1]) * BlockSize[h]) in float */
+            for (uint32_t h = H - 2; h >= 1; --h)
+                virt[h][l][a] = (uint32_t)ceilf((float)virt[h + 1][l][a] * (float)cfg->table_block[h - 1][a]);
+            /* virtualized[0] = ceil(float(virtualized[1]) * (bs - 2.0*c)) */
+            virt[0][l][a] = (uint32_t)ceil((double)(float)virt[1][l][a] * inner);
+        }
     for (uint32_t l = 0; l < L; ++l) {
         for (int a = 0; a < 3; ++a) {
-            /* std::ceil(float(real0) / (bs - 2.0*c)) in double */
-            const uint32_t r1 = (uint32_t)ceil((double)(float)cfg->real[l][a] / inner);
-            g->level_dim[l][a] = r1;
-            /* virtualized[0] = ceil(float(virtualized[1]) * (bs - 2.0*c)) */
-            g->virt_vox[l][a] = (uint32_t)ceil((double)(float)r1 * inner);
+            g->level_dim[l][a] = real[1][l][a];
+            g->virt_vox[l][a] = virt[0][l][a];
             g->rdv[l][a] = (float)cfg->real[l][a] / (float)g->virt_vox[l][a];
             g->ior[l][a] = (float)cfg->initial[l][a] / (float)cfg->real[l][a];
+            g->top_dim[l][a] = real[H - 1][l][a];
         }
     }
     g->level_coord[0] = 0;
-    for (uint32_t l = 1; l < L; ++l)
+    g->top_coord[0] = 0;
+    for (uint32_t l = 1; l < L; ++l) {
         g->level_coord[l] = g->level_coord[l - 1] + g->level_dim[l - 1][0];
-    g->mrpd_dim[0] = g->mrpd_dim[1] = g->mrpd_dim[2] = 0;
+        g->top_coord[l] = g->top_coord[l - 1] + g->top_dim[l - 1][0];
+    }
     for (uint32_t l = 0; l < L; ++l) {
         g->mrpd_dim[0] += g->level_dim[l][0];
         g->mrpd_dim[1] = MAX(g->mrpd_dim[1], g->level_dim[l][1]);
         g->mrpd_dim[2] = MAX(g->mrpd_dim[2], g->level_dim[l][2]);
+        g->top_cells_dim[0] += g->top_dim[l][0];
+        g->top_cells_dim[1] = MAX(g->top_cells_dim[1], g->top_dim[l][1]);
+        g->top_cells_dim[2] = MAX(g->top_cells_dim[2], g->top_dim[l][2]);
     }
     for (int a = 0; a < 3; ++a) {
         g->one_over[a] = 1.f / (float)(cfg->brick_size - 2 * cfg->covering);
         g->pool_dim[a] = cfg->cache_size[a] * cfg->brick_size;
     }
     g->nslots = cfg->cache_size[0] * cfg->cache_size[1] * cfg->cache_size[2];
+    /* _tableCaches[i] = hierarchy index H - 2 - i (CacheManager.hpp:223-233) */
+    g->nb_tables = T;
+    for (uint32_t i = 0; i < T; ++i) {
+        const uint32_t h = H - 2 - i;
+        or_table_geom *t = &g->table[i];
+        for (int a = 0; a < 3; ++a) {
+            t->block[a] = cfg->table_block[h - 1][a];
+            t->blocks[a] = cfg->table_cache[h - 1][a];
+            t->surf[a] = t->block[a] * t->blocks[a];
+            for (uint32_t l = 0; l < L; ++l) {
+                t->virt[l][a] = virt[h][l][a];
+                t->real[l][a] = real[h][l][a];
+            }
+        }
+        t->nblocks = t->blocks[0] * t->blocks[1] * t->blocks[2];
+    }
 }
 
 void or_volume_position(const or_geom *g, uint32_t level, const uint32_t brick[3], float out[3])
@@ -128,6 +165,15 @@ uint64_t or_mrpd_index(const or_geom *g, uint32_t level, const float pos[3])
     const uint32_t y = (uint32_t)(pos[1] * (float)g->level_dim[level][1]);
     const uint32_t z = (uint32_t)(pos[2] * (float)g->level_dim[level][2]);
     return (uint64_t)x + (uint64_t)g->mrpd_dim[0] * ((uint64_t)y + (uint64_t)g->mrpd_dim[1] * z);
+}
+
+uint64_t or_top_index(const or_geom *g, uint32_t level, const float pos[3])
+{
+    /* K_MultiResolutionPageDirectory.hpp:168-180 over the MRPD's levels */
+    const uint32_t x = g->top_coord[level] + (uint32_t)(pos[0] * (float)g->top_dim[level][0]);
+    const uint32_t y = (uint32_t)(pos[1] * (float)g->top_dim[level][1]);
+    const uint32_t z = (uint32_t)(pos[2] * (float)g->top_dim[level][2]);
+    return (uint64_t)x + (uint64_t)g->top_cells_dim[0] * ((uint64_t)y + (uint64_t)g->top_cells_dim[1] * z);
 }
 
 /* ============================================================ preprocessor */
@@ -224,7 +270,12 @@ struct or_cache {
     uint32_t *mrpd, *req, *usage;
     or_lru *lru, *lru_tmp;
     uint8_t *pool;
-    uint64_t mrpd_cells;
+    uint64_t mrpd_cells;      /* brick grid (request buffer) */
+    uint64_t top_cells;       /* the MRPD                   */
+    /* table caches (CacheDevice<uint4>): entries, block usage, LRU */
+    uint32_t *tab[OR_MAX_TABLES], *tab_usage[OR_MAX_TABLES];
+    or_lru *tab_lru[OR_MAX_TABLES], *tab_lru_tmp[OR_MAX_TABLES];
+    uint32_t unreferenced;
     uint32_t ts;
     /* request handler state (RequestHandler.hpp:89-105) */
     int request_done;
@@ -242,6 +293,25 @@ uint32_t *or_cache_usage(or_cache *c) { return c->usage; }
 or_lru *or_cache_lru(or_cache *c) { return c->lru; }
 void *or_cache_pool(or_cache *c) { return c->pool; }
 uint32_t or_cache_timestamp(const or_cache *c) { return c->ts; }
+uint32_t *or_cache_table(or_cache *c, uint32_t i) { return i < c->g.nb_tables ? c->tab[i] : NULL; }
+uint32_t *or_cache_table_usage(or_cache *c, uint32_t i) { return i < c->g.nb_tables ? c->tab_usage[i] : NULL; }
+or_lru *or_cache_table_lru(or_cache *c, uint32_t i) { return i < c->g.nb_tables ? c->tab_lru[i] : NULL; }
+uint32_t or_cache_unreferenced(const or_cache *c) { return c->unreferenced; }
+
+static void lru_init(or_lru *lru, const uint32_t n[3], const uint32_t block[3])
+{
+    /* LRU initialisation, x fastest (CacheDevice.hpp:151-165) */
+    for (uint32_t z = 0; z < n[2]; ++z)
+        for (uint32_t y = 0; y < n[1]; ++y)
+            for (uint32_t x = 0; x < n[0]; ++x) {
+                or_lru *e = &lru[x + y * n[0] + z * n[0] * n[1]];
+                e->cache_pos[0] = x * block[0];
+                e->cache_pos[1] = y * block[1];
+                e->cache_pos[2] = z * block[2];
+                e->level = 0;
+                e->vol_pos[0] = e->vol_pos[1] = e->vol_pos[2] = -1.f;
+            }
+}
 
 or_cache *or_cache_create(const or_cfg *cfg, const void *const *levels)
 {
@@ -250,25 +320,25 @@ or_cache *or_cache_create(const or_cfg *cfg, const void *const *levels)
     or_geometry(cfg, &c->g);
     for (uint32_t l = 0; l < cfg->nb_levels; ++l) c->levels[l] = levels ? levels[l] : NULL;
     c->mrpd_cells = (uint64_t)c->g.mrpd_dim[0] * c->g.mrpd_dim[1] * c->g.mrpd_dim[2];
-    c->mrpd = (uint32_t *)calloc(c->mrpd_cells * 4, sizeof(uint32_t));
+    c->top_cells = (uint64_t)c->g.top_cells_dim[0] * c->g.top_cells_dim[1] * c->g.top_cells_dim[2];
+    c->mrpd = (uint32_t *)calloc(c->top_cells * 4, sizeof(uint32_t));
     c->req = (uint32_t *)calloc(c->mrpd_cells, sizeof(uint32_t));
     c->usage = (uint32_t *)calloc(c->g.nslots, sizeof(uint32_t));
     c->lru = (or_lru *)calloc(c->g.nslots, sizeof(or_lru));
     c->lru_tmp = (or_lru *)calloc(c->g.nslots, sizeof(or_lru));
     const uint64_t pool_vox = (uint64_t)c->g.pool_dim[0] * c->g.pool_dim[1] * c->g.pool_dim[2];
     c->pool = (uint8_t *)calloc(pool_vox, cfg->bytes);
-    /* LRU initialisation, x fastest (CacheDevice.hpp:151-165) */
-    const uint32_t cx = cfg->cache_size[0], cy = cfg->cache_size[1], cz = cfg->cache_size[2];
-    for (uint32_t z = 0; z < cz; ++z)
-        for (uint32_t y = 0; y < cy; ++y)
-            for (uint32_t x = 0; x < cx; ++x) {
-                or_lru *e = &c->lru[x + y * cx + z * cx * cy];
-                e->cache_pos[0] = x * cfg->brick_size;
-                e->cache_pos[1] = y * cfg->brick_size;
-                e->cache_pos[2] = z * cfg->brick_size;
-                e->level = 0;
-                e->vol_pos[0] = e->vol_pos[1] = e->vol_pos[2] = -1.f;
-            }
+    const uint32_t bs3[3] = {cfg->brick_size, cfg->brick_size, cfg->brick_size};
+    lru_init(c->lru, cfg->cache_size, bs3);
+    for (uint32_t i = 0; i < c->g.nb_tables; ++i) {
+        const or_table_geom *t = &c->g.table[i];
+        const uint64_t ne = (uint64_t)t->surf[0] * t->surf[1] * t->surf[2];
+        c->tab[i] = (uint32_t *)calloc(ne * 4, sizeof(uint32_t));
+        c->tab_usage[i] = (uint32_t *)calloc(t->nblocks, sizeof(uint32_t));
+        c->tab_lru[i] = (or_lru *)calloc(t->nblocks, sizeof(or_lru));
+        c->tab_lru_tmp[i] = (or_lru *)calloc(t->nblocks, sizeof(or_lru));
+        lru_init(c->tab_lru[i], t->blocks, t->block);
+    }
     c->ts = 1; /* CacheManager.hpp:152 */
     return c;
 }
@@ -278,28 +348,40 @@ void or_cache_destroy(or_cache *c)
     if (!c) return;
     free(c->mrpd); free(c->req); free(c->usage); free(c->lru); free(c->lru_tmp);
     free(c->pool); free(c->asked); free(c->bb_indexes); free(c->bb_coords); free(c->staging);
+    for (uint32_t i = 0; i < c->g.nb_tables; ++i) {
+        free(c->tab[i]); free(c->tab_usage[i]); free(c->tab_lru[i]); free(c->tab_lru_tmp[i]);
+    }
     free(c);
 }
 
-static inline uint64_t slot_of(const or_cache *c, const uint32_t cp[3])
-{
-    const uint32_t bs = c->cfg.brick_size;
-    const uint32_t cx = c->cfg.cache_size[0], cy = c->cfg.cache_size[1];
-    return (uint64_t)(cp[0] / bs) + (uint64_t)cx * ((cp[1] / bs) + (uint64_t)cy * (cp[2] / bs));
-}
-
-void or_lru_update(or_cache *c)
+static void lru_partition(or_lru **lru, or_lru **tmp, uint32_t n, const uint32_t *usage,
+                          const uint32_t blocks[3], const uint32_t block[3], uint32_t ts)
 {
     /* create_usage_mask (CacheManagerKernels.cu:8-29) then the two copy_if of
      * CacheDevice::update (CacheDevice.hpp:202-219): used entries first, both
      * halves in their previous order. */
-    const uint32_t n = c->g.nslots;
+    or_lru *a = *lru, *b = *tmp;
     uint32_t k = 0;
-    for (uint32_t i = 0; i < n; ++i)
-        if (c->usage[slot_of(c, c->lru[i].cache_pos)] == c->ts) c->lru_tmp[k++] = c->lru[i];
-    for (uint32_t i = 0; i < n; ++i)
-        if (c->usage[slot_of(c, c->lru[i].cache_pos)] != c->ts) c->lru_tmp[k++] = c->lru[i];
-    or_lru *t = c->lru; c->lru = c->lru_tmp; c->lru_tmp = t;
+    for (int pass = 0; pass < 2; ++pass)
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t *cp = a[i].cache_pos;
+            const uint64_t u = (uint64_t)(cp[0] / block[0]) +
+                               (uint64_t)blocks[0] * ((cp[1] / block[1]) + (uint64_t)blocks[1] * (cp[2] / block[2]));
+            if ((usage[u] == ts) == (pass == 0)) b[k++] = a[i];
+        }
+    *lru = b;
+    *tmp = a;
+}
+
+void or_lru_update(or_cache *c)
+{
+    /* CacheManager.hpp:387-406: the data cache, then each table cache */
+    const uint32_t bs3[3] = {c->cfg.brick_size, c->cfg.brick_size, c->cfg.brick_size};
+    lru_partition(&c->lru, &c->lru_tmp, c->g.nslots, c->usage, c->cfg.cache_size, bs3, c->ts);
+    for (uint32_t i = 0; i < c->g.nb_tables; ++i) {
+        const or_table_geom *t = &c->g.table[i];
+        lru_partition(&c->tab_lru[i], &c->tab_lru_tmp[i], t->nblocks, c->tab_usage[i], t->blocks, t->block, c->ts);
+    }
 }
 
 uint32_t or_request_list(or_cache *c, uint32_t *out4, uint32_t cap)
@@ -431,6 +513,185 @@ static void mrpd_set(or_cache *c, uint32_t level, const float pos[3], const uint
     memcpy(c->mrpd + 4 * or_mrpd_index(&c->g, level, pos), v, 4 * sizeof(uint32_t));
 }
 
+/* ------------------------------------------------ page-table caches (tables)
+ *
+ * The hierarchy MRPD -> table 0 -> ... -> table T-1 -> data cache of
+ * CacheManager.hpp:151-236.  An entry is a uint4 (x, y, z, w): w = 0
+ * Unmapped, 1 Mapped, 2 Empty; x, y, z the origin of the block (or brick
+ * slot) it maps in the next cache's surface.  The entry of a position inside
+ * a block at origin o is o + (uint)(pos * virtualized) % BlockSize
+ * (K_CacheDevice.hpp:221-233,316-334, covering 0). */
+
+static inline uint64_t tab_entry(const or_table_geom *t, const uint32_t o[3], uint32_t level, const float pos[3])
+{
+    uint32_t e[3];
+    for (int a = 0; a < 3; ++a) e[a] = o[a] + ((uint32_t)(pos[a] * (float)t->virt[level][a])) % t->block[a];
+    return (uint64_t)e[0] + (uint64_t)t->surf[0] * ((uint64_t)e[1] + (uint64_t)t->surf[1] * e[2]);
+}
+
+/* update_usage_buffer of a table block (K_CacheDevice.hpp:245-250,323-327) */
+static inline void tab_stamp(or_cache *c, uint32_t i, const uint32_t o[3], uint32_t ts)
+{
+    const or_table_geom *t = &c->g.table[i];
+    const uint64_t u = (uint64_t)(o[0] / t->block[0]) +
+                       (uint64_t)t->blocks[0] * ((o[1] / t->block[1]) + (uint64_t)t->blocks[1] * (o[2] / t->block[2]));
+    __atomic_store_n(&c->tab_usage[i][u], ts, __ATOMIC_RELAXED);
+}
+
+/* The page-table walk of get / get_normalized (K_CacheManager.hpp:206-231,
+ * 310-339): the MRPD entry, then one entry per table cache while the entries
+ * are Mapped; each table read stamps its block's usage when stamp != 0.  e
+ * receives the last entry read: w = 1 maps the data brick, 0 Unmapped and 2
+ * Empty end the walk at whichever level they occur.  The reference keeps
+ * walking after an Unmapped table entry with a coarser level pending and then
+ * repeats the same lookup for ever (checkOtherLevel without ++level,
+ * K_CacheManager.hpp:212-222,318-330); here such an entry is Unmapped, as an
+ * Unmapped MRPD entry is. */
+static void walk(or_cache *c, uint32_t level, const float pv[3], int stamp, uint32_t ts, uint32_t e[4])
+{
+    memcpy(e, c->mrpd + 4 * or_top_index(&c->g, level, pv), 4 * sizeof(uint32_t));
+    for (uint32_t i = 0; i < c->g.nb_tables && e[3] == 1; ++i) {
+        if (stamp) tab_stamp(c, i, e, ts);
+        memcpy(e, c->tab[i] + 4 * tab_entry(&c->g.table[i], e, level, pv), 4 * sizeof(uint32_t));
+    }
+}
+
+/* The key of the entries a walk reads: the MRPD cell, then per table cache
+ * the entry's index inside its block.  Two positions with the same key walk
+ * to the same entry. */
+static inline void walk_key(const or_geom *g, uint32_t level, const float pv[3], uint64_t key[1 + OR_MAX_TABLES])
+{
+    key[0] = or_top_index(g, level, pv);
+    for (uint32_t i = 0; i < OR_MAX_TABLES; ++i) {
+        key[1 + i] = 0;
+        if (i >= g->nb_tables) continue;
+        const or_table_geom *t = &g->table[i];
+        uint32_t l[3];
+        for (int a = 0; a < 3; ++a) l[a] = ((uint32_t)(pv[a] * (float)t->virt[level][a])) % t->block[a];
+        key[1 + i] = l[0] + (uint64_t)t->block[0] * (l[1] + (uint64_t)t->block[1] * l[2]);
+    }
+}
+
+static const uint32_t zero4[4] = {0, 0, 0, 0};
+
+/* dereference_old_bricks (CacheManagerKernels.hpp:111-145) for a victim slot
+ * of the data cache: the entry that maps it is cleared.  The reference
+ * clears whatever entry the walk reaches (for a table walk even from an
+ * Unmapped entry, at the origin (0,0,0)); once the table block that held the
+ * slot's entry was evicted, that entry belongs to another brick, whose
+ * mapping it would drop.  Here the walk stops at an entry that is not
+ * Mapped and clears the last entry only if it still names this slot. */
+static void deref_tables(or_cache *c, const or_lru *v)
+{
+    const uint32_t T = c->g.nb_tables;
+    uint32_t e[4];
+    memcpy(e, c->mrpd + 4 * or_top_index(&c->g, v->level, v->vol_pos), sizeof(e));
+    for (uint32_t i = 0; i + 1 < T; ++i) {
+        if (e[3] != 1) return;
+        tab_stamp(c, i, e, c->ts);
+        memcpy(e, c->tab[i] + 4 * tab_entry(&c->g.table[i], e, v->level, v->vol_pos), sizeof(e));
+    }
+    if (e[3] != 1) return;
+    uint32_t *last = c->tab[T - 1] + 4 * tab_entry(&c->g.table[T - 1], e, v->level, v->vol_pos);
+    if (last[3] == 1 && last[0] == v->cache_pos[0] && last[1] == v->cache_pos[1] && last[2] == v->cache_pos[2])
+        memcpy(last, zero4, sizeof(zero4));
+}
+
+/* The entry that maps block `o` of table j for the position (level, pos) it
+ * was assigned to: the MRPD cell for j = 0, else the entry in the table j-1
+ * block the walk reaches; NULL if the walk meets an entry that is not Mapped
+ * or the entry no longer names o.  Table reads above table j-1 stamp usage
+ * (the get calls of upward, CacheManagerKernels.hpp:246-263). */
+static uint32_t *owner_entry(or_cache *c, uint32_t j, uint32_t level, const float pos[3], const uint32_t o[3])
+{
+    uint32_t *p = c->mrpd + 4 * or_top_index(&c->g, level, pos);
+    for (uint32_t i = 0; i < j; ++i) {
+        if (p[3] != 1) return NULL;
+        uint32_t e[3] = {p[0], p[1], p[2]};
+        if (i + 1 < j) tab_stamp(c, i, e, c->ts);
+        p = c->tab[i] + 4 * tab_entry(&c->g.table[i], e, level, pos);
+    }
+    if (p[3] != 1 || p[0] != o[0] || p[1] != o[1] || p[2] != o[2]) return NULL;
+    return p;
+}
+
+/* reference_new_bricks with downward / upward (CacheManagerKernels.hpp
+ * :148-357) for one brick: `child` is the brick's slot origin (w = 1) or, for
+ * an Empty brick, (0,0,0) with w = 2; used[j] counts the blocks of table j
+ * taken from its LRU tail in this load.
+ *
+ *   downward: walk the Mapped levels; the brick is referenced only if the
+ *     walk ends at an Unmapped entry (the MRPD-only rule "insert if
+ *     Unmapped", :205-208); the tables below the last Mapped level each take
+ *     their next LRU-tail block (fill_stack, :128-136).
+ *   upward, bottom table first: the block is cleared (reset_entries), the
+ *     entry of its previous owner is cleared, the new entry written and the
+ *     block's LRU record takes the brick's level and position (:236-301);
+ *     then the new chain is linked into the last Mapped level.
+ *
+ * Departures from the reference, each a defect of its latent multi-table
+ * path: the stack of downward is kept across bricks, so a brick whose MRPD
+ * entry is Unmapped is linked into the block of an earlier brick (:322-350);
+ * fill_stack takes a block also from the table whose entry is already
+ * Mapped (:176-183) and indexes lruSizes / indexLRU of the wrong table
+ * (:130-136); an Empty brick is entered with w = 1 in its table block and
+ * w = 2 in the MRPD, which hides the whole block (:210-214,226-232); the
+ * previous owner's entry is looked up one table too deep (:246-263) and
+ * cleared without checking that it still names the block.  Here only
+ * missing blocks are taken, Empty is entered as w = 2 at the brick's own
+ * entry, owners are cleared only where they still name the block, and a
+ * brick for whose tables this load has no block left stays unreferenced. */
+static void reference_one(or_cache *c, uint32_t level, const float pos[3], const uint32_t child[3], uint32_t cw,
+                          uint32_t used[OR_MAX_TABLES])
+{
+    const uint32_t T = c->g.nb_tables;
+    uint32_t *p = c->mrpd + 4 * or_top_index(&c->g, level, pos);
+    uint32_t d = 0;
+    while (d < T && p[3] == 1) {
+        const uint32_t o[3] = {p[0], p[1], p[2]};
+        tab_stamp(c, d, o, c->ts);
+        p = c->tab[d] + 4 * tab_entry(&c->g.table[d], o, level, pos);
+        ++d;
+    }
+    if (p[3] != 0) return;
+    for (uint32_t j = d; j < T; ++j)
+        if (used[j] >= c->g.table[j].nblocks) {
+            ++c->unreferenced;
+            return;
+        }
+    or_lru *vic[OR_MAX_TABLES];
+    for (uint32_t j = d; j < T; ++j) vic[j] = &c->tab_lru[j][c->g.table[j].nblocks - 1 - used[j]++];
+    for (uint32_t j = T; j-- > d;) {
+        const or_table_geom *t = &c->g.table[j];
+        or_lru *v = vic[j];
+        for (uint32_t z = 0; z < t->block[2]; ++z)
+            for (uint32_t y = 0; y < t->block[1]; ++y)
+                for (uint32_t x = 0; x < t->block[0]; ++x) {
+                    const uint64_t k = (uint64_t)(v->cache_pos[0] + x) +
+                                       (uint64_t)t->surf[0] * ((v->cache_pos[1] + y) + (uint64_t)t->surf[1] * (v->cache_pos[2] + z));
+                    memcpy(c->tab[j] + 4 * k, zero4, sizeof(zero4));
+                }
+        if (v->vol_pos[0] > -1.f) {
+            uint32_t *owner = owner_entry(c, j, v->level, v->vol_pos, v->cache_pos);
+            if (owner) memcpy(owner, zero4, sizeof(zero4));
+        }
+        uint32_t *dst = c->tab[j] + 4 * tab_entry(t, v->cache_pos, level, pos);
+        if (j + 1 == T) {
+            dst[0] = child[0]; dst[1] = child[1]; dst[2] = child[2]; dst[3] = cw;
+        } else {
+            dst[0] = vic[j + 1]->cache_pos[0]; dst[1] = vic[j + 1]->cache_pos[1]; dst[2] = vic[j + 1]->cache_pos[2];
+            dst[3] = 1;
+        }
+        memcpy(v->vol_pos, pos, 3 * sizeof(float));
+        v->level = level;
+    }
+    if (d == T) {
+        p[0] = child[0]; p[1] = child[1]; p[2] = child[2]; p[3] = cw;
+    } else {
+        p[0] = vic[d]->cache_pos[0]; p[1] = vic[d]->cache_pos[1]; p[2] = vic[d]->cache_pos[2]; p[3] = 1;
+    }
+}
+
 static void load_new_bricks(or_cache *c)
 {
     /* CacheManager.hpp:503-632 with CacheManagerKernels.hpp:59-145,308-357
@@ -440,10 +701,12 @@ static void load_new_bricks(or_cache *c)
     const uint32_t nne = c->nb_non_empty;
     const uint32_t N = c->g.nslots;
     or_lru *victims = c->lru + (N - nne);
-    static const uint32_t zero4[4] = {0, 0, 0, 0};
     /* dereference_old_bricks: victims that were mapped lose their entry */
-    for (uint32_t j = 0; j < nne; ++j)
-        if (!(victims[j].vol_pos[0] < 0.f)) mrpd_set(c, victims[j].level, victims[j].vol_pos, zero4);
+    for (uint32_t j = 0; j < nne; ++j) {
+        if (victims[j].vol_pos[0] < 0.f) continue;
+        if (c->g.nb_tables) deref_tables(c, &victims[j]);
+        else mrpd_set(c, victims[j].level, victims[j].vol_pos, zero4);
+    }
     /* compute_lru_volume_position into a zero-initialised draft */
     or_lru *draft = (or_lru *)calloc(n, sizeof(or_lru));
     for (uint32_t k = 0; k < n; ++k) {
@@ -473,6 +736,18 @@ static void load_new_bricks(or_cache *c)
                                          PXY * (victims[j].cache_pos[2] + z);
                     put_voxel(c->pool, dst, bytes, c->staging, src);
                 }
+    }
+    c->unreferenced = 0;
+    if (c->g.nb_tables) {
+        /* reference_new_bricks through the table caches: non-empty bricks,
+         * then the empty ones, one at a time (CacheManagerKernels.hpp:336-353) */
+        uint32_t used[OR_MAX_TABLES] = {0, 0, 0};
+        static const uint32_t origin0[3] = {0, 0, 0};
+        for (uint32_t j = 0; j < nne; ++j)
+            reference_one(c, victims[j].level, victims[j].vol_pos, victims[j].cache_pos, 1, used);
+        for (uint32_t k = nne; k < n; ++k) reference_one(c, draft[k].level, draft[k].vol_pos, origin0, 2, used);
+        free(draft);
+        return;
     }
     /* reference_new_bricks (MRPD only): insert if the entry is Unmapped */
     for (uint32_t j = 0; j < nne; ++j) {
@@ -690,14 +965,15 @@ int or_sample(const or_cache *c, uint32_t lod, const float np[3], double *sum8,
     for (int a = 0; a < 3; ++a) {
         const float pr = np[a] * g->ior[lod][a];
         pv[a] = pr * g->rdv[lod][a];
-        b[a] = (uint32_t)(pv[a] * (float)g->level_dim[lod][a]);
+        /* the brick the walk addresses: the MRPD's, or the last table's */
+        const uint32_t nb = g->nb_tables ? g->table[g->nb_tables - 1].virt[lod][a] : g->level_dim[lod][a];
+        b[a] = (uint32_t)(pv[a] * (float)nb);
         v[a] = (uint32_t)(pv[a] * (float)g->virt_vox[lod][a]);
     }
     if (brick) memcpy(brick, b, sizeof(b));
     if (voxel) memcpy(voxel, v, sizeof(v));
-    const uint64_t mi = (uint64_t)(g->level_coord[lod] + b[0]) +
-                        (uint64_t)g->mrpd_dim[0] * (b[1] + (uint64_t)g->mrpd_dim[1] * b[2]);
-    const uint32_t *e = c->mrpd + 4 * mi;
+    uint32_t e[4];
+    walk((or_cache *)c, lod, pv, 0, 0, e);
     if (e[3] == 0) return 1;
     if (e[3] == 2) return 2;
     const uint32_t cov = c->cfg.covering;
@@ -723,7 +999,12 @@ int or_probe(or_cache *c, int mode, uint32_t level, const float pos[3], double *
             real[a] = pos[a] * g->ior[level][a];
             virt[a] = real[a] * g->rdv[level][a];
         }
-        const uint32_t *e = c->mrpd + 4 * or_mrpd_index(g, level, virt);
+        /* an Unmapped entry at any level of the walk is Unmapped (the
+         * reference's get falls back to the coarser level after an Unmapped
+         * MRPD entry, :241-252; after an Unmapped table entry it repeats the
+         * lookup for ever, see walk) */
+        uint32_t e[4];
+        walk(c, level, virt, 1, c->ts, e);
         if (e[3] == 1) {
             const uint64_t ui = e[0] / bs + (uint64_t)c->cfg.cache_size[0] * (e[1] / bs + (uint64_t)c->cfg.cache_size[1] * (e[2] / bs));
             __atomic_store_n(&c->usage[ui], c->ts, __ATOMIC_RELAXED);
@@ -834,7 +1115,7 @@ static void dvr_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
         if (job->rpos && tb > 0.f && tb == t) pos = mk(job->rpos[4 * i], job->rpos[4 * i + 1], job->rpos[4 * i + 2]);
         const float epx = fabsf(dot3(sub(dir, dxn), dleft) * tfar);
         const float epy = fabsf(dot3(sub(dir, dyn), dup) * tfar);
-        uint32_t prev[3] = {UINT32_MAX, UINT32_MAX, UINT32_MAX};
+        uint64_t prev[1 + OR_MAX_TABLES] = {UINT64_MAX, UINT64_MAX, UINT64_MAX, UINT64_MAX};
         uint32_t entry[4] = {0, 0, 0, 0};
         const float astop = V->alpha_stop;
         const uint32_t N = V->tf_size;
@@ -852,13 +1133,14 @@ static void dvr_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
             /* K_CacheManager::get_normalized (K_CacheManager.hpp:267-361) */
             const f3 pr = mul(np, mk(g->ior[lod][0], g->ior[lod][1], g->ior[lod][2]));
             const f3 pv = mul(pr, mk(g->rdv[lod][0], g->rdv[lod][1], g->rdv[lod][2]));
-            const uint32_t mc[3] = {g->level_coord[lod] + (uint32_t)(pv.x * (float)g->level_dim[lod][0]),
-                                    (uint32_t)(pv.y * (float)g->level_dim[lod][1]),
-                                    (uint32_t)(pv.z * (float)g->level_dim[lod][2])};
-            if (mc[0] != prev[0] || mc[1] != prev[1] || mc[2] != prev[2]) {
-                const uint64_t mi = mc[0] + (uint64_t)g->mrpd_dim[0] * (mc[1] + (uint64_t)g->mrpd_dim[1] * mc[2]);
-                memcpy(entry, c->mrpd + 4 * mi, sizeof(entry));
-                memcpy(prev, mc, sizeof(prev));
+            /* the MRPD entry, then the table caches' (one-entry cache of the
+             * walk's key, K_CacheManager.hpp:297-303) */
+            const float pva[3] = {pv.x, pv.y, pv.z};
+            uint64_t key[1 + OR_MAX_TABLES];
+            walk_key(g, lod, pva, key);
+            if (memcmp(key, prev, sizeof(key)) != 0) {
+                walk(c, lod, pva, 1, ts, entry);
+                memcpy(prev, key, sizeof(key));
                 st[1] += 1;
             }
             if (entry[3] == 0) {
@@ -1016,7 +1298,7 @@ static void mop_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
     if (job->rpos && tb > 0.f && tb == t) pos = mk(job->rpos[4 * i], job->rpos[4 * i + 1], job->rpos[4 * i + 2]);
     const float epx = fabsf(dot3(sub(dir, dxn), dleft) * tfar);
     const float epy = fabsf(dot3(sub(dir, dyn), dup) * tfar);
-    uint32_t prev[3] = {UINT32_MAX, UINT32_MAX, UINT32_MAX};
+    uint64_t prev[1 + OR_MAX_TABLES] = {UINT64_MAX, UINT64_MAX, UINT64_MAX, UINT64_MAX};
     uint32_t entry[4] = {0, 0, 0, 0};
     /* maxOpacity: FP64 the sample s = sum / D in double, FP32 in float */
     double mx = job->cstate[4 * i];
@@ -1034,13 +1316,12 @@ static void mop_pixel(dvr_job *job, uint32_t x, uint32_t y, uint64_t st[4])
         const uint32_t lod = or_compute_lod(V, dl, du, pdx, pdy);
         const f3 pr = mul(np, mk(g->ior[lod][0], g->ior[lod][1], g->ior[lod][2]));
         const f3 pv = mul(pr, mk(g->rdv[lod][0], g->rdv[lod][1], g->rdv[lod][2]));
-        const uint32_t mc[3] = {g->level_coord[lod] + (uint32_t)(pv.x * (float)g->level_dim[lod][0]),
-                                (uint32_t)(pv.y * (float)g->level_dim[lod][1]),
-                                (uint32_t)(pv.z * (float)g->level_dim[lod][2])};
-        if (mc[0] != prev[0] || mc[1] != prev[1] || mc[2] != prev[2]) {
-            const uint64_t mi = mc[0] + (uint64_t)g->mrpd_dim[0] * (mc[1] + (uint64_t)g->mrpd_dim[1] * mc[2]);
-            memcpy(entry, c->mrpd + 4 * mi, sizeof(entry));
-            memcpy(prev, mc, sizeof(prev));
+        const float pva[3] = {pv.x, pv.y, pv.z};
+        uint64_t key[1 + OR_MAX_TABLES];
+        walk_key(g, lod, pva, key);
+        if (memcmp(key, prev, sizeof(key)) != 0) {
+            walk(c, lod, pva, 1, ts, entry);
+            memcpy(prev, key, sizeof(key));
             st[1] += 1;
         }
         if (entry[3] == 0) {

@@ -28,6 +28,7 @@ extern "C" {
 #endif
 
 #define OR_MAX_LEVELS 16
+#define OR_MAX_TABLES 3
 
 /* ---------------------------------------------------------------- geometry */
 
@@ -84,7 +85,26 @@ typedef struct {
     uint32_t cache_size[3];           /* CacheSize[0], in bricks            */
     uint32_t max_requests;            /* _nbMaxRequests (50)                */
     float    empty_threshold;         /* big brick Empty iff all voxels < t */
+    /* Page-table caches between the MRPD and the data cache: CacheSize[k+1]
+     * and BlockSize[k+1] of CacheConfiguration (CacheManager.hpp:138-141);
+     * nb_tables = CacheSize.size() - 1, 0 for an MRPD that maps bricks. */
+    uint32_t nb_tables;
+    uint32_t table_cache[OR_MAX_TABLES][3];   /* blocks per axis            */
+    uint32_t table_block[OR_MAX_TABLES][3];   /* entries per block per axis */
 } or_cfg;
+
+/* One page-table cache of the hierarchy (CacheDevice<uint4>,
+ * CacheDevice.hpp:123-180): table[i] of or_geom is the reference's
+ * _tableCaches[i] (CacheManager.hpp:223-233), i = 0 next to the MRPD, i.e.
+ * hierarchy index nb_tables - i and configuration index nb_tables - 1 - i. */
+typedef struct {
+    uint32_t block[3];                      /* BlockSize                     */
+    uint32_t blocks[3];                     /* CacheSize (blocks per axis)   */
+    uint32_t surf[3];                       /* entries per axis: blocks*block */
+    uint32_t nblocks;
+    uint32_t virt[OR_MAX_LEVELS][3];        /* virtualizedNumberOfEntries    */
+    uint32_t real[OR_MAX_LEVELS][3];        /* realNumberOfEntries           */
+} or_table_geom;
 
 typedef struct {
     uint32_t level_dim[OR_MAX_LEVELS][3];   /* realNumberOfEntries[1]        */
@@ -93,9 +113,18 @@ typedef struct {
     float    ior[OR_MAX_LEVELS][3];         /* initialOverReal               */
     float    rdv[OR_MAX_LEVELS][3];         /* data cache real/virtual       */
     float    one_over[3];                   /* 1/(bs-2c)                     */
-    uint32_t mrpd_dim[3];                   /* also the request buffer shape */
+    uint32_t mrpd_dim[3];                   /* brick grid: the request buffer
+                                               shape, and the MRPD's when
+                                               nb_tables = 0                 */
     uint32_t pool_dim[3];                   /* voxels                        */
     uint32_t nslots;
+    /* the MRPD (realNumberOfEntries.back(), MultiResolutionPageDirectory.hpp
+     * :99-124): the brick grid above when nb_tables = 0 */
+    uint32_t nb_tables;
+    uint32_t top_dim[OR_MAX_LEVELS][3];
+    uint32_t top_coord[OR_MAX_LEVELS];
+    uint32_t top_cells_dim[3];
+    or_table_geom table[OR_MAX_TABLES];
 } or_geom;
 
 /* LRU record (LruContent.hpp:10-15), 28 bytes. */
@@ -107,7 +136,8 @@ typedef struct {
 
 typedef struct or_cache or_cache;
 
-/* CacheManager ctor geometry (CacheManager.hpp:157-285). */
+/* CacheManager ctor geometry (CacheManager.hpp:157-285), any number of
+ * table caches. */
 void or_geometry(const or_cfg *cfg, or_geom *g);
 
 /* Create a cache over a fully host-resident pyramid: levels[l] points to the
@@ -117,12 +147,20 @@ void      or_cache_destroy(or_cache *c);
 const or_geom *or_cache_geom(const or_cache *c);
 
 /* Raw state, exposed for comparison and for trace replay. */
-uint32_t *or_cache_mrpd(or_cache *c);      /* 4 x uint32 per entry          */
-uint32_t *or_cache_request(or_cache *c);   /* uint32 per MRPD cell          */
+uint32_t *or_cache_mrpd(or_cache *c);      /* 4 x uint32 per entry (top_cells_dim) */
+uint32_t *or_cache_request(or_cache *c);   /* uint32 per brick cell (mrpd_dim) */
 uint32_t *or_cache_usage(or_cache *c);     /* uint32 per slot, x-fastest    */
 or_lru   *or_cache_lru(or_cache *c);       /* nslots records in LRU order   */
 void     *or_cache_pool(or_cache *c);      /* pool_dim voxels, x-fastest    */
 uint32_t  or_cache_timestamp(const or_cache *c);
+/* Table cache i (i = 0 next to the MRPD): its uint4 entries (surf, x-fastest),
+ * its per-block usage and its LRU (nblocks records, LRU order). */
+uint32_t *or_cache_table(or_cache *c, uint32_t i);
+uint32_t *or_cache_table_usage(or_cache *c, uint32_t i);
+or_lru   *or_cache_table_lru(or_cache *c, uint32_t i);
+/* Bricks of the last load that the table caches could not reference (their
+ * tables had no victim block left in that load); 0 without table caches. */
+uint32_t  or_cache_unreferenced(const or_cache *c);
 
 /* CacheManager::update with a synchronous request handler
  * (CacheManager.hpp:319-354; the handler finishes inside notify). */
@@ -140,8 +178,12 @@ void or_cache_load_all(or_cache *c);
 
 /* FP32 page-table position of a brick (CacheManagerKernels.cu:75-86). */
 void or_volume_position(const or_geom *g, uint32_t level, const uint32_t brick[3], float out[3]);
-/* MRPD cell of a float position (K_MultiResolutionPageDirectory.hpp:168-180). */
+/* Brick-grid cell of a float position (K_MultiResolutionPageDirectory.hpp
+ * :168-180 over realNumberOfEntries[1]): the MRPD cell when nb_tables = 0,
+ * and the request-buffer cell (K_CacheManager.hpp:365-396). */
 uint64_t or_mrpd_index(const or_geom *g, uint32_t level, const float pos[3]);
+/* MRPD cell of a float position (the top of the hierarchy). */
+uint64_t or_top_index(const or_geom *g, uint32_t level, const float pos[3]);
 
 /* ------------------------------------------------------------------- DVR */
 

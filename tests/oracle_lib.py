@@ -14,6 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
 MAXL = 16
+MAXT = 3
 
 u32 = C.c_uint32
 f32 = C.c_float
@@ -25,6 +26,14 @@ class OrCfg(C.Structure):
         ("big_brick", u32 * 3), ("nb_levels", u32),
         ("initial", (u32 * 3) * MAXL), ("real", (u32 * 3) * MAXL),
         ("cache_size", u32 * 3), ("max_requests", u32), ("empty_threshold", f32),
+        ("nb_tables", u32), ("table_cache", (u32 * 3) * MAXT), ("table_block", (u32 * 3) * MAXT),
+    ]
+
+
+class OrTableGeom(C.Structure):
+    _fields_ = [
+        ("block", u32 * 3), ("blocks", u32 * 3), ("surf", u32 * 3), ("nblocks", u32),
+        ("virt", (u32 * 3) * MAXL), ("real", (u32 * 3) * MAXL),
     ]
 
 
@@ -33,6 +42,8 @@ class OrGeom(C.Structure):
         ("level_dim", (u32 * 3) * MAXL), ("level_coord", u32 * MAXL),
         ("virt_vox", (u32 * 3) * MAXL), ("ior", (f32 * 3) * MAXL), ("rdv", (f32 * 3) * MAXL),
         ("one_over", f32 * 3), ("mrpd_dim", u32 * 3), ("pool_dim", u32 * 3), ("nslots", u32),
+        ("nb_tables", u32), ("top_dim", (u32 * 3) * MAXL), ("top_coord", u32 * MAXL),
+        ("top_cells_dim", u32 * 3), ("table", OrTableGeom * MAXT),
     ]
 
 
@@ -111,6 +122,15 @@ def lib():
         L.or_render_mop.argtypes = [P, C.POINTER(OrView), P, P, P, P, C.c_int, C.c_int, P]
         L.or_sample.restype = C.c_int
         L.or_sample.argtypes = [P, u32, P, C.POINTER(C.c_double), P, P]
+        for name in ("or_cache_table", "or_cache_table_usage"):
+            getattr(L, name).restype = C.POINTER(u32)
+            getattr(L, name).argtypes = [P, u32]
+        L.or_cache_table_lru.restype = C.POINTER(OrLru)
+        L.or_cache_table_lru.argtypes = [P, u32]
+        L.or_cache_unreferenced.restype = u32
+        L.or_cache_unreferenced.argtypes = [P]
+        L.or_top_index.restype = C.c_uint64
+        L.or_top_index.argtypes = [C.POINTER(OrGeom), u32, P]
         _lib = L
     return _lib
 
@@ -179,7 +199,9 @@ def brick_level(level, bs, cov, b0, b1):
 
 
 def make_cfg(initial, real, bs, cov, bytes_, cache_size, bb=(1, 1, 1), max_requests=50,
-             empty_threshold=0.0):
+             empty_threshold=0.0, tables=()):
+    """tables: the page-table caches as (CacheSize[k+1], BlockSize[k+1])
+    pairs in CacheConfiguration order (k = 0 next to the data cache)."""
     cfg = OrCfg()
     cfg.brick_size = bs
     cfg.covering = cov
@@ -192,6 +214,10 @@ def make_cfg(initial, real, bs, cov, bytes_, cache_size, bb=(1, 1, 1), max_reque
     cfg.cache_size[:] = list(cache_size)
     cfg.max_requests = max_requests
     cfg.empty_threshold = empty_threshold
+    cfg.nb_tables = len(tables)
+    for k, (cs, bl) in enumerate(tables):
+        cfg.table_cache[k][:] = [int(v) for v in cs]
+        cfg.table_block[k][:] = [int(v) for v in bl]
     return cfg
 
 
@@ -212,7 +238,10 @@ class Cache:
         self.h = lib().or_cache_create(C.byref(cfg), arr)
         self.g = lib().or_cache_geom(self.h).contents
         self.cells = int(self.g.mrpd_dim[0]) * int(self.g.mrpd_dim[1]) * int(self.g.mrpd_dim[2])
+        d = self.g.top_cells_dim
+        self.top_cells = int(d[0]) * int(d[1]) * int(d[2])
         self.nslots = int(self.g.nslots)
+        self.nb_tables = int(self.g.nb_tables)
 
     def close(self):
         if self.h:
@@ -224,7 +253,24 @@ class Cache:
 
     def mrpd(self):
         p = lib().or_cache_mrpd(self.h)
-        return np.ctypeslib.as_array(p, (self.cells, 4))
+        return np.ctypeslib.as_array(p, (self.top_cells, 4))
+
+    def table(self, i):
+        """Entries of table cache i (i = 0 next to the MRPD): (entries, 4),
+        the surface x-fastest."""
+        t = self.g.table[i]
+        n = int(t.surf[0]) * int(t.surf[1]) * int(t.surf[2])
+        return np.ctypeslib.as_array(lib().or_cache_table(self.h, i), (n, 4))
+
+    def table_usage(self, i):
+        return np.ctypeslib.as_array(lib().or_cache_table_usage(self.h, i), (int(self.g.table[i].nblocks),))
+
+    def table_lru(self, i):
+        p = C.cast(lib().or_cache_table_lru(self.h, i), C.POINTER(u32))
+        return np.ctypeslib.as_array(p, (int(self.g.table[i].nblocks), 7))
+
+    def unreferenced(self):
+        return int(lib().or_cache_unreferenced(self.h))
 
     def request(self):
         return np.ctypeslib.as_array(lib().or_cache_request(self.h), (self.cells,))

@@ -24,7 +24,7 @@ class Scene:
     """A volume, its pyramid, matching oracle and product configurations."""
 
     def __init__(self, dims, bs, cov, bytes_, cache_size=None, max_requests=50, seed=99, big_brick=(1, 1, 1),
-                 empty_threshold=0.0, **blob_kw):
+                 empty_threshold=0.0, tables=(), **blob_kw):
         self.dims, self.bs, self.cov, self.bytes = dims, bs, cov, bytes_
         self.vc = G.volume_cfg(dims, bs, cov, bytes_, big_brick)
         self.initial = self.vc.initial()
@@ -39,10 +39,11 @@ class Scene:
         self.max_requests = max_requests
         self.big_brick = big_brick
         self.empty_threshold = empty_threshold
+        self.tables = tuple(tables)
         self.ocfg = O.make_cfg(self.initial, self.real, bs, cov, bytes_, self.cache_size, bb=big_brick,
-                               max_requests=max_requests, empty_threshold=empty_threshold)
+                               max_requests=max_requests, empty_threshold=empty_threshold, tables=self.tables)
         self.cc = G.cache_cfg(self.cache_size, bs, max_requests=max_requests, sync=True,
-                              empty_threshold=empty_threshold)
+                              empty_threshold=empty_threshold, tables=self.tables)
 
     def oracle_cache(self):
         return O.Cache(self.ocfg, self.levels)
@@ -57,7 +58,8 @@ class Scene:
         """A cache whose pool holds every brick, fully loaded."""
         shape = O.pool_shape(self.total_bricks * self.bs ** 3 * self.bytes, self.bs ** 3 * self.bytes)
         src = G.Source.pyramid(self.vc, self.levels)
-        c = G.Cache(self.vc, G.cache_cfg(shape, self.bs, max_requests=self.max_requests, sync=True), src)
+        c = G.Cache(self.vc, G.cache_cfg(shape, self.bs, max_requests=self.max_requests, sync=True,
+                                         tables=self.tables), src)
         c._src = src
         c.load_all(self.device_levels())
         return c

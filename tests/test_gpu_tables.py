@@ -1,0 +1,245 @@
+"""GPU parity: the multi-level page table -- page-table caches between the
+MRPD and the brick pool -- through the C ABI against the CPU oracle.
+
+After every update the MRPD, every table cache's entries, block usage and
+LRU, the data cache's LRU and usage, the request buffer, the asked list and
+the count of unreferenced bricks must be bit-exact; every launch's render
+counters and t buffer too, and its RGBA within 1/255 of the FP64 oracle.
+The device page-table API (gc_get_normalized / gc_get) walks the tables with
+the oracle's answers and side effects.
+"""
+import numpy as np
+import pytest
+
+import gcvol as G
+import oracle_lib as O
+from scenes import Scene, orbit, tf_table
+
+pytestmark = pytest.mark.gpu
+
+# (CacheSize[k+1], BlockSize[k+1]) in configuration order, k = 0 next to the pool
+ONE_SMALL = [((3, 2, 2), (2, 2, 2))]
+ONE_ODD = [((4, 3, 2), (3, 2, 2))]
+TWO = [((4, 3, 3), (2, 2, 2)), ((2, 2, 1), (2, 1, 2))]
+
+
+def _check_cells(gc):
+    """The ray caster's MRPD words: table-0 block << 2 | w."""
+    m, cells = gc.mrpd(), gc.cell_table()
+    t = gc.g.table[0]
+    blk = m[:, 0] // t.block[0] + t.blocks[0] * (m[:, 1] // t.block[1] + t.blocks[1] * (m[:, 2] // t.block[2]))
+    want = np.where(m[:, 3] == 1, (blk << 2) | 1, m[:, 3])
+    assert np.array_equal(cells, want.astype(np.uint32))
+
+
+def _compare(gc, oc, pool=False):
+    assert gc.ts == oc.ts
+    assert np.array_equal(gc.mrpd(), oc.mrpd()), "MRPD differs"
+    for i in range(oc.nb_tables):
+        assert np.array_equal(gc.table(i), oc.table(i)), "table %d entries differ" % i
+        assert np.array_equal(gc.table_usage(i), oc.table_usage(i)), "table %d usage differs" % i
+        assert np.array_equal(gc.table_lru(i), oc.table_lru(i)), "table %d LRU differs" % i
+    assert np.array_equal(gc.lru(), oc.lru()), "LRU differs"
+    assert np.array_equal(gc.usage(), oc.usage()), "usage differs"
+    assert np.array_equal(gc.request(), oc.request()), "request buffer differs"
+    assert gc.unreferenced() == oc.unreferenced()
+    _check_cells(gc)
+    if pool:
+        assert np.array_equal(gc.pool(), oc.pool()), "pool contents differ"
+
+
+def _render_pair(sc, gc, oc, cam, prec, W, H, tf, d_tf, frame, otb, ocs, compositing=0, tile=0):
+    d_st = G.DeviceBuffer.from_array(np.zeros(4, np.uint64))
+    view = G.make_view(W, H, cam, precision=prec, tile=tile, compositing=compositing)
+    G.render(gc, view, d_tf, len(tf), frame, d_st)
+    G.synchronize()
+    gst = d_st.to_array(np.uint64, (4,))
+    oview = O.make_view(sc.ocfg, W, H, cam, tf)
+    opx = np.zeros(W * H, np.uint32)
+    if compositing:
+        ost = oc.render_mop(oview, otb, ocs, opx, precision=64, nthreads=8)
+    else:
+        ost = oc.render(oview, otb, ocs, opx, precision=64, nthreads=8)
+    return gst, G.unpack_rgba(frame.pixels().reshape(-1)), ost, G.unpack_rgba(opx)
+
+
+@pytest.mark.parametrize("bytes_,tables,prec,tile", [
+    (1, ONE_SMALL, 64, 0), (2, ONE_ODD, 64, 1), (1, TWO, 64, 2), (1, ONE_SMALL, 32, 0), (2, TWO, 32, 3)])
+def test_out_of_core_sequence_with_tables(gpu, bytes_, tables, prec, tile, parity_log):
+    print("tables", tables, "bytes", bytes_, "precision", prec, "tile", tile, flush=True)
+    sc = Scene((64, 64, 64), 8, 1, bytes_, cache_size=(6, 5, 5), max_requests=50, tables=tables)
+    gc, oc = sc.gpu_cache(), sc.oracle_cache()
+    assert gc.nb_tables == len(tables)
+    tf = tf_table(seed=11)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 64, 48
+    frame = G.Frame(W, H, prec)
+    otb = np.zeros(W * H, np.float32)
+    ocs = np.zeros(W * H * 4)
+    errs, steps, unref = [], 0, 0
+    for k in range(3):
+        cam = orbit(3 * k)
+        frame.reset()
+        otb[:] = 0
+        ocs[:] = 0
+        for it in range(40):
+            gst, g, ost, o = _render_pair(sc, gc, oc, cam, prec, W, H, tf, d_tf, frame, otb, ocs, tile=tile)
+            assert gst.tolist() == ost.tolist(), "render counters differ at view %d step %d" % (k, it)
+            errs.append(np.abs(g - o))
+            assert errs[-1].max() <= 1
+            assert np.array_equal(frame.state()[0].reshape(-1), otb), "t buffer differs"
+            gd, od = gc.update(), oc.update()
+            assert gd == od
+            _compare(gc, oc, pool=(it % 5 == 0))
+            ga, gne = gc.asked()
+            oa, one = oc.asked()
+            assert gne == one and np.array_equal(ga, oa)
+            unref += oc.unreferenced()
+            steps += 1
+            if gd:
+                break
+    rec = parity_log(prec, np.concatenate(errs), launches=steps, bytes=bytes_, tables=len(tables))
+    print("launches", steps, "unreferenced bricks", unref, "max err", rec["max_err"], flush=True)
+    assert steps > 3
+
+
+@pytest.mark.parametrize("bytes_", [1, 2])
+def test_maximum_projection_with_tables(gpu, bytes_):
+    sc = Scene((48, 40, 32), 8, 1, bytes_, cache_size=(5, 4, 4), max_requests=40, tables=ONE_SMALL, seed=4)
+    gc, oc = sc.gpu_cache(), sc.oracle_cache()
+    tf = tf_table(seed=2)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 48, 40
+    frame = G.Frame(W, H, 64)
+    otb = np.zeros(W * H, np.float32)
+    ocs = np.zeros(W * H * 4)
+    for it in range(40):
+        gst, g, ost, o = _render_pair(sc, gc, oc, orbit(2), 64, W, H, tf, d_tf, frame, otb, ocs, compositing=1)
+        assert gst.tolist() == ost.tolist(), it
+        assert np.abs(g - o).max() <= 1
+        gd, od = gc.update(), oc.update()
+        assert gd == od
+        _compare(gc, oc)
+        if gd:
+            break
+    assert it > 1
+
+
+@pytest.mark.parametrize("tables", [ONE_SMALL, TWO])
+def test_random_streams_lockstep(gpu, tables):
+    """Random request and usage streams (the loads' inputs) on both caches:
+    the load kernels' results equal the oracle's after every update."""
+    sc = Scene((64, 48, 40), 8, 1, 1, cache_size=(5, 3, 3), max_requests=23, seed=5, nb_blobs=4,
+               sigma=(3.0, 8.0), tables=tables)
+    gc, oc = sc.gpu_cache(), sc.oracle_cache()
+    g = oc.g
+    rng = np.random.default_rng(3 + len(tables))
+    for step in range(40):
+        ts = oc.ts
+        req = oc.request()
+        for _ in range(rng.integers(0, 30)):
+            l = rng.integers(0, sc.vc.nb_levels)
+            b = [rng.integers(0, g.level_dim[l][a]) for a in range(3)]
+            req[(g.level_coord[l] + b[0]) + g.mrpd_dim[0] * (b[1] + g.mrpd_dim[1] * b[2])] = ts
+        u = oc.usage()
+        u[rng.random(oc.nslots) < 0.2] = ts
+        gc.write_request(req)
+        gc.write_usage(u)
+        for i in range(oc.nb_tables):
+            tu = oc.table_usage(i)
+            tu[rng.random(len(tu)) < 0.3] = ts
+            gc.write_table_usage(i, tu)
+        gd, od = gc.update(), oc.update()
+        assert gd == od
+        _compare(gc, oc, pool=(step % 8 == 0))
+
+
+def _request_levels(gc, levels, ts):
+    """Every brick cell of the given levels marked requested at ts."""
+    g = gc.g
+    req = gc.request().reshape(int(g.mrpd_dim[2]), int(g.mrpd_dim[1]), int(g.mrpd_dim[0]))
+    for l in levels:
+        x0 = int(g.level_coord[l])
+        req[:int(g.level_dim[l][2]), :int(g.level_dim[l][1]), x0:x0 + int(g.level_dim[l][0])] = ts
+    return req.reshape(-1)
+
+
+def _map_levels_both(sc, gc, oc, levels):
+    """One request of every brick of the levels (the list cap and the pool
+    hold them all), loaded by both caches."""
+    req = _request_levels(gc, levels, oc.ts)
+    gc.write_request(req)
+    oc.request()[:] = req
+    for _ in range(3):
+        gd, od = gc.update(), oc.update()
+        assert gd == od
+        _compare(gc, oc)
+    assert gd
+
+
+@pytest.mark.parametrize("bytes_,thr", [(1, 0.0), (2, 30.0)])
+def test_device_api_probes_with_tables(gpu, bytes_, thr):
+    sc = Scene((40, 32, 24), 8, 1, bytes_, seed=3, nb_blobs=4, sigma=(3.0, 8.0), max_requests=10000,
+               empty_threshold=thr, tables=[((6, 6, 6), (2, 2, 2))])
+    gc, oc = sc.gpu_cache(), sc.oracle_cache()
+    _map_levels_both(sc, gc, oc, range(1, sc.vc.nb_levels))
+    nl = sc.vc.nb_levels
+    rng = np.random.default_rng(11 + bytes_)
+    n = 2000
+    pos = (rng.random((n, 3)) * 0.99).astype(np.float32)
+    levels = rng.integers(0, nl, n).astype(np.uint32)
+    norm = np.float32(1.0) / np.float32(8.0 * (255.0 if bytes_ == 1 else 65535.0))
+    seen = set()
+    for mode in (0, 1):
+        print("device API probes with tables, mode", mode, "bytes", bytes_, "threshold", thr, flush=True)
+        vals, sts = gc.probe(mode, pos, levels, init=-1.0)
+        for i in range(n):
+            st, raw = oc.probe(mode, int(levels[i]), pos[i])
+            assert sts[i] == st, (mode, i)
+            seen.add((mode, st))
+            if st == 2:
+                want = np.float32(-1.0)
+            elif mode == 0:
+                want = np.float32(raw) * norm if st == 0 else np.float32(0.0)
+            else:
+                want = np.float32(raw)
+            assert vals[i] == want, (mode, i, vals[i], want)
+        assert np.array_equal(gc.usage(), oc.usage()), mode
+        assert np.array_equal(gc.request(), oc.request()), mode
+        assert np.array_equal(gc.table_usage(0), oc.table_usage(0)), mode
+    assert (0, 0) in seen and (0, 1) in seen and (1, 0) in seen
+
+
+def test_resident_tables_match_flat_cache(gpu):
+    """A table cache holding every page block, block sizes dividing the bricks
+    per axis (big bricks of 2): with every brick loaded, the frames equal
+    the MRPD-only cache's bit for bit."""
+    kw = dict(dims=(64, 64, 64), bs=8, cov=1, bytes_=1, big_brick=(2, 2, 2), seed=9, max_requests=100000)
+    flat = Scene(**kw)
+    g0 = O.geometry(Scene(tables=[((1, 1, 1), (2, 2, 2))], **kw).ocfg)
+    nblk = sum(int(np.prod(list(g0.top_dim[l]))) for l in range(flat.vc.nb_levels))
+    tab = Scene(tables=[((nblk, 1, 1), (2, 2, 2))], **kw)
+    caches = []
+    for sc in (flat, tab):
+        c = sc.gpu_cache()
+        c.write_request(_request_levels(c, range(sc.vc.nb_levels), c.ts))
+        assert not c.update() and not c.update()
+        assert c.unreferenced() == 0
+        caches.append(c)
+    a, b = caches
+    assert np.array_equal(a.lru(), b.lru())
+    tf = tf_table(seed=4)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 96, 80
+    for prec in (32, 64):
+        for k in (0, 5):
+            fa, fb = G.Frame(W, H, prec), G.Frame(W, H, prec)
+            view = G.make_view(W, H, orbit(k), precision=prec)
+            sa = G.DeviceBuffer.from_array(np.zeros(4, np.uint64))
+            sb = G.DeviceBuffer.from_array(np.zeros(4, np.uint64))
+            G.render(a, view, d_tf, len(tf), fa, sa)
+            G.render(b, view, d_tf, len(tf), fb, sb)
+            G.synchronize()
+            assert np.array_equal(fa.pixels(), fb.pixels()), (prec, k)
+            assert np.array_equal(fa.state()[0], fb.state()[0]), (prec, k)
+            assert np.array_equal(sa.to_array(np.uint64, (4,)), sb.to_array(np.uint64, (4,))), (prec, k)
