@@ -4,7 +4,11 @@
 // from a user kernel through K_CacheManager::get_normalized until every probe
 // is resident, then render one frame with the plain single-view renderer.
 //
-//   shim_demo <volume.cfg> <out_prefix> [probes_per_axis]
+//   shim_demo <volume.cfg> <out_prefix> [probes_per_axis] [table_block]
+//
+// table_block > 0 adds one page-table cache (CacheSize[1], BlockSize[1]) of
+// cubic blocks of that many bricks per axis, with a block for every page
+// block of every level.
 //
 // Writes <out_prefix>.samples (float per probe, level-major, x fastest),
 // <out_prefix>.status (uint32 VoxelStatus per probe) and <out_prefix>.pixels
@@ -43,7 +47,8 @@ static void hip_check(hipError_t e, const char *what)
 }
 
 template <typename T>
-static int run(const tdns::data::VolumeConfiguration &vc, tdns::data::MetaData &md, const std::string &out, uint32_t n)
+static int run(const tdns::data::VolumeConfiguration &vc, tdns::data::MetaData &md, const std::string &out, uint32_t n,
+               uint32_t table_block)
 {
     // one brick slot per brick of every level: fully resident once loaded
     uint64_t bricks = 0;
@@ -58,6 +63,20 @@ static int run(const tdns::data::VolumeConfiguration &vc, tdns::data::MetaData &
     tdns::data::CacheConfiguration cc;
     cc.CacheSize.emplace_back(shape[0], shape[1], shape[2]);
     cc.BlockSize.push_back(vc.BrickSize);
+    if (table_block) {
+        uint32_t blocks = 0;
+        for (uint32_t l = 0; l < vc.NbLevels; ++l) {
+            uint32_t nb = 1;
+            for (int a = 0; a < 3; ++a) {
+                const uint32_t inner = vc.BrickSize[a] - 2 * vc.Covering[a];
+                const uint32_t b = (vc.RealVolumesSizes[l][a] + inner - 1) / inner;
+                nb *= (b + table_block - 1) / table_block;
+            }
+            blocks += nb;
+        }
+        cc.CacheSize.emplace_back(blocks, 1u, 1u);
+        cc.BlockSize.emplace_back(table_block, table_block, table_block);
+    }
     cc.DataCacheFlags = 1;
     CacheManager<T> manager(vc, md, cc, 0);
 
@@ -137,10 +156,11 @@ static int run(const tdns::data::VolumeConfiguration &vc, tdns::data::MetaData &
 int main(int argc, char **argv)
 {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s <volume.cfg> <out_prefix> [probes_per_axis]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <volume.cfg> <out_prefix> [probes_per_axis] [table_block]\n", argv[0]);
         return 1;
     }
     const uint32_t n = argc > 3 ? (uint32_t)std::atoi(argv[3]) : 16;
+    const uint32_t table_block = argc > 4 ? (uint32_t)std::atoi(argv[4]) : 0;
     try {
         const tdns::data::VolumeConfiguration vc = tdns::data::load_volume_configuration(argv[1]);
         tdns::data::MetaData md;
@@ -149,9 +169,9 @@ int main(int argc, char **argv)
             return 2;
         }
         switch (vc.EncodedBytes) {
-        case 1: return run<uchar1>(vc, md, argv[2], n);
-        case 2: return run<ushort1>(vc, md, argv[2], n);
-        default: return run<float1>(vc, md, argv[2], n);
+        case 1: return run<uchar1>(vc, md, argv[2], n, table_block);
+        case 2: return run<ushort1>(vc, md, argv[2], n, table_block);
+        default: return run<float1>(vc, md, argv[2], n, table_block);
         }
     } catch (const std::exception &e) {
         std::fprintf(stderr, "error: %s\n", e.what());

@@ -333,8 +333,10 @@ public:
             throw std::runtime_error("CacheManager: NumberEncodedBytes does not match the voxel type");
         if (cacheConfiguration.CacheSize.empty() || cacheConfiguration.BlockSize.empty())
             throw std::runtime_error("CacheManager: CacheConfiguration needs CacheSize[0] and BlockSize[0]");
-        if (cacheConfiguration.CacheSize.size() > 1)
-            throw std::runtime_error("CacheManager: multi-level page tables are not supported");
+        if (cacheConfiguration.CacheSize.size() != cacheConfiguration.BlockSize.size())
+            throw std::runtime_error("CacheManager: CacheSize and BlockSize need one entry per cache");
+        if (cacheConfiguration.CacheSize.size() > 1 + GC_MAX_TABLES)
+            throw std::runtime_error("CacheManager: at most GC_MAX_TABLES page-table caches");
         if (!metaData.source()) throw std::runtime_error("CacheManager: MetaData has no brick source");
         const gc_volume_cfg vc = data::to_gc(volumeConfiguration);
         gc_cache_cfg cc{};
@@ -342,6 +344,14 @@ public:
             cc.cache_size[a] = cacheConfiguration.CacheSize[0][a];
             cc.block_size[a] = cacheConfiguration.BlockSize[0][a];
         }
+        // CacheSize[1..] / BlockSize[1..]: the page-table caches between the
+        // MRPD and the pool (CacheManager.hpp:151-236)
+        cc.nb_tables = (uint32_t)cacheConfiguration.CacheSize.size() - 1;
+        for (uint32_t k = 0; k < cc.nb_tables; ++k)
+            for (int a = 0; a < 3; ++a) {
+                cc.table_cache_size[k][a] = cacheConfiguration.CacheSize[k + 1][a];
+                cc.table_block_size[k][a] = cacheConfiguration.BlockSize[k + 1][a];
+            }
         cc.data_cache_flags = cacheConfiguration.DataCacheFlags;
         cc.max_requests = 50;   // _nbMaxRequests (CacheManager.hpp:263)
         cc.sync_requests = 0;   // requests served by the worker thread

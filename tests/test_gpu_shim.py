@@ -30,17 +30,25 @@ CFG = ('<Fields>\n<Field key="BrickSize" value="{bs}"/>\n<Field key="BigBrickSiz
        '<Field key="WorkingDirectory" value="{wd}" />\n<Field key="VolumeFile" value="vol.raw" />\n</Fields>\n')
 
 
-@pytest.mark.parametrize("bytes_", [1, 2])
-def test_shim_demo_matches_oracle(gpu, tmp_path, bytes_):
+@pytest.mark.parametrize("bytes_,table_block", [(1, 0), (2, 0), (1, 2), (2, 3)])
+def test_shim_demo_matches_oracle(gpu, tmp_path, bytes_, table_block):
     dims = (48, 40, 36)
-    sc = Scene(dims, 8, 1, bytes_)
+    tables = ()
+    if table_block:
+        # CacheSize[1]: a block for every page block (as shim_demo sizes it)
+        probe = Scene(dims, 8, 1, bytes_, tables=[((1, 1, 1), (table_block,) * 3)], nb_blobs=0)
+        g = O.geometry(probe.ocfg)
+        nblk = sum(int(np.prod(list(g.top_dim[l]))) for l in range(probe.vc.nb_levels))
+        tables = [((nblk, 1, 1), (table_block,) * 3)]
+    sc = Scene(dims, 8, 1, bytes_, tables=tables)
     sc.vol.tofile(tmp_path / "vol.raw")
     cfg = tmp_path / "vol.cfg"
     cfg.write_text(CFG.format(bs=8, cov=1, nb=bytes_, x=dims[0], y=dims[1], z=dims[2], wd=str(tmp_path) + "/"))
     exe = os.path.join(os.path.dirname(G.LIB_PATH), "shim_demo")
     n = 12
-    print("shim_demo bytes", bytes_, "dims", dims, flush=True)
-    out = subprocess.run([exe, str(cfg), str(tmp_path / "out"), str(n)], capture_output=True, text=True, timeout=120)
+    print("shim_demo bytes", bytes_, "dims", dims, "table block", table_block, flush=True)
+    out = subprocess.run([exe, str(cfg), str(tmp_path / "out"), str(n), str(table_block)], capture_output=True,
+                         text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     fields = out.stdout.split()
     nl = int(fields[1])

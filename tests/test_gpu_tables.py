@@ -243,3 +243,53 @@ def test_resident_tables_match_flat_cache(gpu):
             assert np.array_equal(fa.pixels(), fb.pixels()), (prec, k)
             assert np.array_equal(fa.state()[0], fb.state()[0]), (prec, k)
             assert np.array_equal(sa.to_array(np.uint64, (4,)), sb.to_array(np.uint64, (4,))), (prec, k)
+
+
+def _transparent_pair(max_requests=64, cache_size=(7, 6, 6)):
+    """A flat cache and one whose table cache (blocks of 2x2x2 bricks, big
+    bricks of 2) holds every page block: the same positions and loads."""
+    kw = dict(dims=(64, 64, 64), bs=8, cov=1, bytes_=1, big_brick=(2, 2, 2), seed=12, max_requests=max_requests,
+              cache_size=cache_size)
+    g0 = O.geometry(Scene(tables=[((1, 1, 1), (2, 2, 2))], **kw).ocfg)
+    flat = Scene(**kw)
+    nblk = sum(int(np.prod(list(g0.top_dim[l]))) for l in range(flat.vc.nb_levels))
+    return flat, Scene(tables=[((nblk, 1, 1), (2, 2, 2))], **kw)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_frame_driver_with_tables_matches_flat(gpu, prec):
+    """gc_render_frame (misses, loads and resumed launches until complete):
+    through full transparent tables the frame, its launches and counters
+    equal the flat cache's."""
+    flat, tab = _transparent_pair()
+    a, b = flat.gpu_cache(), tab.gpu_cache()
+    tf = tf_table(seed=8)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 80, 64
+    for k in (1, 4):
+        print("frame driver, precision", prec, "view", k, flush=True)
+        fa, fb = G.Frame(W, H, prec), G.Frame(W, H, prec)
+        view = G.make_view(W, H, orbit(k), precision=prec)
+        ra = G.render_frame(a, view, d_tf, len(tf), fa)
+        rb = G.render_frame(b, view, d_tf, len(tf), fb)
+        assert ra.complete and rb.complete
+        assert ra.launches == rb.launches
+        assert ra.stats.mapped_samples == rb.stats.mapped_samples
+        assert np.array_equal(fa.pixels(), fb.pixels()), k
+        assert np.array_equal(a.lru(), b.lru())
+    assert b.unreferenced() == 0
+
+
+def test_batch_with_tables_matches_flat(gpu):
+    flat, tab = _transparent_pair()
+    a, b = flat.gpu_cache(), tab.gpu_cache()
+    tfs = [tf_table(seed=s) for s in (1, 2, 3)]
+    d_tfs = [G.DeviceBuffer.from_array(t) for t in tfs]
+    sizes = [len(t) for t in tfs]
+    views = [G.make_view(48, 40, orbit(k), precision=32) for k in (0, 2, 5)]
+    ba, bb = G.Batch(a, 4, 3 * 48 * 40, 32), G.Batch(b, 4, 3 * 48 * 40, 32)
+    ra = ba.render(views, d_tfs, sizes)
+    rb = bb.render(views, d_tfs, sizes)
+    assert ra.complete and rb.complete and ra.launches == rb.launches
+    for i in range(3):
+        assert np.array_equal(ba.pixels(i), bb.pixels(i)), i
