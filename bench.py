@@ -492,6 +492,8 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
 # on the GPU on miss
 ONDEMAND_DIMS = (8192, 8192, 4096)
 ONDEMAND_POOL = 128 << 30
+# one page-table cache: 16384 blocks of 8x8x8 bricks (CacheSize[1], BlockSize[1])
+ONDEMAND_TABLES = [((32, 32, 16), (8, 8, 8))]
 
 
 def _lod_span(vc, width, dist):
@@ -512,7 +514,20 @@ def _lod_span(vc, width, dist):
     return level_at(max(dist - 1.0, 1e-6)), level_at(dist + 1.0)
 
 
-def ondemand_workload(G, sessions=32, steps=9, width=1280, height=720, far=200.0, near=1.05):
+def directory_bytes(g):
+    """Device bytes of the page directory a cache keeps besides its request
+    buffer (which the reference sizes by the bricks either way): the MRPD
+    (uint4 per cell) and the ray caster's cell words, plus per table cache
+    its entry words and per-block usage, LRU order (two copies) and owner."""
+    b = int(g.top_cells) * (16 + 4)
+    for i in range(int(g.nb_tables)):
+        t = g.table[i]
+        b3 = int(t.block[0]) * int(t.block[1]) * int(t.block[2])
+        b += int(t.nblocks) * (b3 * 4 + 4 + 8 + 16)
+    return b
+
+
+def ondemand_workload(G, sessions=32, steps=9, width=1280, height=720, far=200.0, near=1.05, tables=()):
     """The on-demand configuration: a virtual 8192x8192x4096 u8 volume
     (256 GiB at level 0, 9 levels, 32^3 bricks, covering 1, 128 GiB pool).
     Level 0 is synthesised on the GPU from seeded hash noise + blobs when a
@@ -529,9 +544,11 @@ def ondemand_workload(G, sessions=32, steps=9, width=1280, height=720, far=200.0
     print("ondemand: cold pyramid build %.0f ms (levels 1-%d resident)" % (build_ms, vc.nb_levels - 1),
           file=sys.stderr, flush=True)
     cs = G.pool_shape(ONDEMAND_POOL, 32 ** 3)
-    cache = G.Cache(vc, G.cache_cfg(cs, 32, max_requests=16384, sync=False), src)
-    tables = [G.tf_samples(G.tf_seeded(SEED + 400 + i, 6), 256) for i in range(sessions)]
-    d_tfs = [G.DeviceBuffer.from_array(t) for t in tables]
+    cache = G.Cache(vc, G.cache_cfg(cs, 32, max_requests=16384, sync=False, tables=tables), src)
+    flat_dir = directory_bytes(G.geometry_compute(vc, G.cache_cfg(cs, 32)))
+    dir_bytes = directory_bytes(cache.g)
+    tfs = [G.tf_samples(G.tf_seeded(SEED + 400 + i, 6), 256) for i in range(sessions)]
+    d_tfs = [G.DeviceBuffer.from_array(t) for t in tfs]
     b = G.Batch(cache, sessions, sessions * width * height, 32)
     out = []
     for k in range(steps):
@@ -551,6 +568,8 @@ def ondemand_workload(G, sessions=32, steps=9, width=1280, height=720, far=200.0
                 "fetch_ms": c1["fetch_ms"] - c0["fetch_ms"], "samples": r.stats.mapped_samples,
                 "complete": bool(r.complete)}
         step["resident_bricks_per_level"] = np.bincount(levels, minlength=vc.nb_levels).tolist()
+        if tables:
+            step["unreferenced_bricks_last_load"] = cache.unreferenced()
         print("ondemand step %d distance %.2f (levels %d..%d): %.0f ms, %d launches, %d bricks, fetch %.0f ms, "
               "%.3g samples" % (k, dist, span[0], span[1], dt * 1e3, r.launches, step["bricks_loaded"],
                                 step["fetch_ms"], step["samples"]), file=sys.stderr, flush=True)
@@ -559,10 +578,15 @@ def ondemand_workload(G, sessions=32, steps=9, width=1280, height=720, far=200.0
     cache.close()
     src.close()
     lat = [s["ms"] for s in out]
+    tdesc = ""
+    if tables:
+        tdesc = "; page-table cache%s %s" % ("s" if len(tables) > 1 else "", ", ".join(
+            "%s blocks of %s bricks" % ("x".join(map(str, c)), "x".join(map(str, b))) for c, b in tables))
     return {"workload": "virtual 8192x8192x4096 u8 (9 levels, 32^3 bricks, covering 1, 128 GiB pool); level 0 "
                         "synthesised on the GPU on request, levels 1-8 its mip pyramid built cold on the GPU; %d "
-                        "sessions of %dx%d zooming %.0f -> %.2f in %d steps (levels 8 -> 0), one batch per step"
-                        % (sessions, width, height, far, near, steps),
+                        "sessions of %dx%d zooming %.0f -> %.2f in %d steps (levels 8 -> 0), one batch per step%s"
+                        % (sessions, width, height, far, near, steps, tdesc),
+            "directory_bytes": dir_bytes, "directory_bytes_without_tables": flat_dir,
             "cold_pyramid_build_ms": build_ms,
             "frames_per_s": sessions * len(out) / (sum(lat) / 1e3), "step_p50_ms": percentile(lat, 50),
             "step_p99_ms": percentile(lat, 99), "steps": out}
@@ -779,7 +803,7 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
     if args.ondemand_only:
-        print(json.dumps({"ondemand": ondemand_workload(G)}))
+        print(json.dumps({"ondemand": ondemand_workload(G), "ondemand_tables": ondemand_workload(G, tables=ONDEMAND_TABLES)}))
         return
     if args.serving_only:
         rates = tuple(int(r) for r in args.serving_rates.split(","))
@@ -941,6 +965,8 @@ def main():
                                               device_source=True, tune_upload=True)
         if not args.no_ondemand:
             out["ondemand"] = ondemand_workload(G)
+            # the same through one page-table cache (MRPD of 8^3-brick blocks)
+            out["ondemand_tables"] = ondemand_workload(G, tables=ONDEMAND_TABLES)
     if args.serving:
         out["serving_long"] = serving_workload(G)
     print(json.dumps(out))

@@ -190,6 +190,24 @@ hipError_t launch_reference_tables(uint4 *mrpd, uint32_t *cells, const TableStat
                                    const SlotMeta *meta, const gc_geometry *d_geom, uint32_t ts,
                                    uint32_t *unreferenced, hipStream_t stream);
 
+// The same for one table cache, in parallel (see kernels_cache.hip): per
+// brick k, cell[k] / loc[k] are its MRPD cell and entry inside its table
+// block and rep[k] the first brick of the load in the same cell (computed on
+// the host; no two bricks share a cell and an entry).  When the load would
+// take as a victim the block of a cell it touches, the kernels leave the
+// state alone and the serial referencing runs instead.
+struct RefScratch {
+    uint32_t *cell, *loc, *rep;          // cap: from the host
+    uint32_t *w0, *rank, *ev;            // cap: MRPD words, victim ranks, event indices
+    uint32_t *inv;                       // table 0: LRU position of each block
+    uint32_t *flag;                      // [0] 1: the serial referencing runs
+};
+hipError_t launch_reference_table1(uint4 *mrpd, uint32_t *cells, const TableState &ts_, const uint32_t *order,
+                                   uint32_t nslots, uint32_t n, uint32_t nne, const uint32_t *asked4,
+                                   const SlotMeta *meta, const gc_geometry *d_geom, uint32_t ts,
+                                   uint32_t *unreferenced, const RefScratch &rs, hipStream_t stream,
+                                   uint32_t nblocks);
+
 hipError_t launch_occupancy(const SlotMeta *meta, uint32_t nslots, uint32_t *count,
                             hipStream_t stream);
 
@@ -306,6 +324,7 @@ struct PageTables {
     uint32_t top_x, top_y;                       // MRPD cells per row, rows per plane
     uint32_t block[GC_MAX_TABLES][3];
     uint32_t b3[GC_MAX_TABLES];                  // entries per block
+    uint32_t pow2[GC_MAX_TABLES];                // 1: every block dimension is a power of two
     const uint32_t *words[GC_MAX_TABLES];
     uint32_t *usage[GC_MAX_TABLES];
 };

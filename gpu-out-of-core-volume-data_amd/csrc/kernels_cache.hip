@@ -689,9 +689,10 @@ __device__ void reference_one(RefCtx &c, uint32_t level, const float vol[3], uin
 
 __global__ void k_reference_tables(uint4 *mrpd, uint32_t *cells, TableState tab, const uint32_t *order, uint32_t N,
                                    uint32_t n, uint32_t nne, const uint32_t *asked4, const SlotMeta *meta,
-                                   const gc_geometry *g, uint32_t ts, uint32_t *unreferenced)
+                                   const gc_geometry *g, uint32_t ts, uint32_t *unreferenced, const uint32_t *gate)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (gate && *gate == 0u) return;   // the parallel referencing ran
     RefCtx c{mrpd, cells, tab, g, ts, {0u, 0u, 0u}};
     uint32_t unref = 0;
     // non-empty bricks (their slots already hold level and position), then
@@ -725,7 +726,166 @@ hipError_t launch_reference_tables(uint4 *mrpd, uint32_t *cells, const TableStat
                                    uint32_t *unreferenced, hipStream_t stream)
 {
     k_reference_tables<<<1, 1, 0, stream>>>(mrpd, cells, tab, order, nslots, n, nne, asked4, meta, d_geom, ts,
-                                            unreferenced);
+                                            unreferenced, nullptr);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------- parallel referencing, T = 1
+// The serial referencing of one table cache, restated for the usual load:
+// in brick order, a brick whose MRPD cell is Unmapped when it comes takes the
+// next LRU-tail block (an event); the cell's later bricks find it Mapped.  A
+// cell Unmapped at the start of the load stays so until its first brick,
+// and a cell Mapped at the start stays Mapped unless its block is taken as a
+// victim.  So if no cell the load touches is Mapped to one of the blocks the
+// load takes, the events are exactly the first bricks of the cells Unmapped
+// at the start, event i takes LRU position nblocks-1-i, the owner a victim
+// loses is a cell the load does not touch, and every brick's entry is
+// written independently.  k_ref_scan checks that condition; when it fails,
+// these kernels change nothing and the serial kernel runs.
+
+// LRU position of every table-0 block; clears the gate and the counter
+__global__ void k_ref_init(const uint32_t *__restrict__ order, uint32_t nb, uint32_t *__restrict__ inv,
+                           uint32_t *flag, uint32_t *unreferenced)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        flag[0] = 0u;
+        *unreferenced = 0u;
+    }
+    if (i < nb) inv[order[i]] = i;
+}
+
+// the MRPD word of each brick's cell at the start of the load, and for a
+// Mapped one its block's distance from the LRU tail
+__global__ void k_ref_classify(const uint32_t *__restrict__ cells, RefScratch rs, uint32_t n, uint32_t nb)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t w = cells[rs.cell[k]];
+    rs.w0[k] = w;
+    rs.rank[k] = (w & 3u) == 1u ? nb - 1u - rs.inv[w >> 2] : UINT32_MAX;
+}
+
+// events in brick order (exclusive scan over first bricks of Unmapped
+// cells) and the gate: 1 when a touched Mapped cell's block is among the
+// min(events, nblocks) victims
+__global__ __launch_bounds__(1024) void k_ref_scan(RefScratch rs, uint32_t n, uint32_t nb)
+{
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t hit;
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + 1023u) / 1024u, b0 = t * per, b1 = min(n, b0 + per);
+    uint32_t c = 0;
+    for (uint32_t k = b0; k < b1; ++k) c += (rs.rep[k] == k && (rs.w0[k] & 3u) == 0u) ? 1u : 0u;
+    part[t] = c;
+    if (t == 0) hit = 0u;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const uint32_t E = part[1023];
+    uint32_t e = part[t] - c;
+    for (uint32_t k = b0; k < b1; ++k) {
+        const bool ev = rs.rep[k] == k && (rs.w0[k] & 3u) == 0u;
+        rs.ev[k] = ev ? e : UINT32_MAX;
+        e += ev ? 1u : 0u;
+    }
+    const uint32_t taken = min(E, nb);
+    bool h = false;
+    for (uint32_t k = b0; k < b1; ++k) h = h || rs.rank[k] < taken;
+    if (h) atomicOr(&hit, 1u);
+    __syncthreads();
+    if (t == 0) rs.flag[0] = hit;
+}
+
+// one workgroup per event brick: the victim block is cleared, its owner's
+// MRPD cell cleared where it names the block, the brick's cell mapped to it
+// and the block's owner recorded
+__global__ __launch_bounds__(256) void k_ref_events(uint4 *__restrict__ mrpd, uint32_t *__restrict__ cells,
+                                                    TableState tab, RefScratch rs, const uint32_t *__restrict__ order,
+                                                    uint32_t N, uint32_t nne, const uint32_t *__restrict__ asked4,
+                                                    const SlotMeta *__restrict__ meta, const gc_geometry *__restrict__ g)
+{
+    const uint32_t k = blockIdx.x;
+    if (rs.flag[0]) return;
+    const uint32_t i = rs.ev[k];
+    const gc_table_geometry &t = g->table[0];
+    if (i == UINT32_MAX || i >= t.nblocks) return;
+    const uint32_t v = tab.order[0][t.nblocks - 1u - i];
+    const uint32_t b3 = table_b3(t);
+    uint32_t *blk = tab.words[0] + (uint64_t)v * b3;
+    for (uint32_t e = threadIdx.x; e < b3; e += blockDim.x) blk[e] = 0u;   // reset_entries
+    if (threadIdx.x != 0) return;
+    SlotMeta &m = tab.meta[0][v];
+    if (m.vol[0] > -1.f) {
+        const uint32_t o = top_index(*g, m.level, m.vol);
+        if (cells[o] == cell_word(v, 1)) {
+            cells[o] = 0u;
+            mrpd[o] = make_uint4(0, 0, 0, 0);
+        }
+    }
+    SlotMeta nm;
+    if (k < nne) {
+        nm = meta[order[N - nne + k]];
+    } else {
+        const uint32_t *a = asked4 + 4 * k;
+        nm.level = a[3];
+        volume_position(*g, a[3], a, nm.vol);
+    }
+    m = nm;
+    const uint32_t c = rs.cell[k];
+    cells[c] = cell_word(v, 1);
+    uint32_t o3[3];
+    block_origin(t, v, o3);
+    mrpd[c] = make_uint4(o3[0], o3[1], o3[2], 1);
+}
+
+// every brick's entry (downward's stamps of the blocks it walks through)
+__global__ void k_ref_insert(TableState tab, RefScratch rs, const uint32_t *__restrict__ order, uint32_t N,
+                             uint32_t n, uint32_t nne, const gc_geometry *__restrict__ g, uint32_t ts,
+                             uint32_t *unreferenced)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || rs.flag[0]) return;
+    const gc_table_geometry &t = g->table[0];
+    const uint32_t r = rs.rep[k], w = rs.w0[r];
+    uint32_t b;
+    if ((w & 3u) == 1u) {
+        b = w >> 2;
+        tab.usage[0][b] = ts;
+    } else if ((w & 3u) == 0u) {
+        const uint32_t i = rs.ev[r];
+        if (i >= t.nblocks) {
+            atomicAdd(unreferenced, 1u);
+            return;
+        }
+        b = tab.order[0][t.nblocks - 1u - i];
+        if (k != r) tab.usage[0][b] = ts;
+    } else {
+        return;   // an Empty MRPD entry
+    }
+    uint32_t *e = tab.words[0] + (uint64_t)b * table_b3(t) + rs.loc[k];
+    if ((*e & 3u) != 0u) return;
+    *e = k < nne ? cell_word(order[N - nne + k], 1) : cell_word(0, 2);
+}
+
+hipError_t launch_reference_table1(uint4 *mrpd, uint32_t *cells, const TableState &tab, const uint32_t *order,
+                                   uint32_t nslots, uint32_t n, uint32_t nne, const uint32_t *asked4,
+                                   const SlotMeta *meta, const gc_geometry *d_geom, uint32_t ts,
+                                   uint32_t *unreferenced, const RefScratch &rs, hipStream_t stream,
+                                   uint32_t nblocks)
+{
+    if (n == 0) return hipSuccess;
+    k_ref_init<<<blocks_for(nblocks, BLK), BLK, 0, stream>>>(tab.order[0], nblocks, rs.inv, rs.flag, unreferenced);
+    k_ref_classify<<<blocks_for(n, BLK), BLK, 0, stream>>>(cells, rs, n, nblocks);
+    k_ref_scan<<<1, 1024, 0, stream>>>(rs, n, nblocks);
+    k_ref_events<<<n, 256, 0, stream>>>(mrpd, cells, tab, rs, order, nslots, nne, asked4, meta, d_geom);
+    k_ref_insert<<<blocks_for(n, BLK), BLK, 0, stream>>>(tab, rs, order, nslots, n, nne, d_geom, ts, unreferenced);
+    k_reference_tables<<<1, 1, 0, stream>>>(mrpd, cells, tab, order, nslots, n, nne, asked4, meta, d_geom, ts,
+                                            unreferenced, rs.flag);
     return hipGetLastError();
 }
 

@@ -543,7 +543,10 @@ __device__ inline void pt_lds_fill(PtLds &Q, const gc_geometry *g, uint32_t l)
 struct PtKey {
     uint32_t top, loc[GC_MAX_TABLES];
 };
-__device__ inline PtKey pt_key(const PageTables &pt, const PtLds &Q, F3 pv)
+// bx, by, bz: the last table's index before the modulo, i.e. the brick
+// index (not part of the key)
+__device__ inline PtKey pt_key(const PageTables &pt, const PtLds &Q, F3 pv, uint32_t &bx, uint32_t &by,
+                               uint32_t &bz)
 {
     PtKey k;
     k.top = Q.tcoord + (uint32_t)(pv.x * Q.tdf[0]) +
@@ -552,10 +555,25 @@ __device__ inline PtKey pt_key(const PageTables &pt, const PtLds &Q, F3 pv)
     for (int i = 0; i < GC_MAX_TABLES; ++i) {
         k.loc[i] = 0u;
         if ((uint32_t)i < pt.n) {
-            const uint32_t lx = ((uint32_t)(pv.x * Q.tvf[i][0])) % pt.block[i][0];
-            const uint32_t ly = ((uint32_t)(pv.y * Q.tvf[i][1])) % pt.block[i][1];
-            const uint32_t lz = ((uint32_t)(pv.z * Q.tvf[i][2])) % pt.block[i][2];
-            k.loc[i] = lx + pt.block[i][0] * (ly + pt.block[i][1] * lz);
+            const uint32_t ix = (uint32_t)(pv.x * Q.tvf[i][0]), iy = (uint32_t)(pv.y * Q.tvf[i][1]),
+                           iz = (uint32_t)(pv.z * Q.tvf[i][2]);
+            const uint32_t Bx = pt.block[i][0], By = pt.block[i][1], Bz = pt.block[i][2];
+            uint32_t lx, ly, lz;
+            if (pt.pow2[i]) {
+                lx = ix & (Bx - 1u);
+                ly = iy & (By - 1u);
+                lz = iz & (Bz - 1u);
+            } else {
+                lx = ix % Bx;
+                ly = iy % By;
+                lz = iz % Bz;
+            }
+            k.loc[i] = lx + Bx * (ly + By * lz);
+            if ((uint32_t)i + 1 == pt.n) {
+                bx = ix;
+                by = iy;
+                bz = iz;
+            }
         }
     }
     return k;
@@ -1108,7 +1126,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     }
                 } else {
                     // the walk's key; walk again when it changes
-                    const PtKey k = pt_key(p.pt, plv[lod], pv);
+                    const PtKey k = pt_key(p.pt, plv[lod], pv, bx, by, bz);
                     if (!pt_same(k, pk_prev)) {
                         pk_prev = k;
                         const uint32_t e = pt_walk(p, k, ts);
@@ -1143,18 +1161,11 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 const uint32_t vy = (uint32_t)(pv.y * vvf.y);
                 const uint32_t vz = (uint32_t)(pv.z * vvf.z);
                 // 24-bit products: voxel and brick indices stay far below 2^24
-                uint32_t lx, ly, lz;
-                if constexpr (PT) {
+                uint32_t lx = vx - __umul24(bx, inner), ly = vy - __umul24(by, inner), lz = vz - __umul24(bz, inner);
+                if (__builtin_expect(lx >= inner || ly >= inner || lz >= inner, 0)) {
                     lx = vx % inner;
                     ly = vy % inner;
                     lz = vz % inner;
-                } else {
-                    lx = vx - __umul24(bx, inner), ly = vy - __umul24(by, inner), lz = vz - __umul24(bz, inner);
-                    if (__builtin_expect(lx >= inner || ly >= inner || lz >= inner, 0)) {
-                        lx = vx % inner;
-                        ly = vy % inner;
-                        lz = vz % inner;
-                    }
                 }
                 ACC s = 0;
                 double sd = 0.0;
@@ -1249,12 +1260,17 @@ __device__ inline ACC lane_read(ACC x, uint32_t k)
 }
 
 // BSC as for k_dvr: 32 when the bricks are 32 with covering 1, else 0.
-template <int BYTES, typename ACC, int G, int BSC = 0>
+// PT: the page directory has table caches: each lane walks its sample's
+// entries (reads only; the stamps of the table blocks go with the sample's
+// consumption) and requests go to the brick-grid cell of the real-volume
+// position (realDivVirtual may differ from 1).
+template <int BYTES, typename ACC, int G, int BSC = 0, bool PT = false>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : 4))) void k_dvr_ray(const DvrLaunch p)
 {
     static_assert(G == 16 || G == 32 || G == 64, "16, 32 or 64 lanes per ray");
     constexpr uint32_t R = 64 / G;   // rays per wave
     __shared__ LevelLds lv[GC_MAX_LEVELS];
+    __shared__ PtLds plv[PT ? GC_MAX_LEVELS : 1];
     __shared__ float tswv[WAVES * R][GC_MAX_LEVELS + 1];
     // dynamic LDS: the views, then the launch's transfer functions
     extern __shared__ __align__(16) uint8_t dyn_lds[];
@@ -1275,6 +1291,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         L.step = p.lod_step[tid];
         L.kexp = p.kexp[tid];
         L.esq = p.esq[tid];
+        if constexpr (PT) pt_lds_fill(plv[tid], g, tid);
     }
     const uint32_t n_views = p.views ? p.n_views : 1u;
     if (p.views) {
@@ -1317,6 +1334,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     float t = 0.f, tfar = 0.f, astop = 0.f, step = 0.f, tnext = FINF;
     F3 pos = f3(0, 0, 0), dir = f3(0, 0, 0), dstep = f3(0, 0, 0);
     uint32_t lod = 0, cell_last = NONE;
+    PtKey key_last{NONE, {NONE, NONE, NONE}};   // PT: the last consumed sample's walk key
     double Q = 1.0, thr = 0.0;   // FP32: exact opacity product and its stop threshold
     uint32_t n_samples = 0, n_visits = 0, n_rays = 0, n_unmapped = 0;   // on each group's first lane
     auto set_level = [&](uint32_t l) {
@@ -1388,6 +1406,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     for (uint32_t l = 1; l < nl && tsw[l] <= t; ++l) lod = l;
                     set_level(lod);
                     cell_last = NONE;
+                    if constexpr (PT) key_last.top = NONE;
                     active = true;
                 }
             }
@@ -1415,14 +1434,52 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         }
         // locate, load and classify each lane's sample
         const LevelLds &L = lv[my_lod];
-        uint32_t cell, vi;
-        locate_cell(my_pos, f3(L.ior[0], L.ior[1], L.ior[2]), f3(L.ldf[0], L.ldf[1], L.ldf[2]),
-                    f3(L.vvf[0], L.vvf[1], L.vvf[2]), L.lcoord, MX, MY, inner, bs, cell, vi);
-        const uint32_t en = p.cells[cell];
+        uint32_t cell, vi, en;
+        bool fresh;
+        PtKey key{};
+        uint32_t tblk[GC_MAX_TABLES] = {NONE, NONE, NONE};   // PT: table blocks the walk read
+        if constexpr (!PT) {
+            locate_cell(my_pos, f3(L.ior[0], L.ior[1], L.ior[2]), f3(L.ldf[0], L.ldf[1], L.ldf[2]),
+                        f3(L.vvf[0], L.vvf[1], L.vvf[2]), L.lcoord, MX, MY, inner, bs, cell, vi);
+            en = p.cells[cell];
+            const uint32_t prev = __shfl_up(cell, 1u, G);
+            fresh = cell != (gl == 0 ? cell_last : prev);
+        } else {
+            F3 tp = my_pos * 0.495f + f3(0.495f, 0.495f, 0.495f);
+            tp.z = 1.0f - tp.z;
+            const F3 np = f3(clamp099(tp.x), clamp099(tp.y), clamp099(tp.z));
+            const F3 pr = np * f3(L.ior[0], L.ior[1], L.ior[2]);
+            const F3 pv = pr * f3(L.rdv[0], L.rdv[1], L.rdv[2]);
+            uint32_t bx = 0, by = 0, bz = 0;
+            key = pt_key(p.pt, plv[my_lod], pv, bx, by, bz);
+            // the request cell (raise_brick_request, K_CacheManager.hpp:365-396)
+            cell = L.lcoord + (uint32_t)(pr.x * L.ldf[0]) +
+                   MX * ((uint32_t)(pr.y * L.ldf[1]) + MY * (uint32_t)(pr.z * L.ldf[2]));
+            const uint32_t vx = (uint32_t)(pv.x * L.vvf[0]), vy = (uint32_t)(pv.y * L.vvf[1]),
+                           vz = (uint32_t)(pv.z * L.vvf[2]);
+            uint32_t lx = vx - __umul24(bx, inner), ly = vy - __umul24(by, inner), lz = vz - __umul24(bz, inner);
+            if (__builtin_expect(lx >= inner || ly >= inner || lz >= inner, 0)) {
+                lx = vx % inner;
+                ly = vy % inner;
+                lz = vz % inner;
+            }
+            vi = __umul24(__umul24(lz, bs) + ly, bs) + lx;
+            en = p.cells[key.top];
+#pragma unroll
+            for (int i = 0; i < GC_MAX_TABLES; ++i) {
+                if ((uint32_t)i < p.pt.n && word_state(en) == 1) {
+                    tblk[i] = word_slot(en);
+                    en = p.pt.words[i][(uint64_t)tblk[i] * p.pt.b3[i] + key.loc[i]];
+                }
+            }
+            PtKey prev;
+            prev.top = __shfl_up(key.top, 1u, G);
+#pragma unroll
+            for (int i = 0; i < GC_MAX_TABLES; ++i) prev.loc[i] = __shfl_up(key.loc[i], 1u, G);
+            fresh = !pt_same(key, gl == 0 ? key_last : prev);
+        }
         const uint32_t ew = word_state(en);
         const uint32_t slot = word_slot(en);
-        const uint32_t prev = __shfl_up(cell, 1u, G);
-        const bool fresh = cell != (gl == 0 ? cell_last : prev);
         const uint64_t off = (active && ew == 1) ? (uint64_t)slot * bbytes + corner + vi * (uint32_t)BYTES : 0ull;
         const uint32_t sum = box_add<BYTES>(box_load<BYTES>(pool + off, row, plane));
         bool skip;
@@ -1484,6 +1541,12 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     pz_k = __shfl(my_pos.z, (int)ks, G);
         const uint32_t lod_k = __shfl(my_lod, (int)ks, G), cell_k = __shfl(cell, (int)ks, G),
                        ew_k = __shfl(ew, (int)ks, G);
+        PtKey key_k{};
+        if constexpr (PT) {
+            key_k.top = __shfl(key.top, (int)ks, G);
+#pragma unroll
+            for (int i = 0; i < GC_MAX_TABLES; ++i) key_k.loc[i] = __shfl(key.loc[i], (int)ks, G);
+        }
         if (active) {
             // samples 0..kend-1 were consumed (all mapped)
             const uint64_t below = kend >= 64 ? ~0ull : ((1ull << kend) - 1ull);
@@ -1492,6 +1555,16 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 n_visits += (uint32_t)__popcll(m_fresh & below);
             }
             if (gl < kend && fresh) p.usage[slot] = ts;
+            if constexpr (PT) {
+                // the walks of the consumed samples, and of the one at kend
+                // when the ray reached it (Unmapped or Empty), stamp the
+                // table blocks they read
+                if ((gl < kend || (gl == kend && !done)) && fresh) {
+#pragma unroll
+                    for (int i = 0; i < GC_MAX_TABLES; ++i)
+                        if (tblk[i] != NONE) p.pt.usage[i][tblk[i]] = ts;
+                }
+            }
             auto store_state = [&](float tt) {
                 if (gl == 0) {
                     gp(V.tbuf)[pix] = tt;
@@ -1510,6 +1583,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 active = false;
             } else if (kend == (uint32_t)G) {
                 cell_last = cell_k;
+                if constexpr (PT) key_last = key_k;
             } else {
                 // sample kend is reached and not mapped
                 if (gl == 0 && ((m_fresh >> kend) & 1ull)) ++n_visits;
@@ -1525,6 +1599,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 } else {
                     // Empty brick: advance by tstep from it (RayCasters.cu:154-159)
                     cell_last = cell_k;
+                    if constexpr (PT) key_last = key_k;
                     t = t_k + tstep;
                     pos = f3(px_k, py_k, pz_k) + dir * tstep;
                     lod = lod_k;
@@ -1726,9 +1801,9 @@ __global__ __launch_bounds__(256) void k_mop(const DvrLaunch p)
                 const F3 np = f3(clamp099(tp.x), clamp099(tp.y), clamp099(tp.z));
                 const F3 pr = np * f3(L.ior[0], L.ior[1], L.ior[2]);
                 const F3 pv = pr * f3(L.rdv[0], L.rdv[1], L.rdv[2]);
-                uint32_t bx = 0, by = 0, bz = 0;   // (with table caches the voxel below takes v mod inner)
+                uint32_t bx = 0, by = 0, bz = 0;
                 if (p.pt.n) {
-                    const PtKey k = pt_key(p.pt, plv[lod], pv);
+                    const PtKey k = pt_key(p.pt, plv[lod], pv, bx, by, bz);
                     if (!pt_same(k, pkp)) {
                         pkp = k;
                         const uint32_t w = pt_walk(p, k, ts);
@@ -1987,7 +2062,8 @@ uint32_t dvr_ray_lds_bytes(uint32_t n_views, uint32_t tf_entries)
 
 bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes)
 {
-    return p.pipelined != 0 && p.pt.n == 0 && (bytes == 1 || bytes == 2);
+    // the pipelined march's configurations, or table caches with covering >= 1
+    return (p.pt.n ? p.covering >= 1 : p.pipelined != 0) && (bytes == 1 || bytes == 2);
 }
 
 // Lanes per ray: a whole wave for the shortest lists (lowest latency per
@@ -1995,8 +2071,15 @@ bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes)
 uint32_t dvr_ray_group(uint32_t rays) { return rays <= 256u ? 64u : 16u; }
 
 template <int BYTES, typename ACC>
-static const void *ray_kernel_of(uint32_t group, bool bs32)
+static const void *ray_kernel_of(uint32_t group, bool bs32, bool pt)
 {
+    if (pt) {
+        switch (group) {
+        case 16: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 16, 0, true>);
+        case 64: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 64, 0, true>);
+        default: return nullptr;
+        }
+    }
     if (sizeof(ACC) == 4 && bs32) {
         switch (group) {
         case 16: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 16, 32>);
@@ -2011,11 +2094,11 @@ static const void *ray_kernel_of(uint32_t group, bool bs32)
     }
 }
 
-static const void *ray_kernel(uint32_t bytes, uint32_t precision, uint32_t group, bool bs32 = false)
+static const void *ray_kernel(uint32_t bytes, uint32_t precision, uint32_t group, bool bs32 = false, bool pt = false)
 {
     const bool d = precision == 64;
-    if (bytes == 1) return d ? ray_kernel_of<1, double>(group, bs32) : ray_kernel_of<1, float>(group, bs32);
-    return d ? ray_kernel_of<2, double>(group, bs32) : ray_kernel_of<2, float>(group, bs32);
+    if (bytes == 1) return d ? ray_kernel_of<1, double>(group, bs32, pt) : ray_kernel_of<1, float>(group, bs32, pt);
+    return d ? ray_kernel_of<2, double>(group, bs32, pt) : ray_kernel_of<2, float>(group, bs32, pt);
 }
 
 uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t lds_bytes, uint32_t group)
@@ -2066,7 +2149,8 @@ hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision
     if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
     if (group == 0) group = dvr_ray_group(rays);
-    const void *k = ray_kernel(bytes, precision, group, p.brick_size == 32 && p.covering == 1);
+    const void *k = ray_kernel(bytes, precision, group, p.pt.n == 0 && p.brick_size == 32 && p.covering == 1,
+                               p.pt.n != 0);
     if (!k) return hipErrorInvalidValue;
     void *args[] = {const_cast<DvrLaunch *>(&p)};
     return hipLaunchKernel(k, dim3(grid), dim3(LANES), args, dvr_ray_lds_bytes(p.views ? p.n_views : 1, p.tf_lds_entries),

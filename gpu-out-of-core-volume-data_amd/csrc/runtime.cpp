@@ -26,6 +26,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -761,6 +762,8 @@ struct Cache {
     TableState tabs{};
     uint32_t *tab_order_tmp[GC_MAX_TABLES] = {};
     uint32_t *d_unref = nullptr;        // bricks the last load left unreferenced
+    RefScratch refs{};                  // parallel referencing (one table)
+    uint32_t *h_ref = nullptr;          // pinned: cap x (cell, loc, rep)
     // host staging
     uint32_t *h_misses = nullptr;       // pinned: [total, pad x3, cap x uint4]
     uint8_t *h_staging = nullptr;       // pinned: cap big bricks
@@ -784,6 +787,7 @@ struct Cache {
     int load_new_bricks();
     int update(int *done);
     int insert_serial_needed(const uint32_t *asked, uint32_t n, int &serial) const;
+    int reference_plan(const uint32_t *asked, uint32_t n, bool &parallel);
 };
 
 void Cache::release()
@@ -804,6 +808,10 @@ void Cache::release()
     }
     hipFree(d_unref);
     d_unref = nullptr;
+    hipFree(refs.cell); hipFree(refs.w0); hipFree(refs.inv); hipFree(refs.flag);
+    refs = RefScratch{};
+    if (h_ref) hipHostFree(h_ref);
+    h_ref = nullptr;
     if (h_misses) hipHostFree(h_misses);
     if (h_staging) hipHostFree(h_staging);
     if (h_meta) hipHostFree(h_meta);
@@ -930,6 +938,17 @@ int Cache::create()
     GC_TRY(dalloc(&d_staging, (size_t)cap * src->bb_bytes));
     GC_TRY(halloc(&h_meta, (size_t)cap * 9 * 4));
     GC_TRY(dalloc(&d_meta, (size_t)cap * 9 * 4));
+    if (g.nb_tables == 1) {
+        GC_TRY(dalloc(&refs.cell, (size_t)cap * 3 * 4));
+        refs.loc = refs.cell + cap;
+        refs.rep = refs.cell + 2 * (size_t)cap;
+        GC_TRY(dalloc(&refs.w0, (size_t)cap * 3 * 4));
+        refs.rank = refs.w0 + cap;
+        refs.ev = refs.w0 + 2 * (size_t)cap;
+        GC_TRY(dalloc(&refs.inv, (size_t)g.table[0].nblocks * 4));
+        GC_TRY(dalloc(&refs.flag, 16));
+        GC_TRY(halloc(&h_ref, (size_t)cap * 3 * 4));
+    }
     handler.c = this;
     if (!cc.sync_requests) handler.start();
     return GC_OK;
@@ -994,6 +1013,35 @@ int Cache::insert_serial_needed(const uint32_t *asked, uint32_t n, int &serial) 
     return GC_OK;
 }
 
+int Cache::reference_plan(const uint32_t *asked, uint32_t n, bool &parallel)
+{
+    // Per brick the MRPD cell, the entry inside its table block and the first
+    // brick of the load in the same cell, for the parallel referencing; two
+    // bricks on one entry (FP32 positions) take the serial one.
+    parallel = false;
+    std::unordered_map<uint32_t, uint32_t> first;
+    std::unordered_set<uint64_t> entries;
+    first.reserve(n * 2);
+    entries.reserve(n * 2);
+    for (uint32_t k = 0; k < n; ++k) {
+        float vol[3];
+        volume_position(g, asked[4 * k + 3], asked + 4 * k, vol);
+        const uint32_t c = top_index(g, asked[4 * k + 3], vol);
+        const uint32_t l = table_local(g.table[0], asked[4 * k + 3], vol);
+        if (c >= g.top_cells || l >= table_b3(g.table[0])) return GC_OK;   // (never: positions lie in [0, 1))
+        if (!entries.insert((uint64_t)c << 32 | l).second) return GC_OK;
+        h_ref[k] = c;
+        h_ref[cap + k] = l;
+        h_ref[2 * (size_t)cap + k] = first.emplace(c, k).first->second;
+    }
+    // (h_ref is rewritten only at the next load, after a synchronised update)
+    for (int a = 0; a < 3; ++a)
+        GC_HIP(hipMemcpyAsync(refs.cell + (size_t)a * cap, h_ref + (size_t)a * cap, (size_t)n * 4,
+                              hipMemcpyHostToDevice, stream));
+    parallel = true;
+    return GC_OK;
+}
+
 int Cache::load_new_bricks()
 {
     // CacheManager.hpp:503-632
@@ -1009,8 +1057,14 @@ int Cache::load_new_bricks()
         GC_HIP(launch_deref_tables(cells, tabs, order, g.nslots, nne, meta, d_geom, ts, stream));
         GC_HIP(launch_assign_copy(order, g.nslots, nne, d_asked, d_bbi, d_bbc, meta, pool, d_staging, d_geom,
                                   vc.big_brick_size, vc.encoded_bytes, stream));
-        GC_HIP(launch_reference_tables(mrpd, cells, tabs, order, g.nslots, n, nne, d_asked, meta, d_geom, ts, d_unref,
-                                       stream));
+        bool parallel = false;
+        if (g.nb_tables == 1) GC_TRY(reference_plan(h_meta, n, parallel));
+        if (parallel)
+            GC_HIP(launch_reference_table1(mrpd, cells, tabs, order, g.nslots, n, nne, d_asked, meta, d_geom, ts,
+                                           d_unref, refs, stream, g.table[0].nblocks));
+        else
+            GC_HIP(launch_reference_tables(mrpd, cells, tabs, order, g.nslots, n, nne, d_asked, meta, d_geom, ts,
+                                           d_unref, stream));
     } else {
         GC_HIP(launch_deref(mrpd, cells, order, g.nslots, nne, meta, d_geom, stream));
         GC_HIP(launch_assign_copy(order, g.nslots, nne, d_asked, d_bbi, d_bbc, meta, pool, d_staging, d_geom,
@@ -1316,7 +1370,11 @@ static void launch_base(const Cache &C, DvrLaunch &L)
     L.pt.top_x = C.g.top_cells_dim[0];
     L.pt.top_y = C.g.top_cells_dim[1];
     for (uint32_t i = 0; i < C.g.nb_tables; ++i) {
-        for (int a = 0; a < 3; ++a) L.pt.block[i][a] = C.g.table[i].block[a];
+        L.pt.pow2[i] = 1;
+        for (int a = 0; a < 3; ++a) {
+            L.pt.block[i][a] = C.g.table[i].block[a];
+            if (L.pt.block[i][a] & (L.pt.block[i][a] - 1)) L.pt.pow2[i] = 0;
+        }
         L.pt.b3[i] = table_b3(C.g.table[i]);
         L.pt.words[i] = C.tabs.words[i];
         L.pt.usage[i] = C.tabs.usage[i];

@@ -293,3 +293,44 @@ def test_batch_with_tables_matches_flat(gpu):
     assert ra.complete and rb.complete and ra.launches == rb.launches
     for i in range(3):
         assert np.array_equal(ba.pixels(i), bb.pixels(i)), i
+
+
+@pytest.mark.parametrize("bytes_,prec,tables", [(1, 32, ONE_ODD), (2, 64, ONE_SMALL), (1, 32, TWO)])
+def test_ray_list_kernel_with_tables_matches_lane_kernel(gpu, bytes_, prec, tables):
+    """Resumed launches through table caches, one lane per ray or one wave
+    per ray: bit-identical frames, counters and cache states (tables
+    included) after every frame."""
+    sc = Scene((96, 80, 64), 8, 1, bytes_, cache_size=(9, 9, 8), max_requests=64, tables=tables)
+    caches = [sc.gpu_cache(), sc.gpu_cache()]
+    G.set_ray_list_max(caches[0], 0)
+    G.set_ray_list_max(caches[1], 1 << 30)
+    tf = tf_table(seed=23)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 80, 64
+    for k in range(3):
+        out = []
+        for c in caches:
+            f = G.Frame(W, H, prec)
+            r = G.render_frame(c, G.make_view(W, H, orbit(2 * k, distance=2.2), alpha_stop=0.95 if k % 2 else 0.99,
+                                              precision=prec), d_tf, len(tf), f, max_launches=60)
+            tb, cs = f.state()
+            out.append((f.pixels().copy(), tb.copy(), cs.copy(), r.as_dict()))
+            f.close()
+        (p0, t0, c0, r0), (p1, t1, c1, r1) = out
+        print("frame", k, "launches", r0["launches"], "lane launches", r1["lane_launches"], "samples",
+              r0["mapped_samples"], "complete", r0["complete"], flush=True)
+        assert r0["launches"] > 1
+        for key in ("launches", "complete", "mapped_samples", "brick_visits", "pixels", "unmapped_rays",
+                    "bricks_loaded"):
+            assert r0[key] == r1[key], (k, key, r0[key], r1[key])
+        assert np.array_equal(p0, p1), "pixels differ at frame %d" % k
+        assert np.array_equal(t0, t1) and np.array_equal(c0.view(np.uint8), c1.view(np.uint8)), k
+        a, b = caches
+        assert np.array_equal(a.mrpd(), b.mrpd()) and np.array_equal(a.lru(), b.lru()), k
+        assert np.array_equal(a.usage(), b.usage()) and np.array_equal(a.request(), b.request()), k
+        for i in range(len(tables)):
+            assert np.array_equal(a.table(i), b.table(i)), (k, i)
+            assert np.array_equal(a.table_usage(i), b.table_usage(i)), (k, i)
+            assert np.array_equal(a.table_lru(i), b.table_lru(i)), (k, i)
+    for c in caches:
+        c.close()
