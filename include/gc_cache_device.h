@@ -20,6 +20,10 @@
  *       Mapped: the texel at e.  Unmapped: raises the request for the asked
  *       level, then tries the next coarser levels; Unmapped (out = 0) if no
  *       level is mapped.  Empty: out is left unchanged.
+ *
+ * With page-table caches (CacheSize[1..]) both walk from the MRPD entry
+ * through the tables (:206-231,310-339), stamping each table block they read;
+ * an Unmapped entry at any level of the walk counts as Unmapped.
  */
 #ifndef GC_CACHE_DEVICE_H
 #define GC_CACHE_DEVICE_H

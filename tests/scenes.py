@@ -68,6 +68,15 @@ class Scene:
         return [G.DeviceBuffer.from_array(l) for l in self.levels]
 
 
+def unit_float(sc):
+    """A float scene's volume scaled to [0, 1] (the transfer function's
+    domain for float voxels), pyramid rebuilt; returns the scene."""
+    assert sc.bytes == 4
+    sc.vol = (sc.vol / np.float32(sc.vol.max())).astype(np.float32)
+    sc.levels = pyramid(sc.vol, sc.vc.nb_levels)
+    return sc
+
+
 def tf_table(seed=5, n=256):
     return G.tf_samples(G.tf_seeded(seed, 6), n)
 

@@ -14,7 +14,7 @@ import pytest
 
 import gcvol as G
 import oracle_lib as O
-from scenes import Scene, orbit, tf_table
+from scenes import Scene, orbit, tf_table, unit_float
 
 pytestmark = pytest.mark.gpu
 
@@ -115,13 +115,15 @@ def test_resident_load_and_render(gpu, bytes_, parity_log):
             assert np.array_equal(gc.request(), oc.request())
 
 
-@pytest.mark.parametrize("bytes_,tile,prec", [(1, 0, 64), (2, 2, 64), (1, 0, 32), (2, 1, 32)])
+@pytest.mark.parametrize("bytes_,tile,prec", [(1, 0, 64), (2, 2, 64), (1, 0, 32), (2, 1, 32), (4, 0, 64), (4, 1, 32)])
 def test_out_of_core_sequence(gpu, bytes_, tile, prec, parity_log):
     """Progressive frames with a pool smaller than the volume: every update's
     LRU, page directory, miss list, pool and buffers match the oracle, and
     every launch's pixels are within 1/255 of the FP64 oracle over the same
     cache state (FP32 or FP64 compositing on the device)."""
     sc = Scene((64, 64, 64), 8, 1, bytes_, cache_size=(6, 5, 5), max_requests=50)
+    if bytes_ == 4:
+        unit_float(sc)
     gc, oc = sc.gpu_cache(), sc.oracle_cache()
     tf = tf_table(seed=11)
     d_tf = G.DeviceBuffer.from_array(tf)

@@ -13,7 +13,7 @@ import pytest
 
 import gcvol as G
 import oracle_lib as O
-from scenes import Scene, orbit, tf_table
+from scenes import Scene, orbit, tf_table, unit_float
 
 pytestmark = pytest.mark.gpu
 
@@ -63,11 +63,14 @@ def _render_pair(sc, gc, oc, cam, prec, W, H, tf, d_tf, frame, otb, ocs, composi
     return gst, G.unpack_rgba(frame.pixels().reshape(-1)), ost, G.unpack_rgba(opx)
 
 
-@pytest.mark.parametrize("bytes_,tables,prec,tile", [
-    (1, ONE_SMALL, 64, 0), (2, ONE_ODD, 64, 1), (1, TWO, 64, 2), (1, ONE_SMALL, 32, 0), (2, TWO, 32, 3)])
-def test_out_of_core_sequence_with_tables(gpu, bytes_, tables, prec, tile, parity_log):
-    print("tables", tables, "bytes", bytes_, "precision", prec, "tile", tile, flush=True)
-    sc = Scene((64, 64, 64), 8, 1, bytes_, cache_size=(6, 5, 5), max_requests=50, tables=tables)
+@pytest.mark.parametrize("bytes_,tables,prec,tile,cov", [
+    (1, ONE_SMALL, 64, 0, 1), (2, ONE_ODD, 64, 1, 1), (1, TWO, 64, 2, 1), (1, ONE_SMALL, 32, 0, 1),
+    (2, TWO, 32, 3, 1), (4, ONE_SMALL, 64, 0, 1), (4, ONE_ODD, 32, 0, 1), (1, ONE_SMALL, 64, 0, 0)])
+def test_out_of_core_sequence_with_tables(gpu, bytes_, tables, prec, tile, cov, parity_log):
+    print("tables", tables, "bytes", bytes_, "precision", prec, "tile", tile, "covering", cov, flush=True)
+    sc = Scene((64, 64, 64), 8, cov, bytes_, cache_size=(6, 5, 5), max_requests=50, tables=tables)
+    if bytes_ == 4:
+        unit_float(sc)
     gc, oc = sc.gpu_cache(), sc.oracle_cache()
     assert gc.nb_tables == len(tables)
     tf = tf_table(seed=11)
