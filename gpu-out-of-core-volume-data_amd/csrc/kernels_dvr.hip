@@ -1920,6 +1920,8 @@ const void *kernel_of(uint32_t tile, bool mt, bool bs32, bool pt)
     // table caches: one variant per packet shape (transfer functions from
     // HBM when a launch has several)
     if (pt) {
+        if (BYTES != 4 && sizeof(ACC) == 4 && (tile & 3) == 0 && bs32)
+            return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 32, true>);
         switch (tile & 3) {
         case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 0, true>);
         case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4, false, 0, true>);
@@ -2046,7 +2048,8 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
         DvrLaunch q = p;
         if (q.tf_lds_tables > 1) q.tf_lds_entries = q.tf_lds_tables = 0;
         void *args[] = {&q};
-        return hipLaunchKernel(dvr_kernel(bytes, precision, tile, false, false, true), dim3(grid), dim3(LANES), args,
+        return hipLaunchKernel(dvr_kernel(bytes, precision, tile, false, q.brick_size == 32 && q.covering == 1, true),
+                               dim3(grid), dim3(LANES), args,
                                dvr_lds_bytes(q.nb_lods, q.views ? q.n_views : 1, q.tf_lds_entries), stream);
     }
     void *args[] = {const_cast<DvrLaunch *>(&p)};
@@ -2074,6 +2077,13 @@ template <int BYTES, typename ACC>
 static const void *ray_kernel_of(uint32_t group, bool bs32, bool pt)
 {
     if (pt) {
+        if (sizeof(ACC) == 4 && bs32) {
+            switch (group) {
+            case 16: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 16, 32, true>);
+            case 64: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 64, 32, true>);
+            default: return nullptr;
+            }
+        }
         switch (group) {
         case 16: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 16, 0, true>);
         case 64: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 64, 0, true>);
@@ -2149,8 +2159,7 @@ hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision
     if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
     if (group == 0) group = dvr_ray_group(rays);
-    const void *k = ray_kernel(bytes, precision, group, p.pt.n == 0 && p.brick_size == 32 && p.covering == 1,
-                               p.pt.n != 0);
+    const void *k = ray_kernel(bytes, precision, group, p.brick_size == 32 && p.covering == 1, p.pt.n != 0);
     if (!k) return hipErrorInvalidValue;
     void *args[] = {const_cast<DvrLaunch *>(&p)};
     return hipLaunchKernel(k, dim3(grid), dim3(LANES), args, dvr_ray_lds_bytes(p.views ? p.n_views : 1, p.tf_lds_entries),

@@ -337,3 +337,48 @@ def test_ray_list_kernel_with_tables_matches_lane_kernel(gpu, bytes_, prec, tabl
             assert np.array_equal(a.table_lru(i), b.table_lru(i)), (k, i)
     for c in caches:
         c.close()
+
+
+@pytest.mark.parametrize("bytes_", [1, 2])
+def test_brick32_with_tables_fp32_against_oracle(gpu, bytes_, parity_log):
+    """Bricks of 32 with covering 1 through a table cache (the constant-stride
+    variants) at FP32, out of core: state bit-exact, counters and t buffer per
+    launch, RGBA within 1/255 of the FP64 oracle; then the frame driver's
+    resumed lists (wave-per-ray) equal to the lane march."""
+    tables = [((4, 4, 2), (2, 2, 2))]
+    sc = Scene((160, 128, 96), 32, 1, bytes_, cache_size=(5, 4, 4), max_requests=40, tables=tables, seed=31)
+    gc, oc = sc.gpu_cache(), sc.oracle_cache()
+    tf = tf_table(seed=9)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 96, 80
+    frame = G.Frame(W, H, 32)
+    otb = np.zeros(W * H, np.float32)
+    ocs = np.zeros(W * H * 4)
+    errs = []
+    for it in range(40):
+        gst, g, ost, o = _render_pair(sc, gc, oc, orbit(1), 32, W, H, tf, d_tf, frame, otb, ocs)
+        assert gst.tolist() == ost.tolist(), it
+        errs.append(np.abs(g - o))
+        assert errs[-1].max() <= 1
+        assert np.array_equal(frame.state()[0].reshape(-1), otb), it
+        gd, od = gc.update(), oc.update()
+        assert gd == od
+        _compare(gc, oc, pool=(it % 6 == 0))
+        if gd:
+            break
+    rec = parity_log(32, np.concatenate(errs), launches=it + 1, bytes=bytes_, tables=1, brick=32)
+    print("brick32 tables launches", it + 1, "max err", rec["max_err"], "rmse %.4f" % rec["rmse"], flush=True)
+    assert it > 1
+    # resumed lists: lane march vs wave-per-ray march, bit-identical
+    caches = [sc.gpu_cache(), sc.gpu_cache()]
+    G.set_ray_list_max(caches[0], 0)
+    G.set_ray_list_max(caches[1], 1 << 30)
+    out = []
+    for c in caches:
+        f = G.Frame(W, H, 32)
+        r = G.render_frame(c, G.make_view(W, H, orbit(3), precision=32), d_tf, len(tf), f, max_launches=60)
+        out.append((f.pixels().copy(), f.state()[0].copy(), r.as_dict()))
+    assert out[0][2]["launches"] == out[1][2]["launches"] and out[0][2]["mapped_samples"] == out[1][2]["mapped_samples"]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(caches[0].table(0), caches[1].table(0))
+    assert np.array_equal(caches[0].table_usage(0), caches[1].table_usage(0))
