@@ -21,6 +21,7 @@ pytestmark = pytest.mark.gpu
 ONE_SMALL = [((3, 2, 2), (2, 2, 2))]
 ONE_ODD = [((4, 3, 2), (3, 2, 2))]
 TWO = [((4, 3, 3), (2, 2, 2)), ((2, 2, 1), (2, 1, 2))]
+THREE = [((4, 3, 3), (2, 2, 1)), ((3, 2, 2), (1, 2, 2)), ((2, 2, 1), (2, 1, 1))]
 
 
 def _check_cells(gc):
@@ -65,7 +66,8 @@ def _render_pair(sc, gc, oc, cam, prec, W, H, tf, d_tf, frame, otb, ocs, composi
 
 @pytest.mark.parametrize("bytes_,tables,prec,tile,cov", [
     (1, ONE_SMALL, 64, 0, 1), (2, ONE_ODD, 64, 1, 1), (1, TWO, 64, 2, 1), (1, ONE_SMALL, 32, 0, 1),
-    (2, TWO, 32, 3, 1), (4, ONE_SMALL, 64, 0, 1), (4, ONE_ODD, 32, 0, 1), (1, ONE_SMALL, 64, 0, 0)])
+    (2, TWO, 32, 3, 1), (4, ONE_SMALL, 64, 0, 1), (4, ONE_ODD, 32, 0, 1), (1, ONE_SMALL, 64, 0, 0),
+    (1, THREE, 64, 0, 1), (2, THREE, 32, 1, 1)])
 def test_out_of_core_sequence_with_tables(gpu, bytes_, tables, prec, tile, cov, parity_log):
     print("tables", tables, "bytes", bytes_, "precision", prec, "tile", tile, "covering", cov, flush=True)
     sc = Scene((64, 64, 64), 8, cov, bytes_, cache_size=(6, 5, 5), max_requests=50, tables=tables)
@@ -128,7 +130,7 @@ def test_maximum_projection_with_tables(gpu, bytes_):
     assert it > 1
 
 
-@pytest.mark.parametrize("tables", [ONE_SMALL, TWO])
+@pytest.mark.parametrize("tables", [ONE_SMALL, TWO, THREE])
 def test_random_streams_lockstep(gpu, tables):
     """Random request and usage streams (the loads' inputs) on both caches:
     the load kernels' results equal the oracle's after every update."""
@@ -298,7 +300,7 @@ def test_batch_with_tables_matches_flat(gpu):
         assert np.array_equal(ba.pixels(i), bb.pixels(i)), i
 
 
-@pytest.mark.parametrize("bytes_,prec,tables", [(1, 32, ONE_ODD), (2, 64, ONE_SMALL), (1, 32, TWO)])
+@pytest.mark.parametrize("bytes_,prec,tables", [(1, 32, ONE_ODD), (2, 64, ONE_SMALL), (1, 32, TWO), (2, 32, THREE)])
 def test_ray_list_kernel_with_tables_matches_lane_kernel(gpu, bytes_, prec, tables):
     """Resumed launches through table caches, one lane per ray or one wave
     per ray: bit-identical frames, counters and cache states (tables

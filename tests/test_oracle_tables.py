@@ -104,7 +104,8 @@ def _drive(caches, view, steps=60, probe_every=0, rng=None):
             return
 
 
-@pytest.mark.parametrize("blocks,bb", [([(2, 2, 2)], (2, 2, 2)), ([(2, 2, 1), (2, 1, 2)], (4, 4, 4))])
+@pytest.mark.parametrize("blocks,bb", [([(2, 2, 2)], (2, 2, 2)), ([(2, 2, 1), (2, 1, 2)], (4, 4, 4)),
+                                       ([(2, 1, 1), (1, 2, 1), (1, 1, 2)], (2, 2, 2))])
 def test_transparent_tables_change_nothing(oracle, blocks, bb):
     kw = dict(dims=(72, 56, 40), bs=8, cov=1, bytes_=1, cache_size=(5, 4, 3), max_requests=17, seed=11,
               big_brick=bb, nb_blobs=5, sigma=(3.0, 9.0))
@@ -269,6 +270,7 @@ def _check_tables(sc, c, rng):
     [((2, 1, 1), (2, 2, 2))],
     [((3, 2, 1), (2, 2, 1))],
     [((3, 3, 2), (2, 2, 2)), ((2, 1, 1), (2, 1, 2))],
+    [((4, 3, 3), (2, 2, 1)), ((3, 2, 2), (1, 2, 2)), ((2, 2, 1), (2, 1, 1))],
 ])
 def test_random_stream_invariants(oracle, tables):
     sc = Scene((64, 48, 40), 8, 1, 1, cache_size=(5, 3, 3), max_requests=23, seed=5, nb_blobs=4,
