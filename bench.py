@@ -449,6 +449,12 @@ def serving_workload(G, rates=(4, 8, 16, 32, 64), seconds=4.0, max_views=4, warm
     while len(warm) < warm_requests:
         warm += make_requests(8.0, SEED + 299 + len(warm))
     warm = warm[:warm_requests]
+    # the warm burst reaches every volume, so that no cache is created inside
+    # the timed window of either loop
+    seen = {w[1] for w in warm}
+    for vol in range(len(SERVE_VOLUMES)):
+        if vol not in seen and (volumes is None or vol in volumes):
+            warm.append((0.0, vol, 256, G.camera(15.0, 30.0, 3.0), 0))
     out = []
     for i, rate in enumerate(rates):
         reqs = make_requests(rate, SEED + 300 + i, count)

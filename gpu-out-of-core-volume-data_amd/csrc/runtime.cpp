@@ -934,7 +934,9 @@ int Cache::create()
     GC_TRY(dalloc(&d_misses, (size_t)cap * 16));
     GC_TRY(dalloc(&d_count, 16));
     GC_TRY(halloc(&h_misses, 16 + (size_t)cap * 16));
-    GC_TRY(halloc(&h_staging, (size_t)cap * src->bb_bytes));
+    // (sources that write bricks on the device fill the device staging
+    // buffer directly: no host staging)
+    if (!src->on_device()) GC_TRY(halloc(&h_staging, (size_t)cap * src->bb_bytes));
     GC_TRY(dalloc(&d_staging, (size_t)cap * src->bb_bytes));
     GC_TRY(halloc(&h_meta, (size_t)cap * 9 * 4));
     GC_TRY(dalloc(&d_meta, (size_t)cap * 9 * 4));
