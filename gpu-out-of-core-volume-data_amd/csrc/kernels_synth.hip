@@ -100,6 +100,93 @@ __global__ __launch_bounds__(256) void k_synth_bricks(SynthLaunch p)
     if (threadIdx.x == 0) p.below[u] = above ? 0u : 1u;
 }
 
+// Bricks whose size is a multiple of 4: each lane writes four consecutive
+// voxels of a row.  Per blob the row weight w_k = (amp_k fz_k(z)) * fy_k(y) is
+// formed once for the four voxels and the four x factors come in one 16-byte
+// LDS read, so a blob costs a lane three LDS reads per four voxels instead of
+// three per voxel (the per-voxel kernel above is bound by LDS issue).  Every
+// voxel still sums fma(w_k, fx_k(x), sum) in blob order from the same
+// products, so the bytes are those of k_synth_bricks.
+template <typename T>
+__global__ __launch_bounds__(256) void k_synth_bricks_x4(SynthLaunch p)
+{
+    extern __shared__ __align__(16) float sf4[];
+    const uint32_t u = blockIdx.x;
+    const uint32_t *key = p.keys + 4ull * u;
+    const uint32_t level = key[3], bs = p.bs, nb = p.nb, inner = bs - 2 * p.cov;
+    if (level > p.max_level) return;   // the whole workgroup: another writer's brick
+    const int64_t o0 = (int64_t)key[0] * inner - p.cov, o1 = (int64_t)key[1] * inner - p.cov,
+                  o2 = (int64_t)key[2] * inner - p.cov;
+    const uint32_t nf = nb * bs;
+    for (uint32_t i = threadIdx.x; i < 3 * nf; i += blockDim.x) {
+        const uint32_t a = i / nf, r = i - a * nf, k = r / bs, j = r - k * bs;
+        const SynthAxis ax = p.axis[level][a];
+        const int64_t c = (a == 0 ? o0 : (a == 1 ? o1 : o2)) + j;
+        float f = 0.f;
+        if (c >= ax.lo && c < ax.lo + (int64_t)ax.len) f = p.tab[ax.off + (uint64_t)k * ax.len + (uint64_t)(c - ax.lo)];
+        if (a == 2) f = p.amp[k] * f;
+        sf4[i] = f;
+    }
+    __syncthreads();
+    const float *fx = sf4, *fy = sf4 + nf, *fz = sf4 + 2 * nf;
+    const int64_t e0 = p.ext[level][0], e1 = p.ext[level][1], e2 = p.ext[level][2];
+    const uint32_t span = 2u * p.noise + 1u;
+    const uint32_t qx = bs / 4u, nq = qx * bs * bs;
+    T *out = reinterpret_cast<T *>(p.dst + (uint64_t)u * bs * bs * bs * sizeof(T));
+    int above = 0;
+    for (uint32_t q = threadIdx.x; q < nq; q += blockDim.x) {
+        const uint32_t x = (q % qx) * 4u, r = q / qx, y = r % bs, z = r / bs;
+        const int64_t gx0 = o0 + x, gy = o1 + y, gz = o2 + z;
+        const bool yz_in = gy >= 0 && gy < e1 && gz >= 0 && gz < e2;
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+        if (yz_in) {
+            for (uint32_t k = 0; k < nb; ++k) {
+                const float w = fz[k * bs + z] * fy[k * bs + y];
+                const float4 f = *reinterpret_cast<const float4 *>(fx + k * bs + x);
+                c0 = __builtin_fmaf(w, f.x, c0);
+                c1 = __builtin_fmaf(w, f.y, c1);
+                c2 = __builtin_fmaf(w, f.z, c2);
+                c3 = __builtin_fmaf(w, f.w, c3);
+            }
+        }
+        const uint32_t rk = p.seed ^ (level * 0x27d4eb2fu) ^ ((uint32_t)gz * 0x165667b1u) ^ ((uint32_t)gy * 0xc2b2ae3du);
+        const float cs[4] = {c0, c1, c2, c3};
+        T vals[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t gx = gx0 + i;
+            float val = 0.f;
+            if (yz_in && gx >= 0 && gx < e0) {
+                int32_t nz = 0;
+                if (p.noise) {
+                    const uint32_t h = synth_hash32(rk ^ ((uint32_t)gx * 0x9e3779b1u));
+                    nz = (int32_t)(uint32_t)(((uint64_t)h * span) >> 32) - (int32_t)p.noise;
+                }
+                float rr = (float)(int32_t)(cs[i] + 0.5f) + (float)nz;
+                rr = rr < 0.f ? 0.f : rr;
+                rr = rr > p.fmax ? p.fmax : rr;
+                val = rr;
+            }
+            above |= !(val < p.thr);
+            if constexpr (sizeof(T) < 4) vals[i] = (T)(int32_t)val;
+            else vals[i] = val;
+        }
+        // four consecutive voxels, one aligned store (x and bs are multiples of 4)
+        T *o = out + x + (uint64_t)bs * (y + (uint64_t)bs * z);
+        if constexpr (sizeof(T) == 1) {
+            *reinterpret_cast<uint32_t *>(o) = (uint32_t)vals[0] | ((uint32_t)vals[1] << 8) |
+                                               ((uint32_t)vals[2] << 16) | ((uint32_t)vals[3] << 24);
+        } else if constexpr (sizeof(T) == 2) {
+            *reinterpret_cast<uint2 *>(o) = make_uint2((uint32_t)vals[0] | ((uint32_t)vals[1] << 16),
+                                                       (uint32_t)vals[2] | ((uint32_t)vals[3] << 16));
+        } else {
+            *reinterpret_cast<float4 *>(o) = make_float4(vals[0], vals[1], vals[2], vals[3]);
+        }
+    }
+    above = __syncthreads_or(above);
+    if (threadIdx.x == 0) p.below[u] = above ? 0u : 1u;
+}
+
 // Grid: (ceil(n[0] / 256), n[1], n[2]) workgroups of 256 lanes, one row
 // segment each.  Dynamic LDS: nb x 256 x factors of the segment, then the nb
 // row weights w_k = (amp_k fz_k(z)) * fy_k(y), the products the brick kernel
@@ -166,6 +253,12 @@ hipError_t launch_synth_bricks(const SynthLaunch &p, hipStream_t s)
 {
     if (p.n == 0) return hipSuccess;
     const size_t lds = (size_t)3 * p.nb * p.bs * sizeof(float);
+    if (p.bs % 4u == 0) {
+        if (p.bytes == 1) k_synth_bricks_x4<uint8_t><<<p.n, 256, lds, s>>>(p);
+        else if (p.bytes == 2) k_synth_bricks_x4<uint16_t><<<p.n, 256, lds, s>>>(p);
+        else k_synth_bricks_x4<float><<<p.n, 256, lds, s>>>(p);
+        return hipGetLastError();
+    }
     if (p.bytes == 1) k_synth_bricks<uint8_t><<<p.n, 256, lds, s>>>(p);
     else if (p.bytes == 2) k_synth_bricks<uint16_t><<<p.n, 256, lds, s>>>(p);
     else k_synth_bricks<float><<<p.n, 256, lds, s>>>(p);

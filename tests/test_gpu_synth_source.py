@@ -20,6 +20,8 @@ CASES = [
     ((96, 80, 64), 32, 1, 2, 64, 2048),
     ((40, 36, 30), 8, 2, 4, 5, 0),
     ((65, 64, 63), 16, 1, 2, 48, 0),
+    # a brick size that is not a multiple of 4 (the per-voxel kernel)
+    ((50, 41, 30), 10, 1, 1, 7, 8),
 ]
 
 
