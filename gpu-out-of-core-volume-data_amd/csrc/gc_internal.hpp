@@ -416,6 +416,11 @@ uint32_t dvr_ray_lds_bytes(uint32_t n_views, uint32_t tf_entries);
 uint32_t dvr_assign_tf_lds(DvrView *views, uint32_t n, uint32_t bytes, uint32_t *tables);
 // group-per-ray march: group = lanes per ray (16 or 64; 0 = by the list
 // length: 64 up to 256 rays, else 16)
+// Resumed-ray lists of up to RAY_G16_MAX, RAY_G8_MAX and RAY_G4_MAX rays run
+// the group-per-ray march with 16, 8 and 4 lanes per ray (64 up to 256 rays);
+// RAY_G4_MAX is the default longest list of the group march.  Captured
+// batch graphs keep 16 lanes and RAY_G16_MAX.
+constexpr uint32_t RAY_G16_MAX = 1u << 16, RAY_G8_MAX = 1u << 17, RAY_G4_MAX = 1u << 18;
 uint32_t dvr_ray_group(uint32_t rays);
 uint32_t dvr_ray_grid(uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t lds_bytes, uint32_t group = 0);
 hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision, uint32_t rays, uint32_t grid,

@@ -1267,7 +1267,7 @@ __device__ inline ACC lane_read(ACC x, uint32_t k)
 template <int BYTES, typename ACC, int G, int BSC = 0, bool PT = false>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : 4))) void k_dvr_ray(const DvrLaunch p)
 {
-    static_assert(G == 16 || G == 32 || G == 64, "16, 32 or 64 lanes per ray");
+    static_assert(G == 4 || G == 8 || G == 16 || G == 32 || G == 64, "4, 8, 16, 32 or 64 lanes per ray");
     constexpr uint32_t R = 64 / G;   // rays per wave
     __shared__ LevelLds lv[GC_MAX_LEVELS];
     __shared__ PtLds plv[PT ? GC_MAX_LEVELS : 1];
@@ -2071,7 +2071,15 @@ bool dvr_ray_supported(const DvrLaunch &p, uint32_t bytes)
 
 // Lanes per ray: a whole wave for the shortest lists (lowest latency per
 // ray), 16 lanes (4 rays per wave) otherwise.
-uint32_t dvr_ray_group(uint32_t rays) { return rays <= 256u ? 64u : 16u; }
+// Lanes per ray for a list of `rays`: the more rays, the fewer lanes each,
+// so that longer lists keep the machine full with shorter per-ray chains.
+uint32_t dvr_ray_group(uint32_t rays)
+{
+    if (rays <= 256u) return 64u;
+    if (rays <= RAY_G16_MAX) return 16u;
+    if (rays <= RAY_G8_MAX) return 8u;
+    return 4u;
+}
 
 template <int BYTES, typename ACC>
 static const void *ray_kernel_of(uint32_t group, bool bs32, bool pt)
@@ -2092,12 +2100,16 @@ static const void *ray_kernel_of(uint32_t group, bool bs32, bool pt)
     }
     if (sizeof(ACC) == 4 && bs32) {
         switch (group) {
+        case 4: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 4, 32>);
+        case 8: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 8, 32>);
         case 16: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 16, 32>);
         case 64: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 64, 32>);
         default: return nullptr;
         }
     }
     switch (group) {
+    case 4: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 4>);
+    case 8: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 8>);
     case 16: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 16>);
     case 64: return reinterpret_cast<const void *>(&k_dvr_ray<BYTES, ACC, 64>);
     default: return nullptr;
@@ -2159,6 +2171,7 @@ hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision
     if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
     if (group == 0) group = dvr_ray_group(rays);
+    if (p.pt.n != 0 && group < 16u) group = 16u;   // no narrower table-walk variants
     const void *k = ray_kernel(bytes, precision, group, p.brick_size == 32 && p.covering == 1, p.pt.n != 0);
     if (!k) return hipErrorInvalidValue;
     void *args[] = {const_cast<DvrLaunch *>(&p)};

@@ -745,7 +745,13 @@ struct Cache {
     hipEvent_t uploaded = nullptr, consumed = nullptr;
     uint32_t ts = 1;
     uint32_t cap = 50;                  // request buffers' capacity (max_requests at create)
-    uint32_t ray_list_max = 1u << 16;   // resumed-ray launches up to this many rays use the wave-per-ray march
+    uint32_t ray_list_max = RAY_G4_MAX;   // resumed-ray launches up to this many rays use the group-per-ray march
+    uint32_t ray_group = 0;               // its lanes per ray (0: from the list length)
+    uint32_t group_for(uint32_t rays, bool tables) const
+    {
+        const uint32_t g = ray_group ? ray_group : dvr_ray_group(rays);
+        return tables && g < 16u ? 16u : g;
+    }
     uint32_t limit = 50;                // bricks per update, <= cap
     // device state
     uint4 *mrpd = nullptr;
@@ -1589,17 +1595,22 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
         GC_HIP(hipEventSynchronize(B.launched));
         for (uint32_t i = 0; i < B.table; ++i) list_rays += B.h_stopped[i];
     }
-    const bool ray_kernel = has_list && packets == 0 && list_rays <= C.ray_list_max &&
+    // (a captured graph keeps 16 lanes per ray, whatever the list length)
+    const uint32_t ray_group = B.use_graph ? 16u : 0u;
+    const bool ray_kernel = has_list && packets == 0 &&
+                            list_rays <= (B.use_graph ? std::min(C.ray_list_max, RAY_G16_MAX) : C.ray_list_max) &&
                             dvr_ray_supported(L, C.vc.encoded_bytes);
     const uint32_t lds = dvr_lds_bytes(L.nb_lods, B.table, tf_entries);
     // (a captured graph keeps one grid for every list length)
     const uint32_t grid = ray_kernel ? dvr_ray_grid(C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays,
-                                                    dvr_ray_lds_bytes(B.table, tf_entries))
+                                                    dvr_ray_lds_bytes(B.table, tf_entries),
+                                                    ray_group ? ray_group : C.group_for(list_rays, L.pt.n != 0))
                                      : dvr_grid(C.vc.encoded_bytes, B.precision, B.tile, packets + (uint32_t)((listed + 63) / 64), lds,
                                                 tf_tables);
     auto launch_k = [&]() -> hipError_t {
         const hipError_t e =
-            ray_kernel ? launch_dvr_ray(L, C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays, grid, s)
+            ray_kernel ? launch_dvr_ray(L, C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays, grid, s,
+                                        ray_group ? ray_group : C.group_for(list_rays, L.pt.n != 0))
                        : launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
         return e == hipSuccess ? launch_collect_stopped(ca, s) : e;
     };
@@ -2483,9 +2494,11 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
     } else if (!*lane) {
         // a short list of resumed rays: a group of lanes per ray
         L.stats = d_stats + 1;
+        const uint32_t group = C.group_for(list_rays, L.pt.n != 0);
         GC_HIP(launch_dvr_ray(L, C.vc.encoded_bytes, view.precision, list_rays,
                               dvr_ray_grid(C.vc.encoded_bytes, view.precision, list_rays,
-                                           dvr_ray_lds_bytes(1, L.tf_lds_entries)), s));
+                                           dvr_ray_lds_bytes(1, L.tf_lds_entries), group),
+                              s, group));
     } else {
         const uint32_t grid = dvr_grid(C.vc.encoded_bytes, view.precision, view.tile,
                                        first ? L.n_packets : (F.w * F.h + 63) / 64,
@@ -2586,6 +2599,15 @@ int gc_cache_set_ray_list_max(gc_cache *c, uint32_t max_rays)
 {
     if (!c) return fail(GC_ERR_ARG, "null cache");
     c->impl.ray_list_max = max_rays;
+    return GC_OK;
+}
+
+int gc_cache_set_ray_group(gc_cache *c, uint32_t lanes)
+{
+    if (!c) return fail(GC_ERR_ARG, "null cache");
+    if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 64)
+        return fail(GC_ERR_ARG, "lanes per ray must be 0, 4, 8, 16 or 64");
+    c->impl.ray_group = lanes;
     return GC_OK;
 }
 

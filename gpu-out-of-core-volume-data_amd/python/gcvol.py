@@ -161,7 +161,7 @@ EXPORTS = [
     "gc_batch_create", "gc_batch_destroy", "gc_batch_set_graph", "gc_batch_render", "gc_batch_read_pixels",
     "gc_batch_set_launch", "gc_batch_begin", "gc_batch_set_view", "gc_batch_clear_view", "gc_batch_launch",
     "gc_batch_pending", "gc_batch_stats", "gc_batch_read_state", "gc_cache_set_max_requests",
-    "gc_cache_set_ray_list_max",
+    "gc_cache_set_ray_list_max", "gc_cache_set_ray_group",
     "gc_cache_busy", "gc_source_create_gpu_mip", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
     "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get", "gc_cache_counters_get",
     "gc_source_create_synthetic_gpu", "gc_source_get_device", "gc_render_views_noalloc", "gc_cache_probe",
@@ -229,7 +229,7 @@ def lib():
             "gc_batch_set_graph": [P, C.c_int],
             "gc_batch_render": [P, C.POINTER(View), C.POINTER(vp), P, u32, u32, C.POINTER(FrameReport)],
             "gc_batch_read_pixels": [P, u32, P], "gc_cache_set_max_requests": [P, u32],
-            "gc_cache_set_ray_list_max": [P, u32],
+            "gc_cache_set_ray_list_max": [P, u32], "gc_cache_set_ray_group": [P, u32],
             "gc_batch_set_launch": [P, u32, u32, u32],
             "gc_batch_begin": [P, C.POINTER(View), C.POINTER(vp), P, u32],
             "gc_batch_set_view": [P, u32, C.POINTER(View), P, u32], "gc_batch_clear_view": [P, u32],
@@ -945,8 +945,13 @@ class Request:
 
 
 def set_ray_list_max(cache, n):
-    """Resumed-ray launches of at most n rays march one wave per ray (0: never)."""
+    """Resumed-ray launches of at most n rays march a group of lanes per ray (0: never)."""
     check(lib().gc_cache_set_ray_list_max(cache.h, n))
+
+
+def set_ray_group(cache, lanes):
+    """Lanes per ray of the group-per-ray march (4, 8, 16, 64; 0: from the list length)."""
+    check(lib().gc_cache_set_ray_group(cache.h, lanes))
 
 
 def set_max_requests(cache, n):

@@ -605,11 +605,16 @@ int gc_render_views_noalloc(gc_cache *c, const gc_view *views, const float *cons
                             void *stream);
 
 /* Launches that resume at most max_rays rays (the rays an earlier launch of
- * the frame stopped at a miss) march them one wave per ray, 64 samples per
- * step, instead of one lane per ray; 0 never does.  The results are the same
+ * the frame stopped at a miss) march them with a group of lanes per ray
+ * (64, 16, 8 or 4 lanes, fewer for longer lists), several samples per step,
+ * instead of one lane per ray; 0 never does.  The results are the same
  * either way.  No reference counterpart (a launch-shape knob, like
  * gc_batch_set_launch). */
 int gc_cache_set_ray_list_max(gc_cache *c, uint32_t max_rays);
+/* Lanes per ray of those launches: 4, 8, 16 or 64, or 0 to pick them from the
+ * list length (the default).  Through page-table caches at least 16.  The
+ * results are the same for every value.  No reference counterpart. */
+int gc_cache_set_ray_group(gc_cache *c, uint32_t lanes);
 
 /* Bricks per update (the upload batch), at most the max_requests the cache
  * was created with. */

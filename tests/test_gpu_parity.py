@@ -215,18 +215,20 @@ def test_large_pool_offsets_match_small_pool(gpu, pool_shape):
     small.close()
 
 
-@pytest.mark.parametrize("bytes_,prec", [(1, 32), (2, 64), (2, 32)])
-def test_ray_list_kernel_matches_lane_kernel(gpu, bytes_, prec):
+@pytest.mark.parametrize("bytes_,prec,lanes", [(1, 32, 0), (2, 64, 0), (2, 32, 0), (1, 32, 4), (2, 32, 8),
+                                               (2, 64, 4)])
+def test_ray_list_kernel_matches_lane_kernel(gpu, bytes_, prec, lanes):
     """Launches that resume rays stopped at a miss run either one lane per ray
-    or one wave per ray (64 samples per step).  The same out-of-core frame
-    sequence rendered both ways, on two fresh caches, must give bit-identical
-    pixels, t buffers, colour state and render counters, and leave the page
-    directory, LRU, usage and request buffers bit-identical after every
-    frame."""
+    or a group of lanes per ray (lanes: 4, 8, or 0 for the list length's
+    choice, 16 or 64 here).  The same out-of-core frame sequence rendered
+    both ways, on two fresh caches, must give bit-identical pixels, t
+    buffers, colour state and render counters, and leave the page directory,
+    LRU, usage and request buffers bit-identical after every frame."""
     sc = Scene((96, 80, 64), 8, 1, bytes_, cache_size=(9, 9, 8), max_requests=64)
     caches = [sc.gpu_cache(), sc.gpu_cache()]
     G.set_ray_list_max(caches[0], 0)            # lane per ray throughout
-    G.set_ray_list_max(caches[1], 1 << 30)      # every resumed launch one wave per ray
+    G.set_ray_list_max(caches[1], 1 << 30)      # every resumed launch a group of lanes per ray
+    G.set_ray_group(caches[1], lanes)
     tf = tf_table(seed=23)
     d_tf = G.DeviceBuffer.from_array(tf)
     W, H = 80, 64
@@ -253,7 +255,7 @@ def test_ray_list_kernel_matches_lane_kernel(gpu, bytes_, prec):
         c.close()
 
 
-@pytest.mark.parametrize("bytes_,resume", [(1, "lane"), (2, "lane"), (2, "group")])
+@pytest.mark.parametrize("bytes_,resume", [(1, "lane"), (2, "lane"), (2, "group"), (1, "group4"), (2, "group8")])
 def test_brick32_kernel_matches_generic(gpu, bytes_, resume):
     """Bricks of 32 with covering 1 at FP32 run variants of the marches
     compiled for those strides (the tile march with 8x8 tiles, the
@@ -265,7 +267,8 @@ def test_brick32_kernel_matches_generic(gpu, bytes_, resume):
     them with the group-per-ray march."""
     sc = Scene((160, 128, 96), 32, 1, bytes_, cache_size=(5, 4, 4), max_requests=64)
     caches = [sc.gpu_cache(), sc.gpu_cache()]
-    G.set_ray_list_max(caches[0], 1 << 30 if resume == "group" else 0)
+    G.set_ray_list_max(caches[0], 1 << 30 if resume.startswith("group") else 0)
+    G.set_ray_group(caches[0], int(resume[5:]) if resume[5:] else 0)
     G.set_ray_list_max(caches[1], 0)            # every launch of the second cache is a generic tile march
     tf = tf_table(seed=31)
     d_tf = G.DeviceBuffer.from_array(tf)
