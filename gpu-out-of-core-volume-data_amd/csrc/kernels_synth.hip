@@ -187,24 +187,31 @@ __global__ __launch_bounds__(256) void k_synth_bricks_x4(SynthLaunch p)
     if (threadIdx.x == 0) p.below[u] = above ? 0u : 1u;
 }
 
-// Grid: (ceil(n[0] / 256), n[1], n[2]) workgroups of 256 lanes, one row
-// segment each.  Dynamic LDS: nb x 256 x factors of the segment, then the nb
-// row weights w_k = (amp_k fz_k(z)) * fy_k(y), the products the brick kernel
-// forms per voxel.
+// A box of voxels (x fastest) of one level of the synthetic volume, the
+// values the bricks of k_synth_bricks hold at the same positions.  A
+// workgroup writes a 256-voxel segment of four rows (y .. y + 3): the rows
+// share the segment's x factors, so a lane reads each factor once for its
+// four voxels (beside the four row weights w_k = (amp_k fz_k(z)) * fy_k(y),
+// the same for every lane), and every voxel sums fma(w_k, fx_k(x), sum) in
+// blob order as the brick kernel does.  Grid: (ceil(n[0] / 256),
+// ceil(n[1] / 4), n[2]).  Dynamic LDS: nb x 256 x factors, then 4 x nb row
+// weights.
 template <typename T>
-__global__ __launch_bounds__(256) void k_synth_box(SynthBox b)
+__global__ __launch_bounds__(256) void k_synth_box_r4(SynthBox b)
 {
-    extern __shared__ float fxs[];
+    extern __shared__ float fxr[];
     const uint32_t nb = b.nb;
-    float *ws = fxs + nb * 256u;
-    const uint32_t x0 = blockIdx.x * 256u, y = blockIdx.y, z = blockIdx.z;
-    const int64_t gy = b.o[1] + y, gz = b.o[2] + z;
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+    float *ws = fxr + nb * 256u;   // ws[r * nb + k]
+    const uint32_t x0 = blockIdx.x * 256u, y0 = blockIdx.y * 4u, z = blockIdx.z;
+    const int64_t gz = b.o[2] + z;
+    for (uint32_t i = threadIdx.x; i < 4u * nb; i += blockDim.x) {
+        const uint32_t r = i / nb, k = i - r * nb;
+        const int64_t gy = b.o[1] + y0 + r;
         const SynthAxis &ay = b.axis[1], &az = b.axis[2];
         const float fy = gy >= ay.lo && gy < ay.lo + (int64_t)ay.len ? b.tab[ay.off + (uint64_t)k * ay.len + (uint64_t)(gy - ay.lo)] : 0.f;
         const float fz0 = gz >= az.lo && gz < az.lo + (int64_t)az.len ? b.tab[az.off + (uint64_t)k * az.len + (uint64_t)(gz - az.lo)] : 0.f;
         const float fz = b.amp[k] * fz0;
-        ws[k] = fz * fy;
+        ws[i] = fz * fy;
     }
     for (uint32_t i = threadIdx.x; i < nb * 256u; i += blockDim.x) {
         const uint32_t k = i / 256u, j = i % 256u;
@@ -212,40 +219,53 @@ __global__ __launch_bounds__(256) void k_synth_box(SynthBox b)
         const SynthAxis &ax = b.axis[0];
         float f = 0.f;
         if (c >= ax.lo && c < ax.lo + (int64_t)ax.len) f = b.tab[ax.off + (uint64_t)k * ax.len + (uint64_t)(c - ax.lo)];
-        fxs[i] = f;
+        fxr[i] = f;
     }
     __syncthreads();
     const uint32_t x = x0 + threadIdx.x;
     if (x >= b.n[0]) return;
     const int64_t gx = b.o[0] + x;
-    float val = 0.f;
-    if (gx >= 0 && gx < (int64_t)b.ext[0] && gy >= 0 && gy < (int64_t)b.ext[1] && gz >= 0 && gz < (int64_t)b.ext[2]) {
-        float c = 0.f;
-        for (uint32_t k = 0; k < nb; ++k) c = __builtin_fmaf(ws[k], fxs[k * 256u + threadIdx.x], c);
-        int32_t nz = 0;
-        if (b.noise) {
-            const uint32_t span = 2u * b.noise + 1u;
-            const uint32_t rk = b.seed ^ (b.level * 0x27d4eb2fu) ^ ((uint32_t)gz * 0x165667b1u) ^ ((uint32_t)gy * 0xc2b2ae3du);
-            const uint32_t h = synth_hash32(rk ^ ((uint32_t)gx * 0x9e3779b1u));
-            nz = (int32_t)(uint32_t)(((uint64_t)h * span) >> 32) - (int32_t)b.noise;
-        }
-        float r = (float)(int32_t)(c + 0.5f) + (float)nz;
-        r = r < 0.f ? 0.f : r;
-        r = r > b.fmax ? b.fmax : r;
-        val = r;
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+    for (uint32_t k = 0; k < nb; ++k) {
+        const float f = fxr[k * 256u + threadIdx.x];
+        c0 = __builtin_fmaf(ws[k], f, c0);
+        c1 = __builtin_fmaf(ws[nb + k], f, c1);
+        c2 = __builtin_fmaf(ws[2u * nb + k], f, c2);
+        c3 = __builtin_fmaf(ws[3u * nb + k], f, c3);
     }
-    synth_store<T>(b.dst, (uint64_t)x + (uint64_t)b.n[0] * ((uint64_t)y + (uint64_t)b.n[1] * z), val);
+    const float cs[4] = {c0, c1, c2, c3};
+#pragma unroll
+    for (uint32_t r = 0; r < 4u; ++r) {
+        const uint32_t y = y0 + r;
+        if (y >= b.n[1]) break;
+        const int64_t gy = b.o[1] + y;
+        float val = 0.f;
+        if (gx >= 0 && gx < (int64_t)b.ext[0] && gy >= 0 && gy < (int64_t)b.ext[1] && gz >= 0 && gz < (int64_t)b.ext[2]) {
+            int32_t nz = 0;
+            if (b.noise) {
+                const uint32_t span = 2u * b.noise + 1u;
+                const uint32_t rk = b.seed ^ (b.level * 0x27d4eb2fu) ^ ((uint32_t)gz * 0x165667b1u) ^ ((uint32_t)gy * 0xc2b2ae3du);
+                const uint32_t h = synth_hash32(rk ^ ((uint32_t)gx * 0x9e3779b1u));
+                nz = (int32_t)(uint32_t)(((uint64_t)h * span) >> 32) - (int32_t)b.noise;
+            }
+            float rr = (float)(int32_t)(cs[r] + 0.5f) + (float)nz;
+            rr = rr < 0.f ? 0.f : rr;
+            rr = rr > b.fmax ? b.fmax : rr;
+            val = rr;
+        }
+        synth_store<T>(b.dst, (uint64_t)x + (uint64_t)b.n[0] * ((uint64_t)y + (uint64_t)b.n[1] * z), val);
+    }
 }
 
 hipError_t launch_synth_box(const SynthBox &b, hipStream_t s)
 {
     if (b.n[0] == 0 || b.n[1] == 0 || b.n[2] == 0) return hipSuccess;
-    if (b.n[1] > 65535u || b.n[2] > 65535u) return hipErrorInvalidValue;
-    const dim3 grid((b.n[0] + 255u) / 256u, b.n[1], b.n[2]);
-    const size_t lds = (size_t)b.nb * 257u * sizeof(float);
-    if (b.bytes == 1) k_synth_box<uint8_t><<<grid, 256, lds, s>>>(b);
-    else if (b.bytes == 2) k_synth_box<uint16_t><<<grid, 256, lds, s>>>(b);
-    else k_synth_box<float><<<grid, 256, lds, s>>>(b);
+    if (b.n[2] > 65535u || (b.n[1] + 3u) / 4u > 65535u) return hipErrorInvalidValue;
+    const dim3 grid((b.n[0] + 255u) / 256u, (b.n[1] + 3u) / 4u, b.n[2]);
+    const size_t lds = (size_t)b.nb * (256u + 4u) * sizeof(float);
+    if (b.bytes == 1) k_synth_box_r4<uint8_t><<<grid, 256, lds, s>>>(b);
+    else if (b.bytes == 2) k_synth_box_r4<uint16_t><<<grid, 256, lds, s>>>(b);
+    else k_synth_box_r4<float><<<grid, 256, lds, s>>>(b);
     return hipGetLastError();
 }
 

@@ -365,7 +365,7 @@ struct DeviceSynthSource : SynthSource {
 // The synthetic volume's level 0 with a resident mip pyramid on the device
 // (the preprocessing of Mipmapper.cu:401-597 for a volume that does not fit
 // anywhere at level 0): a cold build synthesises level 0 in z-slabs of
-// `slab` planes straight into device memory (k_synth_box, the voxels of the
+// `slab` planes straight into device memory (k_synth_box_r4, the voxels of the
 // brick synthesiser) and reduces each slab into level 1 (k_mip2, the
 // Mipmapper rule; slabs start on even planes, so each 2x2x2 group lies in one
 // slab); levels 2.. are reduced from the resident level 1.  Requests then
