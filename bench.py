@@ -793,6 +793,9 @@ def main():
                     help="where the served volumes' bricks are synthesised")
     ap.add_argument("--serving-ab", action="store_true",
                     help="also serve each arrival sequence with a fixed number of views per volume")
+    ap.add_argument("--launch-log", action="store_true",
+                    help="log every ray-caster launch of the timed frames (stderr) and add the last frame's to the "
+                         "output line")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--pool-gib", type=float, default=0.0, help="brick pool size (default: the workload's 4 GiB)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -857,6 +860,8 @@ def main():
     cc = G.cache_cfg(cs, BRICK, max_requests=args.max_requests, sync=False)
     cache = G.Cache(vc, cc, src)
     frame = G.Frame(WIDTH, HEIGHT, args.precision)
+    log = G.LaunchLog(cache) if args.launch_log else None
+    last_log = None
 
     nframes = args.warmup + args.steps
     lat, reports = [], []
@@ -882,6 +887,13 @@ def main():
         if k >= args.warmup:
             lat.append(dt * 1e3)
             reports.append(d)
+            if log is not None:
+                last_log = log.last_frame()
+                for r in last_log:
+                    print("  launch %3d %-5s list %8d group %2d rays %8d samples %11d visits %9d stopped %8d "
+                          "bricks %6d %.3f ms" % (r["launch"], r["kernel"], r["list_rays"], r["group"], r["rays"],
+                                                  r["mapped_samples"], r["brick_visits"], r["unmapped_rays"],
+                                                  r["bricks_loaded"], r["kernel_ms"]), file=sys.stderr, flush=True)
         if k == args.warmup and not args.no_cpu_baseline:
             # untimed: the CPU baseline over this frame's cache state
             cpu = cpu_baseline(G, cache, vc, cam, tf, seconds=args.cpu_seconds,
@@ -953,6 +965,9 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu
+    if log is not None:
+        out["launch_log_last_frame"] = last_log
+        log.close()
     cache.close()
     frame.close()
     if not args.no_extras:

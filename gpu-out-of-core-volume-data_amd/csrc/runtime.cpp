@@ -784,6 +784,8 @@ struct Cache {
     // frame driver
     gc_render_stats *d_frame_stats = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    gc_launch_record *log = nullptr;     // per-launch log of gc_render_frame, or null
+    uint32_t log_cap = 0, *log_count = nullptr;
 
     ~Cache() { release(); }
     void release();
@@ -2549,6 +2551,8 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
         view->tile >= GC_TILES || view->compositing > 1)
         return fail(GC_ERR_ARG, "view does not match the frame");
     GC_HIP(hipMemsetAsync(C.d_frame_stats, 0, 2 * sizeof(gc_render_stats), C.stream));
+    if (C.log) *C.log_count = 0;
+    gc_render_stats logged[2] = {};   // the frame's counters at the last logged launch
     int cur = 0;
     for (uint32_t it = 0; it < max_launches; ++it) {
         uint32_t list_rays = 0;
@@ -2571,6 +2575,26 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
         GC_HIP(hipEventElapsedTime(&ms, C.ev0, C.ev1));
         report->kernel_ms += ms;
         if (lane) report->lane_kernel_ms += ms;
+        gc_launch_record *rec = nullptr;
+        if (C.log && *C.log_count < C.log_cap) {
+            gc_render_stats st[2];
+            GC_HIP(hipMemcpyAsync(st, C.d_frame_stats, sizeof(st), hipMemcpyDeviceToHost, C.stream));
+            GC_HIP(hipStreamSynchronize(C.stream));
+            rec = &C.log[(*C.log_count)++];
+            std::memset(rec, 0, sizeof(*rec));
+            rec->launch = it;
+            rec->kernel = view->compositing == 1 ? 2u : (lane ? 0u : 1u);
+            rec->list_rays = list_rays;
+            rec->group = view->compositing != 1 && !lane ? C.group_for(list_rays, C.g.nb_tables != 0) : 0u;
+            const int k = lane ? 0 : 1;
+            rec->stats.mapped_samples = st[k].mapped_samples - logged[k].mapped_samples;
+            rec->stats.brick_visits = st[k].brick_visits - logged[k].brick_visits;
+            rec->stats.pixels = st[k].pixels - logged[k].pixels;
+            rec->stats.unmapped_rays = st[k].unmapped_rays - logged[k].unmapped_rays;
+            rec->kernel_ms = ms;
+            logged[0] = st[0];
+            logged[1] = st[1];
+        }
         if (done) {
             report->update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             report->complete = 1;
@@ -2582,6 +2606,7 @@ int gc_render_frame(gc_cache *c, const gc_view *view, const float *d_tf, uint32_
         GC_TRY(C.update(&done));
         ++report->updates;
         report->bricks_loaded += before;
+        if (rec) rec->bricks_loaded = before;
         report->update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     gc_render_stats st[2];
@@ -2599,6 +2624,17 @@ int gc_cache_set_ray_list_max(gc_cache *c, uint32_t max_rays)
 {
     if (!c) return fail(GC_ERR_ARG, "null cache");
     c->impl.ray_list_max = max_rays;
+    return GC_OK;
+}
+
+int gc_cache_set_launch_log(gc_cache *c, gc_launch_record *records, uint32_t capacity, uint32_t *count)
+{
+    if (!c) return fail(GC_ERR_ARG, "null cache");
+    if (records && !count) return fail(GC_ERR_ARG, "launch log without a count");
+    c->impl.log = records;
+    c->impl.log_cap = records ? capacity : 0;
+    c->impl.log_count = records ? count : nullptr;
+    if (count) *count = 0;
     return GC_OK;
 }
 

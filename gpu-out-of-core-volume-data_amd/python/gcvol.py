@@ -94,6 +94,11 @@ class RenderStats(C.Structure):
     _fields_ = [("mapped_samples", u64), ("brick_visits", u64), ("pixels", u64), ("unmapped_rays", u64)]
 
 
+class LaunchRecord(C.Structure):
+    _fields_ = [("launch", u32), ("kernel", u32), ("list_rays", u32), ("group", u32), ("stats", RenderStats),
+                ("kernel_ms", C.c_double), ("bricks_loaded", u32), ("reserved", u32)]
+
+
 class FrameReport(C.Structure):
     _fields_ = [
         ("launches", u32), ("updates", u32), ("complete", u32), ("reserved", u32), ("bricks_loaded", u64),
@@ -161,7 +166,7 @@ EXPORTS = [
     "gc_batch_create", "gc_batch_destroy", "gc_batch_set_graph", "gc_batch_render", "gc_batch_read_pixels",
     "gc_batch_set_launch", "gc_batch_begin", "gc_batch_set_view", "gc_batch_clear_view", "gc_batch_launch",
     "gc_batch_pending", "gc_batch_stats", "gc_batch_read_state", "gc_cache_set_max_requests",
-    "gc_cache_set_ray_list_max", "gc_cache_set_ray_group",
+    "gc_cache_set_ray_list_max", "gc_cache_set_ray_group", "gc_cache_set_launch_log",
     "gc_cache_busy", "gc_source_create_gpu_mip", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
     "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get", "gc_cache_counters_get",
     "gc_source_create_synthetic_gpu", "gc_source_get_device", "gc_render_views_noalloc", "gc_cache_probe",
@@ -230,6 +235,7 @@ def lib():
             "gc_batch_render": [P, C.POINTER(View), C.POINTER(vp), P, u32, u32, C.POINTER(FrameReport)],
             "gc_batch_read_pixels": [P, u32, P], "gc_cache_set_max_requests": [P, u32],
             "gc_cache_set_ray_list_max": [P, u32], "gc_cache_set_ray_group": [P, u32],
+            "gc_cache_set_launch_log": [P, P, u32, P],
             "gc_batch_set_launch": [P, u32, u32, u32],
             "gc_batch_begin": [P, C.POINTER(View), C.POINTER(vp), P, u32],
             "gc_batch_set_view": [P, u32, C.POINTER(View), P, u32], "gc_batch_clear_view": [P, u32],
@@ -947,6 +953,32 @@ class Request:
 def set_ray_list_max(cache, n):
     """Resumed-ray launches of at most n rays march a group of lanes per ray (0: never)."""
     check(lib().gc_cache_set_ray_list_max(cache.h, n))
+
+
+class LaunchLog:
+    """Per-launch records of the following render_frame calls on a cache."""
+
+    def __init__(self, cache, capacity=4096):
+        self.cache = cache
+        self.records = (LaunchRecord * capacity)()
+        self.count = C.c_uint32(0)
+        check(lib().gc_cache_set_launch_log(cache.h, self.records, capacity, C.byref(self.count)))
+
+    def last_frame(self):
+        """The last frame's launches as dicts."""
+        out = []
+        for r in self.records[:self.count.value]:
+            out.append({"launch": r.launch, "kernel": ("lane", "group", "mop")[r.kernel], "list_rays": r.list_rays,
+                        "group": r.group, "mapped_samples": r.stats.mapped_samples,
+                        "brick_visits": r.stats.brick_visits, "rays": r.stats.pixels,
+                        "unmapped_rays": r.stats.unmapped_rays, "kernel_ms": r.kernel_ms,
+                        "bricks_loaded": r.bricks_loaded})
+        return out
+
+    def close(self):
+        if self.cache is not None and self.cache.h:
+            check(lib().gc_cache_set_launch_log(self.cache.h, None, 0, None))
+        self.cache = None
 
 
 def set_ray_group(cache, lanes):

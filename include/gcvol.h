@@ -526,6 +526,26 @@ typedef struct gc_frame_report {
     double   lane_kernel_ms;
 } gc_frame_report;
 
+/* One launch of gc_render_frame's loop, for tracing (gc_cache_set_launch_log). */
+typedef struct gc_launch_record {
+    uint32_t launch;             /* index of the launch in its frame           */
+    uint32_t kernel;             /* 0: tile / lane-per-ray march, 1: group-per-
+                                  * ray march, 2: maximum projection           */
+    uint32_t list_rays;          /* rays resumed from the previous launch (0:
+                                  * the first launch, over the view's tiles)   */
+    uint32_t group;              /* lanes per ray of the group march, or 0     */
+    gc_render_stats stats;       /* this launch's counters                     */
+    double   kernel_ms;          /* HIP events around the launch               */
+    uint32_t bricks_loaded;      /* non-empty bricks the update after it loaded */
+    uint32_t reserved;
+} gc_launch_record;
+/* Log each launch of the following gc_render_frame calls on c into
+ * records[0 .. capacity): *count is reset to 0 by every gc_render_frame and
+ * counts the records it wrote (launches past capacity are not logged).
+ * records = null turns the log off.  The buffers must outlive the calls.
+ * No reference counterpart (tracing). */
+int gc_cache_set_launch_log(gc_cache *c, gc_launch_record *records, uint32_t capacity, uint32_t *count);
+
 /* The plain single-view renderer's progressive loop (VolumeRayCaster.cu:
  * 261-330 without the display): launch the ray caster, update the cache,
  * repeat until an update reports no request or max_launches is reached.
