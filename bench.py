@@ -697,7 +697,8 @@ def _pmc_pass(counters, outdir, args):
     cmd = ["rocprofv3", "--pmc"] + list(counters) + ["-d", outdir, "-o", tag, "--", sys.executable,
            os.path.abspath(__file__), "--steps", str(args.steps), "--warmup", str(args.warmup), "--no-cpu-baseline",
            "--no-traffic", "--no-extras", "--calib-copy-mib", str(CALIB_MIB), "--max-requests", str(args.max_requests),
-           "--tile", str(args.tile), "--precision", str(args.precision)]
+           "--tile", str(args.tile), "--precision", str(args.precision)] + \
+          (["--handoff", args.handoff] if args.handoff else [])
     env = dict(os.environ, TMPDIR="/tmp")
     r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=300)
     if r.returncode != 0:
@@ -714,7 +715,14 @@ def _pmc_pass(counters, outdir, args):
     for counter in counters:
         rows = con.execute("select dispatch_id, kernel_name, sum(value) from counters_collection where counter_name = ? "
                            "group by dispatch_id order by dispatch_id", (counter,)).fetchall()
-        dvr = [v for _, k, v in rows if "k_dvr<" in k]
+        # a lane launch's hand-over continuation (a k_dvr_ray dispatch right
+        # after its k_dvr dispatch) counts with it, as in its HIP-event time
+        dvr = []
+        for i, (_, k, v) in enumerate(rows):
+            if "k_dvr<" in k:
+                dvr.append(v)
+            elif "k_dvr_ray<" in k and i > 0 and "k_dvr<" in rows[i - 1][1] and dvr:
+                dvr[-1] += v
         copy = [v for _, k, v in rows if "copy" in k.lower()]
         if len(dvr) <= first_launches or not copy:
             con.close()
@@ -793,6 +801,10 @@ def main():
                     help="where the served volumes' bricks are synthesised")
     ap.add_argument("--serving-ab", action="store_true",
                     help="also serve each arrival sequence with a fixed number of views per volume")
+    ap.add_argument("--handoff", default="",
+                    help="SAMPLES[:LANES[:PCT]]: lane-per-ray launches hand rays that have consumed SAMPLES samples "
+                         "over to a group march of LANES lanes per ray within the launch once PCT %% of its waves have "
+                         "run out of rays (0: off; default: the library's setting)")
     ap.add_argument("--launch-log", action="store_true",
                     help="log every ray-caster launch of the timed frames (stderr) and add the last frame's to the "
                          "output line")
@@ -840,13 +852,18 @@ def main():
     tf = G.tf_samples(G.tf_seeded(SEED, 6), 256)
     d_tf = G.DeviceBuffer.from_array(tf)
     upload_tune = None
+    hs = args.handoff.split(":") if args.handoff else []
+    handoff = tuple(int(hs[i]) if len(hs) > i else 0 for i in range(3)) if hs else None
     if args.max_requests <= 0:
         # the upload batch, picked by the autotuner on three cold frames of
         # the path (fresh caches), before anything is timed
         import autotune
 
         def make_cache():
-            return G.Cache(vc, G.cache_cfg(cs, BRICK, max_requests=16384, sync=False), src)
+            c = G.Cache(vc, G.cache_cfg(cs, BRICK, max_requests=16384, sync=False), src)
+            if handoff is not None:
+                G.set_ray_handoff(c, *handoff)
+            return c
 
         fr = [(G.make_view(WIDTH, HEIGHT, fly_through(pf), alpha_stop=ALPHA_STOP, precision=args.precision,
                            tile=args.tile), d_tf, len(tf)) for pf in (0, 54, 108)]
@@ -859,6 +876,9 @@ def main():
     traffic, traffic_note, l2 = (None, "not measured", None) if args.no_traffic else measure_traffic(args)
     cc = G.cache_cfg(cs, BRICK, max_requests=args.max_requests, sync=False)
     cache = G.Cache(vc, cc, src)
+    if handoff is not None:
+        G.set_ray_handoff(cache, *handoff)
+    handoff = G.get_ray_handoff(cache)
     frame = G.Frame(WIDTH, HEIGHT, args.precision)
     log = G.LaunchLog(cache) if args.launch_log else None
     last_log = None
@@ -891,9 +911,10 @@ def main():
                 last_log = log.last_frame()
                 for r in last_log:
                     print("  launch %3d %-5s list %8d group %2d rays %8d samples %11d visits %9d stopped %8d "
-                          "bricks %6d %.3f ms" % (r["launch"], r["kernel"], r["list_rays"], r["group"], r["rays"],
-                                                  r["mapped_samples"], r["brick_visits"], r["unmapped_rays"],
-                                                  r["bricks_loaded"], r["kernel_ms"]), file=sys.stderr, flush=True)
+                          "bricks %6d %.3f ms handed %d" % (r["launch"], r["kernel"], r["list_rays"], r["group"],
+                                                            r["rays"], r["mapped_samples"], r["brick_visits"],
+                                                            r["unmapped_rays"], r["bricks_loaded"], r["kernel_ms"],
+                                                            r["handed_over"]), file=sys.stderr, flush=True)
         if k == args.warmup and not args.no_cpu_baseline:
             # untimed: the CPU baseline over this frame's cache state
             cpu = cpu_baseline(G, cache, vc, cam, tf, seconds=args.cpu_seconds,
@@ -931,6 +952,7 @@ def main():
             "tile": TILE_NAMES[args.tile],
             "upload_batch": args.max_requests,
             "upload_autotune": upload_tune["table"] if upload_tune else None,
+            "handoff": {"samples": handoff[0], "lanes": handoff[1], "exited_pct": handoff[2]},
         },
         "frames_per_s": len(lat) / total_s if total_s > 0 else 0.0,
         "p50_ms": percentile(lat, 50),
@@ -951,7 +973,9 @@ def main():
             "traffic_note": traffic_note,
             "l2": l2,
             "kernel": "k_dvr<2,%s,%s>" % ("double" if args.precision == 64 else "float",
-                                          ["8,8", "16,4", "32,2", "64,1", "16,4", "32,2"][args.tile]),
+                                          ["8,8", "16,4", "32,2", "64,1", "16,4", "32,2"][args.tile]) +
+                      (" + hand-over k_dvr_ray<2,%s,%d,32>" % ("double" if args.precision == 64 else "float", handoff[1])
+                       if handoff[0] else ""),
             "launches": lane_launches,
             "avg_launch_ms": lane_ms / lane_launches if lane_launches else 0.0,
             "algorithmic_bytes_per_launch": B_lane / lane_launches if lane_launches else 0.0,

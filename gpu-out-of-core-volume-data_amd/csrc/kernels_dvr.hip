@@ -726,6 +726,11 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     double Q = 1.0, thr = 0.0;
     uint32_t n_samples = 0, n_visits = 0;      // per lane
     uint32_t n_rays = 0, n_unmapped = 0;       // per wave
+    // hand-over: the lane's samples before its current ray, and the budget
+    // of samples a ray may consume here before it is handed over
+    uint32_t ray_n0 = 0;
+    uint32_t rounds = 0;
+    bool go_seen = false;
 
     auto set_level = [&](uint32_t l) {
         const LevelLds &L = lv[l];
@@ -751,6 +756,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // a-time trace.  The page cell doubles as the request cell
     // (realDivVirtual == 1).
     const bool piped = !PT && fast_fetch && p.pipelined != 0;
+    const uint32_t budget = piped && p.yield_after != 0 ? p.yield_after : 0xffffffffu;
     // hb: stage B holds a sample (the front placed last half-step); hc: stage
     // C holds one; newray: the next sample placed is the ray's first
     bool hb = false, hc = false, newray = false;
@@ -760,7 +766,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // position of sample C, kept (in LDS, off the register budget) only when
     // a view of the launch asks for exact continuation
     __shared__ F3 pos_c[LANES];
-    const bool keep_pos = p.exact_resume != 0;
+    const bool keep_pos = p.exact_resume != 0 || budget != 0xffffffffu;
     bool freshB = false, freshC = false, reuseC = false;
     uint32_t eX = 0u, eY = 0u;   // cell-table words in flight
     Quad wX{0u, 0u, 0u, 0u}, wY = wX;
@@ -961,6 +967,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                         hb = hc = false;
                         newray = true;
                         have = true;
+                        ray_n0 = n_samples;
                     }
                 }
             }
@@ -973,6 +980,12 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
 
         // ------------------------------------------------------------- march
         bool stopped = false;   // stopped at an Unmapped brick in this round
+        // hand-over allowed in this round (wave-uniform): at once, or once
+        // the go flag of the workgroup's line is up (read every 8th round)
+        if (budget != 0xffffffffu && p.yield_exited != 0 && !go_seen && (++rounds & 7u) == 0u)
+            go_seen = __builtin_amdgcn_readfirstlane(__hip_atomic_load(p.go + (blockIdx.x & 63u) * 32u, __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_AGENT)) != 0;
+        const bool handoff_now = budget != 0xffffffffu && (p.yield_exited == 0 || go_seen);
         if constexpr (BYTES != 4) {
             if (piped) {
                 // One half-step: issue the page entry of the new front sample
@@ -1041,9 +1054,14 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     bool live;
                     if constexpr (sizeof(ACC) == 4) live = hc && (exact ? Q > thr : fc.w < astop) && tC < tfar;
                     else live = hc && fc.w < (ACC)astop && tC < tfar;
+                    // hand-over: a ray past its budget stops before the
+                    // first sample of a mapped brick it enters, so the group
+                    // march that continues it visits and stamps that brick
+                    // as this march would have
+                    const bool yld = handoff_now && live && ewC == 1 && freshC && n_samples - ray_n0 >= budget;
                     const bool unmapped = live && ewC == 0;
-                    const bool cons = live && ewC == 1;
-                    n_visits += (live && freshC) ? 1u : 0u;
+                    const bool cons = live && ewC == 1 && !yld;
+                    n_visits += (live && freshC && !yld) ? 1u : 0u;
                     if (unmapped) {
                         // raise_brick_request, then stop (Unmapped)
                         p.request[cellC] = ts;
@@ -1056,8 +1074,21 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     sum_prev = cons ? sum : sum_prev;
                     composite_if(cons, sum, lodC);
                     n_samples += cons ? 1u : 0u;
-                    const bool end = (hc && !live) || unmapped;
+                    const bool end = (hc && !live) || unmapped || yld;
                     if (end) store_state(tC, keep_pos ? pos_c[tid] : pos_b);
+                    const uint64_t ym = __ballot(yld);
+                    if (ym) {
+                        // the wave's handed-over rays, with one atomic
+                        const uint32_t first = (uint32_t)__builtin_ctzll(ym);
+                        uint32_t base = 0;
+                        if (lane == first) base = atomicAdd(p.yield_count, (uint32_t)__popcll(ym));
+                        base = __shfl(base, (int)first, 64);
+                        if (yld) {
+                            p.yield_list[base + (uint32_t)__popcll(ym & lt_mask)] = (v << DVR_PIXEL_BITS) | pix;
+                            const F3 pc = pos_c[tid];
+                            gp(reinterpret_cast<gf4 *>(p.yield_pos))[pix] = gf4{pc.x, pc.y, pc.z, tC};
+                        }
+                    }
                     // w_old is waited for on every path here (the main path
                     // already has), so its registers are free when the next
                     // half issues loads into them
@@ -1218,6 +1249,13 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         }
     }
 
+    if (p.exited && lane == 0) {
+        // the wave that completes the count raises the go flags
+        const uint32_t n = atomicAdd(p.exited, 1u) + 1u;
+        if (n == p.yield_exited)
+            for (uint32_t k = 0; k < 64; ++k)
+                __hip_atomic_store(p.go + k * 32u, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (p.stats) {
         const uint32_t s0 = wave_sum(n_samples), s1 = wave_sum(n_visits), s2 = n_rays, s3 = n_unmapped;
         if (lane == 0) {
@@ -1372,7 +1410,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     const gf4 c = gp(reinterpret_cast<const gf4 *>(V.cstate))[pix];
                     fc = Acc4<ACC>{c.x, c.y, c.z, c.w};
                 }
-                if (gl == 0) ++n_rays;
+                if (gl == 0 && !p.continuation) ++n_rays;
                 if (!Rs.hit) {
                     fc.x = fc.y = fc.z = (ACC)0;
                     if (gl == 0) gp(V.pixels)[pix] = pack(fc.x, fc.y, fc.z, fc.w);
@@ -2042,6 +2080,8 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
     if (p.list && !p.list_count) return hipErrorInvalidValue;
     if (p.march < 1 || p.march > 8) return hipErrorInvalidValue;
+    if (p.yield_after && (!p.yield_list || !p.yield_count || !p.yield_pos || p.views)) return hipErrorInvalidValue;
+    if (p.yield_exited && (!p.exited || !p.go)) return hipErrorInvalidValue;
     if (p.pt.n) {
         if (p.pipelined) return hipErrorInvalidValue;
         // the table-cache variants keep one LDS transfer function at most

@@ -96,7 +96,7 @@ class RenderStats(C.Structure):
 
 class LaunchRecord(C.Structure):
     _fields_ = [("launch", u32), ("kernel", u32), ("list_rays", u32), ("group", u32), ("stats", RenderStats),
-                ("kernel_ms", C.c_double), ("bricks_loaded", u32), ("reserved", u32)]
+                ("kernel_ms", C.c_double), ("bricks_loaded", u32), ("handed_over", u32)]
 
 
 class FrameReport(C.Structure):
@@ -166,7 +166,8 @@ EXPORTS = [
     "gc_batch_create", "gc_batch_destroy", "gc_batch_set_graph", "gc_batch_render", "gc_batch_read_pixels",
     "gc_batch_set_launch", "gc_batch_begin", "gc_batch_set_view", "gc_batch_clear_view", "gc_batch_launch",
     "gc_batch_pending", "gc_batch_stats", "gc_batch_read_state", "gc_cache_set_max_requests",
-    "gc_cache_set_ray_list_max", "gc_cache_set_ray_group", "gc_cache_set_launch_log",
+    "gc_cache_set_ray_list_max", "gc_cache_set_ray_group", "gc_cache_set_ray_handoff", "gc_cache_get_ray_handoff",
+    "gc_cache_set_launch_log",
     "gc_cache_busy", "gc_source_create_gpu_mip", "gc_server_create", "gc_server_destroy", "gc_server_add_volume", "gc_server_submit",
     "gc_server_poll", "gc_server_take_pixels", "gc_server_drain", "gc_server_stats_get", "gc_cache_counters_get",
     "gc_source_create_synthetic_gpu", "gc_source_get_device", "gc_render_views_noalloc", "gc_cache_probe",
@@ -235,6 +236,8 @@ def lib():
             "gc_batch_render": [P, C.POINTER(View), C.POINTER(vp), P, u32, u32, C.POINTER(FrameReport)],
             "gc_batch_read_pixels": [P, u32, P], "gc_cache_set_max_requests": [P, u32],
             "gc_cache_set_ray_list_max": [P, u32], "gc_cache_set_ray_group": [P, u32],
+            "gc_cache_set_ray_handoff": [P, u32, u32, u32],
+            "gc_cache_get_ray_handoff": [P, P, P, P],
             "gc_cache_set_launch_log": [P, P, u32, P],
             "gc_batch_set_launch": [P, u32, u32, u32],
             "gc_batch_begin": [P, C.POINTER(View), C.POINTER(vp), P, u32],
@@ -972,7 +975,7 @@ class LaunchLog:
                         "group": r.group, "mapped_samples": r.stats.mapped_samples,
                         "brick_visits": r.stats.brick_visits, "rays": r.stats.pixels,
                         "unmapped_rays": r.stats.unmapped_rays, "kernel_ms": r.kernel_ms,
-                        "bricks_loaded": r.bricks_loaded})
+                        "bricks_loaded": r.bricks_loaded, "handed_over": r.handed_over})
         return out
 
     def close(self):
@@ -984,6 +987,22 @@ class LaunchLog:
 def set_ray_group(cache, lanes):
     """Lanes per ray of the group-per-ray march (4, 8, 16, 64; 0: from the list length)."""
     check(lib().gc_cache_set_ray_group(cache.h, lanes))
+
+
+def set_ray_handoff(cache, samples, lanes=0, exited_pct=0):
+    """Hand rays that have consumed `samples` samples in a lane-per-ray launch of
+    render_frame over to a group march of `lanes` lanes per ray (0: 16) within
+    the launch, once exited_pct percent of the launch's waves have run out of
+    rays (0: from the start); samples = 0 turns it off.  The results are the
+    same."""
+    check(lib().gc_cache_set_ray_handoff(cache.h, samples, lanes, exited_pct))
+
+
+def get_ray_handoff(cache):
+    """(samples, lanes, exited_pct) of the cache's hand-over setting."""
+    v = [C.c_uint32(0) for _ in range(3)]
+    check(lib().gc_cache_get_ray_handoff(cache.h, *[C.byref(x) for x in v]))
+    return tuple(x.value for x in v)
 
 
 def set_max_requests(cache, n):

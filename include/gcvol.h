@@ -537,7 +537,9 @@ typedef struct gc_launch_record {
     gc_render_stats stats;       /* this launch's counters                     */
     double   kernel_ms;          /* HIP events around the launch               */
     uint32_t bricks_loaded;      /* non-empty bricks the update after it loaded */
-    uint32_t reserved;
+    uint32_t handed_over;        /* rays the lane march handed over to the group
+                                  * march within the launch (gc_cache_set_ray_
+                                  * handoff); included in the counters above   */
 } gc_launch_record;
 /* Log each launch of the following gc_render_frame calls on c into
  * records[0 .. capacity): *count is reset to 0 by every gc_render_frame and
@@ -635,6 +637,23 @@ int gc_cache_set_ray_list_max(gc_cache *c, uint32_t max_rays);
  * list length (the default).  Through page-table caches at least 16.  The
  * results are the same for every value.  No reference counterpart. */
 int gc_cache_set_ray_group(gc_cache *c, uint32_t lanes);
+/* Hand-over of long rays in gc_render_frame's lane-per-ray launches: a ray
+ * that has consumed `samples` samples in the launch stops before the first
+ * sample of the next mapped brick it enters and a group march of `lanes`
+ * lanes per ray (4, 8, 16 or 64; 0: 16) continues it from the stored state
+ * (exact position included, whatever the view's resume mode) in the same
+ * launch, before the cache update.  With exited_pct > 0 rays are handed
+ * over only once that percentage of the launch's waves have run out of rays:
+ * the rays left in the last waves (the launch's tail) go to the group march.
+ * Without it such a launch lasts as long as its longest rays, which the lane
+ * march walks one sample per step.  Stamps, requests,
+ * counters and pixels are the same either way.  samples = 0 turns it off;
+ * a new cache uses 32 samples, 8 lanes and 75 %.
+ * Used where the pipelined march runs (no page-table caches).  No reference
+ * counterpart (a launch-shape knob). */
+int gc_cache_set_ray_handoff(gc_cache *c, uint32_t samples, uint32_t lanes, uint32_t exited_pct);
+/* The cache's hand-over setting (default 32 samples, 8 lanes, 75 %). */
+int gc_cache_get_ray_handoff(const gc_cache *c, uint32_t *samples, uint32_t *lanes, uint32_t *exited_pct);
 
 /* Bricks per update (the upload batch), at most the max_requests the cache
  * was created with. */
