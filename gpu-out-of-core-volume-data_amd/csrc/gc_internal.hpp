@@ -371,10 +371,10 @@ struct DvrLaunch {
                                 //    caches; see launch_base)
     // Hand-over of long rays (pipelined lane march): a ray that has consumed
     // yield_after samples in the launch stops before the first sample of its
-    // next mapped brick, stores its state as at a miss (without a request)
-    // and its exact position in yield_pos (per pixel, whatever the views'
-    // resume mode), and is appended to yield_list (count *yield_count), for a
-    // group march to continue before the cache update.  0: never.  With
+    // next mapped brick, stores its state as at a miss (without a request),
+    // and is appended to yield_list (count *yield_count) with its exact
+    // position at the same index of yield_pos (whatever the views' resume
+    // mode), for a group march to continue before the cache update.  0: never.  With
     // yield_exited > 0 rays are handed over only once that many waves of the
     // launch have run out of rays (*exited counts them; the wave that
     // reaches the count raises the 64 flags go[32 k], one per 128-byte line,
@@ -387,7 +387,8 @@ struct DvrLaunch {
     uint32_t *yield_list;
     uint32_t *yield_count;
     float4 *yield_pos;
-    uint32_t continuation;      // 1: `list` holds handed-over rays (k_dvr_ray; not counted again as rays)
+    uint32_t continuation;      // 1: `list` holds handed-over rays, their positions in yield_pos
+                                //    (k_dvr_ray; not counted again as rays)
 };
 
 // Ray tiles: 0 8x8, 1 16x4, 2 32x2, 3 64x1 are one 64-ray packet each; 4

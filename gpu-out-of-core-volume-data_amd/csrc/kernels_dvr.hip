@@ -635,7 +635,9 @@ __device__ inline void load_tf_tables(const DvrView *vw, uint32_t n_views, float
 // immediate offsets.
 // PT: the page directory has table caches (the non-pipelined march with the
 // walk through them; otherwise the MRPD maps bricks).
-template <int BYTES, typename ACC, int TW, int TH, bool MT, int BSC = 0, bool PT = false>
+// HO: the hand-over of long rays to the group march is compiled in (used
+// when p.yield_after != 0; the other variants carry none of its registers).
+template <int BYTES, typename ACC, int TW, int TH, bool MT, int BSC = 0, bool PT = false, bool HO = false>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(ACC) == 8 ? 1 : DVR_WAVES_PER_EU))) void k_dvr(const DvrLaunch p)
 {
     static_assert(TW * TH == 64, "one wave per ray packet");
@@ -726,9 +728,9 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     double Q = 1.0, thr = 0.0;
     uint32_t n_samples = 0, n_visits = 0;      // per lane
     uint32_t n_rays = 0, n_unmapped = 0;       // per wave
-    // hand-over: the lane's samples before its current ray, and the budget
-    // of samples a ray may consume here before it is handed over
-    uint32_t ray_n0 = 0;
+    // hand-over: the lane's samples before its current ray (in LDS, off the
+    // register budget: read only in rounds where a hand-over may happen)
+    __shared__ uint32_t ray_n0[HO ? LANES : 1];
     uint32_t rounds = 0;
     bool go_seen = false;
 
@@ -756,7 +758,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // a-time trace.  The page cell doubles as the request cell
     // (realDivVirtual == 1).
     const bool piped = !PT && fast_fetch && p.pipelined != 0;
-    const uint32_t budget = piped && p.yield_after != 0 ? p.yield_after : 0xffffffffu;
+    const uint32_t budget = HO && piped && p.yield_after != 0 ? p.yield_after : 0xffffffffu;
     // hb: stage B holds a sample (the front placed last half-step); hc: stage
     // C holds one; newray: the next sample placed is the ray's first
     bool hb = false, hc = false, newray = false;
@@ -967,7 +969,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                         hb = hc = false;
                         newray = true;
                         have = true;
-                        ray_n0 = n_samples;
+                        if constexpr (HO) ray_n0[tid] = n_samples;
                     }
                 }
             }
@@ -1058,7 +1060,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     // first sample of a mapped brick it enters, so the group
                     // march that continues it visits and stamps that brick
                     // as this march would have
-                    const bool yld = handoff_now && live && ewC == 1 && freshC && n_samples - ray_n0 >= budget;
+                    bool yld = false;
+                    if (handoff_now) yld = live && ewC == 1 && freshC && n_samples - ray_n0[tid] >= budget;
                     const bool unmapped = live && ewC == 0;
                     const bool cons = live && ewC == 1 && !yld;
                     n_visits += (live && freshC && !yld) ? 1u : 0u;
@@ -1076,17 +1079,18 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     n_samples += cons ? 1u : 0u;
                     const bool end = (hc && !live) || unmapped || yld;
                     if (end) store_state(tC, keep_pos ? pos_c[tid] : pos_b);
-                    const uint64_t ym = __ballot(yld);
-                    if (ym) {
+                    const uint64_t ym = HO ? __ballot(yld) : 0ull;
+                    if (HO && ym) {
                         // the wave's handed-over rays, with one atomic
                         const uint32_t first = (uint32_t)__builtin_ctzll(ym);
                         uint32_t base = 0;
                         if (lane == first) base = atomicAdd(p.yield_count, (uint32_t)__popcll(ym));
                         base = __shfl(base, (int)first, 64);
                         if (yld) {
-                            p.yield_list[base + (uint32_t)__popcll(ym & lt_mask)] = (v << DVR_PIXEL_BITS) | pix;
+                            const uint32_t slot_y = base + (uint32_t)__popcll(ym & lt_mask);
+                            p.yield_list[slot_y] = (v << DVR_PIXEL_BITS) | pix;
                             const F3 pc = pos_c[tid];
-                            gp(reinterpret_cast<gf4 *>(p.yield_pos))[pix] = gf4{pc.x, pc.y, pc.z, tC};
+                            gp(reinterpret_cast<gf4 *>(p.yield_pos))[slot_y] = gf4{pc.x, pc.y, pc.z, tC};
                         }
                     }
                     // w_old is waited for on every path here (the main path
@@ -1249,7 +1253,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
         }
     }
 
-    if (p.exited && lane == 0) {
+    if (HO && p.exited && lane == 0) {
         // the wave that completes the count raises the go flags
         const uint32_t n = atomicAdd(p.exited, 1u) + 1u;
         if (n == p.yield_exited)
@@ -1425,7 +1429,11 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                     }
                     t = tnear;
                     pos = Rs.origin + dir * tnear;
-                    if (V.rpos && tb > 0.f && tb == tnear) {
+                    if (p.continuation) {
+                        // a ray handed over by the lane march: its exact position
+                        const gf4 q = gp(reinterpret_cast<const gf4 *>(p.yield_pos))[r];
+                        pos = f3(q.x, q.y, q.z);
+                    } else if (V.rpos && tb > 0.f && tb == tnear) {
                         // exact continuation of a stopped ray
                         const gf4 q = gp(reinterpret_cast<const gf4 *>(V.rpos))[pix];
                         pos = f3(q.x, q.y, q.z);
@@ -1952,7 +1960,7 @@ __global__ __launch_bounds__(256) void k_mop(const DvrLaunch p)
     }
 }
 
-template <int BYTES, typename ACC>
+template <int BYTES, typename ACC, bool HO>
 const void *kernel_of(uint32_t tile, bool mt, bool bs32, bool pt)
 {
     // table caches: one variant per packet shape (transfer functions from
@@ -1969,33 +1977,39 @@ const void *kernel_of(uint32_t tile, bool mt, bool bs32, bool pt)
     }
     // 8x8 tiles of u8/u16 bricks of 32 with covering 1 at FP32: constant strides
     if (BYTES != 4 && sizeof(ACC) == 4 && (tile & 3) == 0 && bs32)
-        return mt ? reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, true, 32>)
-                  : reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 32>);
+        return mt ? reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, true, 32, false, HO>)
+                  : reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 32, false, HO>);
     // float volumes never use LDS tables: one variant
     if (BYTES == 4 || !mt) {
         switch (tile & 3) {
-        case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false>);
-        case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4, false>);
-        case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2, false>);
-        default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1, false>);
+        case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 0, false, HO>);
+        case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4, false, 0, false, HO>);
+        case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2, false, 0, false, HO>);
+        default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1, false, 0, false, HO>);
         }
     }
     switch (tile & 3) {
-    case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, true>);
-    case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4, true>);
-    case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2, true>);
-    default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1, true>);
+    case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, true, 0, false, HO>);
+    case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4, true, 0, false, HO>);
+    case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2, true, 0, false, HO>);
+    default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1, true, 0, false, HO>);
     }
 }
 
-const void *dvr_kernel(uint32_t bytes, uint32_t precision, uint32_t tile, bool mt, bool bs32, bool pt = false)
+// ho: the launch hands rays over (the FP32 u8/u16 marches have variants with
+// and without it; the FP64 ones, at 2 waves per SIMD with registers to
+// spare, always carry it; float volumes never hand over)
+const void *dvr_kernel(uint32_t bytes, uint32_t precision, uint32_t tile, bool mt, bool bs32, bool pt = false,
+                       bool ho = false)
 {
     const bool d = precision == 64;
     tile = packet_kind(tile);
     switch (bytes) {
-    case 1: return d ? kernel_of<1, double>(tile, mt, bs32, pt) : kernel_of<1, float>(tile, mt, bs32, pt);
-    case 2: return d ? kernel_of<2, double>(tile, mt, bs32, pt) : kernel_of<2, float>(tile, mt, bs32, pt);
-    default: return d ? kernel_of<4, double>(tile, mt, bs32, pt) : kernel_of<4, float>(tile, mt, bs32, pt);
+    case 1: return d ? kernel_of<1, double, true>(tile, mt, bs32, pt)
+                     : (ho ? kernel_of<1, float, true>(tile, mt, bs32, pt) : kernel_of<1, float, false>(tile, mt, bs32, pt));
+    case 2: return d ? kernel_of<2, double, true>(tile, mt, bs32, pt)
+                     : (ho ? kernel_of<2, float, true>(tile, mt, bs32, pt) : kernel_of<2, float, false>(tile, mt, bs32, pt));
+    default: return d ? kernel_of<4, double, false>(tile, mt, bs32, pt) : kernel_of<4, float, false>(tile, mt, bs32, pt);
     }
 }
 
@@ -2080,7 +2094,7 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;
     if (p.list && !p.list_count) return hipErrorInvalidValue;
     if (p.march < 1 || p.march > 8) return hipErrorInvalidValue;
-    if (p.yield_after && (!p.yield_list || !p.yield_count || !p.yield_pos || p.views)) return hipErrorInvalidValue;
+    if (p.yield_after && (!p.yield_list || !p.yield_count || !p.yield_pos || bytes == 4)) return hipErrorInvalidValue;
     if (p.yield_exited && (!p.exited || !p.go)) return hipErrorInvalidValue;
     if (p.pt.n) {
         if (p.pipelined) return hipErrorInvalidValue;
@@ -2094,7 +2108,8 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
     }
     void *args[] = {const_cast<DvrLaunch *>(&p)};
     const bool bs32 = p.brick_size == 32 && p.covering == 1;
-    return hipLaunchKernel(dvr_kernel(bytes, precision, tile, p.tf_lds_tables > 1, bs32), dim3(grid), dim3(LANES), args,
+    return hipLaunchKernel(dvr_kernel(bytes, precision, tile, p.tf_lds_tables > 1, bs32, false, p.yield_after != 0),
+                           dim3(grid), dim3(LANES), args,
                            dvr_lds_bytes(p.nb_lods, p.views ? p.n_views : 1, p.tf_lds_entries), stream);
 }
 
@@ -2207,6 +2222,7 @@ hipError_t launch_dvr_ray(const DvrLaunch &p, uint32_t bytes, uint32_t precision
                           hipStream_t stream, uint32_t group)
 {
     if (!p.list || !p.list_count || p.n_packets != 0 || grid == 0) return hipErrorInvalidValue;
+    if (p.continuation && !p.yield_pos) return hipErrorInvalidValue;
     if (!dvr_ray_supported(p, bytes)) return hipErrorInvalidValue;
     if (p.nb_lods == 0 || p.nb_lods > GC_MAX_LEVELS) return hipErrorInvalidValue;
     if (p.views && (p.n_views == 0 || p.n_views > DVR_MAX_VIEWS)) return hipErrorInvalidValue;

@@ -1293,6 +1293,7 @@ int Handler::handle(const std::vector<uint32_t> &list)
 
 // ==================================================================== frame
 constexpr uint32_t HAND_CTL_WORDS = 65 * 32;   // see Frame::hand_ctl
+constexpr uint32_t LANES_PER_WG = 256;          // the ray casters' workgroups
 
 struct Frame {
     uint32_t w = 0, h = 0, precision = 32;
@@ -1309,6 +1310,7 @@ struct Frame {
     uint32_t *hand_ctl = nullptr;
     uint32_t *lists[2] = {nullptr, nullptr};   // rays stopped at a miss
     uint32_t *handed = nullptr;                // rays the lane march handed over in the launch
+    float4 *handed_pos = nullptr;              // and their exact positions
     uint8_t *stop = nullptr;                   // per pixel: stopped at a miss in the last launch
     size_t cbytes() const { return (size_t)w * h * (precision == 64 ? 32 : 16); }
     ~Frame()
@@ -1322,10 +1324,48 @@ struct Frame {
         hipFree(lists[0]);
         hipFree(lists[1]);
         hipFree(handed);
+        hipFree(handed_pos);
         hipFree(hand_ctl);
         hipFree(stop);
     }
 };
+
+// Hand-over of a lane-per-ray launch L (grid workgroups of 4 waves) to the
+// group march H that continues its handed-over rays: L gets the budget, the
+// trigger and the hand-over buffers (list and positions of capacity >= the
+// launch's rays, hand_ctl as Frame::hand_ctl, cleared before L runs), H the
+// same launch over the handed-over list.
+// Whether lane-per-ray launch L over cache C can hand rays over: the
+// pipelined march (u8/u16 voxels, covering >= 1, no table caches) and a group
+// march for the continuation.
+static bool handoff_possible(const Cache &C, const DvrLaunch &L)
+{
+    return C.handoff_after != 0 && L.pipelined && L.pt.n == 0 && C.vc.encoded_bytes != 4 && C.g.covering >= 1 &&
+           dvr_ray_supported(L, C.vc.encoded_bytes);
+}
+
+static void handoff_launches(const Cache &C, DvrLaunch &L, DvrLaunch &H, uint32_t grid, uint32_t *handed,
+                             float4 *handed_pos, uint32_t *hand_ctl)
+{
+    L.yield_after = C.handoff_after;
+    L.yield_exited = C.handoff_exited_pct
+                         ? std::max(1u, (uint32_t)((uint64_t)grid * 4u * C.handoff_exited_pct / 100u))
+                         : 0u;
+    L.exited = hand_ctl + 2;
+    L.go = hand_ctl + 32;
+    L.yield_list = handed;
+    L.yield_count = hand_ctl;
+    L.yield_pos = handed_pos;
+    H = L;
+    H.yield_after = H.yield_exited = 0;
+    H.exited = H.go = nullptr;
+    H.yield_list = H.yield_count = nullptr;
+    H.list = handed;
+    H.list_count = hand_ctl;
+    H.queue = hand_ctl + 1;
+    H.n_packets = 0;
+    H.continuation = 1;
+}
 
 // Exact opacity threshold of a view for FP32 compositing: a ray continues
 // while its opacity product Q > (1 - alpha_stop)^(1/k0) (see kernels_dvr.hip,
@@ -1452,6 +1492,9 @@ struct Batch {
     float4 *rpos = nullptr;             // resume positions (views with resume = 1)
     double *qbuf = nullptr;             // FP32: exact opacity products
     uint32_t *lists[2] = {nullptr, nullptr};
+    uint32_t *handed = nullptr;         // rays handed over in a launch (Frame::handed)
+    float4 *handed_pos = nullptr;
+    uint32_t *hand_ctl = nullptr;       // see Frame::hand_ctl
     uint8_t *stop = nullptr;            // per pixel: stopped at a miss in the last launch
     // [0..7] per-XCD packet cursors, [8], [9] ray-list counts, [10] timestamp,
     // [16..16+max_views) rays stopped per slot in the last launch
@@ -1470,7 +1513,7 @@ struct Batch {
     uint32_t tile = 0, march = 8, refill_min = 16;
     int cur = -1;                       // list holding the last launch's stopped rays (-1: none)
     bool use_graph = false;
-    std::map<std::array<uint32_t, 10>, hipGraphExec_t> graphs;
+    std::map<std::array<uint32_t, 11>, hipGraphExec_t> graphs;
     size_t cbytes_per_pixel() const { return precision == 64 ? 32 : 16; }
     uint64_t slot_pixels() const { return max_pixels / max_views; }
     void drop_graphs()
@@ -1488,6 +1531,9 @@ struct Batch {
         hipFree(qbuf);
         hipFree(lists[0]);
         hipFree(lists[1]);
+        hipFree(handed);
+        hipFree(handed_pos);
+        hipFree(hand_ctl);
         hipFree(stop);
         hipFree(ctl);
         hipFree(d_views);
@@ -1626,11 +1672,30 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
                                                     ray_group ? ray_group : C.group_for(list_rays, L.pt.n != 0))
                                      : dvr_grid(C.vc.encoded_bytes, B.precision, B.tile, packets + (uint32_t)((listed + 63) / 64), lds,
                                                 tf_tables);
+    // lane-per-ray launches hand their tail over to a group march, as the
+    // frame loop's, when they resume rays or hold few rays per lane (a launch
+    // of many fresh views, some 100 rays per lane, spends little time in its
+    // tail and runs faster without the hand-over's registers)
+    const bool handoff = !ray_kernel && handoff_possible(C, L) &&
+                         (has_list || (uint64_t)packets * 64u <= (uint64_t)grid * LANES_PER_WG * 16u);
+    DvrLaunch H{};
+    uint32_t h_grid = 0;
+    if (handoff) {
+        handoff_launches(C, L, H, grid, B.handed, B.handed_pos, B.hand_ctl);
+        h_grid = dvr_ray_grid(C.vc.encoded_bytes, B.precision, ~0u - 64u, dvr_ray_lds_bytes(B.table, tf_entries),
+                              C.handoff_group);
+    }
     auto launch_k = [&]() -> hipError_t {
-        const hipError_t e =
-            ray_kernel ? launch_dvr_ray(L, C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays, grid, s,
-                                        ray_group ? ray_group : C.group_for(list_rays, L.pt.n != 0))
-                       : launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
+        hipError_t e = hipSuccess;
+        if (ray_kernel) {
+            e = launch_dvr_ray(L, C.vc.encoded_bytes, B.precision, B.use_graph ? ~0u - 64u : list_rays, grid, s,
+                               ray_group ? ray_group : C.group_for(list_rays, L.pt.n != 0));
+        } else {
+            if (handoff) e = launch_fill32(B.hand_ctl, HAND_CTL_WORDS, 0, s);
+            if (e == hipSuccess) e = launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
+            if (e == hipSuccess && handoff)
+                e = launch_dvr_ray(H, C.vc.encoded_bytes, B.precision, ~0u - 64u, h_grid, s, C.handoff_group);
+        }
         return e == hipSuccess ? launch_collect_stopped(ca, s) : e;
     };
     GC_HIP(hipEventRecord(B.ev0, s));
@@ -1639,9 +1704,13 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
         // stays valid across updates
         GC_HIP(launch_fill32(B.ctl + 10, 1, C.ts, s));
         L.ts_dev = B.ctl + 10;
-        const std::array<uint32_t, 10> key{B.table, B.tile + (ray_kernel ? 8u : 0u), B.march, B.refill_min, packets, grid,
+        H.ts_dev = L.ts_dev;
+        const std::array<uint32_t, 11> key{B.table, B.tile + (ray_kernel ? 8u : 0u), B.march, B.refill_min, packets, grid,
                                            has_list ? (uint32_t)B.cur : 2u, (uint32_t)out + (append ? 4u : 0u),
-                                           tf_entries, stop_tiles_n};
+                                           tf_entries, stop_tiles_n,
+                                           handoff ? (C.handoff_after << 16) ^ (C.handoff_group << 8) ^
+                                                         C.handoff_exited_pct
+                                                   : 0u};
         auto it = B.graphs.find(key);
         if (it == B.graphs.end()) {
             hipGraph_t g = nullptr;
@@ -2376,6 +2445,7 @@ int gc_frame_create(uint32_t width, uint32_t height, uint32_t precision, gc_fram
     GC_TRY(dalloc(&F.lists[0], (size_t)width * height * 4));
     GC_TRY(dalloc(&F.lists[1], (size_t)width * height * 4));
     GC_TRY(dalloc(&F.handed, (size_t)width * height * 4));
+    GC_TRY(dalloc(&F.handed_pos, (size_t)width * height * sizeof(float4)));
     GC_TRY(dalloc(&F.hand_ctl, HAND_CTL_WORDS * 4));
     GC_TRY(dalloc(&F.stop, ((size_t)width * height + 3) & ~(size_t)3));
     GC_HIP(hipMemset(F.stop, 0, ((size_t)width * height + 3) & ~(size_t)3));
@@ -2530,40 +2600,15 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
         // walks one sample per step: rays past the budget are handed over to
         // a group march that continues them (same state, same stamps and
         // counters) before the cache update.
-        const bool handoff = C.handoff_after != 0 && L.pipelined && L.pt.n == 0 &&
-                             dvr_ray_supported(L, C.vc.encoded_bytes);
+        const bool handoff = handoff_possible(C, L);
+        DvrLaunch H{};
         if (handoff) {
-            L.yield_after = C.handoff_after;
-            L.exited = F.hand_ctl + 2;
-            // (workgroups of 4 waves)
-            L.yield_exited = C.handoff_exited_pct
-                                 ? std::max(1u, (uint32_t)((uint64_t)grid * 4u * C.handoff_exited_pct / 100u))
-                                 : 0u;
-            L.yield_list = F.handed;
-            L.yield_count = F.hand_ctl;
-            L.yield_pos = F.rpos;
-            L.go = F.hand_ctl + 32;
+            handoff_launches(C, L, H, grid, F.handed, F.handed_pos, F.hand_ctl);
             GC_HIP(launch_fill32(F.hand_ctl, HAND_CTL_WORDS, 0, s));
             *handed = true;
         }
         GC_HIP(launch_dvr(L, C.vc.encoded_bytes, view.precision, view.tile, grid, s));
         if (handoff) {
-            DvrLaunch H = L;
-            H.yield_after = 0;
-            H.yield_exited = 0;
-            H.exited = H.go = nullptr;
-            H.yield_list = H.yield_count = nullptr;
-            H.yield_pos = nullptr;
-            // every ray of the list continues exactly at its stored position
-            // (rays it stops at a miss resume as the view says: the next
-            // launch does not read these positions unless view.resume = 1)
-            H.view0.rpos = F.rpos;
-            H.exact_resume = 1;
-            H.list = F.handed;
-            H.list_count = F.hand_ctl;
-            H.queue = F.hand_ctl + 1;
-            H.n_packets = 0;
-            H.continuation = 1;
             const uint32_t group = C.handoff_group;
             GC_HIP(launch_dvr_ray(H, C.vc.encoded_bytes, view.precision, F.w * F.h,
                                   dvr_ray_grid(C.vc.encoded_bytes, view.precision, F.w * F.h,
@@ -2769,6 +2814,9 @@ int gc_batch_create(gc_cache *c, uint32_t max_views, uint64_t max_pixels, uint32
     GC_TRY(dalloc(&B.qbuf, max_pixels * sizeof(double)));
     GC_TRY(dalloc(&B.lists[0], max_pixels * 4));
     GC_TRY(dalloc(&B.lists[1], max_pixels * 4));
+    GC_TRY(dalloc(&B.handed, max_pixels * 4));
+    GC_TRY(dalloc(&B.handed_pos, max_pixels * sizeof(float4)));
+    GC_TRY(dalloc(&B.hand_ctl, HAND_CTL_WORDS * 4));
     GC_TRY(dalloc(&B.stop, (max_pixels + 3) & ~3ull));
     GC_HIP(hipMemset(B.stop, 0, (max_pixels + 3) & ~3ull));
     GC_TRY(dalloc(&B.ctl, (16 + DVR_MAX_VIEWS) * 4));

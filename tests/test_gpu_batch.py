@@ -112,8 +112,10 @@ def _oracle_views(sc, views):
     return out
 
 
-@pytest.mark.parametrize("bytes_,graph,tile", [(1, False, 0), (2, True, 1)])
-def test_batch_out_of_core_lockstep(gpu, bytes_, graph, tile, parity_log):
+@pytest.mark.parametrize("bytes_,graph,tile,handoff", [(1, False, 0, None), (2, True, 1, None), (2, False, 0, (1, 8, 0)),
+                                                        (1, True, 0, (1, 16, 50)), (2, True, 0, (1, 8, 0)),
+                                                        (2, False, 1, (0, 0, 0))])
+def test_batch_out_of_core_lockstep(gpu, bytes_, graph, tile, handoff, parity_log):
     """Batched progressive launches with a pool smaller than the working set.
     After every launch, each view's pixels (<= 1 LSB), t buffer (exact) and
     FP64 colour state match the oracle rendering that view over the same cache
@@ -121,9 +123,14 @@ def test_batch_out_of_core_lockstep(gpu, bytes_, graph, tile, parity_log):
     waiting per view equal the oracle's misses, and every update leaves the
     LRU, page directory, usage, requests and pool bit-identical.  (The batch
     relaunches only the rays that stopped at a miss; the oracle re-runs every
-    ray, whose finished rays neither sample nor touch the cache.)"""
+    ray, whose finished rays neither sample nor touch the cache.)  handoff:
+    the cache's hand-over setting (None: a new cache's; (1, lanes, 0): every
+    ray that has sampled once is handed over to the group march at its next
+    mapped brick; (0, 0, 0): off)."""
     sc = Scene((64, 64, 64), 8, 1, bytes_, cache_size=(9, 8, 8), max_requests=50)
     gc, oc = sc.gpu_cache(), sc.oracle_cache()
+    if handoff is not None:
+        G.set_ray_handoff(gc, *handoff)
     views = views_for(64, tile=tile)[:4]
     d_tfs, sizes = tfs(len(views))
     oviews = _oracle_views(sc, views)

@@ -60,17 +60,22 @@ def test_plain_config_full_size(gpu, c1_scene, prec, parity_log):
         frame.close()
 
 
-@pytest.mark.parametrize("ray_list_max,lanes", [(0, 0), (1 << 30, 0), (1 << 30, 4)])
-def test_out_of_core_bricks32_fp32_lockstep(gpu, ray_list_max, lanes, parity_log):
+@pytest.mark.parametrize("ray_list_max,lanes,handoff", [(0, 0, None), (1 << 30, 0, None), (1 << 30, 4, None),
+                                                        (0, 0, (1, 8, 0)), (0, 0, (0, 0, 0))])
+def test_out_of_core_bricks32_fp32_lockstep(gpu, ray_list_max, lanes, handoff, parity_log):
     """ray_list_max 0: resumed rays run the tile / lane-per-ray march;
     1 << 30: the group-per-ray march (both compiled for bricks of 32), with
-    the lanes per ray of the list length or 4."""
+    the lanes per ray of the list length or 4.  handoff: the hand-over setting
+    of the lane march (None: a new cache's; (1, 8, 0): every ray that has
+    sampled once goes to the group march at its next mapped brick; off)."""
     sc = Scene((192, 160, 128), 32, 1, 2, cache_size=(5, 4, 3), max_requests=32, seed=77, nb_blobs=64,
                sigma=(6.0, 30.0), amp=(15000.0, 65535.0), noise=2048)
     assert sc.total_bricks > 3 * 60
     gc, oc = sc.gpu_cache(), sc.oracle_cache()
     G.set_ray_list_max(gc, ray_list_max)
     G.set_ray_group(gc, lanes)
+    if handoff is not None:
+        G.set_ray_handoff(gc, *handoff)
     tf = tf_table(seed=3)
     d_tf = G.DeviceBuffer.from_array(tf)
     sizes = [(128, 96), (96, 72)]
