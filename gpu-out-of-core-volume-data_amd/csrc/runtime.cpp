@@ -803,6 +803,10 @@ struct Cache {
     int load_new_bricks();
     int update(int *done);
     int insert_serial_needed(const uint32_t *asked, uint32_t n, int &serial) const;
+    // insert_serial_needed's scratch: one bit per page-directory cell, all
+    // clear between calls, and the cells it set
+    mutable std::vector<uint64_t> cell_bits;
+    mutable std::vector<uint64_t> cells_set;
     int reference_plan(const uint32_t *asked, uint32_t n, bool &parallel);
 };
 
@@ -1019,15 +1023,34 @@ int Cache::insert_serial_needed(const uint32_t *asked, uint32_t n, int &serial) 
 {
     // The parallel insert is exact when no two asked bricks share a page-table
     // cell under the FP32 position formula; otherwise replay the reference's
-    // serial order.
-    std::unordered_set<uint64_t> seen;
-    seen.reserve(n * 2);
+    // serial order.  The check runs between a launch and the next one: a
+    // bitset over the cells (cleared again afterwards) keeps it to a few ns
+    // per brick, a hash set for directories above 2^30 cells.
     serial = 0;
+    if (g.mrpd_cells > (1ull << 30)) {
+        std::unordered_set<uint64_t> seen;
+        seen.reserve(n * 2);
+        for (uint32_t k = 0; k < n; ++k) {
+            float vol[3];
+            volume_position(g, asked[4 * k + 3], asked + 4 * k, vol);
+            if (!seen.insert(mrpd_index(g, asked[4 * k + 3], vol)).second) { serial = 1; break; }
+        }
+        return GC_OK;
+    }
+    const size_t words = (size_t)((g.mrpd_cells + 63) / 64);
+    if (cell_bits.size() != words) cell_bits.assign(words, 0ull);
+    cells_set.clear();
     for (uint32_t k = 0; k < n; ++k) {
         float vol[3];
         volume_position(g, asked[4 * k + 3], asked + 4 * k, vol);
-        if (!seen.insert(mrpd_index(g, asked[4 * k + 3], vol)).second) { serial = 1; break; }
+        const uint64_t c = mrpd_index(g, asked[4 * k + 3], vol);
+        if (c >= g.mrpd_cells) { serial = 1; break; }   // (never: positions lie in [0, 1))
+        const uint64_t bit = 1ull << (c & 63);
+        if (cell_bits[c >> 6] & bit) { serial = 1; break; }
+        cell_bits[c >> 6] |= bit;
+        cells_set.push_back(c);
     }
+    for (const uint64_t c : cells_set) cell_bits[c >> 6] &= ~(1ull << (c & 63));
     return GC_OK;
 }
 
@@ -1200,19 +1223,39 @@ int Handler::handle(const std::vector<uint32_t> &list)
         GC_HIP(hipEventSynchronize(c->uploaded));
         upload_pending = false;
     }
-    // unique big bricks, first-occurrence order (std::map of :289)
-    std::map<std::array<uint32_t, 4>, uint32_t> uid_of;
+    // unique big bricks, first-occurrence order (std::map of :289).  The
+    // list holds each requested cell once (a compaction of the request
+    // buffer), so with big bricks of one brick every request is its own big
+    // brick; otherwise a hash of the packed key finds the repeats.  This runs
+    // serially between a launch and the next one.
     std::vector<std::array<uint32_t, 4>> uniq;
     std::vector<uint32_t> uid(nreq);
-    for (uint32_t i = 0; i < nreq; ++i) {
-        const uint32_t *a = &list[4 * i];
-        const std::array<uint32_t, 4> key{a[0] / S.BB[0], a[1] / S.BB[1], a[2] / S.BB[2], a[3]};
-        auto it = uid_of.find(key);
-        if (it == uid_of.end()) {
-            it = uid_of.emplace(key, (uint32_t)uniq.size()).first;
-            uniq.push_back(key);
+    uniq.reserve(nreq);
+    if (S.BB[0] == 1 && S.BB[1] == 1 && S.BB[2] == 1) {
+        for (uint32_t i = 0; i < nreq; ++i) {
+            const uint32_t *a = &list[4 * i];
+            uniq.push_back({a[0], a[1], a[2], a[3]});
+            uid[i] = i;
         }
-        uid[i] = it->second;
+    } else {
+        std::unordered_map<uint64_t, uint32_t> packed;
+        std::map<std::array<uint32_t, 4>, uint32_t> wide;   // keys with a field of 2^16 or more
+        packed.reserve((size_t)nreq * 2);
+        for (uint32_t i = 0; i < nreq; ++i) {
+            const uint32_t *a = &list[4 * i];
+            const std::array<uint32_t, 4> key{a[0] / S.BB[0], a[1] / S.BB[1], a[2] / S.BB[2], a[3]};
+            const uint32_t next = (uint32_t)uniq.size();
+            uint32_t id;
+            if (key[0] < 65536u && key[1] < 65536u && key[2] < 65536u && key[3] < 65536u)
+                id = packed.emplace((uint64_t)key[0] | (uint64_t)key[1] << 16 | (uint64_t)key[2] << 32 |
+                                        (uint64_t)key[3] << 48,
+                                    next)
+                         .first->second;
+            else
+                id = wide.emplace(key, next).first->second;
+            if (id == next) uniq.push_back(key);
+            uid[i] = id;
+        }
     }
     const uint32_t nu = (uint32_t)uniq.size();
     std::vector<int> st(nu, GC_BRICK_SUCCESS);
