@@ -766,7 +766,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     uint32_t lodB = 0, lodC = 0;   // levels of samples B and C
     float tC = 0.f;
     // position of sample C, kept (in LDS, off the register budget) only when
-    // a view of the launch asks for exact continuation
+    // a view of the launch asks for exact continuation or the launch hands
+    // rays over
     __shared__ F3 pos_c[LANES];
     const bool keep_pos = p.exact_resume != 0 || budget != 0xffffffffu;
     bool freshB = false, freshC = false, reuseC = false;

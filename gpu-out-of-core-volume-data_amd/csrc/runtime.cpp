@@ -2640,9 +2640,9 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
                                        first ? L.n_packets : (F.w * F.h + 63) / 64,
                                        dvr_lds_bytes(L.nb_lods, 1, L.tf_lds_entries), L.tf_lds_tables);
         // The launch lasts as long as its longest rays, which the lane march
-        // walks one sample per step: rays past the budget are handed over to
-        // a group march that continues them (same state, same stamps and
-        // counters) before the cache update.
+        // walks one sample per step: once most of its waves have run out of
+        // rays, the rays left are handed over to a group march that continues
+        // them (same state, same stamps and counters) before the cache update.
         const bool handoff = handoff_possible(C, L);
         DvrLaunch H{};
         if (handoff) {
