@@ -758,7 +758,8 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
     // a-time trace.  The page cell doubles as the request cell
     // (realDivVirtual == 1).
     const bool piped = !PT && fast_fetch && p.pipelined != 0;
-    const uint32_t budget = HO && piped && p.yield_after != 0 ? p.yield_after : 0xffffffffu;
+    // (the pipelined march, or the walk through table caches)
+    const uint32_t budget = HO && (piped || PT) && p.yield_after != 0 ? p.yield_after : 0xffffffffu;
     // hb: stage B holds a sample (the front placed last half-step); hc: stage
     // C holds one; newray: the next sample placed is the ray's first
     bool hb = false, hc = false, newray = false;
@@ -1130,6 +1131,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 }
             }
         }
+        bool yielded = false;   // handed over (non-pipelined march through table caches)
         if (!piped && have) {
             bool done = false;
             for (int k = 0; k < march; ++k) {
@@ -1167,6 +1169,15 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                         pk_prev = k;
                         const uint32_t e = pt_walk(p, k, ts);
                         ew = word_state(e);
+                        // hand-over before the visit of a mapped brick: the
+                        // group march walks, stamps and counts it again
+                        // (the walk's stamps carry the same timestamp)
+                        if constexpr (HO) {
+                            if (handoff_now && ew == 1 && n_samples - ray_n0[tid] >= budget) {
+                                yielded = true;
+                                break;
+                            }
+                        }
                         ++n_visits;
                         slot = word_slot(e);
                         bbase = pool + (uint64_t)slot * bbytes + corner;
@@ -1240,9 +1251,23 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(sizeof(AC
                 t += step;
                 pos = pos + dstep;
             }
-            if (done || stopped) {
+            if (done || stopped || yielded) {
                 store_state(t, pos);
                 have = false;
+            }
+        }
+        if constexpr (HO && PT) {
+            const uint64_t ym = __ballot(yielded);
+            if (ym) {
+                const uint32_t first = (uint32_t)__builtin_ctzll(ym);
+                uint32_t base = 0;
+                if (lane == first) base = atomicAdd(p.yield_count, (uint32_t)__popcll(ym));
+                base = __shfl(base, (int)first, 64);
+                if (yielded) {
+                    const uint32_t slot_y = base + (uint32_t)__popcll(ym & lt_mask);
+                    p.yield_list[slot_y] = (v << DVR_PIXEL_BITS) | pix;
+                    gp(reinterpret_cast<gf4 *>(p.yield_pos))[slot_y] = gf4{pos.x, pos.y, pos.z, t};
+                }
             }
         }
         // rays stopped at a miss: flagged for the next launch's ray list
@@ -1968,12 +1993,12 @@ const void *kernel_of(uint32_t tile, bool mt, bool bs32, bool pt)
     // HBM when a launch has several)
     if (pt) {
         if (BYTES != 4 && sizeof(ACC) == 4 && (tile & 3) == 0 && bs32)
-            return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 32, true>);
+            return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 32, true, HO>);
         switch (tile & 3) {
-        case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 0, true>);
-        case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4, false, 0, true>);
-        case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2, false, 0, true>);
-        default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1, false, 0, true>);
+        case 0: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 8, 8, false, 0, true, HO>);
+        case 1: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 16, 4, false, 0, true, HO>);
+        case 2: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 32, 2, false, 0, true, HO>);
+        default: return reinterpret_cast<const void *>(&k_dvr<BYTES, ACC, 64, 1, false, 0, true, HO>);
         }
     }
     // 8x8 tiles of u8/u16 bricks of 32 with covering 1 at FP32: constant strides
@@ -2103,7 +2128,8 @@ hipError_t launch_dvr(const DvrLaunch &p, uint32_t bytes, uint32_t precision, ui
         DvrLaunch q = p;
         if (q.tf_lds_tables > 1) q.tf_lds_entries = q.tf_lds_tables = 0;
         void *args[] = {&q};
-        return hipLaunchKernel(dvr_kernel(bytes, precision, tile, false, q.brick_size == 32 && q.covering == 1, true),
+        return hipLaunchKernel(dvr_kernel(bytes, precision, tile, false, q.brick_size == 32 && q.covering == 1, true,
+                                          q.yield_after != 0),
                                dim3(grid), dim3(LANES), args,
                                dvr_lds_bytes(q.nb_lods, q.views ? q.n_views : 1, q.tf_lds_entries), stream);
     }

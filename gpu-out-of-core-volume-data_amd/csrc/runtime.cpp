@@ -1378,13 +1378,20 @@ struct Frame {
 // trigger and the hand-over buffers (list and positions of capacity >= the
 // launch's rays, hand_ctl as Frame::hand_ctl, cleared before L runs), H the
 // same launch over the handed-over list.
-// Whether lane-per-ray launch L over cache C can hand rays over: the
-// pipelined march (u8/u16 voxels, covering >= 1, no table caches) and a group
-// march for the continuation.
+// Whether lane-per-ray launch L over cache C can hand rays over: u8/u16
+// voxels with covering >= 1, the pipelined march or the walk through table
+// caches, and a group march for the continuation.
 static bool handoff_possible(const Cache &C, const DvrLaunch &L)
 {
-    return C.handoff_after != 0 && L.pipelined && L.pt.n == 0 && C.vc.encoded_bytes != 4 && C.g.covering >= 1 &&
+    return C.handoff_after != 0 && (L.pipelined || L.pt.n != 0) && C.vc.encoded_bytes != 4 && C.g.covering >= 1 &&
            dvr_ray_supported(L, C.vc.encoded_bytes);
+}
+
+// Lanes per ray of the continuation (the table-walk group marches have 16 or
+// more).
+static uint32_t handoff_group_of(const Cache &C, const DvrLaunch &L)
+{
+    return L.pt.n != 0 ? std::max(16u, C.handoff_group) : C.handoff_group;
 }
 
 static void handoff_launches(const Cache &C, DvrLaunch &L, DvrLaunch &H, uint32_t grid, uint32_t *handed,
@@ -1726,7 +1733,7 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
     if (handoff) {
         handoff_launches(C, L, H, grid, B.handed, B.handed_pos, B.hand_ctl);
         h_grid = dvr_ray_grid(C.vc.encoded_bytes, B.precision, ~0u - 64u, dvr_ray_lds_bytes(B.table, tf_entries),
-                              C.handoff_group);
+                              handoff_group_of(C, L));
     }
     auto launch_k = [&]() -> hipError_t {
         hipError_t e = hipSuccess;
@@ -1737,7 +1744,7 @@ static int batch_launch(Batch &B, gc_frame_report *rep, bool skip_list = false)
             if (handoff) e = launch_fill32(B.hand_ctl, HAND_CTL_WORDS, 0, s);
             if (e == hipSuccess) e = launch_dvr(L, C.vc.encoded_bytes, B.precision, B.tile, grid, s);
             if (e == hipSuccess && handoff)
-                e = launch_dvr_ray(H, C.vc.encoded_bytes, B.precision, ~0u - 64u, h_grid, s, C.handoff_group);
+                e = launch_dvr_ray(H, C.vc.encoded_bytes, B.precision, ~0u - 64u, h_grid, s, handoff_group_of(C, L));
         }
         return e == hipSuccess ? launch_collect_stopped(ca, s) : e;
     };
@@ -2652,7 +2659,7 @@ static int frame_launch(Cache &C, Frame &F, const gc_view &view, const float *d_
         }
         GC_HIP(launch_dvr(L, C.vc.encoded_bytes, view.precision, view.tile, grid, s));
         if (handoff) {
-            const uint32_t group = C.handoff_group;
+            const uint32_t group = handoff_group_of(C, L);
             GC_HIP(launch_dvr_ray(H, C.vc.encoded_bytes, view.precision, F.w * F.h,
                                   dvr_ray_grid(C.vc.encoded_bytes, view.precision, F.w * F.h,
                                                dvr_ray_lds_bytes(1, H.tf_lds_entries), group),

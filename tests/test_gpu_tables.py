@@ -384,3 +384,54 @@ def test_brick32_with_tables_fp32_against_oracle(gpu, bytes_, parity_log):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(caches[0].table(0), caches[1].table(0))
     assert np.array_equal(caches[0].table_usage(0), caches[1].table_usage(0))
+
+
+@pytest.mark.parametrize("bytes_,prec,tables,exited", [(1, 32, ONE_ODD, 0), (2, 64, ONE_SMALL, 0), (2, 32, TWO, 0),
+                                                       (1, 32, THREE, 0)])
+def test_handoff_with_tables_matches_lane_march(gpu, bytes_, prec, tables, exited):
+    """The hand-over through table caches (the walking march stops a ray
+    before the visit of a mapped brick, the group march walks, stamps and
+    counts it): bit-identical frames, counters and cache states (tables
+    included) after every frame, with every ray that has sampled once
+    handed over at its next mapped brick (these small launches rarely reach
+    a tail trigger)."""
+    sc = Scene((96, 80, 64), 8, 1, bytes_, cache_size=(9, 9, 8), max_requests=64, tables=tables)
+    caches = [sc.gpu_cache(), sc.gpu_cache()]
+    for c in caches:
+        G.set_ray_list_max(c, 0)
+    G.set_ray_handoff(caches[0], 0)
+    G.set_ray_handoff(caches[1], 1, 16, exited)
+    log = G.LaunchLog(caches[1], capacity=256)
+    tf = tf_table(seed=29)
+    d_tf = G.DeviceBuffer.from_array(tf)
+    W, H = 80, 64
+    handed = 0
+    for k in range(3):
+        out = []
+        for c in caches:
+            f = G.Frame(W, H, prec)
+            r = G.render_frame(c, G.make_view(W, H, orbit(2 * k + 1, distance=2.2), alpha_stop=0.95 if k % 2 else 0.99,
+                                              precision=prec), d_tf, len(tf), f, max_launches=60)
+            tb, cs = f.state()
+            out.append((f.pixels().copy(), tb.copy(), cs.copy(), r.as_dict()))
+            f.close()
+        (p0, t0, c0, r0), (p1, t1, c1, r1) = out
+        n = sum(r["handed_over"] for r in log.last_frame())
+        handed += n
+        print("tables handoff frame", k, "launches", r0["launches"], "handed over", n, flush=True)
+        for key in ("launches", "complete", "mapped_samples", "brick_visits", "pixels", "unmapped_rays",
+                    "bricks_loaded"):
+            assert r0[key] == r1[key], (k, key, r0[key], r1[key])
+        assert np.array_equal(p0, p1), "pixels differ at frame %d" % k
+        assert np.array_equal(t0, t1) and np.array_equal(c0.view(np.uint8), c1.view(np.uint8)), k
+        a, b = caches
+        assert np.array_equal(a.mrpd(), b.mrpd()) and np.array_equal(a.lru(), b.lru()), k
+        assert np.array_equal(a.usage(), b.usage()) and np.array_equal(a.request(), b.request()), k
+        for i in range(len(tables)):
+            assert np.array_equal(a.table(i), b.table(i)), (k, i)
+            assert np.array_equal(a.table_usage(i), b.table_usage(i)), (k, i)
+            assert np.array_equal(a.table_lru(i), b.table_lru(i)), (k, i)
+    assert handed > 0
+    log.close()
+    for c in caches:
+        c.close()
